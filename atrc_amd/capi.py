@@ -1,0 +1,251 @@
+"""ctypes mirror of ``include/atrc_gpu.h`` and a thin Python wrapper.
+
+The structures below must match the C header field for field; the CPU test
+suite checks their sizes against the compiled library.  The wrapper raises
+``AgxError`` with ``agx_last_error()`` on any non-zero status, and loading
+fails loudly when ``libatrc_gpu.so`` has not been built: there is no fallback
+path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+GPU_LIB_PATH = PKG / "libatrc_gpu.so"
+
+AGX_ABI_VERSION = 1
+
+AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
+    AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
+
+GEOM_TRIANGLE_BVH, GEOM_QUAD, GEOM_TRIANGLE, GEOM_SPHERE, GEOM_DISK = range(5)
+MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK = range(5)
+ENV_NONE, ENV_NATIVE_SKY = 0, 1
+AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
+
+F3 = C.c_float * 3
+
+
+class agx_vertex(C.Structure):
+    _fields_ = [("pos", F3), ("nor", F3), ("uv", C.c_float * 2)]
+
+
+class agx_entity_desc(C.Structure):
+    _fields_ = [
+        ("geometry", C.c_int32), ("material", C.c_int32), ("medium_in", C.c_int32), ("medium_out", C.c_int32),
+        ("emit_radiance", F3), ("power", C.c_float), ("no_denoise", C.c_int32), ("tri_count", C.c_uint32),
+        ("tris", C.POINTER(agx_vertex)), ("p", (C.c_float * 3) * 4), ("t", (C.c_float * 2) * 4),
+        ("radius", C.c_float), ("local_to_world", C.c_float * 12), ("world_to_local", C.c_float * 12),
+    ]
+
+
+class agx_material_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("color", F3), ("color2", F3), ("eta", F3), ("k", F3),
+        ("roughness", C.c_float), ("anisotropic", C.c_float), ("ior", C.c_float),
+        ("glass_separate_refraction", C.c_int32),
+    ]
+
+
+class agx_env_desc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("top", F3), ("bottom", F3), ("power", C.c_float)]
+
+
+class agx_camera_desc(C.Structure):
+    _fields_ = [
+        ("pos", F3), ("dst", F3), ("up", F3), ("fov_rad", C.c_float), ("lens_radius", C.c_float),
+        ("focal_distance", C.c_float), ("film_aspect", C.c_float),
+    ]
+
+
+class agx_scene_desc(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("entity_count", C.c_uint32), ("entities", C.POINTER(agx_entity_desc)),
+        ("material_count", C.c_uint32), ("materials", C.POINTER(agx_material_desc)), ("env", agx_env_desc),
+        ("aggregate", C.c_int32), ("max_leaf_size", C.c_int32),
+    ]
+
+
+class agx_render_params(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("min_depth", C.c_int32),
+        ("max_depth", C.c_int32), ("specular_depth", C.c_int32), ("cont_prob", C.c_float), ("use_mis", C.c_int32),
+        ("eps", C.c_float), ("filter_radius", C.c_float), ("seed", C.c_uint64), ("camera", agx_camera_desc),
+        ("collect_stats", C.c_int32), ("sample_begin", C.c_int32),
+    ]
+
+
+class agx_stats_t(C.Structure):
+    _fields_ = [
+        ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
+        ("closest_node_visits", C.c_uint64), ("closest_tri_tests", C.c_uint64), ("closest_hits", C.c_uint64),
+        ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64),
+        ("closest_bvh_bytes", C.c_uint64), ("shadow_bvh_bytes", C.c_uint64),
+        ("total_ms", C.c_double), ("trace_closest_ms", C.c_double), ("trace_shadow_ms", C.c_double),
+        ("shade_ms", C.c_double), ("trace_closest_launches", C.c_uint64), ("trace_shadow_launches", C.c_uint64),
+        ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class agx_debug_summary(C.Structure):
+    _fields_ = [
+        ("entities", C.c_uint32), ("lights", C.c_uint32), ("blas_nodes", C.c_uint32), ("blas_prims", C.c_uint32),
+        ("tlas_nodes", C.c_uint32), ("max_blas_depth", C.c_uint32), ("max_tlas_depth", C.c_uint32),
+        ("device_bytes", C.c_uint64), ("world_centre_x", C.c_float), ("world_centre_y", C.c_float),
+        ("world_centre_z", C.c_float), ("world_radius", C.c_float),
+    ]
+
+
+# every symbol the public headers declare
+EXPORTS = [
+    "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
+    "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary",
+]
+
+
+class AgxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"agx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Loads libatrc_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not GPU_LIB_PATH.exists():
+        raise RuntimeError(f"{GPU_LIB_PATH} is missing: run `python -m atrc_amd.build gpu` (or __graft_entry__.build())")
+    lib = C.CDLL(str(GPU_LIB_PATH))
+    P = C.POINTER
+    fp = P(C.c_float)
+    ip = P(C.c_int32)
+    lib.agx_abi_version.restype = C.c_int
+    lib.agx_create.argtypes = [C.c_int, P(C.c_void_p)]
+    lib.agx_upload_scene.argtypes = [C.c_void_p, P(agx_scene_desc)]
+    lib.agx_render.argtypes = [C.c_void_p, P(agx_render_params), P(C.c_int32), fp, fp, fp, fp]
+    lib.agx_trace_rays.argtypes = [C.c_void_p, fp, fp, fp, fp, C.c_uint32, C.c_int32, ip, ip, fp, fp, fp]
+    lib.agx_get_stats.argtypes = [C.c_void_p, P(agx_stats_t)]
+    lib.agx_last_error.restype = C.c_char_p
+    lib.agx_destroy.argtypes = [C.c_void_p]
+    lib.agx_destroy.restype = None
+    lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
+    lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
+    if lib.agx_abi_version() != AGX_ABI_VERSION:
+        raise RuntimeError("libatrc_gpu.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def _check(code: int) -> None:
+    if code != AGX_OK:
+        raise AgxError(code, load_library().agx_last_error().decode())
+
+
+def fptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def iptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Context:
+    """One device context (agx_create / agx_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.agx_create(device, C.byref(h)))
+        self.h = h
+        self._keep = None
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.agx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_scene(self, desc: "agx_scene_desc") -> None:
+        _check(self.lib.agx_upload_scene(self.h, C.byref(desc)))
+
+    def render(self, params: agx_render_params, cancel: C.c_int32 | None = None) -> dict:
+        wh = params.width * params.height
+        rgb = np.zeros((params.height, params.width, 3), np.float32)
+        alb = np.zeros_like(rgb)
+        nor = np.zeros_like(rgb)
+        den = np.zeros((params.height, params.width), np.float32)
+        code = self.lib.agx_render(self.h, C.byref(params), C.byref(cancel) if cancel is not None else None,
+                                   fptr(rgb), fptr(alb), fptr(nor), fptr(den))
+        if code not in (AGX_OK, AGX_E_CANCELLED):
+            _check(code)
+        assert rgb.size == 3 * wh
+        return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den, "cancelled": code == AGX_E_CANCELLED}
+
+    def trace_rays(self, o: np.ndarray, d: np.ndarray, tmin: np.ndarray, tmax: np.ndarray, any_hit: bool = False):
+        n = len(o)
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        tmin = np.ascontiguousarray(tmin, np.float32)
+        tmax = np.ascontiguousarray(tmax, np.float32)
+        ent = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        _check(self.lib.agx_trace_rays(self.h, fptr(o), fptr(d), fptr(tmin), fptr(tmax), n, 1 if any_hit else 0,
+                                       iptr(ent), iptr(prim), fptr(t), fptr(u), fptr(v)))
+        return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v}
+
+    def stats(self) -> dict:
+        s = agx_stats_t()
+        _check(self.lib.agx_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+
+def debug_triangle_bvh(tris: np.ndarray):
+    """Atrc triangle BVH of a mesh (tris: (n*3, 8) float32 vertex records)."""
+    lib = load_library()
+    tris = np.ascontiguousarray(tris, np.float32)
+    n = len(tris) // 3
+    cnt = C.c_uint32()
+    depth = C.c_uint32()
+    vp = tris.ctypes.data_as(C.POINTER(agx_vertex))
+    _check(lib.agx_debug_triangle_bvh(vp, n, C.byref(cnt), C.byref(depth), None, None))
+    nodes = np.zeros((cnt.value, 8), np.float32)
+    prims = np.zeros((n, 12), np.float32)
+    _check(lib.agx_debug_triangle_bvh(vp, n, C.byref(cnt), C.byref(depth), fptr(nodes), fptr(prims)))
+    return nodes, prims, depth.value
+
+
+def debug_scene_summary(desc: agx_scene_desc) -> dict:
+    lib = load_library()
+    s = agx_debug_summary()
+    _check(lib.agx_debug_scene_summary(C.byref(desc), C.byref(s)))
+    return {name: getattr(s, name) for name, _ in s._fields_}

@@ -1,0 +1,510 @@
+// C ABI of the gfx950 path tracer (include/atrc_gpu.h): device context, scene
+// upload, the wavefront render loop, the ray-query parity hook and counters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/atrc_gpu.h"
+#include "../../include/atrc_gpu_debug.h"
+#include "agx_build.h"
+#include "agx_kernels.h"
+
+using namespace agx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(AGX_E_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T> hipError_t dalloc(T **p, size_t count) {
+    *p = nullptr;
+    if (!count) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <class T> hipError_t upload(T **p, const std::vector<T> &v, hipStream_t st) {
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess || v.empty()) return e;
+    return hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st);
+}
+
+}  // namespace
+
+struct agx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // scene
+    bool has_scene = false;
+    DevScene ds{};
+    std::vector<void *> scene_allocs;
+    int n_lights = 0;
+    // wavefront buffers
+    PathBuffers pb{};
+    std::vector<void *> wave_allocs;
+    size_t cap_paths = 0;
+    int cap_L = -1;
+    int64_t cap_wh = 0;
+    DevCounters *dcnt = nullptr;
+    uint32_t *h_counters = nullptr;  // pinned
+    hipEvent_t ev[8]{};
+    agx_stats_t stats{};
+
+    void free_scene() {
+        for (void *p : scene_allocs) if (p) (void)hipFree(p);
+        scene_allocs.clear();
+        has_scene = false;
+    }
+    void free_wave() {
+        for (void *p : wave_allocs) if (p) (void)hipFree(p);
+        wave_allocs.clear();
+        pb = PathBuffers{};
+        cap_paths = 0;
+        cap_L = -1;
+        cap_wh = 0;
+    }
+    template <class T> hipError_t walloc(T **p, size_t count) {
+        hipError_t e = dalloc(p, count);
+        if (e == hipSuccess) wave_allocs.push_back(*p);
+        return e;
+    }
+    int ensure_wave(size_t n, int L, int64_t wh) {
+        if (n <= cap_paths && L <= cap_L && wh <= cap_wh) return AGX_OK;
+        free_wave();
+        const size_t nl = n * size_t(std::max(L, 1));
+        HIP_TRY(walloc(&pb.coef, n));
+        HIP_TRY(walloc(&pb.radiance, n));
+        HIP_TRY(walloc(&pb.gb_albedo, n));
+        HIP_TRY(walloc(&pb.gb_normal, n));
+        HIP_TRY(walloc(&pb.rng, n));
+        HIP_TRY(walloc(&pb.depth, n));
+        HIP_TRY(walloc(&pb.flags, n));
+        HIP_TRY(walloc(&pb.mis_a, n));
+        HIP_TRY(walloc(&pb.mis_b, n));
+        HIP_TRY(walloc(&pb.nee, nl));
+        HIP_TRY(walloc(&pb.ray_o, 2 * n));
+        HIP_TRY(walloc(&pb.ray_d, 2 * n));
+        HIP_TRY(walloc(&pb.hit, 2 * n));
+        HIP_TRY(walloc(&pb.hit_prim, 2 * n));
+        HIP_TRY(walloc(&pb.sh_o, nl));
+        HIP_TRY(walloc(&pb.sh_d, nl));
+        HIP_TRY(walloc(&pb.occluded, nl));
+        HIP_TRY(walloc(&pb.q_closest, 2 * n));
+        HIP_TRY(walloc(&pb.q_shadow, nl));
+        HIP_TRY(walloc(&pb.q_in, n));
+        HIP_TRY(walloc(&pb.q_out, n));
+        HIP_TRY(walloc(&pb.counters, 4));
+        HIP_TRY(walloc(&pb.film_sum, size_t(10 * wh)));
+        HIP_TRY(walloc(&pb.film_comp, size_t(10 * wh)));
+        HIP_TRY(walloc(&pb.film_count, size_t(wh)));
+        cap_paths = n;
+        cap_L = std::max(L, 1);
+        cap_wh = wh;
+        return AGX_OK;
+    }
+};
+
+namespace {
+
+// Reads the three next-bounce queue sizes (synchronises the stream).
+int read_counters(agx_ctx *c, uint32_t out[3]) {
+    HIP_TRY(hipMemcpyAsync(c->h_counters, c->pb.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    out[0] = c->h_counters[0];
+    out[1] = c->h_counters[1];
+    out[2] = c->h_counters[2];
+    return AGX_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+// ThinLensCamera::init_from_params (thin_lens.cpp:34-55); look_at frame:
+// z = normalize(dst - pos), x = normalize(up x z), y = z x x.
+void camera_frame(const agx_camera_desc &c, RenderConst *rc) {
+    auto sub = [](const float *a, const float *b, float *o) { for (int i = 0; i < 3; ++i) o[i] = a[i] - b[i]; };
+    auto norm = [](float *v) {
+        const float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        for (int i = 0; i < 3; ++i) v[i] = v[i] / l;
+    };
+    auto cross = [](const float *a, const float *b, float *o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float z[3], x[3], y[3];
+    sub(c.dst, c.pos, z);
+    norm(z);
+    cross(c.up, z, x);
+    norm(x);
+    cross(z, x, y);
+    rc->cam_pos = make_float4(c.pos[0], c.pos[1], c.pos[2], 0);
+    rc->cam_x = make_float4(x[0], x[1], x[2], 0);
+    rc->cam_y = make_float4(y[0], y[1], y[2], 0);
+    rc->cam_z = make_float4(z[0], z[1], z[2], 0);
+    rc->focal = c.focal_distance;
+    rc->film_h = 2 * c.focal_distance * std::tan(c.fov_rad / 2);
+    rc->film_w = c.film_aspect * rc->film_h;
+    rc->lens_r = c.lens_radius;
+}
+
+void fill_traversal_stats(const DevCounters hc[2], agx_stats_t *st) {
+    st->closest_node_visits = hc[0].node_visits;
+    st->closest_tri_tests = hc[0].tri_tests;
+    st->closest_hits = hc[0].hits;
+    st->shadow_node_visits = hc[1].node_visits;
+    st->shadow_tri_tests = hc[1].tri_tests;
+    st->closest_bvh_bytes = hc[0].node_visits * st->node_bytes + hc[0].tri_tests * st->prim_bytes +
+                            hc[0].hits * st->info_bytes;
+    st->shadow_bvh_bytes = hc[1].node_visits * st->node_bytes + hc[1].tri_tests * st->prim_bytes;
+}
+
+size_t max_paths_in_flight() {
+    if (const char *s = std::getenv("AGX_MAX_PATHS")) {
+        const long long v = std::atoll(s);
+        if (v >= 1024) return size_t(v);
+    }
+    return size_t(1) << 22;
+}
+
+}  // namespace
+
+extern "C" {
+
+int agx_abi_version(void) { return AGX_ABI_VERSION; }
+
+const char *agx_last_error(void) { return g_err.c_str(); }
+
+int agx_create(int device_ordinal, agx_ctx **out) {
+    if (!out) return fail(AGX_E_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return fail(AGX_E_HIP, "no HIP device available");
+    if (device_ordinal < 0 || device_ordinal >= count) return fail(AGX_E_INVALID_ARG, "device ordinal out of range");
+    HIP_TRY(hipSetDevice(device_ordinal));
+    auto *c = new agx_ctx();
+    c->device = device_ordinal;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(AGX_E_HIP, "hipStreamCreate failed");
+    }
+    if (hipMalloc(reinterpret_cast<void **>(&c->dcnt), 2 * sizeof(DevCounters)) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), 4 * sizeof(uint32_t), hipHostMallocDefault) !=
+            hipSuccess) {
+        agx_destroy(c);
+        return fail(AGX_E_OUT_OF_MEMORY, "allocation of counters failed");
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            agx_destroy(c);
+            return fail(AGX_E_HIP, "hipEventCreate failed");
+        }
+    *out = c;
+    return AGX_OK;
+}
+
+void agx_destroy(agx_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->free_scene();
+    c->free_wave();
+    if (c->dcnt) (void)hipFree(c->dcnt);
+    if (c->h_counters) (void)hipHostFree(c->h_counters);
+    for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
+    if (!c || !desc) return fail(AGX_E_INVALID_ARG, "null argument");
+    HostScene hs;
+    std::string err;
+    const int rc = build_host_scene(*desc, &hs, &err);
+    if (rc != AGX_OK) return fail(rc, err);
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->free_scene();
+    DevScene ds{};
+    DevEntity *ents = nullptr;
+    DevMaterial *mats = nullptr;
+    vec4 *bn = nullptr, *bp = nullptr, *bi = nullptr, *tn = nullptr;
+    int32_t *tp = nullptr, *ai = nullptr;
+    float *ap = nullptr;
+    auto track = [&](void *p) { if (p) c->scene_allocs.push_back(p); };
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) { e = upload(&ents, hs.ents, c->stream); track(ents); }
+    if (e == hipSuccess) { e = upload(&mats, hs.mats, c->stream); track(mats); }
+    if (e == hipSuccess) { e = upload(&bn, hs.blas_nodes, c->stream); track(bn); }
+    if (e == hipSuccess) { e = upload(&bp, hs.blas_prims, c->stream); track(bp); }
+    if (e == hipSuccess) { e = upload(&bi, hs.blas_info, c->stream); track(bi); }
+    if (e == hipSuccess) { e = upload(&tn, hs.tlas_nodes, c->stream); track(tn); }
+    if (e == hipSuccess) { e = upload(&tp, hs.tlas_prims, c->stream); track(tp); }
+    if (e == hipSuccess) { e = upload(&ap, hs.alias_prob, c->stream); track(ap); }
+    if (e == hipSuccess) { e = upload(&ai, hs.alias_idx, c->stream); track(ai); }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        c->free_scene();
+        return fail(e == hipErrorOutOfMemory ? AGX_E_OUT_OF_MEMORY : AGX_E_HIP,
+                    std::string("scene upload: ") + hipGetErrorString(e));
+    }
+    ds.ents = ents;
+    ds.mats = mats;
+    ds.blas_nodes = bn;
+    ds.blas_prims = bp;
+    ds.blas_info = bi;
+    ds.tlas_nodes = tn;
+    ds.tlas_prims = tp;
+    ds.alias_prob = ap;
+    ds.alias_idx = ai;
+    ds.n_ent = int32_t(hs.ents.size());
+    ds.aggregate = hs.aggregate;
+    ds.n_lights = int32_t(hs.lights.size());
+    ds.env_kind = hs.env_kind;
+    for (int i = 0; i < MAX_LIGHTS; ++i) ds.lights[i] = i < ds.n_lights ? hs.lights[size_t(i)] : -2;
+    ds.sky_top = hs.sky_top;
+    ds.sky_bottom = hs.sky_bottom;
+    ds.world = hs.world;
+    c->ds = ds;
+    c->n_lights = ds.n_lights;
+    c->has_scene = true;
+    return AGX_OK;
+}
+
+int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *cancel_flag, float *rgb,
+               float *albedo, float *normal, float *denoise) {
+    if (!c || !p) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0) return fail(AGX_E_INVALID_ARG, "width, height and spp must be positive");
+    if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
+    if (p->max_depth < 1 || p->min_depth < 0 || !(p->cont_prob > 0) || !(p->eps > 0))
+        return fail(AGX_E_INVALID_ARG, "invalid depth / cont_prob / eps");
+    const float fr = p->filter_radius - p->eps;
+    if (!(fr > 0)) return fail(AGX_E_INVALID_ARG, "invalid box filter radius");
+    if (fr > 0.5f) return fail(AGX_E_UNSUPPORTED, "box filter radius above 0.5 + eps is not supported by gpu_pt");
+    HIP_TRY(hipSetDevice(c->device));
+
+    RenderConst rc{};
+    rc.W = p->width;
+    rc.H = p->height;
+    rc.min_depth = p->min_depth;
+    rc.max_depth = p->max_depth;
+    rc.specular_depth = p->specular_depth;
+    rc.cont_prob = p->cont_prob;
+    rc.eps = p->eps;
+    rc.filter_r = fr;
+    rc.seed = p->seed;
+    rc.sample_begin = p->sample_begin;
+    rc.L = c->n_lights;
+    camera_frame(p->camera, &rc);
+
+    const int64_t wh = int64_t(p->width) * p->height;
+    const int64_t total = wh * p->spp;
+    const size_t cap = max_paths_in_flight();
+    int64_t per_pass;
+    if (wh <= int64_t(cap)) per_pass = wh * std::min<int64_t>(p->spp, int64_t(cap) / wh);
+    else per_pass = int64_t(cap);
+    int r = c->ensure_wave(size_t(per_pass), rc.L, wh);
+    if (r != AGX_OK) return r;
+
+    const bool stats = p->collect_stats != 0;
+    agx_stats_t st{};
+    st.node_bytes = 32;
+    st.prim_bytes = 48;
+    st.info_bytes = 96;
+    if (stats) HIP_TRY(hipMemsetAsync(c->dcnt, 0, 2 * sizeof(DevCounters), c->stream));
+    launch_film_clear(c->pb, wh, c->stream);
+    HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    bool cancelled = false;
+    for (int64_t base = 0; base < total && !cancelled; base += per_pass) {
+        rc.work_base = base;
+        rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
+        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+        launch_shade(c->ds, rc, c->pb, 0, c->stream);  // resets the queue counters
+        launch_raygen(c->ds, rc, c->pb, c->stream);
+        HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+        uint32_t q[3];
+        if ((r = read_counters(c, q)) != AGX_OK) return r;
+        st.shade_ms += elapsed(c->ev[2], c->ev[3]);
+        st.paths += uint64_t(rc.n_paths);
+        while (q[2] > 0) {
+            std::swap(c->pb.q_in, c->pb.q_out);
+            HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+            launch_trace_closest(c->ds, c->pb, q[0], stats, c->dcnt, c->stream);
+            HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+            launch_trace_shadow(c->ds, c->pb, q[1], stats, c->dcnt + 1, c->stream);
+            HIP_TRY(hipEventRecord(c->ev[4], c->stream));
+            launch_shade(c->ds, rc, c->pb, q[2], c->stream);
+            HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+            st.closest_rays += q[0];
+            st.shadow_rays += q[1];
+            if (q[0]) st.trace_closest_launches++;
+            if (q[1]) st.trace_shadow_launches++;
+            if ((r = read_counters(c, q)) != AGX_OK) return r;
+            st.trace_closest_ms += elapsed(c->ev[2], c->ev[3]);
+            st.trace_shadow_ms += elapsed(c->ev[3], c->ev[4]);
+            st.shade_ms += elapsed(c->ev[4], c->ev[5]);
+            if (cancel_flag && *cancel_flag) { cancelled = true; break; }
+        }
+        if (cancelled) break;
+        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+        launch_film(rc, c->pb, c->stream);
+        HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+        HIP_TRY(hipEventSynchronize(c->ev[3]));
+        st.shade_ms += elapsed(c->ev[2], c->ev[3]);
+        if (cancel_flag && *cancel_flag) cancelled = true;
+    }
+    HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
+    st.total_ms = elapsed(c->ev[0], c->ev[1]);
+    if (stats) {
+        DevCounters hc[2]{};
+        HIP_TRY(hipMemcpy(hc, c->dcnt, sizeof(hc), hipMemcpyDeviceToHost));
+        fill_traversal_stats(hc, &st);
+    }
+    c->stats = st;
+    // normalise into host outputs
+    if (rgb || albedo || normal || denoise) {
+        float *dout = nullptr;
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&dout), size_t(wh) * 10 * sizeof(float)));
+        launch_film_resolve(c->pb, wh, dout, dout + 3 * wh, dout + 6 * wh, dout + 9 * wh, c->stream);
+        std::vector<float> host(size_t(wh) * 10);
+        hipError_t e = hipMemcpyAsync(host.data(), dout, host.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(dout);
+        if (e != hipSuccess) return fail(AGX_E_HIP, std::string("film readback: ") + hipGetErrorString(e));
+        if (rgb) std::memcpy(rgb, host.data(), size_t(wh) * 3 * sizeof(float));
+        if (albedo) std::memcpy(albedo, host.data() + 3 * wh, size_t(wh) * 3 * sizeof(float));
+        if (normal) std::memcpy(normal, host.data() + 6 * wh, size_t(wh) * 3 * sizeof(float));
+        if (denoise) std::memcpy(denoise, host.data() + 9 * wh, size_t(wh) * sizeof(float));
+    }
+    if (cancelled) return fail(AGX_E_CANCELLED, "render cancelled");
+    return AGX_OK;
+}
+
+int agx_trace_rays(agx_ctx *c, const float *o, const float *d, const float *tmin, const float *tmax, uint32_t n,
+                   int32_t any_hit, int32_t *entity_id, int32_t *prim_id, float *t, float *u, float *v) {
+    if (!c || (n && (!o || !d || !tmin || !tmax || !entity_id))) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
+    if (!n) return AGX_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<float4> ho(n), hd(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        ho[i] = make_float4(o[3 * i], o[3 * i + 1], o[3 * i + 2], tmin[i]);
+        hd[i] = make_float4(d[3 * i], d[3 * i + 1], d[3 * i + 2], tmax[i]);
+    }
+    char *buf = nullptr;
+    const size_t bytes = size_t(n) * (16 + 16 + 4 * 5);
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&buf), bytes));
+    float4 *dO = reinterpret_cast<float4 *>(buf);
+    float4 *dD = dO + n;
+    int32_t *dE = reinterpret_cast<int32_t *>(dD + n);
+    int32_t *dP = dE + n;
+    float *dT = reinterpret_cast<float *>(dP + n);
+    float *dU = dT + n;
+    float *dV = dU + n;
+    hipError_t e = hipMemcpyAsync(dO, ho.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dD, hd.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->dcnt, 0, 2 * sizeof(DevCounters), c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+    if (e == hipSuccess)
+        launch_trace_rays(c->ds, dO, dD, n, any_hit, dE, dP, dT, dU, dV, c->dcnt + (any_hit ? 1 : 0), c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+    std::vector<int32_t> he(n), hp(n);
+    std::vector<float> ht(n), hu(n), hv(n);
+    if (e == hipSuccess) e = hipMemcpyAsync(he.data(), dE, n * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !any_hit) {
+        e = hipMemcpyAsync(hp.data(), dP, n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ht.data(), dT, n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hu.data(), dU, n * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hv.data(), dV, n * 4, hipMemcpyDeviceToHost, c->stream);
+    }
+    DevCounters hc[2]{};
+    if (e == hipSuccess) e = hipMemcpyAsync(hc, c->dcnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(AGX_E_HIP, std::string("trace_rays: ") + hipGetErrorString(e));
+    std::memcpy(entity_id, he.data(), n * 4);
+    if (!any_hit) {
+        if (prim_id) std::memcpy(prim_id, hp.data(), n * 4);
+        if (t) std::memcpy(t, ht.data(), n * 4);
+        if (u) std::memcpy(u, hu.data(), n * 4);
+        if (v) std::memcpy(v, hv.data(), n * 4);
+    }
+    agx_stats_t st{};
+    st.node_bytes = 32;
+    st.prim_bytes = 48;
+    st.info_bytes = 96;
+    if (any_hit) st.shadow_rays = n; else st.closest_rays = n;
+    fill_traversal_stats(hc, &st);
+    st.total_ms = elapsed(c->ev[0], c->ev[1]);
+    if (any_hit) { st.trace_shadow_ms = st.total_ms; st.trace_shadow_launches = 1; }
+    else { st.trace_closest_ms = st.total_ms; st.trace_closest_launches = 1; }
+    c->stats = st;
+    return AGX_OK;
+}
+
+int agx_get_stats(agx_ctx *c, agx_stats_t *out) {
+    if (!c || !out) return fail(AGX_E_INVALID_ARG, "null argument");
+    *out = c->stats;
+    return AGX_OK;
+}
+
+// ---------------------------------------------------------------- host-only test hooks
+int agx_debug_triangle_bvh(const agx_vertex *tris, uint32_t n, uint32_t *node_count, uint32_t *max_depth,
+                           float *nodes_out, float *prims_out) {
+    if (!tris || !n || !node_count) return fail(AGX_E_INVALID_ARG, "null argument");
+    std::vector<vec4> nodes, prims, info;
+    double sa = 0;
+    const int depth = build_triangle_bvh(tris, n, &nodes, &prims, &info, &sa);
+    *node_count = uint32_t(nodes.size() / 2);
+    if (max_depth) *max_depth = uint32_t(depth);
+    if (nodes_out) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(vec4));
+    if (prims_out) std::memcpy(prims_out, prims.data(), prims.size() * sizeof(vec4));
+    return AGX_OK;
+}
+
+int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out) {
+    if (!desc || !out) return fail(AGX_E_INVALID_ARG, "null argument");
+    HostScene hs;
+    std::string err;
+    const int rc = build_host_scene(*desc, &hs, &err);
+    if (rc != AGX_OK) return fail(rc, err);
+    out->entities = uint32_t(hs.ents.size());
+    out->lights = uint32_t(hs.lights.size());
+    out->blas_nodes = uint32_t(hs.blas_nodes.size() / 2);
+    out->blas_prims = uint32_t(hs.blas_prims.size() / 3);
+    out->tlas_nodes = uint32_t(hs.tlas_nodes.size() / 2);
+    out->max_blas_depth = uint32_t(hs.max_blas_depth);
+    out->max_tlas_depth = uint32_t(hs.max_tlas_depth);
+    out->device_bytes = uint64_t(hs.ents.size() * sizeof(DevEntity) + hs.mats.size() * sizeof(DevMaterial) +
+                                 (hs.blas_nodes.size() + hs.blas_prims.size() + hs.blas_info.size() +
+                                  hs.tlas_nodes.size()) * sizeof(vec4) +
+                                 (hs.tlas_prims.size() + hs.alias_idx.size() + hs.alias_prob.size()) * 4);
+    out->world_centre_x = hs.world.x;
+    out->world_centre_y = hs.world.y;
+    out->world_centre_z = hs.world.z;
+    out->world_radius = hs.world.w;
+    return AGX_OK;
+}
+
+}  // extern "C"

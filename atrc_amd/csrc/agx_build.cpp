@@ -1,0 +1,528 @@
+// Host-side scene construction for the gfx950 path tracer.
+//
+// The triangle BVH and the entity BVH are built exactly as Atrc builds them
+// (same split rules, same node order, same float arithmetic), so that in the
+// default mode the GPU traverses the very tree Atrc's CPU `pt` would traverse
+// and ray-query results agree bit for bit:
+//   triangle BVH: src/tracer/src/core/geometry/triangle_bvh.cpp:122-304, 320-361
+//   entity BVH:   src/tracer/src/core/aggregate/entity_bvh.cpp:46-116
+// This file is compiled with -ffp-contract=off.
+#include "agx_build.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <limits>
+
+namespace agx {
+
+namespace {
+
+struct F3 {
+    float x, y, z;
+    float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline F3 add(F3 a, F3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline F3 mul(F3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline F3 divs(F3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross3(F3 a, F3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float len3(F3 a) { return std::sqrt(dot3(a, a)); }
+inline F3 unit(F3 a) { return divs(a, len3(a)); }
+inline F3 neg(F3 a) { return {-a.x, -a.y, -a.z}; }
+inline vec4 v4(F3 a, float w) { return vec4{a.x, a.y, a.z, w}; }
+inline float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+struct Box {
+    F3 lo{std::numeric_limits<float>::max(), std::numeric_limits<float>::max(), std::numeric_limits<float>::max()};
+    F3 hi{std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest(), std::numeric_limits<float>::lowest()};
+    void add_point(F3 p) {
+        lo = {std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z)};
+        hi = {std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z)};
+    }
+    void add_box(const Box &b) {
+        lo = {std::min(lo.x, b.lo.x), std::min(lo.y, b.lo.y), std::min(lo.z, b.lo.z)};
+        hi = {std::max(hi.x, b.hi.x), std::max(hi.y, b.hi.y), std::max(hi.z, b.hi.z)};
+    }
+    // the world_bound() padding Atrc applies to flat geometry (quad.cpp:67-71)
+    void pad_flat() {
+        float *l = &lo.x, *h = &hi.x;
+        for (int i = 0; i < 3; ++i)
+            if (l[i] >= h[i]) l[i] = h[i] - 0.1f * std::abs(h[i]);
+    }
+};
+
+// Duff et al. 2017 orthonormal basis; returns the x axis.
+F3 basis_x_from_z(F3 zin) {
+    const F3 n = unit(zin);
+    const float sign = std::copysign(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    return {1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x};
+}
+
+// dpdu_as_ex (triangle_aux.h:76-88)
+F3 tangent_x(F3 B_A, F3 C_A, float bu, float bv, float cu, float cv, F3 nor) {
+    const float det = bu * cv - bv * cu;
+    if (!det) return basis_x_from_z(nor);
+    const float inv = 1.0f / det;
+    return unit(sub(mul(B_A, cv * inv), mul(C_A, bv * inv)));
+}
+
+float tri_area(F3 B_A, F3 C_A) { return len3(cross3(B_A, C_A)) / 2; }
+
+F3 vpos(const agx_vertex &v) { return {v.pos[0], v.pos[1], v.pos[2]}; }
+F3 vnor(const agx_vertex &v) { return {v.nor[0], v.nor[1], v.nor[2]}; }
+
+// Vose's alias method: same pairing order as the CPU oracle (small/large stacks,
+// last-in first-out), probabilities computed in double and stored as float.
+void build_alias(const std::vector<float> &w, std::vector<float> *prob, std::vector<int32_t> *alias) {
+    const size_t n = w.size();
+    std::vector<float> p(n, 1.0f);
+    std::vector<int32_t> al(n);
+    double sum = 0;
+    for (float x : w) sum += double(x);
+    std::vector<double> q(n);
+    std::vector<int32_t> small, large;
+    for (size_t i = 0; i < n; ++i) {
+        q[i] = double(w[i]) * double(n) / sum;
+        al[i] = int32_t(i);
+        if (q[i] < 1.0) small.push_back(int32_t(i)); else large.push_back(int32_t(i));
+    }
+    while (!small.empty() && !large.empty()) {
+        const int32_t s = small.back();
+        small.pop_back();
+        const int32_t l = large.back();
+        large.pop_back();
+        p[size_t(s)] = float(q[size_t(s)]);
+        al[size_t(s)] = l;
+        q[size_t(l)] = (q[size_t(l)] + q[size_t(s)]) - 1.0;
+        if (q[size_t(l)] < 1.0) small.push_back(l); else large.push_back(l);
+    }
+    for (int32_t i : large) p[size_t(i)] = 1.0f;
+    for (int32_t i : small) p[size_t(i)] = 1.0f;
+    prob->insert(prob->end(), p.begin(), p.end());
+    alias->insert(alias->end(), al.begin(), al.end());
+}
+
+}  // namespace
+
+int build_triangle_bvh(const agx_vertex *tris, uint32_t n, std::vector<vec4> *nodes, std::vector<vec4> *prims,
+                       std::vector<vec4> *info, double *surface_area) {
+    // triangle references in build order
+    struct Ref { uint32_t id; F3 c; };
+    std::vector<Ref> refs(n);
+    float area = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const F3 p0 = vpos(tris[3 * i]), p1 = vpos(tris[3 * i + 1]), p2 = vpos(tris[3 * i + 2]);
+        refs[i].id = i;
+        refs[i].c = divs(add(add(p0, p1), p2), 3.0f);
+        area += tri_area(sub(p1, p0), sub(p2, p0));
+    }
+    if (surface_area) *surface_area = double(area);
+
+    // Breadth-first split (triangle_bvh.cpp:122-223).  Each build node records
+    // its bound, its triangle range and its two children (or -1).
+    struct BNode { Box box; int32_t kid[2]; uint32_t begin, end; };
+    std::vector<BNode> tree;
+    tree.reserve(size_t(n) / 2 + 16);
+    struct Job { int32_t parent; int side; uint32_t begin, end, depth; };
+    std::deque<Job> jobs;
+    jobs.push_back({-1, 0, 0, n, 0});
+    int max_depth = 0;
+    while (!jobs.empty()) {
+        const Job job = jobs.front();
+        jobs.pop_front();
+        Box all, cen;
+        for (uint32_t i = job.begin; i < job.end; ++i) {
+            const uint32_t t = refs[i].id;
+            all.add_point(vpos(tris[3 * t]));
+            all.add_point(vpos(tris[3 * t + 1]));
+            all.add_point(vpos(tris[3 * t + 2]));
+            cen.add_point(refs[i].c);
+        }
+        const int32_t me = int32_t(tree.size());
+        tree.push_back({all, {-1, -1}, job.begin, job.end});
+        if (job.parent >= 0) tree[size_t(job.parent)].kid[job.side] = me;
+        max_depth = std::max(max_depth, int(job.depth));
+        const uint32_t count = job.end - job.begin;
+        if (count <= 5u) continue;  // leaf threshold
+        const F3 ext = sub(cen.hi, cen.lo);
+        const int axis = ext.x > ext.y ? (ext.x > ext.z ? 0 : 2) : (ext.y > ext.z ? 1 : 2);
+        uint32_t mid = job.begin;
+        if (job.depth < 64u) {
+            const float split = 0.5f * (cen.hi[axis] + cen.lo[axis]);
+            for (uint32_t i = job.begin; i < job.end; ++i)
+                if (refs[i].c[axis] < split) std::swap(refs[i], refs[mid++]);
+            if (mid == job.begin || mid == job.end) mid = job.begin + count / 2;
+        } else {
+            std::sort(refs.begin() + job.begin, refs.begin() + job.end,
+                      [axis](const Ref &l, const Ref &r) { return l.c[axis] < r.c[axis]; });
+            mid = job.begin + count / 2;
+        }
+        jobs.push_back({me, 0, job.begin, mid, job.depth + 1});
+        jobs.push_back({me, 1, mid, job.end, job.depth + 1});
+    }
+
+    // Depth-first flattening, left child immediately after its parent
+    // (compact_bvh, triangle_bvh.cpp:225-304).
+    nodes->clear();
+    prims->clear();
+    info->clear();
+    nodes->reserve(tree.size() * 2);
+    prims->reserve(size_t(n) * 3);
+    info->reserve(size_t(n) * 6);
+    struct Visit { int32_t node; int64_t patch; };
+    std::vector<Visit> todo;
+    todo.push_back({0, -1});
+    uint32_t next_prim = 0;
+    while (!todo.empty()) {
+        const Visit v = todo.back();
+        todo.pop_back();
+        const BNode &b = tree[size_t(v.node)];
+        const uint32_t index = uint32_t(nodes->size() / 2);
+        if (v.patch >= 0) (*nodes)[size_t(v.patch)].w = bits_f(index);
+        if (b.kid[0] >= 0 && b.kid[1] >= 0) {
+            nodes->push_back(v4(b.box.lo, bits_f(INTERIOR)));
+            nodes->push_back(v4(b.box.hi, bits_f(0)));
+            todo.push_back({b.kid[1], int64_t(nodes->size()) - 1});
+            todo.push_back({b.kid[0], -1});
+            continue;
+        }
+        const uint32_t first = next_prim, last = next_prim + (b.end - b.begin);
+        nodes->push_back(v4(b.box.lo, bits_f(first)));
+        nodes->push_back(v4(b.box.hi, bits_f(last)));
+        for (uint32_t j = b.begin; j < b.end; ++j) {
+            const uint32_t t = refs[j].id;
+            const agx_vertex &va = tris[3 * t], &vb = tris[3 * t + 1], &vc = tris[3 * t + 2];
+            const F3 A = vpos(va), B_A = sub(vpos(vb), A), C_A = sub(vpos(vc), A);
+            prims->push_back(v4(A, bits_f(t)));
+            prims->push_back(v4(B_A, 0.0f));
+            prims->push_back(v4(C_A, 0.0f));
+            const F3 na = unit(vnor(va)), nb = unit(vnor(vb)), nc = unit(vnor(vc));
+            const float tbu = vb.uv[0] - va.uv[0], tbv = vb.uv[1] - va.uv[1];
+            const float tcu = vc.uv[0] - va.uv[0], tcv = vc.uv[1] - va.uv[1];
+            F3 z = unit(cross3(B_A, C_A));
+            if (dot3(add(add(na, nb), nc), z) < 0) z = neg(z);
+            const F3 x = tangent_x(B_A, C_A, tbu, tbv, tcu, tcv, z);
+            info->push_back(v4(na, va.uv[0]));
+            info->push_back(v4(sub(nb, na), va.uv[1]));
+            info->push_back(v4(sub(nc, na), tbu));
+            info->push_back(v4(x, tbv));
+            info->push_back(v4(z, tcu));
+            info->push_back(vec4{tcv, 0.0f, 0.0f, 0.0f});
+        }
+        next_prim = last;
+    }
+    return max_depth;
+}
+
+namespace {
+
+struct EntRec { int32_t id; Box box; };
+
+struct TlasBuilder {
+    std::vector<vec4> *nodes;
+    std::vector<int32_t> *prims;
+    int max_leaf;
+    int max_depth = 0;
+    // EntityBVH::build_aux (entity_bvh.cpp:46-116): pre-order, median split by
+    // sorting on the bound centre along the largest extent.
+    uint32_t build(EntRec *r, size_t count, int depth) {
+        max_depth = std::max(max_depth, depth);
+        const uint32_t me = uint32_t(nodes->size() / 2);
+        if (count <= size_t(max_leaf) || count < 2) {
+            Box b;
+            const uint32_t first = uint32_t(prims->size());
+            for (size_t i = 0; i < count; ++i) { prims->push_back(r[i].id); b.add_box(r[i].box); }
+            nodes->push_back(v4(b.lo, bits_f(first | TLAS_LEAF)));
+            nodes->push_back(v4(b.hi, bits_f(first + uint32_t(count))));
+            return me;
+        }
+        Box all;
+        for (size_t i = 0; i < count; ++i) all.add_box(r[i].box);
+        float best = -1;
+        int axis = 0;
+        for (int i = 0; i < 3; ++i) {
+            const float l = all.hi[i] - all.lo[i];
+            if (l > best) { best = l; axis = i; }
+        }
+        std::sort(r, r + count, [axis](const EntRec &a, const EntRec &b) {
+            return a.box.lo[axis] + a.box.hi[axis] < b.box.lo[axis] + b.box.hi[axis];
+        });
+        nodes->push_back(v4(all.lo, 0.0f));
+        nodes->push_back(v4(all.hi, 0.0f));
+        const size_t half = count / 2;
+        const uint32_t l = build(r, half, depth + 1);
+        const uint32_t rr = build(r + half, count - half, depth + 1);
+        (*nodes)[size_t(me) * 2].w = bits_f(l);
+        (*nodes)[size_t(me) * 2 + 1].w = bits_f(rr);
+        return me;
+    }
+};
+
+void set_rows(vec4 *rows, const float *m) {
+    for (int r = 0; r < 3; ++r) rows[r] = vec4{m[4 * r], m[4 * r + 1], m[4 * r + 2], m[4 * r + 3]};
+}
+
+F3 apply_point(const float *m, F3 p) {
+    return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+            m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+}
+F3 apply_vec(const float *m, F3 v) {
+    return {m[0] * v.x + m[1] * v.y + m[2] * v.z, m[4] * v.x + m[5] * v.y + m[6] * v.z,
+            m[8] * v.x + m[9] * v.y + m[10] * v.z};
+}
+
+}  // namespace
+
+int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *err) {
+    HostScene &S = *out;
+    S = HostScene();
+    if (desc.abi_version != AGX_ABI_VERSION) { *err = "scene descriptor ABI version mismatch"; return AGX_E_INVALID_ARG; }
+    if (desc.entity_count && !desc.entities) { *err = "entities is NULL"; return AGX_E_INVALID_ARG; }
+    if (desc.material_count && !desc.materials) { *err = "materials is NULL"; return AGX_E_INVALID_ARG; }
+
+    // materials: derive what Material::shade derives per hit
+    for (uint32_t i = 0; i < desc.material_count; ++i) {
+        const agx_material_desc &m = desc.materials[i];
+        DevMaterial d;
+        std::memset(&d, 0, sizeof(d));
+        d.kind = m.kind;
+        const F3 col{m.color[0], m.color[1], m.color[2]};
+        const float PI = 3.14159265358979323846f;
+        switch (m.kind) {
+        case AGX_MAT_IDEAL_DIFFUSE:  // ideal_diffuse.cpp:25-39, diffuse_comp.cpp:5-8
+            d.albedo = v4(col, 0); d.coef = v4(divs(col, PI), 0);
+            break;
+        case AGX_MAT_METAL: {        // metal.cpp + material_creators.cpp:252-266 (eta/k swap)
+            d.albedo = v4(col, 0); d.color = v4(col, 0);
+            d.eta2 = vec4{m.k[0] * m.k[0], m.k[1] * m.k[1], m.k[2] * m.k[2], 0};
+            d.k2 = vec4{m.eta[0] * m.eta[0], m.eta[1] * m.eta[1], m.eta[2] * m.eta[2], 0};
+            const float an = m.anisotropic, r2 = m.roughness * m.roughness;
+            const float aspect = an > 0 ? std::sqrt(1 - 0.9f * an) : 1.0f;   // microfacet_refl_comp.cpp:72-75
+            d.misc = vec4{std::max(0.001f, r2 / aspect), std::max(0.001f, r2 * aspect), 0, 0};
+            break;
+        }
+        case AGX_MAT_GLASS: {        // glass.cpp:180-206
+            if (!(m.ior > 0)) { *err = "glass: invalid ior"; return AGX_E_INVALID_ARG; }
+            const F3 c2 = m.glass_separate_refraction ? F3{m.color2[0], m.color2[1], m.color2[2]} : col;
+            const F3 refr{std::sqrt(c2.x), std::sqrt(c2.y), std::sqrt(c2.z)};
+            d.c_refl = v4(col, 0); d.c_refr = v4(refr, 0);
+            d.albedo = v4(mul(add(col, refr), 0.5f), 0);
+            d.misc = vec4{0, 0, m.ior, 0};
+            break;
+        }
+        case AGX_MAT_MIRROR: {       // mirror.cpp:98-115: ConductorPoint(eta_out = eta, eta_in = 1, k)
+            d.albedo = v4(col, 0);
+            float e2[3], k2[3];
+            for (int c = 0; c < 3; ++c) {
+                const float inv = 1 / m.eta[c];
+                e2[c] = inv * inv;
+                const float kk = m.k[c] / m.eta[c];
+                k2[c] = kk * kk;
+            }
+            d.eta2 = vec4{e2[0], e2[1], e2[2], 0};
+            d.k2 = vec4{k2[0], k2[1], k2[2], 0};
+            break;
+        }
+        case AGX_MAT_IDEAL_BLACK:
+            break;
+        default:
+            *err = "unsupported material kind " + std::to_string(m.kind);
+            return AGX_E_UNSUPPORTED;
+        }
+        S.mats.push_back(d);
+    }
+
+    std::vector<Box> bounds(desc.entity_count);
+    for (uint32_t i = 0; i < desc.entity_count; ++i) {
+        const agx_entity_desc &e = desc.entities[i];
+        DevEntity d;
+        std::memset(&d, 0, sizeof(d));
+        d.kind = e.geometry;
+        d.material = e.material;
+        d.light = -1;
+        d.flags = e.no_denoise ? 1 : 0;
+        if (e.material < 0 || uint32_t(e.material) >= desc.material_count) {
+            *err = "entity " + std::to_string(i) + ": material index out of range";
+            return AGX_E_INVALID_ARG;
+        }
+        if (e.medium_in >= 0 || e.medium_out >= 0) {
+            *err = "entity " + std::to_string(i) + ": participating media are not supported by gpu_pt yet";
+            return AGX_E_UNSUPPORTED;
+        }
+        set_rows(d.l2w, e.local_to_world);
+        set_rows(d.w2l, e.world_to_local);
+        const float ratio = len3(apply_vec(e.local_to_world, F3{1, 0, 0}));
+        auto P = [&](int k) { return F3{e.p[k][0], e.p[k][1], e.p[k][2]}; };
+        Box box;
+        float area = 0;
+        switch (e.geometry) {
+        case AGX_GEOM_QUAD: {                               // quad.cpp:138-164
+            const F3 a = P(0), ba = sub(P(1), a), ca = sub(P(2), a), da = sub(P(3), a);
+            const float tbu = e.t[1][0] - e.t[0][0], tbv = e.t[1][1] - e.t[0][1];
+            const float tcu = e.t[2][0] - e.t[0][0], tcv = e.t[2][1] - e.t[0][1];
+            const float tdu = e.t[3][0] - e.t[0][0], tdv = e.t[3][1] - e.t[0][1];
+            const F3 z = unit(cross3(ba, ca));
+            d.a = v4(a, 0); d.b_a = v4(ba, 0); d.c_a = v4(ca, 0); d.d_a = v4(da, 0);
+            d.z = v4(z, 0);
+            d.x_abc = v4(tangent_x(ba, ca, tbu, tbv, tcu, tcv, z), 0);
+            d.x_acd = v4(tangent_x(ca, da, tcu, tcv, tdu, tdv, z), 0);
+            d.t_ab = vec4{e.t[0][0], e.t[0][1], tbu, tbv};
+            d.t_cd = vec4{tcu, tcv, tdu, tdv};
+            const float a1 = tri_area(ba, ca), a2 = tri_area(ca, da);
+            area = a1 + a2;
+            d.sample_prob = vec4{a1 / area, 0, 0, 0};
+            box.add_point(a); box.add_point(add(a, ba)); box.add_point(add(a, ca)); box.add_point(add(a, da));
+            box.pad_flat();
+            break;
+        }
+        case AGX_GEOM_TRIANGLE: {                           // triangle.cpp:107-125
+            const F3 a = P(0), ba = sub(P(1), a), ca = sub(P(2), a);
+            const float tbu = e.t[1][0] - e.t[0][0], tbv = e.t[1][1] - e.t[0][1];
+            const float tcu = e.t[2][0] - e.t[0][0], tcv = e.t[2][1] - e.t[0][1];
+            const F3 z = unit(cross3(ba, ca));
+            d.a = v4(a, 0); d.b_a = v4(ba, 0); d.c_a = v4(ca, 0);
+            d.z = v4(z, 0);
+            d.x_abc = v4(tangent_x(ba, ca, tbu, tbv, tcu, tcv, z), 0);
+            d.t_ab = vec4{e.t[0][0], e.t[0][1], tbu, tbv};
+            d.t_cd = vec4{tcu, tcv, 0, 0};
+            area = tri_area(apply_vec(e.local_to_world, ba), apply_vec(e.local_to_world, ca));
+            box.add_point(apply_point(e.local_to_world, a));
+            box.add_point(apply_point(e.local_to_world, add(a, ba)));
+            box.add_point(apply_point(e.local_to_world, add(a, ca)));
+            box.pad_flat();
+            break;
+        }
+        case AGX_GEOM_SPHERE: {                             // sphere.cpp:14-24, 60-72
+            if (!(e.radius > 0)) { *err = "invalid sphere radius"; return AGX_E_INVALID_ARG; }
+            const float wr = ratio * e.radius;
+            d.a = vec4{e.radius, e.radius * e.radius, wr, ratio};
+            area = 4 * 3.14159265358979323846f * wr * wr;
+            const F3 c = apply_point(e.local_to_world, F3{0, 0, 0});
+            const float pad = wr + 3e-4f;
+            box.lo = sub(c, F3{pad, pad, pad});
+            box.hi = add(c, F3{pad, pad, pad});
+            break;
+        }
+        case AGX_GEOM_DISK: {                               // disk.cpp:33-47, 85-98
+            if (!(e.radius > 0)) { *err = "invalid disk radius"; return AGX_E_INVALID_ARG; }
+            d.a = vec4{e.radius, e.radius * e.radius, 0, ratio};
+            area = ratio * ratio * 3.14159265358979323846f * (e.radius * e.radius);
+            const float vs = std::min(0.1f * e.radius, 0.01f);
+            for (int k = 0; k < 8; ++k)
+                box.add_point(apply_point(e.local_to_world, F3{(k & 1) ? e.radius : -e.radius,
+                                                               (k & 2) ? e.radius : -e.radius, (k & 4) ? vs : -vs}));
+            break;
+        }
+        case AGX_GEOM_TRIANGLE_BVH: {
+            if (!e.tris || !e.tri_count) { *err = "entity " + std::to_string(i) + ": empty triangle mesh"; return AGX_E_INVALID_ARG; }
+            std::vector<vec4> nodes, prims, info;
+            double sa = 0;
+            const int depth = build_triangle_bvh(e.tris, e.tri_count, &nodes, &prims, &info, &sa);
+            S.max_blas_depth = std::max(S.max_blas_depth, depth);
+            const uint32_t node_base = uint32_t(S.blas_nodes.size() / 2);
+            const uint32_t prim_base = uint32_t(S.blas_prims.size() / 3);
+            // rebase indices to the global arrays
+            for (size_t k = 0; k < nodes.size(); k += 2) {
+                uint32_t s, r;
+                std::memcpy(&s, &nodes[k].w, 4);
+                std::memcpy(&r, &nodes[k + 1].w, 4);
+                if (s == INTERIOR) r += node_base;
+                else { s += prim_base; r += prim_base; }
+                nodes[k].w = bits_f(s);
+                nodes[k + 1].w = bits_f(r);
+            }
+            S.blas_nodes.insert(S.blas_nodes.end(), nodes.begin(), nodes.end());
+            S.blas_prims.insert(S.blas_prims.end(), prims.begin(), prims.end());
+            S.blas_info.insert(S.blas_info.end(), info.begin(), info.end());
+            d.node_base = node_base;
+            d.prim_base = prim_base;
+            d.prim_count = e.tri_count;
+            area = float(sa);
+            // world bound of TriangleBVH (triangle_bvh.cpp:570-583)
+            for (size_t k = 0; k < prims.size(); k += 3) {
+                const F3 a{prims[k].x, prims[k].y, prims[k].z};
+                box.add_point(a);
+                box.add_point(add(a, F3{prims[k + 1].x, prims[k + 1].y, prims[k + 1].z}));
+                box.add_point(add(a, F3{prims[k + 2].x, prims[k + 2].y, prims[k + 2].z}));
+            }
+            box.pad_flat();
+            const bool emissive = e.emit_radiance[0] != 0 || e.emit_radiance[1] != 0 || e.emit_radiance[2] != 0;
+            if (emissive) {
+                std::vector<float> areas(e.tri_count);
+                for (uint32_t k = 0; k < e.tri_count; ++k) {
+                    const vec4 &b = prims[3 * k + 1], &c = prims[3 * k + 2];
+                    areas[k] = tri_area(F3{b.x, b.y, b.z}, F3{c.x, c.y, c.z});
+                }
+                d.alias_base = uint32_t(S.alias_prob.size());
+                build_alias(areas, &S.alias_prob, &S.alias_idx);
+            }
+            S.total_tris += e.tri_count;
+            break;
+        }
+        default:
+            *err = "entity " + std::to_string(i) + ": unsupported geometry kind " + std::to_string(e.geometry);
+            return AGX_E_UNSUPPORTED;
+        }
+        d.emit = vec4{e.emit_radiance[0], e.emit_radiance[1], e.emit_radiance[2], area};
+        bounds[i] = box;
+        S.ents.push_back(d);
+    }
+
+    // light list: environment first, then emissive entities (scene_default.cpp:66-82)
+    if (desc.env.kind != AGX_ENV_NONE) {
+        if (desc.env.kind != AGX_ENV_NATIVE_SKY) { *err = "unsupported environment light"; return AGX_E_UNSUPPORTED; }
+        S.lights.push_back(-1);
+    }
+    for (uint32_t i = 0; i < desc.entity_count; ++i) {
+        const float *er = desc.entities[i].emit_radiance;
+        if (er[0] != 0 || er[1] != 0 || er[2] != 0) {
+            S.ents[i].light = int32_t(S.lights.size());
+            S.lights.push_back(int32_t(i));
+        }
+    }
+    if (S.lights.size() > size_t(MAX_LIGHTS)) {
+        *err = "gpu_pt supports at most " + std::to_string(MAX_LIGHTS) + " lights (scene has " +
+               std::to_string(S.lights.size()) + ")";
+        return AGX_E_UNSUPPORTED;
+    }
+    S.env_kind = desc.env.kind;
+    S.sky_top = vec4{desc.env.top[0], desc.env.top[1], desc.env.top[2], 0};
+    S.sky_bottom = vec4{desc.env.bottom[0], desc.env.bottom[1], desc.env.bottom[2], 0};
+
+    // EnvirLight::preprocess (core/core/light.cpp:6-14)
+    Box world;
+    for (const Box &b : bounds) world.add_box(b);
+    const float diag = len3(sub(world.hi, world.lo));
+    const F3 centre = mul(add(world.lo, world.hi), 0.5f);
+    S.world = vec4{centre.x, centre.y, centre.z, 1.1f * diag / 2};
+
+    S.aggregate = desc.aggregate;
+    if (desc.aggregate == AGX_AGGREGATE_BVH) {
+        std::vector<EntRec> recs(desc.entity_count);
+        for (uint32_t i = 0; i < desc.entity_count; ++i) recs[i] = {int32_t(i), bounds[i]};
+        TlasBuilder tb{&S.tlas_nodes, &S.tlas_prims, desc.max_leaf_size > 0 ? desc.max_leaf_size : 5};
+        if (recs.empty()) {
+            S.tlas_nodes.push_back(vec4{0, 0, 0, bits_f(TLAS_LEAF)});
+            S.tlas_nodes.push_back(vec4{1, 1, 1, bits_f(0)});
+        } else {
+            tb.build(recs.data(), recs.size(), 0);
+        }
+        S.max_tlas_depth = tb.max_depth;
+    } else if (desc.aggregate != AGX_AGGREGATE_NATIVE) {
+        *err = "unknown aggregate kind";
+        return AGX_E_INVALID_ARG;
+    }
+    if (S.max_blas_depth + 2 > BLAS_STACK || S.max_tlas_depth + 2 > TLAS_STACK) {
+        *err = "BVH depth exceeds the traversal stack (blas " + std::to_string(S.max_blas_depth) + ", tlas " +
+               std::to_string(S.max_tlas_depth) + ")";
+        return AGX_E_LIMIT;
+    }
+    return AGX_OK;
+}
+
+}  // namespace agx

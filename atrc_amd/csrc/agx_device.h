@@ -1,0 +1,88 @@
+// Device-resident scene and wavefront state of the gfx950 path tracer.
+//
+// HBM layout (all arrays 16-byte aligned, structure-of-arrays where a kernel
+// streams one field over many paths):
+//   entities   DevEntity[n]            one record per Atrc entity (288 B)
+//   materials  DevMaterial[m]          constant-texture BSDF parameters, pre-derived
+//   blas_nodes float4[2*N]             Atrc triangle-BVH nodes: (low.xyz, start) (high.xyz, end|right)
+//   blas_prims float4[3*P]             (a.xyz, original id) (b-a, 0) (c-a, 0)        48 B / triangle
+//   blas_info  float4[6*P]             interpolated normal / uv deltas and frame      96 B / triangle
+//   tlas_nodes float4[2*T]             entity BVH: (low.xyz, left|leaf-start) (high.xyz, right|leaf-end)
+//   tlas_prims int32[]                 entity indices referenced by TLAS leaves
+//   alias_prob/alias_idx               per-mesh triangle-area alias tables (emissive meshes)
+//   lights     int32[L]                Atrc light order: -1 = environment, else entity index
+#pragma once
+
+#include <stdint.h>
+
+namespace agx {
+
+// 16-byte POD shared by host code and kernels (loads as one dwordx4)
+struct alignas(16) vec4 {
+    float x, y, z, w;
+};
+
+constexpr uint32_t INTERIOR = 0xffffffffu;   // BLAS node: start == INTERIOR marks an interior node
+constexpr uint32_t TLAS_LEAF = 0x80000000u;  // TLAS node: left word has this bit for leaves
+constexpr int MAX_LIGHTS = 16;
+constexpr int BLAS_STACK = 96;               // >= max BLAS depth + 2 (checked at upload)
+constexpr int TLAS_STACK = 64;
+
+struct DevEntity {
+    int32_t kind;          // AGX_GEOM_*
+    int32_t material;
+    int32_t light;         // -1 or index into the light list
+    int32_t flags;         // bit 0: no_denoise
+    vec4 a;                // quad/triangle: a.xyz ; sphere/disk: (radius, radius^2, world_radius, ratio)
+    vec4 b_a, c_a, d_a;    // edges (quad: world space; triangle: local space)
+    vec4 x_abc, x_acd, z;  // geometric frame vectors
+    vec4 t_ab;             // (t_a.x, t_a.y, t_b_a.x, t_b_a.y)
+    vec4 t_cd;             // (t_c_a.x, t_c_a.y, t_d_a.x, t_d_a.y)
+    vec4 l2w[3];           // rows of local_to_world
+    vec4 w2l[3];           // rows of world_to_local
+    vec4 emit;             // (radiance.xyz, surface area)
+    uint32_t node_base, prim_base, prim_count, alias_base;  // mesh ranges in the global arrays
+    vec4 sample_prob;      // (quad: P(ABC), 0, 0, 0)
+};
+static_assert(sizeof(DevEntity) == 304, "DevEntity layout");
+
+struct DevMaterial {
+    int32_t kind;          // AGX_MAT_*
+    int32_t pad0, pad1, pad2;
+    vec4 albedo;           // BSDF::albedo()
+    vec4 coef;             // Lambert albedo / pi
+    vec4 color;            // metal colour
+    vec4 eta2;             // conductor (eta_in / eta_out)^2 ; mirror uses m_eta2
+    vec4 k2;               // conductor (k / eta_out)^2
+    vec4 misc;             // (ax, ay, ior, 0)
+    vec4 c_refl, c_refr;   // glass
+};
+
+struct DevScene {
+    const DevEntity *ents;
+    const DevMaterial *mats;
+    const vec4 *blas_nodes;
+    const vec4 *blas_prims;
+    const vec4 *blas_info;
+    const vec4 *tlas_nodes;
+    const int32_t *tlas_prims;
+    const float *alias_prob;
+    const int32_t *alias_idx;
+    int32_t n_ent;
+    int32_t aggregate;     // AGX_AGGREGATE_*
+    int32_t n_lights;
+    int32_t env_kind;
+    int32_t lights[MAX_LIGHTS];
+    vec4 sky_top, sky_bottom;
+    vec4 world;            // (centre.xyz, radius)
+};
+
+// per-launch counters (collect_stats)
+struct DevCounters {
+    unsigned long long node_visits;
+    unsigned long long tri_tests;
+    unsigned long long hits;
+    unsigned long long pad;
+};
+
+}  // namespace agx

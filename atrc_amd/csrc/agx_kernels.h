@@ -1,0 +1,77 @@
+// Wavefront buffers and kernel launch entry points of the gfx950 path tracer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "agx_device.h"
+
+namespace agx {
+
+// Per-path state (structure of arrays, N = paths in flight).
+struct PathBuffers {
+    float4 *coef;        // [N] path throughput
+    float4 *radiance;    // [N] accumulated radiance
+    float4 *gb_albedo;   // [N] (albedo.xyz, denoise)
+    float4 *gb_normal;   // [N]
+    unsigned long long *rng;  // [N] PCG32 state (the increment is derived from the path id)
+    int2 *depth;         // [N] (depth, s_depth)
+    uint32_t *flags;     // [N] PF_* bits
+    float4 *mis_a;       // [N] MIS-BSDF record: (coef.xyz, |cos|)
+    float4 *mis_b;       // [N]                  (f.xyz, pdf)
+    float4 *nee;         // [N*L] (coef * contribution.xyz, valid)
+    float4 *ray_o;       // [2N] (o.xyz, t_min); 2p = continuation, 2p+1 = MIS-BSDF ray
+    float4 *ray_d;       // [2N] (d.xyz, t_max)
+    float4 *hit;         // [2N] (t, u, v, entity bits), entity = -1 on miss
+    int32_t *hit_prim;   // [2N] primitive slot (mesh: global BVH slot; quad: 0/1)
+    float4 *sh_o;        // [N*L] shadow rays (o.xyz, t_max)
+    float4 *sh_d;        // [N*L] (d.xyz, t_min)
+    uint8_t *occluded;   // [N*L]
+    uint32_t *q_closest; // [2N] ray indices to trace (closest hit)
+    uint32_t *q_shadow;  // [N*L] shadow ray indices
+    uint32_t *q_in;      // [N] paths to shade this bounce
+    uint32_t *q_out;     // [N] paths to shade next bounce
+    uint32_t *counters;  // [4] next-bounce queue sizes: closest, shadow, paths
+    float *film_sum;     // [10*WH] rgb, albedo, normal, denoise
+    float *film_comp;    // [10*WH] Neumaier compensation
+    float *film_count;   // [WH]
+};
+
+enum : uint32_t {
+    PF_CONT = 1u,        // continuation ray 2p is queued
+    PF_MIS = 2u,         // MIS-BSDF ray 2p+1 is queued
+    PF_PENDING = 4u,     // NEE / MIS contributions await resolution
+    PF_KEEP = 8u,        // the camera sample lands inside the box filter
+    PF_MIS_DELTA = 16u,  // the MIS-BSDF sample was a delta lobe
+};
+
+struct RenderConst {
+    int32_t W, H;
+    int32_t min_depth, max_depth, specular_depth;
+    float cont_prob;
+    float eps;
+    float filter_r;      // box radius - eps
+    unsigned long long seed;
+    int64_t work_base;   // first work item of this pass; work = layer * W*H + pixel
+    int32_t n_paths;     // paths in this pass
+    int32_t sample_begin;
+    int32_t L;           // number of lights
+    int32_t pad;
+    float4 cam_pos, cam_x, cam_y, cam_z;  // camera frame
+    float film_w, film_h, lens_r, focal;
+};
+
+void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, hipStream_t st);
+void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+                          hipStream_t st);
+void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+                         hipStream_t st);
+void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t n, hipStream_t st);
+void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st);
+void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st);
+void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,
+                         hipStream_t st);
+// parity hook: rays in device memory, results to device memory
+void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
+                       int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st);
+
+}  // namespace agx

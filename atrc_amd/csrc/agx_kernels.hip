@@ -1,0 +1,1390 @@
+// gfx950 kernels of the wavefront path tracer that reproduces Atrc's `pt`.
+//
+// Pipeline per bounce (one HIP stream, queues compacted with wave-aggregated
+// atomics):
+//   raygen  -> trace_closest (continuation + MIS-BSDF rays) -> trace_shadow (NEE rays)
+//           -> shade (resolve NEE/MIS of the previous vertex, shade the new vertex,
+//              emit next rays, Russian roulette) -> ... -> film
+// Each path owns fixed slots in the ray/hit/NEE arrays, so queue order never
+// changes a result: every path's arithmetic and random numbers are those of
+// Atrc's trace_std for the same PCG32 stream.
+//
+// Reference semantics (paths relative to src/tracer/):
+//   trace_std                         src/render/path_tracing.cpp:15-225
+//   mis_sample_light / mis_sample_bsdf src/render/direct_illum.cpp:11-192
+//   closest / any hit                 src/core/aggregate/{native_aggregate,entity_bvh}.cpp,
+//                                     src/core/geometry/{triangle_bvh,quad,triangle,sphere,disk}.cpp
+// The file is compiled with -ffp-contract=off: the FP32 operation order is the
+// one written here, which is Atrc's.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/atrc_gpu.h"
+#include "agx_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace agx {
+
+namespace {
+
+constexpr float PI_F = 3.14159265358979323846f;
+constexpr float INF_F = __builtin_huge_valf();
+
+// ---------------------------------------------------------------- small vector math
+struct V {
+    float x, y, z;
+};
+__device__ __forceinline__ V mk(float x, float y, float z) { return V{x, y, z}; }
+__device__ __forceinline__ V mk(float4 a) { return V{a.x, a.y, a.z}; }
+__device__ __forceinline__ V mk(const vec4 &a) { return V{a.x, a.y, a.z}; }
+__device__ __forceinline__ float4 ld(const vec4 &a) { return make_float4(a.x, a.y, a.z, a.w); }
+__device__ __forceinline__ V operator+(V a, V b) { return V{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V operator-(V a, V b) { return V{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V operator*(V a, V b) { return V{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ V operator*(V a, float s) { return V{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V operator/(V a, float s) { return V{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ V operator-(V a) { return V{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V cross(V a, V b) {
+    return V{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float length(V a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ V normalize(V a) { return a / length(a); }
+__device__ __forceinline__ bool black(V a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ float cosv(V a, V b) { return dot(a, b) / (length(a) * length(b)); }
+__device__ __forceinline__ float4 f4(V a, float w) { return make_float4(a.x, a.y, a.z, w); }
+// std::min / std::max operand semantics (matter for NaN slab values)
+__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
+
+struct Frame {
+    V x, y, z;
+};
+// Duff et al. 2017 orthonormal basis around z.
+__device__ Frame frame_from_z(V zin) {
+    const V n = normalize(zin);
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    return Frame{mk(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x), mk(b, sign + n.y * n.y * a, -n.y), n};
+}
+__device__ __forceinline__ V to_global(const Frame &f, V v) { return f.x * v.x + f.y * v.y + f.z * v.z; }
+__device__ __forceinline__ V to_local(const Frame &f, V v) { return mk(dot(v, f.x), dot(v, f.y), dot(v, f.z)); }
+__device__ __forceinline__ bool pos_hemi(const Frame &f, V v) { return dot(v, f.z) > 0.0f; }
+// re-orient a frame to a new z, keeping x as close as possible (Gram-Schmidt)
+__device__ Frame rotate_to_new_z(const Frame &f, V new_z) {
+    const V nz = normalize(new_z);
+    const V xp = f.x - nz * dot(f.x, nz);
+    const float l = length(xp);
+    if (!(l > 1e-6f)) return frame_from_z(nz);
+    const V nx = xp / l;
+    return Frame{nx, cross(nz, nx), nz};
+}
+
+__device__ __forceinline__ V xf_point(const vec4 *m, V p) {
+    return mk(m[0].x * p.x + m[0].y * p.y + m[0].z * p.z + m[0].w, m[1].x * p.x + m[1].y * p.y + m[1].z * p.z + m[1].w,
+              m[2].x * p.x + m[2].y * p.y + m[2].z * p.z + m[2].w);
+}
+__device__ __forceinline__ V xf_vec(const vec4 *m, V v) {
+    return mk(m[0].x * v.x + m[0].y * v.y + m[0].z * v.z, m[1].x * v.x + m[1].y * v.y + m[1].z * v.z,
+              m[2].x * v.x + m[2].y * v.y + m[2].z * v.z);
+}
+__device__ __forceinline__ V xf_tvec(const vec4 *m, V v) {
+    return mk(m[0].x * v.x + m[1].x * v.y + m[2].x * v.z, m[0].y * v.x + m[1].y * v.y + m[2].y * v.z,
+              m[0].z * v.x + m[1].z * v.y + m[2].z * v.z);
+}
+// frame of a transformed geometry in world space: normal by the inverse transpose,
+// x by the linear part, re-orthonormalised
+__device__ Frame frame_to_world(const DevEntity &e, const Frame &c) {
+    const V nz = normalize(xf_tvec(e.w2l, c.z));
+    const V mx = xf_vec(e.l2w, c.x);
+    const V xp = mx - nz * dot(mx, nz);
+    const float l = length(xp);
+    if (!(l > 1e-6f)) return frame_from_z(nz);
+    const V nx = xp / l;
+    return Frame{nx, cross(nz, nx), nz};
+}
+
+// ---------------------------------------------------------------- RNG (PCG32 XSH-RR)
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+struct Rng {
+    unsigned long long state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        const unsigned long long old = state;
+        state = old * 6364136223846793005ull + inc;
+        const uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = uint32_t(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    __device__ __forceinline__ float uniform() { return float(next() >> 8) * 0x1p-24f; }
+};
+__device__ __forceinline__ unsigned long long path_seq(unsigned long long pixel, unsigned long long sample) {
+    return (pixel << 24) ^ sample;
+}
+__device__ Rng rng_seed(unsigned long long seed, unsigned long long seq) {
+    Rng r;
+    r.state = 0;
+    r.inc = (seq << 1u) | 1u;
+    r.next();
+    r.state += splitmix64(seed ^ splitmix64(seq));
+    r.next();
+    return r;
+}
+
+// ---------------------------------------------------------------- sampling distributions
+__device__ void unit_disk(float u, float v, float *ox, float *oy) {  // Shirley-Chiu concentric map
+    const float sx = 2 * u - 1, sy = 2 * v - 1;
+    if (sx == 0.0f && sy == 0.0f) { *ox = 0; *oy = 0; return; }
+    float r, th;
+    if (fabsf(sx) > fabsf(sy)) { r = sx; th = (PI_F / 4) * (sy / sx); }
+    else { r = sy; th = PI_F / 2 - (PI_F / 4) * (sx / sy); }
+    *ox = r * cosf(th);
+    *oy = r * sinf(th);
+}
+__device__ V cos_hemisphere(float u, float v, float *pdf) {  // Malley's method
+    float x, y;
+    unit_disk(u, v, &x, &y);
+    const float z = sqrtf(smax(0.0f, 1 - x * x - y * y));
+    *pdf = z / PI_F;
+    return mk(x, y, z);
+}
+__device__ __forceinline__ float cos_hemisphere_pdf(float c) { return c > 0 ? c / PI_F : 0.0f; }
+__device__ V uniform_sphere(float u, float v) {
+    const float z = 1 - 2 * u;
+    const float r = sqrtf(smax(0.0f, 1 - z * z));
+    const float phi = 2 * PI_F * v;
+    return mk(r * cosf(phi), r * sinf(phi), z);
+}
+constexpr float SPHERE_PDF = 1.0f / (4 * PI_F);
+__device__ V uniform_cone(float cmax, float u, float v) {
+    const float c = (1 - u) + u * cmax;
+    const float s = sqrtf(smax(0.0f, 1 - c * c));
+    const float phi = 2 * PI_F * v;
+    return mk(cosf(phi) * s, sinf(phi) * s, c);
+}
+__device__ __forceinline__ float cone_pdf(float cmax) { return 1 / (2 * PI_F * (1 - cmax)); }
+
+// ---------------------------------------------------------------- intersection primitives
+// closest_intersection_with_triangle (utility/triangle_aux.h:38-69)
+__device__ __forceinline__ bool mt_closest(V o, V d, float tmin, float tmax, V A, V BA, V CA, float *t, float *u,
+                                           float *v) {
+    const V s1 = cross(d, CA);
+    const float div = dot(s1, BA);
+    if (!(div != 0.0f)) return false;
+    const float inv = 1 / div;
+    const V oa = o - A;
+    const float alpha = dot(oa, s1) * inv;
+    if (alpha < 0) return false;
+    const V s2 = cross(oa, BA);
+    const float beta = dot(d, s2) * inv;
+    if (beta < 0 || alpha + beta > 1) return false;
+    const float tt = dot(CA, s2) * inv;
+    if (!(tmin <= tt && tt <= tmax)) return false;
+    *t = tt; *u = alpha; *v = beta;
+    return true;
+}
+// has_intersection_with_triangle (triangle_aux.h:9-30)
+__device__ __forceinline__ bool mt_any(V o, V d, float tmin, float tmax, V A, V BA, V CA) {
+    const V s1 = cross(d, CA);
+    const float div = dot(s1, BA);
+    if (!(div != 0.0f)) return false;
+    const float inv = 1 / div;
+    const V oa = o - A;
+    const float alpha = dot(oa, s1) * inv;
+    if (alpha < 0 || alpha > 1) return false;
+    const V s2 = cross(oa, BA);
+    const float beta = dot(d, s2) * inv;
+    if (beta < 0 || alpha + beta > 1) return false;
+    const float tt = dot(CA, s2) * inv;
+    return tmin <= tt && tt <= tmax;
+}
+// Node::has_intersection (triangle_bvh.cpp:75-97)
+__device__ __forceinline__ bool slab(float4 lo, float4 hi, V o, V inv, float tmin, float tmax, float *tenter) {
+    const float nx = inv.x * (lo.x - o.x), ny = inv.y * (lo.y - o.y), nz = inv.z * (lo.z - o.z);
+    const float fx = inv.x * (hi.x - o.x), fy = inv.y * (hi.y - o.y), fz = inv.z * (hi.z - o.z);
+    tmin = smax(tmin, smin(nx, fx));
+    tmin = smax(tmin, smin(ny, fy));
+    tmin = smax(tmin, smin(nz, fz));
+    tmax = smin(tmax, smax(nx, fx));
+    tmax = smin(tmax, smax(ny, fy));
+    tmax = smin(tmax, smax(nz, fz));
+    *tenter = tmin;
+    return tmin <= tmax;
+}
+
+struct Counts {
+    uint32_t visits = 0, tests = 0;
+};
+
+struct HitRec {
+    float t, u, v;
+    int32_t ent, prim;
+};
+
+// UntransformedTriangleBVH::closest_intersection (triangle_bvh.cpp:403-488)
+template <bool STATS>
+__device__ bool blas_closest(const DevScene &s, const DevEntity &e, V o, V d, float tmin, float tmax, HitRec *h,
+                             Counts &c) {
+    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const uint32_t root = e.node_base;
+    float te;
+    if (STATS) c.visits++;
+    if (!slab(ld(s.blas_nodes[2 * root]), ld(s.blas_nodes[2 * root + 1]), o, inv, tmin, tmax, &te)) return false;
+    uint32_t stack[BLAS_STACK];
+    int top = 0;
+    stack[top++] = root;
+    float best = INF_F, bu = 0, bv = 0;
+    uint32_t bslot = 0;
+    while (top) {
+        const uint32_t ni = stack[--top];
+        const float4 lo = ld(s.blas_nodes[2 * ni]), hi = ld(s.blas_nodes[2 * ni + 1]);
+        const uint32_t start = fbits(lo.w), end = fbits(hi.w);
+        if (start != INTERIOR) {
+            for (uint32_t i = start; i < end; ++i) {
+                if (STATS) c.tests++;
+                const vec4 p0 = s.blas_prims[3 * i], p1 = s.blas_prims[3 * i + 1], p2 = s.blas_prims[3 * i + 2];
+                float tt, uu, vv;
+                if (mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv)) {
+                    best = tt; bu = uu; bv = vv; bslot = i;
+                    tmax = tt;
+                }
+            }
+        } else {
+            const uint32_t l = ni + 1, r = end;
+            float tl, tr;
+            if (STATS) c.visits += 2;
+            const bool al = slab(ld(s.blas_nodes[2 * l]), ld(s.blas_nodes[2 * l + 1]), o, inv, tmin, tmax, &tl);
+            const bool ar = slab(ld(s.blas_nodes[2 * r]), ld(s.blas_nodes[2 * r + 1]), o, inv, tmin, tmax, &tr);
+            if (al && ar) {
+                if (tl < tr) { stack[top++] = r; stack[top++] = l; }
+                else { stack[top++] = l; stack[top++] = r; }
+            } else if (al) {
+                stack[top++] = l;
+            } else if (ar) {
+                stack[top++] = r;
+            }
+        }
+    }
+    if (isinf(best)) return false;
+    h->t = best; h->u = bu; h->v = bv; h->prim = int32_t(bslot);
+    return true;
+}
+
+// UntransformedTriangleBVH::has_intersection (triangle_bvh.cpp:363-401)
+template <bool STATS>
+__device__ bool blas_any(const DevScene &s, const DevEntity &e, V o, V d, float tmin, float tmax, Counts &c) {
+    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const uint32_t root = e.node_base;
+    float te;
+    if (STATS) c.visits++;
+    if (!slab(ld(s.blas_nodes[2 * root]), ld(s.blas_nodes[2 * root + 1]), o, inv, tmin, tmax, &te)) return false;
+    uint32_t stack[BLAS_STACK];
+    int top = 0;
+    stack[top++] = root;
+    while (top) {
+        const uint32_t ni = stack[--top];
+        const float4 lo = ld(s.blas_nodes[2 * ni]), hi = ld(s.blas_nodes[2 * ni + 1]);
+        const uint32_t start = fbits(lo.w), end = fbits(hi.w);
+        if (start != INTERIOR) {
+            for (uint32_t i = start; i < end; ++i) {
+                if (STATS) c.tests++;
+                const vec4 p0 = s.blas_prims[3 * i], p1 = s.blas_prims[3 * i + 1], p2 = s.blas_prims[3 * i + 2];
+                if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) return true;
+            }
+        } else {
+            const uint32_t l = ni + 1, r = end;
+            if (STATS) c.visits += 2;
+            if (slab(ld(s.blas_nodes[2 * l]), ld(s.blas_nodes[2 * l + 1]), o, inv, tmin, tmax, &te)) stack[top++] = l;
+            if (slab(ld(s.blas_nodes[2 * r]), ld(s.blas_nodes[2 * r + 1]), o, inv, tmin, tmax, &te)) stack[top++] = r;
+        }
+    }
+    return false;
+}
+
+// sphere_aux.h:10-16
+__device__ __forceinline__ void sphere_abc(V o, V d, float r, float *A, float *B, float *C) {
+    *A = dot(d, d);
+    *B = 2 * dot(d, o);
+    *C = dot(o, o) - r * r;
+}
+
+// Closest hit of one entity; on success *tmax is lowered to the hit distance.
+template <bool STATS>
+__device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tmin, float *tmax, HitRec *h,
+                               Counts &c) {
+    const DevEntity &e = s.ents[ei];
+    switch (e.kind) {
+    case AGX_GEOM_QUAD: {  // quad.cpp:30-58
+        float t, u, v;
+        if (STATS) c.tests++;
+        if (mt_closest(o, d, tmin, *tmax, mk(e.a), mk(e.b_a), mk(e.c_a), &t, &u, &v)) {
+            h->t = t; h->u = u; h->v = v; h->prim = 0; h->ent = ei; *tmax = t;
+            return true;
+        }
+        if (STATS) c.tests++;
+        if (mt_closest(o, d, tmin, *tmax, mk(e.a), mk(e.c_a), mk(e.d_a), &t, &u, &v)) {
+            h->t = t; h->u = u; h->v = v; h->prim = 1; h->ent = ei; *tmax = t;
+            return true;
+        }
+        return false;
+    }
+    case AGX_GEOM_TRIANGLE: {  // triangle.cpp:31-51
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        float t, u, v;
+        if (STATS) c.tests++;
+        if (!mt_closest(lo, ld, tmin, *tmax, mk(e.a), mk(e.b_a), mk(e.c_a), &t, &u, &v)) return false;
+        h->t = t; h->u = u; h->v = v; h->prim = 0; h->ent = ei; *tmax = t;
+        return true;
+    }
+    case AGX_GEOM_SPHERE: {  // sphere_aux.h:66-87
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        float A, B, C;
+        sphere_abc(lo, ld, e.a.x, &A, &B, &C);
+        float delta = B * B - 4 * A * C;
+        if (delta < 0) return false;
+        delta = sqrtf(delta);
+        const float inv2A = 0.5f / A;
+        const float t0 = (-B - delta) * inv2A, t1 = (-B + delta) * inv2A;
+        if (*tmax < t0 || t1 < tmin) return false;
+        const bool not_t0 = t0 < tmin;
+        if (not_t0 && t1 > *tmax) return false;
+        const float t = not_t0 ? t1 : t0;
+        h->t = t; h->u = 0; h->v = 0; h->prim = 0; h->ent = ei; *tmax = t;
+        return true;
+    }
+    case AGX_GEOM_DISK: {  // disk.cpp:63-83
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        const float t = -lo.z / ld.z;
+        if (isinf(t) || !(tmin <= t && t <= *tmax)) return false;
+        const V p = lo + ld * t;
+        const float rad = sqrtf(p.x * p.x + p.y * p.y);
+        if (rad > e.a.x) return false;
+        h->t = t; h->u = 0; h->v = 0; h->prim = 0; h->ent = ei; *tmax = t;
+        return true;
+    }
+    default: {
+        if (!blas_closest<STATS>(s, e, o, d, tmin, *tmax, h, c)) return false;
+        h->ent = ei;
+        *tmax = h->t;
+        return true;
+    }
+    }
+}
+
+template <bool STATS>
+__device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, float tmax, Counts &c) {
+    const DevEntity &e = s.ents[ei];
+    switch (e.kind) {
+    case AGX_GEOM_QUAD:  // quad.cpp:24-28
+        if (STATS) c.tests++;
+        if (mt_any(o, d, tmin, tmax, mk(e.a), mk(e.b_a), mk(e.c_a))) return true;
+        if (STATS) c.tests++;
+        return mt_any(o, d, tmin, tmax, mk(e.a), mk(e.c_a), mk(e.d_a));
+    case AGX_GEOM_TRIANGLE: {
+        if (STATS) c.tests++;
+        return mt_any(xf_point(e.w2l, o), xf_vec(e.w2l, d), tmin, tmax, mk(e.a), mk(e.b_a), mk(e.c_a));
+    }
+    case AGX_GEOM_SPHERE: {  // sphere_aux.h:50-64
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        float A, B, C;
+        sphere_abc(lo, ld, e.a.x, &A, &B, &C);
+        float delta = B * B - 4 * A * C;
+        if (delta < 0) return false;
+        delta = sqrtf(delta);
+        const float inv2A = 0.5f / A;
+        const float t0 = (-B - delta) * inv2A, t1 = (-B + delta) * inv2A;
+        return (tmin <= t0 && t0 <= tmax) || (tmin <= t1 && t1 <= tmax);
+    }
+    case AGX_GEOM_DISK: {  // disk.cpp:49-61
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        const float t = -lo.z / ld.z;
+        if (isinf(t) || !(tmin <= t && t <= tmax)) return false;
+        const V p = lo + ld * t;
+        return p.x * p.x + p.y * p.y <= e.a.y;
+    }
+    default:
+        return blas_any<STATS>(s, e, o, d, tmin, tmax, c);
+    }
+}
+
+// AABB::intersect (common.h:384-395) with std::min/max operand order
+__device__ __forceinline__ bool tlas_box(float4 lo, float4 hi, V o, V inv, float tmin, float tmax) {
+    float te;
+    return slab(lo, hi, o, inv, tmin, tmax, &te);
+}
+
+// Scene closest hit: NativeAggregate (native_aggregate.cpp:31-45) or EntityBVH
+// (entity_bvh.cpp:141-172, left subtree before right, node bound tested on entry).
+template <bool STATS>
+__device__ bool scene_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c) {
+    bool hit = false;
+    if (s.aggregate == AGX_AGGREGATE_BVH) {
+        const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+        uint32_t stack[TLAS_STACK];
+        int top = 0;
+        stack[top++] = 0;
+        while (top) {
+            const uint32_t ni = stack[--top];
+            const float4 lo = ld(s.tlas_nodes[2 * ni]), hi = ld(s.tlas_nodes[2 * ni + 1]);
+            if (STATS) c.visits++;
+            if (!tlas_box(lo, hi, o, inv, tmin, tmax)) continue;
+            const uint32_t a = fbits(lo.w), b = fbits(hi.w);
+            if (a & TLAS_LEAF) {
+                for (uint32_t i = a & ~TLAS_LEAF; i < b; ++i)
+                    hit = entity_closest<STATS>(s, s.tlas_prims[i], o, d, tmin, &tmax, h, c) || hit;
+            } else {
+                stack[top++] = b;
+                stack[top++] = a;
+            }
+        }
+    } else {
+        for (int32_t i = 0; i < s.n_ent; ++i) hit = entity_closest<STATS>(s, i, o, d, tmin, &tmax, h, c) || hit;
+    }
+    return hit;
+}
+
+template <bool STATS>
+__device__ bool scene_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c) {
+    if (s.aggregate == AGX_AGGREGATE_BVH) {
+        const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+        uint32_t stack[TLAS_STACK];
+        int top = 0;
+        stack[top++] = 0;
+        while (top) {
+            const uint32_t ni = stack[--top];
+            const float4 lo = ld(s.tlas_nodes[2 * ni]), hi = ld(s.tlas_nodes[2 * ni + 1]);
+            if (STATS) c.visits++;
+            if (!tlas_box(lo, hi, o, inv, tmin, tmax)) continue;
+            const uint32_t a = fbits(lo.w), b = fbits(hi.w);
+            if (a & TLAS_LEAF) {
+                for (uint32_t i = a & ~TLAS_LEAF; i < b; ++i)
+                    if (entity_any<STATS>(s, s.tlas_prims[i], o, d, tmin, tmax, c)) return true;
+            } else {
+                stack[top++] = b;
+                stack[top++] = a;
+            }
+        }
+        return false;
+    }
+    for (int32_t i = 0; i < s.n_ent; ++i)
+        if (entity_any<STATS>(s, i, o, d, tmin, tmax, c)) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------- surface reconstruction
+struct Surf {
+    V pos;
+    float u, v;  // texture coordinates
+    Frame geo, user;
+    V wr;
+};
+
+// sphere_aux.h:18-48 (local frame and uv of a point on the sphere)
+__device__ void sphere_local(V p, float radius, float eps, Surf *sp) {
+    float phi = (p.x == 0.0f && p.y == 0.0f) ? 0.0f : atan2f(p.y, p.x);
+    if (phi < 0) phi += 2 * PI_F;
+    const float st = smin(1.0f, smax(-1.0f, p.z / radius));
+    const float th = asinf(st);
+    sp->u = 0.5f / PI_F * phi;
+    sp->v = th / PI_F + 0.5f;
+    V ex, ey, ez = normalize(p);
+    if (fabsf(ez.z - 1) < eps) { ex = mk(1, 0, 0); ey = mk(0, 1, 0); ez = mk(0, 0, 1); }
+    else if (fabsf(ez.z + 1) < eps) { ex = mk(0, 1, 0); ey = mk(1, 0, 0); ez = mk(0, 0, -1); }
+    else { ex = cross(mk(0, 0, 1), ez); ey = cross(ez, ex); }
+    sp->geo = Frame{ex, ey, ez};
+    sp->user = sp->geo;
+}
+// disk.cpp:11-28
+__device__ void disk_local(V p, float rad, float radius, Surf *sp) {
+    float phi = (p.y == 0.0f && p.x == 0.0f) ? 0.0f : atan2f(p.y, p.x);
+    if (phi < 0) phi += 2 * PI_F;
+    sp->pos = p;
+    sp->u = phi / (2 * PI_F);
+    sp->v = rad / radius;
+    const V cx = normalize(mk(p.x, p.y, 0));
+    const V cz = mk(0, 0, 1);
+    sp->geo = Frame{cx, cross(cz, cx), cz};
+    sp->user = sp->geo;
+}
+__device__ void mesh_frame(const DevScene &s, uint32_t slot, float bu, float bv, Surf *sp) {
+    const vec4 *in = s.blas_info + size_t(slot) * 6;
+    const vec4 i0 = in[0], i1 = in[1], i2 = in[2], i3 = in[3], i4 = in[4], i5 = in[5];
+    const V x = mk(i3), z = mk(i4);
+    sp->geo = Frame{x, cross(z, x), z};
+    // uv = t_a + t_b_a * u + t_c_a * v
+    sp->u = (i0.w + i2.w * bu) + i4.w * bv;
+    sp->v = (i1.w + i3.w * bu) + i5.x * bv;
+    const V user_z = mk(i0) + mk(i1) * bu + mk(i2) * bv;
+    sp->user = rotate_to_new_z(sp->geo, user_z);
+}
+
+__device__ Surf surface_at(const DevScene &s, const HitRec &h, V o, V d, float eps) {
+    const DevEntity &e = s.ents[h.ent];
+    Surf sp;
+    switch (e.kind) {
+    case AGX_GEOM_QUAD: {
+        sp.pos = o + d * h.t;
+        const V z = mk(e.z);
+        if (h.prim == 0) {
+            const V x = mk(e.x_abc);
+            sp.geo = Frame{x, cross(z, x), z};
+            sp.u = (e.t_ab.x + e.t_ab.z * h.u) + e.t_cd.x * h.v;
+            sp.v = (e.t_ab.y + e.t_ab.w * h.u) + e.t_cd.y * h.v;
+        } else {
+            const V x = mk(e.x_acd);
+            sp.geo = Frame{x, cross(z, x), z};
+            sp.u = (e.t_ab.x + e.t_cd.x * h.u) + e.t_cd.z * h.v;
+            sp.v = (e.t_ab.y + e.t_cd.y * h.u) + e.t_cd.w * h.v;
+        }
+        sp.user = sp.geo;
+        sp.wr = -d;
+        return sp;
+    }
+    case AGX_GEOM_TRIANGLE: {
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        const V z = mk(e.z), x = mk(e.x_abc);
+        Frame f{x, cross(z, x), z};
+        sp.pos = xf_point(e.l2w, lo + ld * h.t);
+        sp.u = (e.t_ab.x + e.t_ab.z * h.u) + e.t_cd.x * h.v;
+        sp.v = (e.t_ab.y + e.t_ab.w * h.u) + e.t_cd.y * h.v;
+        sp.geo = frame_to_world(e, f);
+        sp.user = frame_to_world(e, f);
+        sp.wr = xf_vec(e.l2w, -ld);
+        return sp;
+    }
+    case AGX_GEOM_SPHERE: {
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        const V p = lo + ld * h.t;
+        sphere_local(p, e.a.x, eps, &sp);
+        sp.pos = xf_point(e.l2w, p);
+        const Frame g = sp.geo;
+        sp.geo = frame_to_world(e, g);
+        sp.user = frame_to_world(e, g);
+        sp.wr = xf_vec(e.l2w, -ld);
+        return sp;
+    }
+    case AGX_GEOM_DISK: {
+        const V lo = xf_point(e.w2l, o), ld = xf_vec(e.w2l, d);
+        const V p = lo + ld * h.t;
+        disk_local(p, sqrtf(p.x * p.x + p.y * p.y), e.a.x, &sp);
+        sp.pos = xf_point(e.l2w, p);
+        const Frame g = sp.geo;
+        sp.geo = frame_to_world(e, g);
+        sp.user = frame_to_world(e, g);
+        sp.wr = xf_vec(e.l2w, -ld);
+        return sp;
+    }
+    default: {
+        sp.pos = o + d * h.t;
+        mesh_frame(s, uint32_t(h.prim), h.u, h.v, &sp);
+        sp.wr = -d;
+        return sp;
+    }
+    }
+}
+
+__device__ __forceinline__ V eps_offset(const Surf &sp, V dir, float eps) {  // intersection.h:17-22
+    if (dot(dir, sp.geo.z) > 0) return sp.pos + sp.geo.z * eps;
+    return sp.pos - sp.geo.z * eps;
+}
+
+// ---------------------------------------------------------------- lights
+__device__ __forceinline__ V sky(const DevScene &s, V d) {  // native_sky.cpp:13-20
+    const float ct = smin(1.0f, smax(-1.0f, normalize(d).z));
+    const float t = (ct + 1) / 2;
+    return mk(s.sky_top) * t + mk(s.sky_bottom) * (1 - t);
+}
+// plane through centre + R d with normal d, met by ref + t d
+__device__ __forceinline__ V env_emit_pos(const DevScene &s, V ref, V dir) {
+    const V d = normalize(dir);
+    const float t = s.world.w + dot(mk(s.world) - ref, d);
+    return ref + d * t;
+}
+
+// Geometry::pdf(ref, pos) in area measure
+__device__ float geom_pdf(const DevEntity &e, V ref) {
+    if (e.kind == AGX_GEOM_SPHERE) {  // sphere.cpp:131-143
+        const V lr = xf_point(e.w2l, ref);
+        const float d = length(lr);
+        if (d <= e.a.x) return 1 / e.emit.w;
+        const float ct = smin(e.a.x / d, 1.0f);
+        return cone_pdf(ct) / (e.a.z * e.a.z);
+    }
+    return 1 / e.emit.w;
+}
+
+// GeometryToDiffuseLight::pdf (geometry_to_diffuse_light.cpp:75-90)
+__device__ float area_light_pdf(const DevEntity &e, V ref, V pos, V nor) {
+    if (dot(nor, ref - pos) <= 0) return 0;
+    const float ap = geom_pdf(e, ref);
+    const V s2r = ref - pos;
+    const float dist2 = dot(s2r, s2r);
+    const float ac = fabsf(cosv(nor, s2r));
+    return ap * (dist2 / ac);
+}
+
+struct LightSmp {
+    V pos, nor;
+    float pdf;  // solid angle; 0 = no sample
+};
+
+// Geometry::sample(ref) per kind + GeometryToDiffuseLight::sample (:14-31)
+__device__ LightSmp area_light_sample(const DevScene &s, const DevEntity &e, V ref, float su, float sv, float sw,
+                                      float eps) {
+    LightSmp ls;
+    ls.pdf = 0;
+    V pos, z;
+    float pdf_area;
+    switch (e.kind) {
+    case AGX_GEOM_QUAD: {  // quad.cpp:80-105
+        const float sq = sqrtf(su);
+        const float b1 = 1 - sq, b2 = sv * sq;
+        if (sw < e.sample_prob.x) pos = mk(e.a) + mk(e.b_a) * b1 + mk(e.c_a) * b2;
+        else pos = mk(e.a) + mk(e.c_a) * b1 + mk(e.d_a) * b2;
+        z = mk(e.z);
+        pdf_area = 1 / e.emit.w;
+        break;
+    }
+    case AGX_GEOM_TRIANGLE: {  // triangle.cpp:72-87
+        const float sq = sqrtf(su);
+        const float b1 = 1 - sq, b2 = sv * sq;
+        const V x = mk(e.x_abc), zl = mk(e.z);
+        const Frame f = frame_to_world(e, Frame{x, cross(zl, x), zl});
+        pos = xf_point(e.l2w, mk(e.a) + mk(e.b_a) * b1 + mk(e.c_a) * b2);
+        z = f.z;
+        pdf_area = 1 / e.emit.w;
+        break;
+    }
+    case AGX_GEOM_SPHERE: {  // sphere.cpp:93-123
+        const float radius = e.a.x, wr = e.a.z;
+        const V lref = xf_point(e.w2l, ref);
+        const float d = length(lref);
+        V p;
+        if (d <= radius) {
+            p = uniform_sphere(su, sv) * radius;
+            pdf_area = SPHERE_PDF / (wr * wr);
+        } else {
+            const float ct = smin(radius / d, 1.0f);
+            const V dir = uniform_cone(ct, su, sv);
+            p = normalize(to_global(frame_from_z(lref), dir)) * radius;
+            pdf_area = cone_pdf(ct) / (wr * wr);
+        }
+        Surf sp;
+        sphere_local(p, radius, eps, &sp);
+        pos = xf_point(e.l2w, p);
+        z = frame_to_world(e, sp.geo).z;
+        break;
+    }
+    case AGX_GEOM_DISK: {  // disk.cpp:100-110
+        float dx, dy;
+        unit_disk(su, sv, &dx, &dy);
+        const float px = dx * e.a.x, py = dy * e.a.x;
+        Surf sp;
+        disk_local(mk(px, py, 0), sqrtf(px * px + py * py), e.a.x, &sp);
+        pos = xf_point(e.l2w, mk(px, py, 0));
+        z = frame_to_world(e, sp.geo).z;
+        pdf_area = 1 / e.emit.w;
+        break;
+    }
+    default: {  // UntransformedTriangleBVH::sample (triangle_bvh.cpp:495-518), alias table over areas
+        const int32_t n = int32_t(e.prim_count);
+        const float nu = su * float(n);
+        int32_t idx = int32_t(nu);
+        if (idx >= n) idx = n - 1;
+        const float frac = nu - float(idx);
+        const int32_t pick = frac < s.alias_prob[e.alias_base + idx] ? idx : s.alias_idx[e.alias_base + idx];
+        const uint32_t slot = e.prim_base + uint32_t(pick);
+        const float sq = sqrtf(sv);
+        const float b1 = 1 - sq, b2 = sw * sq;
+        pos = mk(s.blas_prims[3 * slot]) + mk(s.blas_prims[3 * slot + 1]) * b1 + mk(s.blas_prims[3 * slot + 2]) * b2;
+        z = mk(s.blas_info[size_t(slot) * 6 + 4]);
+        pdf_area = 1 / e.emit.w;
+        break;
+    }
+    }
+    if (dot(z, ref - pos) <= 0) return ls;
+    const V s2r = ref - pos;
+    const float dist2 = dot(s2r, s2r);
+    ls.pdf = pdf_area * dist2 / fabsf(cosv(z, s2r));
+    ls.pos = pos;
+    ls.nor = z;
+    return ls;
+}
+
+// ---------------------------------------------------------------- BSDFs
+struct Bsdf {
+    int32_t kind;
+    Frame geo, shd;
+    V albedo, coef, color, eta2, k2, c_refl, c_refr;
+    float ax, ay, ior;
+};
+
+struct BSmp {
+    V dir, f;
+    float pdf;
+    bool delta;
+};
+
+__device__ Bsdf make_bsdf(const DevMaterial &m, const Surf &sp) {
+    Bsdf b;
+    b.kind = m.kind;
+    b.geo = sp.geo;
+    b.shd = sp.user;
+    b.albedo = mk(m.albedo);
+    b.coef = mk(m.coef);
+    b.color = mk(m.color);
+    b.eta2 = mk(m.eta2);
+    b.k2 = mk(m.k2);
+    b.c_refl = mk(m.c_refl);
+    b.c_refr = mk(m.c_refr);
+    b.ax = m.misc.x;
+    b.ay = m.misc.y;
+    b.ior = m.misc.z;
+    return b;
+}
+
+// ConductorPoint::eval (fresnel_point.h:44-64), one channel
+__device__ __forceinline__ float conductor1(float ci, float e2, float ek2) {
+    const float cos2 = ci * ci;
+    const float sin2 = smax(0.0f, 1 - cos2);
+    const float t0 = e2 - ek2 - sin2;
+    const float a2b2 = sqrtf(t0 * t0 + 4.0f * e2 * ek2);
+    const float t1 = a2b2 + cos2;
+    const float a = sqrtf(0.5f * (a2b2 + t0));
+    const float t2 = 2 * ci * a;
+    const float rs = (t1 - t2) / (t1 + t2);
+    const float t3 = cos2 * a2b2 + sin2 * sin2;
+    const float t4 = t2 * sin2;
+    const float rp = rs * (t3 - t4) / (t3 + t4);
+    return 0.5f * (rp + rs);
+}
+__device__ __forceinline__ V conductor(float ci, V e2, V ek2) {
+    if (ci <= 0) return mk(0, 0, 0);
+    return mk(conductor1(ci, e2.x, ek2.x), conductor1(ci, e2.y, ek2.y), conductor1(ci, e2.z, ek2.z));
+}
+// refl_aux::dielectric_fresnel (utility/reflection.h:48-72)
+__device__ float dielectric(float eta_i, float eta_o, float ci) {
+    if (ci < 0) { const float t = eta_i; eta_i = eta_o; eta_o = t; ci = -ci; }
+    const float si = sqrtf(smax(0.0f, 1 - ci * ci));
+    const float st = eta_o / eta_i * si;
+    if (st >= 1) return 1;
+    const float ct = sqrtf(smax(0.0f, 1 - st * st));
+    const float para = (eta_i * ci - eta_o * ct) / (eta_i * ci + eta_o * ct);
+    const float perp = (eta_o * ci - eta_i * ct) / (eta_o * ci + eta_i * ct);
+    return 0.5f * (para * para + perp * perp);
+}
+__device__ __forceinline__ float normal_corr(V g, V sh, V wi, float eps) {  // common.h:239-244
+    const float dem = fabsf(cosv(g, wi));
+    return dem < eps ? 1.0f : fabsf(cosv(sh, wi) / dem);
+}
+__device__ __forceinline__ float phi_of(V w) {  // common.h:220-226
+    if (w.y == 0.0f && w.x == 0.0f) return 0;
+    const float r = atan2f(w.y, w.x);
+    return r < 0 ? r + 2 * PI_F : r;
+}
+__device__ __forceinline__ float tan_theta(V w) {
+    const float t = 1 - w.z * w.z;
+    if (t <= 0) return 0;
+    return sqrtf(t) / w.z;
+}
+__device__ __forceinline__ float smith(float cp, float sp, float tt, float ax, float ay) {  // microfacet.cpp:54-61
+    const float a = ax * cp, b = ay * sp;
+    const float t = a * a + b * b;
+    const float sv = 1 + t * (tt * tt);
+    const float lambda = -0.5f + 0.5f * sqrtf(sv);
+    return 1 / (1 + lambda);
+}
+// GGX reflection eval/pdf (microfacet_refl_comp.cpp:8-65)
+__device__ void ggx_eval_pdf(const Bsdf &b, V lwi, V lwo, V *f, float *pdf) {
+    if (lwi.z <= 0 || lwo.z <= 0) {
+        if (f) *f = mk(0, 0, 0);
+        if (pdf) *pdf = 0;
+        return;
+    }
+    const float cti = lwi.z, cto = lwo.z;
+    const V lwh = normalize(lwi + lwo);
+    const float ctd = dot(lwi, lwh);
+    const float ph = phi_of(lwh);
+    const float sph = sinf(ph), cph = cosf(ph);
+    const float cth = lwh.z;
+    const float sth = sqrtf(smax(0.0f, 1 - cth * cth));
+    const float qa = cph / b.ax, qb = sph / b.ay;
+    const float A = qa * qa + qb * qb;
+    const float RD = sth * sth * A + cth * cth;
+    const float D = 1 / (PI_F * b.ax * b.ay * (RD * RD));
+    const float pi_ = phi_of(lwi), po = phi_of(lwo);
+    const float Gi = smith(cosf(pi_), sinf(pi_), tan_theta(lwi), b.ax, b.ay);
+    const float Go = smith(cosf(po), sinf(po), tan_theta(lwo), b.ax, b.ay);
+    if (f) {
+        const V F = conductor(ctd, b.eta2, b.k2) * b.color;
+        const float G = Gi * Go;
+        *f = F * D * G / fabsf(4 * cti * cto);
+    }
+    if (pdf) *pdf = Go * D / (4 * lwo.z);
+}
+// microfacet::sample_anisotropic_gtr2_vnor (microfacet.cpp:104-128)
+__device__ V ggx_vndf(const Bsdf &b, V ve, float su, float sv, float eps) {
+    const V vh = normalize(mk(b.ax * ve.x, b.ay * ve.y, ve.z));
+    const float lensq = vh.x * vh.x + vh.y * vh.y;
+    const V t1 = lensq > eps ? mk(-vh.y, vh.x, 0) / sqrtf(lensq) : mk(1, 0, 0);
+    const V t2 = cross(vh, t1);
+    const float r = sqrtf(su);
+    const float phi = 2 * PI_F * sv;
+    const float t_1 = r * cosf(phi);
+    const float _t_2 = r * sinf(phi);
+    const float s = 0.5f * (1 + vh.z);
+    const float t_2 = (1 - s) * sqrtf(1 - t_1 * t_1) + s * _t_2;
+    const V nh = t1 * t_1 + t2 * t_2 + vh * sqrtf(smax(0.0f, 1 - t_1 * t_1 - t_2 * t_2));
+    return normalize(mk(b.ax * nh.x, b.ay * nh.y, smax(0.0f, nh.z)));
+}
+
+__device__ __forceinline__ bool fringe(const Bsdf &b, V w) { return pos_hemi(b.shd, w) != pos_hemi(b.geo, w); }
+
+// BSDF::eval (component/aggregate.h:39-64; glass/mirror are delta: 0)
+__device__ V bsdf_eval(const Bsdf &b, V wi, V wo) {
+    if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return mk(0, 0, 0);
+    if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:133-139
+        if (!pos_hemi(b.geo, wi) || !pos_hemi(b.geo, wo)) return mk(0, 0, 0);
+        return b.albedo / PI_F;
+    }
+    const V lwi = normalize(to_local(b.shd, wi));
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (lwi.z == 0.0f || lwo.z == 0.0f) return mk(0, 0, 0);
+    if (b.kind == AGX_MAT_IDEAL_DIFFUSE) {
+        if (lwi.z <= 0 || lwo.z <= 0) return mk(0, 0, 0);
+        return b.coef;
+    }
+    V f;
+    ggx_eval_pdf(b, lwi, lwo, &f, nullptr);
+    return f;
+}
+
+// BSDF::pdf (aggregate.h:218-246)
+__device__ float bsdf_pdf(const Bsdf &b, V wi, V wo) {
+    if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return 0;
+    if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:178-185
+        if (pos_hemi(b.geo, wi) != pos_hemi(b.geo, wo)) return 0;
+        return cos_hemisphere_pdf(normalize(to_local(b.geo, wi)).z);
+    }
+    const V lwi = normalize(to_local(b.shd, wi));
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (lwi.z == 0.0f || lwo.z == 0.0f) return 0;
+    float p;
+    if (b.kind == AGX_MAT_IDEAL_DIFFUSE) p = (lwi.z <= 0 || lwo.z <= 0) ? 0.0f : cos_hemisphere_pdf(lwi.z);
+    else ggx_eval_pdf(b, lwi, lwo, nullptr, &p);
+    return 1.0f * p / 1.0f;
+}
+
+// BSDF::sample (aggregate.h:66-139, glass.cpp:52-91, mirror.cpp:32-50)
+__device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, float eps) {
+    BSmp r;
+    r.dir = mk(0, 0, 0);
+    r.f = mk(0, 0, 0);
+    r.pdf = 0;
+    r.delta = false;
+    if (b.kind == AGX_MAT_IDEAL_BLACK) return r;
+    if (b.kind == AGX_MAT_MIRROR) {
+        const V lo = to_local(b.shd, wo);
+        if (lo.z <= 0) return r;
+        const V nwo = normalize(lo);
+        const V dir = to_global(b.shd, mk(0, 0, 2 * nwo.z) - nwo);
+        const V f = conductor(nwo.z, b.eta2, b.k2) * b.albedo / fabsf(nwo.z);
+        r.dir = dir; r.f = f * normal_corr(b.geo.z, b.shd.z, dir, eps); r.pdf = 1; r.delta = true;
+        return r;
+    }
+    if (b.kind == AGX_MAT_GLASS) {
+        const V nwo = normalize(to_local(b.shd, wo));
+        const V nor = nwo.z > 0 ? mk(0, 0, 1) : mk(0, 0, -1);
+        const float fr = dielectric(b.ior, 1.0f, nwo.z);
+        if (su < fr) {
+            const V lwi = mk(-nwo.x, -nwo.y, nwo.z);
+            const V wi = to_global(b.shd, lwi);
+            const V f = b.c_refl * fr / fabsf(lwi.z);
+            r.dir = wi; r.f = f * normal_corr(b.geo.z, b.shd.z, wi, eps); r.pdf = fr; r.delta = true;
+            return r;
+        }
+        const float eta_i = nwo.z > 0 ? 1.0f : b.ior;
+        const float eta_t = nwo.z > 0 ? b.ior : 1.0f;
+        const float eta = eta_i / eta_t;
+        const float cti = fabsf(nwo.z);
+        const float st2 = eta * eta * smax(0.0f, 1 - cti * cti);
+        if (st2 >= 1) return r;
+        const float ctt = sqrtf(1 - st2);
+        const V nwi = normalize(nor * (eta * cti - ctt) - nwo * eta);
+        const float corr = eta * eta;
+        const V dir = to_global(b.shd, nwi);
+        const V f = b.c_refr * corr * (1 - fr) / fabsf(nwi.z);
+        r.dir = dir; r.f = f * normal_corr(b.geo.z, b.shd.z, dir, eps); r.pdf = 1 - fr; r.delta = true;
+        return r;
+    }
+    if (fringe(b, wo)) {  // bsdf.h:142-159
+        if (!pos_hemi(b.geo, wo)) return r;
+        float p;
+        const V lwi = cos_hemisphere(su, sv, &p);
+        if (p < eps) return r;
+        const V wi = normalize(to_global(b.geo, lwi));
+        r.dir = wi; r.f = b.albedo / PI_F * normal_corr(b.geo.z, b.shd.z, wi, eps); r.pdf = p;
+        return r;
+    }
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (lwo.z == 0.0f) return r;
+    if (!((su - 0.001f) * 1.0f <= 1.0f)) return r;  // single component, weight 1
+    V lwi, f;
+    float p;
+    if (b.kind == AGX_MAT_IDEAL_DIFFUSE) {  // diffuse_comp.cpp:29-45
+        if (lwo.z <= 0) return r;
+        lwi = cos_hemisphere(sv, sw, &p);
+        if (p < 0.001f) return r;
+        f = b.coef;
+    } else {  // microfacet_refl_comp.cpp:85-100
+        if (lwo.z <= 0) return r;
+        const V lwh = normalize(ggx_vndf(b, lwo, sv, sw, eps));
+        if (lwh.z <= 0) return r;
+        lwi = normalize(lwh * (2 * dot(lwo, lwh)) - lwo);
+        if (lwi.z <= 0) return r;
+        ggx_eval_pdf(b, lwi, lwo, &f, &p);
+    }
+    lwi = normalize(lwi);
+    p *= 1.0f;
+    const V wi = to_global(b.shd, lwi);
+    r.dir = wi; r.f = f * normal_corr(b.geo.z, b.shd.z, wi, eps); r.pdf = p;
+    return r;
+}
+
+// ---------------------------------------------------------------- wave-level counter reduction
+__device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounters *cnt) {
+    uint32_t v = c.visits, t = c.tests, h = ok ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+        t += __shfl_xor(t, off);
+        h += __shfl_xor(h, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&cnt->node_visits, (unsigned long long)v);
+        atomicAdd(&cnt->tri_tests, (unsigned long long)t);
+        atomicAdd(&cnt->hits, (unsigned long long)h);
+    }
+}
+
+// ---------------------------------------------------------------- kernels
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers pb, uint32_t n, DevCounters *cnt) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts c;
+    bool found = false;
+    if (tid < n) {
+        const uint32_t ri = pb.q_closest[tid];
+        const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
+        HitRec h;
+        h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
+        found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c);
+        if (!found) h.ent = -1;
+        pb.hit[ri] = make_float4(h.t, h.u, h.v, __int_as_float(h.ent));
+        pb.hit_prim[ri] = h.prim;
+    }
+    if (STATS) flush_counts(c, found, cnt);
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, uint32_t n, DevCounters *cnt) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts c;
+    bool occ = false;
+    if (tid < n) {
+        const uint32_t ri = pb.q_shadow[tid];
+        const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
+        occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c);
+        pb.occluded[ri] = occ ? 1 : 0;
+    }
+    if (STATS) flush_counts(c, false, cnt);
+}
+
+__global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, PathBuffers pb) {
+    const int32_t p = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= rc.n_paths) return;
+    const int64_t wh = int64_t(rc.W) * rc.H;
+    const int64_t w = rc.work_base + p;
+    const int64_t pixel = w % wh;
+    const int64_t sample = int64_t(rc.sample_begin) + w / wh;
+    const int32_t px = int32_t(pixel % rc.W), py = int32_t(pixel / rc.W);
+    Rng rng = rng_seed(rc.seed, path_seq((unsigned long long)pixel, (unsigned long long)sample));
+    // PerPixelRenderer::render_grid (perpixel_renderer.cpp:27-38)
+    const float fu = rng.uniform(), fv = rng.uniform();
+    const float pxf = float(px) + fu, pyf = float(py) + fv;
+    const float au = rng.uniform(), av = rng.uniform();
+    const float fx = pxf / float(rc.W), fy = pyf / float(rc.H);
+    // ThinLensCamera::sample_we (thin_lens.cpp:74-94)
+    const V film = mk((0.5f - fx) * rc.film_w, (fy - 0.5f) * rc.film_h, rc.focal);
+    float dx, dy;
+    unit_disk(au, av, &dx, &dy);
+    const V lens = mk(rc.lens_r * dx, rc.lens_r * dy, 0);
+    const V cx = mk(rc.cam_x), cy = mk(rc.cam_y), cz = mk(rc.cam_z);
+    const V o = mk(rc.cam_pos) + cx * lens.x + cy * lens.y + cz * lens.z;
+    const V l = film - lens;
+    const V d = normalize(cx * l.x + cy * l.y + cz * l.z);
+    // box filter support: the sample reaches its own pixel only inside radius r
+    const bool keep = fabsf(float(px) + 0.5f - pxf) <= rc.filter_r && fabsf(float(py) + 0.5f - pyf) <= rc.filter_r;
+    uint32_t flags = keep ? PF_KEEP : 0u;
+    float4 coef = make_float4(1, 1, 1, 0);
+    pb.radiance[p] = make_float4(0, 0, 0, 0);
+    pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
+    pb.gb_normal[p] = make_float4(0, 0, 0, 0);
+    pb.depth[p] = make_int2(1, 1);
+    bool alive = true;
+    if (1 > rc.min_depth) {  // path_tracing.cpp:30-35 at depth 1
+        if (rng.uniform() > rc.cont_prob) alive = false;
+        else coef = make_float4(coef.x / rc.cont_prob, coef.y / rc.cont_prob, coef.z / rc.cont_prob, 0);
+    }
+    if (alive && 1 <= rc.max_depth) {
+        flags |= PF_CONT;
+        pb.ray_o[2 * p] = f4(o, 0.0f);
+        pb.ray_d[2 * p] = f4(d, INF_F);
+        const uint32_t qi = atomicAdd(&pb.counters[2], 1u);
+        pb.q_out[qi] = uint32_t(p);
+        const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+        pb.q_closest[ci] = uint32_t(2 * p);
+    }
+    pb.coef[p] = coef;
+    pb.flags[p] = flags;
+    pb.rng[p] = rng.state;
+}
+
+// resolve the MIS-BSDF ray of the previous vertex (direct_illum.cpp:130-192)
+__device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, uint32_t flags, float eps) {
+    const float4 ma = pb.mis_a[p], mb = pb.mis_b[p];
+    const V coef = mk(ma), bf = mk(mb);
+    const float absdot = ma.w, pdf = mb.w;
+    const bool delta = (flags & PF_MIS_DELTA) != 0;
+    const float4 h4 = pb.hit[2 * p + 1];
+    const int32_t ent = __float_as_int(h4.w);
+    const V o = mk(pb.ray_o[2 * p + 1]), d = mk(pb.ray_d[2 * p + 1]);
+    if (ent < 0) {
+        if (s.env_kind == AGX_ENV_NONE) return mk(0, 0, 0);
+        const V lr = sky(s, d);
+        if (black(lr)) return mk(0, 0, 0);
+        const V f = lr * bf * absdot;
+        if (delta) return coef * (f / pdf);
+        return coef * (f / (pdf + SPHERE_PDF));
+    }
+    const DevEntity &e = s.ents[ent];
+    if (e.light < 0) return mk(0, 0, 0);
+    HitRec h;
+    h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p + 1];
+    const Surf sp = surface_at(s, h, o, d, eps);
+    const V lrad = dot(sp.geo.z, sp.wr) > 0 ? mk(e.emit) : mk(0, 0, 0);
+    if (black(lrad)) return mk(0, 0, 0);
+    const V f = lrad * bf * absdot;
+    if (delta) return coef * (f / pdf);
+    const float lpdf = area_light_pdf(e, o, sp.pos, sp.geo.z);
+    return coef * (f / (pdf + lpdf));
+}
+
+__global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb, uint32_t n) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    const uint32_t p = pb.q_in[tid];
+    const float eps = rc.eps;
+    uint32_t flags = pb.flags[p];
+    V L = mk(pb.radiance[p]);
+    // 1. direct illumination of the previous vertex (path_tracing.cpp:134-146)
+    if (flags & PF_PENDING) {
+        V direct = mk(0, 0, 0);
+        for (int32_t l = 0; l < rc.L; ++l) {
+            const float4 c = pb.nee[size_t(p) * rc.L + l];
+            if (c.w != 0.0f && !pb.occluded[size_t(p) * rc.L + l]) direct = direct + mk(c);
+        }
+        if (flags & PF_MIS) direct = direct + resolve_mis(s, pb, p, flags, eps);
+        L = L + direct;
+        flags &= ~(PF_PENDING | PF_MIS | PF_MIS_DELTA);
+    }
+    if (!(flags & PF_CONT)) {
+        pb.radiance[p] = f4(L, 0);
+        pb.flags[p] = flags;
+        return;
+    }
+    flags &= ~PF_CONT;
+    V coef = mk(pb.coef[p]);
+    int2 dep = pb.depth[p];
+    const float4 h4 = pb.hit[2 * p];
+    const int32_t ent = __float_as_int(h4.w);
+    const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
+    if (ent < 0) {  // path_tracing.cpp:41-49
+        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * sky(s, rd);
+        pb.radiance[p] = f4(L, 0);
+        pb.flags[p] = flags;
+        return;
+    }
+    HitRec h;
+    h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
+    const Surf sp = surface_at(s, h, ro, rd, eps);
+    const DevEntity &e = s.ents[ent];
+    const Bsdf b = make_bsdf(s.mats[e.material], sp);
+    if (dep.x == 1) {  // :54-60
+        pb.gb_normal[p] = f4(sp.user.z, 0);
+        pb.gb_albedo[p] = f4(b.albedo, (e.flags & 1) ? 0.0f : 1.0f);
+    }
+    if (dep.x == 1 && e.light >= 0)  // :123-130
+        if (dot(sp.geo.z, sp.wr) > 0) L = L + coef * mk(e.emit);
+    Rng rng;
+    rng.state = pb.rng[p];
+    {
+        const int64_t wh = int64_t(rc.W) * rc.H;
+        const int64_t w = rc.work_base + int64_t(p);
+        rng.inc = (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
+    }
+    // 2. NEE over every light (mis_sample_light, direct_illum.cpp:11-42, 72-99)
+    bool pending = false;
+    for (int32_t l = 0; l < rc.L; ++l) {
+        float s5[5];
+        for (int k = 0; k < 5; ++k) s5[k] = rng.uniform();
+        const size_t slot = size_t(p) * rc.L + l;
+        const int32_t li = s.lights[l];
+        V contrib = mk(0, 0, 0);
+        bool valid = false;
+        V so, sd;
+        float tmin = eps, tmax = 0;
+        if (li >= 0) {
+            const DevEntity &le = s.ents[li];
+            const LightSmp ls = area_light_sample(s, le, sp.pos, s5[0], s5[1], s5[3], eps);
+            const V rad = mk(le.emit);
+            if (!black(rad) && ls.pdf != 0.0f) {
+                const float len = length(ls.pos - sp.pos) - eps;
+                if (len > eps) {
+                    const V tl = normalize(ls.pos - sp.pos);
+                    const V f = bsdf_eval(b, tl, sp.wr);
+                    if (!black(f)) {
+                        const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
+                        const float bp = bsdf_pdf(b, tl, sp.wr);
+                        contrib = coef * (ff / (ls.pdf + bp));
+                        valid = true;
+                        so = sp.pos; sd = tl; tmax = len;
+                    }
+                }
+            }
+        } else {
+            const V dir = uniform_sphere(s5[0], s5[1]);
+            const V lpos = env_emit_pos(s, sp.pos, dir);
+            const V rad = sky(s, dir);
+            if (!black(rad)) {
+                const V tl = normalize(lpos - sp.pos);
+                const V f = bsdf_eval(b, tl, sp.wr);
+                if (!black(f)) {
+                    const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
+                    const float bp = bsdf_pdf(b, tl, sp.wr);
+                    contrib = coef * (ff / (SPHERE_PDF + bp));
+                    valid = true;
+                    // Scene::visible(A, B): [eps, |A-B| - eps] along normalize(B - A)
+                    so = sp.pos; sd = normalize(lpos - sp.pos); tmax = length(sp.pos - lpos) - eps;
+                }
+            }
+        }
+        pb.nee[slot] = f4(contrib, valid ? 1.0f : 0.0f);
+        if (valid) {
+            pending = true;
+            pb.sh_o[slot] = f4(so, tmax);
+            pb.sh_d[slot] = f4(sd, tmin);
+            const uint32_t qi = atomicAdd(&pb.counters[1], 1u);
+            pb.q_shadow[qi] = uint32_t(slot);
+        }
+    }
+    // 3. MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
+    {
+        const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+        const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+        if (!black(bs.f)) {
+            const V dir = normalize(bs.dir);
+            pb.ray_o[2 * p + 1] = f4(eps_offset(sp, dir, eps), 0.0f);
+            pb.ray_d[2 * p + 1] = f4(dir, INF_F);
+            pb.mis_a[p] = f4(coef, fabsf(dot(sp.geo.z, dir)));
+            pb.mis_b[p] = f4(bs.f, bs.pdf);
+            flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u);
+            pending = true;
+            const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+            pb.q_closest[ci] = 2 * p + 1;
+        }
+    }
+    if (pending) flags |= PF_PENDING;
+    pb.radiance[p] = f4(L, 0);
+    // 4. continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
+    bool cont = false;
+    {
+        const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+        const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+        if (!black(bs.f) && !(bs.pdf < eps)) {
+            const float abscos = fabsf(cosv(sp.geo.z, bs.dir));
+            coef = coef * (bs.f * abscos / bs.pdf);
+            pb.ray_o[2 * p] = f4(eps_offset(sp, bs.dir, eps), 0.0f);
+            pb.ray_d[2 * p] = f4(normalize(bs.dir), INF_F);
+            if (bs.delta && dep.x >= 2 && dep.y <= rc.specular_depth) { dep.x -= 1; dep.y += 1; }
+            dep.x += 1;
+            if (dep.x <= rc.max_depth) {
+                cont = true;
+                if (dep.x > rc.min_depth) {
+                    if (rng.uniform() > rc.cont_prob) cont = false;
+                    else coef = coef / rc.cont_prob;
+                }
+            }
+        }
+    }
+    if (cont) {
+        flags |= PF_CONT;
+        const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+        pb.q_closest[ci] = 2 * p;
+    }
+    if (flags & (PF_CONT | PF_PENDING)) {
+        const uint32_t qi = atomicAdd(&pb.counters[2], 1u);
+        pb.q_out[qi] = p;
+    }
+    pb.coef[p] = f4(coef, 0);
+    pb.depth[p] = dep;
+    pb.flags[p] = flags;
+    pb.rng[p] = rng.state;
+}
+
+// accumulate the pass into the film in sample order, Neumaier-compensated
+__device__ __forceinline__ void neumaier(float *sum, float *comp, float x) {
+    const float s = *sum;
+    const float t = s + x;
+    if (fabsf(s) >= fabsf(x)) *comp += (s - t) + x;
+    else *comp += (x - t) + s;
+    *sum = t;
+}
+
+__global__ void __launch_bounds__(256) k_film(RenderConst rc, PathBuffers pb) {
+    const int64_t wh = int64_t(rc.W) * rc.H;
+    const int64_t cover = rc.n_paths < wh ? int64_t(rc.n_paths) : wh;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cover) return;
+    const int64_t pixel = (rc.work_base + i) % wh;
+    const int64_t layers = rc.n_paths >= wh ? int64_t(rc.n_paths) / wh : 1;
+    float *sum = pb.film_sum, *cmp = pb.film_comp;
+    for (int64_t j = 0; j < layers; ++j) {
+        const int64_t slot = j * wh + i;
+        if (slot >= rc.n_paths) break;
+        if (!(pb.flags[slot] & PF_KEEP)) continue;
+        const float4 v = pb.radiance[slot];
+        if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
+        const float4 a = pb.gb_albedo[slot], n = pb.gb_normal[slot];
+        const float vals[10] = {v.x, v.y, v.z, a.x, a.y, a.z, n.x, n.y, n.z, a.w};
+        for (int k = 0; k < 10; ++k) neumaier(&sum[k * wh + pixel], &cmp[k * wh + pixel], vals[k]);
+        pb.film_count[pixel] += 1.0f;
+    }
+}
+
+__global__ void k_film_clear(PathBuffers pb, int64_t wh) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= wh) return;
+    for (int k = 0; k < 10; ++k) { pb.film_sum[k * wh + i] = 0; pb.film_comp[k * wh + i] = 0; }
+    pb.film_count[i] = 0;
+}
+
+__global__ void k_film_resolve(PathBuffers pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= wh) return;
+    const float c = pb.film_count[i];
+    const float inv = c > 0 ? 1.0f / c : 1.0f;
+    float v[10];
+    for (int k = 0; k < 10; ++k) v[k] = (pb.film_sum[k * wh + i] + pb.film_comp[k * wh + i]) * inv;
+    for (int k = 0; k < 3; ++k) {
+        rgb[3 * i + k] = v[k];
+        albedo[3 * i + k] = v[3 + k];
+        normal[3 * i + k] = v[6 + k];
+    }
+    denoise[i] = v[9];
+}
+
+__global__ void k_reset_counters(uint32_t *c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0;
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
+                                                    int any_hit, int32_t *ent, int32_t *prim, float *t, float *u,
+                                                    float *v, DevCounters *cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    Counts c;
+    bool found = false;
+    if (i < n) {
+        const float4 o4 = o[i], d4 = d[i];
+        if (any_hit) {
+            found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c);
+            ent[i] = found ? 1 : -1;
+            found = false;
+        } else {
+            HitRec h;
+            h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
+            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c);
+            if (found) {
+                const DevEntity &e = s.ents[h.ent];
+                int32_t pid = h.prim;
+                if (e.kind == AGX_GEOM_TRIANGLE_BVH) pid = int32_t(fbits(s.blas_prims[3 * uint32_t(h.prim)].w));
+                ent[i] = h.ent; prim[i] = pid; t[i] = h.t; u[i] = h.u; v[i] = h.v;
+            } else {
+                ent[i] = -1; prim[i] = -1; t[i] = -1; u[i] = 0; v[i] = 0;
+            }
+        }
+    }
+    if (STATS) flush_counts(c, found, cnt);
+}
+
+inline unsigned grid_of(uint64_t n, unsigned b) { return unsigned((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, hipStream_t st) {
+    if (rc.n_paths <= 0) return;
+    hipLaunchKernelGGL(k_raygen, dim3(grid_of(uint64_t(rc.n_paths), 256)), dim3(256), 0, st, s, rc, pb);
+}
+void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+                          hipStream_t st) {
+    if (!n) return;
+    if (stats) hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+}
+void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+                         hipStream_t st) {
+    if (!n) return;
+    if (stats) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+}
+void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, pb.counters);
+    if (!n) return;
+    hipLaunchKernelGGL(k_shade, dim3(grid_of(n, 256)), dim3(256), 0, st, s, rc, pb, n);
+}
+void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st) {
+    const int64_t wh = int64_t(rc.W) * rc.H;
+    const int64_t cover = rc.n_paths < wh ? int64_t(rc.n_paths) : wh;
+    if (cover <= 0) return;
+    hipLaunchKernelGGL(k_film, dim3(grid_of(uint64_t(cover), 256)), dim3(256), 0, st, rc, pb);
+}
+void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st) {
+    hipLaunchKernelGGL(k_film_clear, dim3(grid_of(uint64_t(wh), 256)), dim3(256), 0, st, pb, wh);
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, pb.counters);
+}
+void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,
+                         hipStream_t st) {
+    hipLaunchKernelGGL(k_film_resolve, dim3(grid_of(uint64_t(wh), 256)), dim3(256), 0, st, pb, wh, rgb, albedo,
+                       normal, denoise);
+}
+void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
+                       int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {
+    if (!n) return;
+    if (cnt)
+        hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, o, d, n, any_hit, ent, prim,
+                           t, u, v, cnt);
+    else
+        hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, o, d, n, any_hit, ent,
+                           prim, t, u, v, cnt);
+}
+
+}  // namespace agx

@@ -1,0 +1,312 @@
+"""Scene descriptors for the C ABI, built from Python, and the seeded synthetic
+scenes the benchmarks and tests use.
+
+``Scene`` mirrors what Atrc's factory creates from JSON (entities with a
+geometry, a material and an optional emission; an environment light; an
+aggregate) and produces an ``agx_scene_desc`` that both the GPU library and
+the CPU oracle accept.  Meshes are float32 arrays of shape (3*n, 8):
+position, normal and uv of each triangle corner, already in world space (Atrc
+transforms triangle_bvh meshes to world space at construction,
+src/tracer/src/core/geometry/triangle_bvh.cpp:533-558).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+IDENTITY = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0)
+
+
+@dataclass
+class Material:
+    kind: int = capi.MAT_IDEAL_DIFFUSE
+    color: tuple = (0.5, 0.5, 0.5)
+    color2: tuple = (0.0, 0.0, 0.0)
+    eta: tuple = (1.0, 1.0, 1.0)
+    k: tuple = (0.0, 0.0, 0.0)
+    roughness: float = 0.5
+    anisotropic: float = 0.0
+    ior: float = 1.5
+    separate_refraction: bool = False
+
+
+@dataclass
+class Entity:
+    geometry: int
+    material: int
+    emit: tuple = (0.0, 0.0, 0.0)
+    tris: np.ndarray | None = None       # (3n, 8) float32 world-space mesh
+    points: tuple = ()                   # quad A..D (world) / triangle A..C (local)
+    uvs: tuple = ()
+    radius: float = 1.0
+    local_to_world: tuple = IDENTITY     # row-major 3x4
+    no_denoise: bool = False
+
+
+@dataclass
+class Scene:
+    entities: list = field(default_factory=list)
+    materials: list = field(default_factory=list)
+    env: dict | None = None              # {"top": (r,g,b), "bottom": (r,g,b)}
+    aggregate: int = capi.AGGREGATE_NATIVE
+    max_leaf_size: int = 5
+
+    def add_material(self, m: Material) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add(self, e: Entity) -> int:
+        self.entities.append(e)
+        return len(self.entities) - 1
+
+    def triangle_count(self) -> int:
+        return sum(len(e.tris) // 3 for e in self.entities if e.tris is not None)
+
+    def to_desc(self):
+        """Returns (agx_scene_desc, keepalive); keep both alive while the descriptor is used."""
+        keep = []
+        ents = (capi.agx_entity_desc * max(1, len(self.entities)))()
+        for i, e in enumerate(self.entities):
+            d = ents[i]
+            d.geometry = e.geometry
+            d.material = e.material
+            d.medium_in = -1
+            d.medium_out = -1
+            d.emit_radiance[:] = [float(x) for x in e.emit]
+            d.power = -1.0
+            d.no_denoise = 1 if e.no_denoise else 0
+            if e.tris is not None:
+                arr = np.ascontiguousarray(e.tris, np.float32)
+                keep.append(arr)
+                d.tri_count = len(arr) // 3
+                d.tris = arr.ctypes.data_as(C.POINTER(capi.agx_vertex))
+            for k, p in enumerate(e.points):
+                d.p[k][:] = [float(x) for x in p]
+            for k, t in enumerate(e.uvs):
+                d.t[k][:] = [float(x) for x in t]
+            d.radius = float(e.radius)
+            m = np.array(e.local_to_world, np.float64).reshape(3, 4)
+            full = np.eye(4)
+            full[:3, :] = m
+            inv = np.linalg.inv(full)[:3, :]
+            d.local_to_world[:] = [float(x) for x in m.reshape(-1).astype(np.float32)]
+            d.world_to_local[:] = [float(x) for x in inv.reshape(-1).astype(np.float32)]
+        mats = (capi.agx_material_desc * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            d = mats[i]
+            d.kind = m.kind
+            d.color[:] = [float(x) for x in m.color]
+            d.color2[:] = [float(x) for x in m.color2]
+            d.eta[:] = [float(x) for x in m.eta]
+            d.k[:] = [float(x) for x in m.k]
+            d.roughness = m.roughness
+            d.anisotropic = m.anisotropic
+            d.ior = m.ior
+            d.glass_separate_refraction = 1 if m.separate_refraction else 0
+        desc = capi.agx_scene_desc()
+        desc.abi_version = capi.AGX_ABI_VERSION
+        desc.entity_count = len(self.entities)
+        desc.entities = C.cast(ents, C.POINTER(capi.agx_entity_desc))
+        desc.material_count = len(self.materials)
+        desc.materials = C.cast(mats, C.POINTER(capi.agx_material_desc))
+        if self.env is not None:
+            desc.env.kind = capi.ENV_NATIVE_SKY
+            desc.env.top[:] = [float(x) for x in self.env["top"]]
+            desc.env.bottom[:] = [float(x) for x in self.env["bottom"]]
+            desc.env.power = -1.0
+        desc.aggregate = self.aggregate
+        desc.max_leaf_size = self.max_leaf_size
+        keep += [ents, mats]
+        return desc, keep
+
+
+def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth: int = 5, max_depth: int = 10,
+                  cont_prob: float = 0.9, specular_depth: int = 20, eps: float = 3e-4, seed: int = 42,
+                  filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0):
+    p = capi.agx_render_params()
+    p.width, p.height, p.spp = width, height, spp
+    p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
+    p.cont_prob = cont_prob
+    p.use_mis = 1
+    p.eps = eps
+    p.filter_radius = filter_radius
+    p.seed = seed
+    p.camera.pos[:] = camera["pos"]
+    p.camera.dst[:] = camera["dst"]
+    p.camera.up[:] = camera.get("up", (0, 0, 1))
+    p.camera.fov_rad = math.radians(camera.get("fov", 40.0))
+    p.camera.lens_radius = camera.get("lens_radius", 0.0)
+    p.camera.focal_distance = camera.get("focal_distance", 1.0)
+    p.camera.film_aspect = width / height
+    p.collect_stats = 1 if collect_stats else 0
+    p.sample_begin = sample_begin
+    return p
+
+
+# ---------------------------------------------------------------------------------- meshes
+
+def mesh_from_indexed(pos: np.ndarray, nor: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    """(3n, 8) vertex records from indexed positions/normals (uv = 0)."""
+    out = np.zeros((len(faces) * 3, 8), np.float32)
+    idx = faces.reshape(-1)
+    out[:, 0:3] = pos[idx]
+    out[:, 3:6] = nor[idx]
+    return out
+
+
+def box_mesh(lo, hi, rot_deg: float = 0.0) -> np.ndarray:
+    """Axis box [lo, hi] rotated about its vertical (z) centre axis: 12 triangles,
+    flat outward normals."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    c = (lo + hi) / 2
+    corners = np.array([[x, y, z] for z in (lo[2], hi[2]) for y in (lo[1], hi[1]) for x in (lo[0], hi[0])])
+    a = math.radians(rot_deg)
+    R = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    corners = (corners - c) @ R.T + c
+    quads = [(0, 2, 3, 1, (0, 0, -1)), (4, 5, 7, 6, (0, 0, 1)), (0, 1, 5, 4, (0, -1, 0)),
+             (2, 6, 7, 3, (0, 1, 0)), (0, 4, 6, 2, (-1, 0, 0)), (1, 3, 7, 5, (1, 0, 0))]
+    tris = []
+    for q0, q1, q2, q3, n in quads:
+        nn = R @ np.array(n, np.float64)
+        for (i, j, k) in ((q0, q1, q2), (q0, q2, q3)):
+            for v in (i, j, k):
+                tris.append([*corners[v], *nn, 0.0, 0.0])
+    return np.array(tris, np.float32)
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: (vertices, faces); 20 * 4^subdiv faces."""
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+         (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+         (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in v]
+    faces = f
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts), np.array(faces, np.int64)
+
+
+def _value_noise(p: np.ndarray, seed: int) -> np.ndarray:
+    """Seeded trilinear lattice value noise in [-1, 1]."""
+    pi = np.floor(p).astype(np.int64)
+    f = p - pi
+    w = f * f * (3 - 2 * f)
+
+    def h(ix, iy, iz):
+        x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        x = (x ^ (x >> 13)) * 1274126177
+        x = x ^ (x >> 16)
+        return (x & 0xFFFF).astype(np.float64) / 32767.5 - 1.0
+
+    out = 0.0
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                wx = w[:, 0] if dx else 1 - w[:, 0]
+                wy = w[:, 1] if dy else 1 - w[:, 1]
+                wz = w[:, 2] if dz else 1 - w[:, 2]
+                out = out + wx * wy * wz * h(pi[:, 0] + dx, pi[:, 1] + dy, pi[:, 2] + dz)
+    return out
+
+
+def vertex_normals(pos: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    fn = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
+    n = np.zeros_like(pos)
+    for k in range(3):
+        np.add.at(n, faces[:, k], fn)
+    return n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+# ---------------------------------------------------------------------------------- scenes
+
+def cornell_box() -> tuple[Scene, dict, dict]:
+    """Cornell-box-like scene: 5 wall quads, 2 rotated 12-triangle boxes, a ceiling
+    quad light of radiance (17, 12, 4); z is up.  Returns (scene, camera, trace settings)."""
+    L = 5.55
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    white = s.add_material(Material(color=(0.75, 0.75, 0.75)))
+    red = s.add_material(Material(color=(0.63, 0.06, 0.05)))
+    green = s.add_material(Material(color=(0.14, 0.45, 0.09)))
+    quad = lambda a, b, c, d, m, emit=(0, 0, 0): s.add(Entity(capi.GEOM_QUAD, m, emit=emit, points=(a, b, c, d)))
+    quad((0, 0, 0), (L, 0, 0), (L, L, 0), (0, L, 0), white)          # floor, normal +z
+    quad((0, 0, L), (0, L, L), (L, L, L), (L, 0, L), white)          # ceiling, -z
+    quad((0, L, 0), (L, L, 0), (L, L, L), (0, L, L), white)          # back wall, -y
+    quad((0, 0, 0), (0, L, 0), (0, L, L), (0, 0, L), red)            # left wall, +x
+    quad((L, 0, 0), (L, 0, L), (L, L, L), (L, L, 0), green)          # right wall, -x
+    s.add(Entity(capi.GEOM_TRIANGLE_BVH, white, tris=box_mesh((1.0, 0.9, 0.0), (2.65, 2.55, 1.65), 15.0)))
+    s.add(Entity(capi.GEOM_TRIANGLE_BVH, white, tris=box_mesh((2.9, 2.7, 0.0), (4.55, 4.35, 3.3), -18.0)))
+    h = L - 0.01
+    quad((2.125, 2.25, h), (2.125, 3.3, h), (3.425, 3.3, h), (3.425, 2.25, h), white, emit=(17, 12, 4))
+    camera = {"pos": (L / 2, -8.0, L / 2), "dst": (L / 2, 0.0, L / 2), "up": (0, 0, 1), "fov": 40.0}
+    trace = {"min_depth": 2, "max_depth": 5, "cont_prob": 0.9}
+    return s, camera, trace
+
+
+def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, extent: float = 50.0,
+                       ground: bool = True) -> tuple[Scene, dict, dict]:
+    """Seeded procedural mesh scene: n icospheres (20*4^subdiv triangles each) of
+    radius U[0.2, 2] in a [0, extent]^3 box, radially displaced by value noise of
+    amplitude 0.1 r, on a ground quad; half Lambert (albedo U[0.1, 0.9]), half
+    GGX conductor (roughness U[0.05, 0.6], gold or copper); native_sky.
+    200 spheres at subdivision 4 give 1,024,000 triangles."""
+    rng = np.random.default_rng(seed)
+    base_v, faces = icosphere(subdiv)
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    # physical (eta, k) of gold and copper at R, G, B; Atrc's metal reads the JSON
+    # "eta" as k and "k" as eta (material_creators.cpp:252-266), so they are stored swapped
+    gold = ((0.143, 0.374, 1.442), (3.983, 2.386, 1.603))
+    copper = ((0.200, 0.924, 1.102), (3.912, 2.452, 2.142))
+    for i in range(n_spheres):
+        r = rng.uniform(0.2, 2.0)
+        c = rng.uniform(0.0, extent, 3)
+        disp = 1.0 + 0.1 * _value_noise(base_v * 3.0 + rng.uniform(0, 100, 3), seed * 1000 + i)
+        pos = base_v * (r * disp)[:, None] + c
+        nor = vertex_normals(pos, faces)
+        if rng.uniform() < 0.5:
+            m = s.add_material(Material(kind=capi.MAT_IDEAL_DIFFUSE, color=tuple(rng.uniform(0.1, 0.9, 3))))
+        else:
+            phys_eta, phys_k = gold if rng.uniform() < 0.5 else copper
+            m = s.add_material(Material(kind=capi.MAT_METAL, color=(1.0, 1.0, 1.0), eta=phys_k, k=phys_eta,
+                                        roughness=float(rng.uniform(0.05, 0.6)), anisotropic=0.0))
+        s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, tris=mesh_from_indexed(pos, nor, faces)))
+    if ground:
+        g = s.add_material(Material(color=(0.5, 0.5, 0.5)))
+        lo, hi, z = -100.0, extent + 100.0, -2.5
+        s.add(Entity(capi.GEOM_QUAD, g, points=((lo, lo, z), (hi, lo, z), (hi, hi, z), (lo, hi, z))))
+    s.env = {"top": (0.9, 0.95, 1.2), "bottom": (0.25, 0.22, 0.2)}
+    camera = {"pos": (-25.0, -35.0, 45.0), "dst": (25.0, 25.0, 15.0), "up": (0, 0, 1), "fov": 40.0}
+    trace = {"min_depth": 5, "max_depth": 8, "cont_prob": 0.9}
+    return s, camera, trace
+
+
+def random_rays(scene_bounds: tuple, n: int, seed: int, inside: bool = True):
+    """Seeded rays: origins uniform in the box, directions uniform on the sphere."""
+    rng = np.random.default_rng(seed)
+    lo, hi = (np.asarray(x, np.float64) for x in scene_bounds)
+    o = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o.astype(np.float32), d.astype(np.float32)

@@ -1,0 +1,219 @@
+/*
+ * atrc_gpu.h — C ABI of the MI355X (gfx950) path tracer that stands in for
+ * Atrc's CPU `pt` renderer.
+ *
+ * Reference interface replaced (paths relative to the Atrc source tree):
+ *   - Renderer::render(FilmFilterApplier, Scene&, RendererInteractor&) -> RenderTarget
+ *       src/tracer/include/agz/tracer/core/renderer.h:16-64
+ *   - PathTracingRenderer / create_pt_renderer / PTRendererParams
+ *       src/tracer/src/core/renderer/pt.cpp:8-48, src/tracer/include/agz/tracer/create/renderer.h:10-24
+ *   - Scene::closest_intersection / has_intersection (ray-query parity hook)
+ *       src/tracer/include/agz/tracer/core/scene.h:42-98, src/tracer/src/core/scene/scene_default.cpp:131-147
+ *
+ * Atrc's Scene is query-only, so the scene crosses this boundary as plain
+ * descriptors (agx_scene_desc) that the host factory fills when it builds the
+ * scene from Atrc JSON.  Every pointer is host memory owned by the caller and is
+ * only read during the call that receives it.  No call throws; each returns an
+ * AGX_* status and agx_last_error() gives the message of the calling thread's
+ * last failure.  One context is not re-entrant; distinct contexts are
+ * independent.
+ */
+#ifndef ATRC_GPU_H
+#define ATRC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AGX_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------- */
+enum {
+    AGX_OK              = 0,
+    AGX_E_INVALID_ARG   = 1, /* malformed descriptor / argument (ObjectConstructionException) */
+    AGX_E_HIP           = 2, /* HIP runtime failure */
+    AGX_E_NO_SCENE      = 3, /* render / trace before agx_upload_scene */
+    AGX_E_UNSUPPORTED   = 4, /* a feature the GPU path does not implement */
+    AGX_E_CANCELLED     = 5, /* *cancel_flag became non-zero; outputs hold the partial image */
+    AGX_E_OUT_OF_MEMORY = 6,
+    AGX_E_LIMIT         = 7  /* a built structure exceeds a kernel limit (e.g. BVH depth) */
+};
+
+/* ---- scene descriptors -------------------------------------------------------------- */
+
+/* Atrc geometry "type" values.  src/factory/src/creator/geometry_creators.cpp */
+enum {
+    AGX_GEOM_TRIANGLE_BVH = 0, /* "triangle_bvh" / "triangle_bvh_noembree" (world-space triangles) */
+    AGX_GEOM_QUAD         = 1, /* "quad": A,B,C,D already in world space */
+    AGX_GEOM_TRIANGLE     = 2, /* "triangle": local space + transform */
+    AGX_GEOM_SPHERE       = 3, /* "sphere": local space + transform */
+    AGX_GEOM_DISK         = 4  /* "disk": local space + transform */
+};
+
+/* Atrc material "type" values.  src/factory/src/creator/material_creators.cpp */
+enum {
+    AGX_MAT_IDEAL_DIFFUSE = 0, /* color = albedo */
+    AGX_MAT_METAL         = 1, /* color, eta, k (JSON names), roughness, anisotropic */
+    AGX_MAT_GLASS         = 2, /* color = reflection colour, color2 = refraction colour (before sqrt), ior */
+    AGX_MAT_MIRROR        = 3, /* color = color_map, eta, k */
+    AGX_MAT_IDEAL_BLACK   = 4
+};
+
+enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
+enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
+enum { AGX_MEDIUM_VOID = 0 };
+
+/* One mesh vertex; three consecutive vertices form a triangle. */
+typedef struct agx_vertex {
+    float pos[3];
+    float nor[3];
+    float uv[2];
+} agx_vertex;
+
+/* One Atrc "geometric" entity (src/factory/src/creator/entity_creators.cpp:29-60). */
+typedef struct agx_entity_desc {
+    int32_t geometry;          /* AGX_GEOM_* */
+    int32_t material;          /* index into agx_scene_desc.materials */
+    int32_t medium_in;         /* index into media, -1 = void */
+    int32_t medium_out;
+    float   emit_radiance[3];  /* non-black => the entity is a diffuse area light */
+    float   power;             /* user-specified light power, <= 0 = derived */
+    int32_t no_denoise;
+    uint32_t tri_count;        /* AGX_GEOM_TRIANGLE_BVH */
+    const agx_vertex *tris;    /* AGX_GEOM_TRIANGLE_BVH: 3*tri_count vertices in world space */
+    float   p[4][3];           /* quad A..D (world) / triangle A..C (local) */
+    float   t[4][2];           /* corner texture coordinates */
+    float   radius;            /* sphere, disk */
+    float   local_to_world[12];/* row-major 3x4 (triangle, sphere, disk) */
+    float   world_to_local[12];/* its inverse, row-major 3x4 */
+} agx_entity_desc;
+
+/* Constant-texture material parameters (image textures are not part of this path). */
+typedef struct agx_material_desc {
+    int32_t kind;              /* AGX_MAT_* */
+    float   color[3];
+    float   color2[3];
+    float   eta[3];            /* as written in the JSON ("eta") */
+    float   k[3];              /* as written in the JSON ("k") */
+    float   roughness;
+    float   anisotropic;
+    float   ior;
+    int32_t glass_separate_refraction; /* 1 if color_refraction_map was given separately */
+} agx_material_desc;
+
+typedef struct agx_env_desc {
+    int32_t kind;              /* AGX_ENV_* */
+    float   top[3];
+    float   bottom[3];
+    float   power;             /* <= 0 = derived */
+} agx_env_desc;
+
+/* Thin-lens camera ("thin_lens"; "pinhole" maps onto it).  src/tracer/src/core/camera/thin_lens.cpp */
+typedef struct agx_camera_desc {
+    float pos[3];
+    float dst[3];
+    float up[3];
+    float fov_rad;
+    float lens_radius;
+    float focal_distance;
+    float film_aspect;         /* width / height */
+} agx_camera_desc;
+
+typedef struct agx_scene_desc {
+    uint32_t abi_version;      /* AGX_ABI_VERSION */
+    uint32_t entity_count;
+    const agx_entity_desc *entities;
+    uint32_t material_count;
+    const agx_material_desc *materials;
+    agx_env_desc env;
+    int32_t aggregate;         /* AGX_AGGREGATE_* */
+    int32_t max_leaf_size;     /* entity BVH leaf size (default 5) */
+} agx_scene_desc;
+
+/* PTRendererParams + the rendering settings that reach the renderer. */
+typedef struct agx_render_params {
+    int32_t width, height;
+    int32_t spp;
+    int32_t min_depth;         /* default 5 */
+    int32_t max_depth;         /* default 10 */
+    int32_t specular_depth;    /* default 20 */
+    float   cont_prob;         /* default 0.9 */
+    int32_t use_mis;           /* must be 1 (trace_std) */
+    float   eps;               /* global EPS, default 3e-4 */
+    float   filter_radius;     /* box filter radius as given (default 0.5) */
+    uint64_t seed;             /* per-(pixel, sample) PCG32 streams derive from it */
+    agx_camera_desc camera;
+    int32_t collect_stats;     /* 1: count node visits / triangle tests (slower) */
+    int32_t sample_begin;      /* first sample index rendered (progressive rendering), normally 0 */
+} agx_render_params;
+
+/* Counters and timers of the last agx_render / agx_trace_rays call. */
+typedef struct agx_stats_t {
+    uint64_t closest_rays;     /* closest-hit queries (continuation + MIS-BSDF rays) */
+    uint64_t shadow_rays;      /* any-hit queries (NEE shadow rays) */
+    uint64_t paths;            /* camera samples traced */
+    uint64_t closest_node_visits;  /* BVH node AABB tests in closest-hit queries (collect_stats only) */
+    uint64_t closest_tri_tests;    /* triangle tests in closest-hit queries (collect_stats only) */
+    uint64_t closest_hits;         /* closest-hit queries that hit (collect_stats only) */
+    uint64_t shadow_node_visits;   /* same for any-hit queries */
+    uint64_t shadow_tri_tests;
+    uint64_t closest_bvh_bytes;    /* algorithmic bytes = visits*node + tests*prim + hits*info */
+    uint64_t shadow_bvh_bytes;     /* visits*node + tests*prim */
+    double   total_ms;         /* device time of the whole render (HIP events) */
+    double   trace_closest_ms; /* device time in the closest-hit kernel */
+    double   trace_shadow_ms;  /* device time in the any-hit kernel */
+    double   shade_ms;         /* device time in raygen + shade + film kernels */
+    uint64_t trace_closest_launches;
+    uint64_t trace_shadow_launches;
+    uint32_t node_bytes, prim_bytes, info_bytes; /* record sizes used for the byte counts */
+    uint32_t reserved;
+} agx_stats_t;
+
+typedef struct agx_ctx agx_ctx;
+
+/* Version of this library's ABI (AGX_ABI_VERSION). */
+int  agx_abi_version(void);
+
+/* Open a context on a HIP device: its own stream and device buffers. */
+int  agx_create(int device_ordinal, agx_ctx **out);
+
+/* Build the BVHs (Atrc's midpoint/median triangle BVH and entity BVH) on the host
+ * and copy scene, BVHs and light tables to device memory.  Replaces any
+ * previously uploaded scene.  (DefaultSceneCreator + Aggregate::build +
+ * Scene::start_rendering, src/factory/src/creator/scene_creators.cpp:15-66) */
+int  agx_upload_scene(agx_ctx *ctx, const agx_scene_desc *scene);
+
+/* Render with Atrc `trace_std` semantics (NEE over every light with balance-
+ * heuristic MIS, Russian roulette, specular depth extension) and a box film.
+ * Outputs are host buffers, row-major, row 0 = film y in [0,1) (Atrc's
+ * RenderTarget, before save_to_img flips it): rgb W*H*3, albedo W*H*3,
+ * normal W*H*3, denoise W*H; any may be NULL.  cancel_flag (may be NULL) is
+ * polled between bounces; a cancelled render returns AGX_E_CANCELLED with the
+ * samples finished so far normalised into the outputs. */
+int  agx_render(agx_ctx *ctx, const agx_render_params *params,
+                const volatile int32_t *cancel_flag,
+                float *rgb, float *albedo, float *normal, float *denoise);
+
+/* Ray-query parity hook: n rays in host memory (o, d: n*3 floats; tmin, tmax: n).
+ * any_hit = 0: closest hit; outputs entity id (-1 = miss), primitive id (mesh:
+ * original triangle index; quad: 0 = ABC, 1 = ACD; others 0), t, and the
+ * barycentric (u, v) of the hit triangle.  any_hit = 1: entity_id = 1 if
+ * occluded else -1.  Output pointers other than entity_id may be NULL. */
+int  agx_trace_rays(agx_ctx *ctx, const float *o, const float *d,
+                    const float *tmin, const float *tmax, uint32_t n, int32_t any_hit,
+                    int32_t *entity_id, int32_t *prim_id, float *t, float *u, float *v);
+
+int  agx_get_stats(agx_ctx *ctx, agx_stats_t *out);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char *agx_last_error(void);
+
+void agx_destroy(agx_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATRC_GPU_H */

@@ -1,0 +1,38 @@
+/*
+ * atrc_gpu_debug.h — host-only inspection hooks of libatrc_gpu.so, used by the
+ * CPU test-suite to check the scene build without a GPU.  They have no
+ * counterpart in Atrc; they expose what agx_upload_scene builds from the same
+ * descriptor (the BVHs of src/tracer/src/core/geometry/triangle_bvh.cpp:122-304
+ * and src/tracer/src/core/aggregate/entity_bvh.cpp:46-116).
+ */
+#ifndef ATRC_GPU_DEBUG_H
+#define ATRC_GPU_DEBUG_H
+
+#include "atrc_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct agx_debug_summary {
+    uint32_t entities, lights;
+    uint32_t blas_nodes, blas_prims, tlas_nodes;
+    uint32_t max_blas_depth, max_tlas_depth;
+    uint64_t device_bytes;
+    float world_centre_x, world_centre_y, world_centre_z, world_radius;
+} agx_debug_summary;
+
+/* Atrc triangle BVH of one mesh.  nodes_out (may be NULL): node_count * 8
+ * floats, per node (low.xyz, start bits, high.xyz, end-or-right bits);
+ * prims_out (may be NULL): n * 12 floats (a.xyz, original id bits, b-a, 0, c-a, 0). */
+int agx_debug_triangle_bvh(const agx_vertex *tris, uint32_t n, uint32_t *node_count, uint32_t *max_depth,
+                           float *nodes_out, float *prims_out);
+
+/* Build (without uploading) the device scene and report its sizes. */
+int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATRC_GPU_DEBUG_H */

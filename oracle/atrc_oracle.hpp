@@ -1,0 +1,1652 @@
+// ORACLE — test infrastructure only.
+//
+// CPU restatement of Atrc's `pt` renderer (trace_std + PerPixelRenderer) and of
+// the scene queries beneath it, templated on the real type: float is the FP32
+// restatement that the GPU path is checked against ray for ray and path for
+// path; double is the FP64 high-spp reference.  Only tests, the smoke check and
+// the CPU-baseline leg of the benchmark use it, as the checker; the product
+// (atrc_amd/) never links it.
+//
+// Every function cites the Atrc file:line it follows (paths relative to
+// src/tracer/ unless they start with src/).  Where Atrc calls its agz-utils
+// submodule, which is not part of the available source, the function states
+// the published method it uses instead; those are pinned by the tests in
+// tests/test_oracle_*.py against closed forms, not against Atrc.
+//
+// Deviations from Atrc, each deliberate:
+//  * RNG: one PCG32 stream per (pixel, sample) instead of one per worker thread
+//    (src/core/renderer/perpixel_renderer.cpp:77-80), so a render is
+//    reproducible and the GPU can replay the same random numbers.
+//  * uniform float = (pcg32() >> 8) * 2^-24 instead of libstdc++'s
+//    uniform_real_distribution<float>.
+//  * Film sums use Neumaier-compensated accumulation.
+//  * Compiled with -ffp-contract=off so the FP32 arithmetic is the one written.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <queue>
+#include <stack>
+#include <stdexcept>
+#include <thread>
+#include <vector>
+
+#include "../include/atrc_gpu.h"
+
+namespace oracle {
+
+// ------------------------------------------------------------------------------------
+// vector math (agz-utils tvec3 / float3 semantics: component-wise, dot = (xx+yy)+zz)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct V3 {
+    T x = 0, y = 0, z = 0;
+    V3() = default;
+    V3(T a, T b, T c) : x(a), y(b), z(c) {}
+    explicit V3(T a) : x(a), y(a), z(a) {}
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    T &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    V3 operator+(const V3 &o) const { return {x + o.x, y + o.y, z + o.z}; }
+    V3 operator-(const V3 &o) const { return {x - o.x, y - o.y, z - o.z}; }
+    V3 operator*(const V3 &o) const { return {x * o.x, y * o.y, z * o.z}; }
+    V3 operator/(const V3 &o) const { return {x / o.x, y / o.y, z / o.z}; }
+    V3 operator*(T s) const { return {x * s, y * s, z * s}; }
+    V3 operator/(T s) const { return {x / s, y / s, z / s}; }
+    V3 operator-() const { return {-x, -y, -z}; }
+    V3 &operator+=(const V3 &o) { x += o.x; y += o.y; z += o.z; return *this; }
+    V3 &operator*=(const V3 &o) { x *= o.x; y *= o.y; z *= o.z; return *this; }
+    V3 &operator*=(T s) { x *= s; y *= s; z *= s; return *this; }
+    V3 &operator/=(T s) { x /= s; y /= s; z /= s; return *this; }
+    bool is_black() const { return x == 0 && y == 0 && z == 0; }
+    bool is_finite() const { return std::isfinite(x) && std::isfinite(y) && std::isfinite(z); }
+    T length_square() const { return x * x + y * y + z * z; }
+    T length() const { return std::sqrt(length_square()); }
+    V3 normalize() const { return *this / length(); }
+    T max_elem() const { return std::max(x, std::max(y, z)); }
+};
+template<class T> inline V3<T> operator*(T s, const V3<T> &v) { return v * s; }
+template<class T> inline T dot(const V3<T> &a, const V3<T> &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template<class T> inline V3<T> cross(const V3<T> &a, const V3<T> &b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template<class T> inline T cos_between(const V3<T> &a, const V3<T> &b) { return dot(a, b) / (a.length() * b.length()); }
+template<class T> inline V3<T> vsqrt(const V3<T> &a) { return {std::sqrt(a.x), std::sqrt(a.y), std::sqrt(a.z)}; }
+
+template<class T> struct V2 {
+    T x = 0, y = 0;
+    V2() = default;
+    V2(T a, T b) : x(a), y(b) {}
+    V2 operator+(const V2 &o) const { return {x + o.x, y + o.y}; }
+    V2 operator-(const V2 &o) const { return {x - o.x, y - o.y}; }
+    V2 operator*(T s) const { return {x * s, y * s}; }
+};
+template<class T> inline V2<T> operator*(T s, const V2<T> &v) { return v * s; }
+
+template<class T> constexpr T PI = T(3.14159265358979323846);
+
+// Orthonormal frame (agz-utils tcoord3; unpinned: methods named per function).
+template<class T> struct Coord {
+    V3<T> x, y, z;
+    Coord() : x(1, 0, 0), y(0, 1, 0), z(0, 0, 1) {}
+    Coord(const V3<T> &a, const V3<T> &b, const V3<T> &c) : x(a), y(b), z(c) {}
+    // Duff, Burgess, Christensen et al., "Building an Orthonormal Basis, Revisited", JCGT 2017.
+    static Coord from_z(const V3<T> &zin) {
+        const V3<T> n = zin.normalize();
+        const T sign = std::copysign(T(1), n.z);
+        const T a = T(-1) / (sign + n.z);
+        const T b = n.x * n.y * a;
+        const V3<T> ex(T(1) + sign * n.x * n.x * a, sign * b, -sign * n.x);
+        const V3<T> ey(b, sign + n.y * n.y * a, -n.y);
+        return Coord(ex, ey, n);
+    }
+    V3<T> local_to_global(const V3<T> &v) const { return x * v.x + y * v.y + z * v.z; }
+    V3<T> global_to_local(const V3<T> &v) const { return {dot(v, x), dot(v, y), dot(v, z)}; }
+    bool in_positive_z_hemisphere(const V3<T> &v) const { return dot(v, z) > 0; }
+    // Gram-Schmidt of the old x against the new z; y completes a right-handed frame.
+    Coord rotate_to_new_z(const V3<T> &new_z) const {
+        const V3<T> nz = new_z.normalize();
+        const V3<T> xp = x - nz * dot(x, nz);
+        const T l = xp.length();
+        if (!(l > T(1e-6))) return from_z(nz);
+        const V3<T> nx = xp / l;
+        return Coord(nx, cross(nz, nx), nz);
+    }
+};
+
+// Row-major 3x4 affine transform.
+template<class T> struct Affine {
+    T m[12];
+    static Affine from(const float *f) { Affine a; for (int i = 0; i < 12; ++i) a.m[i] = T(f[i]); return a; }
+    V3<T> point(const V3<T> &p) const {
+        return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3],
+                m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+                m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
+    }
+    V3<T> vec(const V3<T> &v) const {
+        return {m[0] * v.x + m[1] * v.y + m[2] * v.z,
+                m[4] * v.x + m[5] * v.y + m[6] * v.z,
+                m[8] * v.x + m[9] * v.y + m[10] * v.z};
+    }
+    // transpose of the linear part applied to v (normals use the inverse's transpose)
+    V3<T> tvec(const V3<T> &v) const {
+        return {m[0] * v.x + m[4] * v.y + m[8] * v.z,
+                m[1] * v.x + m[5] * v.y + m[9] * v.z,
+                m[2] * v.x + m[6] * v.y + m[10] * v.z};
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// RNG: PCG32 (O'Neill 2014, XSH-RR 64/32), one stream per (pixel, sample)
+// ------------------------------------------------------------------------------------
+
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    static constexpr uint64_t MULT = 6364136223846793005ull;
+    uint32_t next() {
+        const uint64_t old = state;
+        state = old * MULT + inc;
+        const uint32_t xorshifted = uint32_t(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = uint32_t(old >> 59u);
+        return (xorshifted >> rot) | (xorshifted << ((-rot) & 31));
+    }
+    // pcg32_srandom_r(initstate, initseq)
+    void seed(uint64_t initstate, uint64_t initseq) {
+        state = 0;
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+    // stream of sample `sample` of pixel `pixel` (pixel = py * width + px)
+    void seed_path(uint64_t seed, uint64_t pixel, uint64_t sample) {
+        const uint64_t seq = (pixel << 24) ^ sample;
+        this->seed(splitmix64(seed ^ splitmix64(seq)), seq);
+    }
+    float uniform() { return float(next() >> 8) * 0x1p-24f; }
+};
+
+template<class T> struct Sampler {
+    Pcg32 rng;
+    T u1() { return T(rng.uniform()); }
+};
+
+// ------------------------------------------------------------------------------------
+// sampling distributions (agz-utils math::distribution — unpinned; methods named)
+// ------------------------------------------------------------------------------------
+
+// Shirley & Chiu 1997 concentric square-to-disk map.
+template<class T> inline V2<T> uniform_on_unit_disk(T u, T v) {
+    const T sx = 2 * u - 1, sy = 2 * v - 1;
+    if (sx == 0 && sy == 0) return {0, 0};
+    T r, theta;
+    if (std::abs(sx) > std::abs(sy)) { r = sx; theta = (PI<T> / 4) * (sy / sx); }
+    else { r = sy; theta = PI<T> / 2 - (PI<T> / 4) * (sx / sy); }
+    return {r * std::cos(theta), r * std::sin(theta)};
+}
+// Malley's method: cosine-weighted hemisphere; pdf = cos/pi.
+template<class T> inline void zweighted_on_hemisphere(T u, T v, V3<T> *dir, T *pdf) {
+    const V2<T> d = uniform_on_unit_disk(u, v);
+    const T z = std::sqrt(std::max(T(0), 1 - d.x * d.x - d.y * d.y));
+    *dir = V3<T>(d.x, d.y, z);
+    *pdf = z / PI<T>;
+}
+template<class T> inline T zweighted_on_hemisphere_pdf(T cos_theta) { return cos_theta > 0 ? cos_theta / PI<T> : 0; }
+// z = 1 - 2u, phi = 2 pi v; pdf = 1/(4 pi).
+template<class T> inline V3<T> uniform_on_sphere(T u, T v) {
+    const T z = 1 - 2 * u;
+    const T r = std::sqrt(std::max(T(0), 1 - z * z));
+    const T phi = 2 * PI<T> * v;
+    return {r * std::cos(phi), r * std::sin(phi), z};
+}
+template<class T> constexpr T uniform_on_sphere_pdf = T(1) / (4 * PI<T>);
+// pbrt UniformSampleCone; pdf = 1 / (2 pi (1 - cos_max)).
+template<class T> inline V3<T> uniform_on_cone(T cos_max, T u, T v) {
+    const T cos_t = (1 - u) + u * cos_max;
+    const T sin_t = std::sqrt(std::max(T(0), 1 - cos_t * cos_t));
+    const T phi = 2 * PI<T> * v;
+    return {std::cos(phi) * sin_t, std::sin(phi) * sin_t, cos_t};
+}
+template<class T> inline T uniform_on_cone_pdf(T cos_max) { return 1 / (2 * PI<T> * (1 - cos_max)); }
+// Barycentric (b1, b2) of a uniform point: (1 - sqrt(u), v sqrt(u)).
+template<class T> inline V2<T> uniform_on_triangle(T u, T v) {
+    const T su = std::sqrt(u);
+    return {1 - su, v * su};
+}
+
+// Walker/Vose alias table, single-uniform lookup.
+template<class T> struct AliasTable {
+    std::vector<T> prob;
+    std::vector<int32_t> alias;
+    void build(const std::vector<T> &w) {
+        const size_t n = w.size();
+        prob.assign(n, T(1));
+        alias.assign(n, 0);
+        double sum = 0;
+        for (T x : w) sum += double(x);
+        std::vector<double> scaled(n);
+        std::vector<int32_t> small, large;
+        for (size_t i = 0; i < n; ++i) {
+            scaled[i] = double(w[i]) * double(n) / sum;
+            alias[i] = int32_t(i);
+            (scaled[i] < 1.0 ? small : large).push_back(int32_t(i));
+        }
+        while (!small.empty() && !large.empty()) {
+            const int32_t s = small.back(); small.pop_back();
+            const int32_t l = large.back(); large.pop_back();
+            prob[s] = T(scaled[s]);
+            alias[s] = l;
+            scaled[l] = (scaled[l] + scaled[s]) - 1.0;
+            (scaled[l] < 1.0 ? small : large).push_back(l);
+        }
+        for (int32_t i : large) prob[i] = T(1);
+        for (int32_t i : small) prob[i] = T(1);
+    }
+    int32_t sample(T u) const {
+        const int32_t n = int32_t(prob.size());
+        const T nu = u * T(n);
+        int32_t idx = int32_t(nu);
+        if (idx >= n) idx = n - 1;
+        const T frac = nu - T(idx);
+        return frac < prob[idx] ? idx : alias[idx];
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// rays, AABBs, triangles  (common.h:256-395, utility/triangle_aux.h:9-88)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct Ray {
+    V3<T> o, d;
+    T t_min = 0, t_max = std::numeric_limits<T>::infinity();
+    Ray() = default;
+    Ray(const V3<T> &o_, const V3<T> &d_, T tmin = 0, T tmax = std::numeric_limits<T>::infinity())
+        : o(o_), d(d_), t_min(tmin), t_max(tmax) {}
+    V3<T> at(T t) const { return o + d * t; }
+    bool between(T t) const { return t_min <= t && t <= t_max; }  // common.h:320-323
+};
+
+template<class T> struct AABB {
+    V3<T> low{std::numeric_limits<T>::max()}, high{std::numeric_limits<T>::lowest()};
+    void grow(const V3<T> &p) {
+        low.x = std::min(low.x, p.x); low.y = std::min(low.y, p.y); low.z = std::min(low.z, p.z);
+        high.x = std::max(high.x, p.x); high.y = std::max(high.y, p.y); high.z = std::max(high.z, p.z);
+    }
+    void grow(const AABB &b) {
+        low.x = std::min(low.x, b.low.x); low.y = std::min(low.y, b.low.y); low.z = std::min(low.z, b.low.z);
+        high.x = std::max(high.x, b.high.x); high.y = std::max(high.y, b.high.y); high.z = std::max(high.z, b.high.z);
+    }
+    // AABB::intersect (common.h:384-395), written with std::min/std::max operand order
+    // of triangle_bvh.cpp:87-93 (the SIMD elem_min order of agz-utils is unpinned).
+    bool intersect(const V3<T> &o, const V3<T> &inv, T t_min, T t_max) const {
+        T tn = t_min, tf = t_max;
+        for (int a = 0; a < 3; ++a) {
+            const T n = inv[a] * (low[a] - o[a]);
+            const T f = inv[a] * (high[a] - o[a]);
+            tn = std::max(tn, std::min(n, f));
+            tf = std::min(tf, std::max(n, f));
+        }
+        return tn <= tf;
+    }
+};
+
+struct Counters {
+    uint64_t closest = 0, shadow = 0, node_visits = 0, tri_tests = 0, hits = 0;
+};
+
+// has_intersection_with_triangle (triangle_aux.h:9-30)
+template<class T>
+inline bool mt_any(const Ray<T> &r, const V3<T> &A, const V3<T> &B_A, const V3<T> &C_A) {
+    const V3<T> s1 = cross(r.d, C_A);
+    const T div = dot(s1, B_A);
+    if (!div) return false;
+    const T inv_div = 1 / div;
+    const V3<T> o_A = r.o - A;
+    const T alpha = dot(o_A, s1) * inv_div;
+    if (alpha < 0 || alpha > 1) return false;
+    const V3<T> s2 = cross(o_A, B_A);
+    const T beta = dot(r.d, s2) * inv_div;
+    if (beta < 0 || alpha + beta > 1) return false;
+    const T t = dot(C_A, s2) * inv_div;
+    return r.between(t);
+}
+
+// closest_intersection_with_triangle (triangle_aux.h:38-69)
+template<class T>
+inline bool mt_closest(const Ray<T> &r, const V3<T> &A, const V3<T> &B_A, const V3<T> &C_A,
+                       T *t_out, T *u_out, T *v_out) {
+    const V3<T> s1 = cross(r.d, C_A);
+    const T div = dot(s1, B_A);
+    if (!div) return false;
+    const T inv_div = 1 / div;
+    const V3<T> o_A = r.o - A;
+    const T alpha = dot(o_A, s1) * inv_div;
+    if (alpha < 0) return false;
+    const V3<T> s2 = cross(o_A, B_A);
+    const T beta = dot(r.d, s2) * inv_div;
+    if (beta < 0 || alpha + beta > 1) return false;
+    const T t = dot(C_A, s2) * inv_div;
+    if (!r.between(t)) return false;
+    *t_out = t; *u_out = alpha; *v_out = beta;
+    return true;
+}
+
+template<class T> inline T triangle_area(const V3<T> &B_A, const V3<T> &C_A) { return cross(B_A, C_A).length() / 2; }
+
+// dpdu_as_ex (triangle_aux.h:76-88)
+template<class T>
+inline V3<T> dpdu_as_ex(const V3<T> &B_A, const V3<T> &C_A, const V2<T> &b_a, const V2<T> &c_a, const V3<T> &nor) {
+    const T m00 = b_a.x, m01 = b_a.y, m10 = c_a.x, m11 = c_a.y;
+    const T det = m00 * m11 - m01 * m10;
+    if (!det) return Coord<T>::from_z(nor).x;
+    const T inv_det = 1 / det;
+    return (B_A * (m11 * inv_det) - C_A * (m01 * inv_det)).normalize();
+}
+
+// ------------------------------------------------------------------------------------
+// intersections  (core/intersection.h:10-93)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct SurfacePoint {
+    V3<T> pos;
+    V2<T> uv;
+    Coord<T> geometry_coord, user_coord;
+    V3<T> eps_offset(const V3<T> &dir, T eps) const {  // intersection.h:17-22
+        if (dot(dir, geometry_coord.z) > 0) return pos + geometry_coord.z * eps;
+        return pos - geometry_coord.z * eps;
+    }
+};
+
+template<class T> struct GeometryHit : SurfacePoint<T> {
+    T t = -1;
+    V3<T> wr;
+    int32_t prim = 0;
+    T bu = 0, bv = 0;  // barycentrics of the hit triangle (parity output)
+};
+
+template<class T> struct EntityHit : GeometryHit<T> {
+    int32_t entity = -1;
+};
+
+// ------------------------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------------------------
+
+// Atrc's default triangle BVH (core/geometry/triangle_bvh.cpp:24-488).
+template<class T> struct TriangleBVH {
+    struct Node { T low[3], high[3]; uint32_t start, end_or_right; bool is_leaf() const { return start != 0xffffffffu; } };
+    struct Prim { V3<T> a, b_a, c_a; };
+    struct PrimInfo { V3<T> n_a, n_b_a, n_c_a; V2<T> t_a, t_b_a, t_c_a; V3<T> x, z; };
+    std::vector<Node> nodes;
+    std::vector<Prim> prims;
+    std::vector<PrimInfo> infos;
+    std::vector<uint32_t> orig_id;  // original triangle index of BVH slot i
+    AliasTable<T> area_table;
+    T surface_area = 0;
+    AABB<T> world_bound;
+    int max_depth = 0;
+
+    // Node::has_intersection (triangle_bvh.cpp:75-97)
+    static bool node_hit(const Node &nd, const T *ori, const T *inv, T t_min, T t_max, T *t_enter) {
+        const T nx = inv[0] * (nd.low[0] - ori[0]);
+        const T ny = inv[1] * (nd.low[1] - ori[1]);
+        const T nz = inv[2] * (nd.low[2] - ori[2]);
+        const T fx = inv[0] * (nd.high[0] - ori[0]);
+        const T fy = inv[1] * (nd.high[1] - ori[1]);
+        const T fz = inv[2] * (nd.high[2] - ori[2]);
+        t_min = std::max(t_min, std::min(nx, fx));
+        t_min = std::max(t_min, std::min(ny, fy));
+        t_min = std::max(t_min, std::min(nz, fz));
+        t_max = std::min(t_max, std::max(nx, fx));
+        t_max = std::min(t_max, std::max(ny, fy));
+        t_max = std::min(t_max, std::max(nz, fz));
+        *t_enter = t_min;
+        return t_min <= t_max;
+    }
+
+    void build(const agx_vertex *v, uint32_t n) {
+        struct BTri { uint32_t id; V3<T> p[3]; V3<T> nor[3]; V2<T> uv[3]; V3<T> centroid; };
+        std::vector<BTri> tris(n);
+        surface_area = 0;
+        world_bound = AABB<T>();
+        for (uint32_t i = 0; i < n; ++i) {
+            BTri &b = tris[i];
+            b.id = i;
+            for (int k = 0; k < 3; ++k) {
+                const agx_vertex &vv = v[3 * i + k];
+                b.p[k] = V3<T>(T(vv.pos[0]), T(vv.pos[1]), T(vv.pos[2]));
+                b.nor[k] = V3<T>(T(vv.nor[0]), T(vv.nor[1]), T(vv.nor[2]));
+                b.uv[k] = V2<T>(T(vv.uv[0]), T(vv.uv[1]));
+            }
+            b.centroid = (b.p[0] + b.p[1] + b.p[2]) / T(3);            // :331-334
+            surface_area += triangle_area(b.p[1] - b.p[0], b.p[2] - b.p[0]);
+            world_bound.grow(b.p[0]); world_bound.grow(b.p[1]); world_bound.grow(b.p[2]);
+        }
+        // build_bvh (triangle_bvh.cpp:122-223): breadth-first, midpoint split on the
+        // largest centroid extent while depth < 64, then sort + median.
+        struct BNode { AABB<T> bound; int left = -1, right = -1; uint32_t start = 0, end = 0; int depth = 0; };
+        std::vector<BNode> bn;
+        struct Task { int *fill; int fill_node; bool fill_left; uint32_t start, end, depth; };
+        std::queue<Task> tasks;
+        int root = -1;
+        tasks.push({&root, -1, false, 0, n, 0});
+        const uint32_t leaf_threshold = 5, depth_threshold = 64;
+        max_depth = 0;
+        while (!tasks.empty()) {
+            const Task task = tasks.front(); tasks.pop();
+            AABB<T> all, cb;
+            for (uint32_t i = task.start; i < task.end; ++i) {
+                all.grow(tris[i].p[0]); all.grow(tris[i].p[1]); all.grow(tris[i].p[2]);
+                cb.grow(tris[i].centroid);
+            }
+            const int idx = int(bn.size());
+            bn.push_back(BNode());
+            bn[idx].bound = all;
+            bn[idx].depth = int(task.depth);
+            max_depth = std::max(max_depth, int(task.depth));
+            if (task.fill_node < 0) root = idx;
+            else if (task.fill_left) bn[task.fill_node].left = idx;
+            else bn[task.fill_node].right = idx;
+            const uint32_t cnt = task.end - task.start;
+            if (cnt <= leaf_threshold) {
+                bn[idx].start = task.start; bn[idx].end = task.end;
+                continue;
+            }
+            const V3<T> cd = cb.high - cb.low;
+            const int axis = cd[0] > cd[1] ? (cd[0] > cd[2] ? 0 : 2) : (cd[1] > cd[2] ? 1 : 2);
+            uint32_t mid;
+            if (task.depth < depth_threshold) {
+                const T split = T(0.5) * (cb.high[axis] + cb.low[axis]);
+                mid = task.start;
+                for (uint32_t i = task.start; i < task.end; ++i)
+                    if (tris[i].centroid[axis] < split) std::swap(tris[i], tris[mid++]);
+                if (mid == task.start || mid == task.end) mid = task.start + cnt / 2;
+            } else {
+                std::sort(tris.begin() + task.start, tris.begin() + task.end,
+                          [axis](const BTri &L, const BTri &R) { return L.centroid[axis] < R.centroid[axis]; });
+                mid = task.start + cnt / 2;
+            }
+            tasks.push({nullptr, idx, true, task.start, mid, task.depth + 1});
+            tasks.push({nullptr, idx, false, mid, task.end, task.depth + 1});
+        }
+        // compact_bvh (triangle_bvh.cpp:225-304): depth-first, left child at idx+1.
+        nodes.clear(); prims.clear(); infos.clear(); orig_id.clear();
+        nodes.reserve(bn.size()); prims.resize(n); infos.resize(n); orig_id.resize(n);
+        struct CT { int b; int64_t fill; };
+        std::stack<CT> st;
+        st.push({root, -1});
+        uint32_t next_prim = 0;
+        while (!st.empty()) {
+            const CT ct = st.top(); st.pop();
+            const BNode &b = bn[ct.b];
+            const uint32_t my = uint32_t(nodes.size());
+            if (ct.fill >= 0) nodes[size_t(ct.fill)].end_or_right = my;
+            Node nd;
+            for (int k = 0; k < 3; ++k) { nd.low[k] = b.bound.low[k]; nd.high[k] = b.bound.high[k]; }
+            if (b.left >= 0 && b.right >= 0) {
+                nd.start = 0xffffffffu; nd.end_or_right = 0;
+                nodes.push_back(nd);
+                st.push({b.right, int64_t(my)});
+                st.push({b.left, -1});
+            } else {
+                const uint32_t s = next_prim, e = next_prim + (b.end - b.start);
+                nd.start = s; nd.end_or_right = e;
+                nodes.push_back(nd);
+                for (uint32_t i = s, j = b.start; i < e; ++i, ++j) {
+                    const BTri &tri = tris[j];
+                    Prim &pr = prims[i];
+                    PrimInfo &pi = infos[i];
+                    orig_id[i] = tri.id;
+                    pr.a = tri.p[0]; pr.b_a = tri.p[1] - tri.p[0]; pr.c_a = tri.p[2] - tri.p[0];
+                    const V3<T> na = tri.nor[0].normalize(), nb = tri.nor[1].normalize(), nc = tri.nor[2].normalize();
+                    pi.n_a = na; pi.n_b_a = nb - na; pi.n_c_a = nc - na;
+                    pi.t_a = tri.uv[0]; pi.t_b_a = tri.uv[1] - tri.uv[0]; pi.t_c_a = tri.uv[2] - tri.uv[0];
+                    pi.z = cross(pr.b_a, pr.c_a).normalize();
+                    if (dot(na + nb + nc, pi.z) < 0) pi.z = -pi.z;
+                    pi.x = dpdu_as_ex(pr.b_a, pr.c_a, pi.t_b_a, pi.t_c_a, pi.z);
+                }
+                next_prim = e;
+            }
+        }
+        std::vector<T> areas(n);
+        for (uint32_t i = 0; i < n; ++i) areas[i] = triangle_area(prims[i].b_a, prims[i].c_a);
+        area_table.build(areas);
+        // TriangleBVH::TriangleBVH world bound with flat-axis padding (:570-583)
+        AABB<T> wb;
+        for (auto &p : prims) { wb.grow(p.a); wb.grow(p.a + p.b_a); wb.grow(p.a + p.c_a); }
+        for (int i = 0; i < 3; ++i)
+            if (wb.low[i] >= wb.high[i]) wb.low[i] = wb.high[i] - T(0.1) * std::abs(wb.high[i]);
+        world_bound = wb;
+    }
+
+    // UntransformedTriangleBVH::has_intersection (:363-401)
+    bool any_hit(const Ray<T> &r, Counters *c) const {
+        const T inv[3] = {1 / r.d.x, 1 / r.d.y, 1 / r.d.z};
+        const T ori[3] = {r.o.x, r.o.y, r.o.z};
+        T t;
+        if (c) c->node_visits++;
+        if (!node_hit(nodes[0], ori, inv, r.t_min, r.t_max, &t)) return false;
+        uint32_t stack[256];
+        int top = 0;
+        stack[top++] = 0;
+        while (top) {
+            const uint32_t ni = stack[--top];
+            const Node &nd = nodes[ni];
+            if (nd.is_leaf()) {
+                for (uint32_t i = nd.start; i < nd.end_or_right; ++i) {
+                    if (c) c->tri_tests++;
+                    if (mt_any(r, prims[i].a, prims[i].b_a, prims[i].c_a)) return true;
+                }
+            } else {
+                if (c) c->node_visits += 2;
+                if (node_hit(nodes[ni + 1], ori, inv, r.t_min, r.t_max, &t)) stack[top++] = ni + 1;
+                if (node_hit(nodes[nd.end_or_right], ori, inv, r.t_min, r.t_max, &t)) stack[top++] = nd.end_or_right;
+            }
+        }
+        return false;
+    }
+
+    // UntransformedTriangleBVH::closest_intersection (:403-488); fills slot index,
+    // t and barycentrics; surface() reconstructs the rest.
+    bool closest(Ray<T> r, T *t_out, T *u_out, T *v_out, uint32_t *slot_out, Counters *c) const {
+        const T ori[3] = {r.o.x, r.o.y, r.o.z};
+        const T inv[3] = {1 / r.d.x, 1 / r.d.y, 1 / r.d.z};
+        T tmp;
+        if (c) c->node_visits++;
+        if (!node_hit(nodes[0], ori, inv, r.t_min, r.t_max, &tmp)) return false;
+        uint32_t stack[256];
+        int top = 0;
+        stack[top++] = 0;
+        T best_t = std::numeric_limits<T>::infinity(), bu = 0, bv = 0;
+        uint32_t best = 0;
+        while (top) {
+            const uint32_t ni = stack[--top];
+            const Node &nd = nodes[ni];
+            if (nd.is_leaf()) {
+                for (uint32_t i = nd.start; i < nd.end_or_right; ++i) {
+                    if (c) c->tri_tests++;
+                    T tt, uu, vv;
+                    if (mt_closest(r, prims[i].a, prims[i].b_a, prims[i].c_a, &tt, &uu, &vv)) {
+                        best_t = tt; bu = uu; bv = vv; best = i;
+                        r.t_max = tt;
+                    }
+                }
+            } else {
+                if (c) c->node_visits += 2;
+                T tl, tr;
+                const bool al = node_hit(nodes[ni + 1], ori, inv, r.t_min, r.t_max, &tl);
+                const bool ar = node_hit(nodes[nd.end_or_right], ori, inv, r.t_min, r.t_max, &tr);
+                if (al && ar) {
+                    if (tl < tr) { stack[top++] = nd.end_or_right; stack[top++] = ni + 1; }
+                    else { stack[top++] = ni + 1; stack[top++] = nd.end_or_right; }
+                } else if (al) stack[top++] = ni + 1;
+                else if (ar) stack[top++] = nd.end_or_right;
+            }
+        }
+        if (std::isinf(best_t)) return false;
+        *t_out = best_t; *u_out = bu; *v_out = bv; *slot_out = best;
+        return true;
+    }
+
+    void surface(uint32_t slot, T bu, T bv, SurfacePoint<T> *sp) const {  // :472-485
+        const PrimInfo &pi = infos[slot];
+        sp->geometry_coord = Coord<T>(pi.x, cross(pi.z, pi.x), pi.z);
+        sp->uv = pi.t_a + pi.t_b_a * bu + pi.t_c_a * bv;
+        const V3<T> user_z = pi.n_a + pi.n_b_a * bu + pi.n_c_a * bv;
+        sp->user_coord = sp->geometry_coord.rotate_to_new_z(user_z);
+    }
+
+    // UntransformedTriangleBVH::sample (:495-518)
+    SurfacePoint<T> sample(T *pdf, T su, T sv, T sw) const {
+        const int32_t idx = area_table.sample(su);
+        const Prim &pr = prims[idx];
+        const V2<T> uv = uniform_on_triangle(sv, sw);
+        SurfacePoint<T> sp;
+        sp.pos = pr.a + pr.b_a * uv.x + pr.c_a * uv.y;
+        surface(uint32_t(idx), uv.x, uv.y, &sp);
+        *pdf = 1 / surface_area;
+        return sp;
+    }
+};
+
+template<class T> struct Geometry {
+    int kind = AGX_GEOM_QUAD;
+    // quad (core/geometry/quad.cpp): world-space a, edges; triangle (triangle.cpp): local
+    V3<T> a, b_a, c_a, d_a;
+    V2<T> t_a, t_b_a, t_c_a, t_d_a;
+    V3<T> x_abc, x_acd, z;
+    T area = 1, sample_abc_prob = 0;
+    // transformed geometry (transformed_geometry.h:31-82)
+    Affine<T> l2w, w2l;
+    T ratio = 1;
+    T radius = 1, radius2 = 1, world_radius = 1;
+    std::unique_ptr<TriangleBVH<T>> mesh;
+    AABB<T> bound;
+
+    Ray<T> to_local(const Ray<T> &r) const { return Ray<T>(w2l.point(r.o), w2l.vec(r.d), r.t_min, r.t_max); }
+    Coord<T> coord_to_world(const Coord<T> &c) const {
+        const V3<T> nz = w2l.tvec(c.z).normalize();
+        const V3<T> mx = l2w.vec(c.x);
+        const V3<T> xp = mx - nz * dot(mx, nz);
+        const T l = xp.length();
+        if (!(l > T(1e-6))) return Coord<T>::from_z(nz);
+        const V3<T> nx = xp / l;
+        return Coord<T>(nx, cross(nz, nx), nz);
+    }
+    void sp_to_world(SurfacePoint<T> *sp) const {
+        sp->pos = l2w.point(sp->pos);
+        sp->geometry_coord = coord_to_world(sp->geometry_coord);
+        sp->user_coord = coord_to_world(sp->user_coord);
+    }
+
+    void init(const agx_entity_desc &e) {
+        kind = e.geometry;
+        auto P = [&](int i) { return V3<T>(T(e.p[i][0]), T(e.p[i][1]), T(e.p[i][2])); };
+        auto TT = [&](int i) { return V2<T>(T(e.t[i][0]), T(e.t[i][1])); };
+        l2w = Affine<T>::from(e.local_to_world);
+        w2l = Affine<T>::from(e.world_to_local);
+        ratio = l2w.vec(V3<T>(1, 0, 0)).length();
+        bound = AABB<T>();
+        if (kind == AGX_GEOM_QUAD) {                            // quad.cpp:138-164, 60-73
+            a = P(0); b_a = P(1) - a; c_a = P(2) - a; d_a = P(3) - a;
+            t_a = TT(0); t_b_a = TT(1) - t_a; t_c_a = TT(2) - t_a; t_d_a = TT(3) - t_a;
+            z = cross(b_a, c_a).normalize();
+            x_abc = dpdu_as_ex(b_a, c_a, t_b_a, t_c_a, z);
+            x_acd = dpdu_as_ex(c_a, d_a, t_c_a, t_d_a, z);
+            const T a1 = triangle_area(b_a, c_a), a2 = triangle_area(c_a, d_a);
+            area = a1 + a2;
+            sample_abc_prob = a1 / area;
+            bound.grow(a); bound.grow(a + b_a); bound.grow(a + c_a); bound.grow(a + d_a);
+            pad_flat();
+        } else if (kind == AGX_GEOM_TRIANGLE) {                 // triangle.cpp:107-125, 53-65
+            a = P(0); b_a = P(1) - a; c_a = P(2) - a;
+            t_a = TT(0); t_b_a = TT(1) - t_a; t_c_a = TT(2) - t_a;
+            z = cross(b_a, c_a).normalize();
+            x_abc = dpdu_as_ex(b_a, c_a, t_b_a, t_c_a, z);
+            area = triangle_area(l2w.vec(b_a), l2w.vec(c_a));
+            bound.grow(l2w.point(a)); bound.grow(l2w.point(a + b_a)); bound.grow(l2w.point(a + c_a));
+            pad_flat();
+        } else if (kind == AGX_GEOM_SPHERE) {                   // sphere.cpp:14-24, 60-67
+            radius = T(e.radius);
+            if (!(radius > 0)) throw std::runtime_error("invalid sphere radius");
+            world_radius = ratio * radius;
+            area = 4 * PI<T> * world_radius * world_radius;
+            const V3<T> c = l2w.point(V3<T>(0));
+            bound.low = c - V3<T>(world_radius + T(3e-4));
+            bound.high = c + V3<T>(world_radius + T(3e-4));
+        } else if (kind == AGX_GEOM_DISK) {                     // disk.cpp:33-47, 85-93
+            radius = T(e.radius); radius2 = radius * radius;
+            if (!(radius > 0)) throw std::runtime_error("invalid disk radius");
+            area = ratio * ratio * PI<T> * radius2;
+            const T vs = std::min(T(0.1) * radius, T(0.01));
+            const V3<T> lo(-radius, -radius, -vs), hi(radius, radius, vs);
+            for (int i = 0; i < 8; ++i)
+                bound.grow(l2w.point(V3<T>(i & 1 ? hi.x : lo.x, i & 2 ? hi.y : lo.y, i & 4 ? hi.z : lo.z)));
+        } else if (kind == AGX_GEOM_TRIANGLE_BVH) {
+            if (!e.tris || !e.tri_count) throw std::runtime_error("empty triangle mesh");
+            mesh.reset(new TriangleBVH<T>());
+            mesh->build(e.tris, e.tri_count);
+            area = mesh->surface_area;
+            bound = mesh->world_bound;
+        } else {
+            throw std::runtime_error("unknown geometry kind");
+        }
+    }
+    void pad_flat() {
+        for (int i = 0; i < 3; ++i)
+            if (bound.low[i] >= bound.high[i]) bound.low[i] = bound.high[i] - T(0.1) * std::abs(bound.high[i]);
+    }
+
+    bool any_hit(const Ray<T> &r, Counters *c) const {
+        switch (kind) {
+        case AGX_GEOM_QUAD:                                      // quad.cpp:24-28
+            if (c) c->tri_tests += 1;
+            if (mt_any(r, a, b_a, c_a)) return true;
+            if (c) c->tri_tests += 1;
+            return mt_any(r, a, c_a, d_a);
+        case AGX_GEOM_TRIANGLE: {
+            if (c) c->tri_tests += 1;
+            return mt_any(to_local(r), a, b_a, c_a);
+        }
+        case AGX_GEOM_SPHERE: return sphere_any(to_local(r));
+        case AGX_GEOM_DISK: {                                    // disk.cpp:49-61
+            const Ray<T> lr = to_local(r);
+            const T t = -lr.o.z / lr.d.z;
+            if (std::isinf(t) || !lr.between(t)) return false;
+            const V3<T> p = lr.at(t);
+            return p.x * p.x + p.y * p.y <= radius2;
+        }
+        default: return mesh->any_hit(r, c);
+        }
+    }
+
+    // sphere_aux.h:10-16, 50-64
+    void sphere_abc(const Ray<T> &r, T *A, T *B, T *C) const {
+        *A = r.d.length_square(); *B = 2 * dot(r.d, r.o); *C = r.o.length_square() - radius * radius;
+    }
+    bool sphere_any(const Ray<T> &r) const {
+        T A, B, C; sphere_abc(r, &A, &B, &C);
+        T delta = B * B - 4 * A * C;
+        if (delta < 0) return false;
+        delta = std::sqrt(delta);
+        const T inv2A = T(0.5) / A;
+        const T t0 = (-B - delta) * inv2A, t1 = (-B + delta) * inv2A;
+        return r.between(t0) || r.between(t1);
+    }
+    // sphere_aux.h:18-48
+    void sphere_local_uv_coord(const V3<T> &p, SurfacePoint<T> *sp, T eps) const {
+        T phi = (!p.x && !p.y) ? T(0) : std::atan2(p.y, p.x);
+        if (phi < 0) phi += 2 * PI<T>;
+        const T sin_theta = std::min(T(1), std::max(T(-1), p.z / radius));
+        const T theta = std::asin(sin_theta);
+        sp->uv = V2<T>(T(0.5) / PI<T> * phi, theta / PI<T> + T(0.5));
+        V3<T> ex, ey, ez = p.normalize();
+        if (std::abs(ez.z - 1) < eps) { ex = V3<T>(1, 0, 0); ey = V3<T>(0, 1, 0); ez = V3<T>(0, 0, 1); }
+        else if (std::abs(ez.z + 1) < eps) { ex = V3<T>(0, 1, 0); ey = V3<T>(1, 0, 0); ez = V3<T>(0, 0, -1); }
+        else { ex = cross(V3<T>(0, 0, 1), ez); ey = cross(ez, ex); }
+        sp->geometry_coord = Coord<T>(ex, ey, ez);
+        sp->user_coord = sp->geometry_coord;
+    }
+    // disk.cpp:11-28
+    void disk_local_sp(const V3<T> &p, T rad, SurfacePoint<T> *sp) const {
+        T phi = (!p.y && !p.x) ? T(0) : std::atan2(p.y, p.x);
+        if (phi < 0) phi += 2 * PI<T>;
+        sp->pos = p;
+        sp->uv = V2<T>(phi / (2 * PI<T>), rad / radius);
+        const V3<T> cx = V3<T>(p.x, p.y, 0).normalize();
+        const V3<T> cz(0, 0, 1);
+        sp->geometry_coord = Coord<T>(cx, cross(cz, cx), cz);
+        sp->user_coord = sp->geometry_coord;
+    }
+
+    bool closest(const Ray<T> &r, GeometryHit<T> *h, T eps, Counters *c) const {
+        switch (kind) {
+        case AGX_GEOM_QUAD: {                                   // quad.cpp:30-58
+            T t, u, v;
+            if (c) c->tri_tests += 1;
+            if (mt_closest(r, a, b_a, c_a, &t, &u, &v)) {
+                h->pos = r.at(t);
+                h->geometry_coord = Coord<T>(x_abc, cross(z, x_abc), z);
+                h->uv = t_a + t_b_a * u + t_c_a * v;
+                h->user_coord = h->geometry_coord;
+                h->wr = -r.d; h->t = t; h->prim = 0; h->bu = u; h->bv = v;
+                return true;
+            }
+            if (c) c->tri_tests += 1;
+            if (mt_closest(r, a, c_a, d_a, &t, &u, &v)) {
+                h->pos = r.at(t);
+                h->geometry_coord = Coord<T>(x_acd, cross(z, x_acd), z);
+                h->uv = t_a + t_c_a * u + t_d_a * v;
+                h->user_coord = h->geometry_coord;
+                h->wr = -r.d; h->t = t; h->prim = 1; h->bu = u; h->bv = v;
+                return true;
+            }
+            return false;
+        }
+        case AGX_GEOM_TRIANGLE: {                               // triangle.cpp:31-51
+            const Ray<T> lr = to_local(r);
+            T t, u, v;
+            if (c) c->tri_tests += 1;
+            if (!mt_closest(lr, a, b_a, c_a, &t, &u, &v)) return false;
+            h->pos = lr.at(t);
+            h->geometry_coord = Coord<T>(x_abc, cross(z, x_abc), z);
+            h->uv = t_a + t_b_a * u + t_c_a * v;
+            h->user_coord = h->geometry_coord;
+            h->t = t; h->prim = 0; h->bu = u; h->bv = v;
+            sp_to_world(h);
+            h->wr = l2w.vec(-lr.d);
+            return true;
+        }
+        case AGX_GEOM_SPHERE: {                                 // sphere.cpp:33-58
+            const Ray<T> lr = to_local(r);
+            T A, B, C; sphere_abc(lr, &A, &B, &C);
+            T delta = B * B - 4 * A * C;
+            if (delta < 0) return false;
+            delta = std::sqrt(delta);
+            const T inv2A = T(0.5) / A;
+            const T t0 = (-B - delta) * inv2A, t1 = (-B + delta) * inv2A;
+            if (lr.t_max < t0 || t1 < lr.t_min) return false;
+            const bool not_t0 = t0 < lr.t_min;
+            if (not_t0 && t1 > lr.t_max) return false;
+            const T t = not_t0 ? t1 : t0;
+            const V3<T> p = lr.at(t);
+            h->pos = p;
+            sphere_local_uv_coord(p, h, eps);
+            h->t = t; h->prim = 0; h->bu = 0; h->bv = 0;
+            sp_to_world(h);
+            h->wr = l2w.vec(-lr.d);
+            return true;
+        }
+        case AGX_GEOM_DISK: {                                   // disk.cpp:63-83
+            const Ray<T> lr = to_local(r);
+            const T tv = -lr.o.z / lr.d.z;
+            if (std::isinf(tv) || !lr.between(tv)) return false;
+            const V3<T> p = lr.at(tv);
+            const T rad = std::sqrt(p.x * p.x + p.y * p.y);
+            if (rad > radius) return false;
+            disk_local_sp(p, rad, h);
+            h->t = tv; h->prim = 0; h->bu = 0; h->bv = 0;
+            sp_to_world(h);
+            h->wr = l2w.vec(-lr.d);
+            return true;
+        }
+        default: {
+            T t, u, v; uint32_t slot;
+            if (!mesh->closest(r, &t, &u, &v, &slot, c)) return false;
+            h->pos = r.at(t);
+            mesh->surface(slot, u, v, h);
+            h->t = t; h->wr = -r.d; h->prim = int32_t(mesh->orig_id[slot]); h->bu = u; h->bv = v;
+            return true;
+        }
+        }
+    }
+
+    // Geometry::sample(ref, pdf, sam) per kind; pdf w.r.t. area
+    SurfacePoint<T> sample(const V3<T> &ref, T *pdf, T su, T sv, T sw, T eps) const {
+        SurfacePoint<T> sp;
+        switch (kind) {
+        case AGX_GEOM_QUAD: {                                   // quad.cpp:80-105
+            const V2<T> bi = uniform_on_triangle(su, sv);
+            if (sw < sample_abc_prob) {
+                sp.pos = a + b_a * bi.x + c_a * bi.y;
+                sp.geometry_coord = Coord<T>(x_abc, cross(z, x_abc), z);
+                sp.uv = t_a + t_b_a * bi.x + t_c_a * bi.y;
+            } else {
+                sp.pos = a + c_a * bi.x + d_a * bi.y;
+                sp.geometry_coord = Coord<T>(x_acd, cross(z, x_acd), z);
+                sp.uv = t_a + t_c_a * bi.x + t_d_a * bi.y;
+            }
+            sp.user_coord = sp.geometry_coord;
+            *pdf = 1 / area;
+            return sp;
+        }
+        case AGX_GEOM_TRIANGLE: {                               // triangle.cpp:72-87
+            const V2<T> bi = uniform_on_triangle(su, sv);
+            sp.pos = a + b_a * bi.x + c_a * bi.y;
+            sp.geometry_coord = Coord<T>(x_abc, cross(z, x_abc), z);
+            sp.uv = t_a + t_b_a * bi.x + t_c_a * bi.y;
+            sp.user_coord = sp.geometry_coord;
+            sp_to_world(&sp);
+            *pdf = 1 / area;
+            return sp;
+        }
+        case AGX_GEOM_SPHERE: {                                 // sphere.cpp:93-123
+            const V3<T> lref = w2l.point(ref);
+            const T d = lref.length();
+            if (d <= radius) {
+                const V3<T> up = uniform_on_sphere(su, sv);
+                *pdf = uniform_on_sphere_pdf<T> / (world_radius * world_radius);
+                const V3<T> p = up * radius;
+                sp.pos = p;
+                sphere_local_uv_coord(p, &sp, eps);
+                sp_to_world(&sp);
+                return sp;
+            }
+            const T cos_t = std::min(radius / d, T(1));
+            const V3<T> dir = uniform_on_cone(cos_t, su, sv);
+            const T lpdf = uniform_on_cone_pdf(cos_t);
+            const V3<T> p = Coord<T>::from_z(lref).local_to_global(dir).normalize() * radius;
+            sp.pos = p;
+            sphere_local_uv_coord(p, &sp, eps);
+            sp_to_world(&sp);
+            *pdf = lpdf / (world_radius * world_radius);
+            return sp;
+        }
+        case AGX_GEOM_DISK: {                                   // disk.cpp:100-110
+            const V2<T> dp = uniform_on_unit_disk(su, sv);
+            const V2<T> p2 = dp * radius;
+            *pdf = 1 / area;
+            disk_local_sp(V3<T>(p2.x, p2.y, 0), std::sqrt(p2.x * p2.x + p2.y * p2.y), &sp);
+            sp_to_world(&sp);
+            return sp;
+        }
+        default:
+            return mesh->sample(pdf, su, sv, sw);
+        }
+    }
+
+    // Geometry::pdf(ref, pos), area measure
+    T pdf(const V3<T> &ref) const {
+        if (kind == AGX_GEOM_SPHERE) {                          // sphere.cpp:131-143
+            const V3<T> lref = w2l.point(ref);
+            const T d = lref.length();
+            if (d <= radius) return 1 / area;
+            const T cos_t = std::min(radius / d, T(1));
+            return uniform_on_cone_pdf(cos_t) / (world_radius * world_radius);
+        }
+        return 1 / area;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// BSDFs  (core/bsdf.h:113-202, material/*)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct BSDFSample {
+    V3<T> dir; V3<T> f; T pdf = 0; bool is_delta = false;
+};
+
+template<class T> struct BSDF {
+    int kind = AGX_MAT_IDEAL_DIFFUSE;
+    Coord<T> geo, shd;
+    V3<T> albedo;         // AggregateBSDF albedo_ / mirror rc / glass mean colour
+    V3<T> coef;           // diffuse albedo/pi
+    // metal GGX
+    V3<T> color, eta2, k2;   // ConductorPoint eta_2_, eta_k_2_ (eta_out = 1)
+    T ax = 1, ay = 1;
+    // glass
+    V3<T> c_refl, c_refr; T ior = 1.5f;
+    // mirror
+    V3<T> m_eta2, m_k2;      // eta_in = 1, eta_out = ior
+
+    // ---- LocalBSDF black-fringe helpers (bsdf.h:122-187)
+    bool fringe(const V3<T> &w) const { return shd.in_positive_z_hemisphere(w) != geo.in_positive_z_hemisphere(w); }
+    V3<T> albedo_v() const { return albedo; }
+
+    // conductor Fresnel (fresnel_point.h:44-64)
+    static V3<T> conductor(T cos_i, const V3<T> &e2, const V3<T> &ek2) {
+        if (cos_i <= 0) return V3<T>(0);
+        const T cos2 = cos_i * cos_i;
+        const T sin2 = std::max(T(0), 1 - cos2);
+        V3<T> out;
+        for (int ch = 0; ch < 3; ++ch) {
+            const T t0 = e2[ch] - ek2[ch] - sin2;
+            const T a2b2 = std::sqrt(t0 * t0 + T(4) * e2[ch] * ek2[ch]);
+            const T t1 = a2b2 + cos2;
+            const T aa = std::sqrt(T(0.5) * (a2b2 + t0));
+            const T t2 = 2 * cos_i * aa;
+            const T rs = (t1 - t2) / (t1 + t2);
+            const T t3 = cos2 * a2b2 + sin2 * sin2;
+            const T t4 = t2 * sin2;
+            const T rp = rs * (t3 - t4) / (t3 + t4);
+            out[ch] = T(0.5) * (rp + rs);
+        }
+        return out;
+    }
+    // refl_aux::dielectric_fresnel (utility/reflection.h:48-72)
+    static T dielectric(T eta_i, T eta_o, T cos_i) {
+        if (cos_i < 0) { std::swap(eta_i, eta_o); cos_i = -cos_i; }
+        const T sin_i = std::sqrt(std::max(T(0), 1 - cos_i * cos_i));
+        const T sin_t = eta_o / eta_i * sin_i;
+        if (sin_t >= 1) return 1;
+        const T cos_t = std::sqrt(std::max(T(0), 1 - sin_t * sin_t));
+        const T para = (eta_i * cos_i - eta_o * cos_t) / (eta_i * cos_i + eta_o * cos_t);
+        const T perp = (eta_o * cos_i - eta_i * cos_t) / (eta_o * cos_i + eta_i * cos_t);
+        return T(0.5) * (para * para + perp * perp);
+    }
+    static T normal_corr(const V3<T> &g, const V3<T> &s, const V3<T> &wi, T eps) {  // common.h:239-244
+        const T dem = std::abs(cos_between(g, wi));
+        return dem < eps ? T(1) : std::abs(cos_between(s, wi) / dem);
+    }
+    static T phi_of(const V3<T> &w) {  // common.h:220-226
+        if (!w.y && !w.x) return 0;
+        const T r = std::atan2(w.y, w.x);
+        return r < 0 ? r + 2 * PI<T> : r;
+    }
+    static T tan_theta(const V3<T> &w) { const T t = 1 - w.z * w.z; if (t <= 0) return 0; return std::sqrt(t) / w.z; }
+
+    // GGX component eval / pdf (microfacet_refl_comp.cpp:8-65, utility/microfacet.cpp:44-61)
+    void ggx_eval_pdf(const V3<T> &lwi, const V3<T> &lwo, V3<T> *f, T *pdf) const {
+        if (lwi.z <= 0 || lwo.z <= 0) { if (f) *f = V3<T>(0); if (pdf) *pdf = 0; return; }
+        const T cti = lwi.z, cto = lwo.z;
+        const V3<T> lwh = (lwi + lwo).normalize();
+        const T ctd = dot(lwi, lwh);
+        const T phi_h = phi_of(lwh);
+        const T sph = std::sin(phi_h), cph = std::cos(phi_h);
+        const T cth = lwh.z;
+        const T sth = std::sqrt(std::max(T(0), 1 - cth * cth));
+        const T A = (cph / ax) * (cph / ax) + (sph / ay) * (sph / ay);
+        const T RD = sth * sth * A + cth * cth;
+        const T D = 1 / (PI<T> * ax * ay * (RD * RD));
+        const T phi_i = phi_of(lwi), phi_o = phi_of(lwo);
+        const T spi = std::sin(phi_i), cpi = std::cos(phi_i);
+        const T spo = std::sin(phi_o), cpo = std::cos(phi_o);
+        auto smith = [&](T cp, T sp, T tt) {
+            const T t = (ax * cp) * (ax * cp) + (ay * sp) * (ay * sp);
+            const T sv = 1 + t * (tt * tt);
+            const T lambda = -T(0.5) + T(0.5) * std::sqrt(sv);
+            return 1 / (1 + lambda);
+        };
+        const T Gi = smith(cpi, spi, tan_theta(lwi));
+        const T Go = smith(cpo, spo, tan_theta(lwo));
+        if (f) {
+            const V3<T> F = conductor(ctd, eta2, k2) * color;
+            const T G = Gi * Go;
+            *f = F * D * G / std::abs(4 * cti * cto);
+        }
+        if (pdf) *pdf = Go * D / (4 * lwo.z);
+    }
+    // microfacet::sample_anisotropic_gtr2_vnor (microfacet.cpp:104-128)
+    V3<T> ggx_vndf(const V3<T> &ve, T su, T sv, T eps) const {
+        const V3<T> vh = V3<T>(ax * ve.x, ay * ve.y, ve.z).normalize();
+        const T lensq = vh.x * vh.x + vh.y * vh.y;
+        const V3<T> t1 = lensq > eps ? V3<T>(-vh.y, vh.x, 0) / std::sqrt(lensq) : V3<T>(1, 0, 0);
+        const V3<T> t2 = cross(vh, t1);
+        const T r = std::sqrt(su);
+        const T phi = 2 * PI<T> * sv;
+        const T t_1 = r * std::cos(phi);
+        const T _t_2 = r * std::sin(phi);
+        const T s = T(0.5) * (1 + vh.z);
+        const T t_2 = (1 - s) * std::sqrt(1 - t_1 * t_1) + s * _t_2;
+        const V3<T> nh = t1 * t_1 + t2 * t_2 + vh * std::sqrt(std::max(T(0), 1 - t_1 * t_1 - t_2 * t_2));
+        return V3<T>(ax * nh.x, ay * nh.y, std::max(T(0), nh.z)).normalize();
+    }
+
+    V3<T> eval(const V3<T> &wi, const V3<T> &wo) const {
+        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK) return V3<T>(0);
+        // AggregateBSDF::eval (component/aggregate.h:39-64)
+        if (fringe(wi) || fringe(wo)) {
+            if (!geo.in_positive_z_hemisphere(wi) || !geo.in_positive_z_hemisphere(wo)) return V3<T>(0);
+            return albedo / PI<T>;
+        }
+        const V3<T> lwi = shd.global_to_local(wi).normalize();
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (!lwi.z || !lwo.z) return V3<T>(0);
+        if (kind == AGX_MAT_IDEAL_DIFFUSE) {                    // diffuse_comp.cpp:10-15
+            if (lwi.z <= 0 || lwo.z <= 0) return V3<T>(0);
+            return coef;
+        }
+        V3<T> f; ggx_eval_pdf(lwi, lwo, &f, nullptr);
+        return f;
+    }
+
+    T pdf(const V3<T> &wi, const V3<T> &wo) const {
+        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK) return 0;
+        // AggregateBSDF::pdf (aggregate.h:218-246)
+        if (fringe(wi) || fringe(wo)) {                         // bsdf.h:178-185
+            if (geo.in_positive_z_hemisphere(wi) != geo.in_positive_z_hemisphere(wo)) return 0;
+            const V3<T> l = geo.global_to_local(wi).normalize();
+            return zweighted_on_hemisphere_pdf(l.z);
+        }
+        const V3<T> lwi = shd.global_to_local(wi).normalize();
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (!lwi.z || !lwo.z) return 0;
+        T p;
+        if (kind == AGX_MAT_IDEAL_DIFFUSE) {
+            p = (lwi.z <= 0 || lwo.z <= 0) ? T(0) : zweighted_on_hemisphere_pdf(lwi.z);
+        } else {
+            ggx_eval_pdf(lwi, lwo, nullptr, &p);
+        }
+        return T(1) * p / T(1);
+    }
+
+    BSDFSample<T> sample(const V3<T> &wo, T su, T sv, T sw, T eps) const {
+        BSDFSample<T> res;
+        if (kind == AGX_MAT_IDEAL_BLACK) return res;
+        if (kind == AGX_MAT_MIRROR) {                           // mirror.cpp:32-50
+            const V3<T> lo = shd.global_to_local(wo);
+            if (lo.z <= 0) return res;
+            const V3<T> nwo = lo.normalize();
+            const V3<T> dir = shd.local_to_global(V3<T>(0, 0, 2 * nwo.z) - nwo);
+            const V3<T> f = conductor(nwo.z, m_eta2, m_k2) * albedo / std::abs(nwo.z);
+            res.dir = dir; res.f = f * normal_corr(geo.z, shd.z, dir, eps); res.pdf = 1; res.is_delta = true;
+            return res;
+        }
+        if (kind == AGX_MAT_GLASS) {                            // glass.cpp:52-91
+            const V3<T> nwo = shd.global_to_local(wo).normalize();
+            const V3<T> nor = nwo.z > 0 ? V3<T>(0, 0, 1) : V3<T>(0, 0, -1);
+            const T fr = dielectric(ior, T(1), nwo.z);
+            if (su < fr) {
+                const V3<T> lwi(-nwo.x, -nwo.y, nwo.z);
+                const V3<T> wi = shd.local_to_global(lwi);
+                const V3<T> f = c_refl * fr / std::abs(lwi.z);
+                res.dir = wi; res.f = f * normal_corr(geo.z, shd.z, wi, eps); res.pdf = fr; res.is_delta = true;
+                return res;
+            }
+            const T eta_i = nwo.z > 0 ? T(1) : ior;
+            const T eta_t = nwo.z > 0 ? ior : T(1);
+            const T eta = eta_i / eta_t;
+            const T cti = std::abs(nwo.z);
+            const T st2 = eta * eta * std::max(T(0), 1 - cti * cti);
+            if (st2 >= 1) return res;
+            const T ctt = std::sqrt(1 - st2);
+            const V3<T> nwi = (nor * (eta * cti - ctt) - nwo * eta).normalize();
+            const T corr = eta * eta;
+            const V3<T> dir = shd.local_to_global(nwi);
+            const V3<T> f = c_refr * corr * (1 - fr) / std::abs(nwi.z);
+            res.dir = dir; res.f = f * normal_corr(geo.z, shd.z, dir, eps); res.pdf = 1 - fr; res.is_delta = true;
+            return res;
+        }
+        // AggregateBSDF<1>::sample (aggregate.h:66-139)
+        if (fringe(wo)) {                                        // bsdf.h:142-159
+            if (!geo.in_positive_z_hemisphere(wo)) return res;
+            V3<T> lwi; T p;
+            zweighted_on_hemisphere(su, sv, &lwi, &p);
+            if (p < eps) return res;
+            const V3<T> wi = geo.local_to_global(lwi).normalize();
+            res.dir = wi; res.f = albedo / PI<T> * normal_corr(geo.z, shd.z, wi, eps); res.pdf = p;
+            return res;
+        }
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (!lwo.z) return res;
+        // single component of weight 1: (u - 0.001) * 1 <= 1 always selects it
+        if (!((su - T(0.001)) * T(1) <= T(1))) return res;
+        V3<T> lwi, f; T p;
+        if (kind == AGX_MAT_IDEAL_DIFFUSE) {                    // diffuse_comp.cpp:29-45
+            if (lwo.z <= 0) return res;
+            zweighted_on_hemisphere(sv, sw, &lwi, &p);
+            if (p < T(0.001)) return res;
+            f = coef;
+        } else {                                                 // microfacet_refl_comp.cpp:85-100
+            if (lwo.z <= 0) return res;
+            const V3<T> lwh = ggx_vndf(lwo, sv, sw, eps).normalize();
+            if (lwh.z <= 0) return res;
+            lwi = (lwh * (2 * dot(lwo, lwh)) - lwo).normalize();
+            if (lwi.z <= 0) return res;
+            ggx_eval_pdf(lwi, lwo, &f, &p);
+        }
+        lwi = lwi.normalize();
+        p *= T(1);
+        const V3<T> wi = shd.local_to_global(lwi);
+        res.dir = wi; res.f = f * normal_corr(geo.z, shd.z, wi, eps); res.pdf = p;
+        return res;
+    }
+};
+
+template<class T> struct Material {
+    agx_material_desc d;
+    // Material::shade -> BSDF (ideal_diffuse.cpp:25-39, metal.cpp:39-62, glass.cpp:180-206, mirror.cpp:98-115)
+    BSDF<T> shade(const SurfacePoint<T> &sp, V3<T> *shading_normal) const {
+        BSDF<T> b;
+        b.kind = d.kind;
+        b.geo = sp.geometry_coord;
+        b.shd = sp.user_coord;
+        const V3<T> col(T(d.color[0]), T(d.color[1]), T(d.color[2]));
+        switch (d.kind) {
+        case AGX_MAT_IDEAL_DIFFUSE:
+            b.albedo = col; b.coef = col / PI<T>;
+            break;
+        case AGX_MAT_METAL: {
+            // create_metal forwards (eta, k) into Metal(color, k, eta): the JSON "k"
+            // becomes eta_in and the JSON "eta" becomes k (material_creators.cpp:252-266,
+            // metal.cpp:23-29, 65-78).
+            b.albedo = col; b.color = col;
+            const V3<T> je(T(d.eta[0]), T(d.eta[1]), T(d.eta[2]));
+            const V3<T> jk(T(d.k[0]), T(d.k[1]), T(d.k[2]));
+            b.eta2 = jk * jk;      // (eta_in / eta_out)^2, eta_out = 1
+            b.k2 = je * je;        // (k / eta_out)^2
+            const T rough = T(d.roughness), an = T(d.anisotropic);
+            const T aspect = an > 0 ? std::sqrt(1 - T(0.9) * an) : T(1);   // microfacet_refl_comp.cpp:72-75
+            b.ax = std::max(T(0.001), rough * rough / aspect);
+            b.ay = std::max(T(0.001), rough * rough * aspect);
+            break;
+        }
+        case AGX_MAT_GLASS: {
+            b.ior = T(d.ior);
+            b.c_refl = col;
+            const V3<T> c2(T(d.color2[0]), T(d.color2[1]), T(d.color2[2]));
+            b.c_refr = d.glass_separate_refraction ? vsqrt(c2) : vsqrt(col);
+            b.albedo = (b.c_refl + b.c_refr) * T(0.5);
+            break;
+        }
+        case AGX_MAT_MIRROR: {
+            // ConductorPoint(eta_out = eta, eta_in = 1, k) (mirror.cpp:104-105)
+            b.albedo = col;
+            for (int ch = 0; ch < 3; ++ch) {
+                const T eo = T(d.eta[ch]), kk = T(d.k[ch]);
+                b.m_eta2[ch] = (1 / eo) * (1 / eo);
+                b.m_k2[ch] = (kk / eo) * (kk / eo);
+            }
+            break;
+        }
+        default:
+            b.albedo = V3<T>(0);
+            break;
+        }
+        *shading_normal = sp.user_coord.z;
+        return b;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// scene, lights  (core/scene/scene_default.cpp, entity/area_light, envir_light/native_sky.cpp)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct Entity {
+    Geometry<T> geom;
+    int32_t material = 0;
+    V3<T> emit;
+    bool is_light = false;
+    bool no_denoise = false;
+};
+
+template<class T> struct LightSample {
+    V3<T> pos, nor, radiance; T pdf = 0;
+};
+
+template<class T> struct Scene {
+    std::vector<Entity<T>> entities;
+    std::vector<Material<T>> materials;
+    int env_kind = AGX_ENV_NONE;
+    V3<T> sky_top, sky_bottom;
+    T world_radius = 1;
+    V3<T> world_centre;
+    std::vector<int32_t> lights;  // -1 = environment, else entity index (scene_default.cpp:66-82)
+    int aggregate = AGX_AGGREGATE_NATIVE;
+    // entity BVH (core/aggregate/entity_bvh.cpp)
+    struct ENode { AABB<T> bound; bool leaf; size_t a, b; };  // leaf: [a, b) into eprims; else children a, b
+    std::vector<ENode> enodes;
+    std::vector<int32_t> eprims;
+    int max_leaf = 5;
+    T eps = T(3e-4);
+
+    void build(const agx_scene_desc &d) {
+        if (d.abi_version != AGX_ABI_VERSION) throw std::runtime_error("abi version mismatch");
+        entities.clear(); entities.resize(d.entity_count);
+        for (uint32_t i = 0; i < d.entity_count; ++i) {
+            const agx_entity_desc &e = d.entities[i];
+            Entity<T> &en = entities[i];
+            en.geom.init(e);
+            if (e.material < 0 || uint32_t(e.material) >= d.material_count) throw std::runtime_error("bad material index");
+            en.material = e.material;
+            en.emit = V3<T>(T(e.emit_radiance[0]), T(e.emit_radiance[1]), T(e.emit_radiance[2]));
+            en.is_light = !en.emit.is_black();
+            en.no_denoise = e.no_denoise != 0;
+        }
+        materials.resize(d.material_count);
+        for (uint32_t i = 0; i < d.material_count; ++i) materials[i].d = d.materials[i];
+        env_kind = d.env.kind;
+        sky_top = V3<T>(T(d.env.top[0]), T(d.env.top[1]), T(d.env.top[2]));
+        sky_bottom = V3<T>(T(d.env.bottom[0]), T(d.env.bottom[1]), T(d.env.bottom[2]));
+        lights.clear();
+        if (env_kind != AGX_ENV_NONE) lights.push_back(-1);
+        for (uint32_t i = 0; i < d.entity_count; ++i) if (entities[i].is_light) lights.push_back(int32_t(i));
+        // Scene::start_rendering -> EnvirLight::preprocess (scene_default.cpp:162-172, core/core/light.cpp:6-14)
+        AABB<T> wb;
+        for (auto &en : entities) wb.grow(en.geom.bound);
+        const T diag = (wb.high - wb.low).length();
+        world_radius = T(1.1) * diag / 2;
+        world_centre = (wb.low + wb.high) * T(0.5);
+        aggregate = d.aggregate;
+        max_leaf = d.max_leaf_size > 0 ? d.max_leaf_size : 5;
+        enodes.clear(); eprims.clear();
+        if (aggregate == AGX_AGGREGATE_BVH) {
+            if (entities.empty()) { enodes.push_back({AABB<T>{V3<T>(0), V3<T>(1)}, true, 0, 0}); }
+            else {
+                std::vector<std::pair<int32_t, AABB<T>>> recs;
+                for (size_t i = 0; i < entities.size(); ++i) recs.push_back({int32_t(i), entities[i].geom.bound});
+                build_ebvh(recs.data(), recs.size());
+            }
+        }
+    }
+    // EntityBVH::build_aux (entity_bvh.cpp:46-116)
+    size_t build_ebvh(std::pair<int32_t, AABB<T>> *recs, size_t count) {
+        if (count <= size_t(max_leaf) || count < 2) {
+            const size_t s = eprims.size();
+            AABB<T> b;
+            for (size_t i = 0; i < count; ++i) { eprims.push_back(recs[i].first); b.grow(recs[i].second); }
+            enodes.push_back({b, true, s, s + count});
+            return enodes.size() - 1;
+        }
+        AABB<T> all;
+        for (size_t i = 0; i < count; ++i) all.grow(recs[i].second);
+        T best = -1; int axis = 0;
+        for (int i = 0; i < 3; ++i) { const T l = all.high[i] - all.low[i]; if (l > best) { best = l; axis = i; } }
+        std::sort(recs, recs + count, [axis](const std::pair<int32_t, AABB<T>> &L, const std::pair<int32_t, AABB<T>> &R) {
+            return L.second.low[axis] + L.second.high[axis] < R.second.low[axis] + R.second.high[axis];
+        });
+        const size_t idx = enodes.size();
+        enodes.push_back({all, false, 0, 0});
+        const size_t half = count / 2;
+        const size_t l = build_ebvh(recs, half);
+        const size_t r = build_ebvh(recs + half, count - half);
+        enodes[idx].a = l; enodes[idx].b = r;
+        return idx;
+    }
+
+    bool entity_closest(int32_t ei, Ray<T> &r, EntityHit<T> *h, Counters *c) const {
+        GeometryHit<T> g;
+        if (!entities[size_t(ei)].geom.closest(r, &g, eps, c)) return false;
+        static_cast<GeometryHit<T> &>(*h) = g;
+        h->entity = ei;
+        r.t_max = g.t;
+        return true;
+    }
+    bool ebvh_closest(size_t ni, const V3<T> &inv, Ray<T> &r, EntityHit<T> *h, Counters *c) const {
+        const ENode &n = enodes[ni];
+        if (c) c->node_visits++;
+        if (!n.bound.intersect(r.o, inv, r.t_min, r.t_max)) return false;
+        if (n.leaf) {
+            bool ret = false;
+            for (size_t i = n.a; i < n.b; ++i) ret = entity_closest(eprims[i], r, h, c) || ret;
+            return ret;
+        }
+        const bool L = ebvh_closest(n.a, inv, r, h, c);
+        const bool R = ebvh_closest(n.b, inv, r, h, c);
+        return L || R;
+    }
+    bool ebvh_any(size_t ni, const V3<T> &inv, const Ray<T> &r, Counters *c) const {
+        const ENode &n = enodes[ni];
+        if (c) c->node_visits++;
+        if (!n.bound.intersect(r.o, inv, r.t_min, r.t_max)) return false;
+        if (n.leaf) {
+            for (size_t i = n.a; i < n.b; ++i) if (entities[size_t(eprims[i])].geom.any_hit(r, c)) return true;
+            return false;
+        }
+        return ebvh_any(n.a, inv, r, c) || ebvh_any(n.b, inv, r, c);
+    }
+
+    // DefaultScene::closest_intersection -> Aggregate (native_aggregate.cpp:31-45, entity_bvh.cpp:209-215)
+    bool closest(const Ray<T> &r0, EntityHit<T> *h, Counters *c) const {
+        if (c) c->closest++;
+        Ray<T> r = r0;
+        bool ret = false;
+        if (aggregate == AGX_AGGREGATE_BVH) {
+            const V3<T> inv(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+            ret = ebvh_closest(0, inv, r, h, c);
+        } else {
+            for (size_t i = 0; i < entities.size(); ++i) ret = entity_closest(int32_t(i), r, h, c) || ret;
+        }
+        if (ret && c) c->hits++;
+        return ret;
+    }
+    bool any(const Ray<T> &r, Counters *c) const {
+        if (c) c->shadow++;
+        if (aggregate == AGX_AGGREGATE_BVH) {
+            const V3<T> inv(1 / r.d.x, 1 / r.d.y, 1 / r.d.z);
+            return ebvh_any(0, inv, r, c);
+        }
+        for (auto &e : entities) if (e.geom.any_hit(r, c)) return true;
+        return false;
+    }
+    bool visible(const V3<T> &A, const V3<T> &B, Counters *c) const {  // scene_default.cpp:136-141
+        const T dis = (A - B).length();
+        return !any(Ray<T>(A, (B - A).normalize(), eps, dis - eps), c);
+    }
+
+    // NativeSky (native_sky.cpp:13-20, 71-105)
+    V3<T> sky_radiance(const V3<T> &d) const {
+        const T ct = std::min(T(1), std::max(T(-1), d.normalize().z));
+        const T s = (ct + 1) / 2;
+        return sky_top * s + sky_bottom * (1 - s);
+    }
+    V3<T> env_emit_pos(const V3<T> &ref, const V3<T> &ref_to_light) const {
+        // point where the ray ref + t d meets the plane through centre + R d, normal d
+        const V3<T> d = ref_to_light.normalize();
+        const T t = world_radius + dot(world_centre - ref, d);
+        return ref + d * t;
+    }
+
+    // GeometryToDiffuseLight::sample (geometry_to_diffuse_light.cpp:14-31)
+    LightSample<T> area_sample(int32_t ei, const V3<T> &ref, const T s[5]) const {
+        LightSample<T> ls;
+        const Entity<T> &en = entities[size_t(ei)];
+        T pdf_area;
+        const SurfacePoint<T> sp = en.geom.sample(ref, &pdf_area, s[0], s[1], s[3], eps);
+        if (dot(sp.geometry_coord.z, ref - sp.pos) <= 0) return ls;
+        const V3<T> spt_to_ref = ref - sp.pos;
+        const T dist2 = spt_to_ref.length_square();
+        ls.pdf = pdf_area * dist2 / std::abs(cos_between(sp.geometry_coord.z, spt_to_ref));
+        ls.pos = sp.pos; ls.nor = sp.geometry_coord.z; ls.radiance = en.emit;
+        return ls;
+    }
+    // GeometryToDiffuseLight::radiance / pdf (:68-90)
+    V3<T> area_radiance(int32_t ei, const V3<T> &nor, const V3<T> &light_to_out) const {
+        return dot(nor, light_to_out) > 0 ? entities[size_t(ei)].emit : V3<T>(0);
+    }
+    T area_pdf(int32_t ei, const V3<T> &ref, const V3<T> &pos, const V3<T> &nor) const {
+        if (dot(nor, ref - pos) <= 0) return 0;
+        const T area_pdf = entities[size_t(ei)].geom.pdf(ref);
+        const V3<T> spt_to_ref = ref - pos;
+        const T dist2 = spt_to_ref.length_square();
+        const T abscos = std::abs(cos_between(nor, spt_to_ref));
+        return area_pdf * (dist2 / abscos);
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// trace_std  (render/path_tracing.cpp:15-225, render/direct_illum.cpp:11-192)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct PixelResult {
+    V3<T> value, albedo, normal;
+    T denoise = 1;
+};
+
+struct TraceParams { int min_depth = 5, max_depth = 10, specular_depth = 20; float cont_prob = 0.9f; };
+
+template<class T>
+PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, Sampler<T> &sam, Counters *c) {
+    const T eps = scene.eps;
+    V3<T> coef(1);
+    PixelResult<T> pixel;
+    for (int depth = 1, s_depth = 1; depth <= p.max_depth; ++depth) {
+        if (depth > p.min_depth) {                                          // :30-35
+            if (sam.u1() > T(p.cont_prob)) return pixel;
+            coef /= T(p.cont_prob);
+        }
+        EntityHit<T> inct;
+        if (!scene.closest(r, &inct, c)) {                                  // :39-49
+            if (depth == 1 && scene.env_kind != AGX_ENV_NONE) pixel.value += coef * scene.sky_radiance(r.d);
+            return pixel;
+        }
+        const Entity<T> &ent = scene.entities[size_t(inct.entity)];
+        V3<T> shading_normal;
+        const BSDF<T> bsdf = scene.materials[size_t(ent.material)].shade(inct, &shading_normal);
+        if (depth == 1) {                                                   // :54-60
+            pixel.normal = shading_normal;
+            pixel.albedo = bsdf.albedo;
+            if (ent.no_denoise) pixel.denoise = 0;
+        }
+        // void medium: no scattering, throughput 1 (medium_void.cpp:28-33)
+        if (depth == 1 && ent.is_light)                                     // :123-130
+            pixel.value += coef * scene.area_radiance(inct.entity, inct.geometry_coord.z, inct.wr);
+
+        V3<T> direct;                                                       // :134-146
+        for (int32_t light : scene.lights) {
+            T s5[5];
+            for (int k = 0; k < 5; ++k) s5[k] = sam.u1();
+            direct += coef * mis_sample_light(scene, light, inct, bsdf, s5, c);
+        }
+        {
+            T s3[3];
+            for (int k = 0; k < 3; ++k) s3[k] = sam.u1();
+            direct += coef * mis_sample_bsdf(scene, inct, bsdf, s3, c);
+        }
+        pixel.value += direct * (T(1) / T(1));
+
+        T s3[3];                                                            // :150-169
+        for (int k = 0; k < 3; ++k) s3[k] = sam.u1();
+        const BSDFSample<T> bs = bsdf.sample(inct.wr, s3[0], s3[1], s3[2], eps);
+        if (bs.f.is_black() || bs.pdf < eps) return pixel;
+        const T abscos = std::abs(cos_between(inct.geometry_coord.z, bs.dir));
+        coef *= bs.f * abscos / bs.pdf;
+        r = Ray<T>(inct.eps_offset(bs.dir, eps), bs.dir.normalize());
+        if (bs.is_delta && depth >= 2 && s_depth <= p.specular_depth) { --depth; ++s_depth; }
+    }
+    return pixel;
+}
+
+// mis_sample_area_light / mis_sample_envir_light (direct_illum.cpp:11-42, 72-99)
+template<class T>
+V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> &inct, const BSDF<T> &bsdf,
+                       const T s5[5], Counters *c) {
+    const T eps = scene.eps;
+    if (light >= 0) {
+        const LightSample<T> ls = scene.area_sample(light, inct.pos, s5);
+        if (ls.radiance.is_black() || !ls.pdf) return V3<T>(0);
+        const T len = (ls.pos - inct.pos).length() - eps;
+        if (len <= eps) return V3<T>(0);
+        const V3<T> to_light = (ls.pos - inct.pos).normalize();
+        if (scene.any(Ray<T>(inct.pos, to_light, eps, len), c)) return V3<T>(0);
+        const V3<T> f = bsdf.eval(to_light, inct.wr);
+        if (f.is_black()) return V3<T>(0);
+        const V3<T> ff = ls.radiance * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
+        const T bpdf = bsdf.pdf(to_light, inct.wr);
+        return ff / (ls.pdf + bpdf);
+    }
+    // native sky: uniform sphere direction, emit position on the bounding plane
+    const V3<T> dir = uniform_on_sphere(s5[0], s5[1]);
+    const V3<T> pos = scene.env_emit_pos(inct.pos, dir);
+    const V3<T> rad = scene.sky_radiance(dir);
+    if (rad.is_black()) return V3<T>(0);
+    if (!scene.visible(inct.pos, pos, c)) return V3<T>(0);
+    const V3<T> to_light = (pos - inct.pos).normalize();
+    const V3<T> f = bsdf.eval(to_light, inct.wr);
+    if (f.is_black()) return V3<T>(0);
+    const V3<T> ff = rad * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
+    const T bpdf = bsdf.pdf(to_light, inct.wr);
+    return ff / (uniform_on_sphere_pdf<T> + bpdf);
+}
+
+// mis_sample_bsdf (direct_illum.cpp:130-192)
+template<class T>
+V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSDF<T> &bsdf, const T s3[3], Counters *c) {
+    const T eps = scene.eps;
+    BSDFSample<T> bs = bsdf.sample(inct.wr, s3[0], s3[1], s3[2], eps);
+    if (bs.f.is_black()) return V3<T>(0);
+    bs.dir = bs.dir.normalize();
+    const Ray<T> nr(inct.eps_offset(bs.dir, eps), bs.dir);
+    EntityHit<T> h;
+    if (!scene.closest(nr, &h, c)) {
+        if (scene.env_kind == AGX_ENV_NONE) return V3<T>(0);
+        const V3<T> lr = scene.sky_radiance(nr.d);
+        if (lr.is_black()) return V3<T>(0);
+        const V3<T> f = lr * bs.f * std::abs(dot(inct.geometry_coord.z, nr.d));
+        if (bs.is_delta) return f / bs.pdf;
+        return f / (bs.pdf + uniform_on_sphere_pdf<T>);
+    }
+    if (!scene.entities[size_t(h.entity)].is_light) return V3<T>(0);
+    const V3<T> lrad = scene.area_radiance(h.entity, h.geometry_coord.z, h.wr);
+    if (lrad.is_black()) return V3<T>(0);
+    const V3<T> f = lrad * bs.f * std::abs(dot(inct.geometry_coord.z, nr.d));
+    if (bs.is_delta) return f / bs.pdf;
+    const T lpdf = scene.area_pdf(h.entity, nr.o, h.pos, h.geometry_coord.z);
+    return f / (bs.pdf + lpdf);
+}
+
+// ------------------------------------------------------------------------------------
+// camera  (camera/thin_lens.cpp:34-94; look_at is agz-utils, unpinned:
+//          z = normalize(dst - pos), x = normalize(up x z), y = z x x)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct Camera {
+    V3<T> pos, cx, cy, cz;
+    T film_w = 1, film_h = 1, lens_radius = 0, focal = 1;
+    void init(const agx_camera_desc &c) {
+        const V3<T> p(T(c.pos[0]), T(c.pos[1]), T(c.pos[2]));
+        const V3<T> d(T(c.dst[0]), T(c.dst[1]), T(c.dst[2]));
+        const V3<T> u(T(c.up[0]), T(c.up[1]), T(c.up[2]));
+        focal = T(c.focal_distance);
+        film_h = 2 * focal * std::tan(T(c.fov_rad) / 2);
+        film_w = T(c.film_aspect) * film_h;
+        lens_radius = T(c.lens_radius);
+        pos = p;
+        cz = (d - p).normalize();
+        cx = cross(u, cz).normalize();
+        cy = cross(cz, cx);
+    }
+    Ray<T> sample(T fx, T fy, T au, T av) const {
+        const V3<T> film((T(0.5) - fx) * film_w, (fy - T(0.5)) * film_h, focal);
+        const V2<T> ds = uniform_on_unit_disk(au, av);
+        const V3<T> lens(lens_radius * ds.x, lens_radius * ds.y, 0);
+        const V3<T> o = pos + cx * lens.x + cy * lens.y + cz * lens.z;
+        const V3<T> l = film - lens;
+        const V3<T> dir = (cx * l.x + cy * l.y + cz * l.z).normalize();
+        return Ray<T>(o, dir);
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// per-pixel renderer  (core/renderer/perpixel_renderer.cpp:13-193, core/framebuffer.h:49-80,
+//                      film_filter/box_filter.cpp:13-23)
+// ------------------------------------------------------------------------------------
+
+template<class T> struct Kahan {
+    T sum = 0, c = 0;
+    void add(T x) {  // Neumaier
+        const T t = sum + x;
+        if (std::abs(sum) >= std::abs(x)) c += (sum - t) + x; else c += (x - t) + sum;
+        sum = t;
+    }
+    T get() const { return sum + c; }
+};
+
+struct RenderOut {
+    double *rgb = nullptr, *albedo = nullptr, *normal = nullptr, *denoise = nullptr;
+};
+
+template<class T>
+void render(const Scene<T> &scene, const agx_render_params &rp, int threads, const RenderOut &out,
+            Counters *total, int row_begin = 0, int row_end = -1) {
+    Camera<T> cam;
+    cam.init(rp.camera);
+    TraceParams tp;
+    tp.min_depth = rp.min_depth; tp.max_depth = rp.max_depth; tp.specular_depth = rp.specular_depth;
+    tp.cont_prob = rp.cont_prob;
+    const int W = rp.width, H = rp.height;
+    if (row_end < 0) row_end = H;
+    const T radius = T(rp.filter_radius) - scene.eps;            // BoxFilter radius_ = r - EPS
+    if (!(radius > 0)) throw std::runtime_error("invalid box filter radius");
+    const int tile = 32;
+    const int tiles_x = (W + tile - 1) / tile, tiles_y = (row_end - row_begin + tile - 1) / tile;
+    std::atomic<int> next{0};
+    std::vector<Counters> cs(size_t(std::max(1, threads)));
+    auto worker = [&](int tid) {
+        Counters &c = cs[size_t(tid)];
+        for (;;) {
+            const int task = next.fetch_add(1);
+            if (task >= tiles_x * tiles_y) break;
+            const int tx = task % tiles_x, ty = task / tiles_x;
+            const int x0 = tx * tile, y0 = row_begin + ty * tile;
+            for (int py = y0; py < std::min(y0 + tile, row_end); ++py)
+                for (int px = x0; px < std::min(x0 + tile, W); ++px) {
+                    Kahan<T> acc[10];
+                    T weight = 0;
+                    const uint64_t pix = uint64_t(py) * uint64_t(W) + uint64_t(px);
+                    for (int s = 0; s < rp.spp; ++s) {
+                        Sampler<T> sam;
+                        sam.rng.seed_path(rp.seed, pix, uint64_t(rp.sample_begin + s));
+                        const float fu = sam.rng.uniform(), fv = sam.rng.uniform();
+                        // pixel coordinates are float in Atrc (real = float)
+                        const float pxf = float(px) + fu, pyf = float(py) + fv;
+                        const T au = sam.u1(), av = sam.u1();
+                        const Ray<T> ray = cam.sample(T(pxf / float(W)), T(pyf / float(H)), au, av);
+                        const PixelResult<T> pr = trace_std(tp, scene, ray, sam, &c);
+                        if (!pr.value.is_finite()) continue;
+                        // apply_image_filter with box radius r: a sample reaches only its
+                        // own pixel, and only if |px + 0.5 - sample| <= r on both axes
+                        const T xr = std::abs(T(float(px) + 0.5f - pxf));
+                        const T yr = std::abs(T(float(py) + 0.5f - pyf));
+                        if (xr > radius || yr > radius) continue;
+                        weight += 1;
+                        for (int k = 0; k < 3; ++k) {
+                            acc[k].add(pr.value[k]); acc[3 + k].add(pr.albedo[k]); acc[6 + k].add(pr.normal[k]);
+                        }
+                        acc[9].add(pr.denoise);
+                    }
+                    const size_t o = size_t(py) * size_t(W) + size_t(px);
+                    const double inv = weight > 0 ? 1.0 / double(weight) : 1.0;
+                    for (int k = 0; k < 3; ++k) {
+                        if (out.rgb) out.rgb[3 * o + k] = double(acc[k].get()) * inv;
+                        if (out.albedo) out.albedo[3 * o + k] = double(acc[3 + k].get()) * inv;
+                        if (out.normal) out.normal[3 * o + k] = double(acc[6 + k].get()) * inv;
+                    }
+                    if (out.denoise) out.denoise[o] = double(acc[9].get()) * inv;
+                }
+        }
+    };
+    std::vector<std::thread> pool;
+    const int nt = std::max(1, threads);
+    for (int i = 1; i < nt; ++i) pool.emplace_back(worker, i);
+    worker(0);
+    for (auto &t : pool) t.join();
+    if (total)
+        for (auto &c : cs) {
+            total->closest += c.closest; total->shadow += c.shadow; total->node_visits += c.node_visits;
+            total->tri_tests += c.tri_tests; total->hits += c.hits;
+        }
+}
+
+}  // namespace oracle

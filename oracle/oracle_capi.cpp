@@ -1,0 +1,176 @@
+// ORACLE — test infrastructure only (see atrc_oracle.hpp).
+// extern "C" entry points so tests and the benchmark's CPU-baseline leg can
+// drive the restatement through ctypes.
+#include <chrono>
+#include <cstdio>
+#include <string>
+
+#include "atrc_oracle.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct Holder {
+    int precision = 32;
+    oracle::Scene<float> sf;
+    oracle::Scene<double> sd;
+};
+
+template<class T> void copy_counters(const oracle::Counters &c, uint64_t *out) {
+    if (!out) return;
+    out[0] = c.closest; out[1] = c.shadow; out[2] = c.node_visits; out[3] = c.tri_tests; out[4] = c.hits;
+}
+
+template<class T>
+int trace(const oracle::Scene<T> &sc, const float *o, const float *d, const float *tmin, const float *tmax,
+          uint32_t n, int any_hit, int32_t *ent, int32_t *prim, float *t, float *u, float *v, uint64_t *cnt) {
+    oracle::Counters c;
+    for (uint32_t i = 0; i < n; ++i) {
+        const oracle::Ray<T> r(oracle::V3<T>(T(o[3 * i]), T(o[3 * i + 1]), T(o[3 * i + 2])),
+                               oracle::V3<T>(T(d[3 * i]), T(d[3 * i + 1]), T(d[3 * i + 2])),
+                               T(tmin[i]), T(tmax[i]));
+        if (any_hit) {
+            ent[i] = sc.any(r, &c) ? 1 : -1;
+            continue;
+        }
+        oracle::EntityHit<T> h;
+        if (sc.closest(r, &h, &c)) {
+            ent[i] = h.entity;
+            if (prim) prim[i] = h.prim;
+            if (t) t[i] = float(h.t);
+            if (u) u[i] = float(h.bu);
+            if (v) v[i] = float(h.bv);
+        } else {
+            ent[i] = -1;
+            if (prim) prim[i] = -1;
+            if (t) t[i] = -1;
+            if (u) u[i] = 0;
+            if (v) v[i] = 0;
+        }
+    }
+    copy_counters<T>(c, cnt);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *oracle_last_error() { return g_err.c_str(); }
+
+void *oracle_scene_create(const agx_scene_desc *d, int precision, float eps) {
+    try {
+        auto *h = new Holder();
+        h->precision = precision;
+        if (precision == 64) { h->sd.eps = double(eps); h->sd.build(*d); }
+        else { h->sf.eps = eps; h->sf.build(*d); }
+        return h;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+void oracle_scene_destroy(void *p) { delete static_cast<Holder *>(p); }
+
+int oracle_trace_rays(void *p, const float *o, const float *d, const float *tmin, const float *tmax, uint32_t n,
+                      int any_hit, int32_t *ent, int32_t *prim, float *t, float *u, float *v, uint64_t *counters) {
+    auto *h = static_cast<Holder *>(p);
+    if (!h || !o || !d || !tmin || !tmax || !ent) { g_err = "null argument"; return 1; }
+    if (h->precision == 64) return trace(h->sd, o, d, tmin, tmax, n, any_hit, ent, prim, t, u, v, counters);
+    return trace(h->sf, o, d, tmin, tmax, n, any_hit, ent, prim, t, u, v, counters);
+}
+
+// Render rows [row_begin, row_end) (row_end < 0: all) with `threads` workers.
+int oracle_render(void *p, const agx_render_params *rp, int threads, double *rgb, double *albedo, double *normal,
+                  double *denoise, uint64_t *counters, int row_begin, int row_end, double *seconds) {
+    auto *h = static_cast<Holder *>(p);
+    if (!h || !rp) { g_err = "null argument"; return 1; }
+    try {
+        oracle::RenderOut out{rgb, albedo, normal, denoise};
+        oracle::Counters c;
+        const auto t0 = std::chrono::steady_clock::now();
+        if (h->precision == 64) {
+            h->sd.eps = double(rp->eps);
+            oracle::render(h->sd, *rp, threads, out, &c, row_begin, row_end);
+        } else {
+            h->sf.eps = rp->eps;
+            oracle::render(h->sf, *rp, threads, out, &c, row_begin, row_end);
+        }
+        const auto t1 = std::chrono::steady_clock::now();
+        if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+        copy_counters<float>(c, counters);
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// PCG32 uniform stream of (seed, pixel, sample): n floats.
+void oracle_pcg_stream(uint64_t seed, uint64_t pixel, uint64_t sample, uint32_t n, float *out) {
+    oracle::Pcg32 r;
+    r.seed_path(seed, pixel, sample);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.uniform();
+}
+
+// Raw PCG32 outputs for pcg32_srandom_r(initstate, initseq).
+void oracle_pcg_raw(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t *out) {
+    oracle::Pcg32 r;
+    r.seed(initstate, initseq);
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.next();
+}
+
+// Distribution samplers: kind 0 disk(u,v)->(x,y), 1 cos-hemisphere->(x,y,z,pdf),
+// 2 sphere->(x,y,z), 3 cone(cos_max=w)->(x,y,z,pdf), 4 triangle->(b1,b2), 5 from_z frame (u,v,w = z) -> 9 floats
+void oracle_distribution(int kind, double u, double v, double w, double *out) {
+    using namespace oracle;
+    switch (kind) {
+    case 0: { auto d = uniform_on_unit_disk(u, v); out[0] = d.x; out[1] = d.y; break; }
+    case 1: { V3<double> dd; double p; zweighted_on_hemisphere(u, v, &dd, &p); out[0] = dd.x; out[1] = dd.y; out[2] = dd.z; out[3] = p; break; }
+    case 2: { auto d = uniform_on_sphere(u, v); out[0] = d.x; out[1] = d.y; out[2] = d.z; break; }
+    case 3: { auto d = uniform_on_cone(w, u, v); out[0] = d.x; out[1] = d.y; out[2] = d.z; out[3] = uniform_on_cone_pdf(w); break; }
+    case 4: { auto d = uniform_on_triangle(u, v); out[0] = d.x; out[1] = d.y; break; }
+    case 5: {
+        auto c = Coord<double>::from_z(V3<double>(u, v, w));
+        const V3<double> *ax[3] = {&c.x, &c.y, &c.z};
+        for (int i = 0; i < 3; ++i) { out[3 * i] = ax[i]->x; out[3 * i + 1] = ax[i]->y; out[3 * i + 2] = ax[i]->z; }
+        break;
+    }
+    default: break;
+    }
+}
+
+// BSDF hooks in the local shading frame (geometry frame = shading frame = identity
+// unless geo_z is given).  mode 0: eval(wi, wo) -> f[3]; 1: pdf -> out[0];
+// 2: sample(wo, u, v, w) -> dir[3], f[3], pdf, is_delta.
+int oracle_bsdf(const agx_material_desc *m, int precision, const double *geo_z, const double *wi, const double *wo,
+                double u, double v, double w, int mode, double *out) {
+    using namespace oracle;
+    auto run = [&](auto zero) -> int {
+        using T = decltype(zero);
+        Material<T> mat; mat.d = *m;
+        SurfacePoint<T> sp;
+        if (geo_z) sp.geometry_coord = Coord<T>::from_z(V3<T>(T(geo_z[0]), T(geo_z[1]), T(geo_z[2])));
+        V3<T> sn;
+        BSDF<T> b = mat.shade(sp, &sn);
+        const V3<T> vo{T(wo[0]), T(wo[1]), T(wo[2])};
+        if (mode == 0) {
+            const V3<T> f = b.eval(V3<T>(T(wi[0]), T(wi[1]), T(wi[2])), vo);
+            out[0] = double(f.x); out[1] = double(f.y); out[2] = double(f.z);
+        } else if (mode == 1) {
+            out[0] = double(b.pdf(V3<T>(T(wi[0]), T(wi[1]), T(wi[2])), vo));
+        } else {
+            const BSDFSample<T> s = b.sample(vo, T(u), T(v), T(w), T(3e-4));
+            out[0] = double(s.dir.x); out[1] = double(s.dir.y); out[2] = double(s.dir.z);
+            out[3] = double(s.f.x); out[4] = double(s.f.y); out[5] = double(s.f.z);
+            out[6] = double(s.pdf); out[7] = s.is_delta ? 1.0 : 0.0;
+        }
+        return 0;
+    };
+    if (precision == 64) return run(double(0));
+    return run(float(0));
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of liboracle.so (the CPU restatement of Atrc `pt`).  Imported
+only by tests, the smoke check and the benchmark's CPU-baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+    lib = C.CDLL(str(LIB_PATH))
+    P = C.POINTER
+    lib.oracle_last_error.restype = C.c_char_p
+    lib.oracle_scene_create.restype = C.c_void_p
+    lib.oracle_scene_create.argtypes = [C.c_void_p, C.c_int, C.c_float]
+    lib.oracle_scene_destroy.argtypes = [C.c_void_p]
+    lib.oracle_trace_rays.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float), C.c_uint32,
+                                      C.c_int, P(C.c_int32), P(C.c_int32), P(C.c_float), P(C.c_float), P(C.c_float),
+                                      P(C.c_uint64)]
+    lib.oracle_render.argtypes = [C.c_void_p, C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                  P(C.c_double), P(C.c_uint64), C.c_int, C.c_int, P(C.c_double)]
+    lib.oracle_pcg_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_float)]
+    lib.oracle_pcg_raw.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_uint32)]
+    lib.oracle_distribution.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, P(C.c_double)]
+    lib.oracle_bsdf.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
+                                C.c_double, C.c_double, C.c_int, P(C.c_double)]
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+COUNTER_NAMES = ("closest", "shadow", "node_visits", "tri_tests", "hits")
+
+
+class OracleScene:
+    """A scene restated on the CPU in float (precision=32) or double (64)."""
+
+    def __init__(self, desc, precision: int = 32, eps: float = 3e-4):
+        self.lib = load()
+        self.h = self.lib.oracle_scene_create(C.byref(desc), precision, eps)
+        if not self.h:
+            raise RuntimeError("oracle scene: " + self.lib.oracle_last_error().decode())
+
+    def close(self):
+        if self.h:
+            self.lib.oracle_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def trace_rays(self, o, d, tmin, tmax, any_hit=False):
+        n = len(o)
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        tmin = np.ascontiguousarray(tmin, np.float32)
+        tmax = np.ascontiguousarray(tmax, np.float32)
+        ent = np.zeros(n, np.int32)
+        prim = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        cnt = np.zeros(5, np.uint64)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+        rc = self.lib.oracle_trace_rays(self.h, _f(o), _f(d), _f(tmin), _f(tmax), n, 1 if any_hit else 0, ip(ent),
+                                        ip(prim), _f(t), _f(u), _f(v), cnt.ctypes.data_as(C.POINTER(C.c_uint64)))
+        if rc:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v, "counters": dict(zip(COUNTER_NAMES, cnt.tolist()))}
+
+    def render(self, params, threads: int | None = None, rows: tuple | None = None, aovs: bool = True):
+        W, H = params.width, params.height
+        rgb = np.zeros((H, W, 3), np.float64)
+        alb = np.zeros((H, W, 3), np.float64) if aovs else None
+        nor = np.zeros((H, W, 3), np.float64) if aovs else None
+        den = np.zeros((H, W), np.float64) if aovs else None
+        cnt = np.zeros(5, np.uint64)
+        secs = C.c_double()
+        r0, r1 = rows if rows else (0, -1)
+        th = threads if threads else (os.cpu_count() or 1)
+        rc = self.lib.oracle_render(self.h, C.byref(params), th, _d(rgb), _d(alb), _d(nor), _d(den),
+                                    cnt.ctypes.data_as(C.POINTER(C.c_uint64)), r0, r1, C.byref(secs))
+        if rc:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den, "seconds": secs.value,
+                "counters": dict(zip(COUNTER_NAMES, cnt.tolist())), "threads": th}
+
+
+def pcg_stream(seed: int, pixel: int, sample: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    load().oracle_pcg_stream(seed, pixel, sample, n, _f(out))
+    return out
+
+
+def pcg_raw(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    load().oracle_pcg_raw(initstate, initseq, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def distribution(kind: int, u: float, v: float, w: float = 0.0) -> np.ndarray:
+    out = np.zeros(9, np.float64)
+    load().oracle_distribution(kind, u, v, w, _d(out))
+    return out
+
+
+def bsdf(material_desc, mode: int, wi=(0, 0, 1), wo=(0, 0, 1), u=0.5, v=0.5, w=0.5, precision=64, geo_z=None):
+    out = np.zeros(8, np.float64)
+    wi = np.asarray(wi, np.float64)
+    wo = np.asarray(wo, np.float64)
+    gz = None if geo_z is None else np.asarray(geo_z, np.float64)
+    load().oracle_bsdf(C.byref(material_desc), precision, _d(gz), _d(wi), _d(wo), u, v, w, mode, _d(out))
+    return out
