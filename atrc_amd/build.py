@@ -57,19 +57,15 @@ def build_gpu() -> Path:
 
 
 def build_host() -> list[Path]:
+    """C++ host: config, factory, gpu_pt renderer (linked against libatrc_gpu.so)."""
     srcs = sorted(str(p) for p in HOST.glob("*.cpp") if p.name != "cli_main.cpp")
-    if not srcs:
-        return []
     flags = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-parameter", f"-I{ROOT / 'include'}"]
+    link = [f"-L{PKG}", "-latrc_gpu", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-pthread"]
     tmp = HOST_LIB.with_suffix(".so.tmp")
-    _run(["g++", *flags, "-shared", "-o", str(tmp), *srcs, "-ldl", "-pthread"])
+    _run(["g++", *flags, "-shared", "-o", str(tmp), *srcs, *link])
     os.replace(tmp, HOST_LIB)
-    out = [HOST_LIB]
-    main = HOST / "cli_main.cpp"
-    if main.exists():
-        _run(["g++", *flags, "-o", str(CLI), str(main), *srcs, "-ldl", "-pthread"])
-        out.append(CLI)
-    return out
+    _run(["g++", *flags, "-o", str(CLI), str(HOST / "cli_main.cpp"), f"-L{PKG}", "-latrc_host", *link])
+    return [HOST_LIB, CLI]
 
 
 def build_oracle() -> Path:

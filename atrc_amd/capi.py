@@ -249,3 +249,76 @@ def debug_scene_summary(desc: agx_scene_desc) -> dict:
     s = agx_debug_summary()
     _check(lib.agx_debug_scene_summary(C.byref(desc), C.byref(s)))
     return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+# ---------------------------------------------------------------------------------- C++ host
+HOST_LIB_PATH = PKG / "libatrc_host.so"
+
+
+class atrc_host_session(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("eps", C.c_float), ("filter_radius", C.c_float),
+                ("camera", agx_camera_desc), ("session_count", C.c_int32)]
+
+
+_host = None
+
+
+def load_host() -> C.CDLL:
+    global _host
+    if _host is not None:
+        return _host
+    if not HOST_LIB_PATH.exists():
+        raise RuntimeError(f"{HOST_LIB_PATH} is missing: run `python -m atrc_amd.build host`")
+    lib = C.CDLL(str(HOST_LIB_PATH))
+    lib.atrc_host_load.restype = C.c_void_p
+    lib.atrc_host_load.argtypes = [C.c_char_p, C.c_char_p]
+    lib.atrc_host_scene_desc.restype = C.POINTER(agx_scene_desc)
+    lib.atrc_host_scene_desc.argtypes = [C.c_void_p]
+    lib.atrc_host_session_info.argtypes = [C.c_void_p, C.c_int, C.POINTER(atrc_host_session)]
+    fp = C.POINTER(C.c_float)
+    lib.atrc_host_execute.argtypes = [C.c_void_p, C.c_int, fp, fp, fp, fp]
+    lib.atrc_host_free.argtypes = [C.c_void_p]
+    lib.atrc_host_free.restype = None
+    lib.atrc_host_last_error.restype = C.c_char_p
+    _host = lib
+    return lib
+
+
+class HostScene:
+    """A scene and its render sessions created by the C++ host from Atrc JSON."""
+
+    def __init__(self, json_text: str, scene_dir: str):
+        self.lib = load_host()
+        self.h = self.lib.atrc_host_load(json_text.encode(), str(scene_dir).encode())
+        if not self.h:
+            raise RuntimeError("atrc host: " + self.lib.atrc_host_last_error().decode())
+
+    def desc(self) -> agx_scene_desc:
+        return self.lib.atrc_host_scene_desc(self.h).contents
+
+    def session(self, i: int = 0) -> atrc_host_session:
+        s = atrc_host_session()
+        if self.lib.atrc_host_session_info(self.h, i, C.byref(s)):
+            raise RuntimeError(self.lib.atrc_host_last_error().decode())
+        return s
+
+    def execute(self, i: int = 0) -> dict:
+        s = self.session(i)
+        rgb = np.zeros((s.height, s.width, 3), np.float32)
+        alb = np.zeros_like(rgb)
+        nor = np.zeros_like(rgb)
+        den = np.zeros((s.height, s.width), np.float32)
+        if self.lib.atrc_host_execute(self.h, i, fptr(rgb), fptr(alb), fptr(nor), fptr(den)):
+            raise RuntimeError("atrc host: " + self.lib.atrc_host_last_error().decode())
+        return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den}
+
+    def close(self):
+        if self.h:
+            self.lib.atrc_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

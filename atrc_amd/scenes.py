@@ -310,3 +310,92 @@ def random_rays(scene_bounds: tuple, n: int, seed: int, inside: bool = True):
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     return o.astype(np.float32), d.astype(np.float32)
+
+
+# ---------------------------------------------------------------------------------- Atrc JSON
+
+def _texture(c):
+    c = [float(x) for x in (c if isinstance(c, (tuple, list, np.ndarray)) else (c,))]
+    return {"type": "constant", "texel": c}
+
+
+def _material_json(m: Material) -> dict:
+    if m.kind == capi.MAT_IDEAL_DIFFUSE:
+        return {"type": "ideal_diffuse", "albedo": _texture(m.color)}
+    if m.kind == capi.MAT_METAL:
+        return {"type": "metal", "color": _texture(m.color), "eta": _texture(m.eta), "k": _texture(m.k),
+                "roughness": _texture(m.roughness), "anisotropic": _texture(m.anisotropic)}
+    if m.kind == capi.MAT_GLASS:
+        out = {"type": "glass", "ior": _texture(m.ior)}
+        if m.separate_refraction:
+            out["color_reflection_map"] = _texture(m.color)
+            out["color_refraction_map"] = _texture(m.color2)
+        else:
+            out["color_map"] = _texture(m.color)
+        return out
+    if m.kind == capi.MAT_MIRROR:
+        return {"type": "mirror", "color_map": _texture(m.color), "eta": _texture(m.eta), "k": _texture(m.k)}
+    return {"type": "ideal_black"}
+
+
+def _transform_json(l2w) -> list:
+    m = np.array(l2w, np.float64).reshape(3, 4)
+    return [{"type": "translate", "offset": [float(x) for x in m[:, 3]]},
+            {"type": "mat3", "row_major": [float(x) for x in m[:, :3].reshape(-1)]}]
+
+
+def to_atrc_json(scene: Scene, camera: dict, trace: dict, width: int, height: int, spp: int, mesh_dir,
+                 renderer: str = "gpu_pt", seed: int = 42, post_processors: list | None = None,
+                 eps: float | None = None) -> str:
+    """Atrc JSON ({"scene", "rendering"}) for a Python-built scene; meshes are
+    written as .bm files into mesh_dir and referenced via ${scene-directory}."""
+    import json
+    from pathlib import Path
+
+    mesh_dir = Path(mesh_dir)
+    ents = []
+    for i, e in enumerate(scene.entities):
+        if e.geometry == capi.GEOM_QUAD:
+            g = {"type": "quad", **{k: [float(x) for x in p] for k, p in zip("ABCD", e.points)}}
+            for k, t in zip(("tA", "tB", "tC", "tD"), e.uvs):
+                g[k] = [float(x) for x in t]
+        elif e.geometry == capi.GEOM_TRIANGLE_BVH:
+            name = f"mesh_{i}.bm"
+            write_bin_mesh(mesh_dir / name, e.tris)
+            g = {"type": "triangle_bvh", "filename": "${scene-directory}/" + name}
+        elif e.geometry == capi.GEOM_TRIANGLE:
+            g = {"type": "triangle", **{k: [float(x) for x in p] for k, p in zip("ABC", e.points)},
+                 "transform": _transform_json(e.local_to_world)}
+        else:
+            g = {"type": "sphere" if e.geometry == capi.GEOM_SPHERE else "disk", "radius": float(e.radius),
+                 "transform": _transform_json(e.local_to_world)}
+        ent = {"type": "geometric", "geometry": g, "material": _material_json(scene.materials[e.material])}
+        if any(e.emit):
+            ent["emit_radiance"] = [float(x) for x in e.emit]
+        if e.no_denoise:
+            ent["no_denoise"] = 1
+        ents.append(ent)
+    sc = {"type": "default", "entities": ents}
+    if scene.env is not None:
+        sc["env"] = {"type": "native_sky", "top": [float(x) for x in scene.env["top"]],
+                     "bottom": [float(x) for x in scene.env["bottom"]]}
+    sc["aggregate"] = ({"type": "bvh", "max_leaf_size": scene.max_leaf_size}
+                       if scene.aggregate == capi.AGGREGATE_BVH else {"type": "native"})
+    cam = {"type": "thin_lens", "pos": [float(x) for x in camera["pos"]], "dst": [float(x) for x in camera["dst"]],
+           "up": [float(x) for x in camera.get("up", (0, 0, 1))], "fov": float(camera.get("fov", 40.0))}
+    rend = {"type": renderer, "spp": spp, "min_depth": trace.get("min_depth", 5),
+            "max_depth": trace.get("max_depth", 10), "cont_prob": trace.get("cont_prob", 0.9), "seed": seed}
+    rendering = {"width": width, "height": height, "camera": cam, "renderer": rend, "reporter": {"type": "noout"}}
+    if post_processors:
+        rendering["post_processors"] = post_processors
+    if eps is not None:
+        rendering["eps"] = eps
+    return json.dumps({"scene": sc, "rendering": rendering}, indent=1)
+
+
+def write_bin_mesh(path, tris: np.ndarray) -> None:
+    """Atrc's .bm format: uint64 triangle count, then 3 vertices of 8 floats each."""
+    tris = np.ascontiguousarray(tris, np.float32)
+    with open(path, "wb") as f:
+        f.write(np.uint64(len(tris) // 3).tobytes())
+        f.write(tris.tobytes())

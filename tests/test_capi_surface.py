@@ -1,0 +1,70 @@
+"""The C-ABI libraries load without a GPU and export every declared symbol;
+descriptor validation and error paths work on the host."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from atrc_amd import capi, scenes
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared(header: str) -> set:
+    text = (ROOT / "include" / header).read_text()
+    return set(re.findall(r"^\s*(?:int|void|const char \*|const agx_scene_desc \*|void \*)\s*\*?\s*(\w+)\s*\(", text,
+                          re.M))
+
+
+def test_gpu_library_exports_every_declared_symbol():
+    lib = capi.load_library()
+    names = declared("atrc_gpu.h") | declared("atrc_gpu_debug.h")
+    assert {"agx_create", "agx_render", "agx_trace_rays", "agx_upload_scene"} <= names
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(capi.EXPORTS) == names
+    assert lib.agx_abi_version() == capi.AGX_ABI_VERSION
+
+
+def test_host_library_exports_every_declared_symbol():
+    lib = capi.load_host()
+    for n in declared("atrc_host.h"):
+        assert hasattr(lib, n), n
+
+
+def test_null_arguments_are_rejected():
+    lib = capi.load_library()
+    assert lib.agx_create(0, None) == capi.AGX_E_INVALID_ARG
+    assert lib.agx_upload_scene(None, None) == capi.AGX_E_INVALID_ARG
+    assert lib.agx_render(None, None, None, None, None, None, None) == capi.AGX_E_INVALID_ARG
+    assert b"null" in lib.agx_last_error()
+
+
+def test_descriptor_validation():
+    s, _, _ = scenes.cornell_box()
+    d, keep = s.to_desc()
+    summ = capi.debug_scene_summary(d)
+    assert summ["entities"] == 8 and summ["lights"] == 1 and summ["blas_prims"] == 24
+    assert summ["max_tlas_depth"] >= 1
+    # bad material index
+    s.entities[0].material = 99
+    d, keep = s.to_desc()
+    with pytest.raises(capi.AgxError) as e:
+        capi.debug_scene_summary(d)
+    assert e.value.code == capi.AGX_E_INVALID_ARG
+    # more lights than the kernels support
+    s2 = scenes.Scene()
+    m = s2.add_material(scenes.Material())
+    for i in range(20):
+        s2.add(scenes.Entity(capi.GEOM_QUAD, m, emit=(1, 1, 1),
+                             points=((i, 0, 0), (i + 1, 0, 0), (i + 1, 1, 0), (i, 1, 0))))
+    d, keep = s2.to_desc()
+    with pytest.raises(capi.AgxError) as e:
+        capi.debug_scene_summary(d)
+    assert e.value.code == capi.AGX_E_UNSUPPORTED
+    # wrong ABI version
+    d.abi_version = 999
+    with pytest.raises(capi.AgxError):
+        capi.debug_scene_summary(d)

@@ -80,12 +80,13 @@ def test_white_furnace_sphere(gpu_ctx):
     s.env = {"top": (1.0, 1.0, 1.0), "bottom": (1.0, 1.0, 1.0)}
     desc, keep = s.to_desc()
     gpu_ctx.upload_scene(desc)
-    cam = {"pos": (0.0, -4.0, 0.0), "dst": (0.0, 0.0, 0.0), "fov": 30.0}
+    # the sphere subtends asin(1/4) = 14.5 deg: about 14 px of radius at fov 60
+    cam = {"pos": (0.0, -4.0, 0.0), "dst": (0.0, 0.0, 0.0), "fov": 60.0}
     p = scenes.render_params(64, 64, 256, cam, min_depth=3, max_depth=10, seed=5)
     log("[render parity] furnace sphere 64x64 256 spp")
     img = gpu_ctx.render(p)["image"]
     yy, xx = np.mgrid[0:64, 0:64]
-    inner = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 < 12 ** 2
-    outer = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 > 30 ** 2
+    inner = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 < 10 ** 2
+    outer = (xx - 31.5) ** 2 + (yy - 31.5) ** 2 > 18 ** 2
     assert abs(img[inner].mean() - rho) < 0.01 * rho
     assert abs(img[outer].mean() - 1.0) < 1e-5

@@ -48,7 +48,7 @@ def test_cornell_rays(gpu_ctx):
     o, d = scenes.random_rays(((0.05, 0.05, 0.05), (5.5, 5.5, 5.5)), 200_000, seed=7)
     n = len(o)
     frac = _compare(gpu_ctx, s, o, d, np.zeros(n, np.float32), np.full(n, np.inf, np.float32), "cornell")
-    assert frac > 0.99  # closed room: nearly every ray hits
+    assert frac > 0.8  # the box is open at the front (y = 0)
 
 
 def test_cornell_native_aggregate_and_bounded_rays(gpu_ctx):

@@ -1,0 +1,122 @@
+"""The C++ host (libatrc_host.so): Atrc JSON -> configuration tree -> factory
+-> agx_scene_desc, without a GPU.
+
+Semantics checked against Atrc's factory and config code:
+  src/factory/src/utility/config_cvt.cpp:39-105   (floats via std::to_string: 6 decimals)
+  src/tracer/src/utility/config.cpp:36-59, 195-314 (vec3 of 1 or 3 values; transform arrays)
+  src/factory/include/agz/factory/context.h:179-208 ("reference" creator)
+  src/factory/src/creator/scene_creators.cpp:39-43  (entity types ending in "//" are skipped)
+"""
+import ctypes as C
+import json
+import math
+
+import numpy as np
+import pytest
+
+from atrc_amd import capi, scenes
+from oracle import py_oracle as po
+
+
+def quad_json(A, B, C_, D, transform=None, emit=None, albedo=0.5):
+    g = {"type": "quad", "A": A, "B": B, "C": C_, "D": D}
+    if transform is not None:
+        g["transform"] = transform
+    e = {"type": "geometric", "geometry": g,
+         "material": {"type": "ideal_diffuse", "albedo": {"type": "constant", "texel": [albedo]}}}
+    if emit is not None:
+        e["emit_radiance"] = emit
+    return e
+
+
+def wrap(entities, extra_scene=None, renderer=None):
+    sc = {"type": "default", "entities": entities}
+    sc.update(extra_scene or {})
+    return json.dumps({"scene": sc, "rendering": {
+        "width": 8, "height": 6, "camera": {"type": "thin_lens", "pos": [0, -5, 0], "dst": [0, 0, 0], "up": [0, 0, 1],
+                                            "fov": 45},
+        "renderer": renderer or {"type": "gpu_pt", "spp": 4}, "reporter": {"type": "noout"}}})
+
+
+def entity(desc, i):
+    return desc.entities[i]
+
+
+def test_transform_array_applies_last_first():
+    # [translate(1,0,0), rotate_z(90 deg)]: rotate first, then translate
+    t = [{"type": "translate", "offset": [1, 0, 0]}, {"type": "rotate_z", "deg": 90}]
+    h = capi.HostScene(wrap([quad_json([1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 0], transform=t)]), ".")
+    e = entity(h.desc(), 0)
+    np.testing.assert_allclose(list(e.p[0]), [1, 1, 0], atol=1e-6)   # (1,0,0) -> (0,1,0) -> (1,1,0)
+    np.testing.assert_allclose(list(e.p[1]), [0, 1, 0], atol=1e-6)   # (1,1,0) -> (-1,1,0) -> (0,1,0)
+    t2 = [{"type": "scale", "ratio": [2, 3, 4]}, {"type": "rotate", "axis": [0, 0, 1], "rad": math.pi / 2}]
+    h2 = capi.HostScene(wrap([quad_json([1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 0], transform=t2)]), ".")
+    np.testing.assert_allclose(list(entity(h2.desc(), 0).p[0]), [0, 3, 0], atol=1e-6)
+
+
+def test_vec3_broadcast_and_six_decimal_floats():
+    # a 1-element array is a vec3 of three equal values; |x| < 5e-7 reads back as 0
+    h = capi.HostScene(wrap([quad_json([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], emit=[2.5],
+                                       albedo=4e-7)]), ".")
+    e = entity(h.desc(), 0)
+    assert list(e.emit_radiance) == [2.5, 2.5, 2.5]
+    assert list(h.desc().materials[0].color) == [0.0, 0.0, 0.0]
+    # 0.1234567 -> "0.123457"
+    h = capi.HostScene(wrap([quad_json([0.1234567, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])]), ".")
+    assert entity(h.desc(), 0).p[0][0] == np.float32(0.123457)
+
+
+def test_skip_commented_entities_reference_and_aggregate():
+    ents = [quad_json([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]),
+            {"type": "geometric//", "geometry": {"type": "nonsense"}},
+            {"type": "reference", "name": ["lib", "lamp"]}]
+    extra = {"lib": {"lamp": quad_json([0, 0, 2], [0, 1, 2], [1, 1, 2], [1, 0, 2], emit=[3, 3, 3])},
+             "aggregate": {"type": "bvh", "max_leaf_size": 3},
+             "env": {"type": "native_sky", "top": [1, 1, 1], "bottom": [0.2, 0.2, 0.2]}}
+    h = capi.HostScene(wrap(ents, extra), ".")
+    d = h.desc()
+    assert d.entity_count == 2 and d.aggregate == capi.AGGREGATE_BVH and d.max_leaf_size == 3
+    assert list(entity(d, 1).emit_radiance) == [3, 3, 3]
+    assert d.env.kind == capi.ENV_NATIVE_SKY
+    s = h.session(0)
+    assert (s.width, s.height) == (8, 6) and abs(s.camera.film_aspect - 8 / 6) < 1e-7
+    assert abs(s.camera.fov_rad - math.radians(45)) < 1e-6 and s.filter_radius == 0.5
+
+
+@pytest.mark.parametrize("bad,msg", [
+    ({"type": "geometric", "geometry": {"type": "unknown_geom"}, "material": {"type": "ideal_black"}},
+     "unknown creator type name: unknown_geom"),
+    ({"type": "geometric", "geometry": {"type": "quad", "A": [0, 0], "B": [1, 0, 0], "C": [1, 1, 0], "D": [0, 1, 0]},
+      "material": {"type": "ideal_black"}}, "invalid array size"),
+    ({"type": "geometric", "geometry": {"type": "sphere", "radius": 1},
+      "material": {"type": "disney"}}, "not supported by gpu_pt"),
+])
+def test_errors_carry_the_creation_hierarchy(bad, msg):
+    with pytest.raises(RuntimeError) as ei:
+        capi.HostScene(wrap([bad]), ".")
+    assert msg in str(ei.value) and "in creating object with factory" in str(ei.value)
+
+
+def test_cornell_json_equals_python_scene(tmp_path):
+    """The Cornell scene written as Atrc JSON (+ .bm meshes) and read back by the
+    host yields the same descriptor as the Python-built scene, so the oracle
+    renders identical images from both."""
+    s, cam, tr = scenes.cornell_box()
+    text = scenes.to_atrc_json(s, cam, tr, 24, 24, 2, tmp_path)
+    h = capi.HostScene(text, str(tmp_path))
+    hd = h.desc()
+    pd, keep = s.to_desc()
+    assert hd.entity_count == pd.entity_count
+    for i in range(pd.entity_count):
+        a, b = hd.entities[i], pd.entities[i]
+        assert a.geometry == b.geometry and list(a.emit_radiance) == list(b.emit_radiance)
+        assert [list(x) for x in a.p] == [list(x) for x in b.p]
+        if a.geometry == capi.GEOM_TRIANGLE_BVH:
+            n = a.tri_count * 3 * 8
+            ta = np.ctypeslib.as_array(C.cast(a.tris, C.POINTER(C.c_float)), (n,))
+            tb = np.ctypeslib.as_array(C.cast(b.tris, C.POINTER(C.c_float)), (n,))
+            np.testing.assert_array_equal(ta, tb)
+    p = scenes.render_params(24, 24, 2, cam, seed=3, **tr)
+    ia = po.OracleScene(hd, 32).render(p, threads=4)["image"]
+    ib = po.OracleScene(pd, 32).render(p, threads=4)["image"]
+    np.testing.assert_array_equal(ia, ib)
