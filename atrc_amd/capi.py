@@ -105,7 +105,7 @@ class agx_debug_summary(C.Structure):
 # every symbol the public headers declare
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
-    "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary",
+    "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
 ]
 
 
@@ -140,6 +140,7 @@ def load_library() -> C.CDLL:
     lib.agx_destroy.restype = None
     lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
+    lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
     if lib.agx_abi_version() != AGX_ABI_VERSION:
         raise RuntimeError("libatrc_gpu.so ABI version mismatch")
     _lib = lib
@@ -222,6 +223,12 @@ class Context:
         _check(self.lib.agx_trace_rays(self.h, fptr(o), fptr(d), fptr(tmin), fptr(tmax), n, 1 if any_hit else 0,
                                        iptr(ent), iptr(prim), fptr(t), fptr(u), fptr(v)))
         return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v}
+
+    def copy_kernel(self, nbytes: int) -> float:
+        """One plain copy kernel of nbytes (traffic-counter calibration); device ms."""
+        ms = C.c_double()
+        _check(self.lib.agx_debug_copy_kernel(self.h, nbytes, C.byref(ms)))
+        return ms.value
 
     def stats(self) -> dict:
         s = agx_stats_t()

@@ -60,9 +60,20 @@ struct agx_ctx {
     size_t cap_paths = 0;
     int cap_L = -1;
     int64_t cap_wh = 0;
+    uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
-    uint32_t *h_counters = nullptr;  // pinned
+    uint32_t *h_counters = nullptr;  // pinned ring: RING x 4 queue sizes read back asynchronously
+    static constexpr int RING = 64;
     hipEvent_t ev[8]{};
+    std::vector<hipEvent_t> pool;    // timing / readback events, reused across renders
+    hipEvent_t event(size_t i) {
+        while (pool.size() <= i) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[i];
+    }
     agx_stats_t stats{};
 
     void free_scene() {
@@ -74,6 +85,7 @@ struct agx_ctx {
         for (void *p : wave_allocs) if (p) (void)hipFree(p);
         wave_allocs.clear();
         pb = PathBuffers{};
+        counters = nullptr;
         cap_paths = 0;
         cap_L = -1;
         cap_wh = 0;
@@ -108,7 +120,7 @@ struct agx_ctx {
         HIP_TRY(walloc(&pb.q_shadow, nl));
         HIP_TRY(walloc(&pb.q_in, n));
         HIP_TRY(walloc(&pb.q_out, n));
-        HIP_TRY(walloc(&pb.counters, 4));
+        HIP_TRY(walloc(&counters, 8));
         HIP_TRY(walloc(&pb.film_sum, size_t(10 * wh)));
         HIP_TRY(walloc(&pb.film_comp, size_t(10 * wh)));
         HIP_TRY(walloc(&pb.film_count, size_t(wh)));
@@ -120,16 +132,6 @@ struct agx_ctx {
 };
 
 namespace {
-
-// Reads the three next-bounce queue sizes (synchronises the stream).
-int read_counters(agx_ctx *c, uint32_t out[3]) {
-    HIP_TRY(hipMemcpyAsync(c->h_counters, c->pb.counters, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    out[0] = c->h_counters[0];
-    out[1] = c->h_counters[1];
-    out[2] = c->h_counters[2];
-    return AGX_OK;
-}
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0;
@@ -207,7 +209,8 @@ int agx_create(int device_ordinal, agx_ctx **out) {
         return fail(AGX_E_HIP, "hipStreamCreate failed");
     }
     if (hipMalloc(reinterpret_cast<void **>(&c->dcnt), 2 * sizeof(DevCounters)) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), 4 * sizeof(uint32_t), hipHostMallocDefault) !=
+        hipHostMalloc(reinterpret_cast<void **>(&c->h_counters), agx_ctx::RING * 4 * sizeof(uint32_t),
+                      hipHostMallocDefault) !=
             hipSuccess) {
         agx_destroy(c);
         return fail(AGX_E_OUT_OF_MEMORY, "allocation of counters failed");
@@ -230,6 +233,7 @@ void agx_destroy(agx_ctx *c) {
     if (c->dcnt) (void)hipFree(c->dcnt);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->pool) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -278,6 +282,9 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     ds.n_ent = int32_t(hs.ents.size());
     ds.aggregate = hs.aggregate;
     ds.n_lights = int32_t(hs.lights.size());
+    // stack entries needed: TLAS pending nodes (depth + 1) + BLAS (depth + 1) + 1
+    ds.lds_stack = (hs.max_tlas_depth + 1) + (hs.max_blas_depth + 1) + 1 <= LDS_STACK ? 1 : 0;
+    if (const char *e = std::getenv("AGX_FORCE_PRIVATE_STACK")) if (e[0] == '1') ds.lds_stack = 0;
     ds.env_kind = hs.env_kind;
     for (int i = 0; i < MAX_LIGHTS; ++i) ds.lights[i] = i < ds.n_lights ? hs.lights[size_t(i)] : -2;
     ds.sky_top = hs.sky_top;
@@ -333,48 +340,105 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     if (stats) HIP_TRY(hipMemsetAsync(c->dcnt, 0, 2 * sizeof(DevCounters), c->stream));
     launch_film_clear(c->pb, wh, c->stream);
     HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    // Timing events: (start, end, kind) per launch group; read after the render.
+    struct Timed { size_t a, b; int kind; };
+    std::vector<Timed> timed;
+    size_t next_ev = 0;
+    auto mark = [&](size_t *idx) -> int {
+        hipEvent_t e = c->event(next_ev);
+        if (!e) return fail(AGX_E_HIP, "hipEventCreate failed");
+        *idx = next_ev++;
+        HIP_TRY(hipEventRecord(e, c->stream));
+        return AGX_OK;
+    };
+    // Queue sizes are read back asynchronously into a pinned ring; the host
+    // checks termination two bounces behind, so the GPU never idles waiting.
+    struct Readback { size_t ev; int slot; };
+    std::vector<Readback> reads;
+    int ring_next = 0;
+    auto readback = [&](const uint32_t *dev) -> int {
+        const int slot = ring_next++ % agx_ctx::RING;
+        HIP_TRY(hipMemcpyAsync(c->h_counters + 4 * slot, dev, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        size_t e;
+        if (int rr = mark(&e)) return rr;
+        reads.push_back({e, slot});
+        return AGX_OK;
+    };
+    const uint32_t n_closest_max = uint32_t(2 * per_pass), n_shadow_max = uint32_t(per_pass * std::max(rc.L, 1));
+    const int max_bounces = p->max_depth + std::max(p->specular_depth, 0) + 2;
+    if (max_bounces + 1 > agx_ctx::RING)
+        return fail(AGX_E_UNSUPPORTED, "max_depth + specular_depth too large (at most " +
+                                           std::to_string(agx_ctx::RING - 3) + ")");
     bool cancelled = false;
     for (int64_t base = 0; base < total && !cancelled; base += per_pass) {
         rc.work_base = base;
         rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
-        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-        launch_shade(c->ds, rc, c->pb, 0, c->stream);  // resets the queue counters
+        uint32_t *cin = c->counters, *cout = c->counters + 4;
+        size_t a, b;
+        if ((r = mark(&a))) return r;
+        launch_reset_counters(cout, c->stream);
+        c->pb.cnt_out = cout;
         launch_raygen(c->ds, rc, c->pb, c->stream);
-        HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-        uint32_t q[3];
-        if ((r = read_counters(c, q)) != AGX_OK) return r;
-        st.shade_ms += elapsed(c->ev[2], c->ev[3]);
+        if ((r = mark(&b))) return r;
+        timed.push_back({a, b, 2});
+        if ((r = readback(cout))) return r;
         st.paths += uint64_t(rc.n_paths);
-        while (q[2] > 0) {
+        const size_t first_read = reads.size() - 1;
+        std::swap(cin, cout);
+        std::swap(c->pb.q_in, c->pb.q_out);
+        for (int bounce = 0; bounce < max_bounces; ++bounce) {
+            c->pb.cnt_in = cin;
+            c->pb.cnt_out = cout;
+            size_t e0, e1, e2, e3;
+            if ((r = mark(&e0))) return r;
+            launch_trace_closest(c->ds, c->pb, n_closest_max, stats, c->dcnt, c->stream);
+            if ((r = mark(&e1))) return r;
+            launch_trace_shadow(c->ds, c->pb, n_shadow_max, stats, c->dcnt + 1, c->stream);
+            if ((r = mark(&e2))) return r;
+            launch_reset_counters(cout, c->stream);
+            launch_shade(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
+            if ((r = mark(&e3))) return r;
+            timed.push_back({e0, e1, 0});
+            timed.push_back({e1, e2, 1});
+            timed.push_back({e2, e3, 2});
+            if ((r = readback(cout))) return r;
+            std::swap(cin, cout);
             std::swap(c->pb.q_in, c->pb.q_out);
-            HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-            launch_trace_closest(c->ds, c->pb, q[0], stats, c->dcnt, c->stream);
-            HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-            launch_trace_shadow(c->ds, c->pb, q[1], stats, c->dcnt + 1, c->stream);
-            HIP_TRY(hipEventRecord(c->ev[4], c->stream));
-            launch_shade(c->ds, rc, c->pb, q[2], c->stream);
-            HIP_TRY(hipEventRecord(c->ev[5], c->stream));
+            // the queue produced two bounces ago: if it was empty, everything since is a no-op
+            if (bounce >= 1) {
+                const Readback &rb = reads[first_read + size_t(bounce)];
+                HIP_TRY(hipEventSynchronize(c->pool[rb.ev]));
+                if (c->h_counters[4 * rb.slot + 2] == 0) break;
+            }
+            if (cancel_flag && *cancel_flag) { cancelled = true; break; }
+        }
+        if (cancelled) break;
+        if ((r = mark(&a))) return r;
+        launch_film(rc, c->pb, c->stream);
+        if ((r = mark(&b))) return r;
+        timed.push_back({a, b, 2});
+        HIP_TRY(hipEventSynchronize(c->pool[b]));
+        // sum the ray queue sizes of this pass (every read has completed)
+        for (size_t k = first_read; k < reads.size(); ++k) {
+            const uint32_t *q = c->h_counters + 4 * reads[k].slot;
             st.closest_rays += q[0];
             st.shadow_rays += q[1];
             if (q[0]) st.trace_closest_launches++;
             if (q[1]) st.trace_shadow_launches++;
-            if ((r = read_counters(c, q)) != AGX_OK) return r;
-            st.trace_closest_ms += elapsed(c->ev[2], c->ev[3]);
-            st.trace_shadow_ms += elapsed(c->ev[3], c->ev[4]);
-            st.shade_ms += elapsed(c->ev[4], c->ev[5]);
-            if (cancel_flag && *cancel_flag) { cancelled = true; break; }
         }
-        if (cancelled) break;
-        HIP_TRY(hipEventRecord(c->ev[2], c->stream));
-        launch_film(rc, c->pb, c->stream);
-        HIP_TRY(hipEventRecord(c->ev[3], c->stream));
-        HIP_TRY(hipEventSynchronize(c->ev[3]));
-        st.shade_ms += elapsed(c->ev[2], c->ev[3]);
+        reads.resize(first_read);
+        ring_next = 0;
         if (cancel_flag && *cancel_flag) cancelled = true;
     }
     HIP_TRY(hipEventRecord(c->ev[1], c->stream));
     HIP_TRY(hipEventSynchronize(c->ev[1]));
     st.total_ms = elapsed(c->ev[0], c->ev[1]);
+    for (const Timed &t : timed) {
+        const double ms = elapsed(c->pool[t.a], c->pool[t.b]);
+        if (t.kind == 0) st.trace_closest_ms += ms;
+        else if (t.kind == 1) st.trace_shadow_ms += ms;
+        else st.shade_ms += ms;
+    }
     if (stats) {
         DevCounters hc[2]{};
         HIP_TRY(hipMemcpy(hc, c->dcnt, sizeof(hc), hipMemcpyDeviceToHost));
@@ -480,6 +544,24 @@ int agx_debug_triangle_bvh(const agx_vertex *tris, uint32_t n, uint32_t *node_co
     if (max_depth) *max_depth = uint32_t(depth);
     if (nodes_out) std::memcpy(nodes_out, nodes.data(), nodes.size() * sizeof(vec4));
     if (prims_out) std::memcpy(prims_out, prims.data(), prims.size() * sizeof(vec4));
+    return AGX_OK;
+}
+
+int agx_debug_copy_kernel(agx_ctx *c, uint64_t bytes, double *ms) {
+    if (!c || bytes < 16) return fail(AGX_E_INVALID_ARG, "invalid argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const uint64_t n = bytes / 16;
+    float4 *buf = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&buf), 2 * n * sizeof(float4)));
+    hipError_t e = hipMemsetAsync(buf, 0, n * sizeof(float4), c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+    if (e == hipSuccess) launch_copy_calibration(buf, buf + n, n, c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[1]);
+    if (e == hipSuccess && ms) *ms = elapsed(c->ev[0], c->ev[1]);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(AGX_E_HIP, std::string("copy kernel: ") + hipGetErrorString(e));
     return AGX_OK;
 }
 

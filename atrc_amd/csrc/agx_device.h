@@ -27,6 +27,7 @@ constexpr uint32_t TLAS_LEAF = 0x80000000u;  // TLAS node: left word has this bi
 constexpr int MAX_LIGHTS = 16;
 constexpr int BLAS_STACK = 96;               // >= max BLAS depth + 2 (checked at upload)
 constexpr int TLAS_STACK = 64;
+constexpr int LDS_STACK = 32;                // LDS stack column per thread (TLAS + BLAS entries)
 
 struct DevEntity {
     int32_t kind;          // AGX_GEOM_*
@@ -72,6 +73,7 @@ struct DevScene {
     int32_t aggregate;     // AGX_AGGREGATE_*
     int32_t n_lights;
     int32_t env_kind;
+    int32_t lds_stack;     // 1: TLAS depth + BLAS depth fit the LDS stack column
     int32_t lights[MAX_LIGHTS];
     vec4 sky_top, sky_bottom;
     vec4 world;            // (centre.xyz, radius)

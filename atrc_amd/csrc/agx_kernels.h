@@ -30,7 +30,8 @@ struct PathBuffers {
     uint32_t *q_shadow;  // [N*L] shadow ray indices
     uint32_t *q_in;      // [N] paths to shade this bounce
     uint32_t *q_out;     // [N] paths to shade next bounce
-    uint32_t *counters;  // [4] next-bounce queue sizes: closest, shadow, paths
+    const uint32_t *cnt_in;  // [4] sizes of the queues consumed this bounce: closest, shadow, paths
+    uint32_t *cnt_out;       // [4] sizes of the queues produced for the next bounce
     float *film_sum;     // [10*WH] rgb, albedo, normal, denoise
     float *film_comp;    // [10*WH] Neumaier compensation
     float *film_count;   // [WH]
@@ -61,11 +62,15 @@ struct RenderConst {
 };
 
 void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, hipStream_t st);
-void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+// The launch grids cover the capacity (max_n); each kernel reads its real
+// queue size from pb.cnt_in, so the host never waits between bounces.
+void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                           hipStream_t st);
-void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st);
-void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t n, hipStream_t st);
+void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st);
+void launch_reset_counters(uint32_t *c, hipStream_t st);
+void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st);
 void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st);
 void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st);
 void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,

@@ -229,22 +229,39 @@ struct HitRec {
     int32_t ent, prim;
 };
 
+// Traversal stacks.  TLAS and BLAS share one stack: a BLAS traversal uses the
+// entries above the TLAS entries pending at its entity.  LdsStack keeps one
+// column per thread in LDS ([depth][thread], conflict-free at equal depth);
+// PrivStack is the private-memory fallback for scenes whose BVH depth exceeds
+// the LDS column (chosen at upload).
+struct LdsStack {
+    uint32_t *col;  // &lds[0][threadIdx.x]
+    int top;
+    __device__ __forceinline__ void push(uint32_t v) { col[top * 256] = v; ++top; }
+    __device__ __forceinline__ uint32_t pop() { --top; return col[top * 256]; }
+};
+struct PrivStack {
+    uint32_t s[BLAS_STACK + TLAS_STACK];
+    int top;
+    __device__ __forceinline__ void push(uint32_t v) { s[top++] = v; }
+    __device__ __forceinline__ uint32_t pop() { return s[--top]; }
+};
+
 // UntransformedTriangleBVH::closest_intersection (triangle_bvh.cpp:403-488)
-template <bool STATS>
+template <bool STATS, class Stack>
 __device__ bool blas_closest(const DevScene &s, const DevEntity &e, V o, V d, float tmin, float tmax, HitRec *h,
-                             Counts &c) {
+                             Counts &c, Stack &stk) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
     const uint32_t root = e.node_base;
     float te;
     if (STATS) c.visits++;
     if (!slab(ld(s.blas_nodes[2 * root]), ld(s.blas_nodes[2 * root + 1]), o, inv, tmin, tmax, &te)) return false;
-    uint32_t stack[BLAS_STACK];
-    int top = 0;
-    stack[top++] = root;
+    const int bottom = stk.top;
+    stk.push(root);
     float best = INF_F, bu = 0, bv = 0;
     uint32_t bslot = 0;
-    while (top) {
-        const uint32_t ni = stack[--top];
+    while (stk.top > bottom) {
+        const uint32_t ni = stk.pop();
         const float4 lo = ld(s.blas_nodes[2 * ni]), hi = ld(s.blas_nodes[2 * ni + 1]);
         const uint32_t start = fbits(lo.w), end = fbits(hi.w);
         if (start != INTERIOR) {
@@ -264,12 +281,12 @@ __device__ bool blas_closest(const DevScene &s, const DevEntity &e, V o, V d, fl
             const bool al = slab(ld(s.blas_nodes[2 * l]), ld(s.blas_nodes[2 * l + 1]), o, inv, tmin, tmax, &tl);
             const bool ar = slab(ld(s.blas_nodes[2 * r]), ld(s.blas_nodes[2 * r + 1]), o, inv, tmin, tmax, &tr);
             if (al && ar) {
-                if (tl < tr) { stack[top++] = r; stack[top++] = l; }
-                else { stack[top++] = l; stack[top++] = r; }
+                if (tl < tr) { stk.push(r); stk.push(l); }
+                else { stk.push(l); stk.push(r); }
             } else if (al) {
-                stack[top++] = l;
+                stk.push(l);
             } else if (ar) {
-                stack[top++] = r;
+                stk.push(r);
             }
         }
     }
@@ -279,31 +296,31 @@ __device__ bool blas_closest(const DevScene &s, const DevEntity &e, V o, V d, fl
 }
 
 // UntransformedTriangleBVH::has_intersection (triangle_bvh.cpp:363-401)
-template <bool STATS>
-__device__ bool blas_any(const DevScene &s, const DevEntity &e, V o, V d, float tmin, float tmax, Counts &c) {
+template <bool STATS, class Stack>
+__device__ bool blas_any(const DevScene &s, const DevEntity &e, V o, V d, float tmin, float tmax, Counts &c,
+                         Stack &stk) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
     const uint32_t root = e.node_base;
     float te;
     if (STATS) c.visits++;
     if (!slab(ld(s.blas_nodes[2 * root]), ld(s.blas_nodes[2 * root + 1]), o, inv, tmin, tmax, &te)) return false;
-    uint32_t stack[BLAS_STACK];
-    int top = 0;
-    stack[top++] = root;
-    while (top) {
-        const uint32_t ni = stack[--top];
+    const int bottom = stk.top;
+    stk.push(root);
+    while (stk.top > bottom) {
+        const uint32_t ni = stk.pop();
         const float4 lo = ld(s.blas_nodes[2 * ni]), hi = ld(s.blas_nodes[2 * ni + 1]);
         const uint32_t start = fbits(lo.w), end = fbits(hi.w);
         if (start != INTERIOR) {
             for (uint32_t i = start; i < end; ++i) {
                 if (STATS) c.tests++;
                 const vec4 p0 = s.blas_prims[3 * i], p1 = s.blas_prims[3 * i + 1], p2 = s.blas_prims[3 * i + 2];
-                if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) return true;
+                if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) { stk.top = bottom; return true; }
             }
         } else {
             const uint32_t l = ni + 1, r = end;
             if (STATS) c.visits += 2;
-            if (slab(ld(s.blas_nodes[2 * l]), ld(s.blas_nodes[2 * l + 1]), o, inv, tmin, tmax, &te)) stack[top++] = l;
-            if (slab(ld(s.blas_nodes[2 * r]), ld(s.blas_nodes[2 * r + 1]), o, inv, tmin, tmax, &te)) stack[top++] = r;
+            if (slab(ld(s.blas_nodes[2 * l]), ld(s.blas_nodes[2 * l + 1]), o, inv, tmin, tmax, &te)) stk.push(l);
+            if (slab(ld(s.blas_nodes[2 * r]), ld(s.blas_nodes[2 * r + 1]), o, inv, tmin, tmax, &te)) stk.push(r);
         }
     }
     return false;
@@ -317,9 +334,9 @@ __device__ __forceinline__ void sphere_abc(V o, V d, float r, float *A, float *B
 }
 
 // Closest hit of one entity; on success *tmax is lowered to the hit distance.
-template <bool STATS>
+template <bool STATS, class Stack>
 __device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tmin, float *tmax, HitRec *h,
-                               Counts &c) {
+                               Counts &c, Stack &stk) {
     const DevEntity &e = s.ents[ei];
     switch (e.kind) {
     case AGX_GEOM_QUAD: {  // quad.cpp:30-58
@@ -371,7 +388,7 @@ __device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tm
         return true;
     }
     default: {
-        if (!blas_closest<STATS>(s, e, o, d, tmin, *tmax, h, c)) return false;
+        if (!blas_closest<STATS>(s, e, o, d, tmin, *tmax, h, c, stk)) return false;
         h->ent = ei;
         *tmax = h->t;
         return true;
@@ -379,8 +396,8 @@ __device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tm
     }
 }
 
-template <bool STATS>
-__device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, float tmax, Counts &c) {
+template <bool STATS, class Stack>
+__device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, float tmax, Counts &c, Stack &stk) {
     const DevEntity &e = s.ents[ei];
     switch (e.kind) {
     case AGX_GEOM_QUAD:  // quad.cpp:24-28
@@ -411,7 +428,7 @@ __device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, 
         return p.x * p.x + p.y * p.y <= e.a.y;
     }
     default:
-        return blas_any<STATS>(s, e, o, d, tmin, tmax, c);
+        return blas_any<STATS>(s, e, o, d, tmin, tmax, c, stk);
     }
 }
 
@@ -423,59 +440,57 @@ __device__ __forceinline__ bool tlas_box(float4 lo, float4 hi, V o, V inv, float
 
 // Scene closest hit: NativeAggregate (native_aggregate.cpp:31-45) or EntityBVH
 // (entity_bvh.cpp:141-172, left subtree before right, node bound tested on entry).
-template <bool STATS>
-__device__ bool scene_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c) {
+template <bool STATS, class Stack>
+__device__ bool scene_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c, Stack &stk) {
     bool hit = false;
+    stk.top = 0;
     if (s.aggregate == AGX_AGGREGATE_BVH) {
         const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
-        uint32_t stack[TLAS_STACK];
-        int top = 0;
-        stack[top++] = 0;
-        while (top) {
-            const uint32_t ni = stack[--top];
+        stk.push(0);
+        while (stk.top > 0) {
+            const uint32_t ni = stk.pop();
             const float4 lo = ld(s.tlas_nodes[2 * ni]), hi = ld(s.tlas_nodes[2 * ni + 1]);
             if (STATS) c.visits++;
             if (!tlas_box(lo, hi, o, inv, tmin, tmax)) continue;
             const uint32_t a = fbits(lo.w), b = fbits(hi.w);
             if (a & TLAS_LEAF) {
                 for (uint32_t i = a & ~TLAS_LEAF; i < b; ++i)
-                    hit = entity_closest<STATS>(s, s.tlas_prims[i], o, d, tmin, &tmax, h, c) || hit;
+                    hit = entity_closest<STATS>(s, s.tlas_prims[i], o, d, tmin, &tmax, h, c, stk) || hit;
             } else {
-                stack[top++] = b;
-                stack[top++] = a;
+                stk.push(b);
+                stk.push(a);
             }
         }
     } else {
-        for (int32_t i = 0; i < s.n_ent; ++i) hit = entity_closest<STATS>(s, i, o, d, tmin, &tmax, h, c) || hit;
+        for (int32_t i = 0; i < s.n_ent; ++i) hit = entity_closest<STATS>(s, i, o, d, tmin, &tmax, h, c, stk) || hit;
     }
     return hit;
 }
 
-template <bool STATS>
-__device__ bool scene_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c) {
+template <bool STATS, class Stack>
+__device__ bool scene_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c, Stack &stk) {
+    stk.top = 0;
     if (s.aggregate == AGX_AGGREGATE_BVH) {
         const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
-        uint32_t stack[TLAS_STACK];
-        int top = 0;
-        stack[top++] = 0;
-        while (top) {
-            const uint32_t ni = stack[--top];
+        stk.push(0);
+        while (stk.top > 0) {
+            const uint32_t ni = stk.pop();
             const float4 lo = ld(s.tlas_nodes[2 * ni]), hi = ld(s.tlas_nodes[2 * ni + 1]);
             if (STATS) c.visits++;
             if (!tlas_box(lo, hi, o, inv, tmin, tmax)) continue;
             const uint32_t a = fbits(lo.w), b = fbits(hi.w);
             if (a & TLAS_LEAF) {
                 for (uint32_t i = a & ~TLAS_LEAF; i < b; ++i)
-                    if (entity_any<STATS>(s, s.tlas_prims[i], o, d, tmin, tmax, c)) return true;
+                    if (entity_any<STATS>(s, s.tlas_prims[i], o, d, tmin, tmax, c, stk)) return true;
             } else {
-                stack[top++] = b;
-                stack[top++] = a;
+                stk.push(b);
+                stk.push(a);
             }
         }
         return false;
     }
     for (int32_t i = 0; i < s.n_ent; ++i)
-        if (entity_any<STATS>(s, i, o, d, tmin, tmax, c)) return true;
+        if (entity_any<STATS>(s, i, o, d, tmin, tmax, c, stk)) return true;
     return false;
 }
 
@@ -975,8 +990,11 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
-template <bool STATS>
-__global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers pb, uint32_t n, DevCounters *cnt) {
+// Stack selection: USE_LDS = the LDS column (LDS_STACK entries per thread).
+template <bool STATS, bool USE_LDS>
+__global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
+    const uint32_t n = pb.cnt_in[0];
+    if (blockIdx.x * blockDim.x >= n) return;  // whole block past the queue
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
@@ -985,7 +1003,14 @@ __global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers p
         const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
         HitRec h;
         h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-        found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c);
+        if (USE_LDS) {
+            __shared__ uint32_t lds[LDS_STACK][256];
+            LdsStack stk{&lds[0][threadIdx.x], 0};
+            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
+        } else {
+            PrivStack stk;
+            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
+        }
         if (!found) h.ent = -1;
         pb.hit[ri] = make_float4(h.t, h.u, h.v, __int_as_float(h.ent));
         pb.hit_prim[ri] = h.prim;
@@ -993,15 +1018,24 @@ __global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers p
     if (STATS) flush_counts(c, found, cnt);
 }
 
-template <bool STATS>
-__global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, uint32_t n, DevCounters *cnt) {
+template <bool STATS, bool USE_LDS>
+__global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
+    const uint32_t n = pb.cnt_in[1];
+    if (blockIdx.x * blockDim.x >= n) return;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool occ = false;
     if (tid < n) {
         const uint32_t ri = pb.q_shadow[tid];
         const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
-        occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c);
+        if (USE_LDS) {
+            __shared__ uint32_t lds[LDS_STACK][256];
+            LdsStack stk{&lds[0][threadIdx.x], 0};
+            occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c, stk);
+        } else {
+            PrivStack stk;
+            occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c, stk);
+        }
         pb.occluded[ri] = occ ? 1 : 0;
     }
     if (STATS) flush_counts(c, false, cnt);
@@ -1047,9 +1081,9 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         flags |= PF_CONT;
         pb.ray_o[2 * p] = f4(o, 0.0f);
         pb.ray_d[2 * p] = f4(d, INF_F);
-        const uint32_t qi = atomicAdd(&pb.counters[2], 1u);
+        const uint32_t qi = atomicAdd(&pb.cnt_out[2], 1u);
         pb.q_out[qi] = uint32_t(p);
-        const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
         pb.q_closest[ci] = uint32_t(2 * p);
     }
     pb.coef[p] = coef;
@@ -1087,7 +1121,8 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     return coef * (f / (pdf + lpdf));
 }
 
-__global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb, uint32_t n) {
+__global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
+    const uint32_t n = pb.cnt_in[2];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= n) return;
     const uint32_t p = pb.q_in[tid];
@@ -1191,7 +1226,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             pending = true;
             pb.sh_o[slot] = f4(so, tmax);
             pb.sh_d[slot] = f4(sd, tmin);
-            const uint32_t qi = atomicAdd(&pb.counters[1], 1u);
+            const uint32_t qi = atomicAdd(&pb.cnt_out[1], 1u);
             pb.q_shadow[qi] = uint32_t(slot);
         }
     }
@@ -1207,7 +1242,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             pb.mis_b[p] = f4(bs.f, bs.pdf);
             flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u);
             pending = true;
-            const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+            const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
             pb.q_closest[ci] = 2 * p + 1;
         }
     }
@@ -1236,11 +1271,11 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     if (cont) {
         flags |= PF_CONT;
-        const uint32_t ci = atomicAdd(&pb.counters[0], 1u);
+        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
         pb.q_closest[ci] = 2 * p;
     }
     if (flags & (PF_CONT | PF_PENDING)) {
-        const uint32_t qi = atomicAdd(&pb.counters[2], 1u);
+        const uint32_t qi = atomicAdd(&pb.cnt_out[2], 1u);
         pb.q_out[qi] = p;
     }
     pb.coef[p] = f4(coef, 0);
@@ -1305,7 +1340,7 @@ __global__ void k_reset_counters(uint32_t *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
-template <bool STATS>
+template <bool STATS, bool USE_LDS>
 __global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
                                                     int any_hit, int32_t *ent, int32_t *prim, float *t, float *u,
                                                     float *v, DevCounters *cnt) {
@@ -1315,13 +1350,27 @@ __global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o,
     if (i < n) {
         const float4 o4 = o[i], d4 = d[i];
         if (any_hit) {
-            found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c);
+            if (USE_LDS) {
+                __shared__ uint32_t lds[LDS_STACK][256];
+                LdsStack stk{&lds[0][threadIdx.x], 0};
+                found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c, stk);
+            } else {
+                PrivStack stk;
+                found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c, stk);
+            }
             ent[i] = found ? 1 : -1;
             found = false;
         } else {
             HitRec h;
             h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c);
+            if (USE_LDS) {
+                __shared__ uint32_t lds[LDS_STACK][256];
+                LdsStack stk{&lds[0][threadIdx.x], 0};
+                found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
+            } else {
+                PrivStack stk;
+                found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
+            }
             if (found) {
                 const DevEntity &e = s.ents[h.ent];
                 int32_t pid = h.prim;
@@ -1335,6 +1384,13 @@ __global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o,
     if (STATS) flush_counts(c, found, cnt);
 }
 
+// plain streaming copy, used to calibrate HBM traffic counters
+__global__ void __launch_bounds__(256) k_copy_calibration(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                          uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 inline unsigned grid_of(uint64_t n, unsigned b) { return unsigned((n + b - 1) / b); }
 
 }  // namespace
@@ -1344,22 +1400,36 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
     if (rc.n_paths <= 0) return;
     hipLaunchKernelGGL(k_raygen, dim3(grid_of(uint64_t(rc.n_paths), 256)), dim3(256), 0, st, s, rc, pb);
 }
-void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+#define AGX_LAUNCH_STACKED(K, grid, st, ...)                                                        \
+    do {                                                                                          \
+        if (s.lds_stack) {                                                                        \
+            if (stats) hipLaunchKernelGGL((K<true, true>), grid, dim3(256), 0, st, __VA_ARGS__);   \
+            else hipLaunchKernelGGL((K<false, true>), grid, dim3(256), 0, st, __VA_ARGS__);        \
+        } else {                                                                                  \
+            if (stats) hipLaunchKernelGGL((K<true, false>), grid, dim3(256), 0, st, __VA_ARGS__);  \
+            else hipLaunchKernelGGL((K<false, false>), grid, dim3(256), 0, st, __VA_ARGS__);       \
+        }                                                                                         \
+    } while (0)
+
+void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                           hipStream_t st) {
-    if (!n) return;
-    if (stats) hipLaunchKernelGGL(k_trace_closest<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
-    else hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+    if (!max_n) return;
+    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, 256)), st, s, pb, cnt);
 }
-void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t n, bool stats, DevCounters *cnt,
+void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st) {
-    if (!n) return;
-    if (stats) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
-    else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, pb, n, cnt);
+    if (!max_n) return;
+    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, 256)), st, s, pb, cnt);
 }
-void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t n, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, pb.counters);
-    if (!n) return;
-    hipLaunchKernelGGL(k_shade, dim3(grid_of(n, 256)), dim3(256), 0, st, s, rc, pb, n);
+void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
+    if (!max_n) return;
+    hipLaunchKernelGGL(k_shade, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+}
+void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, st, src, dst, n);
+}
+void launch_reset_counters(uint32_t *c, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c);
 }
 void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st) {
     const int64_t wh = int64_t(rc.W) * rc.H;
@@ -1369,7 +1439,6 @@ void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st) {
 }
 void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st) {
     hipLaunchKernelGGL(k_film_clear, dim3(grid_of(uint64_t(wh), 256)), dim3(256), 0, st, pb, wh);
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, pb.counters);
 }
 void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,
                          hipStream_t st) {
@@ -1379,12 +1448,8 @@ void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *a
 void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {
     if (!n) return;
-    if (cnt)
-        hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, o, d, n, any_hit, ent, prim,
-                           t, u, v, cnt);
-    else
-        hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid_of(n, 256)), dim3(256), 0, st, s, o, d, n, any_hit, ent,
-                           prim, t, u, v, cnt);
+    const bool stats = cnt != nullptr;
+    AGX_LAUNCH_STACKED(k_trace_rays, dim3(grid_of(n, 256)), st, s, o, d, n, any_hit, ent, prim, t, u, v, cnt);
 }
 
 }  // namespace agx

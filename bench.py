@@ -89,6 +89,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stats-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration-mb", type=int, default=256,
+                    help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
@@ -106,6 +108,12 @@ def main() -> int:
     ctx.upload_scene(desc)
     log(f"[bench] scene uploaded (BVH build + copy) in {time.time() - t0:.1f} s")
 
+    copy = None
+    if args.copy_calibration_mb > 0:
+        nbytes = args.copy_calibration_mb << 20
+        ms = ctx.copy_kernel(nbytes)
+        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}
+        log(f"[bench] copy calibration kernel: {nbytes >> 20} MiB in {ms:.3f} ms ({copy['GB/s']:.0f} GB/s r+w)")
     # warmup
     if args.warmup > 0:
         ctx.render(scenes.render_params(W, H, args.warmup, cam, seed=42, **tr))
@@ -153,6 +161,8 @@ def main() -> int:
         "device_ms": {k: st[k] for k in ("total_ms", "trace_closest_ms", "trace_shadow_ms", "shade_ms")},
         "roofline": roofline,
         "guided_vs_plain_relmse_ratio": None,
+        "copy_calibration": copy,
+        "traversal_stack": "private" if os.environ.get("AGX_FORCE_PRIVATE_STACK") == "1" else "auto (LDS when it fits)",
     }
     ctx.close()
     if not args.no_cpu_baseline:

@@ -31,6 +31,11 @@ int agx_debug_triangle_bvh(const agx_vertex *tris, uint32_t n, uint32_t *node_co
 /* Build (without uploading) the device scene and report its sizes. */
 int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out);
 
+/* Run one plain device-to-device copy kernel (k_copy_calibration) of `bytes`
+ * bytes on the context's stream and wait for it: a known-size reference for
+ * calibrating hardware traffic counters.  Returns its device time in *ms. */
+int agx_debug_copy_kernel(agx_ctx *ctx, uint64_t bytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif
