@@ -22,7 +22,10 @@ AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_C
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
 
 GEOM_TRIANGLE_BVH, GEOM_QUAD, GEOM_TRIANGLE, GEOM_SPHERE, GEOM_DISK = range(5)
-MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK = range(5)
+MEDIUM_VOID, MEDIUM_HOMOGENEOUS, MEDIUM_HETEROGENEOUS = range(3)
+TEX3D_CONSTANT, TEX3D_GRID = range(2)
+WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
+MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE = range(6)
 ENV_NONE, ENV_NATIVE_SKY = 0, 1
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
 
@@ -50,6 +53,22 @@ class agx_material_desc(C.Structure):
     ]
 
 
+class agx_texture3d_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("linear", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+        ("depth", C.c_int32), ("channels", C.c_int32), ("data", C.POINTER(C.c_float)), ("value", F3),
+        ("transform", C.c_float * 12), ("wrap", C.c_int32 * 3), ("inv_gamma", C.c_float),
+    ]
+
+
+class agx_medium_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("max_scattering_count", C.c_int32), ("sigma_a", F3), ("sigma_s", F3), ("g", C.c_float),
+        ("local_to_world", C.c_float * 12), ("world_to_local", C.c_float * 12), ("density", agx_texture3d_desc),
+        ("albedo", agx_texture3d_desc), ("g3", agx_texture3d_desc), ("white_for_indirect", C.c_int32),
+    ]
+
+
 class agx_env_desc(C.Structure):
     _fields_ = [("kind", C.c_int32), ("top", F3), ("bottom", F3), ("power", C.c_float)]
 
@@ -65,7 +84,8 @@ class agx_scene_desc(C.Structure):
     _fields_ = [
         ("abi_version", C.c_uint32), ("entity_count", C.c_uint32), ("entities", C.POINTER(agx_entity_desc)),
         ("material_count", C.c_uint32), ("materials", C.POINTER(agx_material_desc)), ("env", agx_env_desc),
-        ("aggregate", C.c_int32), ("max_leaf_size", C.c_int32),
+        ("aggregate", C.c_int32), ("max_leaf_size", C.c_int32), ("medium_count", C.c_uint32),
+        ("media", C.POINTER(agx_medium_desc)),
     ]
 
 

@@ -330,12 +330,76 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         }
         case AGX_MAT_IDEAL_BLACK:
             break;
+        case AGX_MAT_INVISIBLE_SURFACE:  // invisible_surface.cpp:47-50: albedo 1
+            d.albedo = vec4{1, 1, 1, 0};
+            break;
         default:
             *err = "unsupported material kind " + std::to_string(m.kind);
             return AGX_E_UNSUPPORTED;
         }
         S.mats.push_back(d);
     }
+
+    // media (medium_creators.cpp; homogeneous.cpp:23-41; heterogeneous.cpp:27-45)
+    if (desc.medium_count && !desc.media) { *err = "media is NULL"; return AGX_E_INVALID_ARG; }
+    if (desc.medium_count > 4096) { *err = "at most 4096 media"; return AGX_E_LIMIT; }
+    for (uint32_t i = 0; i < desc.medium_count; ++i) {
+        const agx_medium_desc &m = desc.media[i];
+        DevMedium dm;
+        std::memset(&dm, 0, sizeof(dm));
+        dm.kind = m.kind;
+        dm.max_scat = m.max_scattering_count;
+        dm.white_for_indirect = m.white_for_indirect;
+        if (m.kind == AGX_MEDIUM_HOMOGENEOUS) {
+            if (!(m.g > -1 && m.g < 1)) { *err = "homogeneous medium: invalid g value"; return AGX_E_INVALID_ARG; }
+            dm.sa = vec4{m.sigma_a[0], m.sigma_a[1], m.sigma_a[2], 0};
+            dm.ss = vec4{m.sigma_s[0], m.sigma_s[1], m.sigma_s[2], 0};
+            dm.st = vec4{m.sigma_a[0] + m.sigma_s[0], m.sigma_a[1] + m.sigma_s[1], m.sigma_a[2] + m.sigma_s[2], 0};
+            dm.misc = vec4{m.g, 0, 0, 0};
+        } else if (m.kind == AGX_MEDIUM_HETEROGENEOUS) {
+            set_rows(dm.w2l, m.world_to_local);
+            auto tex = [&](const agx_texture3d_desc &t, DevTex3D *o, float *max_real) -> bool {
+                o->kind = t.kind;
+                o->linear = t.linear;
+                o->inv_gamma = t.inv_gamma > 0 ? t.inv_gamma : 1.0f;
+                for (int k = 0; k < 3; ++k) o->wrap[k] = t.wrap[k];
+                o->value = vec4{t.value[0], t.value[1], t.value[2], 0};
+                set_rows(o->xf, t.transform);
+                // max_real is taken over the stored texels, before inv_gamma
+                // (image3d.cpp:108-147); a constant texture ignores inv_gamma (constant3d.cpp:43-58)
+                if (t.kind == AGX_TEX3D_CONSTANT) {
+                    o->w = o->h = o->d = o->ch = 1;
+                    if (max_real) *max_real = t.value[0];
+                    return true;
+                }
+                if (t.kind != AGX_TEX3D_GRID || !t.data || t.width <= 0 || t.height <= 0 || t.depth <= 0 ||
+                    (t.channels != 1 && t.channels != 3))
+                    return false;
+                o->w = t.width; o->h = t.height; o->d = t.depth; o->ch = t.channels;
+                o->offset = uint32_t(S.grids.size());
+                const size_t n = size_t(t.width) * size_t(t.height) * size_t(t.depth) * size_t(t.channels);
+                S.grids.insert(S.grids.end(), t.data, t.data + n);
+                if (max_real) {
+                    float mx = -std::numeric_limits<float>::max();
+                    for (size_t k = 0; k < n; k += size_t(t.channels)) mx = std::max(mx, t.data[k]);
+                    *max_real = mx;
+                }
+                return true;
+            };
+            float maxd = 0;
+            if (!tex(m.density, &dm.density, &maxd) || !tex(m.albedo, &dm.albedo, nullptr) || !tex(m.g3, &dm.g3, nullptr)) {
+                *err = "heterogeneous medium: invalid 3d texture";
+                return AGX_E_INVALID_ARG;
+            }
+            maxd = std::max(maxd, 3e-4f);
+            dm.misc = vec4{0, maxd, 1 / maxd, 0};
+        } else if (m.kind != AGX_MEDIUM_VOID) {
+            *err = "unknown medium kind";
+            return AGX_E_INVALID_ARG;
+        }
+        S.media.push_back(dm);
+    }
+    if (S.grids.size() > size_t(0xffffffffu)) { *err = "3d grids too large"; return AGX_E_LIMIT; }
 
     std::vector<Box> bounds(desc.entity_count);
     for (uint32_t i = 0; i < desc.entity_count; ++i) {
@@ -350,10 +414,12 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
             *err = "entity " + std::to_string(i) + ": material index out of range";
             return AGX_E_INVALID_ARG;
         }
-        if (e.medium_in >= 0 || e.medium_out >= 0) {
-            *err = "entity " + std::to_string(i) + ": participating media are not supported by gpu_pt yet";
-            return AGX_E_UNSUPPORTED;
+        if (e.medium_in >= int32_t(desc.medium_count) || e.medium_out >= int32_t(desc.medium_count)) {
+            *err = "entity " + std::to_string(i) + ": medium index out of range";
+            return AGX_E_INVALID_ARG;
         }
+        d.med_in = e.medium_in < 0 ? -1 : e.medium_in;
+        d.med_out = e.medium_out < 0 ? -1 : e.medium_out;
         set_rows(d.l2w, e.local_to_world);
         set_rows(d.w2l, e.world_to_local);
         const float ratio = len3(apply_vec(e.local_to_world, F3{1, 0, 0}));

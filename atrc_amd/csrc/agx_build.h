@@ -16,6 +16,8 @@ struct HostScene {
     std::vector<int32_t> tlas_prims, alias_idx;
     std::vector<float> alias_prob;
     std::vector<int32_t> lights;
+    std::vector<DevMedium> media;
+    std::vector<float> grids;
     int32_t aggregate = AGX_AGGREGATE_NATIVE;
     int32_t env_kind = AGX_ENV_NONE;
     vec4 sky_top{0, 0, 0, 0}, sky_bottom{0, 0, 0, 0}, world{0, 0, 0, 1};

@@ -34,6 +34,8 @@ struct DevEntity {
     int32_t material;
     int32_t light;         // -1 or index into the light list
     int32_t flags;         // bit 0: no_denoise
+    int32_t med_in, med_out;  // media on the -z / +z side of the geometric normal, -1 = void
+    int32_t pad_m0, pad_m1;
     vec4 a;                // quad/triangle: a.xyz ; sphere/disk: (radius, radius^2, world_radius, ratio)
     vec4 b_a, c_a, d_a;    // edges (quad: world space; triangle: local space)
     vec4 x_abc, x_acd, z;  // geometric frame vectors
@@ -45,7 +47,30 @@ struct DevEntity {
     uint32_t node_base, prim_base, prim_count, alias_base;  // mesh ranges in the global arrays
     vec4 sample_prob;      // (quad: P(ABC), 0, 0, 0)
 };
-static_assert(sizeof(DevEntity) == 304, "DevEntity layout");
+static_assert(sizeof(DevEntity) == 320, "DevEntity layout");
+
+// Texture3D (constant or grid); grids live in DevScene::grids
+struct DevTex3D {
+    int32_t kind, linear, w, h;
+    int32_t d, ch;
+    uint32_t offset;       // first float of the grid in DevScene::grids
+    float inv_gamma;
+    int32_t wrap[3];
+    int32_t pad;
+    vec4 value;
+    vec4 xf[3];            // rows of the uvw transform
+};
+
+struct DevMedium {
+    int32_t kind;          // AGX_MEDIUM_*
+    int32_t max_scat;
+    int32_t white_for_indirect;
+    int32_t pad;
+    vec4 sa, ss, st;       // homogeneous coefficients (xyz)
+    vec4 misc;             // (g, max density, 1 / max density, 0)
+    vec4 w2l[3];           // heterogeneous: world -> unit cube
+    DevTex3D density, albedo, g3;
+};
 
 struct DevMaterial {
     int32_t kind;          // AGX_MAT_*
@@ -69,6 +94,8 @@ struct DevScene {
     const int32_t *tlas_prims;
     const float *alias_prob;
     const int32_t *alias_idx;
+    const DevMedium *media;
+    const float *grids;
     int32_t n_ent;
     int32_t aggregate;     // AGX_AGGREGATE_*
     int32_t n_lights;

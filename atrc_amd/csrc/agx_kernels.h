@@ -13,8 +13,9 @@ struct PathBuffers {
     float4 *radiance;    // [N] accumulated radiance
     float4 *gb_albedo;   // [N] (albedo.xyz, denoise)
     float4 *gb_normal;   // [N]
-    unsigned long long *rng;  // [N] PCG32 state (the increment is derived from the path id)
-    int2 *depth;         // [N] (depth, s_depth)
+    unsigned long long *rng;     // [N] PCG32 state (the increment is derived from the path id)
+    unsigned long long *rng_tr;  // [N] second stream: ratio-tracking transmittance draws
+    int4 *depth;         // [N] (depth, s_depth, medium scattering count, 0)
     uint32_t *flags;     // [N] PF_* bits
     float4 *mis_a;       // [N] MIS-BSDF record: (coef.xyz, |cos|)
     float4 *mis_b;       // [N]                  (f.xyz, pdf)
@@ -43,7 +44,10 @@ enum : uint32_t {
     PF_PENDING = 4u,     // NEE / MIS contributions await resolution
     PF_KEEP = 8u,        // the camera sample lands inside the box filter
     PF_MIS_DELTA = 16u,  // the MIS-BSDF sample was a delta lobe
+    PF_MIS_MED_SHIFT = 16u,  // bits 16..31: 1 + medium index the MIS-BSDF ray travels through (0 = void)
 };
+// key of the transmittance stream (initstate = splitmix(seed ^ key ^ splitmix(seq)))
+constexpr unsigned long long TR_STREAM_KEY = 0xA5A5F00DC0FFEE11ull;
 
 struct RenderConst {
     int32_t W, H;

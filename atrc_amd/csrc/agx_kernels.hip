@@ -906,6 +906,15 @@ __device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, f
     r.pdf = 0;
     r.delta = false;
     if (b.kind == AGX_MAT_IDEAL_BLACK) return r;
+    if (b.kind == AGX_MAT_INVISIBLE_SURFACE) {  // invisible_surface.cpp:26-31
+        const float cv = fabsf(cosv(b.geo.z, wo));
+        const float d = cv < eps ? 1.0f : cv;
+        r.dir = mk(-wo.x, -wo.y, -wo.z);
+        r.f = mk(1.0f / d, 1.0f / d, 1.0f / d);
+        r.pdf = 1;
+        r.delta = true;
+        return r;
+    }
     if (b.kind == AGX_MAT_MIRROR) {
         const V lo = to_local(b.shd, wo);
         if (lo.z <= 0) return r;
@@ -1071,7 +1080,8 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     pb.radiance[p] = make_float4(0, 0, 0, 0);
     pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
     pb.gb_normal[p] = make_float4(0, 0, 0, 0);
-    pb.depth[p] = make_int2(1, 1);
+    pb.depth[p] = make_int4(1, 1, 0, 0);
+    pb.rng_tr[p] = rng_seed(rc.seed ^ TR_STREAM_KEY, path_seq((unsigned long long)pixel, (unsigned long long)sample)).state;
     bool alive = true;
     if (1 > rc.min_depth) {  // path_tracing.cpp:30-35 at depth 1
         if (rng.uniform() > rc.cont_prob) alive = false;
@@ -1091,8 +1101,180 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     pb.rng[p] = rng.state;
 }
 
-// resolve the MIS-BSDF ray of the previous vertex (direct_illum.cpp:130-192)
-__device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, uint32_t flags, float eps) {
+// ---------------------------------------------------------------- participating media
+// Texture3D lookup: image3d = transform, wrap per axis, nearest or trilinear over
+// texel centres, ^inv_gamma (core/texture3d.h:208-233); constant3d returns its
+// texel unchanged (constant3d.cpp:48-58).
+__device__ __forceinline__ float wrap3(int mode, float x) {
+    if (mode == AGX_WRAP_REPEAT) return smin(1.0f, smax(0.0f, x - floorf(x)));
+    if (mode == AGX_WRAP_MIRROR) {
+        const int ip = int(x >= 0 ? x : floorf(x));
+        const float v = (ip & 1) ? 1 - x + float(ip) : x - float(ip);
+        return smin(1.0f, smax(0.0f, v));
+    }
+    return smin(1.0f, smax(0.0f, x));
+}
+__device__ __forceinline__ V texel3(const DevScene &s, const DevTex3D &t, int x, int y, int z) {
+    const float *g = s.grids + t.offset + ((size_t(z) * size_t(t.h) + size_t(y)) * size_t(t.w) + size_t(x)) * size_t(t.ch);
+    if (t.ch == 1) return mk(g[0], g[0], g[0]);
+    return mk(g[0], g[1], g[2]);
+}
+__device__ V tex3d(const DevScene &s, const DevTex3D &t, V uvw) {
+    if (t.kind == AGX_TEX3D_CONSTANT) return mk(t.value);
+    V r;
+    {
+        const V q = xf_point(t.xf, uvw);
+        const float u = wrap3(t.wrap[0], q.x), v = wrap3(t.wrap[1], q.y), w = wrap3(t.wrap[2], q.z);
+        if (!t.linear) {
+            const int x = min(t.w - 1, int(u * float(t.w))), y = min(t.h - 1, int(v * float(t.h)));
+            const int z = min(t.d - 1, int(w * float(t.d)));
+            r = texel3(s, t, x, y, z);
+        } else {
+            const float fu = u * float(t.w) - 0.5f, fv = v * float(t.h) - 0.5f, fw = w * float(t.d) - 0.5f;
+            const float bu = floorf(fu), bv = floorf(fv), bw = floorf(fw);
+            const float du = fu - bu, dv = fv - bv, dw = fw - bw;
+            const int x0 = min(t.w - 1, max(0, int(bu))), x1 = min(t.w - 1, max(0, int(bu) + 1));
+            const int y0 = min(t.h - 1, max(0, int(bv))), y1 = min(t.h - 1, max(0, int(bv) + 1));
+            const int z0 = min(t.d - 1, max(0, int(bw))), z1 = min(t.d - 1, max(0, int(bw) + 1));
+            auto lerp = [](V a, V b, float k) { return a + (b - a) * k; };
+            const V c00 = lerp(texel3(s, t, x0, y0, z0), texel3(s, t, x1, y0, z0), du);
+            const V c10 = lerp(texel3(s, t, x0, y1, z0), texel3(s, t, x1, y1, z0), du);
+            const V c01 = lerp(texel3(s, t, x0, y0, z1), texel3(s, t, x1, y0, z1), du);
+            const V c11 = lerp(texel3(s, t, x0, y1, z1), texel3(s, t, x1, y1, z1), du);
+            r = lerp(lerp(c00, c10, dv), lerp(c01, c11, dv), dw);
+        }
+    }
+    if (t.inv_gamma != 1.0f) r = mk(powf(r.x, t.inv_gamma), powf(r.y, t.inv_gamma), powf(r.z, t.inv_gamma));
+    return r;
+}
+
+// Henyey-Greenstein phase function (utility/phase_function.h:8-90)
+__device__ __forceinline__ float hg_phase(float g, float u) {
+    const float g2 = g * g;
+    const float dem = 1 + g2 - 2 * g * u;
+    return (1 - g2) / (4 * PI_F * dem * sqrtf(dem));
+}
+__device__ __forceinline__ float hg_eval(float g, V wi, V wo) { return hg_phase(g, -cosv(wi, wo)); }
+__device__ void hg_sample(float g, V wo, float su, float sv, V *dir, float *f) {
+    const float sn = 2 * su - 1;
+    float u;
+    if (fabsf(g) < 0.001f) u = sn;
+    else {
+        const float g2 = g * g;
+        const float q = (1 - g2) / (1 + g * sn);
+        u = (1 + g2 - q * q) / (2 * g);
+    }
+    const float ct = -u;
+    const float st = sqrtf(smax(0.0f, 1 - ct * ct));
+    const float phi = 2 * PI_F * sv;
+    *dir = to_global(frame_from_z(wo), mk(st * sinf(phi), st * cosf(phi), ct));
+    *f = hg_phase(g, u);
+}
+
+__device__ __forceinline__ V vlerp(V a, V b, float t) { return a + (b - a) * t; }
+
+// Bound on delta/ratio-tracking steps per segment (expected count = length x max
+// density; a segment needing more is outside any scene this path renders).  The
+// exit test is !(t < t_max) so that a NaN segment length ends the loop at once.
+constexpr int MAX_TRACKING_STEPS = 1 << 20;
+
+// Medium::tr (homogeneous.cpp:48-57, heterogeneous.cpp:52-72); r = the stream the caller designates
+__device__ V medium_tr(const DevScene &s, int32_t mi, V a, V b, Rng &r) {
+    if (mi < 0) return mk(1, 1, 1);
+    const DevMedium &m = s.media[mi];
+    if (m.kind == AGX_MEDIUM_HOMOGENEOUS) {
+        const float d = length(a - b);
+        return mk(expf(-m.st.x * d), expf(-m.st.y * d), expf(-m.st.z * d));
+    }
+    if (m.kind == AGX_MEDIUM_HETEROGENEOUS) {
+        float result = 1, t = 0;
+        const float t_max = length(a - b);
+        for (int step = 0; step < MAX_TRACKING_STEPS; ++step) {
+            t += -logf(1 - r.uniform()) * m.misc.z;
+            if (!(t < t_max)) break;
+            const V unit = xf_point(m.w2l, vlerp(a, b, t / t_max));
+            result *= 1 - tex3d(s, m.density, unit).x / m.misc.y;
+        }
+        return mk(result, result, result);
+    }
+    return mk(1, 1, 1);
+}
+
+struct MScat {
+    bool happened;
+    V pos, wr, thr;
+    float g;
+};
+
+// Medium::sample_scattering (homogeneous.cpp:62-110, heterogeneous.cpp:81-123)
+__device__ MScat medium_sample(const DevScene &s, const DevMedium &m, V a, V b, Rng &r, Rng &rtr, bool indirect,
+                               float eps) {
+    MScat o;
+    o.happened = false;
+    o.thr = mk(1, 1, 1);
+    o.g = 0;
+    if (m.kind == AGX_MEDIUM_HOMOGENEOUS) {
+        const float sam = r.uniform();
+        if (black(mk(m.ss))) {
+            const float d = length(a - b);
+            o.thr = mk(expf(-m.st.x * d), expf(-m.st.y * d), expf(-m.st.z * d));
+            return o;
+        }
+        const float t_max = length(b - a);
+        int ch = int(sam * 3.0f);  // extract_uniform_int(u, 0, 3)
+        if (ch > 2) ch = 2;
+        const float nsam = sam * 3.0f - float(ch);
+        const float stc = ch == 0 ? m.st.x : (ch == 1 ? m.st.y : m.st.z);
+        const float stv = -logf(nsam) / stc;
+        const bool in_medium = stv < t_max;
+        const float tt = smin(stv, t_max);
+        const V trv = mk(expf(-m.st.x * tt), expf(-m.st.y * tt), expf(-m.st.z * tt));
+        const V dens = in_medium ? mk(m.st) * trv : trv;
+        float pdf = 0;
+        pdf += dens.x;
+        pdf += dens.y;
+        pdf += dens.z;
+        pdf /= 3;
+        pdf = smax(pdf, eps);
+        V thr = trv / pdf;
+        if (in_medium) thr = thr * mk(m.ss);
+        o.thr = thr;
+        if (in_medium) {
+            o.happened = true;
+            o.pos = vlerp(a, b, stv / t_max);
+            o.wr = (a - b) / t_max;
+            o.g = m.misc.x;
+        }
+        return o;
+    }
+    if (m.kind == AGX_MEDIUM_HETEROGENEOUS) {
+        const float t_max = length(a - b);
+        float t = 0;
+        for (int step = 0; step < MAX_TRACKING_STEPS; ++step) {
+            t += -logf(1 - r.uniform()) * m.misc.z;
+            if (!(t < t_max)) break;
+            const V pos = vlerp(a, b, t / t_max);
+            const V unit = xf_point(m.w2l, pos);
+            const float dn = tex3d(s, m.density, unit).x;
+            if (r.uniform() < dn / m.misc.y) {
+                o.happened = true;
+                o.pos = pos;
+                o.wr = (a - b) / t_max;
+                o.thr = (m.white_for_indirect && indirect) ? mk(1, 1, 1) : tex3d(s, m.albedo, unit);
+                o.g = tex3d(s, m.g3, unit).x;
+                return o;
+            }
+        }
+        return o;
+    }
+    return o;
+}
+
+__device__ __forceinline__ int32_t medium_side(const DevEntity &e, V d, V n) { return dot(d, n) >= 0 ? e.med_out : e.med_in; }
+
+// resolve the MIS-BSDF ray of the previous vertex (direct_illum.cpp:130-192, 194-253;
+// a medium vertex stores |cos| = 1 and pdf = f, so one formula serves both)
+__device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, uint32_t flags, float eps, Rng &rtr) {
     const float4 ma = pb.mis_a[p], mb = pb.mis_b[p];
     const V coef = mk(ma), bf = mk(mb);
     const float absdot = ma.w, pdf = mb.w;
@@ -1115,10 +1297,18 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     const Surf sp = surface_at(s, h, o, d, eps);
     const V lrad = dot(sp.geo.z, sp.wr) > 0 ? mk(e.emit) : mk(0, 0, 0);
     if (black(lrad)) return mk(0, 0, 0);
-    const V f = lrad * bf * absdot;
+    const int32_t med = int32_t(flags >> PF_MIS_MED_SHIFT) - 1;
+    const V f = medium_tr(s, med, o, sp.pos, rtr) * lrad * bf * absdot;
     if (delta) return coef * (f / pdf);
     const float lpdf = area_light_pdf(e, o, sp.pos, sp.geo.z);
     return coef * (f / (pdf + lpdf));
+}
+
+__device__ __forceinline__ void push_shadow(const PathBuffers &pb, size_t slot, V o, V d, float tmin, float tmax) {
+    pb.sh_o[slot] = f4(o, tmax);
+    pb.sh_d[slot] = f4(d, tmin);
+    const uint32_t qi = atomicAdd(&pb.cnt_out[1], 1u);
+    pb.q_shadow[qi] = uint32_t(slot);
 }
 
 __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
@@ -1129,25 +1319,37 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const float eps = rc.eps;
     uint32_t flags = pb.flags[p];
     V L = mk(pb.radiance[p]);
-    // 1. direct illumination of the previous vertex (path_tracing.cpp:134-146)
+    Rng rtr;
+    rtr.state = pb.rng_tr[p];
+    Rng rng;
+    rng.state = pb.rng[p];
+    {
+        const int64_t wh = int64_t(rc.W) * rc.H;
+        const int64_t w = rc.work_base + int64_t(p);
+        rng.inc = (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
+        rtr.inc = rng.inc;
+    }
+    // 1. direct illumination of the previous vertex (path_tracing.cpp:85-97, 134-146)
     if (flags & PF_PENDING) {
         V direct = mk(0, 0, 0);
         for (int32_t l = 0; l < rc.L; ++l) {
             const float4 c = pb.nee[size_t(p) * rc.L + l];
             if (c.w != 0.0f && !pb.occluded[size_t(p) * rc.L + l]) direct = direct + mk(c);
         }
-        if (flags & PF_MIS) direct = direct + resolve_mis(s, pb, p, flags, eps);
+        if (flags & PF_MIS) direct = direct + resolve_mis(s, pb, p, flags, eps, rtr);
         L = L + direct;
         flags &= ~(PF_PENDING | PF_MIS | PF_MIS_DELTA);
+        flags &= (1u << PF_MIS_MED_SHIFT) - 1u;
     }
     if (!(flags & PF_CONT)) {
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
+        pb.rng_tr[p] = rtr.state;
         return;
     }
     flags &= ~PF_CONT;
     V coef = mk(pb.coef[p]);
-    int2 dep = pb.depth[p];
+    int4 dep = pb.depth[p];
     const float4 h4 = pb.hit[2 * p];
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
@@ -1155,6 +1357,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * sky(s, rd);
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
+        pb.rng_tr[p] = rtr.state;
         return;
     }
     HitRec h;
@@ -1166,109 +1369,186 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         pb.gb_normal[p] = f4(sp.user.z, 0);
         pb.gb_albedo[p] = f4(b.albedo, (e.flags & 1) ? 0.0f : 1.0f);
     }
-    if (dep.x == 1 && e.light >= 0)  // :123-130
-        if (dot(sp.geo.z, sp.wr) > 0) L = L + coef * mk(e.emit);
-    Rng rng;
-    rng.state = pb.rng[p];
-    {
-        const int64_t wh = int64_t(rc.W) * rc.H;
-        const int64_t w = rc.work_base + int64_t(p);
-        rng.inc = (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
-    }
-    // 2. NEE over every light (mis_sample_light, direct_illum.cpp:11-42, 72-99)
     bool pending = false;
-    for (int32_t l = 0; l < rc.L; ++l) {
-        float s5[5];
-        for (int k = 0; k < 5; ++k) s5[k] = rng.uniform();
-        const size_t slot = size_t(p) * rc.L + l;
-        const int32_t li = s.lights[l];
-        V contrib = mk(0, 0, 0);
-        bool valid = false;
-        V so, sd;
-        float tmin = eps, tmax = 0;
-        if (li >= 0) {
-            const DevEntity &le = s.ents[li];
-            const LightSmp ls = area_light_sample(s, le, sp.pos, s5[0], s5[1], s5[3], eps);
-            const V rad = mk(le.emit);
-            if (!black(rad) && ls.pdf != 0.0f) {
-                const float len = length(ls.pos - sp.pos) - eps;
-                if (len > eps) {
-                    const V tl = normalize(ls.pos - sp.pos);
-                    const V f = bsdf_eval(b, tl, sp.wr);
-                    if (!black(f)) {
-                        const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
-                        const float bp = bsdf_pdf(b, tl, sp.wr);
-                        contrib = coef * (ff / (ls.pdf + bp));
-                        valid = true;
-                        so = sp.pos; sd = tl; tmax = len;
+    bool cont = false;
+    // medium between the ray origin and the hit (path_tracing.cpp:64-119)
+    const int32_t mi = medium_side(e, sp.wr, sp.geo.z);
+    bool scattered = false;
+    if (mi >= 0) {
+        const DevMedium &m = s.media[mi];
+        if (dep.z < m.max_scat) {
+            const MScat ms = medium_sample(s, m, ro, sp.pos, rng, rtr, dep.z > 0, eps);
+            coef = coef * ms.thr;
+            if (ms.happened) {
+                scattered = true;
+                dep.z += 1;
+                // NEE at the medium point: area lights only; the environment draws nothing
+                for (int32_t l = 0; l < rc.L; ++l) {
+                    const size_t slot = size_t(p) * rc.L + l;
+                    const int32_t li = s.lights[l];
+                    pb.nee[slot] = make_float4(0, 0, 0, 0);
+                    if (li < 0) continue;
+                    float s5[5];
+                    for (int k = 0; k < 5; ++k) s5[k] = rng.uniform();
+                    const DevEntity &le = s.ents[li];
+                    const LightSmp ls = area_light_sample(s, le, ms.pos, s5[0], s5[1], s5[3], eps);
+                    const V rad = mk(le.emit);
+                    if (black(rad) || ls.pdf == 0.0f) continue;
+                    const V tl = normalize(ls.pos - ms.pos);
+                    const float fp = hg_eval(ms.g, tl, ms.wr);
+                    if (fp == 0.0f) continue;
+                    const V f = medium_tr(s, mi, ms.pos, ls.pos, rtr) * rad * mk(fp, fp, fp);
+                    pb.nee[slot] = f4(coef * (f / (ls.pdf + fp)), 1.0f);
+                    pending = true;
+                    // Scene::visible(light, point): from the light towards the point
+                    const float dis = length(ls.pos - ms.pos);
+                    push_shadow(pb, slot, ls.pos, normalize(ms.pos - ls.pos), eps, dis - eps);
+                }
+                {  // MIS-BSDF ray with the phase function (direct_illum.cpp:194-253)
+                    const float u0 = rng.uniform(), u1 = rng.uniform();
+                    (void)rng.uniform();
+                    V dir;
+                    float f;
+                    hg_sample(ms.g, ms.wr, u0, u1, &dir, &f);
+                    if (f != 0.0f) {
+                        dir = normalize(dir);
+                        pb.ray_o[2 * p + 1] = f4(ms.pos, 0.0f);
+                        pb.ray_d[2 * p + 1] = f4(dir, INF_F);
+                        pb.mis_a[p] = f4(coef, 1.0f);
+                        pb.mis_b[p] = make_float4(f, f, f, f);
+                        flags |= PF_MIS | (uint32_t(mi + 1) << PF_MIS_MED_SHIFT);
+                        pending = true;
+                        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
+                        pb.q_closest[ci] = 2 * p + 1;
+                    }
+                }
+                {  // phase-function continuation (path_tracing.cpp:99-108)
+                    const float u0 = rng.uniform(), u1 = rng.uniform();
+                    (void)rng.uniform();
+                    V dir;
+                    float f;
+                    hg_sample(ms.g, ms.wr, u0, u1, &dir, &f);
+                    if (f != 0.0f && !(f < eps)) {
+                        pb.ray_o[2 * p] = f4(ms.pos, 0.0f);
+                        pb.ray_d[2 * p] = f4(normalize(dir), INF_F);
+                        coef = coef * (mk(f, f, f) / f);
+                        dep.x += 1;
+                        if (dep.x <= rc.max_depth) {
+                            cont = true;
+                            if (dep.x > rc.min_depth) {
+                                if (rng.uniform() > rc.cont_prob) cont = false;
+                                else coef = coef / rc.cont_prob;
+                            }
+                        }
                     }
                 }
             }
         } else {
-            const V dir = uniform_sphere(s5[0], s5[1]);
-            const V lpos = env_emit_pos(s, sp.pos, dir);
-            const V rad = sky(s, dir);
-            if (!black(rad)) {
-                const V tl = normalize(lpos - sp.pos);
-                const V f = bsdf_eval(b, tl, sp.wr);
-                if (!black(f)) {
-                    const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
-                    const float bp = bsdf_pdf(b, tl, sp.wr);
-                    contrib = coef * (ff / (SPHERE_PDF + bp));
-                    valid = true;
-                    // Scene::visible(A, B): [eps, |A-B| - eps] along normalize(B - A)
-                    so = sp.pos; sd = normalize(lpos - sp.pos); tmax = length(sp.pos - lpos) - eps;
-                }
+            // medium->ab (absorption only) when the scattering count is exhausted
+            if (m.kind == AGX_MEDIUM_HOMOGENEOUS) {
+                const float d = length(ro - sp.pos);
+                coef = coef * mk(expf(-m.sa.x * d), expf(-m.sa.y * d), expf(-m.sa.z * d));
+            } else {
+                coef = coef * medium_tr(s, mi, ro, sp.pos, rng);
             }
         }
-        pb.nee[slot] = f4(contrib, valid ? 1.0f : 0.0f);
-        if (valid) {
-            pending = true;
-            pb.sh_o[slot] = f4(so, tmax);
-            pb.sh_d[slot] = f4(sd, tmin);
-            const uint32_t qi = atomicAdd(&pb.cnt_out[1], 1u);
-            pb.q_shadow[qi] = uint32_t(slot);
-        }
     }
-    // 3. MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
-    {
-        const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-        const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
-        if (!black(bs.f)) {
-            const V dir = normalize(bs.dir);
-            pb.ray_o[2 * p + 1] = f4(eps_offset(sp, dir, eps), 0.0f);
-            pb.ray_d[2 * p + 1] = f4(dir, INF_F);
-            pb.mis_a[p] = f4(coef, fabsf(dot(sp.geo.z, dir)));
-            pb.mis_b[p] = f4(bs.f, bs.pdf);
-            flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u);
-            pending = true;
-            const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
-            pb.q_closest[ci] = 2 * p + 1;
+    if (!scattered) {
+        dep.z = 0;
+        if (dep.x == 1 && e.light >= 0)  // :123-130
+            if (dot(sp.geo.z, sp.wr) > 0) L = L + coef * mk(e.emit);
+        // NEE over every light (mis_sample_light, direct_illum.cpp:11-42, 72-99)
+        for (int32_t l = 0; l < rc.L; ++l) {
+            float s5[5];
+            for (int k = 0; k < 5; ++k) s5[k] = rng.uniform();
+            const size_t slot = size_t(p) * rc.L + l;
+            const int32_t li = s.lights[l];
+            V contrib = mk(0, 0, 0);
+            bool valid = false;
+            V so, sd;
+            float tmax = 0;
+            if (li >= 0) {
+                const DevEntity &le = s.ents[li];
+                const LightSmp ls = area_light_sample(s, le, sp.pos, s5[0], s5[1], s5[3], eps);
+                const V rad = mk(le.emit);
+                if (!black(rad) && ls.pdf != 0.0f) {
+                    const float len = length(ls.pos - sp.pos) - eps;
+                    if (len > eps) {
+                        const V tl = normalize(ls.pos - sp.pos);
+                        const V f = bsdf_eval(b, tl, sp.wr);
+                        if (!black(f)) {
+                            const int32_t lm = medium_side(e, tl, sp.geo.z);
+                            const V ff = medium_tr(s, lm, ls.pos, sp.pos, rtr) * rad * f * fabsf(cosv(tl, sp.geo.z));
+                            const float bp = bsdf_pdf(b, tl, sp.wr);
+                            contrib = coef * (ff / (ls.pdf + bp));
+                            valid = true;
+                            so = sp.pos; sd = tl; tmax = len;
+                        }
+                    }
+                }
+            } else {
+                const V dir = uniform_sphere(s5[0], s5[1]);
+                const V lpos = env_emit_pos(s, sp.pos, dir);
+                const V rad = sky(s, dir);
+                if (!black(rad)) {
+                    const V tl = normalize(lpos - sp.pos);
+                    const V f = bsdf_eval(b, tl, sp.wr);
+                    if (!black(f)) {
+                        const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
+                        const float bp = bsdf_pdf(b, tl, sp.wr);
+                        contrib = coef * (ff / (SPHERE_PDF + bp));
+                        valid = true;
+                        // Scene::visible(A, B): [eps, |A-B| - eps] along normalize(B - A)
+                        so = sp.pos; sd = normalize(lpos - sp.pos); tmax = length(sp.pos - lpos) - eps;
+                    }
+                }
+            }
+            pb.nee[slot] = f4(contrib, valid ? 1.0f : 0.0f);
+            if (valid) {
+                pending = true;
+                push_shadow(pb, slot, so, sd, eps, tmax);
+            }
+        }
+        // MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
+        {
+            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+            if (!black(bs.f)) {
+                const V dir = normalize(bs.dir);
+                pb.ray_o[2 * p + 1] = f4(eps_offset(sp, dir, eps), 0.0f);
+                pb.ray_d[2 * p + 1] = f4(dir, INF_F);
+                pb.mis_a[p] = f4(coef, fabsf(dot(sp.geo.z, dir)));
+                pb.mis_b[p] = f4(bs.f, bs.pdf);
+                const int32_t mm = medium_side(e, bs.dir, sp.geo.z);
+                flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u) | (uint32_t(mm + 1) << PF_MIS_MED_SHIFT);
+                pending = true;
+                const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
+                pb.q_closest[ci] = 2 * p + 1;
+            }
+        }
+        // continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
+        {
+            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+            if (!black(bs.f) && !(bs.pdf < eps)) {
+                const float abscos = fabsf(cosv(sp.geo.z, bs.dir));
+                coef = coef * (bs.f * abscos / bs.pdf);
+                pb.ray_o[2 * p] = f4(eps_offset(sp, bs.dir, eps), 0.0f);
+                pb.ray_d[2 * p] = f4(normalize(bs.dir), INF_F);
+                if (bs.delta && dep.x >= 2 && dep.y <= rc.specular_depth) { dep.x -= 1; dep.y += 1; }
+                dep.x += 1;
+                if (dep.x <= rc.max_depth) {
+                    cont = true;
+                    if (dep.x > rc.min_depth) {
+                        if (rng.uniform() > rc.cont_prob) cont = false;
+                        else coef = coef / rc.cont_prob;
+                    }
+                }
+            }
         }
     }
     if (pending) flags |= PF_PENDING;
     pb.radiance[p] = f4(L, 0);
-    // 4. continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
-    bool cont = false;
-    {
-        const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-        const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
-        if (!black(bs.f) && !(bs.pdf < eps)) {
-            const float abscos = fabsf(cosv(sp.geo.z, bs.dir));
-            coef = coef * (bs.f * abscos / bs.pdf);
-            pb.ray_o[2 * p] = f4(eps_offset(sp, bs.dir, eps), 0.0f);
-            pb.ray_d[2 * p] = f4(normalize(bs.dir), INF_F);
-            if (bs.delta && dep.x >= 2 && dep.y <= rc.specular_depth) { dep.x -= 1; dep.y += 1; }
-            dep.x += 1;
-            if (dep.x <= rc.max_depth) {
-                cont = true;
-                if (dep.x > rc.min_depth) {
-                    if (rng.uniform() > rc.cont_prob) cont = false;
-                    else coef = coef / rc.cont_prob;
-                }
-            }
-        }
-    }
     if (cont) {
         flags |= PF_CONT;
         const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
@@ -1282,6 +1562,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     pb.depth[p] = dep;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
+    pb.rng_tr[p] = rtr.state;
 }
 
 // accumulate the pass into the film in sample order, Neumaier-compensated

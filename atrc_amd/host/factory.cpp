@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
+#include <limits>
 #include <fstream>
 #include <sstream>
 
@@ -15,6 +16,7 @@ namespace atrc {
 
 // ---------------------------------------------------------------- factory plumbing
 template <> Factory<Texture2D> &CreatingContext::factory<Texture2D>() { return texture2d; }
+template <> Factory<Texture3D> &CreatingContext::factory<Texture3D>() { return texture3d; }
 template <> Factory<Geometry> &CreatingContext::factory<Geometry>() { return geometry; }
 template <> Factory<Material> &CreatingContext::factory<Material>() { return material; }
 template <> Factory<Medium> &CreatingContext::factory<Medium>() { return medium; }
@@ -168,7 +170,19 @@ void save_bin_mesh(const std::string &filename, const std::vector<agx_vertex> &t
 const agx_scene_desc &Scene::desc() {
     ents_.clear();
     mats_.clear();
+    meds_.clear();
     std::map<const Material *, int32_t> mat_index;
+    std::map<const Medium *, int32_t> med_index;
+    // one descriptor per distinct medium object; "void" is index -1
+    auto medium_index = [&](const std::shared_ptr<Medium> &m) -> int32_t {
+        if (!m || m->desc.kind == AGX_MEDIUM_VOID) return -1;
+        auto it = med_index.find(m.get());
+        if (it == med_index.end()) {
+            it = med_index.emplace(m.get(), int32_t(meds_.size())).first;
+            meds_.push_back(m->desc);
+        }
+        return it->second;
+    };
     for (auto &e : entities) {
         agx_entity_desc d = e->desc;
         auto it = mat_index.find(e->material.get());
@@ -177,6 +191,8 @@ const agx_scene_desc &Scene::desc() {
             mats_.push_back(e->material->desc);
         }
         d.material = it->second;
+        d.medium_in = medium_index(e->med_in);
+        d.medium_out = medium_index(e->med_out);
         if (e->mesh) {
             d.tris = e->mesh->data();
             d.tri_count = uint32_t(e->mesh->size() / 3);
@@ -192,6 +208,8 @@ const agx_scene_desc &Scene::desc() {
     if (env) desc_.env = env->desc;
     desc_.aggregate = aggregate.kind;
     desc_.max_leaf_size = aggregate.max_leaf_size;
+    desc_.medium_count = uint32_t(meds_.size());
+    desc_.media = meds_.empty() ? nullptr : meds_.data();
     return desc_;
 }
 
@@ -207,6 +225,80 @@ Vec3 texture_value(CreatingContext &ctx, const ConfigGroup &params, const std::s
 void set_transforms(agx_entity_desc &d, const Transform3 &l2w) {
     l2w.to_rows12(d.local_to_world);
     l2w.inverse().to_rows12(d.world_to_local);
+}
+
+// Texture3DCommonParams::full_transform (core/texture3d.h:89-125): flips, the
+// uvw permutation, then the user transform; the product applies the user
+// transform first (agz-utils Transform3 product order, parity unpinned).
+Transform3 tex3d_full_transform(const ConfigGroup &p) {
+    Transform3 ret;
+    if (p.child_int_or("inv_u", 0)) ret = ret * (Transform3::translate(1, 0, 0) * Transform3::scale(-1, 1, 1));
+    if (p.child_int_or("inv_v", 0)) ret = ret * (Transform3::translate(0, 1, 0) * Transform3::scale(1, -1, 1));
+    if (p.child_int_or("inv_w", 0)) ret = ret * (Transform3::translate(0, 0, 1) * Transform3::scale(1, 1, -1));
+    int perm[3] = {0, 1, 2};
+    if (p.find_child("uvw_perm")) {
+        const ConfigArray &a = p.child_array("uvw_perm");
+        if (a.size() != 3) throw ObjectConstructionException("uvw_perm needs 3 integers");
+        for (int i = 0; i < 3; ++i) perm[i] = a.at_int(size_t(i));
+    }
+    Transform3 P;
+    for (int r = 0; r < 3; ++r) {
+        if (perm[r] < 0 || perm[r] >= 3) throw ObjectConstructionException("invalid uvw perm value");
+        for (int c = 0; c < 4; ++c) P.m[r][c] = 0;
+        P.m[r][perm[r]] = 1;
+    }
+    ret = ret * P;
+    if (p.find_child("transform")) ret = ret * p.child_transform3("transform");
+    return ret;
+}
+
+int32_t wrap_mode(const std::string &s) {  // Texture3D::str_to_wrap_func (texture3d.h:165-175)
+    if (s == "clamp") return AGX_WRAP_CLAMP;
+    if (s == "repeat") return AGX_WRAP_REPEAT;
+    if (s == "mirror") return AGX_WRAP_MIRROR;
+    throw ObjectConstructionException("invalid wrap function name: " + s);
+}
+
+// image3d texel files (factory/src/utility/texture3d_loader.cpp): a header of
+// width, height, depth, then texels in (z, y, x) order; "real" = 1 float,
+// "spec" = 3 floats, "gray8" = 1 byte / 255, "rgb8" = 3 bytes / 255.  The ASCII
+// gray8/rgb8 readers extract single characters, as Atrc's operator>> on uint8_t does.
+std::shared_ptr<std::vector<float>> load_texels(const std::string &filename, bool binary, const std::string &format,
+                                                int *w, int *h, int *d, int *ch) {
+    std::ifstream f(filename, binary ? std::ios::in | std::ios::binary : std::ios::in);
+    if (!f) throw ObjectConstructionException("failed to open file: " + filename);
+    int32_t W = 0, H = 0, D = 0;
+    if (binary) {
+        f.read(reinterpret_cast<char *>(&W), 4);
+        f.read(reinterpret_cast<char *>(&H), 4);
+        f.read(reinterpret_cast<char *>(&D), 4);
+    } else {
+        f >> W >> H >> D;
+    }
+    if (!f || W <= 0 || H <= 0 || D <= 0) throw ObjectConstructionException("failed to read width/height/depth from file");
+    int C;
+    if (format == "real" || format == "gray8") C = 1;
+    else if (format == "spec" || format == "rgb8") C = 3;
+    else throw ObjectConstructionException("unknown image3d format: " + format);
+    const size_t n = size_t(W) * size_t(H) * size_t(D) * size_t(C);
+    auto out = std::make_shared<std::vector<float>>(n);
+    const bool bytes = format == "gray8" || format == "rgb8";
+    for (size_t i = 0; i < n; ++i) {
+        if (bytes) {
+            unsigned char c = 0;
+            if (binary) f.read(reinterpret_cast<char *>(&c), 1);
+            else f >> c;
+            (*out)[i] = float(c) / 255.0f;
+        } else {
+            float v = 0;
+            if (binary) f.read(reinterpret_cast<char *>(&v), 4);
+            else f >> v;
+            (*out)[i] = v;
+        }
+        if (!f) throw ObjectConstructionException("failed to parse texel data from file");
+    }
+    *w = W; *h = H; *d = D; *ch = C;
+    return out;
 }
 
 void reject_normal_map(const ConfigGroup &params) {
@@ -350,31 +442,99 @@ CreatingContext::CreatingContext() {
         m->desc.kind = AGX_MAT_IDEAL_BLACK;
         return m;
     });
-    for (const char *unsupported : {"disney", "dream_works_fabric", "invisible_surface", "paper", "phong"})
+    material.add_creator("invisible_surface", [](const ConfigGroup &p, CreatingContext &) {
+        if (p.find_child("bssrdf_dmfp")) throw ObjectConstructionException("BSSRDF surfaces are not supported by gpu_pt");
+        auto m = std::make_shared<Material>();
+        m->desc.kind = AGX_MAT_INVISIBLE_SURFACE;
+        return m;
+    });
+    for (const char *unsupported : {"disney", "dream_works_fabric", "paper", "phong"})
         material.add_creator(unsupported, [n = std::string(unsupported)](const ConfigGroup &, CreatingContext &)
                                               -> std::shared_ptr<Material> {
             throw ObjectConstructionException("material type '" + n + "' is not supported by gpu_pt yet");
         });
 
-    // medium (medium_creators.cpp): only "void" on this path
+    // texture3d (texture3d_creators.cpp:18-207)
+    texture3d.add_creator("constant", [](const ConfigGroup &p, CreatingContext &) {
+        auto t = std::make_shared<Texture3D>();
+        tex3d_full_transform(p);  // validated as Atrc does; constant3d ignores it
+        t->desc.kind = AGX_TEX3D_CONSTANT;
+        t->desc.width = t->desc.height = t->desc.depth = t->desc.channels = 1;
+        copy3(t->desc.value, p.child_spectrum("texel"));
+        Transform3().to_rows12(t->desc.transform);
+        t->desc.inv_gamma = p.child_real_or("inv_gamma", 1);
+        t->desc.linear = 1;
+        return t;
+    });
+    texture3d.add_creator("image3d", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto t = std::make_shared<Texture3D>();
+        tex3d_full_transform(p).to_rows12(t->desc.transform);
+        t->desc.wrap[0] = wrap_mode(p.child_str_or("wrap_u", "clamp"));
+        t->desc.wrap[1] = wrap_mode(p.child_str_or("wrap_v", "clamp"));
+        t->desc.wrap[2] = wrap_mode(p.child_str_or("wrap_w", "clamp"));
+        t->desc.inv_gamma = p.child_real_or("inv_gamma", 1);
+        t->desc.linear = p.child_str_or("sampler", "linear") == "linear" ? 1 : 0;
+        const std::string format = p.child_str("format");
+        int w, h, d, c;
+        if (p.find_child("ascii_filename"))
+            t->grid = load_texels(ctx.path_mapper->map(p.child_str("ascii_filename")), false, format, &w, &h, &d, &c);
+        else if (p.find_child("binary_filename"))
+            t->grid = load_texels(ctx.path_mapper->map(p.child_str("binary_filename")), true, format, &w, &h, &d, &c);
+        else if (p.find_child("image_filenames"))
+            throw ObjectConstructionException("image3d from 2d image slices is not supported by gpu_pt");
+        else
+            throw ObjectConstructionException("input filename is unspecified");
+        t->desc.kind = AGX_TEX3D_GRID;
+        t->desc.width = w; t->desc.height = h; t->desc.depth = d; t->desc.channels = c;
+        t->desc.data = t->grid->data();
+        return t;
+    });
+
+    // medium (medium_creators.cpp:9-88)
     medium.add_creator("void", [](const ConfigGroup &, CreatingContext &) {
         auto m = std::make_shared<Medium>();
         m->type = "void";
+        m->desc.kind = AGX_MEDIUM_VOID;
         return m;
     });
-    for (const char *unsupported : {"homogeneous", "heterogeneous"})
-        medium.add_creator(unsupported, [n = std::string(unsupported)](const ConfigGroup &, CreatingContext &)
-                                            -> std::shared_ptr<Medium> {
-            throw ObjectConstructionException("medium type '" + n + "' is not supported by gpu_pt yet");
-        });
+    medium.add_creator("homogeneous", [](const ConfigGroup &p, CreatingContext &) {
+        auto m = std::make_shared<Medium>();
+        m->type = "homogeneous";
+        m->desc.kind = AGX_MEDIUM_HOMOGENEOUS;
+        copy3(m->desc.sigma_a, p.child_spectrum("sigma_a"));
+        copy3(m->desc.sigma_s, p.child_spectrum("sigma_s"));
+        m->desc.g = p.child_real("g");
+        if (m->desc.g <= -1 || m->desc.g >= 1)  // homogeneous.cpp:33-35
+            throw ObjectConstructionException("invalid g value: " + std::to_string(m->desc.g));
+        m->desc.max_scattering_count = p.child_int_or("max_scattering_count", std::numeric_limits<int>::max());
+        return m;
+    });
+    medium.add_creator("heterogeneous", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto m = std::make_shared<Medium>();
+        m->type = "heterogeneous";
+        m->desc.kind = AGX_MEDIUM_HETEROGENEOUS;
+        const Transform3 l2w = p.child_transform3("transform");
+        l2w.to_rows12(m->desc.local_to_world);
+        l2w.inverse().to_rows12(m->desc.world_to_local);
+        m->density = ctx.create<Texture3D>(p.child_group("density"));
+        m->albedo = ctx.create<Texture3D>(p.child_group("albedo"));
+        m->g = ctx.create<Texture3D>(p.child_group("g"));
+        m->desc.density = m->density->desc;
+        m->desc.albedo = m->albedo->desc;
+        m->desc.g3 = m->g->desc;
+        m->desc.max_scattering_count = p.child_int_or("max_scattering_count", std::numeric_limits<int>::max());
+        m->desc.white_for_indirect = p.child_int_or("white_for_indirect", 0) != 0;
+        return m;
+    });
 
     // entity (entity_creators.cpp:29-60)
     entity.add_creator("geometric", [](const ConfigGroup &p, CreatingContext &ctx) {
         auto e = std::make_shared<Entity>();
         auto g = ctx.create<Geometry>(p.child_group("geometry"));
         e->material = ctx.create<Material>(p.child_group("material"));
-        if (auto n = p.find_child_group("med_in")) ctx.create<Medium>(*n);
-        if (auto n = p.find_child_group("med_out")) ctx.create<Medium>(*n);
+        // create_medium_interface (entity_creators.cpp:11-27): missing = void
+        if (auto n = p.find_child_group("med_in")) e->med_in = ctx.create<Medium>(*n);
+        if (auto n = p.find_child_group("med_out")) e->med_out = ctx.create<Medium>(*n);
         e->desc = g->proto;
         e->mesh = g->mesh;
         e->desc.medium_in = -1;
@@ -473,6 +633,7 @@ CreatingContext::CreatingContext() {
 }
 
 template std::shared_ptr<Texture2D> CreatingContext::create<Texture2D>(const ConfigGroup &);
+template std::shared_ptr<Texture3D> CreatingContext::create<Texture3D>(const ConfigGroup &);
 template std::shared_ptr<Geometry> CreatingContext::create<Geometry>(const ConfigGroup &);
 template std::shared_ptr<Material> CreatingContext::create<Material>(const ConfigGroup &);
 template std::shared_ptr<Medium> CreatingContext::create<Medium>(const ConfigGroup &);

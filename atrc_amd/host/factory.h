@@ -31,13 +31,20 @@ struct Geometry {
 struct Material {
     agx_material_desc desc{};
 };
+struct Texture3D {
+    agx_texture3d_desc desc{};       // data points into grid (image3d) or is NULL (constant)
+    std::shared_ptr<std::vector<float>> grid;
+};
 struct Medium {
-    std::string type;                // only "void" reaches the GPU path
+    std::string type;                // "void", "homogeneous" or "heterogeneous"
+    agx_medium_desc desc{};
+    std::shared_ptr<Texture3D> density, albedo, g;
 };
 struct Entity {
     agx_entity_desc desc{};
     std::shared_ptr<std::vector<agx_vertex>> mesh;
     std::shared_ptr<Material> material;
+    std::shared_ptr<Medium> med_in, med_out;  // NULL or "void" = no medium
 };
 struct EnvirLight {
     agx_env_desc desc{};
@@ -70,6 +77,7 @@ public:
 private:
     std::vector<agx_entity_desc> ents_;
     std::vector<agx_material_desc> mats_;
+    std::vector<agx_medium_desc> meds_;
     agx_scene_desc desc_{};
 };
 
@@ -120,6 +128,7 @@ public:
     float film_aspect = 1;  // argument Atrc passes to camera creators
 
     Factory<Texture2D> texture2d{"texture2d"};
+    Factory<Texture3D> texture3d{"texture3d"};
     Factory<Geometry> geometry{"geometry"};
     Factory<Material> material{"material"};
     Factory<Medium> medium{"medium"};

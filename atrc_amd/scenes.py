@@ -36,6 +36,61 @@ class Material:
 
 
 @dataclass
+class Texture3D:
+    """constant3d (value) or image3d (grid of shape (depth, height, width[, 3]), float32)."""
+    value: tuple = (1.0, 1.0, 1.0)
+    grid: np.ndarray | None = None
+    linear: bool = True
+    transform: tuple = IDENTITY          # uvw -> texture space, row-major 3x4
+    wrap: tuple = (capi.WRAP_CLAMP, capi.WRAP_CLAMP, capi.WRAP_CLAMP)
+    inv_gamma: float = 1.0
+
+
+@dataclass
+class Medium:
+    """homogeneous (sigma_a, sigma_s, g) or heterogeneous (density/albedo/g grids in the
+    unit cube of local_to_world) participating medium."""
+    kind: int = capi.MEDIUM_HOMOGENEOUS
+    sigma_a: tuple = (0.0, 0.0, 0.0)
+    sigma_s: tuple = (0.0, 0.0, 0.0)
+    g: float = 0.0
+    max_scattering_count: int = 2 ** 31 - 1
+    local_to_world: tuple = IDENTITY
+    density: Texture3D = field(default_factory=lambda: Texture3D(value=(1.0, 1.0, 1.0)))
+    albedo: Texture3D = field(default_factory=lambda: Texture3D(value=(1.0, 1.0, 1.0)))
+    g3: Texture3D = field(default_factory=lambda: Texture3D(value=(0.0, 0.0, 0.0)))
+    white_for_indirect: bool = False
+
+
+def _affine_pair(l2w):
+    m = np.array(l2w, np.float64).reshape(3, 4)
+    full = np.eye(4)
+    full[:3, :] = m
+    inv = np.linalg.inv(full)[:3, :]
+    return ([float(x) for x in m.reshape(-1).astype(np.float32)],
+            [float(x) for x in inv.reshape(-1).astype(np.float32)])
+
+
+def _fill_tex3d(d, t: Texture3D, keep: list) -> None:
+    d.linear = 1 if t.linear else 0
+    d.value[:] = [float(x) for x in t.value]
+    d.transform[:] = [float(x) for x in t.transform]
+    d.wrap[:] = [int(x) for x in t.wrap]
+    d.inv_gamma = float(t.inv_gamma)
+    if t.grid is None:
+        d.kind = capi.TEX3D_CONSTANT
+        d.width = d.height = d.depth = d.channels = 1
+        return
+    g = np.ascontiguousarray(t.grid, np.float32)
+    if g.ndim == 3:
+        g = g[..., None]
+    d.kind = capi.TEX3D_GRID
+    d.depth, d.height, d.width, d.channels = (int(x) for x in g.shape)
+    keep.append(g)
+    d.data = g.ctypes.data_as(C.POINTER(C.c_float))
+
+
+@dataclass
 class Entity:
     geometry: int
     material: int
@@ -46,6 +101,8 @@ class Entity:
     radius: float = 1.0
     local_to_world: tuple = IDENTITY     # row-major 3x4
     no_denoise: bool = False
+    med_in: int = -1                     # index into Scene.media, -1 = void
+    med_out: int = -1
 
 
 @dataclass
@@ -55,6 +112,11 @@ class Scene:
     env: dict | None = None              # {"top": (r,g,b), "bottom": (r,g,b)}
     aggregate: int = capi.AGGREGATE_NATIVE
     max_leaf_size: int = 5
+    media: list = field(default_factory=list)
+
+    def add_medium(self, m: Medium) -> int:
+        self.media.append(m)
+        return len(self.media) - 1
 
     def add_material(self, m: Material) -> int:
         self.materials.append(m)
@@ -75,8 +137,8 @@ class Scene:
             d = ents[i]
             d.geometry = e.geometry
             d.material = e.material
-            d.medium_in = -1
-            d.medium_out = -1
+            d.medium_in = int(e.med_in)
+            d.medium_out = int(e.med_out)
             d.emit_radiance[:] = [float(x) for x in e.emit]
             d.power = -1.0
             d.no_denoise = 1 if e.no_denoise else 0
@@ -90,12 +152,7 @@ class Scene:
             for k, t in enumerate(e.uvs):
                 d.t[k][:] = [float(x) for x in t]
             d.radius = float(e.radius)
-            m = np.array(e.local_to_world, np.float64).reshape(3, 4)
-            full = np.eye(4)
-            full[:3, :] = m
-            inv = np.linalg.inv(full)[:3, :]
-            d.local_to_world[:] = [float(x) for x in m.reshape(-1).astype(np.float32)]
-            d.world_to_local[:] = [float(x) for x in inv.reshape(-1).astype(np.float32)]
+            d.local_to_world[:], d.world_to_local[:] = _affine_pair(e.local_to_world)
         mats = (capi.agx_material_desc * max(1, len(self.materials)))()
         for i, m in enumerate(self.materials):
             d = mats[i]
@@ -121,7 +178,22 @@ class Scene:
             desc.env.power = -1.0
         desc.aggregate = self.aggregate
         desc.max_leaf_size = self.max_leaf_size
-        keep += [ents, mats]
+        meds = (capi.agx_medium_desc * max(1, len(self.media)))()
+        for i, m in enumerate(self.media):
+            d = meds[i]
+            d.kind = m.kind
+            d.max_scattering_count = int(m.max_scattering_count)
+            d.sigma_a[:] = [float(x) for x in m.sigma_a]
+            d.sigma_s[:] = [float(x) for x in m.sigma_s]
+            d.g = float(m.g)
+            d.local_to_world[:], d.world_to_local[:] = _affine_pair(m.local_to_world)
+            _fill_tex3d(d.density, m.density, keep)
+            _fill_tex3d(d.albedo, m.albedo, keep)
+            _fill_tex3d(d.g3, m.g3, keep)
+            d.white_for_indirect = 1 if m.white_for_indirect else 0
+        desc.medium_count = len(self.media)
+        desc.media = C.cast(meds, C.POINTER(capi.agx_medium_desc))
+        keep += [ents, mats, meds]
         return desc, keep
 
 
@@ -263,6 +335,106 @@ def cornell_box() -> tuple[Scene, dict, dict]:
     camera = {"pos": (L / 2, -8.0, L / 2), "dst": (L / 2, 0.0, L / 2), "up": (0, 0, 1), "fov": 40.0}
     trace = {"min_depth": 2, "max_depth": 5, "cont_prob": 0.9}
     return s, camera, trace
+
+
+def fbm_grid(n: int, seed: int, octaves: int = 4, base: float = 4.0) -> np.ndarray:
+    """Seeded n^3 fBm density in [0, 1] (value noise, 'octaves' octaves, lowest frequency
+    'base' cells across the cube), indexed [z, y, x]."""
+    c = (np.arange(n, dtype=np.float64) + 0.5) / n
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    acc = np.zeros(len(p))
+    amp, freq, norm = 1.0, base, 0.0
+    for o in range(octaves):
+        acc += amp * _value_noise(p * freq, seed + 17 * o)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return np.clip(0.5 + 0.5 * acc / norm * 1.6, 0.0, 1.0).reshape(n, n, n).astype(np.float32)
+
+
+def _box_frame(lo, hi, rot_deg: float) -> tuple:
+    """3x4 local_to_world taking the unit cube onto box_mesh(lo, hi, rot_deg)."""
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
+    a = math.radians(rot_deg)
+    R = np.array([[math.cos(a), -math.sin(a), 0], [math.sin(a), math.cos(a), 0], [0, 0, 1]])
+    Lm = R @ np.diag(hi - lo)
+    t = (lo + hi) / 2 - Lm @ np.array([0.5, 0.5, 0.5])
+    return tuple(float(v) for v in np.concatenate([Lm, t[:, None]], axis=1).reshape(-1))
+
+
+def volumetric_cornell(grid_n: int = 16, seed: int = 5, sky: bool = True) -> tuple[Scene, dict, dict]:
+    """The Cornell-like scene with both boxes turned into medium boundaries
+    (invisible_surface): the short box holds a homogeneous medium (chromatic
+    sigma_a/sigma_s, HG g = -0.2), the tall one a heterogeneous medium on a
+    seeded grid_n^3 fBm density (max 2), constant albedo 0.9 and g = 0.3; a dim
+    native_sky lights the room through its open front."""
+    s, cam, tr = cornell_box()
+    inv = s.add_material(Material(kind=capi.MAT_INVISIBLE_SURFACE))
+    homo = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.05, 0.1, 0.2), sigma_s=(0.6, 0.5, 0.4),
+                               g=-0.2))
+    dens = fbm_grid(grid_n, seed) * 2.0
+    het = s.add_medium(Medium(kind=capi.MEDIUM_HETEROGENEOUS,
+                              local_to_world=_box_frame((2.9, 2.7, 0.0), (4.55, 4.35, 3.3), -18.0),
+                              density=Texture3D(grid=dens),
+                              albedo=Texture3D(value=(0.9, 0.9, 0.9)),
+                              g3=Texture3D(value=(0.3, 0.3, 0.3))))
+    short, tall = s.entities[5], s.entities[6]
+    short.material, short.med_in = inv, homo
+    tall.material, tall.med_in = inv, het
+    if sky:
+        s.env = {"top": (0.3, 0.35, 0.45), "bottom": (0.1, 0.1, 0.1)}
+    return s, cam, dict(tr, max_depth=10)
+
+
+def medium_furnace(albedo_one: bool = True, n_tris_subdiv: int = 2, radius: float = 2.0,
+                   sigma_s: float = 0.8, heterogeneous: bool = False) -> tuple[Scene, dict, dict]:
+    """Closed emitter: an inward-facing icosphere of radius `radius` emitting radiance 1
+    (ideal_black) filled with a medium of albedo 1 (sigma_a = 0).  The radiance field
+    inside is exactly 1 everywhere; the camera sits at the centre."""
+    s = Scene(aggregate=capi.AGGREGATE_NATIVE)
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    pos, faces = icosphere(n_tris_subdiv)
+    pos = pos * radius
+    faces = faces[:, ::-1].copy()                    # inward winding: geometric normal points to the centre
+    tris = mesh_from_indexed(pos, -pos / radius, faces)
+    if heterogeneous:
+        m = Medium(kind=capi.MEDIUM_HETEROGENEOUS,
+                   local_to_world=(2 * radius, 0, 0, -radius, 0, 2 * radius, 0, -radius, 0, 0, 2 * radius, -radius),
+                   density=Texture3D(grid=fbm_grid(8, 11) * sigma_s + 0.05),
+                   albedo=Texture3D(value=(1.0, 1.0, 1.0)), g3=Texture3D(value=(0.5, 0.5, 0.5)))
+    else:
+        m = Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.0, 0.0, 0.0), sigma_s=(sigma_s, sigma_s, sigma_s), g=0.4)
+    mi = s.add_medium(m)
+    # the normal side of the inward-facing shell is its inside
+    s.add(Entity(capi.GEOM_TRIANGLE_BVH, black, emit=(1.0, 1.0, 1.0), tris=tris, med_out=mi))
+    cam = {"pos": (0.0, 0.0, 0.0), "dst": (0.0, 1.0, 0.0), "up": (0, 0, 1), "fov": 70.0}
+    return s, cam, {"min_depth": 5, "max_depth": 40, "cont_prob": 0.9}
+
+
+def absorbing_slab(sigma_a=(0.3, 0.6, 1.2), thickness: float = 1.5) -> tuple[Scene, dict, dict]:
+    """An emitting quad (radiance 2) seen through a slab between two invisible_surface
+    quads filled with a purely absorbing medium: the image is 2 * exp(-sigma_a * L),
+    L = thickness / cos(angle of the camera ray to the slab normal)."""
+    s = Scene(aggregate=capi.AGGREGATE_NATIVE)
+    inv = s.add_material(Material(kind=capi.MAT_INVISIBLE_SURFACE))
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    mi = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=tuple(sigma_a), sigma_s=(0.0, 0.0, 0.0), g=0.0))
+    big = 50.0
+    y0, y1 = 2.0, 2.0 + thickness
+    # front face: normal -y (towards the camera), the medium on its back side
+    # (a quad's geometric normal is (B - A) x (C - A))
+    s.add(Entity(capi.GEOM_QUAD, inv, points=((-big, y0, -big), (big, y0, -big), (big, y0, big), (-big, y0, big)),
+                 med_in=mi))
+    # back face: normal +y, the medium on its back side
+    s.add(Entity(capi.GEOM_QUAD, inv, points=((-big, y1, -big), (-big, y1, big), (big, y1, big), (big, y1, -big)),
+                 med_in=mi))
+    # emitter facing the camera (normal -y)
+    s.add(Entity(capi.GEOM_QUAD, black, emit=(2.0, 2.0, 2.0),
+                 points=((-big, 8.0, -big), (big, 8.0, -big), (big, 8.0, big), (-big, 8.0, big))))
+    cam = {"pos": (0.0, 0.0, 0.0), "dst": (0.0, 1.0, 0.0), "up": (0, 0, 1), "fov": 30.0}
+    return s, cam, {"min_depth": 8, "max_depth": 8, "cont_prob": 1.0}
 
 
 def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, extent: float = 50.0,

@@ -61,12 +61,19 @@ def cpu_baseline(desc, cam, tr, W, H, target_s: float):
 
     threads = max(1, min(16, os.cpu_count() or 1))
     ref = po.OracleScene(desc, 32)
-    p = scenes.render_params(W, H, 1, cam, seed=777, **tr)
-    rows = 4
-    r = ref.render(p, threads=threads, rows=(0, rows), aovs=False)
-    per_row = max(r["seconds"], 1e-3) / rows
-    rows = int(min(H, max(rows, target_s / per_row)))
+    # size the sample from a 16-row probe: a band of rows at 1 spp, or the whole
+    # frame at several spp when one frame takes less than the target
+    probe = 16
+    r = ref.render(scenes.render_params(W, H, 1, cam, seed=777, **tr), threads=threads,
+                   rows=((H - probe) // 2, (H - probe) // 2 + probe), aovs=False)
+    per_row = max(r["seconds"], 1e-4) / probe
+    rows_needed = target_s / per_row
+    if rows_needed <= H:
+        rows, spp = int(max(probe, rows_needed)), 1
+    else:
+        rows, spp = H, int(min(256, max(1, round(rows_needed / H))))
     r0 = (H - rows) // 2
+    p = scenes.render_params(W, H, spp, cam, seed=777, **tr)
     r = ref.render(p, threads=threads, rows=(r0, r0 + rows), aovs=False)
     c = r["counters"]
     rays = c["closest"] + c["shadow"]
@@ -75,9 +82,22 @@ def cpu_baseline(desc, cam, tr, W, H, target_s: float):
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"rows {r0}..{r0 + rows} of the {W}x{H} frame at 1 spp ({rays} ray queries, {r['seconds']:.1f} s)",
-        "samples_per_s": rows * W / r["seconds"],
+        "sample": f"rows {r0}..{r0 + rows} of the {W}x{H} frame at {spp} spp ({rays} ray queries, "
+                  f"{r['seconds']:.1f} s on {threads} threads)",
+        "samples_per_s": rows * W * spp / r["seconds"],
     }
+
+
+def pmc_traffic(kernel: str, workload_key: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary of
+    this workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy kernel)."""
+    f = ROOT / "profiles" / f"{kernel}_{workload_key}_pmc.txt"
+    if not f.exists():
+        return None, None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1].strip()), str(f.relative_to(ROOT))
+    return None, None
 
 
 def main() -> int:
@@ -142,9 +162,10 @@ def main() -> int:
 
     closest_bytes = bytes_per_closest * st["closest_rays"]
     achieved = closest_bytes / (st["trace_closest_ms"] * 1e-3) / 1e9
+    traffic, traffic_src = pmc_traffic("k_trace_closest", args.config)
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
         "kernel": "k_trace_closest",
         "avg_launch_ms": st["trace_closest_ms"] / max(1, st["trace_closest_launches"]),
         "bytes_per_launch": closest_bytes / max(1, st["trace_closest_launches"]),
@@ -162,7 +183,7 @@ def main() -> int:
         "roofline": roofline,
         "guided_vs_plain_relmse_ratio": None,
         "copy_calibration": copy,
-        "traversal_stack": "private" if os.environ.get("AGX_FORCE_PRIVATE_STACK") == "1" else "auto (LDS when it fits)",
+        "traversal_stack": "lds" if os.environ.get("AGX_TRAVERSAL_STACK") == "lds" else "private",
     }
     ctx.close()
     if not args.no_cpu_baseline:

@@ -58,12 +58,15 @@ enum {
     AGX_MAT_METAL         = 1, /* color, eta, k (JSON names), roughness, anisotropic */
     AGX_MAT_GLASS         = 2, /* color = reflection colour, color2 = refraction colour (before sqrt), ior */
     AGX_MAT_MIRROR        = 3, /* color = color_map, eta, k */
-    AGX_MAT_IDEAL_BLACK   = 4
+    AGX_MAT_IDEAL_BLACK   = 4,
+    AGX_MAT_INVISIBLE_SURFACE = 5  /* medium boundary: passes every ray straight through */
 };
 
 enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
 enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
-enum { AGX_MEDIUM_VOID = 0 };
+enum { AGX_MEDIUM_VOID = 0, AGX_MEDIUM_HOMOGENEOUS = 1, AGX_MEDIUM_HETEROGENEOUS = 2 };
+enum { AGX_TEX3D_CONSTANT = 0, AGX_TEX3D_GRID = 1 };
+enum { AGX_WRAP_CLAMP = 0, AGX_WRAP_REPEAT = 1, AGX_WRAP_MIRROR = 2 };
 
 /* One mesh vertex; three consecutive vertices form a triangle. */
 typedef struct agx_vertex {
@@ -89,6 +92,37 @@ typedef struct agx_entity_desc {
     float   local_to_world[12];/* row-major 3x4 (triangle, sphere, disk) */
     float   world_to_local[12];/* its inverse, row-major 3x4 */
 } agx_entity_desc;
+
+/* Atrc Texture3D: "constant3d" or "image3d" (src/tracer/include/agz/tracer/core/texture3d.h,
+ * src/tracer/src/core/texture3d/image3d.cpp).  A lookup at uvw applies
+ * `transform` (uvw -> texture coordinates), the wrap of each axis, then nearest
+ * or trilinear filtering over texel centres, then ^inv_gamma. */
+typedef struct agx_texture3d_desc {
+    int32_t kind;              /* AGX_TEX3D_* */
+    int32_t linear;            /* grid: 1 = trilinear, 0 = nearest */
+    int32_t width, height, depth;
+    int32_t channels;          /* grid: 1 = real, 3 = spectrum */
+    const float *data;         /* grid: depth*height*width*channels floats, x fastest */
+    float   value[3];          /* constant value */
+    float   transform[12];     /* row-major 3x4 */
+    int32_t wrap[3];           /* AGX_WRAP_* for u, v, w */
+    float   inv_gamma;
+} agx_texture3d_desc;
+
+/* Atrc media (src/factory/src/creator/medium_creators.cpp, src/tracer/src/core/medium/). */
+typedef struct agx_medium_desc {
+    int32_t kind;              /* AGX_MEDIUM_* */
+    int32_t max_scattering_count;
+    float   sigma_a[3];        /* homogeneous */
+    float   sigma_s[3];
+    float   g;                 /* homogeneous Henyey-Greenstein g */
+    float   local_to_world[12];/* heterogeneous: unit cube -> world, row-major 3x4 */
+    float   world_to_local[12];
+    agx_texture3d_desc density;/* heterogeneous: real */
+    agx_texture3d_desc albedo; /* heterogeneous: spectrum */
+    agx_texture3d_desc g3;     /* heterogeneous: real */
+    int32_t white_for_indirect;
+} agx_medium_desc;
 
 /* Constant-texture material parameters (image textures are not part of this path). */
 typedef struct agx_material_desc {
@@ -130,6 +164,8 @@ typedef struct agx_scene_desc {
     agx_env_desc env;
     int32_t aggregate;         /* AGX_AGGREGATE_* */
     int32_t max_leaf_size;     /* entity BVH leaf size (default 5) */
+    uint32_t medium_count;
+    const agx_medium_desc *media; /* referenced by agx_entity_desc.medium_in / medium_out */
 } agx_scene_desc;
 
 /* PTRendererParams + the rendering settings that reach the renderer. */

@@ -177,6 +177,9 @@ struct Pcg32 {
     float uniform() { return float(next() >> 8) * 0x1p-24f; }
 };
 
+// key of the second per-path stream (ratio-tracking transmittance draws)
+constexpr uint64_t TR_STREAM_KEY = 0xA5A5F00DC0FFEE11ull;
+
 template<class T> struct Sampler {
     Pcg32 rng;
     T u1() { return T(rng.uniform()); }
@@ -1044,7 +1047,8 @@ template<class T> struct BSDF {
     }
 
     V3<T> eval(const V3<T> &wi, const V3<T> &wo) const {
-        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK) return V3<T>(0);
+        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK ||
+            kind == AGX_MAT_INVISIBLE_SURFACE) return V3<T>(0);
         // AggregateBSDF::eval (component/aggregate.h:39-64)
         if (fringe(wi) || fringe(wo)) {
             if (!geo.in_positive_z_hemisphere(wi) || !geo.in_positive_z_hemisphere(wo)) return V3<T>(0);
@@ -1062,7 +1066,8 @@ template<class T> struct BSDF {
     }
 
     T pdf(const V3<T> &wi, const V3<T> &wo) const {
-        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK) return 0;
+        if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK ||
+            kind == AGX_MAT_INVISIBLE_SURFACE) return 0;
         // AggregateBSDF::pdf (aggregate.h:218-246)
         if (fringe(wi) || fringe(wo)) {                         // bsdf.h:178-185
             if (geo.in_positive_z_hemisphere(wi) != geo.in_positive_z_hemisphere(wo)) return 0;
@@ -1084,6 +1089,11 @@ template<class T> struct BSDF {
     BSDFSample<T> sample(const V3<T> &wo, T su, T sv, T sw, T eps) const {
         BSDFSample<T> res;
         if (kind == AGX_MAT_IDEAL_BLACK) return res;
+        if (kind == AGX_MAT_INVISIBLE_SURFACE) {                // invisible_surface.cpp:26-31
+            const T cv = std::abs(cos_between(geo.z, wo));
+            res.dir = -wo; res.f = V3<T>(1) / (cv < eps ? T(1) : cv); res.pdf = 1; res.is_delta = true;
+            return res;
+        }
         if (kind == AGX_MAT_MIRROR) {                           // mirror.cpp:32-50
             const V3<T> lo = shd.global_to_local(wo);
             if (lo.z <= 0) return res;
@@ -1200,12 +1210,256 @@ template<class T> struct Material {
             }
             break;
         }
+        case AGX_MAT_INVISIBLE_SURFACE:                     // invisible_surface.cpp:47-50
+            b.albedo = V3<T>(1);
+            break;
         default:
             b.albedo = V3<T>(0);
             break;
         }
         *shading_normal = sp.user_coord.z;
         return b;
+    }
+};
+
+// ------------------------------------------------------------------------------------
+// participating media  (core/medium/{medium_void,homogeneous,heterogeneous}.cpp,
+//                       utility/phase_function.h, core/texture3d.h, texture3d/image3d.cpp)
+// ------------------------------------------------------------------------------------
+
+// Texture3D lookup: transform, per-axis wrap, nearest or trilinear over texel
+// centres (agz-utils linear_sample3d / nearest_sample3d: unpinned), ^inv_gamma.
+template<class T> struct Texture3D {
+    int kind = AGX_TEX3D_CONSTANT;
+    bool linear = true;
+    int w = 1, h = 1, d = 1, ch = 1;
+    std::vector<float> data;
+    V3<T> value;
+    Affine<T> xf;
+    int wrap[3] = {0, 0, 0};
+    T inv_gamma = 1;
+    void init(const agx_texture3d_desc &t) {
+        kind = t.kind;
+        linear = t.linear != 0;
+        value = V3<T>(T(t.value[0]), T(t.value[1]), T(t.value[2]));
+        xf = Affine<T>::from(t.transform);
+        for (int i = 0; i < 3; ++i) wrap[i] = t.wrap[i];
+        inv_gamma = T(t.inv_gamma > 0 ? t.inv_gamma : 1);
+        if (kind == AGX_TEX3D_GRID) {
+            w = t.width; h = t.height; d = t.depth; ch = t.channels;
+            if (w <= 0 || h <= 0 || d <= 0 || (ch != 1 && ch != 3) || !t.data)
+                throw std::runtime_error("invalid 3d texture grid");
+            data.assign(t.data, t.data + size_t(w) * size_t(h) * size_t(d) * size_t(ch));
+        }
+    }
+    static T wrapf(int mode, T x) {  // texture3d.h: wrap_clamp / wrap_repeat / wrap_mirror
+        if (mode == AGX_WRAP_REPEAT) return std::min(T(1), std::max(T(0), x - std::floor(x)));
+        if (mode == AGX_WRAP_MIRROR) {
+            const int ip = int(x >= 0 ? x : std::floor(x));
+            const T v = (ip & 1) ? 1 - x + T(ip) : x - T(ip);
+            return std::min(T(1), std::max(T(0), v));
+        }
+        return std::min(T(1), std::max(T(0), x));
+    }
+    V3<T> texel(int x, int y, int z) const {
+        const size_t o = ((size_t(z) * size_t(h) + size_t(y)) * size_t(w) + size_t(x)) * size_t(ch);
+        if (ch == 1) return V3<T>(T(data[o]));
+        return V3<T>(T(data[o]), T(data[o + 1]), T(data[o + 2]));
+    }
+    // constant3d overrides sample_spectrum/sample_real: no transform, no wrap, no
+    // inv_gamma (constant3d.cpp:43-58); image3d goes through texture3d.h:208-233
+    V3<T> sample(const V3<T> &uvw) const {
+        if (kind == AGX_TEX3D_CONSTANT) return value;
+        const V3<T> t = xf.point(uvw);
+        const T u = wrapf(wrap[0], t.x), v = wrapf(wrap[1], t.y), q = wrapf(wrap[2], t.z);
+        V3<T> r;
+        if (!linear) {
+            const int x = std::min(w - 1, int(u * T(w))), y = std::min(h - 1, int(v * T(h))), z = std::min(d - 1, int(q * T(d)));
+            r = texel(x, y, z);
+        } else {
+            const T fu = u * T(w) - T(0.5), fv = v * T(h) - T(0.5), fq = q * T(d) - T(0.5);
+            const T bu = std::floor(fu), bv = std::floor(fv), bq = std::floor(fq);
+            const T du = fu - bu, dv = fv - bv, dq = fq - bq;
+            auto cl = [](int i, int n) { return std::min(n - 1, std::max(0, i)); };
+            const int x0 = cl(int(bu), w), x1 = cl(int(bu) + 1, w);
+            const int y0 = cl(int(bv), h), y1 = cl(int(bv) + 1, h);
+            const int z0 = cl(int(bq), d), z1 = cl(int(bq) + 1, d);
+            auto lerp = [](const V3<T> &a, const V3<T> &b, T t_) { return a + (b - a) * t_; };
+            const V3<T> c00 = lerp(texel(x0, y0, z0), texel(x1, y0, z0), du);
+            const V3<T> c10 = lerp(texel(x0, y1, z0), texel(x1, y1, z0), du);
+            const V3<T> c01 = lerp(texel(x0, y0, z1), texel(x1, y0, z1), du);
+            const V3<T> c11 = lerp(texel(x0, y1, z1), texel(x1, y1, z1), du);
+            r = lerp(lerp(c00, c10, dv), lerp(c01, c11, dv), dq);
+        }
+        return gamma(r);
+    }
+    V3<T> gamma(const V3<T> &c) const {
+        if (inv_gamma == 1) return c;
+        return V3<T>(std::pow(c.x, inv_gamma), std::pow(c.y, inv_gamma), std::pow(c.z, inv_gamma));
+    }
+    // max_real of the stored texels, before inv_gamma (image3d.cpp:108-147, constant3d.cpp:43-46)
+    T max_real() const {
+        if (kind == AGX_TEX3D_CONSTANT) return value.x;
+        T m = -std::numeric_limits<T>::max();
+        for (size_t i = 0; i < data.size(); i += size_t(ch)) m = std::max(m, T(data[i]));
+        return m;
+    }
+};
+
+// Henyey-Greenstein phase function as a BSDF (utility/phase_function.h:8-90)
+template<class T> struct HG {
+    T g = 0;
+    T phase(T u) const {
+        const T g2 = g * g;
+        const T dem = 1 + g2 - 2 * g * u;
+        return (1 - g2) / (4 * PI<T> * dem * std::sqrt(dem));
+    }
+    T eval(const V3<T> &wi, const V3<T> &wo) const { return phase(-cos_between(wi, wo)); }
+    void sample(const V3<T> &wo, T su, T sv, V3<T> *dir, T *f) const {
+        const T s = 2 * su - 1;
+        T u;
+        if (std::abs(g) < T(0.001)) u = s;
+        else {
+            const T g2 = g * g;
+            const T q = (1 - g2) / (1 + g * s);
+            u = (1 + g2 - q * q) / (2 * g);
+        }
+        const T ct = -u;
+        const T st = std::sqrt(std::max(T(0), 1 - ct * ct));
+        const T phi = 2 * PI<T> * sv;
+        const V3<T> local(st * std::sin(phi), st * std::cos(phi), ct);
+        *dir = Coord<T>::from_z(wo).local_to_global(local);
+        *f = phase(u);
+    }
+};
+
+template<class T> struct MediumScatter {
+    bool happened = false;
+    V3<T> pos, wr, throughput{1};
+    HG<T> phase;
+};
+
+// Tracking loops stop after this many steps and on a NaN segment length; for
+// finite segments below that count they are Atrc's loops (heterogeneous.cpp:58-71, 88-120).
+constexpr int MAX_TRACKING_STEPS = 1 << 20;
+
+template<class T> struct Medium {
+    int kind = AGX_MEDIUM_VOID;
+    int max_scat = std::numeric_limits<int>::max();
+    V3<T> sa, ss, st;
+    T g = 0;
+    Affine<T> l2w, w2l;
+    Texture3D<T> density, albedo, gtex;
+    T max_density = 1, inv_max = 1;
+    bool white_for_indirect = false;
+    T eps = T(3e-4);
+
+    void init(const agx_medium_desc &m, T eps_) {
+        eps = eps_;
+        kind = m.kind;
+        max_scat = m.max_scattering_count;
+        if (kind == AGX_MEDIUM_HOMOGENEOUS) {               // homogeneous.cpp:23-41
+            sa = V3<T>(T(m.sigma_a[0]), T(m.sigma_a[1]), T(m.sigma_a[2]));
+            ss = V3<T>(T(m.sigma_s[0]), T(m.sigma_s[1]), T(m.sigma_s[2]));
+            st = sa + ss;
+            g = T(m.g);
+            if (g <= -1 || g >= 1) throw std::runtime_error("invalid g value");
+        } else if (kind == AGX_MEDIUM_HETEROGENEOUS) {      // heterogeneous.cpp:27-45
+            l2w = Affine<T>::from(m.local_to_world);
+            w2l = Affine<T>::from(m.world_to_local);
+            density.init(m.density);
+            albedo.init(m.albedo);
+            gtex.init(m.g3);
+            max_density = std::max(density.max_real(), eps);
+            inv_max = 1 / max_density;
+            white_for_indirect = m.white_for_indirect != 0;
+        } else if (kind != AGX_MEDIUM_VOID) {
+            throw std::runtime_error("unknown medium kind");
+        }
+    }
+    static T dist(const V3<T> &a, const V3<T> &b) { return (a - b).length(); }
+    static V3<T> lerp(const V3<T> &a, const V3<T> &b, T t) { return a + (b - a) * t; }
+
+    // Medium::tr; `s` is the stream the caller designates (ratio tracking draws)
+    V3<T> tr(const V3<T> &a, const V3<T> &b, Sampler<T> &s) const {
+        if (kind == AGX_MEDIUM_HOMOGENEOUS) {               // homogeneous.cpp:48-57
+            const V3<T> e = -st * dist(a, b);
+            return V3<T>(std::exp(e.x), std::exp(e.y), std::exp(e.z));
+        }
+        if (kind == AGX_MEDIUM_HETEROGENEOUS) {             // heterogeneous.cpp:52-72
+            T result = 1, t = 0;
+            const T t_max = dist(a, b);
+            for (int step = 0; step < MAX_TRACKING_STEPS; ++step) {
+                t += -std::log(1 - s.u1()) * inv_max;
+                if (!(t < t_max)) break;
+                const V3<T> unit = w2l.point(lerp(a, b, t / t_max));
+                result *= 1 - density.sample(unit).x / max_density;
+            }
+            return V3<T>(result);
+        }
+        return V3<T>(1);
+    }
+    V3<T> ab(const V3<T> &a, const V3<T> &b, Sampler<T> &s) const {
+        if (kind == AGX_MEDIUM_HOMOGENEOUS) {
+            const V3<T> e = -sa * dist(a, b);
+            return V3<T>(std::exp(e.x), std::exp(e.y), std::exp(e.z));
+        }
+        if (kind == AGX_MEDIUM_HETEROGENEOUS) return tr(a, b, s);  // heterogeneous.cpp:74-79
+        return V3<T>(1);
+    }
+    // Medium::sample_scattering (homogeneous.cpp:62-110, heterogeneous.cpp:81-123)
+    MediumScatter<T> sample_scattering(const V3<T> &a, const V3<T> &b, Sampler<T> &s, bool indirect) const {
+        MediumScatter<T> r;
+        if (kind == AGX_MEDIUM_HOMOGENEOUS) {
+            const T sam = s.u1();
+            if (ss.is_black()) { r.throughput = tr(a, b, s); return r; }
+            const T t_max = dist(a, b);
+            // extract_uniform_int(u, 0, 3): channel and the re-scaled remainder
+            int chn = int(sam * T(3));
+            if (chn > 2) chn = 2;
+            const T nsam = sam * T(3) - T(chn);
+            const T stv = -std::log(nsam) / st[chn];
+            const bool in_medium = stv < t_max;
+            V3<T> trv;
+            for (int i = 0; i < 3; ++i) trv[i] = std::exp(-st[i] * std::min(stv, t_max));
+            const V3<T> dens = in_medium ? st * trv : trv;
+            T pdf = 0;
+            for (int i = 0; i < 3; ++i) pdf += dens[i];
+            pdf /= 3;
+            pdf = std::max(pdf, eps);
+            V3<T> thr = trv / pdf;
+            if (in_medium) thr *= ss;
+            r.throughput = thr;
+            if (in_medium) {
+                r.happened = true;
+                r.pos = lerp(a, b, stv / t_max);
+                r.wr = (a - b) / t_max;
+                r.phase.g = g;
+            }
+            return r;
+        }
+        if (kind == AGX_MEDIUM_HETEROGENEOUS) {
+            const T t_max = dist(a, b);
+            T t = 0;
+            for (int step = 0; step < MAX_TRACKING_STEPS; ++step) {
+                t += -std::log(1 - s.u1()) * inv_max;
+                if (!(t < t_max)) break;
+                const V3<T> pos = lerp(a, b, t / t_max);
+                const V3<T> unit = w2l.point(pos);
+                const T dn = density.sample(unit).x;
+                if (s.u1() < dn / max_density) {
+                    const V3<T> alb = white_for_indirect && indirect ? V3<T>(1) : albedo.sample(unit);
+                    r.happened = true;
+                    r.pos = pos;
+                    r.wr = (a - b) / t_max;
+                    r.throughput = alb;
+                    r.phase.g = gtex.sample(unit).x;
+                    return r;
+                }
+            }
+            return r;
+        }
+        return r;  // void: no scattering, throughput 1
     }
 };
 
@@ -1219,6 +1473,7 @@ template<class T> struct Entity {
     V3<T> emit;
     bool is_light = false;
     bool no_denoise = false;
+    int32_t med_in = -1, med_out = -1;  // index into Scene::media, -1 = void
 };
 
 template<class T> struct LightSample {
@@ -1226,8 +1481,17 @@ template<class T> struct LightSample {
 };
 
 template<class T> struct Scene {
+    using Real = T;
     std::vector<Entity<T>> entities;
     std::vector<Material<T>> materials;
+    std::vector<Medium<T>> media;
+    Medium<T> void_medium;
+    // EntityIntersection::medium(d) (intersection.h:52-65): the side of the surface d points to
+    const Medium<T> &medium_of(const EntityHit<T> &h, const V3<T> &d) const {
+        const Entity<T> &e = entities[size_t(h.entity)];
+        const int32_t m = dot(d, h.geometry_coord.z) >= 0 ? e.med_out : e.med_in;
+        return m < 0 ? void_medium : media[size_t(m)];
+    }
     int env_kind = AGX_ENV_NONE;
     V3<T> sky_top, sky_bottom;
     T world_radius = 1;
@@ -1253,7 +1517,14 @@ template<class T> struct Scene {
             en.emit = V3<T>(T(e.emit_radiance[0]), T(e.emit_radiance[1]), T(e.emit_radiance[2]));
             en.is_light = !en.emit.is_black();
             en.no_denoise = e.no_denoise != 0;
+            en.med_in = e.medium_in;
+            en.med_out = e.medium_out;
+            if (en.med_in >= int32_t(d.medium_count) || en.med_out >= int32_t(d.medium_count))
+                throw std::runtime_error("bad medium index");
         }
+        media.clear();
+        media.resize(d.medium_count);
+        for (uint32_t i = 0; i < d.medium_count; ++i) media[i].init(d.media[i], eps);
         materials.resize(d.material_count);
         for (uint32_t i = 0; i < d.material_count; ++i) materials[i].d = d.materials[i];
         env_kind = d.env.kind;
@@ -1417,10 +1688,28 @@ template<class T> struct PixelResult {
 struct TraceParams { int min_depth = 5, max_depth = 10, specular_depth = 20; float cont_prob = 0.9f; };
 
 template<class T>
-PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, Sampler<T> &sam, Counters *c) {
+V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> &inct, const BSDF<T> &bsdf,
+                       const T s5[5], Sampler<T> &trs, Counters *c);
+template<class T>
+V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSDF<T> &bsdf, const T s3[3],
+                      Sampler<T> &trs, Counters *c);
+template<class T>
+V3<T> mis_sample_light_medium(const Scene<T> &scene, int32_t light, const MediumScatter<T> &ms,
+                              const Medium<T> &med, Sampler<T> &sam, Sampler<T> &trs, Counters *c);
+template<class T>
+V3<T> mis_sample_bsdf_medium(const Scene<T> &scene, const MediumScatter<T> &ms, const Medium<T> &med,
+                             const T s3[3], Sampler<T> &trs, Counters *c);
+
+// `sam` is the path's stream in Atrc's draw order; `trs` feeds the ratio-tracking
+// transmittance estimates of NEE and MIS-BSDF rays (a separate stream: the GPU
+// evaluates them at another point of the pipeline than Atrc's call order).
+template<class T>
+PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, Sampler<T> &sam, Sampler<T> &trs,
+                         Counters *c) {
     const T eps = scene.eps;
     V3<T> coef(1);
     PixelResult<T> pixel;
+    int scattering_count = 0;
     for (int depth = 1, s_depth = 1; depth <= p.max_depth; ++depth) {
         if (depth > p.min_depth) {                                          // :30-35
             if (sam.u1() > T(p.cont_prob)) return pixel;
@@ -1439,7 +1728,35 @@ PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, 
             pixel.albedo = bsdf.albedo;
             if (ent.no_denoise) pixel.denoise = 0;
         }
-        // void medium: no scattering, throughput 1 (medium_void.cpp:28-33)
+        // medium between r.o and the hit (:64-117)
+        const Medium<T> &med = scene.medium_of(inct, inct.wr);
+        if (scattering_count < med.max_scat) {
+            const MediumScatter<T> ms = med.sample_scattering(r.o, inct.pos, sam, scattering_count > 0);
+            coef *= ms.throughput;
+            if (ms.happened) {
+                ++scattering_count;
+                V3<T> direct;
+                for (int32_t light : scene.lights) direct += coef * mis_sample_light_medium(scene, light, ms, med, sam, trs, c);
+                {
+                    T s3[3];
+                    for (int k = 0; k < 3; ++k) s3[k] = sam.u1();
+                    direct += coef * mis_sample_bsdf_medium(scene, ms, med, s3, trs, c);
+                }
+                pixel.value += direct / T(1);
+                const T u0 = sam.u1(), u1 = sam.u1();
+                (void)sam.u1();
+                V3<T> dir;
+                T f;
+                ms.phase.sample(ms.wr, u0, u1, &dir, &f);
+                if (f == 0 || f < eps) return pixel;
+                r = Ray<T>(ms.pos, dir.normalize());
+                coef *= V3<T>(f) / f;
+                continue;
+            }
+        } else {
+            coef *= med.ab(r.o, inct.pos, sam);
+        }
+        scattering_count = 0;
         if (depth == 1 && ent.is_light)                                     // :123-130
             pixel.value += coef * scene.area_radiance(inct.entity, inct.geometry_coord.z, inct.wr);
 
@@ -1447,12 +1764,12 @@ PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, 
         for (int32_t light : scene.lights) {
             T s5[5];
             for (int k = 0; k < 5; ++k) s5[k] = sam.u1();
-            direct += coef * mis_sample_light(scene, light, inct, bsdf, s5, c);
+            direct += coef * mis_sample_light(scene, light, inct, bsdf, s5, trs, c);
         }
         {
             T s3[3];
             for (int k = 0; k < 3; ++k) s3[k] = sam.u1();
-            direct += coef * mis_sample_bsdf(scene, inct, bsdf, s3, c);
+            direct += coef * mis_sample_bsdf(scene, inct, bsdf, s3, trs, c);
         }
         pixel.value += direct * (T(1) / T(1));
 
@@ -1471,7 +1788,7 @@ PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, 
 // mis_sample_area_light / mis_sample_envir_light (direct_illum.cpp:11-42, 72-99)
 template<class T>
 V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> &inct, const BSDF<T> &bsdf,
-                       const T s5[5], Counters *c) {
+                       const T s5[5], Sampler<T> &trs, Counters *c) {
     const T eps = scene.eps;
     if (light >= 0) {
         const LightSample<T> ls = scene.area_sample(light, inct.pos, s5);
@@ -1479,11 +1796,15 @@ V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> 
         const T len = (ls.pos - inct.pos).length() - eps;
         if (len <= eps) return V3<T>(0);
         const V3<T> to_light = (ls.pos - inct.pos).normalize();
-        if (scene.any(Ray<T>(inct.pos, to_light, eps, len), c)) return V3<T>(0);
+        // Atrc tests visibility first; the factors only multiply, so the
+        // visibility test is moved last (where the wavefront learns it) and the
+        // transmittance draws happen in the same order on CPU and GPU
+        const Medium<T> &med = scene.medium_of(inct, to_light);
         const V3<T> f = bsdf.eval(to_light, inct.wr);
         if (f.is_black()) return V3<T>(0);
-        const V3<T> ff = ls.radiance * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
+        const V3<T> ff = med.tr(ls.pos, inct.pos, trs) * ls.radiance * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
         const T bpdf = bsdf.pdf(to_light, inct.wr);
+        if (scene.any(Ray<T>(inct.pos, to_light, eps, len), c)) return V3<T>(0);
         return ff / (ls.pdf + bpdf);
     }
     // native sky: uniform sphere direction, emit position on the bounding plane
@@ -1491,25 +1812,28 @@ V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> 
     const V3<T> pos = scene.env_emit_pos(inct.pos, dir);
     const V3<T> rad = scene.sky_radiance(dir);
     if (rad.is_black()) return V3<T>(0);
-    if (!scene.visible(inct.pos, pos, c)) return V3<T>(0);
     const V3<T> to_light = (pos - inct.pos).normalize();
     const V3<T> f = bsdf.eval(to_light, inct.wr);
     if (f.is_black()) return V3<T>(0);
     const V3<T> ff = rad * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
     const T bpdf = bsdf.pdf(to_light, inct.wr);
+    if (!scene.visible(inct.pos, pos, c)) return V3<T>(0);
     return ff / (uniform_on_sphere_pdf<T> + bpdf);
 }
 
 // mis_sample_bsdf (direct_illum.cpp:130-192)
 template<class T>
-V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSDF<T> &bsdf, const T s3[3], Counters *c) {
+V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSDF<T> &bsdf, const T s3[3],
+                      Sampler<T> &trs, Counters *c) {
     const T eps = scene.eps;
     BSDFSample<T> bs = bsdf.sample(inct.wr, s3[0], s3[1], s3[2], eps);
     if (bs.f.is_black()) return V3<T>(0);
     bs.dir = bs.dir.normalize();
     const Ray<T> nr(inct.eps_offset(bs.dir, eps), bs.dir);
     EntityHit<T> h;
-    if (!scene.closest(nr, &h, c)) {
+    const bool hit = scene.closest(nr, &h, c);
+    const Medium<T> &med = scene.medium_of(inct, bs.dir);
+    if (!hit) {
         if (scene.env_kind == AGX_ENV_NONE) return V3<T>(0);
         const V3<T> lr = scene.sky_radiance(nr.d);
         if (lr.is_black()) return V3<T>(0);
@@ -1520,10 +1844,53 @@ V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSD
     if (!scene.entities[size_t(h.entity)].is_light) return V3<T>(0);
     const V3<T> lrad = scene.area_radiance(h.entity, h.geometry_coord.z, h.wr);
     if (lrad.is_black()) return V3<T>(0);
-    const V3<T> f = lrad * bs.f * std::abs(dot(inct.geometry_coord.z, nr.d));
+    const V3<T> f = med.tr(nr.o, h.pos, trs) * lrad * bs.f * std::abs(dot(inct.geometry_coord.z, nr.d));
     if (bs.is_delta) return f / bs.pdf;
     const T lpdf = scene.area_pdf(h.entity, nr.o, h.pos, h.geometry_coord.z);
     return f / (bs.pdf + lpdf);
+}
+
+// mis_sample_area_light at a medium point (direct_illum.cpp:44-70); the
+// environment light contributes nothing there and draws nothing (:101-108)
+template<class T>
+V3<T> mis_sample_light_medium(const Scene<T> &scene, int32_t light, const MediumScatter<T> &ms,
+                              const Medium<T> &med, Sampler<T> &sam, Sampler<T> &trs, Counters *c) {
+    if (light < 0) return V3<T>(0);
+    T s5[5];
+    for (int k = 0; k < 5; ++k) s5[k] = sam.u1();
+    const LightSample<T> ls = scene.area_sample(light, ms.pos, s5);
+    if (ls.radiance.is_black() || !ls.pdf) return V3<T>(0);
+    const V3<T> to_light = (ls.pos - ms.pos).normalize();
+    const T fp = ms.phase.eval(to_light, ms.wr);
+    if (fp == 0) return V3<T>(0);
+    const V3<T> f = med.tr(ms.pos, ls.pos, trs) * ls.radiance * V3<T>(fp);
+    if (!scene.visible(ls.pos, ms.pos, c)) return V3<T>(0);  // visibility last (see mis_sample_light)
+    return f / (ls.pdf + fp);
+}
+
+// mis_sample_bsdf at a medium point (direct_illum.cpp:194-253)
+template<class T>
+V3<T> mis_sample_bsdf_medium(const Scene<T> &scene, const MediumScatter<T> &ms, const Medium<T> &med,
+                             const T s3[3], Sampler<T> &trs, Counters *c) {
+    V3<T> dir;
+    T f;
+    ms.phase.sample(ms.wr, s3[0], s3[1], &dir, &f);
+    if (f == 0) return V3<T>(0);
+    dir = dir.normalize();
+    const Ray<T> nr(ms.pos, dir);
+    EntityHit<T> h;
+    if (!scene.closest(nr, &h, c)) {
+        if (scene.env_kind == AGX_ENV_NONE) return V3<T>(0);
+        const V3<T> lf = scene.sky_radiance(nr.d);
+        if (lf.is_black()) return V3<T>(0);
+        return lf * V3<T>(f) / (f + uniform_on_sphere_pdf<T>);
+    }
+    if (!scene.entities[size_t(h.entity)].is_light) return V3<T>(0);
+    const V3<T> lf = scene.area_radiance(h.entity, h.geometry_coord.z, h.wr);
+    if (lf.is_black()) return V3<T>(0);
+    const V3<T> ff = med.tr(nr.o, h.pos, trs) * lf * V3<T>(f);
+    const T lpdf = scene.area_pdf(h.entity, nr.o, h.pos, h.geometry_coord.z);
+    return ff / (f + lpdf);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1606,14 +1973,15 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                     T weight = 0;
                     const uint64_t pix = uint64_t(py) * uint64_t(W) + uint64_t(px);
                     for (int s = 0; s < rp.spp; ++s) {
-                        Sampler<T> sam;
+                        Sampler<T> sam, trs;
                         sam.rng.seed_path(rp.seed, pix, uint64_t(rp.sample_begin + s));
+                        trs.rng.seed_path(rp.seed ^ TR_STREAM_KEY, pix, uint64_t(rp.sample_begin + s));
                         const float fu = sam.rng.uniform(), fv = sam.rng.uniform();
                         // pixel coordinates are float in Atrc (real = float)
                         const float pxf = float(px) + fu, pyf = float(py) + fv;
                         const T au = sam.u1(), av = sam.u1();
                         const Ray<T> ray = cam.sample(T(pxf / float(W)), T(pyf / float(H)), au, av);
-                        const PixelResult<T> pr = trace_std(tp, scene, ray, sam, &c);
+                        const PixelResult<T> pr = trace_std(tp, scene, ray, sam, trs, &c);
                         if (!pr.value.is_finite()) continue;
                         // apply_image_filter with box radius r: a sample reaches only its
                         // own pixel, and only if |px + 0.5 - sample| <= r on both axes

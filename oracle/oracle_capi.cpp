@@ -173,4 +173,73 @@ int oracle_bsdf(const agx_material_desc *m, int precision, const double *geo_z, 
     return run(float(0));
 }
 
+// Medium queries on medium `mi` of a created scene, n trials along the segment
+// a -> b, trial i drawing from the PCG stream (seed, pixel = i, sample = 0).
+// mode 0: tr -> 3 floats; 1: ab -> 3; 2: sample_scattering(indirect) ->
+// happened, pos[3], wr[3], throughput[3], g (11 floats).
+int oracle_medium(void *h, int mi, int mode, const double *a, const double *b, int indirect, uint64_t seed,
+                  uint32_t n, double *out) {
+    using namespace oracle;
+    auto *H = static_cast<Holder *>(h);
+    auto run = [&](const auto &sc) -> int {
+        using T = typename std::decay_t<decltype(sc)>::Real;
+        if (mi < 0 || size_t(mi) >= sc.media.size()) { g_err = "medium index out of range"; return -1; }
+        const Medium<T> &m = sc.media[size_t(mi)];
+        const V3<T> A{T(a[0]), T(a[1]), T(a[2])}, B{T(b[0]), T(b[1]), T(b[2])};
+        for (uint32_t i = 0; i < n; ++i) {
+            Sampler<T> s;
+            s.rng.seed_path(seed, i, 0);
+            if (mode == 0 || mode == 1) {
+                const V3<T> r = mode == 0 ? m.tr(A, B, s) : m.ab(A, B, s);
+                out[3 * i] = double(r.x); out[3 * i + 1] = double(r.y); out[3 * i + 2] = double(r.z);
+            } else {
+                const MediumScatter<T> r = m.sample_scattering(A, B, s, indirect != 0);
+                double *o = out + 11 * size_t(i);
+                o[0] = r.happened ? 1 : 0;
+                o[1] = double(r.pos.x); o[2] = double(r.pos.y); o[3] = double(r.pos.z);
+                o[4] = double(r.wr.x); o[5] = double(r.wr.y); o[6] = double(r.wr.z);
+                o[7] = double(r.throughput.x); o[8] = double(r.throughput.y); o[9] = double(r.throughput.z);
+                o[10] = double(r.phase.g);
+            }
+        }
+        return 0;
+    };
+    return H->precision == 64 ? run(H->sd) : run(H->sf);
+}
+
+// 3D texture lookups of medium `mi`: which 0 density, 1 albedo, 2 g; n points -> 3 floats each.
+int oracle_tex3d(void *h, int mi, int which, const double *uvw, uint32_t n, double *out) {
+    using namespace oracle;
+    auto *H = static_cast<Holder *>(h);
+    auto run = [&](const auto &sc) -> int {
+        using T = typename std::decay_t<decltype(sc)>::Real;
+        if (mi < 0 || size_t(mi) >= sc.media.size()) { g_err = "medium index out of range"; return -1; }
+        const Medium<T> &m = sc.media[size_t(mi)];
+        const Texture3D<T> &t = which == 0 ? m.density : (which == 1 ? m.albedo : m.gtex);
+        for (uint32_t i = 0; i < n; ++i) {
+            const V3<T> r = t.sample(V3<T>{T(uvw[3 * i]), T(uvw[3 * i + 1]), T(uvw[3 * i + 2])});
+            out[3 * i] = double(r.x); out[3 * i + 1] = double(r.y); out[3 * i + 2] = double(r.z);
+        }
+        return 0;
+    };
+    return H->precision == 64 ? run(H->sd) : run(H->sf);
+}
+
+// Henyey-Greenstein: mode 0 eval(wi, wo) -> 1 value; 1 sample(wo, u, v) -> dir[3], f.
+int oracle_hg(double g, int mode, const double *wi, const double *wo, double u, double v, double *out) {
+    using namespace oracle;
+    HG<double> hg;
+    hg.g = g;
+    const V3<double> vo{wo[0], wo[1], wo[2]};
+    if (mode == 0) {
+        out[0] = hg.eval(V3<double>{wi[0], wi[1], wi[2]}, vo);
+    } else {
+        V3<double> d;
+        double f;
+        hg.sample(vo, u, v, &d, &f);
+        out[0] = d.x; out[1] = d.y; out[2] = d.z; out[3] = f;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -39,6 +39,11 @@ def load() -> C.CDLL:
     lib.oracle_distribution.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, P(C.c_double)]
     lib.oracle_bsdf.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), C.c_double,
                                 C.c_double, C.c_double, C.c_int, P(C.c_double)]
+    lib.oracle_medium.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), C.c_int, C.c_uint64,
+                                  C.c_uint32, P(C.c_double)]
+    lib.oracle_tex3d.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), C.c_uint32, P(C.c_double)]
+    lib.oracle_hg.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
+                              P(C.c_double)]
     _lib = lib
     return lib
 
@@ -90,6 +95,26 @@ class OracleScene:
             raise RuntimeError(self.lib.oracle_last_error().decode())
         return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v, "counters": dict(zip(COUNTER_NAMES, cnt.tolist()))}
 
+    def medium(self, mi: int, mode: str, a, b, n: int, seed: int = 1, indirect: bool = False) -> np.ndarray:
+        """n independent trials of Medium::tr ('tr'), ::ab ('ab') or ::sample_scattering
+        ('scatter': columns happened, pos[3], wr[3], throughput[3], g) along a -> b."""
+        m = {"tr": 0, "ab": 1, "scatter": 2}[mode]
+        width = 11 if m == 2 else 3
+        out = np.zeros((n, width), np.float64)
+        a = np.ascontiguousarray(a, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        if self.lib.oracle_medium(self.h, mi, m, _d(a), _d(b), int(indirect), seed, n, _d(out)) != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return out
+
+    def tex3d(self, mi: int, which: str, uvw) -> np.ndarray:
+        uvw = np.ascontiguousarray(np.atleast_2d(uvw), np.float64)
+        out = np.zeros((len(uvw), 3), np.float64)
+        w = {"density": 0, "albedo": 1, "g": 2}[which]
+        if self.lib.oracle_tex3d(self.h, mi, w, _d(uvw), len(uvw), _d(out)) != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return out
+
     def render(self, params, threads: int | None = None, rows: tuple | None = None, aovs: bool = True):
         W, H = params.width, params.height
         rgb = np.zeros((H, W, 3), np.float64)
@@ -133,3 +158,16 @@ def bsdf(material_desc, mode: int, wi=(0, 0, 1), wo=(0, 0, 1), u=0.5, v=0.5, w=0
     gz = None if geo_z is None else np.asarray(geo_z, np.float64)
     load().oracle_bsdf(C.byref(material_desc), precision, _d(gz), _d(wi), _d(wo), u, v, w, mode, _d(out))
     return out
+
+
+def hg_eval(g: float, wi, wo) -> float:
+    out = np.zeros(1, np.float64)
+    load().oracle_hg(g, 0, _d(np.asarray(wi, np.float64)), _d(np.asarray(wo, np.float64)), 0.0, 0.0, _d(out))
+    return float(out[0])
+
+
+def hg_sample(g: float, wo, u: float, v: float):
+    out = np.zeros(4, np.float64)
+    z = np.zeros(3, np.float64)
+    load().oracle_hg(g, 1, _d(z), _d(np.asarray(wo, np.float64)), u, v, _d(out))
+    return out[:3], float(out[3])

@@ -120,3 +120,86 @@ def test_cornell_json_equals_python_scene(tmp_path):
     ia = po.OracleScene(hd, 32).render(p, threads=4)["image"]
     ib = po.OracleScene(pd, 32).render(p, threads=4)["image"]
     np.testing.assert_array_equal(ia, ib)
+
+
+def _medium_box_json(material, med_in, med_out=None):
+    e = {"type": "geometric",
+         "geometry": {"type": "quad", "A": [0, 0, 0], "B": [1, 0, 0], "C": [1, 1, 0], "D": [0, 1, 0]},
+         "material": material, "med_in": med_in}
+    if med_out is not None:
+        e["med_out"] = med_out
+    return e
+
+
+def test_media_and_texture3d_creators(tmp_path):
+    # image3d "real" binary file: int32 width, height, depth, then float texels in (z, y, x) order
+    W, H, D = 3, 2, 4
+    grid = np.arange(W * H * D, dtype=np.float32).reshape(D, H, W) / 10
+    with open(tmp_path / "dens.bin", "wb") as f:
+        f.write(np.array([W, H, D], np.int32).tobytes())
+        f.write(grid.tobytes())
+    # gray8 ASCII: Atrc's operator>> on uint8_t reads single characters
+    (tmp_path / "alb.txt").write_text("1 1 2\n AZ")
+    homo = {"type": "homogeneous", "sigma_a": [0.1, 0.2, 0.3], "sigma_s": [0.5], "g": 0.25,
+            "max_scattering_count": 7}
+    het = {"type": "heterogeneous",
+           "transform": [{"type": "scale", "ratio": [2, 2, 2]}],
+           "density": {"type": "image3d", "format": "real", "binary_filename": "${scene-directory}/dens.bin",
+                       "sampler": "nearest", "wrap_u": "repeat", "wrap_w": "mirror", "inv_gamma": 2},
+           "albedo": {"type": "image3d", "format": "gray8", "ascii_filename": "${scene-directory}/alb.txt"},
+           "g": {"type": "constant", "texel": [0.3]},
+           "white_for_indirect": 1}
+    inv = {"type": "invisible_surface"}
+    ents = [_medium_box_json(inv, homo), _medium_box_json(inv, het, {"type": "void"}),
+            _medium_box_json({"type": "ideal_black"}, {"type": "void"})]
+    h = capi.HostScene(wrap(ents), str(tmp_path))
+    d = h.desc()
+    assert d.medium_count == 2
+    assert d.materials[d.entities[0].material].kind == capi.MAT_INVISIBLE_SURFACE
+    assert (d.entities[0].medium_in, d.entities[0].medium_out) == (0, -1)
+    assert (d.entities[1].medium_in, d.entities[1].medium_out) == (1, -1)
+    assert (d.entities[2].medium_in, d.entities[2].medium_out) == (-1, -1)
+    m0, m1 = d.media[0], d.media[1]
+    assert m0.kind == capi.MEDIUM_HOMOGENEOUS and m0.max_scattering_count == 7
+    np.testing.assert_allclose(list(m0.sigma_a), [0.1, 0.2, 0.3], rtol=1e-6)
+    np.testing.assert_allclose(list(m0.sigma_s), [0.5, 0.5, 0.5], rtol=1e-6)
+    assert m0.g == pytest.approx(0.25)
+    assert m1.kind == capi.MEDIUM_HETEROGENEOUS and m1.white_for_indirect == 1
+    assert m1.max_scattering_count == 2 ** 31 - 1
+    np.testing.assert_allclose(list(m1.local_to_world), [2, 0, 0, 0, 0, 2, 0, 0, 0, 0, 2, 0], atol=1e-6)
+    np.testing.assert_allclose(list(m1.world_to_local), [0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0], atol=1e-6)
+    t = m1.density
+    assert (t.kind, t.width, t.height, t.depth, t.channels, t.linear) == (capi.TEX3D_GRID, W, H, D, 1, 0)
+    assert list(t.wrap) == [capi.WRAP_REPEAT, capi.WRAP_CLAMP, capi.WRAP_MIRROR] and t.inv_gamma == 2
+    np.testing.assert_array_equal(np.ctypeslib.as_array(t.data, (W * H * D,)), grid.ravel())
+    a = m1.albedo
+    assert (a.width, a.height, a.depth, a.channels, a.linear) == (1, 1, 2, 1, 1)
+    np.testing.assert_allclose(np.ctypeslib.as_array(a.data, (2,)), [ord("A") / 255, ord("Z") / 255], rtol=1e-6)
+    assert m1.g3.kind == capi.TEX3D_CONSTANT and m1.g3.value[0] == pytest.approx(0.3)
+    # the oracle accepts the same descriptor and reads the texels through it
+    o = po.OracleScene(d, 64)
+    v = o.tex3d(1, "density", [[0.5 / 3, 0.25, 0.125]])[0]
+    assert v[0] == pytest.approx(float(grid[0, 0, 0]) ** 2)
+
+
+def test_texture3d_common_params():
+    # inv_u flips u, uvw_perm permutes, then the user transform (applied first)
+    het = {"type": "heterogeneous", "transform": [],
+           "density": {"type": "image3d", "format": "real", "ascii_filename": "${scene-directory}/d.txt",
+                       "inv_u": 1, "uvw_perm": [2, 0, 1], "transform": [{"type": "translate", "offset": [0.1, 0, 0]}]},
+           "albedo": {"type": "constant", "texel": [1]}, "g": {"type": "constant", "texel": [0]}}
+    import tempfile, os
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "d.txt"), "w") as f:
+            f.write("1 1 1 0.5")
+        h = capi.HostScene(wrap([_medium_box_json({"type": "invisible_surface"}, het)]), td)
+        m = np.array(list(h.desc().media[0].density.transform)).reshape(3, 4)
+    # p -> T: p + (0.1,0,0) -> Perm: (p_z, p_x + 0.1, p_y) -> inv_u: (1 - p_z, p_x + 0.1, p_y)
+    p = np.array([0.2, 0.3, 0.7])
+    np.testing.assert_allclose(m[:, :3] @ p + m[:, 3], [1 - 0.7, 0.3, 0.3], atol=1e-6)
+    with pytest.raises(RuntimeError, match="wrap"):
+        bad = dict(het, density=dict(het["density"], wrap_v="bogus"))
+        with tempfile.TemporaryDirectory() as td:
+            with open(os.path.join(td, "d.txt"), "w") as f:
+                f.write("1 1 1 0.5")
+            capi.HostScene(wrap([_medium_box_json({"type": "invisible_surface"}, bad)]), td)
