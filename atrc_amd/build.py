@@ -49,9 +49,9 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
 
 
 def build_gpu() -> Path:
-    srcs = [CSRC / "agx_kernels.hip", CSRC / "agx_api.cpp", CSRC / "agx_build.cpp"]
+    srcs = [CSRC / "agx_kernels.hip", CSRC / "agx_api.cpp", CSRC / "agx_build.cpp", CSRC / "agx_bvh4.cpp"]
     tmp = GPU_LIB.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(tmp), *map(str, srcs)])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-pthread", "-Wl,--no-undefined", "-o", str(tmp), *map(str, srcs)])
     os.replace(tmp, GPU_LIB)
     return GPU_LIB
 

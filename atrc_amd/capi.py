@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 1
+AGX_ABI_VERSION = 2
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -27,6 +27,7 @@ TEX3D_CONSTANT, TEX3D_GRID = range(2)
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
 MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE = range(6)
 ENV_NONE, ENV_NATIVE_SKY = 0, 1
+ACCEL_ATRC, ACCEL_BVH4 = 0, 1
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
 
 F3 = C.c_float * 3
@@ -85,7 +86,7 @@ class agx_scene_desc(C.Structure):
         ("abi_version", C.c_uint32), ("entity_count", C.c_uint32), ("entities", C.POINTER(agx_entity_desc)),
         ("material_count", C.c_uint32), ("materials", C.POINTER(agx_material_desc)), ("env", agx_env_desc),
         ("aggregate", C.c_int32), ("max_leaf_size", C.c_int32), ("medium_count", C.c_uint32),
-        ("media", C.POINTER(agx_medium_desc)),
+        ("media", C.POINTER(agx_medium_desc)), ("accel", C.c_int32),
     ]
 
 
@@ -118,7 +119,8 @@ class agx_debug_summary(C.Structure):
         ("entities", C.c_uint32), ("lights", C.c_uint32), ("blas_nodes", C.c_uint32), ("blas_prims", C.c_uint32),
         ("tlas_nodes", C.c_uint32), ("max_blas_depth", C.c_uint32), ("max_tlas_depth", C.c_uint32),
         ("device_bytes", C.c_uint64), ("world_centre_x", C.c_float), ("world_centre_y", C.c_float),
-        ("world_centre_z", C.c_float), ("world_radius", C.c_float),
+        ("world_centre_z", C.c_float), ("world_radius", C.c_float), ("bvh4_nodes", C.c_uint64),
+        ("bvh4_leaves", C.c_uint64), ("bvh4_depth", C.c_uint32), ("pad_", C.c_uint32), ("bvh4_sah", C.c_double),
     ]
 
 
@@ -126,6 +128,7 @@ class agx_debug_summary(C.Structure):
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
+    "agx_debug_bvh4",
 ]
 
 
@@ -161,6 +164,7 @@ def load_library() -> C.CDLL:
     lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
     lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
+    lib.agx_debug_bvh4.argtypes = [P(agx_scene_desc), P(C.c_uint64), P(C.c_uint64), P(C.c_float), P(C.c_float)]
     if lib.agx_abi_version() != AGX_ABI_VERSION:
         raise RuntimeError("libatrc_gpu.so ABI version mismatch")
     _lib = lib
@@ -276,6 +280,17 @@ def debug_scene_summary(desc: agx_scene_desc) -> dict:
     s = agx_debug_summary()
     _check(lib.agx_debug_scene_summary(C.byref(desc), C.byref(s)))
     return {name: getattr(s, name) for name, _ in s._fields_}
+
+
+def debug_bvh4(desc: agx_scene_desc) -> tuple:
+    """(nodes (n, 8, 4) float32, prims (m, 3, 4) float32) of an ACCEL_BVH4 descriptor."""
+    lib = load_library()
+    nn, npr = C.c_uint64(0), C.c_uint64(0)
+    _check(lib.agx_debug_bvh4(C.byref(desc), C.byref(nn), C.byref(npr), None, None))
+    nodes = np.zeros((nn.value, 8, 4), np.float32)
+    prims = np.zeros((npr.value, 3, 4), np.float32)
+    _check(lib.agx_debug_bvh4(C.byref(desc), C.byref(nn), C.byref(npr), fptr(nodes), fptr(prims)))
+    return nodes, prims
 
 
 # ---------------------------------------------------------------------------------- C++ host

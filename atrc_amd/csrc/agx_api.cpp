@@ -54,6 +54,7 @@ struct agx_ctx {
     DevScene ds{};
     std::vector<void *> scene_allocs;
     int n_lights = 0;
+    int accel = AGX_ACCEL_ATRC;
     // wavefront buffers
     PathBuffers pb{};
     std::vector<void *> wave_allocs;
@@ -251,7 +252,7 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     DevScene ds{};
     DevEntity *ents = nullptr;
     DevMaterial *mats = nullptr;
-    vec4 *bn = nullptr, *bp = nullptr, *bi = nullptr, *tn = nullptr;
+    vec4 *bn = nullptr, *bp = nullptr, *bi = nullptr, *tn = nullptr, *b4n = nullptr, *b4p = nullptr;
     int32_t *tp = nullptr, *ai = nullptr;
     float *ap = nullptr, *gr = nullptr;
     DevMedium *md = nullptr;
@@ -268,6 +269,8 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     if (e == hipSuccess) { e = upload(&ai, hs.alias_idx, c->stream); track(ai); }
     if (e == hipSuccess) { e = upload(&md, hs.media, c->stream); track(md); }
     if (e == hipSuccess) { e = upload(&gr, hs.grids, c->stream); track(gr); }
+    if (e == hipSuccess) { e = upload(&b4n, hs.b4_nodes, c->stream); track(b4n); }
+    if (e == hipSuccess) { e = upload(&b4p, hs.b4_prims, c->stream); track(b4p); }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         c->free_scene();
@@ -285,6 +288,16 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     ds.alias_idx = ai;
     ds.media = md;
     ds.grids = gr;
+    ds.accel = hs.accel;
+    ds.b4_nodes = b4n;
+    ds.b4_prims = b4p;
+    // a BVH4 traversal keeps at most 3 pending children per level
+    ds.bvh4_large_stack = 3 * hs.b4_depth + 1 > BVH4_STACK_SMALL ? 1 : 0;
+    if (hs.accel == AGX_ACCEL_BVH4 && 3 * hs.b4_depth + 1 > BVH4_STACK_LARGE) {
+        c->free_scene();
+        return fail(AGX_E_LIMIT, "BVH4 depth " + std::to_string(hs.b4_depth) + " exceeds the traversal stack");
+    }
+    c->accel = hs.accel;
     ds.n_ent = int32_t(hs.ents.size());
     ds.aggregate = hs.aggregate;
     ds.n_lights = int32_t(hs.lights.size());
@@ -346,7 +359,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
 
     const bool stats = p->collect_stats != 0;
     agx_stats_t st{};
-    st.node_bytes = 32;
+    st.node_bytes = c->accel == AGX_ACCEL_BVH4 ? 16 * BVH4_NODE_VEC4 : 32;
     st.prim_bytes = 48;
     st.info_bytes = 96;
     if (stats) HIP_TRY(hipMemsetAsync(c->dcnt, 0, 2 * sizeof(DevCounters), c->stream));
@@ -528,7 +541,7 @@ int agx_trace_rays(agx_ctx *c, const float *o, const float *d, const float *tmin
         if (v) std::memcpy(v, hv.data(), n * 4);
     }
     agx_stats_t st{};
-    st.node_bytes = 32;
+    st.node_bytes = c->accel == AGX_ACCEL_BVH4 ? 16 * BVH4_NODE_VEC4 : 32;
     st.prim_bytes = 48;
     st.info_bytes = 96;
     if (any_hit) st.shadow_rays = n; else st.closest_rays = n;
@@ -603,6 +616,26 @@ int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out) 
     out->world_centre_y = hs.world.y;
     out->world_centre_z = hs.world.z;
     out->world_radius = hs.world.w;
+    out->bvh4_nodes = hs.b4_nodes.size() / BVH4_NODE_VEC4;
+    out->bvh4_leaves = hs.b4_leaves;
+    out->bvh4_depth = uint32_t(hs.b4_depth);
+    out->bvh4_sah = hs.b4_sah;
+    out->device_bytes += (hs.b4_nodes.size() + hs.b4_prims.size()) * sizeof(vec4);
+    return AGX_OK;
+}
+
+int agx_debug_bvh4(const agx_scene_desc *desc, uint64_t *n_nodes, uint64_t *n_prims, float *nodes_out,
+                   float *prims_out) {
+    if (!desc || !n_nodes || !n_prims) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (desc->accel != AGX_ACCEL_BVH4) return fail(AGX_E_INVALID_ARG, "descriptor does not ask for a BVH4");
+    HostScene hs;
+    std::string err;
+    const int rc = build_host_scene(*desc, &hs, &err);
+    if (rc != AGX_OK) return fail(rc, err);
+    *n_nodes = hs.b4_nodes.size() / BVH4_NODE_VEC4;
+    *n_prims = hs.b4_prims.size() / 3;
+    if (nodes_out) std::memcpy(nodes_out, hs.b4_nodes.data(), hs.b4_nodes.size() * sizeof(vec4));
+    if (prims_out) std::memcpy(prims_out, hs.b4_prims.data(), hs.b4_prims.size() * sizeof(vec4));
     return AGX_OK;
 }
 

@@ -8,6 +8,7 @@
 //   entity BVH:   src/tracer/src/core/aggregate/entity_bvh.cpp:46-116
 // This file is compiled with -ffp-contract=off.
 #include "agx_build.h"
+#include "agx_bvh4.h"
 
 #include <algorithm>
 #include <cmath>
@@ -566,6 +567,73 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
     const float diag = len3(sub(world.hi, world.lo));
     const F3 centre = mul(add(world.lo, world.hi), 0.5f);
     S.world = vec4{centre.x, centre.y, centre.z, 1.1f * diag / 2};
+
+    S.accel = desc.accel;
+    if (desc.accel == AGX_ACCEL_BVH4) {
+        // one reference per mesh triangle (in the mesh BVH's slot order) and per analytic entity
+        std::vector<PrimRef> refs;
+        std::vector<uint32_t> ref_ent, ref_slot;
+        refs.reserve(size_t(S.total_tris) + desc.entity_count);
+        ref_ent.reserve(refs.capacity());
+        ref_slot.reserve(refs.capacity());
+        for (uint32_t i = 0; i < desc.entity_count; ++i) {
+            const DevEntity &d = S.ents[i];
+            if (d.kind == AGX_GEOM_TRIANGLE_BVH) {
+                for (uint32_t k = 0; k < d.prim_count; ++k) {
+                    const uint32_t slot = d.prim_base + k;
+                    const vec4 &a = S.blas_prims[3 * size_t(slot)], &ba = S.blas_prims[3 * size_t(slot) + 1],
+                               &ca = S.blas_prims[3 * size_t(slot) + 2];
+                    const float xs[3] = {a.x, a.x + ba.x, a.x + ca.x}, ys[3] = {a.y, a.y + ba.y, a.y + ca.y},
+                                zs[3] = {a.z, a.z + ba.z, a.z + ca.z};
+                    PrimRef r;
+                    r.lo[0] = std::min({xs[0], xs[1], xs[2]}); r.hi[0] = std::max({xs[0], xs[1], xs[2]});
+                    r.lo[1] = std::min({ys[0], ys[1], ys[2]}); r.hi[1] = std::max({ys[0], ys[1], ys[2]});
+                    r.lo[2] = std::min({zs[0], zs[1], zs[2]}); r.hi[2] = std::max({zs[0], zs[1], zs[2]});
+                    refs.push_back(r);
+                    ref_ent.push_back(i);
+                    ref_slot.push_back(slot);
+                }
+            } else {
+                PrimRef r;
+                r.lo[0] = bounds[i].lo.x; r.lo[1] = bounds[i].lo.y; r.lo[2] = bounds[i].lo.z;
+                r.hi[0] = bounds[i].hi.x; r.hi[1] = bounds[i].hi.y; r.hi[2] = bounds[i].hi.z;
+                refs.push_back(r);
+                ref_ent.push_back(i);
+                ref_slot.push_back(BVH4_ANALYTIC);
+            }
+        }
+        if (refs.empty()) { *err = "scene has no primitives"; return AGX_E_INVALID_ARG; }
+        Bvh4 b4;
+        try {
+            build_bvh4(refs, 4, &b4);
+        } catch (const std::exception &ex) {
+            *err = std::string("BVH4 build: ") + ex.what();
+            return AGX_E_LIMIT;
+        }
+        S.b4_nodes = std::move(b4.nodes);
+        S.b4_prims.resize(3 * refs.size());
+        for (size_t k = 0; k < b4.order.size(); ++k) {
+            const uint32_t r = b4.order[k];
+            vec4 *o = &S.b4_prims[3 * k];
+            const uint32_t slot = ref_slot[r];
+            if (slot != BVH4_ANALYTIC) {
+                o[0] = S.blas_prims[3 * size_t(slot)];
+                o[1] = S.blas_prims[3 * size_t(slot) + 1];
+                o[2] = S.blas_prims[3 * size_t(slot) + 2];
+            } else {
+                o[0] = o[1] = o[2] = vec4{0, 0, 0, 0};
+            }
+            o[0].w = bits_f(ref_ent[r]);
+            o[1].w = bits_f(slot);
+            o[2].w = 0;
+        }
+        S.b4_depth = b4.depth;
+        S.b4_leaves = b4.leaves;
+        S.b4_sah = b4.sah;
+    } else if (desc.accel != AGX_ACCEL_ATRC) {
+        *err = "unknown acceleration structure kind";
+        return AGX_E_INVALID_ARG;
+    }
 
     S.aggregate = desc.aggregate;
     if (desc.aggregate == AGX_AGGREGATE_BVH) {

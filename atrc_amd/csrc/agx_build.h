@@ -18,6 +18,11 @@ struct HostScene {
     std::vector<int32_t> lights;
     std::vector<DevMedium> media;
     std::vector<float> grids;
+    int32_t accel = AGX_ACCEL_ATRC;
+    std::vector<vec4> b4_nodes, b4_prims;  // AGX_ACCEL_BVH4
+    int b4_depth = 0;
+    uint64_t b4_leaves = 0;
+    double b4_sah = 0;
     int32_t aggregate = AGX_AGGREGATE_NATIVE;
     int32_t env_kind = AGX_ENV_NONE;
     vec4 sky_top{0, 0, 0, 0}, sky_bottom{0, 0, 0, 0}, world{0, 0, 0, 1};

@@ -1,0 +1,323 @@
+// Binned-SAH binary BVH collapsed into a 4-wide BVH (production acceleration
+// mode; the layout is described in agx_device.h).
+#include "agx_bvh4.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <stdexcept>
+
+namespace agx {
+namespace {
+
+constexpr int BINS = 32;
+constexpr float C_TRAV = 1.0f;   // SAH cost of a node visit relative to one primitive test
+constexpr int MEDIAN_DEPTH = 40; // below this binary depth splits are object medians
+constexpr uint32_t PARALLEL_MIN = 1u << 16;
+
+struct Box {
+    float lo[3] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
+                   std::numeric_limits<float>::infinity()};
+    float hi[3] = {-std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity(),
+                   -std::numeric_limits<float>::infinity()};
+    void grow(const float *l, const float *h) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], l[k]);
+            hi[k] = std::max(hi[k], h[k]);
+        }
+    }
+    void grow(const Box &b) { grow(b.lo, b.hi); }
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (!(dx >= 0) || !(dy >= 0) || !(dz >= 0)) return 0;
+        return 2 * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+struct BNode {
+    Box box;
+    int32_t child[2] = {-1, -1};
+    uint32_t start = 0, count = 0;  // leaf range in `order` when child[0] < 0
+};
+
+// Each subtree is built into its own node vector (so that large subtrees can be
+// built on worker threads) and spliced into the parent's afterwards.
+struct Subtree {
+    std::vector<BNode> nodes;  // nodes[0] = subtree root; child indices are local
+};
+
+struct Builder {
+    const std::vector<PrimRef> &refs;
+    std::vector<float> cen;     // 3 per reference
+    std::vector<uint32_t> order;
+    int max_leaf;
+
+    Builder(const std::vector<PrimRef> &r, int ml) : refs(r), max_leaf(ml) {
+        cen.resize(3 * r.size());
+        order.resize(r.size());
+        for (size_t i = 0; i < r.size(); ++i) {
+            for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (r[i].lo[k] + r[i].hi[k]);
+            order[i] = uint32_t(i);
+        }
+    }
+
+    void make_leaf(Subtree &t, int32_t self, uint32_t b, uint32_t e) {
+        t.nodes[size_t(self)].start = b;
+        t.nodes[size_t(self)].count = e - b;
+    }
+
+    // Builds [b, e) into t; returns the local index of its root.
+    int32_t build(Subtree &t, uint32_t b, uint32_t e, int depth) {
+        const int32_t self = int32_t(t.nodes.size());
+        t.nodes.emplace_back();
+        Box box, cb;
+        for (uint32_t i = b; i < e; ++i) {
+            const uint32_t r = order[i];
+            box.grow(refs[r].lo, refs[r].hi);
+            cb.grow(&cen[3 * r], &cen[3 * r]);
+        }
+        t.nodes[size_t(self)].box = box;
+        const uint32_t n = e - b;
+        if (n <= 1 || (depth >= MEDIAN_DEPTH && n <= uint32_t(max_leaf))) {
+            make_leaf(t, self, b, e);
+            return self;
+        }
+        // best SAH split over the three axes
+        int best_axis = -1, best_bin = -1;
+        float best_cost = std::numeric_limits<float>::infinity();
+        const float parent_area = box.area();
+        if (depth < MEDIAN_DEPTH) {
+            for (int axis = 0; axis < 3; ++axis) {
+                const float ext = cb.hi[axis] - cb.lo[axis];
+                if (!(ext > 0)) continue;
+                const float scale = float(BINS) / ext;
+                Box bins[BINS];
+                uint32_t cnt[BINS] = {};
+                for (uint32_t i = b; i < e; ++i) {
+                    const uint32_t r = order[i];
+                    int k = int((cen[3 * r + axis] - cb.lo[axis]) * scale);
+                    k = std::min(BINS - 1, std::max(0, k));
+                    bins[k].grow(refs[r].lo, refs[r].hi);
+                    ++cnt[k];
+                }
+                float right_area[BINS];
+                uint32_t right_cnt[BINS];
+                Box acc;
+                uint32_t c = 0;
+                for (int k = BINS - 1; k > 0; --k) {
+                    acc.grow(bins[k]);
+                    c += cnt[k];
+                    right_area[k] = acc.area();
+                    right_cnt[k] = c;
+                }
+                acc = Box();
+                c = 0;
+                for (int k = 0; k < BINS - 1; ++k) {
+                    acc.grow(bins[k]);
+                    c += cnt[k];
+                    if (c == 0 || right_cnt[k + 1] == 0) continue;
+                    const float cost = acc.area() * float(c) + right_area[k + 1] * float(right_cnt[k + 1]);
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = axis;
+                        best_bin = k;
+                    }
+                }
+            }
+        }
+        if (best_axis >= 0 && parent_area > 0) {
+            const float split_cost = C_TRAV + best_cost / parent_area;
+            if (n <= uint32_t(max_leaf) && float(n) <= split_cost) {
+                make_leaf(t, self, b, e);
+                return self;
+            }
+        } else if (n <= uint32_t(max_leaf)) {
+            make_leaf(t, self, b, e);  // coincident centroids, few references
+            return self;
+        }
+        uint32_t mid;
+        if (best_axis >= 0) {
+            const float ext = cb.hi[best_axis] - cb.lo[best_axis];
+            const float scale = float(BINS) / ext;
+            const float lo = cb.lo[best_axis];
+            auto it = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t r) {
+                int k = int((cen[3 * r + best_axis] - lo) * scale);
+                k = std::min(BINS - 1, std::max(0, k));
+                return k <= best_bin;
+            });
+            mid = uint32_t(it - order.begin());
+        } else {
+            // object median on the largest centroid extent (deep trees, coincident centroids)
+            int axis = 0;
+            for (int k = 1; k < 3; ++k)
+                if (cb.hi[k] - cb.lo[k] > cb.hi[axis] - cb.lo[axis]) axis = k;
+            mid = b + n / 2;
+            std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
+                             [&](uint32_t x, uint32_t y) { return cen[3 * x + axis] < cen[3 * y + axis]; });
+        }
+        if (mid == b || mid == e) mid = b + n / 2;
+        if (n >= PARALLEL_MIN) {
+            // left half on a worker thread, right half here; splice both
+            auto fut = std::async(std::launch::async, [this, b, mid, depth]() {
+                Subtree l;
+                l.nodes.reserve(2 * size_t(mid - b) / size_t(std::max(1, max_leaf)) + 1);
+                build(l, b, mid, depth + 1);
+                return l;
+            });
+            Subtree r;
+            build(r, mid, e, depth + 1);
+            Subtree l = fut.get();
+            const int32_t lbase = int32_t(t.nodes.size());
+            splice(t, l);
+            const int32_t rbase = int32_t(t.nodes.size());
+            splice(t, r);
+            t.nodes[size_t(self)].child[0] = lbase;
+            t.nodes[size_t(self)].child[1] = rbase;
+        } else {
+            const int32_t l = build(t, b, mid, depth + 1);
+            const int32_t r = build(t, mid, e, depth + 1);
+            t.nodes[size_t(self)].child[0] = l;
+            t.nodes[size_t(self)].child[1] = r;
+        }
+        return self;
+    }
+
+    static void splice(Subtree &dst, const Subtree &src) {
+        const int32_t base = int32_t(dst.nodes.size());
+        for (const BNode &n : src.nodes) {
+            BNode m = n;
+            if (m.child[0] >= 0) {
+                m.child[0] += base;
+                m.child[1] += base;
+            }
+            dst.nodes.push_back(m);
+        }
+    }
+};
+
+uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+float bitsf(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Widens a box by a few ulps so that the rounded slab test of a ray that
+// meets a primitive exactly on its bound never culls it.
+void pad(Box *b) {
+    for (int k = 0; k < 3; ++k) {
+        const float m = std::max(std::fabs(b->lo[k]), std::fabs(b->hi[k]));
+        const float e = m * 0x1p-20f + 0x1p-60f;
+        b->lo[k] -= e;
+        b->hi[k] += e;
+    }
+}
+
+struct Collapser {
+    const std::vector<BNode> &bn;
+    Bvh4 *out;
+    int max_depth = 0;
+
+    uint32_t leaf_code(const BNode &n) const {
+        return BVH4_LEAF | ((n.count - 1) << 27) | n.start;
+    }
+
+    // Emits the 4-wide node for binary node `bi` (an interior node or a lone
+    // leaf root); returns its index.
+    uint32_t emit(int32_t bi, int depth) {
+        max_depth = std::max(max_depth, depth);
+        std::vector<int32_t> kids;
+        const BNode &self = bn[size_t(bi)];
+        if (self.child[0] < 0) {
+            kids.push_back(bi);
+        } else {
+            kids.push_back(self.child[0]);
+            kids.push_back(self.child[1]);
+            while (kids.size() < 4) {
+                int pick = -1;
+                float best = -1;
+                for (size_t k = 0; k < kids.size(); ++k) {
+                    const BNode &c = bn[size_t(kids[k])];
+                    if (c.child[0] < 0) continue;
+                    const float a = c.box.area();
+                    if (a > best) { best = a; pick = int(k); }
+                }
+                if (pick < 0) break;
+                const BNode &c = bn[size_t(kids[size_t(pick)])];
+                kids[size_t(pick)] = c.child[0];
+                kids.insert(kids.begin() + pick + 1, c.child[1]);
+            }
+        }
+        const uint32_t idx = uint32_t(out->nodes.size() / BVH4_NODE_VEC4);
+        out->nodes.resize(out->nodes.size() + BVH4_NODE_VEC4, vec4{0, 0, 0, 0});
+        uint32_t codes[4] = {BVH4_EMPTY, BVH4_EMPTY, BVH4_EMPTY, BVH4_EMPTY};
+        Box boxes[4];
+        for (size_t k = 0; k < kids.size(); ++k) {
+            const BNode &c = bn[size_t(kids[k])];
+            boxes[k] = c.box;
+            pad(&boxes[k]);
+            if (c.child[0] < 0) {
+                codes[k] = leaf_code(c);
+                out->leaves++;
+            } else {
+                codes[k] = emit(kids[k], depth + 1);
+            }
+        }
+        vec4 *n = &out->nodes[size_t(idx) * BVH4_NODE_VEC4];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = codes[k] != BVH4_EMPTY;
+            (&n[0].x)[k] = used ? boxes[k].lo[0] : 0;
+            (&n[1].x)[k] = used ? boxes[k].hi[0] : 0;
+            (&n[2].x)[k] = used ? boxes[k].lo[1] : 0;
+            (&n[3].x)[k] = used ? boxes[k].hi[1] : 0;
+            (&n[4].x)[k] = used ? boxes[k].lo[2] : 0;
+            (&n[5].x)[k] = used ? boxes[k].hi[2] : 0;
+            (&n[6].x)[k] = bitsf(codes[k]);
+        }
+        return idx;
+    }
+};
+
+}  // namespace
+
+void build_bvh4(const std::vector<PrimRef> &refs, int max_leaf, Bvh4 *out) {
+    if (refs.empty()) throw std::invalid_argument("build_bvh4: no primitives");
+    if (refs.size() > size_t(BVH4_FIRST_MASK)) throw std::invalid_argument("build_bvh4: too many primitives");
+    max_leaf = std::min(BVH4_MAX_LEAF, std::max(1, max_leaf));
+    Builder b(refs, max_leaf);
+    Subtree t;
+    t.nodes.reserve(2 * refs.size() / size_t(max_leaf) + 16);
+    b.build(t, 0, uint32_t(refs.size()), 0);
+    *out = Bvh4{};
+    out->nodes.reserve(t.nodes.size() / 2 * BVH4_NODE_VEC4 + BVH4_NODE_VEC4);
+    Collapser c{t.nodes, out};
+    c.emit(0, 1);
+    out->depth = c.max_depth;
+    out->order = std::move(b.order);
+    // SAH cost of the 4-wide tree relative to the root area
+    const float root_area = std::max(t.nodes[0].box.area(), 1e-30f);
+    double cost = 0;
+    const size_t nn = out->nodes.size() / BVH4_NODE_VEC4;
+    for (size_t i = 0; i < nn; ++i) {
+        const vec4 *n = &out->nodes[i * BVH4_NODE_VEC4];
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t code = fbits((&n[6].x)[k]);
+            if (code == BVH4_EMPTY) continue;
+            Box bx;
+            bx.lo[0] = (&n[0].x)[k]; bx.hi[0] = (&n[1].x)[k];
+            bx.lo[1] = (&n[2].x)[k]; bx.hi[1] = (&n[3].x)[k];
+            bx.lo[2] = (&n[4].x)[k]; bx.hi[2] = (&n[5].x)[k];
+            const double a = double(bx.area()) / root_area;
+            cost += (code & BVH4_LEAF) ? a * double(((code >> 27) & 15u) + 1) : a * C_TRAV;
+        }
+    }
+    out->sah = cost + C_TRAV;
+}
+
+}  // namespace agx

@@ -25,6 +25,21 @@ struct alignas(16) vec4 {
 constexpr uint32_t INTERIOR = 0xffffffffu;   // BLAS node: start == INTERIOR marks an interior node
 constexpr uint32_t TLAS_LEAF = 0x80000000u;  // TLAS node: left word has this bit for leaves
 constexpr int MAX_LIGHTS = 16;
+// BVH4 (production acceleration mode): a node is 8 vec4 = 128 B:
+//   [0] lo.x of children 0..3, [1] hi.x, [2] lo.y, [3] hi.y, [4] lo.z, [5] hi.z,
+//   [6] child codes (uint bits), [7] unused.
+// Child code: BVH4_EMPTY, an interior node index (bit 31 clear), or a leaf
+// BVH4_LEAF | (count - 1) << 27 | first primitive.  A primitive is 3 vec4:
+//   (A, entity), (B - A, slot), (C - A, 0); slot = index into blas_prims/blas_info
+//   of a mesh triangle, or BVH4_ANALYTIC for a whole analytic entity.
+constexpr int BVH4_NODE_VEC4 = 8;
+constexpr uint32_t BVH4_EMPTY = 0xffffffffu;
+constexpr uint32_t BVH4_LEAF = 0x80000000u;
+constexpr uint32_t BVH4_FIRST_MASK = 0x07ffffffu;
+constexpr int BVH4_MAX_LEAF = 8;
+constexpr uint32_t BVH4_ANALYTIC = 0xffffffffu;
+constexpr int BVH4_STACK_SMALL = 64;   // traversal stack entries; the large variant
+constexpr int BVH4_STACK_LARGE = 192;  // serves trees deeper than (64 - 1) / 3 levels
 constexpr int BLAS_STACK = 96;               // >= max BLAS depth + 2 (checked at upload)
 constexpr int TLAS_STACK = 64;
 constexpr int LDS_STACK = 32;                // LDS stack column per thread (TLAS + BLAS entries)
@@ -101,6 +116,10 @@ struct DevScene {
     int32_t n_lights;
     int32_t env_kind;
     int32_t lds_stack;     // 1: TLAS depth + BLAS depth fit the LDS stack column
+    int32_t accel;         // AGX_ACCEL_*
+    int32_t bvh4_large_stack;  // 1: the BVH4 is too deep for BVH4_STACK_SMALL
+    const vec4 *b4_nodes;
+    const vec4 *b4_prims;
     int32_t lights[MAX_LIGHTS];
     vec4 sky_top, sky_bottom;
     vec4 world;            // (centre.xyz, radius)

@@ -333,11 +333,11 @@ __device__ __forceinline__ void sphere_abc(V o, V d, float r, float *A, float *B
     *C = dot(o, o) - r * r;
 }
 
-// Closest hit of one entity; on success *tmax is lowered to the hit distance.
-template <bool STATS, class Stack>
-__device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tmin, float *tmax, HitRec *h,
-                               Counts &c, Stack &stk) {
-    const DevEntity &e = s.ents[ei];
+// Closest hit of one analytic entity (quad, triangle, sphere, disk); on success
+// *tmax is lowered to the hit distance.
+template <bool STATS>
+__device__ bool analytic_closest(const DevEntity &e, int32_t ei, V o, V d, float tmin, float *tmax, HitRec *h,
+                                 Counts &c) {
     switch (e.kind) {
     case AGX_GEOM_QUAD: {  // quad.cpp:30-58
         float t, u, v;
@@ -387,18 +387,25 @@ __device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tm
         h->t = t; h->u = 0; h->v = 0; h->prim = 0; h->ent = ei; *tmax = t;
         return true;
     }
-    default: {
-        if (!blas_closest<STATS>(s, e, o, d, tmin, *tmax, h, c, stk)) return false;
-        h->ent = ei;
-        *tmax = h->t;
-        return true;
-    }
+    default:
+        return false;
     }
 }
 
+// Closest hit of one entity; on success *tmax is lowered to the hit distance.
 template <bool STATS, class Stack>
-__device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, float tmax, Counts &c, Stack &stk) {
+__device__ bool entity_closest(const DevScene &s, int32_t ei, V o, V d, float tmin, float *tmax, HitRec *h,
+                               Counts &c, Stack &stk) {
     const DevEntity &e = s.ents[ei];
+    if (e.kind != AGX_GEOM_TRIANGLE_BVH) return analytic_closest<STATS>(e, ei, o, d, tmin, tmax, h, c);
+    if (!blas_closest<STATS>(s, e, o, d, tmin, *tmax, h, c, stk)) return false;
+    h->ent = ei;
+    *tmax = h->t;
+    return true;
+}
+
+template <bool STATS>
+__device__ bool analytic_any(const DevEntity &e, V o, V d, float tmin, float tmax, Counts &c) {
     switch (e.kind) {
     case AGX_GEOM_QUAD:  // quad.cpp:24-28
         if (STATS) c.tests++;
@@ -428,8 +435,15 @@ __device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, 
         return p.x * p.x + p.y * p.y <= e.a.y;
     }
     default:
-        return blas_any<STATS>(s, e, o, d, tmin, tmax, c, stk);
+        return false;
     }
+}
+
+template <bool STATS, class Stack>
+__device__ bool entity_any(const DevScene &s, int32_t ei, V o, V d, float tmin, float tmax, Counts &c, Stack &stk) {
+    const DevEntity &e = s.ents[ei];
+    if (e.kind != AGX_GEOM_TRIANGLE_BVH) return analytic_any<STATS>(e, o, d, tmin, tmax, c);
+    return blas_any<STATS>(s, e, o, d, tmin, tmax, c, stk);
 }
 
 // AABB::intersect (common.h:384-395) with std::min/max operand order
@@ -493,6 +507,175 @@ __device__ bool scene_any(const DevScene &s, V o, V d, float tmin, float tmax, C
         if (entity_any<STATS>(s, i, o, d, tmin, tmax, c, stk)) return true;
     return false;
 }
+
+// ---------------------------------------------------------------- BVH4 (production mode)
+// Slab test of one child box with Atrc's arithmetic, NaN axes ignored (fminf /
+// fmaxf) and the exit distance widened by 2^-21 relative, so that a ray meeting
+// a primitive exactly on its bound is never culled.  Returns the entry
+// distance, or +INF for a miss.
+__device__ __forceinline__ float b4_slab(float lx, float hx, float ly, float hy, float lz, float hz, V o, V inv,
+                                         float tmin, float tmax) {
+    const float nx = inv.x * (lx - o.x), fx = inv.x * (hx - o.x);
+    const float ny = inv.y * (ly - o.y), fy = inv.y * (hy - o.y);
+    const float nz = inv.z * (lz - o.z), fz = inv.z * (hz - o.z);
+    const float t0 = fmaxf(fmaxf(tmin, fminf(nx, fx)), fmaxf(fminf(ny, fy), fminf(nz, fz)));
+    const float t1 = fminf(fminf(tmax, fmaxf(nx, fx)), fminf(fmaxf(ny, fy), fmaxf(nz, fz)));
+    return t0 <= t1 * 1.000000477f ? t0 : INF_F;
+}
+
+__device__ __forceinline__ void b4_children(const float4 *n, V o, V inv, float tmin, float tmax, float t[4],
+                                            uint32_t k[4]) {
+    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], cc = n[6];
+    k[0] = __float_as_uint(cc.x); k[1] = __float_as_uint(cc.y);
+    k[2] = __float_as_uint(cc.z); k[3] = __float_as_uint(cc.w);
+    t[0] = k[0] != BVH4_EMPTY ? b4_slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tmax) : INF_F;
+    t[1] = k[1] != BVH4_EMPTY ? b4_slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tmax) : INF_F;
+    t[2] = k[2] != BVH4_EMPTY ? b4_slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tmax) : INF_F;
+    t[3] = k[3] != BVH4_EMPTY ? b4_slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tmax) : INF_F;
+}
+
+__device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
+    if (t[b] < t[a]) {
+        const float tt = t[a]; t[a] = t[b]; t[b] = tt;
+        const uint32_t kk = k[a]; k[a] = k[b]; k[b] = kk;
+    }
+}
+
+// Closest hit over the BVH4: children visited near-first, pending children
+// kept with their entry distance and skipped once a closer hit is known.
+template <bool STATS, int CAP>
+__device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c) {
+    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
+    const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
+    uint32_t stk_k[CAP];
+    float stk_t[CAP];
+    int sp = 0;
+    uint32_t code = 0;
+    bool hit = false;
+    for (;;) {
+        if (!(code & BVH4_LEAF)) {
+            if (STATS) c.visits++;
+            float t[4];
+            uint32_t k[4];
+            b4_children(N + size_t(code) * BVH4_NODE_VEC4, o, inv, tmin, tmax, t, k);
+            cswap(t, k, 0, 1); cswap(t, k, 2, 3); cswap(t, k, 0, 2); cswap(t, k, 1, 3); cswap(t, k, 1, 2);
+            if (t[0] != INF_F) {
+                if (t[3] != INF_F && sp < CAP) { stk_k[sp] = k[3]; stk_t[sp] = t[3]; ++sp; }
+                if (t[2] != INF_F && sp < CAP) { stk_k[sp] = k[2]; stk_t[sp] = t[2]; ++sp; }
+                if (t[1] != INF_F && sp < CAP) { stk_k[sp] = k[1]; stk_t[sp] = t[1]; ++sp; }
+                code = k[0];
+                continue;
+            }
+        } else {
+            const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
+            for (uint32_t i = first; i < first + cnt; ++i) {
+                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
+                const int32_t ent = __float_as_int(p0.w);
+                const uint32_t slot = __float_as_uint(p1.w);
+                if (slot != BVH4_ANALYTIC) {
+                    if (STATS) c.tests++;
+                    const float4 p2 = P[3 * size_t(i) + 2];
+                    float tt, uu, vv;
+                    if (mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv)) {
+                        tmax = tt;
+                        h->t = tt; h->u = uu; h->v = vv; h->ent = ent; h->prim = int32_t(slot);
+                        hit = true;
+                    }
+                } else if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tmax, h, c)) {
+                    hit = true;
+                }
+            }
+        }
+        // next pending child that may still hold a closer hit
+        for (;;) {
+            if (sp == 0) return hit;
+            --sp;
+            if (stk_t[sp] <= tmax) { code = stk_k[sp]; break; }
+        }
+    }
+}
+
+template <bool STATS, int CAP>
+__device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c) {
+    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
+    const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
+    uint32_t stk_k[CAP];
+    int sp = 0;
+    uint32_t code = 0;
+    for (;;) {
+        if (!(code & BVH4_LEAF)) {
+            if (STATS) c.visits++;
+            float t[4];
+            uint32_t k[4];
+            b4_children(N + size_t(code) * BVH4_NODE_VEC4, o, inv, tmin, tmax, t, k);
+            uint32_t next = BVH4_EMPTY;
+            for (int j = 0; j < 4; ++j) {
+                if (t[j] == INF_F) continue;
+                if (next == BVH4_EMPTY) next = k[j];
+                else if (sp < CAP) stk_k[sp++] = k[j];
+            }
+            if (next != BVH4_EMPTY) { code = next; continue; }
+        } else {
+            const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
+            for (uint32_t i = first; i < first + cnt; ++i) {
+                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
+                const uint32_t slot = __float_as_uint(p1.w);
+                if (slot != BVH4_ANALYTIC) {
+                    if (STATS) c.tests++;
+                    const float4 p2 = P[3 * size_t(i) + 2];
+                    if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) return true;
+                } else if (analytic_any<STATS>(s.ents[__float_as_int(p0.w)], o, d, tmin, tmax, c)) {
+                    return true;
+                }
+            }
+        }
+        if (sp == 0) return false;
+        code = stk_k[--sp];
+    }
+}
+
+// Traversal variants: Atrc's structures with the private or the LDS stack, or
+// the BVH4 with the small or the large private stack.
+enum { TM_ATRC_PRIV = 0, TM_ATRC_LDS = 1, TM_B4_SMALL = 2, TM_B4_LARGE = 3 };
+
+template <bool STATS, int TM>
+__device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h,
+                                              Counts &c, uint32_t *lds_col) {
+    if constexpr (TM == TM_ATRC_LDS) {
+        LdsStack stk{lds_col, 0};
+        return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
+    } else if constexpr (TM == TM_ATRC_PRIV) {
+        PrivStack stk;
+        return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
+    } else if constexpr (TM == TM_B4_SMALL) {
+        return b4_closest<STATS, BVH4_STACK_SMALL>(s, o, d, tmin, tmax, h, c);
+    } else {
+        return b4_closest<STATS, BVH4_STACK_LARGE>(s, o, d, tmin, tmax, h, c);
+    }
+}
+
+template <bool STATS, int TM>
+__device__ __forceinline__ bool query_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c,
+                                          uint32_t *lds_col) {
+    if constexpr (TM == TM_ATRC_LDS) {
+        LdsStack stk{lds_col, 0};
+        return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
+    } else if constexpr (TM == TM_ATRC_PRIV) {
+        PrivStack stk;
+        return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
+    } else if constexpr (TM == TM_B4_SMALL) {
+        return b4_any<STATS, BVH4_STACK_SMALL>(s, o, d, tmin, tmax, c);
+    } else {
+        return b4_any<STATS, BVH4_STACK_LARGE>(s, o, d, tmin, tmax, c);
+    }
+}
+
+// LDS stack columns exist only in the TM_ATRC_LDS instantiations
+#define AGX_LDS_COLUMN(TM)                                                  \
+    __shared__ uint32_t lds_stack_[(TM) == TM_ATRC_LDS ? LDS_STACK : 1][256]; \
+    uint32_t *lds_col = &lds_stack_[0][threadIdx.x]
 
 // ---------------------------------------------------------------- surface reconstruction
 struct Surf {
@@ -999,11 +1182,12 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
-// Stack selection: USE_LDS = the LDS column (LDS_STACK entries per thread).
-template <bool STATS, bool USE_LDS>
+// TM = traversal variant (TM_*).
+template <bool STATS, int TM>
 __global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[0];
     if (blockIdx.x * blockDim.x >= n) return;  // whole block past the queue
+    AGX_LDS_COLUMN(TM);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
@@ -1012,14 +1196,7 @@ __global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers p
         const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
         HitRec h;
         h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-        if (USE_LDS) {
-            __shared__ uint32_t lds[LDS_STACK][256];
-            LdsStack stk{&lds[0][threadIdx.x], 0};
-            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
-        } else {
-            PrivStack stk;
-            found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
-        }
+        found = query_closest<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
         if (!found) h.ent = -1;
         pb.hit[ri] = make_float4(h.t, h.u, h.v, __int_as_float(h.ent));
         pb.hit_prim[ri] = h.prim;
@@ -1027,24 +1204,18 @@ __global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers p
     if (STATS) flush_counts(c, found, cnt);
 }
 
-template <bool STATS, bool USE_LDS>
+template <bool STATS, int TM>
 __global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[1];
     if (blockIdx.x * blockDim.x >= n) return;
+    AGX_LDS_COLUMN(TM);
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool occ = false;
     if (tid < n) {
         const uint32_t ri = pb.q_shadow[tid];
         const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
-        if (USE_LDS) {
-            __shared__ uint32_t lds[LDS_STACK][256];
-            LdsStack stk{&lds[0][threadIdx.x], 0};
-            occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c, stk);
-        } else {
-            PrivStack stk;
-            occ = scene_any<STATS>(s, mk(o4), mk(d4), d4.w, o4.w, c, stk);
-        }
+        occ = query_any<STATS, TM>(s, mk(o4), mk(d4), d4.w, o4.w, c, lds_col);
         pb.occluded[ri] = occ ? 1 : 0;
     }
     if (STATS) flush_counts(c, false, cnt);
@@ -1621,37 +1792,24 @@ __global__ void k_reset_counters(uint32_t *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
-template <bool STATS, bool USE_LDS>
+template <bool STATS, int TM>
 __global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
                                                     int any_hit, int32_t *ent, int32_t *prim, float *t, float *u,
                                                     float *v, DevCounters *cnt) {
+    AGX_LDS_COLUMN(TM);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
     if (i < n) {
         const float4 o4 = o[i], d4 = d[i];
         if (any_hit) {
-            if (USE_LDS) {
-                __shared__ uint32_t lds[LDS_STACK][256];
-                LdsStack stk{&lds[0][threadIdx.x], 0};
-                found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c, stk);
-            } else {
-                PrivStack stk;
-                found = scene_any<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, c, stk);
-            }
+            found = query_any<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, c, lds_col);
             ent[i] = found ? 1 : -1;
             found = false;
         } else {
             HitRec h;
             h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-            if (USE_LDS) {
-                __shared__ uint32_t lds[LDS_STACK][256];
-                LdsStack stk{&lds[0][threadIdx.x], 0};
-                found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
-            } else {
-                PrivStack stk;
-                found = scene_closest<STATS>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, stk);
-            }
+            found = query_closest<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
             if (found) {
                 const DevEntity &e = s.ents[h.ent];
                 int32_t pid = h.prim;
@@ -1681,14 +1839,20 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
     if (rc.n_paths <= 0) return;
     hipLaunchKernelGGL(k_raygen, dim3(grid_of(uint64_t(rc.n_paths), 256)), dim3(256), 0, st, s, rc, pb);
 }
+#define AGX_LAUNCH_TM(K, TM, grid, st, ...)                                                        \
+    do {                                                                                          \
+        if (stats) hipLaunchKernelGGL((K<true, TM>), grid, dim3(256), 0, st, __VA_ARGS__);         \
+        else hipLaunchKernelGGL((K<false, TM>), grid, dim3(256), 0, st, __VA_ARGS__);              \
+    } while (0)
 #define AGX_LAUNCH_STACKED(K, grid, st, ...)                                                        \
     do {                                                                                          \
-        if (s.lds_stack) {                                                                        \
-            if (stats) hipLaunchKernelGGL((K<true, true>), grid, dim3(256), 0, st, __VA_ARGS__);   \
-            else hipLaunchKernelGGL((K<false, true>), grid, dim3(256), 0, st, __VA_ARGS__);        \
+        if (s.accel == AGX_ACCEL_BVH4) {                                                          \
+            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4_LARGE, grid, st, __VA_ARGS__);         \
+            else AGX_LAUNCH_TM(K, TM_B4_SMALL, grid, st, __VA_ARGS__);                            \
+        } else if (s.lds_stack) {                                                                 \
+            AGX_LAUNCH_TM(K, TM_ATRC_LDS, grid, st, __VA_ARGS__);                                 \
         } else {                                                                                  \
-            if (stats) hipLaunchKernelGGL((K<true, false>), grid, dim3(256), 0, st, __VA_ARGS__);  \
-            else hipLaunchKernelGGL((K<false, false>), grid, dim3(256), 0, st, __VA_ARGS__);       \
+            AGX_LAUNCH_TM(K, TM_ATRC_PRIV, grid, st, __VA_ARGS__);                                \
         }                                                                                         \
     } while (0)
 

@@ -113,6 +113,7 @@ class Scene:
     aggregate: int = capi.AGGREGATE_NATIVE
     max_leaf_size: int = 5
     media: list = field(default_factory=list)
+    accel: int = capi.ACCEL_ATRC         # acceleration structure the GPU builds at upload
 
     def add_medium(self, m: Medium) -> int:
         self.media.append(m)
@@ -191,6 +192,7 @@ class Scene:
             _fill_tex3d(d.albedo, m.albedo, keep)
             _fill_tex3d(d.g3, m.g3, keep)
             d.white_for_indirect = 1 if m.white_for_indirect else 0
+        desc.accel = int(self.accel)
         desc.medium_count = len(self.media)
         desc.media = C.cast(meds, C.POINTER(capi.agx_medium_desc))
         keep += [ents, mats, meds]

@@ -11,10 +11,13 @@ scene, BVHs and path state already resident in HBM.
 value = (closest-hit + any-hit ray queries) / wall time of the timed region.
 
 roofline: the dominant kernel is the closest-hit traversal.  Its algorithmic
-bytes = node AABB tests * 32 B + triangle tests * 48 B + hits * 96 B, counted
+bytes = node fetches * node size (128 B BVH4 node with 4 child boxes, or 32 B
+Atrc binary node per box test) + triangle tests * 48 B + hits * 96 B, counted
 by a stats pass over the first samples (per-ray averages, scaled to the timed
 rays), divided by that kernel's device time measured with HIP events on the
-library's stream during the timed region; peak = 8 TB/s HBM.
+library's stream during the timed region; peak = 8 TB/s HBM.  traffic = HBM
+bytes per launch of the same kernel from the committed rocprofv3 FETCH_SIZE /
+WRITE_SIZE summary of this workload (profiles/).
 
 cpu_baseline: the CPU restatement of Atrc `pt` (FP32, Atrc's 32x32 tile
 scheduler, one worker per core) on a band of rows of the same frame at 1 spp,
@@ -106,6 +109,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="mesh1m", choices=["mesh1m", "cornell"])
+    ap.add_argument("--accel", default="bvh4", choices=["bvh4", "atrc"],
+                    help="acceleration structure: bvh4 (SAH BVH4, production) or atrc (Atrc's own BVHs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stats-steps", type=int, default=2)
@@ -121,6 +126,7 @@ def main() -> int:
 
     t0 = time.time()
     s, cam, tr, W, H, workload = build_scene(args)
+    s.accel = capi.ACCEL_BVH4 if args.accel == "bvh4" else capi.ACCEL_ATRC
     desc, keep = s.to_desc()
     log(f"[bench] scene built: {s.triangle_count()} triangles, {len(s.entities)} entities ({time.time() - t0:.1f} s)")
     ctx = capi.Context(0)
@@ -162,7 +168,7 @@ def main() -> int:
 
     closest_bytes = bytes_per_closest * st["closest_rays"]
     achieved = closest_bytes / (st["trace_closest_ms"] * 1e-3) / 1e9
-    traffic, traffic_src = pmc_traffic("k_trace_closest", args.config)
+    traffic, traffic_src = pmc_traffic("k_trace_closest", f"{args.config}_{args.accel}")
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
@@ -176,7 +182,7 @@ def main() -> int:
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": workload, "width": W, "height": H, "spp_per_step": 1,
                    "triangles": s.triangle_count(), "max_depth": tr["max_depth"], "min_depth": tr["min_depth"],
-                   "cont_prob": tr["cont_prob"]},
+                   "cont_prob": tr["cont_prob"], "accel": args.accel},
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "paths": st["paths"]},
         "samples_per_s": st["paths"] / wall,
         "device_ms": {k: st[k] for k in ("total_ms", "trace_closest_ms", "trace_shadow_ms", "shade_ms")},

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 1
+#define AGX_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -64,6 +64,14 @@ enum {
 
 enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
 enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
+/* Acceleration structure built at upload.
+ *   AGX_ACCEL_ATRC: Atrc's own BVHs and traversal order (triangle_bvh.cpp,
+ *     entity_bvh.cpp / native_aggregate.cpp): identical hits, node visits and
+ *     primitive tests to Atrc's CPU traversal.
+ *   AGX_ACCEL_BVH4: one 4-wide SAH BVH over all primitives, near-first
+ *     traversal: identical hits (same primitive, t, u, v) except which of two
+ *     primitives at exactly equal t wins; far fewer node visits. */
+enum { AGX_ACCEL_ATRC = 0, AGX_ACCEL_BVH4 = 1 };
 enum { AGX_MEDIUM_VOID = 0, AGX_MEDIUM_HOMOGENEOUS = 1, AGX_MEDIUM_HETEROGENEOUS = 2 };
 enum { AGX_TEX3D_CONSTANT = 0, AGX_TEX3D_GRID = 1 };
 enum { AGX_WRAP_CLAMP = 0, AGX_WRAP_REPEAT = 1, AGX_WRAP_MIRROR = 2 };
@@ -166,6 +174,7 @@ typedef struct agx_scene_desc {
     int32_t max_leaf_size;     /* entity BVH leaf size (default 5) */
     uint32_t medium_count;
     const agx_medium_desc *media; /* referenced by agx_entity_desc.medium_in / medium_out */
+    int32_t accel;             /* AGX_ACCEL_* */
 } agx_scene_desc;
 
 /* PTRendererParams + the rendering settings that reach the renderer. */

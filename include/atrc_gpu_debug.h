@@ -20,6 +20,10 @@ typedef struct agx_debug_summary {
     uint32_t max_blas_depth, max_tlas_depth;
     uint64_t device_bytes;
     float world_centre_x, world_centre_y, world_centre_z, world_radius;
+    /* AGX_ACCEL_BVH4 scenes: node count, node levels, leaves, SAH cost */
+    uint64_t bvh4_nodes, bvh4_leaves;
+    uint32_t bvh4_depth, pad_;
+    double bvh4_sah;
 } agx_debug_summary;
 
 /* Atrc triangle BVH of one mesh.  nodes_out (may be NULL): node_count * 8
@@ -30,6 +34,12 @@ int agx_debug_triangle_bvh(const agx_vertex *tris, uint32_t n, uint32_t *node_co
 
 /* Build (without uploading) the device scene and report its sizes. */
 int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out);
+
+/* The BVH4 of an AGX_ACCEL_BVH4 descriptor (layout in atrc_amd/csrc/agx_device.h):
+ * *n_nodes nodes of 32 floats and *n_prims primitives of 12 floats.  Call with
+ * NULL outputs to get the counts, then with buffers of that size. */
+int agx_debug_bvh4(const agx_scene_desc *desc, uint64_t *n_nodes, uint64_t *n_prims, float *nodes_out,
+                   float *prims_out);
 
 /* Run one plain device-to-device copy kernel (k_copy_calibration) of `bytes`
  * bytes on the context's stream and wait for it: a known-size reference for

@@ -1,0 +1,116 @@
+"""Structure of the production BVH4 (AGX_ACCEL_BVH4) built at upload, checked
+on the CPU through agx_debug_bvh4: a tree over every primitive exactly once,
+leaves of at most 4 primitives, every child box enclosing its subtree, and
+primitive records that carry the mesh triangles of Atrc's own triangle BVH
+(src/tracer/src/core/geometry/triangle_bvh.cpp:24-27, 225-304) unchanged.
+"""
+import numpy as np
+import pytest
+
+from atrc_amd import capi, scenes
+
+EMPTY = 0xFFFFFFFF
+LEAF = 0x80000000
+ANALYTIC = 0xFFFFFFFF
+
+
+def _walk(nodes, prims):
+    codes = nodes[:, 6, :].view(np.uint32)
+    seen_nodes = np.zeros(len(nodes), np.int32)
+    seen_prims = np.zeros(len(prims), np.int32)
+    leaf_sizes = []
+    # subtree bounds of every node, bottom-up via explicit DFS order
+    order, stack = [], [0]
+    while stack:
+        n = stack.pop()
+        order.append(n)
+        for k in range(4):
+            c = int(codes[n, k])
+            if c != EMPTY and not (c & LEAF):
+                seen_nodes[c] += 1
+                stack.append(c)
+    A = prims[:, 0, :3].astype(np.float64)
+    B = A + prims[:, 1, :3]
+    Cc = A + prims[:, 2, :3]
+    slot = prims[:, 1, 3].view(np.uint32)
+    tri = slot != ANALYTIC
+    lo_p = np.minimum(np.minimum(A, B), Cc)
+    hi_p = np.maximum(np.maximum(A, B), Cc)
+    sub_lo = np.full((len(nodes), 3), np.inf)
+    sub_hi = np.full((len(nodes), 3), -np.inf)
+    for n in reversed(order):
+        for k in range(4):
+            c = int(codes[n, k])
+            if c == EMPTY:
+                continue
+            box_lo = np.array([nodes[n, 0, k], nodes[n, 2, k], nodes[n, 4, k]], np.float64)
+            box_hi = np.array([nodes[n, 1, k], nodes[n, 3, k], nodes[n, 5, k]], np.float64)
+            if c & LEAF:
+                first, cnt = c & 0x07FFFFFF, ((c >> 27) & 15) + 1
+                leaf_sizes.append(cnt)
+                idx = np.arange(first, first + cnt)
+                seen_prims[idx] += 1
+                t = idx[tri[idx]]
+                if len(t):
+                    assert (lo_p[t] >= box_lo).all() and (hi_p[t] <= box_hi).all()
+                    sub_lo[n] = np.minimum(sub_lo[n], lo_p[t].min(axis=0))
+                    sub_hi[n] = np.maximum(sub_hi[n], hi_p[t].max(axis=0))
+            else:
+                assert (sub_lo[c] >= box_lo).all() and (sub_hi[c] <= box_hi).all()
+                sub_lo[n] = np.minimum(sub_lo[n], sub_lo[c])
+                sub_hi[n] = np.maximum(sub_hi[n], sub_hi[c])
+    assert seen_nodes[0] == 0 and (seen_nodes[1:] == 1).all()
+    assert (seen_prims == 1).all()
+    assert max(leaf_sizes) <= 4
+    return slot
+
+
+@pytest.mark.parametrize("which", ["spheres", "cornell"])
+def test_bvh4_structure(which):
+    if which == "spheres":
+        s, _, _ = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=3, extent=20.0)
+    else:
+        s, _, _ = scenes.cornell_box()
+    s.accel = capi.ACCEL_BVH4
+    desc, keep = s.to_desc()
+    nodes, prims = capi.debug_bvh4(desc)
+    slot = _walk(nodes, prims)
+    ent = prims[:, 0, 3].view(np.int32)
+    # one record per analytic entity and per mesh triangle
+    mesh_ents = [i for i, e in enumerate(s.entities) if e.tris is not None]
+    n_tris = sum(len(s.entities[i].tris) // 3 for i in mesh_ents)
+    assert len(prims) == n_tris + len(s.entities) - len(mesh_ents)
+    analytic = slot == ANALYTIC
+    assert sorted(ent[analytic].tolist()) == [i for i in range(len(s.entities)) if i not in mesh_ents]
+    # mesh triangles: the records of Atrc's triangle BVH, slots contiguous per entity in entity order
+    base = 0
+    for i in mesh_ents:
+        tris = s.entities[i].tris
+        n = len(tris) // 3
+        ref_nodes, ref_prims, _ = capi.debug_triangle_bvh(tris)
+        sel = np.where(~analytic & (ent == i))[0]
+        assert len(sel) == n
+        local = slot[sel].astype(np.int64) - base
+        assert sorted(local.tolist()) == list(range(n))
+        np.testing.assert_array_equal(prims[sel, :, :3], ref_prims.reshape(n, 3, 4)[local][:, :, :3])
+        base += n
+    summ = capi.debug_scene_summary(desc)
+    assert summ["bvh4_nodes"] == len(nodes) and summ["bvh4_depth"] >= 1
+    assert summ["bvh4_depth"] * 3 + 1 <= 192
+
+
+def test_bvh4_is_shallower_than_atrc_bvh():
+    s, _, _ = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=3, extent=20.0)
+    s.accel = capi.ACCEL_BVH4
+    s.aggregate = capi.AGGREGATE_BVH
+    desc, keep = s.to_desc()
+    summ = capi.debug_scene_summary(desc)
+    assert summ["bvh4_depth"] < summ["max_blas_depth"] + summ["max_tlas_depth"]
+
+
+def test_unknown_accel_rejected():
+    s, _, _ = scenes.cornell_box()
+    s.accel = 7
+    desc, keep = s.to_desc()
+    with pytest.raises(capi.AgxError):
+        capi.debug_scene_summary(desc)

@@ -14,7 +14,7 @@ import math
 import numpy as np
 import pytest
 
-from atrc_amd import scenes
+from atrc_amd import capi, scenes
 from oracle import py_oracle as po
 
 from conftest import log
@@ -24,10 +24,12 @@ from test_oracle_media import _pixel_cos
 pytestmark = pytest.mark.gpu
 
 
-def test_volumetric_cornell_path_parity(gpu_ctx):
+@pytest.mark.parametrize("accel", [capi.ACCEL_ATRC, capi.ACCEL_BVH4])
+def test_volumetric_cornell_path_parity(gpu_ctx, accel):
     s, cam, tr = scenes.volumetric_cornell(grid_n=16)
-    _path_parity(gpu_ctx, s, cam, tr, 128, 128, "volumetric cornell (homogeneous + heterogeneous media)",
-                 min_frac=0.97)
+    s.accel = accel
+    _path_parity(gpu_ctx, s, cam, tr, 128, 128, f"volumetric cornell (homogeneous + heterogeneous media, "
+                 f"accel {accel})", min_frac=0.97)
 
 
 def test_absorbing_slab_gpu(gpu_ctx):

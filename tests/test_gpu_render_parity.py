@@ -46,15 +46,19 @@ def _path_parity(gpu_ctx, scene, camera, trace, W, H, label, min_frac=0.98):
     return frac
 
 
-def test_cornell_path_parity(gpu_ctx):
+@pytest.mark.parametrize("accel", [capi.ACCEL_ATRC, capi.ACCEL_BVH4])
+def test_cornell_path_parity(gpu_ctx, accel):
     s, cam, tr = scenes.cornell_box()
-    _path_parity(gpu_ctx, s, cam, tr, 128, 128, "cornell")
+    s.accel = accel
+    _path_parity(gpu_ctx, s, cam, tr, 128, 128, f"cornell (accel {accel})")
 
 
-def test_spheres_path_parity(gpu_ctx):
+@pytest.mark.parametrize("accel", [capi.ACCEL_ATRC, capi.ACCEL_BVH4])
+def test_spheres_path_parity(gpu_ctx, accel):
     s, cam, tr = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=3, extent=20.0)
+    s.accel = accel
     cam = {"pos": (-12.0, -16.0, 24.0), "dst": (10.0, 10.0, 8.0), "fov": 40.0}
-    _path_parity(gpu_ctx, s, cam, tr, 128, 96, "spheres24 (Lambert + GGX metal + sky)")
+    _path_parity(gpu_ctx, s, cam, tr, 128, 96, f"spheres24 (Lambert + GGX metal + sky, accel {accel})")
 
 
 def test_cornell_converged(gpu_ctx):
