@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 2
+AGX_ABI_VERSION = 3
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -25,7 +25,7 @@ GEOM_TRIANGLE_BVH, GEOM_QUAD, GEOM_TRIANGLE, GEOM_SPHERE, GEOM_DISK = range(5)
 MEDIUM_VOID, MEDIUM_HOMOGENEOUS, MEDIUM_HETEROGENEOUS = range(3)
 TEX3D_CONSTANT, TEX3D_GRID = range(2)
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
-MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE = range(6)
+MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE, MAT_DISNEY = range(7)
 ENV_NONE, ENV_NATIVE_SKY = 0, 1
 ACCEL_ATRC, ACCEL_BVH4 = 0, 1
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
@@ -50,7 +50,9 @@ class agx_material_desc(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("color", F3), ("color2", F3), ("eta", F3), ("k", F3),
         ("roughness", C.c_float), ("anisotropic", C.c_float), ("ior", C.c_float),
-        ("glass_separate_refraction", C.c_int32),
+        ("glass_separate_refraction", C.c_int32), ("metallic", C.c_float), ("transmission", C.c_float),
+        ("transmission_roughness", C.c_float), ("specular_tint", C.c_float), ("sheen", C.c_float),
+        ("sheen_tint", C.c_float), ("clearcoat", C.c_float), ("clearcoat_gloss", C.c_float),
     ]
 
 

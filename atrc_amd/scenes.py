@@ -33,6 +33,15 @@ class Material:
     anisotropic: float = 0.0
     ior: float = 1.5
     separate_refraction: bool = False
+    # disney: color = base_color, color2 = specular_scale
+    metallic: float = 0.0
+    transmission: float = 0.0
+    transmission_roughness: float | None = None   # None = roughness (material_creators.cpp:71-73)
+    specular_tint: float = 0.0
+    sheen: float = 0.0
+    sheen_tint: float = 0.0
+    clearcoat: float = 0.0
+    clearcoat_gloss: float = 1.0
 
 
 @dataclass
@@ -166,6 +175,14 @@ class Scene:
             d.anisotropic = m.anisotropic
             d.ior = m.ior
             d.glass_separate_refraction = 1 if m.separate_refraction else 0
+            d.metallic = m.metallic
+            d.transmission = m.transmission
+            d.transmission_roughness = m.roughness if m.transmission_roughness is None else m.transmission_roughness
+            d.specular_tint = m.specular_tint
+            d.sheen = m.sheen
+            d.sheen_tint = m.sheen_tint
+            d.clearcoat = m.clearcoat
+            d.clearcoat_gloss = m.clearcoat_gloss
         desc = capi.agx_scene_desc()
         desc.abi_version = capi.AGX_ABI_VERSION
         desc.entity_count = len(self.entities)

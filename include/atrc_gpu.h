@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 2
+#define AGX_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -59,7 +59,9 @@ enum {
     AGX_MAT_GLASS         = 2, /* color = reflection colour, color2 = refraction colour (before sqrt), ior */
     AGX_MAT_MIRROR        = 3, /* color = color_map, eta, k */
     AGX_MAT_IDEAL_BLACK   = 4,
-    AGX_MAT_INVISIBLE_SURFACE = 5  /* medium boundary: passes every ray straight through */
+    AGX_MAT_INVISIBLE_SURFACE = 5, /* medium boundary: passes every ray straight through */
+    AGX_MAT_DISNEY        = 6  /* color = base_color, color2 = specular_scale, roughness,
+                                  anisotropic, ior, and the disney_* fields */
 };
 
 enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
@@ -143,6 +145,15 @@ typedef struct agx_material_desc {
     float   anisotropic;
     float   ior;
     int32_t glass_separate_refraction; /* 1 if color_refraction_map was given separately */
+    /* disney (material_creators.cpp:56-112) */
+    float   metallic;
+    float   transmission;
+    float   transmission_roughness;
+    float   specular_tint;
+    float   sheen;
+    float   sheen_tint;
+    float   clearcoat;
+    float   clearcoat_gloss;
 } agx_material_desc;
 
 typedef struct agx_env_desc {

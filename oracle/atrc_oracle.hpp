@@ -952,6 +952,12 @@ template<class T> struct BSDF {
     V3<T> c_refl, c_refr; T ior = 1.5f;
     // mirror
     V3<T> m_eta2, m_k2;      // eta_in = 1, eta_out = ior
+    // disney (material/disney.cpp:18-457): C = albedo
+    V3<T> d_tint, d_spec_scale;
+    T d_metallic = 0, d_rough = 0, d_spec_tint = 0, d_sheen = 0, d_sheen_tint = 0, d_clearcoat = 0;
+    T d_trans = 0, d_ior = T(1.5), d_tax = 1, d_tay = 1, d_ccr = T(0.01);
+    T w_diffuse = T(0.25), w_spec = T(0.25), w_cc = T(0.25), w_trans = T(0.25);
+    T eps_ = T(3e-4);        // EPS() of the |cos| < EPS tests in eval / pdf
 
     // ---- LocalBSDF black-fringe helpers (bsdf.h:122-187)
     bool fringe(const V3<T> &w) const { return shd.in_positive_z_hemisphere(w) != geo.in_positive_z_hemisphere(w); }
@@ -1046,7 +1052,282 @@ template<class T> struct BSDF {
         return V3<T>(ax * nh.x, ay * nh.y, std::max(T(0), nh.z)).normalize();
     }
 
+    // ---- microfacet distributions (material/utility/microfacet.cpp:13-105)
+    static T sqr(T x) { return x * x; }
+    static T aniso_gtr2(T sph, T cph, T sth, T cth, T ax_, T ay_) {
+        const T A = sqr(cph / ax_) + sqr(sph / ay_);
+        const T RD = sqr(sth) * A + sqr(cth);
+        return 1 / (PI<T> * ax_ * ay_ * sqr(RD));
+    }
+    static T smith_aniso(T cp, T sp, T ax_, T ay_, T tt) {
+        const T t = sqr(ax_ * cp) + sqr(ay_ * sp);
+        const T sv = 1 + t * sqr(tt);
+        const T lambda = -T(0.5) + T(0.5) * std::sqrt(sv);
+        return 1 / (1 + lambda);
+    }
+    static T gtr1(T sth, T cth, T a) {
+        const T U = sqr(a) - 1;
+        const T LD = 2 * PI<T> * std::log(a);
+        const T RD = sqr(a * cth) + sqr(sth);
+        return U / (LD * RD);
+    }
+    static T smith_gtr2(T tt, T a) {
+        if (!tt) return 1;
+        const T root = a * tt;
+        return 2 / (1 + std::sqrt(1 + root * root));
+    }
+    static V3<T> sample_aniso_gtr2(T ax_, T ay_, T u, T v) {
+        T sph = ay_ * std::sin(2 * PI<T> * u);
+        T cph = ax_ * std::cos(2 * PI<T> * u);
+        const T nor = 1 / std::sqrt(sqr(sph) + sqr(cph));
+        sph *= nor;
+        cph *= nor;
+        const T A = sqr(cph / ax_) + sqr(sph / ay_);
+        const T cth = std::sqrt(A * (1 - v) / ((1 - A) * v + A));
+        const T sth = std::sqrt(std::max(T(0), 1 - sqr(cth)));
+        return V3<T>(sth * cph, sth * sph, cth).normalize();
+    }
+    static V3<T> sample_gtr1(T a, T u, T v) {
+        const T phi = 2 * PI<T> * u;
+        const T ct = std::sqrt((std::pow(a, 2 - 2 * v) - 1) / (sqr(a) - 1));
+        const T st = std::sqrt(std::max(T(0), 1 - ct * ct));
+        return V3<T>(st * std::cos(phi), st * std::sin(phi), ct).normalize();
+    }
+    // refl_aux::refract (utility/reflection.h:31-40); false = total internal reflection
+    static bool refract(const V3<T> &w, const V3<T> &n, T eta, V3<T> *out) {
+        const T ci = dot(w, n);
+        const T s2t = sqr(eta) * (1 - sqr(ci));
+        const T c2t = 1 - s2t;
+        if (c2t <= 0) return false;
+        const T ct = std::sqrt(c2t);
+        *out = (n * (eta * ci - ct) - w * eta).normalize();
+        return true;
+    }
+    static T one_minus_5(T x) { const T t = 1 - x; const T t2 = t * t; return t2 * t2 * t; }
+    static T cos_2_sin(T c) { return std::sqrt(std::max(T(0), 1 - c * c)); }
+    static bool zero(const V3<T> &v) { return !v.x && !v.y && !v.z; }
+    // math::mix (agz-utils, parity unpinned): GLSL mix, a (1 - t) + b t
+    static V3<T> mixv(const V3<T> &a, const V3<T> &b, T t) { return a * (1 - t) + b * t; }
+
+    // ---- Disney BSDF (material/disney.cpp:77-395)
+    V3<T> d_diffuse(T ci, T co, T cd) const {
+        const V3<T> f_lambert = albedo / PI<T>;
+        const T FL = one_minus_5(ci), FV = one_minus_5(co);
+        const T RR = 2 * d_rough * cd * cd;
+        const V3<T> retro = albedo / PI<T> * RR * (FL + FV + FL * FV * (RR - 1));
+        return f_lambert * (1 - T(0.5) * FL) * (1 - T(0.5) * FV) + retro;
+    }
+    V3<T> d_sheen_f(T cd) const { return mixv(V3<T>(1), d_tint, d_sheen_tint) * (4 * d_sheen) * one_minus_5(cd); }
+    V3<T> d_clearcoat_f(T ci, T co, T tti, T tto, T sth, T cth, T cd) const {
+        const T D = gtr1(sth, cth, d_ccr);
+        const T F = T(0.04) + (1 - T(0.04)) * one_minus_5(cd);
+        const T G = smith_gtr2(tti, T(0.25)) * smith_gtr2(tto, T(0.25));
+        return V3<T>(d_clearcoat * D * F * G / std::abs(4 * ci * co));
+    }
+    void d_angles(const V3<T> &lwi, const V3<T> &lwo, T tax, T tay, const V3<T> &lwh, T *D, T *G) const {
+        const T phi_h = phi_of(lwh);
+        const T sph = std::sin(phi_h), cph = std::cos(phi_h);
+        const T cth = lwh.z, sth = cos_2_sin(cth);
+        *D = aniso_gtr2(sph, cph, sth, cth, tax, tay);
+        const T phi_i = phi_of(lwi), phi_o = phi_of(lwo);
+        const T spi = std::sin(phi_i), cpi = std::cos(phi_i);
+        const T spo = std::sin(phi_o), cpo = std::cos(phi_o);
+        *G = smith_aniso(cpi, spi, tax, tay, tan_theta(lwi)) * smith_aniso(cpo, spo, tax, tay, tan_theta(lwo));
+    }
+    V3<T> d_trans_f(const V3<T> &lwi, const V3<T> &lwo) const {
+        const T ci = lwi.z, co = lwo.z;
+        const T eta = co > 0 ? d_ior : 1 / d_ior;
+        V3<T> lwh = (lwo + lwi * eta).normalize();
+        if (lwh.z < 0) lwh = -lwh;
+        const T cd = dot(lwo, lwh);
+        const T F = dielectric(d_ior, 1, cd);
+        T D, G;
+        d_angles(lwi, lwo, d_tax, d_tay, lwh, &D, &G);
+        const T sdem = cd + eta * dot(lwi, lwh);
+        const T corr = 1 / eta;  // TransMode::Radiance
+        const T val = (1 - F) * D * G * eta * eta * dot(lwi, lwh) * dot(lwo, lwh) * corr * corr /
+                      (ci * co * sdem * sdem);
+        const T tf = co > 0 ? d_trans : 1;
+        return vsqrt(albedo) * ((1 - d_metallic) * tf) * std::abs(val);
+    }
+    V3<T> d_inner_refl_f(const V3<T> &lwi, const V3<T> &lwo) const {
+        const V3<T> lwh = -(lwi + lwo).normalize();
+        const T cd = dot(lwo, lwh);
+        const T F = dielectric(d_ior, 1, cd);
+        T D, G;
+        d_angles(lwi, lwo, d_tax, d_tay, lwh, &D, &G);
+        return albedo * d_trans * std::abs(F * D * G / (4 * lwi.z * lwo.z));
+    }
+    V3<T> d_specular_f(const V3<T> &lwi, const V3<T> &lwo) const {
+        const T ci = lwi.z, co = lwo.z;
+        const V3<T> lwh = (lwi + lwo).normalize();
+        const T cd = dot(lwi, lwh);
+        const V3<T> cspec = mixv(mixv(V3<T>(1), d_tint, d_spec_tint), albedo, d_metallic);
+        const V3<T> diel = cspec * dielectric(d_ior, 1, cd);
+        const V3<T> cond = cspec + (V3<T>(1) - cspec) * one_minus_5(cd);
+        const V3<T> F = mixv(d_spec_scale * diel, cond, d_metallic);
+        T D, G;
+        d_angles(lwi, lwo, ax, ay, lwh, &D, &G);
+        return F * D * G / std::abs(4 * ci * co);
+    }
+    V3<T> disney_eval(const V3<T> &wi, const V3<T> &wo) const {
+        if (fringe(wi) || fringe(wo)) {
+            if (!geo.in_positive_z_hemisphere(wi) || !geo.in_positive_z_hemisphere(wo)) return V3<T>(0);
+            return albedo / PI<T>;
+        }
+        const V3<T> lwi = shd.global_to_local(wi).normalize();
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (std::abs(lwi.z) < eps_ || std::abs(lwo.z) < eps_) return V3<T>(0);
+        if (lwi.z * lwo.z < 0) {
+            if (!d_trans) return V3<T>(0);
+            return d_trans_f(lwi, lwo) * normal_corr(geo.z, shd.z, wi, eps_);
+        }
+        if (lwi.z < 0 && lwo.z < 0) {
+            if (!d_trans) return V3<T>(0);
+            return d_inner_refl_f(lwi, lwo) * normal_corr(geo.z, shd.z, wi, eps_);
+        }
+        if (lwi.z <= 0 || lwo.z <= 0) return V3<T>(0);
+        const T ci = lwi.z, co = lwo.z;
+        const V3<T> lwh = (lwi + lwo).normalize();
+        const T cd = dot(lwi, lwh);
+        V3<T> diffuse, sheen;
+        if (d_metallic < 1) {
+            diffuse = d_diffuse(ci, co, cd);
+            if (d_sheen > 0) sheen = d_sheen_f(cd);
+        }
+        const V3<T> specular = d_specular_f(lwi, lwo);
+        V3<T> clearcoat;
+        if (d_clearcoat > 0) {
+            const T cth = lwh.z;
+            clearcoat = d_clearcoat_f(ci, co, tan_theta(lwi), tan_theta(lwo), cos_2_sin(cth), cth, cd);
+        }
+        const V3<T> value = (diffuse + sheen) * ((1 - d_metallic) * (1 - d_trans)) + specular + clearcoat;
+        return value * normal_corr(geo.z, shd.z, wi, eps_);
+    }
+    T d_pdf_trans(const V3<T> &lwi, const V3<T> &lwo) const {
+        const T eta = lwo.z > 0 ? d_ior : 1 / d_ior;
+        V3<T> lwh = (lwo + lwi * eta).normalize();
+        if (lwh.z < 0) lwh = -lwh;
+        if (((lwo.z > 0) != (dot(lwh, lwo) > 0)) || ((lwi.z > 0) != (dot(lwh, lwi) > 0))) return 0;
+        const T sdem = dot(lwo, lwh) + eta * dot(lwi, lwh);
+        const T dwh_to_dwi = eta * eta * dot(lwi, lwh) / (sdem * sdem);
+        const T phi_h = phi_of(lwh);
+        const T cth = lwh.z;
+        const T D = aniso_gtr2(std::sin(phi_h), std::cos(phi_h), cos_2_sin(cth), cth, d_tax, d_tay);
+        return std::abs(cth * D * dwh_to_dwi);
+    }
+    T d_pdf_inner(const V3<T> &lwi, const V3<T> &lwo) const {
+        const V3<T> lwh = -(lwi + lwo).normalize();
+        const T phi_h = phi_of(lwh);
+        const T cth = lwh.z;
+        const T cd = dot(lwi, lwh);
+        const T D = aniso_gtr2(std::sin(phi_h), std::cos(phi_h), cos_2_sin(cth), cth, d_tax, d_tay);
+        return std::abs(cth * D / (4 * cd));
+    }
+    T disney_pdf(const V3<T> &wi, const V3<T> &wo) const {
+        if (fringe(wi) || fringe(wo)) {
+            if (geo.in_positive_z_hemisphere(wi) != geo.in_positive_z_hemisphere(wo)) return 0;
+            return zweighted_on_hemisphere_pdf(geo.global_to_local(wi).normalize().z);
+        }
+        const V3<T> lwi = shd.global_to_local(wi).normalize();
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (std::abs(lwi.z) < eps_ || std::abs(lwo.z) < eps_) return 0;
+        if (lwo.z < 0) {
+            if (!d_trans) return 0;
+            const T mF = std::min(T(0.9), std::max(T(0.1), dielectric(d_ior, 1, lwo.z)));
+            if (lwi.z > 0) return (1 - mF) * d_pdf_trans(lwi, lwo);
+            if (!mF) return 0;
+            return mF * d_pdf_inner(lwi, lwo);
+        }
+        if (lwi.z < 0) return w_trans * d_pdf_trans(lwi, lwo);
+        const T diffuse = zweighted_on_hemisphere_pdf(lwi.z);
+        // pdf_specular_clearcoat (disney.cpp:317-348): sin(phi_o) taken as |sin|, as Atrc does
+        const V3<T> lwh = (lwi + lwo).normalize();
+        const T phi_h = phi_of(lwh);
+        const T sph = std::sin(phi_h), cph = std::cos(phi_h);
+        const T cth = lwh.z, sth = cos_2_sin(cth);
+        const T cd = dot(lwi, lwh);
+        const T cpo = std::cos(phi_of(lwo));
+        const T spo = cos_2_sin(cpo);
+        const T spec_D = aniso_gtr2(sph, cph, sth, cth, ax, ay);
+        const T pdf_spec = smith_aniso(cpo, spo, ax, ay, tan_theta(lwo)) * spec_D / (4 * lwo.z);
+        const T cc_D = gtr1(sth, cth, d_ccr);
+        const T pdf_cc = cth * cc_D / (4 * cd);
+        return w_diffuse * diffuse + w_spec * pdf_spec + w_cc * pdf_cc;
+    }
+    V3<T> d_sample_trans(const V3<T> &lwo, T u, T v) const {
+        const V3<T> lwh = sample_aniso_gtr2(d_tax, d_tay, u, v);
+        if (lwh.z <= 0) return V3<T>(0);
+        if ((lwo.z > 0) != (dot(lwh, lwo) > 0)) return V3<T>(0);
+        const T eta = lwo.z > 0 ? 1 / d_ior : d_ior;
+        const V3<T> owh = dot(lwh, lwo) > 0 ? lwh : -lwh;
+        V3<T> r;
+        if (!refract(lwo, owh, eta, &r)) return V3<T>(0);
+        const V3<T> lwi = r.normalize();
+        if (lwi.z * lwo.z > 0 || ((lwi.z > 0) != (dot(lwh, lwi) > 0))) return V3<T>(0);
+        return lwi;
+    }
+    V3<T> d_sample_inner(const V3<T> &lwo, T u, T v) const {
+        const V3<T> lwh = sample_aniso_gtr2(d_tax, d_tay, u, v);
+        if (lwh.z <= 0) return V3<T>(0);
+        const V3<T> lwi = lwh * (2 * dot(lwo, lwh)) - lwo;
+        if (lwi.z > 0) return V3<T>(0);
+        return lwi.normalize();
+    }
+    BSDFSample<T> disney_sample(const V3<T> &wo, T su, T sv, T sw, T eps) const {
+        BSDFSample<T> res;
+        if (fringe(wo)) {                                        // bsdf.h:142-159
+            if (!geo.in_positive_z_hemisphere(wo)) return res;
+            V3<T> lwi; T p;
+            zweighted_on_hemisphere(su, sv, &lwi, &p);
+            if (p < eps) return res;
+            const V3<T> wi = geo.local_to_global(lwi).normalize();
+            res.dir = wi; res.f = albedo / PI<T> * normal_corr(geo.z, shd.z, wi, eps); res.pdf = p;
+            return res;
+        }
+        const V3<T> lwo = shd.global_to_local(wo).normalize();
+        if (std::abs(lwo.z) < eps) return res;
+        V3<T> lwi;
+        if (lwo.z < 0) {                                         // disney.cpp:536-560
+            if (!d_trans) return res;
+            const T mF = std::min(T(0.9), std::max(T(0.1), dielectric(d_ior, 1, lwo.z)));
+            lwi = su >= mF ? d_sample_trans(lwo, sv, sw) : d_sample_inner(lwo, sv, sw);
+        } else {                                                 // disney.cpp:562-590
+            // a diffuse pick (sel < w_diffuse) samples the cosine lobe, but the
+            // if/else chain that follows always overwrites that direction
+            T sel = su;
+            if (!(sel < w_diffuse)) sel -= w_diffuse;
+            if (sel < w_trans) {
+                lwi = d_sample_trans(lwo, sv, sw);
+            } else {
+                sel -= w_trans;
+                if (sel < w_spec) {
+                    const V3<T> lwh = ggx_vndf(lwo, sv, sw, eps).normalize();
+                    lwi = V3<T>(0);
+                    if (lwh.z > 0) {
+                        const V3<T> c = (lwh * (2 * dot(lwo, lwh)) - lwo).normalize();
+                        if (c.z > 0) lwi = c;
+                    }
+                } else {
+                    const V3<T> lwh = sample_gtr1(d_ccr, sv, sw);
+                    lwi = V3<T>(0);
+                    if (lwh.z > 0) {
+                        const V3<T> c = (lwh * (2 * dot(lwo, lwh)) - lwo).normalize();
+                        if (c.z > 0) lwi = c;
+                    }
+                }
+            }
+        }
+        if (zero(lwi)) return res;
+        const V3<T> wi = shd.local_to_global(lwi);
+        res.dir = wi;
+        res.f = disney_eval(wi, wo);
+        res.pdf = disney_pdf(wi, wo);
+        return res;
+    }
+
     V3<T> eval(const V3<T> &wi, const V3<T> &wo) const {
+        if (kind == AGX_MAT_DISNEY) return disney_eval(wi, wo);
         if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK ||
             kind == AGX_MAT_INVISIBLE_SURFACE) return V3<T>(0);
         // AggregateBSDF::eval (component/aggregate.h:39-64)
@@ -1066,6 +1347,7 @@ template<class T> struct BSDF {
     }
 
     T pdf(const V3<T> &wi, const V3<T> &wo) const {
+        if (kind == AGX_MAT_DISNEY) return disney_pdf(wi, wo);
         if (kind == AGX_MAT_GLASS || kind == AGX_MAT_MIRROR || kind == AGX_MAT_IDEAL_BLACK ||
             kind == AGX_MAT_INVISIBLE_SURFACE) return 0;
         // AggregateBSDF::pdf (aggregate.h:218-246)
@@ -1087,6 +1369,7 @@ template<class T> struct BSDF {
     }
 
     BSDFSample<T> sample(const V3<T> &wo, T su, T sv, T sw, T eps) const {
+        if (kind == AGX_MAT_DISNEY) return disney_sample(wo, su, sv, sw, eps);
         BSDFSample<T> res;
         if (kind == AGX_MAT_IDEAL_BLACK) return res;
         if (kind == AGX_MAT_INVISIBLE_SURFACE) {                // invisible_surface.cpp:26-31
@@ -1164,12 +1447,22 @@ template<class T> struct BSDF {
     }
 };
 
+// Spectrum::lum (agz-utils, parity unpinned): Rec. 709 luminance
+template<class T> inline T spectrum_lum(const V3<T> &c) { return T(0.2126) * c.x + T(0.7152) * c.y + T(0.0722) * c.z; }
+// DisneyBSDF::to_tint (disney.cpp:72-76)
+template<class T> inline V3<T> disney_tint(const V3<T> &c) {
+    const T l = spectrum_lum(c);
+    return l > 0 ? c / l : V3<T>(1);
+}
+
 template<class T> struct Material {
     agx_material_desc d;
+    T eps = T(3e-4);
     // Material::shade -> BSDF (ideal_diffuse.cpp:25-39, metal.cpp:39-62, glass.cpp:180-206, mirror.cpp:98-115)
     BSDF<T> shade(const SurfacePoint<T> &sp, V3<T> *shading_normal) const {
         BSDF<T> b;
         b.kind = d.kind;
+        b.eps_ = eps;
         b.geo = sp.geometry_coord;
         b.shd = sp.user_coord;
         const V3<T> col(T(d.color[0]), T(d.color[1]), T(d.color[2]));
@@ -1213,6 +1506,37 @@ template<class T> struct Material {
         case AGX_MAT_INVISIBLE_SURFACE:                     // invisible_surface.cpp:47-50
             b.albedo = V3<T>(1);
             break;
+        case AGX_MAT_DISNEY: {                              // disney.cpp:405-457, 732-768
+            b.albedo = col;
+            b.d_tint = disney_tint(col);
+            b.d_metallic = T(d.metallic);
+            b.d_rough = T(d.roughness);
+            b.d_spec_scale = V3<T>(T(d.color2[0]), T(d.color2[1]), T(d.color2[2]));
+            b.d_spec_tint = T(d.specular_tint);
+            b.d_sheen = T(d.sheen);
+            b.d_sheen_tint = T(d.sheen_tint);
+            b.d_trans = T(d.transmission);
+            b.d_ior = std::max(T(1.01), T(d.ior));
+            const T an = T(d.anisotropic);
+            const T aspect = an > 0 ? std::sqrt(1 - T(0.9) * an) : T(1);
+            const T r = T(d.roughness), tr = T(d.transmission_roughness);
+            b.ax = std::max(T(0.001), r * r / aspect);
+            b.ay = std::max(T(0.001), r * r * aspect);
+            b.d_tax = std::max(T(0.001), tr * tr / aspect);
+            b.d_tay = std::max(T(0.001), tr * tr * aspect);
+            b.d_clearcoat = T(d.clearcoat);
+            const T g = T(d.clearcoat_gloss);
+            T ccr = T(0.1) * (1 - g) + T(0) * g;             // mix(0.1, 0, gloss)
+            ccr *= ccr;
+            b.d_ccr = std::max(ccr, T(0.0001));
+            const T A = std::min(T(0.7), std::max(T(0.3), spectrum_lum(col) * (1 - b.d_metallic)));
+            const T B = 1 - A;
+            b.w_diffuse = A * (1 - b.d_trans);
+            b.w_trans = A * b.d_trans;
+            b.w_spec = B * 2 / (2 + b.d_clearcoat);
+            b.w_cc = B * b.d_clearcoat / (2 + b.d_clearcoat);
+            break;
+        }
         default:
             b.albedo = V3<T>(0);
             break;
@@ -1526,7 +1850,10 @@ template<class T> struct Scene {
         media.resize(d.medium_count);
         for (uint32_t i = 0; i < d.medium_count; ++i) media[i].init(d.media[i], eps);
         materials.resize(d.material_count);
-        for (uint32_t i = 0; i < d.material_count; ++i) materials[i].d = d.materials[i];
+        for (uint32_t i = 0; i < d.material_count; ++i) {
+            materials[i].d = d.materials[i];
+            materials[i].eps = eps;
+        }
         env_kind = d.env.kind;
         sky_top = V3<T>(T(d.env.top[0]), T(d.env.top[1]), T(d.env.top[2]));
         sky_bottom = V3<T>(T(d.env.bottom[0]), T(d.env.bottom[1]), T(d.env.bottom[2]));
