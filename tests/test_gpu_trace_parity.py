@@ -45,10 +45,12 @@ def _compare(gpu_ctx, scene, o, d, tmin, tmax, label, accel=capi.ACCEL_ATRC):
     if accel == capi.ACCEL_ATRC:
         assert same.all()
     else:
-        # a different winner only at exactly equal t
+        # a different winner only at exactly equal t: coplanar surfaces (the Cornell
+        # boxes stand on the floor, so rays starting inside a box meet its bottom
+        # face and the floor at the same t)
         diff = ~same
         log(f"  equal-t ties resolved differently: {int(diff.sum())}")
-        assert diff.mean() < 1e-3
+        assert diff.mean() < 1e-2
         np.testing.assert_array_equal(g["t"][diff].view(np.uint32), r["t"][diff].view(np.uint32))
         hit = hit & same
     # bit-exact FP32 distances and barycentrics
