@@ -334,6 +334,31 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         case AGX_MAT_INVISIBLE_SURFACE:  // invisible_surface.cpp:47-50: albedo 1
             d.albedo = vec4{1, 1, 1, 0};
             break;
+        case AGX_MAT_DISNEY: {       // DisneyBSDF::DisneyBSDF (disney.cpp:405-457)
+            d.albedo = v4(col, 0);
+            d.color = v4(col, 0);
+            // Spectrum::lum: Rec. 709 luminance; to_tint (disney.cpp:72-76)
+            const float lum = 0.2126f * col.x + 0.7152f * col.y + 0.0722f * col.z;
+            d.c_refl = lum > 0 ? vec4{col.x / lum, col.y / lum, col.z / lum, 0} : vec4{1, 1, 1, 0};
+            d.c_refr = vec4{m.color2[0], m.color2[1], m.color2[2], 0};
+            const float ior = std::max(1.01f, m.ior);
+            const float an = m.anisotropic;
+            const float aspect = an > 0 ? std::sqrt(1 - 0.9f * an) : 1.0f;
+            const float r = m.roughness, tr = m.transmission_roughness;
+            d.misc = vec4{std::max(0.001f, r * r / aspect), std::max(0.001f, r * r * aspect), ior, r};
+            const float g = m.clearcoat_gloss;
+            float ccr = 0.1f * (1 - g) + 0.0f * g;  // mix(0.1, 0, gloss)
+            ccr *= ccr;
+            ccr = std::max(ccr, 0.0001f);
+            const float A = std::min(0.7f, std::max(0.3f, lum * (1 - m.metallic)));
+            const float B = 1 - A;
+            d.d0 = vec4{m.metallic, m.specular_tint, m.sheen, m.sheen_tint};
+            d.d1 = vec4{m.clearcoat, m.transmission, std::max(0.001f, tr * tr / aspect),
+                        std::max(0.001f, tr * tr * aspect)};
+            d.d2 = vec4{ccr, A * (1 - m.transmission), A * m.transmission, B * 2 / (2 + m.clearcoat)};
+            d.d3 = vec4{B * m.clearcoat / (2 + m.clearcoat), 0, 0, 0};
+            break;
+        }
         default:
             *err = "unsupported material kind " + std::to_string(m.kind);
             return AGX_E_UNSUPPORTED;

@@ -95,8 +95,12 @@ struct DevMaterial {
     vec4 color;            // metal colour
     vec4 eta2;             // conductor (eta_in / eta_out)^2 ; mirror uses m_eta2
     vec4 k2;               // conductor (k / eta_out)^2
-    vec4 misc;             // (ax, ay, ior, 0)
-    vec4 c_refl, c_refr;   // glass
+    vec4 misc;             // (ax, ay, ior, 0); disney (ax, ay, ior, roughness)
+    vec4 c_refl, c_refr;   // glass; disney (tint, specular_scale)
+    vec4 d0;               // disney (metallic, specular_tint, sheen, sheen_tint)
+    vec4 d1;               // disney (clearcoat, transmission, trans_ax, trans_ay)
+    vec4 d2;               // disney (clearcoat_roughness, w_diffuse, w_transmission, w_specular)
+    vec4 d3;               // disney (w_clearcoat, 0, 0, 0)
 };
 
 struct DevScene {

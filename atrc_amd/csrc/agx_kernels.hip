@@ -923,6 +923,8 @@ struct Bsdf {
     Frame geo, shd;
     V albedo, coef, color, eta2, k2, c_refl, c_refr;
     float ax, ay, ior;
+    float eps;
+    const DevMaterial *m;  // Disney parameters are read from the record when needed
 };
 
 struct BSmp {
@@ -931,9 +933,11 @@ struct BSmp {
     bool delta;
 };
 
-__device__ Bsdf make_bsdf(const DevMaterial &m, const Surf &sp) {
+__device__ Bsdf make_bsdf(const DevMaterial &m, const Surf &sp, float eps) {
     Bsdf b;
     b.kind = m.kind;
+    b.eps = eps;
+    b.m = &m;
     b.geo = sp.geo;
     b.shd = sp.user;
     b.albedo = mk(m.albedo);
@@ -1046,8 +1050,274 @@ __device__ V ggx_vndf(const Bsdf &b, V ve, float su, float sv, float eps) {
 
 __device__ __forceinline__ bool fringe(const Bsdf &b, V w) { return pos_hemi(b.shd, w) != pos_hemi(b.geo, w); }
 
+// ---------------------------------------------------------------- Disney (material/disney.cpp:18-457)
+__device__ __forceinline__ float sqr(float x) { return x * x; }
+__device__ __forceinline__ float aniso_gtr2(float sph, float cph, float sth, float cth, float ax, float ay) {
+    const float A = sqr(cph / ax) + sqr(sph / ay);
+    const float RD = sqr(sth) * A + sqr(cth);
+    return 1 / (PI_F * ax * ay * sqr(RD));
+}
+__device__ __forceinline__ float smith_aniso(float cp, float sp, float ax, float ay, float tt) {
+    const float t = sqr(ax * cp) + sqr(ay * sp);
+    const float sv = 1 + t * sqr(tt);
+    const float lambda = -0.5f + 0.5f * sqrtf(sv);
+    return 1 / (1 + lambda);
+}
+__device__ __forceinline__ float gtr1(float sth, float cth, float a) {  // microfacet.cpp:84-90
+    const float U = sqr(a) - 1;
+    const float LD = 2 * PI_F * logf(a);
+    const float RD = sqr(a * cth) + sqr(sth);
+    return U / (LD * RD);
+}
+__device__ __forceinline__ float smith_gtr2(float tt, float a) {  // microfacet.cpp:23-29
+    if (tt == 0.0f) return 1;
+    const float root = a * tt;
+    return 2 / (1 + sqrtf(1 + root * root));
+}
+__device__ V sample_aniso_gtr2(float ax, float ay, float u, float v) {  // microfacet.cpp:63-82
+    float sph = ay * sinf(2 * PI_F * u);
+    float cph = ax * cosf(2 * PI_F * u);
+    const float nor = 1 / sqrtf(sqr(sph) + sqr(cph));
+    sph *= nor;
+    cph *= nor;
+    const float A = sqr(cph / ax) + sqr(sph / ay);
+    const float cth = sqrtf(A * (1 - v) / ((1 - A) * v + A));
+    const float sth = sqrtf(smax(0.0f, 1 - sqr(cth)));
+    return normalize(mk(sth * cph, sth * sph, cth));
+}
+__device__ V sample_gtr1(float a, float u, float v) {  // microfacet.cpp:92-102
+    const float phi = 2 * PI_F * u;
+    const float ct = sqrtf((powf(a, 2 - 2 * v) - 1) / (sqr(a) - 1));
+    const float st = sqrtf(smax(0.0f, 1 - ct * ct));
+    return normalize(mk(st * cosf(phi), st * sinf(phi), ct));
+}
+__device__ __forceinline__ float one_minus_5(float x) {
+    const float t = 1 - x;
+    const float t2 = t * t;
+    return t2 * t2 * t;
+}
+__device__ __forceinline__ float cos_2_sin(float c) { return sqrtf(smax(0.0f, 1 - c * c)); }
+__device__ __forceinline__ V mixv(V a, V b, float t) { return a * (1 - t) + b * t; }  // math::mix
+__device__ __forceinline__ bool zerov(V v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; }
+
+__device__ void d_angles(V lwi, V lwo, float tax, float tay, V lwh, float *D, float *G) {
+    const float ph = phi_of(lwh);
+    const float sph = sinf(ph), cph = cosf(ph);
+    const float cth = lwh.z, sth = cos_2_sin(cth);
+    *D = aniso_gtr2(sph, cph, sth, cth, tax, tay);
+    const float pi_ = phi_of(lwi), po = phi_of(lwo);
+    const float spi = sinf(pi_), cpi = cosf(pi_);
+    const float spo = sinf(po), cpo = cosf(po);
+    *G = smith_aniso(cpi, spi, tax, tay, tan_theta(lwi)) * smith_aniso(cpo, spo, tax, tay, tan_theta(lwo));
+}
+
+__device__ V disney_eval(const Bsdf &b, V wi, V wo) {
+    if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:133-139
+        if (!pos_hemi(b.geo, wi) || !pos_hemi(b.geo, wo)) return mk(0, 0, 0);
+        return b.albedo / PI_F;
+    }
+    const DevMaterial &m = *b.m;
+    const V lwi = normalize(to_local(b.shd, wi));
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (fabsf(lwi.z) < b.eps || fabsf(lwo.z) < b.eps) return mk(0, 0, 0);
+    const float metallic = m.d0.x, trans = m.d1.y, ior = m.misc.z;
+    const V C = b.albedo;
+    if (lwi.z * lwo.z < 0) {  // f_trans (disney.cpp:114-163)
+        if (trans == 0.0f) return mk(0, 0, 0);
+        const float ci = lwi.z, co = lwo.z;
+        const float eta = co > 0 ? ior : 1 / ior;
+        V lwh = normalize(lwo + lwi * eta);
+        if (lwh.z < 0) lwh = -lwh;
+        const float cd = dot(lwo, lwh);
+        const float F = dielectric(ior, 1, cd);
+        float D, G;
+        d_angles(lwi, lwo, m.d1.z, m.d1.w, lwh, &D, &G);
+        const float sdem = cd + eta * dot(lwi, lwh);
+        const float corr = 1 / eta;
+        const float val = (1 - F) * D * G * eta * eta * dot(lwi, lwh) * dot(lwo, lwh) * corr * corr /
+                          (ci * co * sdem * sdem);
+        const float tf = co > 0 ? trans : 1.0f;
+        const V sq = mk(sqrtf(C.x), sqrtf(C.y), sqrtf(C.z));
+        return sq * ((1 - metallic) * tf) * fabsf(val) * normal_corr(b.geo.z, b.shd.z, wi, b.eps);
+    }
+    if (lwi.z < 0 && lwo.z < 0) {  // f_inner_refl (disney.cpp:165-198)
+        if (trans == 0.0f) return mk(0, 0, 0);
+        const V lwh = -normalize(lwi + lwo);
+        const float cd = dot(lwo, lwh);
+        const float F = dielectric(ior, 1, cd);
+        float D, G;
+        d_angles(lwi, lwo, m.d1.z, m.d1.w, lwh, &D, &G);
+        return C * trans * fabsf(F * D * G / (4 * lwi.z * lwo.z)) * normal_corr(b.geo.z, b.shd.z, wi, b.eps);
+    }
+    if (lwi.z <= 0 || lwo.z <= 0) return mk(0, 0, 0);
+    const float ci = lwi.z, co = lwo.z;
+    const V lwh = normalize(lwi + lwo);
+    const float cd = dot(lwi, lwh);
+    const V tint = mk(m.c_refl);
+    V diffuse = mk(0, 0, 0), sheen = mk(0, 0, 0);
+    if (metallic < 1) {  // f_diffuse, f_sheen (disney.cpp:77-96)
+        const V f_lambert = C / PI_F;
+        const float FL = one_minus_5(ci), FV = one_minus_5(co);
+        const float RR = 2 * m.misc.w * cd * cd;
+        const V retro = C / PI_F * RR * (FL + FV + FL * FV * (RR - 1));
+        diffuse = f_lambert * (1 - 0.5f * FL) * (1 - 0.5f * FV) + retro;
+        if (m.d0.z > 0) sheen = mixv(mk(1, 1, 1), tint, m.d0.w) * (4 * m.d0.z) * one_minus_5(cd);
+    }
+    V specular;
+    {  // f_specular (disney.cpp:200-236)
+        const V cspec = mixv(mixv(mk(1, 1, 1), tint, m.d0.y), C, metallic);
+        const V diel = cspec * dielectric(ior, 1, cd);
+        const V cond = cspec + (mk(1, 1, 1) - cspec) * one_minus_5(cd);
+        const V F = mixv(mk(m.c_refr) * diel, cond, metallic);
+        float D, G;
+        d_angles(lwi, lwo, b.ax, b.ay, lwh, &D, &G);
+        specular = F * D * G / fabsf(4 * ci * co);
+    }
+    V clearcoat = mk(0, 0, 0);
+    if (m.d1.x > 0) {  // f_clearcoat (disney.cpp:98-112)
+        const float cth = lwh.z;
+        const float D = gtr1(cos_2_sin(cth), cth, m.d2.x);
+        const float F = 0.04f + (1 - 0.04f) * one_minus_5(cd);
+        const float G = smith_gtr2(tan_theta(lwi), 0.25f) * smith_gtr2(tan_theta(lwo), 0.25f);
+        const float c = m.d1.x * D * F * G / fabsf(4 * ci * co);
+        clearcoat = mk(c, c, c);
+    }
+    const V value = (diffuse + sheen) * ((1 - metallic) * (1 - trans)) + specular + clearcoat;
+    return value * normal_corr(b.geo.z, b.shd.z, wi, b.eps);
+}
+
+__device__ float d_pdf_trans(const DevMaterial &m, V lwi, V lwo) {  // disney.cpp:350-377
+    const float ior = m.misc.z;
+    const float eta = lwo.z > 0 ? ior : 1 / ior;
+    V lwh = normalize(lwo + lwi * eta);
+    if (lwh.z < 0) lwh = -lwh;
+    if (((lwo.z > 0) != (dot(lwh, lwo) > 0)) || ((lwi.z > 0) != (dot(lwh, lwi) > 0))) return 0;
+    const float sdem = dot(lwo, lwh) + eta * dot(lwi, lwh);
+    const float dwh_to_dwi = eta * eta * dot(lwi, lwh) / (sdem * sdem);
+    const float ph = phi_of(lwh);
+    const float cth = lwh.z;
+    const float D = aniso_gtr2(sinf(ph), cosf(ph), cos_2_sin(cth), cth, m.d1.z, m.d1.w);
+    return fabsf(cth * D * dwh_to_dwi);
+}
+__device__ float d_pdf_inner(const DevMaterial &m, V lwi, V lwo) {  // disney.cpp:379-395
+    const V lwh = -normalize(lwi + lwo);
+    const float ph = phi_of(lwh);
+    const float cth = lwh.z;
+    const float cd = dot(lwi, lwh);
+    const float D = aniso_gtr2(sinf(ph), cosf(ph), cos_2_sin(cth), cth, m.d1.z, m.d1.w);
+    return fabsf(cth * D / (4 * cd));
+}
+
+__device__ float disney_pdf(const Bsdf &b, V wi, V wo) {  // disney.cpp:608-644
+    if (fringe(b, wi) || fringe(b, wo)) {
+        if (pos_hemi(b.geo, wi) != pos_hemi(b.geo, wo)) return 0;
+        return cos_hemisphere_pdf(normalize(to_local(b.geo, wi)).z);
+    }
+    const DevMaterial &m = *b.m;
+    const V lwi = normalize(to_local(b.shd, wi));
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (fabsf(lwi.z) < b.eps || fabsf(lwo.z) < b.eps) return 0;
+    if (lwo.z < 0) {
+        if (m.d1.y == 0.0f) return 0;
+        const float mF = smin(0.9f, smax(0.1f, dielectric(m.misc.z, 1, lwo.z)));
+        if (lwi.z > 0) return (1 - mF) * d_pdf_trans(m, lwi, lwo);
+        if (mF == 0.0f) return 0;
+        return mF * d_pdf_inner(m, lwi, lwo);
+    }
+    if (lwi.z < 0) return m.d2.z * d_pdf_trans(m, lwi, lwo);
+    const float diffuse = cos_hemisphere_pdf(lwi.z);
+    // pdf_specular_clearcoat (disney.cpp:317-348): sin(phi_o) taken as |sin|, as Atrc does
+    const V lwh = normalize(lwi + lwo);
+    const float ph = phi_of(lwh);
+    const float sph = sinf(ph), cph = cosf(ph);
+    const float cth = lwh.z, sth = cos_2_sin(cth);
+    const float cd = dot(lwi, lwh);
+    const float cpo = cosf(phi_of(lwo));
+    const float spo = cos_2_sin(cpo);
+    const float spec_D = aniso_gtr2(sph, cph, sth, cth, b.ax, b.ay);
+    const float pdf_spec = smith_aniso(cpo, spo, b.ax, b.ay, tan_theta(lwo)) * spec_D / (4 * lwo.z);
+    const float cc_D = gtr1(sth, cth, m.d2.x);
+    const float pdf_cc = cth * cc_D / (4 * cd);
+    return m.d2.y * diffuse + m.d2.w * pdf_spec + m.d3.x * pdf_cc;
+}
+
+__device__ V d_sample_trans(const DevMaterial &m, V lwo, float u, float v) {  // disney.cpp:270-294
+    const V lwh = sample_aniso_gtr2(m.d1.z, m.d1.w, u, v);
+    if (lwh.z <= 0) return mk(0, 0, 0);
+    if ((lwo.z > 0) != (dot(lwh, lwo) > 0)) return mk(0, 0, 0);
+    const float eta = lwo.z > 0 ? 1 / m.misc.z : m.misc.z;
+    const V owh = dot(lwh, lwo) > 0 ? lwh : -lwh;
+    // refl_aux::refract (utility/reflection.h:31-40)
+    const float ci = dot(lwo, owh);
+    const float s2t = sqr(eta) * (1 - sqr(ci));
+    const float c2t = 1 - s2t;
+    if (c2t <= 0) return mk(0, 0, 0);
+    const float ct = sqrtf(c2t);
+    const V lwi = normalize(normalize(owh * (eta * ci - ct) - lwo * eta));
+    if (lwi.z * lwo.z > 0 || ((lwi.z > 0) != (dot(lwh, lwi) > 0))) return mk(0, 0, 0);
+    return lwi;
+}
+
+__device__ BSmp disney_sample(const Bsdf &b, V wo, float su, float sv, float sw, float eps) {  // disney.cpp:527-596
+    BSmp r;
+    r.dir = mk(0, 0, 0);
+    r.f = mk(0, 0, 0);
+    r.pdf = 0;
+    r.delta = false;
+    if (fringe(b, wo)) {  // bsdf.h:142-159
+        if (!pos_hemi(b.geo, wo)) return r;
+        float p;
+        const V lwi = cos_hemisphere(su, sv, &p);
+        if (p < eps) return r;
+        const V wi = normalize(to_global(b.geo, lwi));
+        r.dir = wi;
+        r.f = b.albedo / PI_F * normal_corr(b.geo.z, b.shd.z, wi, eps);
+        r.pdf = p;
+        return r;
+    }
+    const DevMaterial &m = *b.m;
+    const V lwo = normalize(to_local(b.shd, wo));
+    if (fabsf(lwo.z) < eps) return r;
+    V lwi = mk(0, 0, 0);
+    if (lwo.z < 0) {
+        if (m.d1.y == 0.0f) return r;
+        const float mF = smin(0.9f, smax(0.1f, dielectric(m.misc.z, 1, lwo.z)));
+        if (su >= mF) {
+            lwi = d_sample_trans(m, lwo, sv, sw);
+        } else {  // sample_inner_refl (disney.cpp:296-310)
+            const V lwh = sample_aniso_gtr2(m.d1.z, m.d1.w, sv, sw);
+            if (lwh.z > 0) {
+                const V c = lwh * (2 * dot(lwo, lwh)) - lwo;
+                if (!(c.z > 0)) lwi = normalize(c);
+            }
+        }
+    } else {
+        // a diffuse pick (sel < w_diffuse) samples the cosine lobe, but the
+        // if/else chain that follows always overwrites that direction
+        float sel = su;
+        if (!(sel < m.d2.y)) sel -= m.d2.y;
+        if (sel < m.d2.z) {
+            lwi = d_sample_trans(m, lwo, sv, sw);
+        } else {
+            sel -= m.d2.z;
+            const V lwh = sel < m.d2.w ? normalize(ggx_vndf(b, lwo, sv, sw, eps)) : sample_gtr1(m.d2.x, sv, sw);
+            if (lwh.z > 0) {
+                const V c = normalize(lwh * (2 * dot(lwo, lwh)) - lwo);
+                if (c.z > 0) lwi = c;
+            }
+        }
+    }
+    if (zerov(lwi)) return r;
+    const V wi = to_global(b.shd, lwi);
+    r.dir = wi;
+    r.f = disney_eval(b, wi, wo);
+    r.pdf = disney_pdf(b, wi, wo);
+    return r;
+}
+
 // BSDF::eval (component/aggregate.h:39-64; glass/mirror are delta: 0)
 __device__ V bsdf_eval(const Bsdf &b, V wi, V wo) {
+    if (b.kind == AGX_MAT_DISNEY) return disney_eval(b, wi, wo);
     if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return mk(0, 0, 0);
     if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:133-139
         if (!pos_hemi(b.geo, wi) || !pos_hemi(b.geo, wo)) return mk(0, 0, 0);
@@ -1067,6 +1337,7 @@ __device__ V bsdf_eval(const Bsdf &b, V wi, V wo) {
 
 // BSDF::pdf (aggregate.h:218-246)
 __device__ float bsdf_pdf(const Bsdf &b, V wi, V wo) {
+    if (b.kind == AGX_MAT_DISNEY) return disney_pdf(b, wi, wo);
     if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return 0;
     if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:178-185
         if (pos_hemi(b.geo, wi) != pos_hemi(b.geo, wo)) return 0;
@@ -1088,6 +1359,7 @@ __device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, f
     r.f = mk(0, 0, 0);
     r.pdf = 0;
     r.delta = false;
+    if (b.kind == AGX_MAT_DISNEY) return disney_sample(b, wo, su, sv, sw, eps);
     if (b.kind == AGX_MAT_IDEAL_BLACK) return r;
     if (b.kind == AGX_MAT_INVISIBLE_SURFACE) {  // invisible_surface.cpp:26-31
         const float cv = fabsf(cosv(b.geo.z, wo));
@@ -1535,7 +1807,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
     const Surf sp = surface_at(s, h, ro, rd, eps);
     const DevEntity &e = s.ents[ent];
-    const Bsdf b = make_bsdf(s.mats[e.material], sp);
+    const Bsdf b = make_bsdf(s.mats[e.material], sp, eps);
     if (dep.x == 1) {  // :54-60
         pb.gb_normal[p] = f4(sp.user.z, 0);
         pb.gb_albedo[p] = f4(b.albedo, (e.flags & 1) ? 0.0f : 1.0f);

@@ -301,6 +301,13 @@ std::shared_ptr<std::vector<float>> load_texels(const std::string &filename, boo
     return out;
 }
 
+// helper::child_texture_or_constant (factory/include/agz/factory/help.h:11-19): the
+// constant texel of child `name`, or `def` when the child is absent
+Vec3 texture_value_or(CreatingContext &ctx, const ConfigGroup &params, const std::string &name, float def) {
+    if (auto n = params.find_child_group(name)) return ctx.create<Texture2D>(*n)->texel;
+    return Vec3{def, def, def};
+}
+
 void reject_normal_map(const ConfigGroup &params) {
     if (params.find_child("normal_map"))
         throw ObjectConstructionException("normal maps are not supported by gpu_pt");
@@ -448,7 +455,29 @@ CreatingContext::CreatingContext() {
         m->desc.kind = AGX_MAT_INVISIBLE_SURFACE;
         return m;
     });
-    for (const char *unsupported : {"disney", "dream_works_fabric", "paper", "phong"})
+    material.add_creator("disney", [](const ConfigGroup &p, CreatingContext &ctx) {  // material_creators.cpp:46-112
+        reject_normal_map(p);
+        if (p.find_child("bssrdf_dmfp")) throw ObjectConstructionException("BSSRDF surfaces are not supported by gpu_pt");
+        auto m = std::make_shared<Material>();
+        m->desc.kind = AGX_MAT_DISNEY;
+        copy3(m->desc.color, texture_value(ctx, p, "base_color"));
+        m->desc.metallic = texture_value(ctx, p, "metallic").x;
+        m->desc.roughness = texture_value(ctx, p, "roughness").x;
+        m->desc.transmission = texture_value_or(ctx, p, "transmission", 0).x;
+        m->desc.transmission_roughness = p.find_child_group("transmission_roughness")
+                                             ? texture_value(ctx, p, "transmission_roughness").x
+                                             : m->desc.roughness;
+        m->desc.ior = texture_value_or(ctx, p, "ior", 1.5f).x;
+        copy3(m->desc.color2, texture_value_or(ctx, p, "specular_scale", 1));
+        m->desc.specular_tint = texture_value_or(ctx, p, "specular_tint", 0).x;
+        m->desc.anisotropic = texture_value_or(ctx, p, "anisotropic", 0).x;
+        m->desc.sheen = texture_value_or(ctx, p, "sheen", 0).x;
+        m->desc.sheen_tint = texture_value_or(ctx, p, "sheen_tint", 0).x;
+        m->desc.clearcoat = texture_value_or(ctx, p, "clearcoat", 0).x;
+        m->desc.clearcoat_gloss = texture_value_or(ctx, p, "clearcoat_gloss", 1).x;
+        return m;
+    });
+    for (const char *unsupported : {"dream_works_fabric", "paper", "phong"})
         material.add_creator(unsupported, [n = std::string(unsupported)](const ConfigGroup &, CreatingContext &)
                                               -> std::shared_ptr<Material> {
             throw ObjectConstructionException("material type '" + n + "' is not supported by gpu_pt yet");

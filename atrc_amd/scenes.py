@@ -456,6 +456,43 @@ def absorbing_slab(sigma_a=(0.3, 0.6, 1.2), thickness: float = 1.5) -> tuple[Sce
     return s, cam, {"min_depth": 8, "max_depth": 8, "cont_prob": 1.0}
 
 
+def disney_spheres() -> tuple[Scene, dict, dict]:
+    """A 3x3 grid of Disney spheres (metallic, rough, anisotropic, sheen, clearcoat,
+    rough transmission) on a Lambert ground, lit by a quad light and a native_sky;
+    two of them are triangle meshes so that shading normals differ from geometric ones."""
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    ground = s.add_material(Material(color=(0.6, 0.6, 0.6)))
+    s.add(Entity(capi.GEOM_QUAD, ground, points=((-20, -20, 0), (20, -20, 0), (20, 20, 0), (-20, 20, 0))))
+    D = capi.MAT_DISNEY
+    mats = [
+        Material(kind=D, color=(0.8, 0.2, 0.2), roughness=0.5),
+        Material(kind=D, color=(0.9, 0.7, 0.3), metallic=1.0, roughness=0.25),
+        Material(kind=D, color=(0.3, 0.5, 0.9), metallic=0.5, roughness=0.4, anisotropic=0.7),
+        Material(kind=D, color=(0.5, 0.8, 0.4), roughness=0.8, sheen=1.0, sheen_tint=0.5),
+        Material(kind=D, color=(0.1, 0.1, 0.1), roughness=0.3, clearcoat=1.0, clearcoat_gloss=0.8),
+        Material(kind=D, color=(0.95, 0.95, 0.95), roughness=0.15, transmission=0.9, ior=1.5,
+                 transmission_roughness=0.2),
+        Material(kind=D, color=(0.7, 0.7, 0.9), roughness=0.35, specular_tint=0.8, color2=(0.6, 0.6, 0.6)),
+        Material(kind=D, color=(0.9, 0.5, 0.6), roughness=0.6, transmission=0.5, ior=1.33, metallic=0.2),
+        Material(kind=D, color=(0.4, 0.4, 0.4), roughness=0.05, metallic=0.8, clearcoat=0.5, anisotropic=0.3),
+    ]
+    pos, faces = icosphere(3)
+    for k, m in enumerate(mats):
+        mi = s.add_material(m)
+        cx, cy = (k % 3 - 1) * 2.5, (k // 3 - 1) * 2.5
+        if k in (2, 5):
+            v = pos * 1.0 + np.array([cx, cy, 1.0])
+            s.add(Entity(capi.GEOM_TRIANGLE_BVH, mi, tris=mesh_from_indexed(v, pos, faces)))
+        else:
+            s.add(Entity(capi.GEOM_SPHERE, mi, radius=1.0, local_to_world=(1, 0, 0, cx, 0, 1, 0, cy, 0, 0, 1, 1.0)))
+    light = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    s.add(Entity(capi.GEOM_QUAD, light, emit=(12.0, 11.0, 9.0),
+                 points=((-2, -2, 8), (-2, 2, 8), (2, 2, 8), (2, -2, 8))))
+    s.env = {"top": (0.4, 0.5, 0.7), "bottom": (0.1, 0.1, 0.1)}
+    cam = {"pos": (0.0, -11.0, 7.0), "dst": (0.0, 0.0, 0.8), "up": (0, 0, 1), "fov": 45.0}
+    return s, cam, {"min_depth": 3, "max_depth": 8, "cont_prob": 0.9}
+
+
 def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, extent: float = 50.0,
                        ground: bool = True) -> tuple[Scene, dict, dict]:
     """Seeded procedural mesh scene: n icospheres (20*4^subdiv triangles each) of

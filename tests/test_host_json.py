@@ -89,7 +89,7 @@ def test_skip_commented_entities_reference_and_aggregate():
     ({"type": "geometric", "geometry": {"type": "quad", "A": [0, 0], "B": [1, 0, 0], "C": [1, 1, 0], "D": [0, 1, 0]},
       "material": {"type": "ideal_black"}}, "invalid array size"),
     ({"type": "geometric", "geometry": {"type": "sphere", "radius": 1},
-      "material": {"type": "disney"}}, "not supported by gpu_pt"),
+      "material": {"type": "phong"}}, "not supported by gpu_pt"),
 ])
 def test_errors_carry_the_creation_hierarchy(bad, msg):
     with pytest.raises(RuntimeError) as ei:
@@ -203,3 +203,27 @@ def test_texture3d_common_params():
             with open(os.path.join(td, "d.txt"), "w") as f:
                 f.write("1 1 1 0.5")
             capi.HostScene(wrap([_medium_box_json({"type": "invisible_surface"}, bad)]), td)
+
+
+def test_disney_creator_defaults():
+    # material_creators.cpp:56-112: required base_color / metallic / roughness, the
+    # rest default (transmission 0, ior 1.5, specular_scale 1, clearcoat_gloss 1,
+    # transmission_roughness = roughness)
+    c = lambda v: {"type": "constant", "texel": v if isinstance(v, list) else [v]}
+    mat = {"type": "disney", "base_color": c([0.8, 0.4, 0.2]), "metallic": c(0.3), "roughness": c(0.45),
+           "clearcoat": c(0.5)}
+    e = {"type": "geometric", "geometry": {"type": "sphere", "radius": 1}, "material": mat}
+    h = capi.HostScene(wrap([e]), ".")   # keeps the descriptor alive
+    m = h.desc().materials[0]
+    assert m.kind == capi.MAT_DISNEY
+    np.testing.assert_allclose(list(m.color), [0.8, 0.4, 0.2], rtol=1e-6)
+    np.testing.assert_allclose(list(m.color2), [1, 1, 1])
+    assert (m.metallic, m.roughness, m.transmission, m.ior) == pytest.approx((0.3, 0.45, 0.0, 1.5))
+    assert m.transmission_roughness == pytest.approx(0.45) and m.clearcoat_gloss == 1.0
+    assert (m.clearcoat, m.sheen, m.sheen_tint, m.specular_tint, m.anisotropic) == pytest.approx((0.5, 0, 0, 0, 0))
+    mat2 = dict(mat, transmission_roughness=c(0.1), ior=c(1.33), specular_scale=c([0.5, 0.6, 0.7]))
+    e2 = dict(e, material=mat2)
+    h2 = capi.HostScene(wrap([e2]), ".")
+    m2 = h2.desc().materials[0]
+    assert (m2.transmission_roughness, m2.ior) == pytest.approx((0.1, 1.33))
+    np.testing.assert_allclose(list(m2.color2), [0.5, 0.6, 0.7], rtol=1e-6)
