@@ -456,6 +456,79 @@ def absorbing_slab(sigma_a=(0.3, 0.6, 1.2), thickness: float = 1.5) -> tuple[Sce
     return s, cam, {"min_depth": 8, "max_depth": 8, "cont_prob": 1.0}
 
 
+def _room_quads(s: Scene, lo, hi, m: int, skip=()) -> None:
+    """Inward-facing quads of the box [lo, hi] (names: floor, ceiling, x0, x1, y0, y1)."""
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    faces = {
+        "floor": ((x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0)),     # +z
+        "ceiling": ((x0, y0, z1), (x0, y1, z1), (x1, y1, z1), (x1, y0, z1)),   # -z
+        "x0": ((x0, y0, z0), (x0, y1, z0), (x0, y1, z1), (x0, y0, z1)),        # +x
+        "x1": ((x1, y0, z0), (x1, y0, z1), (x1, y1, z1), (x1, y1, z0)),        # -x
+        "y0": ((x0, y0, z0), (x0, y0, z1), (x1, y0, z1), (x1, y0, z0)),        # +y
+        "y1": ((x0, y1, z0), (x1, y1, z0), (x1, y1, z1), (x0, y1, z1)),        # -y
+    }
+    for k, q in faces.items():
+        if k not in skip:
+            s.add(Entity(capi.GEOM_QUAD, m, points=q))
+
+
+def guided_room(n_clutter_tris: int = 500_000, seed: int = 7, gap: float = 0.05) -> tuple[Scene, dict, dict]:
+    """Hard indirect lighting: a closed 10 x 10 x 4 room of Lambert walls (albedo
+    0.8) whose only light is a 0.5 x 0.5 emitter in a second room behind a door
+    gap `gap` wide; seeded clutter of displaced icospheres and boxes
+    (n_clutter_tris triangles) on the floor of the first room.  Light reaches
+    the first room only through the gap."""
+    rng = np.random.default_rng(seed)
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    wall = s.add_material(Material(color=(0.8, 0.8, 0.8)))
+    # room A [0,10]x[0,10]x[0,4] without its x = 10 wall; room B [10,13]x[0,10]x[0,4] without its x = 10 wall
+    _room_quads(s, (0, 0, 0), (10, 10, 4), wall, skip=("x1",))
+    _room_quads(s, (10, 0, 0), (13, 10, 4), wall, skip=("x0",))
+    # the dividing wall at x = 10, two-sided (one quad per side), with a door gap
+    # y in [5 - gap/2, 5 + gap/2], z in [0, 2.5]
+    g0, g1, dh = 5 - gap / 2, 5 + gap / 2, 2.5
+    def wall_pieces(x, sign):
+        pieces = [((0, 0), (g0, 4)), ((g1, 0), (10, 4)), ((g0, dh), (g1, 4))]
+        for (ya, za), (yb, zb) in pieces:
+            if sign > 0:   # normal -x (faces room A)
+                q = ((x, ya, za), (x, ya, zb), (x, yb, zb), (x, yb, za))
+            else:          # normal +x (faces room B)
+                q = ((x, ya, za), (x, yb, za), (x, yb, zb), (x, ya, zb))
+            s.add(Entity(capi.GEOM_QUAD, wall, points=q))
+    wall_pieces(10.0, +1)
+    wall_pieces(10.0, -1)
+    # 0.5 x 0.5 light on room B's ceiling near its far wall, facing down
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    lz = 4 - 1e-3
+    s.add(Entity(capi.GEOM_QUAD, black, emit=(40.0, 38.0, 34.0),
+                 points=((12.0, 2.0, lz), (12.0, 2.5, lz), (12.5, 2.5, lz), (12.5, 2.0, lz))))
+    # clutter: displaced icospheres (subdivision 4, 5120 triangles) and 12-triangle boxes
+    clutter_mat = [s.add_material(Material(color=tuple(rng.uniform(0.2, 0.8, 3)))) for _ in range(8)]
+    pos, faces = icosphere(4)
+    nrm = pos / np.linalg.norm(pos, axis=1, keepdims=True)
+    tris_left = n_clutter_tris
+    k = 0
+    while tris_left >= 12:
+        c = np.array([rng.uniform(0.5, 9.0), rng.uniform(0.5, 9.5), 0.0])
+        m = clutter_mat[k % len(clutter_mat)]
+        k += 1
+        if tris_left >= len(faces) and rng.random() < 0.8:
+            r = rng.uniform(0.15, 0.45)
+            disp = 1 + 0.1 * _value_noise(pos * 3 + rng.uniform(0, 100, 3), seed + k)
+            v = nrm * (r * disp)[:, None] + c + np.array([0, 0, r])
+            s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, tris=mesh_from_indexed(v, vertex_normals(v, faces), faces)))
+            tris_left -= len(faces)
+        else:
+            h = rng.uniform(0.1, 0.8)
+            w = rng.uniform(0.1, 0.6, 2)
+            s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, tris=box_mesh(c - np.array([w[0], w[1], 0]) / 2,
+                                                                  c + np.array([w[0], w[1], 2 * h]) / 2,
+                                                                  rng.uniform(0, 90))))
+            tris_left -= 12
+    cam = {"pos": (1.0, 2.0, 2.6), "dst": (9.0, 6.5, 1.0), "up": (0, 0, 1), "fov": 60.0}
+    return s, cam, {"min_depth": 5, "max_depth": 10, "cont_prob": 0.9}
+
+
 def disney_spheres() -> tuple[Scene, dict, dict]:
     """A 3x3 grid of Disney spheres (metallic, rough, anisotropic, sheen, clearcoat,
     rough transmission) on a Lambert ground, lit by a quad light and a native_sky;
