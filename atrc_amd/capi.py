@@ -222,12 +222,13 @@ class Context:
     def upload_scene(self, desc: "agx_scene_desc") -> None:
         _check(self.lib.agx_upload_scene(self.h, C.byref(desc)))
 
-    def render(self, params: agx_render_params, cancel: C.c_int32 | None = None) -> dict:
+    def render(self, params: agx_render_params, cancel: C.c_int32 | None = None, aovs: bool = True) -> dict:
+        """agx_render; aovs=False reads back the image only (albedo / normal / denoise stay None)."""
         wh = params.width * params.height
         rgb = np.zeros((params.height, params.width, 3), np.float32)
-        alb = np.zeros_like(rgb)
-        nor = np.zeros_like(rgb)
-        den = np.zeros((params.height, params.width), np.float32)
+        alb = np.zeros_like(rgb) if aovs else None
+        nor = np.zeros_like(rgb) if aovs else None
+        den = np.zeros((params.height, params.width), np.float32) if aovs else None
         code = self.lib.agx_render(self.h, C.byref(params), C.byref(cancel) if cancel is not None else None,
                                    fptr(rgb), fptr(alb), fptr(nor), fptr(den))
         if code not in (AGX_OK, AGX_E_CANCELLED):

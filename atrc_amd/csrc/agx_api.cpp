@@ -64,6 +64,9 @@ struct agx_ctx {
     uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
     uint32_t *h_counters = nullptr;  // pinned ring: RING x 4 queue sizes read back asynchronously
+    float *d_resolve = nullptr;      // normalised film planes (10 floats per pixel), kept between renders
+    float *h_resolve = nullptr;      // pinned staging of the planes the caller asked for
+    size_t resolve_cap = 0;          // pixels both buffers hold
     static constexpr int RING = 64;
     hipEvent_t ev[8]{};
     std::vector<hipEvent_t> pool;    // timing / readback events, reused across renders
@@ -234,6 +237,8 @@ void agx_destroy(agx_ctx *c) {
     c->free_wave();
     if (c->dcnt) (void)hipFree(c->dcnt);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
+    if (c->d_resolve) (void)hipFree(c->d_resolve);
+    if (c->h_resolve) (void)hipHostFree(c->h_resolve);
     for (auto &e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto &e : c->pool) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -473,18 +478,31 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     c->stats = st;
     // normalise into host outputs
     if (rgb || albedo || normal || denoise) {
-        float *dout = nullptr;
-        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&dout), size_t(wh) * 10 * sizeof(float)));
+        // device planes and pinned staging persist across renders (no allocation per call)
+        if (size_t(wh) > c->resolve_cap) {
+            if (c->d_resolve) (void)hipFree(c->d_resolve);
+            if (c->h_resolve) (void)hipHostFree(c->h_resolve);
+            c->d_resolve = nullptr;
+            c->h_resolve = nullptr;
+            c->resolve_cap = 0;
+            HIP_TRY(hipMalloc(reinterpret_cast<void **>(&c->d_resolve), size_t(wh) * 10 * sizeof(float)));
+            HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&c->h_resolve), size_t(wh) * 10 * sizeof(float),
+                                  hipHostMallocDefault));
+            c->resolve_cap = size_t(wh);
+        }
+        float *dout = c->d_resolve, *hout = c->h_resolve;
         launch_film_resolve(c->pb, wh, dout, dout + 3 * wh, dout + 6 * wh, dout + 9 * wh, c->stream);
-        std::vector<float> host(size_t(wh) * 10);
-        hipError_t e = hipMemcpyAsync(host.data(), dout, host.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        hipError_t e = hipSuccess;
+        struct Plane { float *user; size_t off, n; };
+        const Plane planes[4] = {{rgb, 0, 3}, {albedo, 3, 3}, {normal, 6, 3}, {denoise, 9, 1}};
+        for (const Plane &pl : planes)
+            if (pl.user && e == hipSuccess)
+                e = hipMemcpyAsync(hout + pl.off * size_t(wh), dout + pl.off * size_t(wh),
+                                   pl.n * size_t(wh) * sizeof(float), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        (void)hipFree(dout);
         if (e != hipSuccess) return fail(AGX_E_HIP, std::string("film readback: ") + hipGetErrorString(e));
-        if (rgb) std::memcpy(rgb, host.data(), size_t(wh) * 3 * sizeof(float));
-        if (albedo) std::memcpy(albedo, host.data() + 3 * wh, size_t(wh) * 3 * sizeof(float));
-        if (normal) std::memcpy(normal, host.data() + 6 * wh, size_t(wh) * 3 * sizeof(float));
-        if (denoise) std::memcpy(denoise, host.data() + 9 * wh, size_t(wh) * sizeof(float));
+        for (const Plane &pl : planes)
+            if (pl.user) std::memcpy(pl.user, hout + pl.off * size_t(wh), pl.n * size_t(wh) * sizeof(float));
     }
     if (cancelled) return fail(AGX_E_CANCELLED, "render cancelled");
     return AGX_OK;

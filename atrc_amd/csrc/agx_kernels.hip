@@ -139,6 +139,23 @@ __device__ Rng rng_seed(unsigned long long seed, unsigned long long seq) {
     return r;
 }
 
+// ---------------------------------------------------------------- queues
+// Appends one entry for every active lane with `pred` set: a single atomic per
+// wave on the queue counter (per-lane atomics on one address serialise in L2),
+// then each lane takes base + its rank among the appending lanes.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    const uint32_t lane = __lane_id();
+    const uint32_t rank = uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+    uint32_t base = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1;
+        if (int(lane) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader);
+    }
+    return base + rank;
+}
+
 // ---------------------------------------------------------------- sampling distributions
 __device__ void unit_disk(float u, float v, float *ox, float *oy) {  // Shirley-Chiu concentric map
     const float sx = 2 * u - 1, sy = 2 * v - 1;
@@ -1530,13 +1547,16 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         if (rng.uniform() > rc.cont_prob) alive = false;
         else coef = make_float4(coef.x / rc.cont_prob, coef.y / rc.cont_prob, coef.z / rc.cont_prob, 0);
     }
-    if (alive && 1 <= rc.max_depth) {
+    const bool launch = alive && 1 <= rc.max_depth;
+    if (launch) {
         flags |= PF_CONT;
         pb.ray_o[2 * p] = f4(o, 0.0f);
         pb.ray_d[2 * p] = f4(d, INF_F);
-        const uint32_t qi = atomicAdd(&pb.cnt_out[2], 1u);
+    }
+    const uint32_t qi = wave_append(&pb.cnt_out[2], launch);
+    const uint32_t ci = wave_append(&pb.cnt_out[0], launch);
+    if (launch) {
         pb.q_out[qi] = uint32_t(p);
-        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
         pb.q_closest[ci] = uint32_t(2 * p);
     }
     pb.coef[p] = coef;
@@ -1747,11 +1767,10 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     return coef * (f / (pdf + lpdf));
 }
 
-__device__ __forceinline__ void push_shadow(const PathBuffers &pb, size_t slot, V o, V d, float tmin, float tmax) {
+// writes the shadow ray of slot p * L + l; k_shade queues the slots at its end
+__device__ __forceinline__ void put_shadow(const PathBuffers &pb, size_t slot, V o, V d, float tmin, float tmax) {
     pb.sh_o[slot] = f4(o, tmax);
     pb.sh_d[slot] = f4(d, tmin);
-    const uint32_t qi = atomicAdd(&pb.cnt_out[1], 1u);
-    pb.q_shadow[qi] = uint32_t(slot);
 }
 
 __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
@@ -1814,6 +1833,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     bool pending = false;
     bool cont = false;
+    bool push_mis = false;
+    uint32_t shadow_mask = 0;  // bit l: the shadow ray of light l is traced
     // medium between the ray origin and the hit (path_tracing.cpp:64-119)
     const int32_t mi = medium_side(e, sp.wr, sp.geo.z);
     bool scattered = false;
@@ -1845,7 +1866,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                     pending = true;
                     // Scene::visible(light, point): from the light towards the point
                     const float dis = length(ls.pos - ms.pos);
-                    push_shadow(pb, slot, ls.pos, normalize(ms.pos - ls.pos), eps, dis - eps);
+                    put_shadow(pb, slot, ls.pos, normalize(ms.pos - ls.pos), eps, dis - eps);
+                    shadow_mask |= 1u << l;
                 }
                 {  // MIS-BSDF ray with the phase function (direct_illum.cpp:194-253)
                     const float u0 = rng.uniform(), u1 = rng.uniform();
@@ -1861,8 +1883,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         pb.mis_b[p] = make_float4(f, f, f, f);
                         flags |= PF_MIS | (uint32_t(mi + 1) << PF_MIS_MED_SHIFT);
                         pending = true;
-                        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
-                        pb.q_closest[ci] = 2 * p + 1;
+                        push_mis = true;
                     }
                 }
                 {  // phase-function continuation (path_tracing.cpp:99-108)
@@ -1949,7 +1970,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             pb.nee[slot] = f4(contrib, valid ? 1.0f : 0.0f);
             if (valid) {
                 pending = true;
-                push_shadow(pb, slot, so, sd, eps, tmax);
+                put_shadow(pb, slot, so, sd, eps, tmax);
+                shadow_mask |= 1u << l;
             }
         }
         // MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
@@ -1965,8 +1987,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                 const int32_t mm = medium_side(e, bs.dir, sp.geo.z);
                 flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u) | (uint32_t(mm + 1) << PF_MIS_MED_SHIFT);
                 pending = true;
-                const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
-                pb.q_closest[ci] = 2 * p + 1;
+                push_mis = true;
             }
         }
         // continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
@@ -1992,14 +2013,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     if (pending) flags |= PF_PENDING;
     pb.radiance[p] = f4(L, 0);
-    if (cont) {
-        flags |= PF_CONT;
-        const uint32_t ci = atomicAdd(&pb.cnt_out[0], 1u);
-        pb.q_closest[ci] = 2 * p;
-    }
-    if (flags & (PF_CONT | PF_PENDING)) {
-        const uint32_t qi = atomicAdd(&pb.cnt_out[2], 1u);
-        pb.q_out[qi] = p;
+    if (cont) flags |= PF_CONT;
+    // queue the rays of this vertex (wave-aggregated appends)
+    {
+        const uint32_t ci = wave_append(&pb.cnt_out[0], cont);
+        if (cont) pb.q_closest[ci] = 2 * p;
+        const uint32_t mi_ = wave_append(&pb.cnt_out[0], push_mis);
+        if (push_mis) pb.q_closest[mi_] = 2 * p + 1;
+        for (int32_t l = 0; l < rc.L; ++l) {
+            const bool v = (shadow_mask >> l) & 1u;
+            const uint32_t si = wave_append(&pb.cnt_out[1], v);
+            if (v) pb.q_shadow[si] = uint32_t(size_t(p) * rc.L + l);
+        }
+        const bool again = (flags & (PF_CONT | PF_PENDING)) != 0;
+        const uint32_t qi = wave_append(&pb.cnt_out[2], again);
+        if (again) pb.q_out[qi] = p;
     }
     pb.coef[p] = f4(coef, 0);
     pb.depth[p] = dep;

@@ -157,7 +157,7 @@ def main() -> int:
     # timed region: the library synchronises its stream before returning
     p = scenes.render_params(W, H, args.steps, cam, seed=42, sample_begin=args.warmup + st_steps, **tr)
     t0 = time.perf_counter()
-    ctx.render(p)
+    ctx.render(p, aovs=False)   # the image comes back to the host; the AOV planes are not asked for
     t1 = time.perf_counter()
     st = ctx.stats()
     wall = t1 - t0
