@@ -49,7 +49,8 @@ def _run(cmd: list[str], cwd: Path | None = None) -> None:
 
 
 def build_gpu() -> Path:
-    srcs = [CSRC / "agx_kernels.hip", CSRC / "agx_api.cpp", CSRC / "agx_build.cpp", CSRC / "agx_bvh4.cpp"]
+    srcs = [CSRC / "agx_kernels.hip", CSRC / "agx_guide.hip", CSRC / "agx_api.cpp", CSRC / "agx_build.cpp",
+            CSRC / "agx_bvh4.cpp"]
     tmp = GPU_LIB.with_suffix(".so.tmp")
     _run([HIPCC, *HIP_FLAGS, "-shared", "-pthread", "-Wl,--no-undefined", "-o", str(tmp), *map(str, srcs)])
     os.replace(tmp, GPU_LIB)

@@ -130,8 +130,9 @@ class agx_debug_summary(C.Structure):
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
-    "agx_debug_bvh4",
+    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train",
 ]
+GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
 
 
 class AgxError(RuntimeError):
@@ -167,6 +168,9 @@ def load_library() -> C.CDLL:
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
     lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
     lib.agx_debug_bvh4.argtypes = [P(agx_scene_desc), P(C.c_uint64), P(C.c_uint64), P(C.c_float), P(C.c_float)]
+    lib.agx_debug_guide_infer.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32, P(C.c_float)]
+    lib.agx_debug_guide_train.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
+                                          C.c_uint32, C.c_float, C.c_float, C.c_int, P(C.c_float)]
     if lib.agx_abi_version() != AGX_ABI_VERSION:
         raise RuntimeError("libatrc_gpu.so ABI version mismatch")
     _lib = lib
@@ -250,6 +254,27 @@ class Context:
         _check(self.lib.agx_trace_rays(self.h, fptr(o), fptr(d), fptr(tmin), fptr(tmax), n, 1 if any_hit else 0,
                                        iptr(ent), iptr(prim), fptr(t), fptr(u), fptr(v)))
         return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v}
+
+    def guide_infer(self, params: np.ndarray, inputs: np.ndarray) -> np.ndarray:
+        """Path-guiding network on (n, 8) inputs -> (n, 40): 8 x (mu.xyz, kappa), 8 weights."""
+        params = np.ascontiguousarray(params, np.float32)
+        inputs = np.ascontiguousarray(inputs, np.float32)
+        assert params.size == GUIDE_PARAMS and inputs.ndim == 2 and inputs.shape[1] == 8
+        out = np.zeros((len(inputs), 40), np.float32)
+        _check(self.lib.agx_debug_guide_infer(self.h, fptr(params), fptr(inputs), len(inputs), fptr(out)))
+        return out
+
+    def guide_train(self, params, m, v, samples, wscale: float, lr: float, t: int):
+        """One training step; returns (params, m, v, gradient)."""
+        params = np.array(params, np.float32)
+        m = np.array(m, np.float32)
+        v = np.array(v, np.float32)
+        samples = np.ascontiguousarray(samples, np.float32)
+        assert samples.ndim == 2 and samples.shape[1] == 12
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(self.lib.agx_debug_guide_train(self.h, fptr(params), fptr(m), fptr(v), fptr(samples), len(samples),
+                                              wscale, lr, t, fptr(g)))
+        return params, m, v, g
 
     def copy_kernel(self, nbytes: int) -> float:
         """One plain copy kernel of nbytes (traffic-counter calibration); device ms."""

@@ -12,6 +12,8 @@
 
 #include "../../include/atrc_gpu.h"
 #include "../../include/atrc_gpu_debug.h"
+#include "agx_guide.h"
+#include "agx_guide_kernels.h"
 #include "agx_build.h"
 #include "agx_kernels.h"
 
@@ -191,6 +193,13 @@ size_t max_paths_in_flight() {
     }
     return size_t(1) << 22;
 }
+
+// device buffer freed at scope exit (debug entry points only)
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    template <class T> T *get() { return static_cast<T *>(p); }
+};
 
 }  // namespace
 
@@ -654,6 +663,63 @@ int agx_debug_bvh4(const agx_scene_desc *desc, uint64_t *n_nodes, uint64_t *n_pr
     *n_prims = hs.b4_prims.size() / 3;
     if (nodes_out) std::memcpy(nodes_out, hs.b4_nodes.data(), hs.b4_nodes.size() * sizeof(vec4));
     if (prims_out) std::memcpy(prims_out, hs.b4_prims.data(), hs.b4_prims.size() * sizeof(vec4));
+    return AGX_OK;
+}
+
+int agx_debug_guide_infer(agx_ctx *c, const float *params, const float *inputs, uint32_t n, float *out) {
+    static_assert(AGX_GUIDE_PARAMS == G_PARAMS, "guide parameter count");
+    if (!c || !params || (n && (!inputs || !out))) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (!n) return AGX_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipError_t e = guide_prepare(c->device); e != hipSuccess)
+        return fail(AGX_E_UNSUPPORTED, std::string("guiding kernels: ") + hipGetErrorString(e) + " (LDS per block " +
+                                           std::to_string(guide_smem_bytes()) + " B needed)");
+    DevBuf dp, dpack, dbias, din, dout;
+    HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dpack.p, G_PACK * 2));
+    HIP_TRY(hipMalloc(&dbias.p, G_BIASES * sizeof(float)));
+    HIP_TRY(hipMalloc(&din.p, size_t(n) * G_IN * sizeof(float)));
+    HIP_TRY(hipMalloc(&dout.p, size_t(n) * G_ROW_OUT * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(dp.p, params, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(din.p, inputs, size_t(n) * G_IN * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    launch_guide_pack(dp.get<float>(), dpack.get<__bf16>(), dbias.get<float>(), c->stream);
+    launch_guide_infer(dpack.get<__bf16>(), dbias.get<float>(), din.get<float>(), dout.get<float>(), nullptr, n,
+                       c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, size_t(n) * G_ROW_OUT * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return AGX_OK;
+}
+
+int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const float *samples, uint32_t n,
+                          float wscale, float lr, int t, float *grad_out) {
+    if (!c || !params || !m || !v || (n && !samples)) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (t < 1) return fail(AGX_E_INVALID_ARG, "Adam step t must be >= 1");
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipError_t e = guide_prepare(c->device); e != hipSuccess)
+        return fail(AGX_E_UNSUPPORTED, std::string("guiding kernels: ") + hipGetErrorString(e));
+    constexpr int BLOCKS = 64;
+    DevBuf dp, dm, dv, ds, dpart, dg;
+    HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dm.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dv.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&ds.p, (size_t(n) + 1) * (G_IN + 4) * sizeof(float)));
+    HIP_TRY(hipMalloc(&dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dg.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMemcpyAsync(dp.p, params, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dm.p, m, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dv.p, v, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (n) HIP_TRY(hipMemcpyAsync(ds.p, samples, size_t(n) * (G_IN + 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    launch_guide_train(dp.get<float>(), ds.get<float>(), n, wscale, dpart.get<float>(), BLOCKS, c->stream);
+    launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
+    if (lr != 0.0f) launch_guide_adam(dp.get<float>(), dm.get<float>(), dv.get<float>(), dg.get<float>(), lr, 0.9f,
+                                      0.99f, 1e-8f, t, c->stream);
+    HIP_TRY(hipGetLastError());
+    if (grad_out) HIP_TRY(hipMemcpyAsync(grad_out, dg.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(params, dp.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(m, dm.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(v, dv.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return AGX_OK;
 }
 

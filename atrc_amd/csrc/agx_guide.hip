@@ -1,0 +1,416 @@
+// Path-guiding MLP on the gfx950 matrix cores: weight packing, batched
+// inference (one-blob encoding, 4 x 64 ReLU, mixture decoding), training
+// (forward, backward, KL-divergence gradient of the vMF mixture) and Adam.
+// Layout constants are in agx_guide.h.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "agx_guide.h"
+#include "agx_guide_kernels.h"
+
+namespace agx {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float PI_G = 3.14159265358979323846f;
+constexpr int ACT_STRIDE = G_HID + 8;      // bf16 per staged activation row (16-byte aligned rows)
+constexpr int OST_STRIDE = G_OUT + 1;      // floats per staged output row
+
+// v_mfma_f32_16x16x32_bf16 operand layouts (wave64):
+//   A (16 x 32): lane l holds row l % 16, k = 8 * (l / 16) + i, i = 0..7
+//   B (32 x 16): lane l holds column l % 16, k = 8 * (l / 16) + i
+//   D (16 x 16): lane l holds column l % 16, rows 4 * (l / 16) + j, j = 0..3
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// LDS of one wave must be visible to its other lanes before they read it
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one-blob feature of input value v for bin b: Gaussian of one bin width
+__device__ __forceinline__ float one_blob(float v, int b) {
+    const float x = v * float(G_BINS) - (float(b) + 0.5f);
+    return __expf(-0.5f * x * x);
+}
+
+__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// FP32 master parameters -> BF16 fragment-ordered weights + FP32 biases.
+// packed[off(l) + ((s * T + t) * 64 + lane) * 8 + i] = W_l[32 s + 8 (lane / 16) + i][16 t + lane % 16]
+__global__ void __launch_bounds__(256) k_guide_pack(const float *params, __bf16 *packed, float *bias) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid < G_PACK) {
+        int l = 0;
+        while (l + 1 < G_LAYERS && gid >= g_pack_off(l + 1)) ++l;
+        const int local = gid - g_pack_off(l);
+        const int i = local & 7, lane = (local >> 3) & 63, frag = local >> 9;
+        const int T = g_out(l) / 16;
+        const int s = frag / T, t = frag % T;
+        const int k = 32 * s + 8 * (lane / 16) + i, n = 16 * t + (lane % 16);
+        packed[gid] = __bf16(params[g_w_off(l) + k * g_out(l) + n]);
+    }
+    if (gid < G_BIASES) {
+        int l = 0;
+        while (l + 1 < G_LAYERS && gid >= g_bias_off(l + 1)) ++l;
+        bias[gid] = params[g_b_off(l) + gid - g_bias_off(l)];
+    }
+}
+
+struct GuideSmem {
+    __bf16 w[G_PACK];
+    float b[G_BIASES];
+    __bf16 act[4][16][ACT_STRIDE];
+    float ost[4][16][OST_STRIDE];
+};
+
+// Hidden layer l (64 -> 64 or 64 -> 32) for the 16 rows staged in act[wave]:
+// reads the activations, then (after the wave has read them) overwrites them
+// with ReLU(x W + b), or writes the raw outputs to ost for the last layer.
+template <int L>
+__device__ __forceinline__ void hidden_layer(GuideSmem &sm, int wave, int lane) {
+    constexpr int T = g_out(L) / 16;
+    const int r = lane % 16, g = lane / 16;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(&sm.act[wave][r][8 * g]);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&sm.act[wave][r][32 + 8 * g]);
+    f32x4 acc[T];
+    #pragma unroll
+    for (int t = 0; t < T; ++t) {
+        acc[t] = f32x4{0, 0, 0, 0};
+        const bf16x8 *w0 = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(L) + ((0 * T + t) * 64 + lane) * 8]);
+        const bf16x8 *w1 = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(L) + ((1 * T + t) * 64 + lane) * 8]);
+        acc[t] = mfma(a0, *w0, acc[t]);
+        acc[t] = mfma(a1, *w1, acc[t]);
+    }
+    wave_sync();
+    #pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int col = 16 * t + r;
+        const float bias = sm.b[g_bias_off(L) + col];
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = acc[t][j] + bias;
+            if (L == G_LAYERS - 1) sm.ost[wave][4 * g + j][col] = v;
+            else sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
+        }
+    }
+    wave_sync();
+}
+
+// Batched inference: row i = in[8 i .. 8 i + 7] -> out[40 i ..]: 8 lobes of
+// (mu.xyz, kappa) then 8 normalised lobe weights.  Persistent blocks; each
+// wave takes 16-row tiles.
+__global__ void __launch_bounds__(256) k_guide_infer(const __bf16 *packed, const float *bias, const float *in,
+                                                     float *out, const uint32_t *n_ptr, uint32_t n_fixed) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    GuideSmem &sm = *reinterpret_cast<GuideSmem *>(smem_raw);
+    const uint32_t n = n_ptr ? *n_ptr : n_fixed;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    if (uint32_t(blockIdx.x) * 64u >= n) return;  // whole block past the rows (block-uniform)
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(packed);
+        uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
+        for (int i = tid; i < G_PACK / 8; i += 256) dst[i] = src[i];
+        for (int i = tid; i < G_BIASES; i += 256) sm.b[i] = bias[i];
+    }
+    __syncthreads();
+    const int r = lane % 16, g = lane / 16;
+    for (uint32_t tile = blockIdx.x * 4u + uint32_t(wave); tile * 16u < n; tile += gridDim.x * 4u) {
+        const uint32_t row = tile * 16u + uint32_t(r);
+        const bool valid = row < n;
+        float x[G_IN];
+        #pragma unroll
+        for (int s = 0; s < G_IN; ++s) {
+            const float v = valid ? in[size_t(row) * G_IN + s] : 0.0f;
+            x[s] = fminf(1.0f, fmaxf(0.0f, v));
+        }
+        // layer 0: k-step s covers the 32 bins of input s
+        f32x4 acc[4];
+        #pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int s = 0; s < G_IN; ++s) {
+            bf16x8 a;
+            #pragma unroll
+            for (int i = 0; i < 8; ++i) a[i] = __bf16(one_blob(x[s], 8 * g + i));
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 *w = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(0) + ((s * 4 + t) * 64 + lane) * 8]);
+                acc[t] = mfma(a, *w, acc[t]);
+            }
+        }
+        #pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int col = 16 * t + r;
+            const float b = sm.b[g_bias_off(0) + col];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = acc[t][j] + b;
+                sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
+            }
+        }
+        wave_sync();
+        hidden_layer<1>(sm, wave, lane);
+        hidden_layer<2>(sm, wave, lane);
+        hidden_layer<3>(sm, wave, lane);
+        hidden_layer<4>(sm, wave, lane);
+        // decode: lane (r, g) decodes lobes 2g and 2g + 1 of row r
+        #pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int lobe = 2 * g + q;
+            const float *o = &sm.ost[wave][r][4 * lobe];
+            const float th = PI_G * sigmoid(o[1]), ph = 2.0f * PI_G * sigmoid(o[2]);
+            const float kappa = __expf(fminf(G_LOGK_MAX, fmaxf(G_LOGK_MIN, o[3])));
+            float st, ct, sp, cp;
+            __sincosf(th, &st, &ct);
+            __sincosf(ph, &sp, &cp);
+            if (valid) {
+                float4 *dst = reinterpret_cast<float4 *>(out + size_t(row) * G_ROW_OUT + 4 * lobe);
+                *dst = make_float4(st * cp, st * sp, ct, kappa);
+            }
+        }
+        if (g == 0 && valid) {  // softmax of the 8 weight logits
+            float mx = -3.0e38f;
+            #pragma unroll
+            for (int j = 0; j < G_LOBES; ++j) mx = fmaxf(mx, sm.ost[wave][r][4 * j]);
+            float e[G_LOBES], sum = 0;
+            #pragma unroll
+            for (int j = 0; j < G_LOBES; ++j) { e[j] = __expf(sm.ost[wave][r][4 * j] - mx); sum += e[j]; }
+            #pragma unroll
+            for (int j = 0; j < G_LOBES; ++j) out[size_t(row) * G_ROW_OUT + 4 * G_LOBES + j] = e[j] / sum;
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- training
+// A training sample: the 8 network inputs, the sampled direction (world space,
+// unit) and its weight F / p (incident radiance luminance over the density the
+// direction was drawn with).  Loss = -sum_s w_s log q(dir_s) / sum_s w_s (the
+// KL divergence between the normalised target and the mixture, up to a constant).
+constexpr int G_SAMPLE = G_IN + 4;   // floats per sample
+constexpr int TS = 16;               // samples per chunk
+
+struct TrainSmem {
+    float x0[TS][G_FEAT];            // one-blob features
+    float h[4][TS][G_HID];           // hidden activations (after ReLU)
+    float o[TS][G_OUT];              // raw outputs, then d loss / d outputs
+    float d[2][TS][G_HID];           // back-propagated gradients (ping-pong)
+    float smp[TS][G_SAMPLE];
+};
+
+// log of the vMF density kappa / (2 pi (1 - e^-2kappa)) e^{kappa (mu.w - 1)}
+__device__ __forceinline__ float vmf_log_norm(float kappa) {
+    return logf(kappa) - logf(2.0f * PI_G) - logf(-expm1f(-2.0f * kappa));
+}
+
+// d(-log q(dir)) / d(raw outputs) of one sample, times its weight
+__device__ void mixture_grad(const float *o, const float *dir, float w, float *g) {
+    float logit_max = -3.0e38f;
+    for (int j = 0; j < G_LOBES; ++j) logit_max = fmaxf(logit_max, o[4 * j]);
+    float wsum = 0, wj[G_LOBES];
+    for (int j = 0; j < G_LOBES; ++j) { wj[j] = expf(o[4 * j] - logit_max); wsum += wj[j]; }
+    float lt[G_LOBES], kap[G_LOBES], cosm[G_LOBES], s1[G_LOBES], s2[G_LOBES], mdt[G_LOBES], mdp[G_LOBES];
+    float lmax = -3.0e38f;
+    for (int j = 0; j < G_LOBES; ++j) {
+        wj[j] /= wsum;
+        s1[j] = 1.0f / (1.0f + expf(-o[4 * j + 1]));
+        s2[j] = 1.0f / (1.0f + expf(-o[4 * j + 2]));
+        const float th = PI_G * s1[j], ph = 2.0f * PI_G * s2[j];
+        const float lk = fminf(G_LOGK_MAX, fmaxf(G_LOGK_MIN, o[4 * j + 3]));
+        kap[j] = expf(lk);
+        const float st = sinf(th), ct = cosf(th), sp = sinf(ph), cp = cosf(ph);
+        cosm[j] = st * cp * dir[0] + st * sp * dir[1] + ct * dir[2];
+        mdt[j] = ct * cp * dir[0] + ct * sp * dir[1] - st * dir[2];   // dir . dmu/dtheta
+        mdp[j] = -st * sp * dir[0] + st * cp * dir[1];                 // dir . dmu/dphi
+        lt[j] = logf(fmaxf(wj[j], 1e-30f)) + vmf_log_norm(kap[j]) + kap[j] * (cosm[j] - 1.0f);
+        lmax = fmaxf(lmax, lt[j]);
+    }
+    float qs = 0;
+    for (int j = 0; j < G_LOBES; ++j) qs += expf(lt[j] - lmax);
+    const float logq = lmax + logf(qs);
+    for (int j = 0; j < G_LOBES; ++j) {
+        const float r = expf(lt[j] - logq);
+        g[4 * j] = -w * (r - wj[j]);
+        g[4 * j + 1] = -w * r * kap[j] * mdt[j] * PI_G * s1[j] * (1.0f - s1[j]);
+        g[4 * j + 2] = -w * r * kap[j] * mdp[j] * 2.0f * PI_G * s2[j] * (1.0f - s2[j]);
+        const float lk_raw = o[4 * j + 3];
+        const bool free_k = lk_raw > G_LOGK_MIN && lk_raw < G_LOGK_MAX;
+        const float e2 = expf(-2.0f * kap[j]);
+        const float dlogv = 1.0f / kap[j] - 2.0f * e2 / (-expm1f(-2.0f * kap[j])) + (cosm[j] - 1.0f);
+        g[4 * j + 3] = free_k ? -w * r * dlogv * kap[j] : 0.0f;
+    }
+}
+
+// Forward + backward of chunks of TS samples; block b accumulates its
+// gradient contributions into partial[b][G_PARAMS] (no atomics: the sum over
+// blocks is taken by k_guide_adam in a fixed order, so training is deterministic).
+__global__ void __launch_bounds__(256) k_guide_train(const float *params, const float *samples, uint32_t n,
+                                                     float wscale, float *partial) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    TrainSmem &sm = *reinterpret_cast<TrainSmem *>(smem_raw);
+    float *grad = partial + size_t(blockIdx.x) * G_PARAMS;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < G_PARAMS; i += 256) grad[i] = 0.0f;
+    for (uint32_t c0 = blockIdx.x * TS; c0 < n; c0 += gridDim.x * TS) {
+        const int cnt = int(min(uint32_t(TS), n - c0));
+        __syncthreads();
+        for (int i = tid; i < TS * G_SAMPLE; i += 256) {
+            const int sI = i / G_SAMPLE, k = i % G_SAMPLE;
+            sm.smp[sI][k] = sI < cnt ? samples[size_t(c0 + sI) * G_SAMPLE + k] : 0.0f;
+        }
+        __syncthreads();
+        for (int i = tid; i < TS * G_FEAT; i += 256) {
+            const int sI = i / G_FEAT, f = i % G_FEAT;
+            const float v = fminf(1.0f, fmaxf(0.0f, sm.smp[sI][f / G_BINS]));
+            sm.x0[sI][f] = one_blob(v, f % G_BINS);
+        }
+        __syncthreads();
+        // forward: thread (n = tid % 64, quarter = tid / 64) computes 4 samples of column n
+        for (int l = 0; l < G_LAYERS; ++l) {
+            const int in_l = g_in(l), out_l = g_out(l);
+            const float *W = params + g_w_off(l), *B = params + g_b_off(l);
+            for (int job = tid; job < out_l * (TS / 4); job += 256) {
+                const int col = job % out_l, sq = job / out_l;
+                float acc[4];
+                for (int q = 0; q < 4; ++q) acc[q] = B[col];
+                for (int k = 0; k < in_l; ++k) {
+                    const float wk = W[k * out_l + col];
+                    for (int q = 0; q < 4; ++q) {
+                        const int sI = 4 * sq + q;
+                        const float a = l == 0 ? sm.x0[sI][k] : sm.h[l - 1][sI][k];
+                        acc[q] += a * wk;
+                    }
+                }
+                for (int q = 0; q < 4; ++q) {
+                    const int sI = 4 * sq + q;
+                    if (l < G_LAYERS - 1) sm.h[l][sI][col] = fmaxf(acc[q], 0.0f);
+                    else sm.o[sI][col] = acc[q];
+                }
+            }
+            __syncthreads();
+        }
+        // d loss / d outputs
+        if (tid < TS) {
+            float g[G_OUT];
+            const float w = tid < cnt ? sm.smp[tid][G_IN + 3] * wscale : 0.0f;
+            if (w != 0.0f) mixture_grad(sm.o[tid], &sm.smp[tid][G_IN], w, g);
+            else for (int j = 0; j < G_OUT; ++j) g[j] = 0.0f;
+            for (int j = 0; j < G_OUT; ++j) sm.o[tid][j] = g[j];
+        }
+        __syncthreads();
+        // backward: layer l has input activations a (x0 or h[l-1]) and output gradient dy
+        for (int l = G_LAYERS - 1; l >= 0; --l) {
+            const int in_l = g_in(l), out_l = g_out(l);
+            const float *W = params + g_w_off(l);
+            float *dy_buf = l == G_LAYERS - 1 ? &sm.o[0][0] : &sm.d[l % 2][0][0];
+            const int dy_stride = l == G_LAYERS - 1 ? G_OUT : G_HID;
+            // weight and bias gradients: thread owns (k, col) pairs
+            for (int job = tid; job < in_l * out_l; job += 256) {
+                const int k = job / out_l, col = job % out_l;
+                float acc = 0;
+                for (int sI = 0; sI < TS; ++sI) {
+                    const float a = l == 0 ? sm.x0[sI][k] : sm.h[l - 1][sI][k];
+                    acc += a * dy_buf[sI * dy_stride + col];
+                }
+                grad[g_w_off(l) + job] += acc;
+            }
+            for (int col = tid; col < out_l; col += 256) {
+                float acc = 0;
+                for (int sI = 0; sI < TS; ++sI) acc += dy_buf[sI * dy_stride + col];
+                grad[g_b_off(l) + col] += acc;
+            }
+            if (l > 0) {
+                // gradient wrt the input activations of layer l, through ReLU
+                float *dx = &sm.d[(l - 1) % 2][0][0];
+                for (int job = tid; job < TS * in_l; job += 256) {
+                    const int sI = job / in_l, k = job % in_l;
+                    float acc = 0;
+                    for (int col = 0; col < out_l; ++col) acc += dy_buf[sI * dy_stride + col] * W[k * out_l + col];
+                    dx[sI * G_HID + k] = sm.h[l - 1][sI][k] > 0.0f ? acc : 0.0f;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Sum of the per-block partial gradients in a fixed order.
+__global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int blocks, float *grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G_PARAMS) return;
+    float g = 0;
+    for (int b = 0; b < blocks; ++b) g += partial[size_t(b) * G_PARAMS + i];
+    grad[i] = g;
+}
+
+// One Adam step on the FP32 master parameters: m, v moments; lr; bias
+// corrections for step t.
+__global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, float *v, const float *grad, float lr,
+                                                    float b1, float b2, float eps, int t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G_PARAMS) return;
+    const float g = grad[i];
+    const float mi = b1 * m[i] + (1 - b1) * g;
+    const float vi = b2 * v[i] + (1 - b2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    const float mh = mi / (1 - powf(b1, float(t)));
+    const float vh = vi / (1 - powf(b2, float(t)));
+    params[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+}  // namespace
+
+size_t guide_smem_bytes() { return sizeof(GuideSmem); }
+size_t guide_train_smem_bytes() { return sizeof(TrainSmem); }
+
+void launch_guide_train(const float *params, const float *samples, uint32_t n, float wscale, float *partial,
+                        int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), sizeof(TrainSmem), st, params, samples, n, wscale,
+                       partial);
+}
+
+void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_reduce, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, grad);
+}
+
+void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
+                       int t, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_adam, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, params, m, v, grad, lr, b1, b2,
+                       eps, t);
+}
+
+void launch_guide_pack(const float *params, __bf16 *packed, float *bias, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_pack, dim3((G_PACK + 255) / 256), dim3(256), 0, st, params, packed, bias);
+}
+
+hipError_t guide_prepare(int device) {
+    int lds = 0;
+    hipError_t e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    if (e != hipSuccess) return e;
+    if (size_t(lds) < sizeof(GuideSmem) || size_t(lds) < sizeof(TrainSmem)) return hipErrorInvalidConfiguration;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GuideSmem)));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
+}
+
+void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,
+                        uint32_t n_max, hipStream_t st) {
+    if (!n_max) return;
+    // persistent: at most 2 blocks per CU resident (LDS), enough to cover the rows
+    const uint32_t tiles = (n_max + 63) / 64;
+    const uint32_t grid = tiles < 1024u ? tiles : 1024u;
+    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), sizeof(GuideSmem), st, packed, bias, in, out, n_dev,
+                       n_max);
+}
+
+}  // namespace agx

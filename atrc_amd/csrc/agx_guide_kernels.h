@@ -1,0 +1,31 @@
+// Launchers of the path-guiding network kernels (agx_guide.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace agx {
+
+// dynamic LDS bytes of the inference kernel
+size_t guide_smem_bytes();
+// checks the device's LDS limit against the kernels' needs and raises their
+// dynamic-LDS attribute; hipErrorInvalidConfiguration if the device has too little
+hipError_t guide_prepare(int device);
+// FP32 master parameters (G_PARAMS) -> BF16 fragment-ordered weights (G_PACK) + FP32 biases (G_BIASES)
+void launch_guide_pack(const float *params, __bf16 *packed, float *bias, hipStream_t st);
+// rows: n_dev (device count, may be NULL) or n_max; in: 8 floats per row; out: G_ROW_OUT floats per row
+void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,
+                        uint32_t n_max, hipStream_t st);
+
+// training: samples of G_IN + 4 floats (inputs, direction, weight); the loss
+// weights are samples' weight x wscale; blocks x G_PARAMS partial gradients
+size_t guide_train_smem_bytes();
+void launch_guide_train(const float *params, const float *samples, uint32_t n, float wscale, float *partial,
+                        int blocks, hipStream_t st);
+void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st);
+void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
+                       int t, hipStream_t st);
+
+}  // namespace agx

@@ -41,6 +41,18 @@ int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out);
 int agx_debug_bvh4(const agx_scene_desc *desc, uint64_t *n_nodes, uint64_t *n_prims, float *nodes_out,
                    float *prims_out);
 
+/* Path-guiding network (no counterpart in Atrc), exposed for its numerics tests.
+ * params: AGX_GUIDE_PARAMS floats (per layer l = 0..4: W_l [in][out] row-major,
+ * then b_l; layer sizes 256-64-64-64-64-32).
+ * infer: inputs n x 8 in [0, 1] -> out n x 40: 8 x (mu.xyz, kappa), 8 weights. */
+#define AGX_GUIDE_PARAMS 31008
+int agx_debug_guide_infer(agx_ctx *ctx, const float *params, const float *inputs, uint32_t n, float *out);
+/* One training step on n samples of 12 floats (8 inputs, direction xyz, weight):
+ * grad_out (may be NULL) receives the gradient of sum_s w_s wscale (-log q(dir_s));
+ * params, m, v are updated in place by one Adam step (t >= 1) unless lr == 0. */
+int agx_debug_guide_train(agx_ctx *ctx, float *params, float *m, float *v, const float *samples, uint32_t n,
+                          float wscale, float lr, int t, float *grad_out);
+
 /* Run one plain device-to-device copy kernel (k_copy_calibration) of `bytes`
  * bytes on the context's stream and wait for it: a known-size reference for
  * calibrating hardware traffic counters.  Returns its device time in *ms. */
