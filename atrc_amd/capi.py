@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 3
+AGX_ABI_VERSION = 4
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -97,7 +97,9 @@ class agx_render_params(C.Structure):
         ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("min_depth", C.c_int32),
         ("max_depth", C.c_int32), ("specular_depth", C.c_int32), ("cont_prob", C.c_float), ("use_mis", C.c_int32),
         ("eps", C.c_float), ("filter_radius", C.c_float), ("seed", C.c_uint64), ("camera", agx_camera_desc),
-        ("collect_stats", C.c_int32), ("sample_begin", C.c_int32),
+        ("collect_stats", C.c_int32), ("sample_begin", C.c_int32), ("guide", C.c_int32), ("guide_alpha", C.c_float),
+        ("guide_train", C.c_int32), ("guide_batch", C.c_int32), ("guide_steps", C.c_int32), ("guide_lr", C.c_float),
+        ("guide_reset", C.c_int32), ("guide_seed", C.c_uint32),
     ]
 
 
@@ -109,7 +111,8 @@ class agx_stats_t(C.Structure):
         ("closest_bvh_bytes", C.c_uint64), ("shadow_bvh_bytes", C.c_uint64),
         ("total_ms", C.c_double), ("trace_closest_ms", C.c_double), ("trace_shadow_ms", C.c_double),
         ("shade_ms", C.c_double), ("trace_closest_launches", C.c_uint64), ("trace_shadow_launches", C.c_uint64),
-        ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
+        ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_samples", C.c_uint64),
+        ("guide_steps", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

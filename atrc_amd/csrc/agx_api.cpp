@@ -70,6 +70,17 @@ struct agx_ctx {
     float *h_resolve = nullptr;      // pinned staging of the planes the caller asked for
     size_t resolve_cap = 0;          // pixels both buffers hold
     static constexpr int RING = 64;
+    // path guiding: network state (persists across renders) and its wave buffers
+    static constexpr int GUIDE_TRAIN_BLOCKS = 256;
+    bool guide_ready = false;
+    float *g_params = nullptr, *g_m = nullptr, *g_v = nullptr, *g_grad = nullptr, *g_partial = nullptr;
+    float *g_bias = nullptr, *g_samples = nullptr, *g_wsum = nullptr;
+    __bf16 *g_packed = nullptr;
+    uint32_t *g_count = nullptr;
+    uint32_t g_batch_cap = 0;
+    uint64_t g_steps = 0;
+    size_t g_wave_cap = 0;
+    std::vector<void *> guide_allocs, guide_wave_allocs;
     hipEvent_t ev[8]{};
     std::vector<hipEvent_t> pool;    // timing / readback events, reused across renders
     hipEvent_t event(size_t i) {
@@ -87,9 +98,24 @@ struct agx_ctx {
         scene_allocs.clear();
         has_scene = false;
     }
+    void free_guide() {
+        for (void *p : guide_allocs) if (p) (void)hipFree(p);
+        guide_allocs.clear();
+        guide_ready = false;
+        g_batch_cap = 0;
+        g_steps = 0;
+    }
+    void free_guide_wave() {
+        for (void *p : guide_wave_allocs) if (p) (void)hipFree(p);
+        guide_wave_allocs.clear();
+        pb.g_feat = pb.g_out = pb.g_rec = nullptr;
+        pb.g_rec_state = nullptr;
+        g_wave_cap = 0;
+    }
     void free_wave() {
         for (void *p : wave_allocs) if (p) (void)hipFree(p);
         wave_allocs.clear();
+        free_guide_wave();
         pb = PathBuffers{};
         counters = nullptr;
         cap_paths = 0;
@@ -100,6 +126,67 @@ struct agx_ctx {
         hipError_t e = dalloc(p, count);
         if (e == hipSuccess) wave_allocs.push_back(*p);
         return e;
+    }
+    template <class T> hipError_t galloc(std::vector<void *> &list, T **p, size_t count) {
+        hipError_t e = dalloc(p, count);
+        if (e == hipSuccess) list.push_back(*p);
+        return e;
+    }
+    // per-path buffers of guided rendering (allocated on the first guided render)
+    int ensure_guide_wave(size_t n) {
+        if (n <= g_wave_cap) return AGX_OK;
+        free_guide_wave();
+        const size_t nr = n / GUIDE_REC_STRIDE + 1;
+        HIP_TRY(galloc(guide_wave_allocs, &pb.g_feat, n * 8));
+        HIP_TRY(galloc(guide_wave_allocs, &pb.g_out, n * GUIDE_ROW));
+        HIP_TRY(galloc(guide_wave_allocs, &pb.g_rec, nr * GUIDE_MAX_REC * GUIDE_REC_FLOATS));
+        HIP_TRY(galloc(guide_wave_allocs, &pb.g_rec_state, nr));
+        g_wave_cap = n;
+        return AGX_OK;
+    }
+    // network state; parameters initialised uniform in +-1/sqrt(fan-in) (weights)
+    // and +-0.1 (biases) from a splitmix64 stream of `seed`, moments zero
+    int guide_init(uint32_t seed, uint32_t batch) {
+        if (!guide_ready) {
+            HIP_TRY(galloc(guide_allocs, &g_params, G_PARAMS));
+            HIP_TRY(galloc(guide_allocs, &g_m, G_PARAMS));
+            HIP_TRY(galloc(guide_allocs, &g_v, G_PARAMS));
+            HIP_TRY(galloc(guide_allocs, &g_grad, G_PARAMS));
+            HIP_TRY(galloc(guide_allocs, &g_partial, size_t(GUIDE_TRAIN_BLOCKS) * G_PARAMS));
+            HIP_TRY(galloc(guide_allocs, &g_bias, G_BIASES));
+            HIP_TRY(galloc(guide_allocs, &g_packed, G_PACK));
+            HIP_TRY(galloc(guide_allocs, &g_count, 4));
+            HIP_TRY(galloc(guide_allocs, &g_wsum, 4));
+        }
+        std::vector<float> h(G_PARAMS);
+        uint64_t x = 0x243F6A8885A308D3ull ^ uint64_t(seed);
+        auto next = [&]() {  // splitmix64 -> uniform in [-1, 1)
+            x += 0x9E3779B97F4A7C15ull;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            return float(double(z >> 11) * 0x1p-53 * 2.0 - 1.0);
+        };
+        for (int l = 0; l < G_LAYERS; ++l) {
+            const float bw = 1.0f / std::sqrt(float(g_in(l)));
+            for (int i = 0; i < g_in(l) * g_out(l); ++i) h[size_t(g_w_off(l) + i)] = bw * next();
+            for (int i = 0; i < g_out(l); ++i) h[size_t(g_b_off(l) + i)] = 0.1f * next();
+        }
+        HIP_TRY(hipMemcpyAsync(g_params, h.data(), G_PARAMS * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipMemsetAsync(g_m, 0, G_PARAMS * sizeof(float), stream));
+        HIP_TRY(hipMemsetAsync(g_v, 0, G_PARAMS * sizeof(float), stream));
+        launch_guide_pack(g_params, g_packed, g_bias, stream);
+        HIP_TRY(hipStreamSynchronize(stream));
+        g_steps = 0;
+        guide_ready = true;
+        return ensure_guide_batch(batch);
+    }
+    int ensure_guide_batch(uint32_t batch) {
+        if (batch <= g_batch_cap) return AGX_OK;
+        HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
+        g_batch_cap = batch;
+        return AGX_OK;
     }
     int ensure_wave(size_t n, int L, int64_t wh) {
         if (n <= cap_paths && L <= cap_L && wh <= cap_wh) return AGX_OK;
@@ -244,6 +331,7 @@ void agx_destroy(agx_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->free_scene();
     c->free_wave();
+    c->free_guide();
     if (c->dcnt) (void)hipFree(c->dcnt);
     if (c->h_counters) (void)hipHostFree(c->h_counters);
     if (c->d_resolve) (void)hipFree(c->d_resolve);
@@ -361,6 +449,21 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     rc.sample_begin = p->sample_begin;
     rc.L = c->n_lights;
     camera_frame(p->camera, &rc);
+    const bool guide = p->guide != 0;
+    const bool guide_train = guide && p->guide_train != 0;
+    const uint32_t guide_batch = p->guide_batch > 0 ? uint32_t(p->guide_batch) : 65536u;
+    const int guide_steps = p->guide_steps > 0 ? p->guide_steps : 4;
+    const float guide_lr = p->guide_lr > 0 ? p->guide_lr : 1e-2f;
+    if (guide) {
+        const float a = p->guide_alpha > 0 ? p->guide_alpha : 0.5f;
+        if (!(a > 0 && a < 1)) return fail(AGX_E_INVALID_ARG, "guide_alpha must lie in (0, 1)");
+        if (guide_batch > (1u << 20)) return fail(AGX_E_INVALID_ARG, "guide_batch above 1 << 20");
+        if (hipError_t e = guide_prepare(c->device); e != hipSuccess)
+            return fail(AGX_E_UNSUPPORTED, std::string("path guiding: ") + hipGetErrorString(e));
+        rc.guide = 1;
+        rc.guide_alpha = a;
+        rc.guide_record = guide_train ? 1 : 0;
+    }
 
     const int64_t wh = int64_t(p->width) * p->height;
     const int64_t total = wh * p->spp;
@@ -370,6 +473,14 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     else per_pass = int64_t(cap);
     int r = c->ensure_wave(size_t(per_pass), rc.L, wh);
     if (r != AGX_OK) return r;
+    if (guide) {
+        if ((r = c->ensure_guide_wave(size_t(per_pass))) != AGX_OK) return r;
+        if (!c->guide_ready || p->guide_reset) {
+            if ((r = c->guide_init(p->guide_seed, guide_batch)) != AGX_OK) return r;
+        } else if ((r = c->ensure_guide_batch(guide_batch)) != AGX_OK) {
+            return r;
+        }
+    }
 
     const bool stats = p->collect_stats != 0;
     agx_stats_t st{};
@@ -409,6 +520,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         return fail(AGX_E_UNSUPPORTED, "max_depth + specular_depth too large (at most " +
                                            std::to_string(agx_ctx::RING - 3) + ")");
     bool cancelled = false;
+    uint32_t guide_samples_h = 0;
     for (int64_t base = 0; base < total && !cancelled; base += per_pass) {
         rc.work_base = base;
         rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
@@ -432,6 +544,16 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             if ((r = mark(&e0))) return r;
             launch_trace_closest(c->ds, c->pb, n_closest_max, stats, c->dcnt, c->stream);
             if ((r = mark(&e1))) return r;
+            if (guide) {
+                // the network's mixture for every vertex about to be shaded
+                launch_guide_features(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
+                launch_guide_infer(c->g_packed, c->g_bias, c->pb.g_feat, c->pb.g_out, cin + 2, uint32_t(rc.n_paths),
+                                   c->stream);
+                size_t eg;
+                if ((r = mark(&eg))) return r;
+                timed.push_back({e1, eg, 3});
+                e1 = eg;
+            }
             launch_trace_shadow(c->ds, c->pb, n_shadow_max, stats, c->dcnt + 1, c->stream);
             if ((r = mark(&e2))) return r;
             launch_reset_counters(cout, c->stream);
@@ -456,7 +578,30 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         launch_film(rc, c->pb, c->stream);
         if ((r = mark(&b))) return r;
         timed.push_back({a, b, 2});
+        if (guide_train) {
+            // samples from this pass's recording paths, then Adam steps, then the
+            // BF16 weights of the next pass
+            size_t ta, tb;
+            if ((r = mark(&ta))) return r;
+            HIP_TRY(hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
+            HIP_TRY(hipMemsetAsync(c->g_wsum, 0, sizeof(float), c->stream));
+            launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_wsum, c->stream);
+            for (int k = 0; k < guide_steps; ++k) {
+                launch_guide_train(c->g_params, c->g_samples, c->g_count, guide_batch, c->g_wsum, c->g_partial,
+                                   agx_ctx::GUIDE_TRAIN_BLOCKS, c->stream);
+                launch_guide_reduce(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_grad, c->stream);
+                c->g_steps += 1;
+                launch_guide_adam(c->g_params, c->g_m, c->g_v, c->g_grad, guide_lr, 0.9f, 0.99f, 1e-8f,
+                                  int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream);
+            }
+            launch_guide_pack(c->g_params, c->g_packed, c->g_bias, c->stream);
+            HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            if ((r = mark(&tb))) return r;
+            timed.push_back({ta, tb, 4});
+            b = tb;
+        }
         HIP_TRY(hipEventSynchronize(c->pool[b]));
+        if (guide_train) st.guide_samples += std::min<uint32_t>(guide_samples_h, guide_batch);
         // sum the ray queue sizes of this pass (every read has completed)
         for (size_t k = first_read; k < reads.size(); ++k) {
             const uint32_t *q = c->h_counters + 4 * reads[k].slot;
@@ -477,6 +622,8 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         const double ms = elapsed(c->pool[t.a], c->pool[t.b]);
         if (t.kind == 0) st.trace_closest_ms += ms;
         else if (t.kind == 1) st.trace_shadow_ms += ms;
+        else if (t.kind == 3) st.guide_infer_ms += ms;
+        else if (t.kind == 4) st.guide_train_ms += ms;
         else st.shade_ms += ms;
     }
     if (stats) {
@@ -484,6 +631,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         HIP_TRY(hipMemcpy(hc, c->dcnt, sizeof(hc), hipMemcpyDeviceToHost));
         fill_traversal_stats(hc, &st);
     }
+    st.guide_steps = c->g_steps;
     c->stats = st;
     // normalise into host outputs
     if (rgb || albedo || normal || denoise) {
@@ -699,18 +847,28 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
     if (hipError_t e = guide_prepare(c->device); e != hipSuccess)
         return fail(AGX_E_UNSUPPORTED, std::string("guiding kernels: ") + hipGetErrorString(e));
     constexpr int BLOCKS = 64;
-    DevBuf dp, dm, dv, ds, dpart, dg;
+    DevBuf dp, dm, dv, ds, dpart, dg, dscal;
     HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dm.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dv.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&ds.p, (size_t(n) + 1) * (G_IN + 4) * sizeof(float)));
     HIP_TRY(hipMalloc(&dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dg.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dscal.p, 2 * sizeof(float)));
+    {
+        // device scalars of the training kernel: sample count and weight sum (= 1 / wscale)
+        const uint32_t nn = n;
+        const float ws = wscale > 0.0f ? 1.0f / wscale : 0.0f;
+        HIP_TRY(hipMemcpyAsync(dscal.p, &nn, 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(static_cast<char *>(dscal.p) + 4, &ws, 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
     HIP_TRY(hipMemcpyAsync(dp.p, params, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dm.p, m, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(dv.p, v, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
     if (n) HIP_TRY(hipMemcpyAsync(ds.p, samples, size_t(n) * (G_IN + 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    launch_guide_train(dp.get<float>(), ds.get<float>(), n, wscale, dpart.get<float>(), BLOCKS, c->stream);
+    launch_guide_train(dp.get<float>(), ds.get<float>(), dscal.get<uint32_t>(), n, dscal.get<float>() + 1,
+                       dpart.get<float>(), BLOCKS, c->stream);
     launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
     if (lr != 0.0f) launch_guide_adam(dp.get<float>(), dm.get<float>(), dv.get<float>(), dg.get<float>(), lr, 0.9f,
                                       0.99f, 1e-8f, t, c->stream);

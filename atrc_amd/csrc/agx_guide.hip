@@ -251,10 +251,14 @@ __device__ void mixture_grad(const float *o, const float *dir, float w, float *g
 // Forward + backward of chunks of TS samples; block b accumulates its
 // gradient contributions into partial[b][G_PARAMS] (no atomics: the sum over
 // blocks is taken by k_guide_adam in a fixed order, so training is deterministic).
-__global__ void __launch_bounds__(256) k_guide_train(const float *params, const float *samples, uint32_t n,
-                                                     float wscale, float *partial) {
+__global__ void __launch_bounds__(256) k_guide_train(const float *params, const float *samples,
+                                                     const uint32_t *n_dev, uint32_t cap, const float *wsum_dev,
+                                                     float *partial) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     TrainSmem &sm = *reinterpret_cast<TrainSmem *>(smem_raw);
+    const uint32_t n = min(*n_dev, cap);
+    const float wsum = *wsum_dev;
+    const float wscale = wsum > 0.0f ? 1.0f / wsum : 0.0f;
     float *grad = partial + size_t(blockIdx.x) * G_PARAMS;
     const int tid = threadIdx.x;
     for (int i = tid; i < G_PARAMS; i += 256) grad[i] = 0.0f;
@@ -371,10 +375,10 @@ __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, flo
 size_t guide_smem_bytes() { return sizeof(GuideSmem); }
 size_t guide_train_smem_bytes() { return sizeof(TrainSmem); }
 
-void launch_guide_train(const float *params, const float *samples, uint32_t n, float wscale, float *partial,
-                        int blocks, hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), sizeof(TrainSmem), st, params, samples, n, wscale,
-                       partial);
+void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
+                        const float *wsum_dev, float *partial, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), sizeof(TrainSmem), st, params, samples, n_dev, cap,
+                       wsum_dev, partial);
 }
 
 void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st) {

@@ -19,11 +19,12 @@ void launch_guide_pack(const float *params, __bf16 *packed, float *bias, hipStre
 void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,
                         uint32_t n_max, hipStream_t st);
 
-// training: samples of G_IN + 4 floats (inputs, direction, weight); the loss
-// weights are samples' weight x wscale; blocks x G_PARAMS partial gradients
+// training: min(*n_dev, cap) samples of G_IN + 4 floats (inputs, direction,
+// weight); the loss weights are the samples' weights / *wsum_dev; blocks x
+// G_PARAMS partial gradients
 size_t guide_train_smem_bytes();
-void launch_guide_train(const float *params, const float *samples, uint32_t n, float wscale, float *partial,
-                        int blocks, hipStream_t st);
+void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
+                        const float *wsum_dev, float *partial, int blocks, hipStream_t st);
 void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st);
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
                        int t, hipStream_t st);

@@ -36,7 +36,20 @@ struct PathBuffers {
     float *film_sum;     // [10*WH] rgb, albedo, normal, denoise
     float *film_comp;    // [10*WH] Neumaier compensation
     float *film_count;   // [WH]
+    // path guiding (RenderConst::guide)
+    float *g_feat;       // [N*8] network inputs of the vertices in q_in order
+    float *g_out;        // [N*40] decoded mixture of those vertices (8 x (mu.xyz, kappa), 8 weights)
+    float *g_rec;        // [N/GUIDE_REC_STRIDE * GUIDE_MAX_REC * GUIDE_REC_FLOATS] training records
+    int2 *g_rec_state;   // [N/GUIDE_REC_STRIDE] (records made, record of the previous vertex or -1)
 };
+
+// Path guiding: every GUIDE_REC_STRIDE-th path keeps training records of its
+// first GUIDE_MAX_REC guided vertices: inputs[8], direction[3], mixture pdf,
+// throughput after the vertex[3], incident radiance estimate[3].
+constexpr int GUIDE_REC_STRIDE = 16;
+constexpr int GUIDE_MAX_REC = 16;
+constexpr int GUIDE_REC_FLOATS = 18;
+constexpr int GUIDE_ROW = 40;
 
 enum : uint32_t {
     PF_CONT = 1u,        // continuation ray 2p is queued
@@ -60,7 +73,10 @@ struct RenderConst {
     int32_t n_paths;     // paths in this pass
     int32_t sample_begin;
     int32_t L;           // number of lights
-    int32_t pad;
+    int32_t guide;       // 1: guided continuation sampling
+    float guide_alpha;   // probability of drawing from the learned density
+    int32_t guide_record;  // 1: guided vertices of the recording paths keep training records
+    int32_t pad2;
     float4 cam_pos, cam_x, cam_y, cam_z;  // camera frame
     float film_w, film_h, lens_r, focal;
 };
@@ -73,6 +89,13 @@ void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max
 void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st);
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st);
+// network inputs of the vertices about to be shaded (q_in order)
+void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
+                           hipStream_t st);
+// training samples (inputs, direction, weight) from the records of finished paths;
+// appends at most cap samples, counts them in *count and sums their weights in *wsum
+void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
+                         uint32_t *count, float *wsum, hipStream_t st);
 void launch_reset_counters(uint32_t *c, hipStream_t st);
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st);
 void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st);

@@ -1559,6 +1559,7 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         pb.q_out[qi] = uint32_t(p);
         pb.q_closest[ci] = uint32_t(2 * p);
     }
+    if (rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE] = make_int2(0, -1);
     pb.coef[p] = coef;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
@@ -1773,6 +1774,98 @@ __device__ __forceinline__ void put_shadow(const PathBuffers &pb, size_t slot, V
     pb.sh_d[slot] = f4(d, tmin);
 }
 
+// ---------------------------------------------------------------- path guiding
+// Network inputs of a surface vertex, each in [0, 1]: position in the scene's
+// bounding sphere, shading normal, and the spherical angles of the direction
+// towards the previous vertex.
+__device__ void guide_inputs(const DevScene &s, const Surf &sp, float *x) {
+    const float inv = 0.5f / s.world.w;
+    x[0] = (sp.pos.x - s.world.x) * inv + 0.5f;
+    x[1] = (sp.pos.y - s.world.y) * inv + 0.5f;
+    x[2] = (sp.pos.z - s.world.z) * inv + 0.5f;
+    const V n = normalize(sp.user.z);
+    x[3] = 0.5f * (n.x + 1);
+    x[4] = 0.5f * (n.y + 1);
+    x[5] = 0.5f * (n.z + 1);
+    const V w = normalize(sp.wr);
+    x[6] = acosf(smin(1.0f, smax(-1.0f, w.z))) / PI_F;
+    x[7] = (atan2f(w.y, w.x) + PI_F) / (2 * PI_F);
+}
+__device__ __forceinline__ bool guidable(int32_t kind) {
+    return kind == AGX_MAT_IDEAL_DIFFUSE || kind == AGX_MAT_METAL || kind == AGX_MAT_DISNEY;
+}
+// von Mises-Fisher lobe: density kappa / (2 pi (1 - e^-2kappa)) e^{kappa (mu.d - 1)}
+__device__ __forceinline__ float vmf_pdf(V mu, float k, V d) {
+    return k / (2 * PI_F * -expm1f(-2 * k)) * expf(k * (dot(mu, d) - 1));
+}
+// mixture density of a decoded network row at the unit direction d
+__device__ float guide_pdf(const float *G, V d) {
+    float q = 0;
+    for (int j = 0; j < 8; ++j) {
+        const float4 l = *reinterpret_cast<const float4 *>(G + 4 * j);
+        q += G[32 + j] * vmf_pdf(mk(l.x, l.y, l.z), l.w, d);
+    }
+    return q;
+}
+// lobe j by inverse CDF of the weights with u0, then the stable vMF sampler
+// w = 1 + log(u + (1 - u) e^-2kappa) / kappa around mu
+__device__ V guide_sample(const float *G, float u0, float u1, float u2) {
+    int j = 0;
+    float c = G[32];
+    while (j < 7 && u0 >= c) { ++j; c += G[32 + j]; }
+    const float4 l = *reinterpret_cast<const float4 *>(G + 4 * j);
+    const float k = l.w;
+    const float w = 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * k)) / k;
+    const float sw = sqrtf(smax(0.0f, 1 - w * w));
+    const float phi = 2 * PI_F * u2;
+    return normalize(to_global(frame_from_z(mk(l.x, l.y, l.z)), mk(sw * cosf(phi), sw * sinf(phi), w)));
+}
+__device__ __forceinline__ float lum3(V c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+__global__ void __launch_bounds__(256) k_guide_features(DevScene s, RenderConst rc, PathBuffers pb) {
+    const uint32_t n = pb.cnt_in[2];
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    const uint32_t p = pb.q_in[tid];
+    float x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const float4 h4 = pb.hit[2 * p];
+    const int32_t ent = __float_as_int(h4.w);
+    if ((pb.flags[p] & PF_CONT) && ent >= 0) {
+        HitRec h;
+        h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
+        const Surf sp = surface_at(s, h, mk(pb.ray_o[2 * p]), mk(pb.ray_d[2 * p]), rc.eps);
+        guide_inputs(s, sp, x);
+    }
+    float4 *dst = reinterpret_cast<float4 *>(pb.g_feat + size_t(tid) * 8);
+    dst[0] = make_float4(x[0], x[1], x[2], x[3]);
+    dst[1] = make_float4(x[4], x[5], x[6], x[7]);
+}
+
+// Training samples from the records of the recording paths: weight = luminance
+// of the incident radiance estimate / density the direction was drawn with.
+__global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffers pb, float *samples, uint32_t cap,
+                                                      uint32_t *count, float *wsum) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t r = gid / GUIDE_MAX_REC, j = gid % GUIDE_MAX_REC;
+    const uint32_t n_rec_paths = uint32_t(rc.n_paths + GUIDE_REC_STRIDE - 1) / GUIDE_REC_STRIDE;
+    bool ok = false;
+    float w = 0;
+    const float *rec = nullptr;
+    if (r < n_rec_paths && int32_t(j) < pb.g_rec_state[r].x) {
+        rec = pb.g_rec + (size_t(r) * GUIDE_MAX_REC + j) * GUIDE_REC_FLOATS;
+        const float pdf = rec[11];
+        w = lum3(mk(rec[15], rec[16], rec[17])) / pdf;
+        ok = pdf > 0 && w > 0 && isfinite(w);
+    }
+    const uint32_t idx = wave_append(count, ok);
+    if (ok && idx < cap) {
+        float *dst = samples + size_t(idx) * 12;
+        for (int k = 0; k < 11; ++k) dst[k] = rec[k];
+        dst[11] = w;
+        atomicAdd(wsum, w);
+    }
+}
+
 __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
     const uint32_t n = pb.cnt_in[2];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1800,6 +1893,19 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         }
         if (flags & PF_MIS) direct = direct + resolve_mis(s, pb, p, flags, eps, rtr);
         L = L + direct;
+        if (rc.guide_record && p % GUIDE_REC_STRIDE == 0 && !black(direct)) {
+            // light reaching the previous vertex arrives at every earlier recorded
+            // vertex along its sampled direction: incident radiance += direct / T
+            const int2 st = pb.g_rec_state[p / GUIDE_REC_STRIDE];
+            float *recs = pb.g_rec + size_t(p / GUIDE_REC_STRIDE) * GUIDE_MAX_REC * GUIDE_REC_FLOATS;
+            for (int j = 0; j < st.x; ++j) {
+                if (j == st.y) continue;
+                float *rec = recs + j * GUIDE_REC_FLOATS;
+                if (rec[12] > 0) rec[15] += direct.x / rec[12];
+                if (rec[13] > 0) rec[16] += direct.y / rec[13];
+                if (rec[14] > 0) rec[17] += direct.z / rec[14];
+            }
+        }
         flags &= ~(PF_PENDING | PF_MIS | PF_MIS_DELTA);
         flags &= (1u << PF_MIS_MED_SHIFT) - 1u;
     }
@@ -1810,6 +1916,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         return;
     }
     flags &= ~PF_CONT;
+    // this vertex has made no training record (yet): its direct light will reach every earlier record
+    if (rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE].y = -1;
     V coef = mk(pb.coef[p]);
     int4 dep = pb.depth[p];
     const float4 h4 = pb.hit[2 * p];
@@ -1991,7 +2099,55 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             }
         }
         // continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
-        {
+        if (rc.guide && guidable(b.kind)) {
+            // one-sample mixture of the learned density (probability alpha) and the BSDF
+            const float usel = rng.uniform();
+            const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+            const float *G = pb.g_out + size_t(tid) * GUIDE_ROW;
+            const float a = rc.guide_alpha;
+            V dir, f;
+            float pbsdf;
+            if (usel < a) {
+                dir = guide_sample(G, u0, u1, u2);
+                f = bsdf_eval(b, dir, sp.wr);
+                pbsdf = bsdf_pdf(b, dir, sp.wr);
+            } else {
+                const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+                dir = normalize(bs.dir);
+                f = bs.f;
+                pbsdf = bs.pdf;
+            }
+            const float pdf = a * guide_pdf(G, dir) + (1 - a) * pbsdf;
+            if (!black(f) && !(pdf < eps) && isfinite(pdf)) {
+                const float abscos = fabsf(cosv(sp.geo.z, dir));
+                coef = coef * (f * abscos / pdf);
+                pb.ray_o[2 * p] = f4(eps_offset(sp, dir, eps), 0.0f);
+                pb.ray_d[2 * p] = f4(dir, INF_F);
+                dep.x += 1;
+                if (dep.x <= rc.max_depth) {
+                    cont = true;
+                    if (dep.x > rc.min_depth) {
+                        if (rng.uniform() > rc.cont_prob) cont = false;
+                        else coef = coef / rc.cont_prob;
+                    }
+                }
+                if (cont && rc.guide_record && p % GUIDE_REC_STRIDE == 0) {
+                    int2 st = pb.g_rec_state[p / GUIDE_REC_STRIDE];
+                    if (st.x < GUIDE_MAX_REC) {
+                        float *rec = pb.g_rec + (size_t(p / GUIDE_REC_STRIDE) * GUIDE_MAX_REC + st.x) * GUIDE_REC_FLOATS;
+                        const float *x = pb.g_feat + size_t(tid) * 8;
+                        for (int k = 0; k < 8; ++k) rec[k] = x[k];
+                        rec[8] = dir.x; rec[9] = dir.y; rec[10] = dir.z;
+                        rec[11] = pdf;
+                        rec[12] = coef.x; rec[13] = coef.y; rec[14] = coef.z;
+                        rec[15] = rec[16] = rec[17] = 0;
+                        st.y = st.x;
+                        st.x += 1;
+                        pb.g_rec_state[p / GUIDE_REC_STRIDE] = st;
+                    }
+                }
+            }
+        } else {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
             const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
             if (!black(bs.f) && !(bs.pdf < eps)) {
@@ -2169,6 +2325,19 @@ void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
     hipLaunchKernelGGL(k_shade, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+}
+
+void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
+                           hipStream_t st) {
+    if (!max_n) return;
+    hipLaunchKernelGGL(k_guide_features, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+}
+
+void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
+                         uint32_t *count, float *wsum, hipStream_t st) {
+    const uint64_t n = uint64_t((rc.n_paths + GUIDE_REC_STRIDE - 1) / GUIDE_REC_STRIDE) * GUIDE_MAX_REC;
+    if (!n) return;
+    hipLaunchKernelGGL(k_guide_gather, dim3(grid_of(n, 256)), dim3(256), 0, st, rc, pb, samples, cap, count, wsum);
 }
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, st, src, dst, n);

@@ -218,7 +218,9 @@ class Scene:
 
 def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth: int = 5, max_depth: int = 10,
                   cont_prob: float = 0.9, specular_depth: int = 20, eps: float = 3e-4, seed: int = 42,
-                  filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0):
+                  filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0,
+                  guide: bool = False, guide_alpha: float = 0.5, guide_train: bool = True, guide_batch: int = 65536,
+                  guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0):
     p = capi.agx_render_params()
     p.width, p.height, p.spp = width, height, spp
     p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
@@ -236,6 +238,14 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
     p.camera.film_aspect = width / height
     p.collect_stats = 1 if collect_stats else 0
     p.sample_begin = sample_begin
+    p.guide = 1 if guide else 0
+    p.guide_alpha = guide_alpha
+    p.guide_train = 1 if guide_train else 0
+    p.guide_batch = guide_batch
+    p.guide_steps = guide_steps
+    p.guide_lr = guide_lr
+    p.guide_reset = 1 if guide_reset else 0
+    p.guide_seed = guide_seed
     return p
 
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 3
+#define AGX_ABI_VERSION 4
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -203,6 +203,21 @@ typedef struct agx_render_params {
     agx_camera_desc camera;
     int32_t collect_stats;     /* 1: count node visits / triangle tests (slower) */
     int32_t sample_begin;      /* first sample index rendered (progressive rendering), normally 0 */
+    /* Path guiding (no counterpart in Atrc; 0 keeps Atrc's estimator). At a
+     * Lambert, GGX-metal or Disney vertex the continuation direction is drawn
+     * from a one-sample mixture: with probability guide_alpha from a learned
+     * density (8 von Mises-Fisher lobes predicted by a 4 x 64 MLP from the
+     * vertex position, normal and view direction), else from the BSDF; the
+     * estimate stays unbiased.  The network lives in the context and keeps
+     * learning across renders. */
+    int32_t guide;             /* 1: guided continuation sampling */
+    float   guide_alpha;       /* in (0, 1); default 0.5 */
+    int32_t guide_train;       /* 1: train after every pass of this render */
+    int32_t guide_batch;       /* training samples per pass, at most 1 << 20 (default 65536) */
+    int32_t guide_steps;       /* Adam steps per pass (default 4) */
+    float   guide_lr;          /* Adam learning rate (default 1e-2) */
+    int32_t guide_reset;       /* 1: re-initialise the network from guide_seed first */
+    uint32_t guide_seed;
 } agx_render_params;
 
 /* Counters and timers of the last agx_render / agx_trace_rays call. */
@@ -223,6 +238,10 @@ typedef struct agx_stats_t {
     double   shade_ms;         /* device time in raygen + shade + film kernels */
     uint64_t trace_closest_launches;
     uint64_t trace_shadow_launches;
+    double   guide_infer_ms;   /* device time of the guiding network's features + inference */
+    double   guide_train_ms;   /* device time of gathering samples + training steps */
+    uint64_t guide_samples;    /* training samples used */
+    uint64_t guide_steps;      /* Adam steps taken by the context's network so far */
     uint32_t node_bytes, prim_bytes, info_bytes; /* record sizes used for the byte counts */
     uint32_t reserved;
 } agx_stats_t;

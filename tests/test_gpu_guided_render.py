@@ -1,0 +1,52 @@
+"""Guided path tracing (no counterpart in Atrc): the continuation direction is
+drawn from a one-sample mixture of the BSDF and the learned vMF mixture, with
+the mixture density in the estimator, so the image must stay the one Atrc's
+estimator converges to; on light that enters through a narrow gap the learned
+density must cut the error at equal sample count."""
+import numpy as np
+import pytest
+
+from atrc_amd import scenes
+from oracle import py_oracle as po
+
+from conftest import log
+from test_gpu_render_parity import relmse
+
+pytestmark = pytest.mark.gpu
+
+
+def test_guided_cornell_converges_to_the_oracle(gpu_ctx):
+    s, cam, tr = scenes.cornell_box()
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    W = H = 32
+    log("[guided] cornell 32x32: GPU guided (training on) 4096 spp vs FP64 oracle 4096 spp")
+    g = gpu_ctx.render(scenes.render_params(W, H, 4096, cam, seed=99, guide=True, guide_reset=True,
+                                            guide_seed=3, **tr))["image"].astype(np.float64)
+    st = gpu_ctx.stats()
+    log(f"  training samples {st['guide_samples']}, Adam steps {st['guide_steps']}, "
+        f"infer {st['guide_infer_ms']:.1f} ms, train {st['guide_train_ms']:.1f} ms")
+    assert st["guide_steps"] > 0 and st["guide_samples"] > 0
+    r = po.OracleScene(desc, 64).render(scenes.render_params(W, H, 4096, cam, seed=7, **tr), threads=8)["image"]
+    e = relmse(g, r)
+    log(f"  relMSE {e:.3e}")
+    assert np.isfinite(g).all() and e < 3e-3
+
+
+def test_guiding_reduces_error_through_a_door_gap(gpu_ctx):
+    s, cam, tr = scenes.guided_room(n_clutter_tris=50_000, seed=7)
+    s.accel = 1
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    W = H = 64
+    log("[guided] door-gap room 64x64: reference plain 16384 spp")
+    ref = gpu_ctx.render(scenes.render_params(W, H, 16384, cam, seed=1, **tr), aovs=False)["image"].astype(np.float64)
+    plain = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=2, **tr), aovs=False)["image"].astype(np.float64)
+    # train on a warm-up render, then render the measured image (training continues)
+    gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=3, guide=True, guide_reset=True, **tr), aovs=False)
+    guided = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=4, guide=True, **tr),
+                            aovs=False)["image"].astype(np.float64)
+    ep, eg = relmse(plain, ref), relmse(guided, ref)
+    log(f"  relMSE at 256 spp: plain {ep:.3e}, guided {eg:.3e} (ratio {ep / eg:.2f})")
+    assert np.isfinite(guided).all()
+    assert eg < ep
