@@ -400,6 +400,12 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
         return fail(AGX_E_LIMIT, "BVH4 depth " + std::to_string(hs.b4_depth) + " exceeds the traversal stack");
     }
     c->accel = hs.accel;
+    ds.features = hs.media.empty() ? 0u : SF_MEDIA;
+    for (const DevMaterial &m : hs.mats) {
+        if (m.kind == AGX_MAT_DISNEY) ds.features |= SF_DISNEY;
+        if (m.kind == AGX_MAT_GLASS || m.kind == AGX_MAT_MIRROR || m.kind == AGX_MAT_INVISIBLE_SURFACE)
+            ds.features |= SF_DELTA;
+    }
     ds.n_ent = int32_t(hs.ents.size());
     ds.aggregate = hs.aggregate;
     ds.n_lights = int32_t(hs.lights.size());
@@ -607,6 +613,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             const uint32_t *q = c->h_counters + 4 * reads[k].slot;
             st.closest_rays += q[0];
             st.shadow_rays += q[1];
+            if (guide && k + 1 < reads.size()) st.guide_rows += q[2];  // shaded by the next bounce
         }
         // every bounce launched both traversal kernels (an empty queue exits at once)
         st.trace_closest_launches += uint64_t(reads.size() - first_read - 1);

@@ -122,11 +122,22 @@ struct DevScene {
     int32_t lds_stack;     // 1: TLAS depth + BLAS depth fit the LDS stack column
     int32_t accel;         // AGX_ACCEL_*
     int32_t bvh4_large_stack;  // 1: the BVH4 is too deep for BVH4_STACK_SMALL
+    uint32_t features;     // SF_* the scene uses (media, Disney, delta materials)
     const vec4 *b4_nodes;
     const vec4 *b4_prims;
     int32_t lights[MAX_LIGHTS];
     vec4 sky_top, sky_bottom;
     vec4 world;            // (centre.xyz, radius)
+};
+
+// Features a shading kernel is compiled with; scenes without a feature use the
+// variant without its code (fewer registers, higher occupancy).
+enum : uint32_t {
+    SF_MEDIA = 1u,    // participating media
+    SF_DISNEY = 2u,   // Disney BSDF
+    SF_DELTA = 4u,    // glass, mirror, invisible_surface
+    SF_GUIDE = 8u,    // path guiding
+    SF_ALL = 15u,
 };
 
 // per-launch counters (collect_stats)

@@ -1333,8 +1333,9 @@ __device__ BSmp disney_sample(const Bsdf &b, V wo, float su, float sv, float sw,
 }
 
 // BSDF::eval (component/aggregate.h:39-64; glass/mirror are delta: 0)
+template <uint32_t F = SF_ALL>
 __device__ V bsdf_eval(const Bsdf &b, V wi, V wo) {
-    if (b.kind == AGX_MAT_DISNEY) return disney_eval(b, wi, wo);
+    if ((F & SF_DISNEY) && b.kind == AGX_MAT_DISNEY) return disney_eval(b, wi, wo);
     if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return mk(0, 0, 0);
     if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:133-139
         if (!pos_hemi(b.geo, wi) || !pos_hemi(b.geo, wo)) return mk(0, 0, 0);
@@ -1353,8 +1354,9 @@ __device__ V bsdf_eval(const Bsdf &b, V wi, V wo) {
 }
 
 // BSDF::pdf (aggregate.h:218-246)
+template <uint32_t F = SF_ALL>
 __device__ float bsdf_pdf(const Bsdf &b, V wi, V wo) {
-    if (b.kind == AGX_MAT_DISNEY) return disney_pdf(b, wi, wo);
+    if ((F & SF_DISNEY) && b.kind == AGX_MAT_DISNEY) return disney_pdf(b, wi, wo);
     if (b.kind != AGX_MAT_IDEAL_DIFFUSE && b.kind != AGX_MAT_METAL) return 0;
     if (fringe(b, wi) || fringe(b, wo)) {  // bsdf.h:178-185
         if (pos_hemi(b.geo, wi) != pos_hemi(b.geo, wo)) return 0;
@@ -1370,15 +1372,16 @@ __device__ float bsdf_pdf(const Bsdf &b, V wi, V wo) {
 }
 
 // BSDF::sample (aggregate.h:66-139, glass.cpp:52-91, mirror.cpp:32-50)
+template <uint32_t F = SF_ALL>
 __device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, float eps) {
     BSmp r;
     r.dir = mk(0, 0, 0);
     r.f = mk(0, 0, 0);
     r.pdf = 0;
     r.delta = false;
-    if (b.kind == AGX_MAT_DISNEY) return disney_sample(b, wo, su, sv, sw, eps);
+    if ((F & SF_DISNEY) && b.kind == AGX_MAT_DISNEY) return disney_sample(b, wo, su, sv, sw, eps);
     if (b.kind == AGX_MAT_IDEAL_BLACK) return r;
-    if (b.kind == AGX_MAT_INVISIBLE_SURFACE) {  // invisible_surface.cpp:26-31
+    if ((F & SF_DELTA) && b.kind == AGX_MAT_INVISIBLE_SURFACE) {  // invisible_surface.cpp:26-31
         const float cv = fabsf(cosv(b.geo.z, wo));
         const float d = cv < eps ? 1.0f : cv;
         r.dir = mk(-wo.x, -wo.y, -wo.z);
@@ -1387,7 +1390,7 @@ __device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, f
         r.delta = true;
         return r;
     }
-    if (b.kind == AGX_MAT_MIRROR) {
+    if ((F & SF_DELTA) && b.kind == AGX_MAT_MIRROR) {
         const V lo = to_local(b.shd, wo);
         if (lo.z <= 0) return r;
         const V nwo = normalize(lo);
@@ -1396,7 +1399,7 @@ __device__ BSmp bsdf_sample(const Bsdf &b, V wo, float su, float sv, float sw, f
         r.dir = dir; r.f = f * normal_corr(b.geo.z, b.shd.z, dir, eps); r.pdf = 1; r.delta = true;
         return r;
     }
-    if (b.kind == AGX_MAT_GLASS) {
+    if ((F & SF_DELTA) && b.kind == AGX_MAT_GLASS) {
         const V nwo = normalize(to_local(b.shd, wo));
         const V nor = nwo.z > 0 ? mk(0, 0, 1) : mk(0, 0, -1);
         const float fr = dielectric(b.ior, 1.0f, nwo.z);
@@ -1738,6 +1741,7 @@ __device__ __forceinline__ int32_t medium_side(const DevEntity &e, V d, V n) { r
 
 // resolve the MIS-BSDF ray of the previous vertex (direct_illum.cpp:130-192, 194-253;
 // a medium vertex stores |cos| = 1 and pdf = f, so one formula serves both)
+template <uint32_t F>
 __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, uint32_t flags, float eps, Rng &rtr) {
     const float4 ma = pb.mis_a[p], mb = pb.mis_b[p];
     const V coef = mk(ma), bf = mk(mb);
@@ -1762,7 +1766,7 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     const V lrad = dot(sp.geo.z, sp.wr) > 0 ? mk(e.emit) : mk(0, 0, 0);
     if (black(lrad)) return mk(0, 0, 0);
     const int32_t med = int32_t(flags >> PF_MIS_MED_SHIFT) - 1;
-    const V f = medium_tr(s, med, o, sp.pos, rtr) * lrad * bf * absdot;
+    const V f = ((F & SF_MEDIA) ? medium_tr(s, med, o, sp.pos, rtr) : mk(1, 1, 1)) * lrad * bf * absdot;
     if (delta) return coef * (f / pdf);
     const float lpdf = area_light_pdf(e, o, sp.pos, sp.geo.z);
     return coef * (f / (pdf + lpdf));
@@ -1866,6 +1870,7 @@ __global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffer
     }
 }
 
+template <uint32_t F>
 __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
     const uint32_t n = pb.cnt_in[2];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1891,9 +1896,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             const float4 c = pb.nee[size_t(p) * rc.L + l];
             if (c.w != 0.0f && !pb.occluded[size_t(p) * rc.L + l]) direct = direct + mk(c);
         }
-        if (flags & PF_MIS) direct = direct + resolve_mis(s, pb, p, flags, eps, rtr);
+        if (flags & PF_MIS) direct = direct + resolve_mis<F>(s, pb, p, flags, eps, rtr);
         L = L + direct;
-        if (rc.guide_record && p % GUIDE_REC_STRIDE == 0 && !black(direct)) {
+        if ((F & SF_GUIDE) && rc.guide_record && p % GUIDE_REC_STRIDE == 0 && !black(direct)) {
             // light reaching the previous vertex arrives at every earlier recorded
             // vertex along its sampled direction: incident radiance += direct / T
             const int2 st = pb.g_rec_state[p / GUIDE_REC_STRIDE];
@@ -1917,7 +1922,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     flags &= ~PF_CONT;
     // this vertex has made no training record (yet): its direct light will reach every earlier record
-    if (rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE].y = -1;
+    if ((F & SF_GUIDE) && rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE].y = -1;
     V coef = mk(pb.coef[p]);
     int4 dep = pb.depth[p];
     const float4 h4 = pb.hit[2 * p];
@@ -1944,9 +1949,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     bool push_mis = false;
     uint32_t shadow_mask = 0;  // bit l: the shadow ray of light l is traced
     // medium between the ray origin and the hit (path_tracing.cpp:64-119)
-    const int32_t mi = medium_side(e, sp.wr, sp.geo.z);
+    const int32_t mi = (F & SF_MEDIA) ? medium_side(e, sp.wr, sp.geo.z) : -1;
     bool scattered = false;
-    if (mi >= 0) {
+    if ((F & SF_MEDIA) && mi >= 0) {
         const DevMedium &m = s.media[mi];
         if (dep.z < m.max_scat) {
             const MScat ms = medium_sample(s, m, ro, sp.pos, rng, rtr, dep.z > 0, eps);
@@ -2047,11 +2052,12 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                     const float len = length(ls.pos - sp.pos) - eps;
                     if (len > eps) {
                         const V tl = normalize(ls.pos - sp.pos);
-                        const V f = bsdf_eval(b, tl, sp.wr);
+                        const V f = bsdf_eval<F>(b, tl, sp.wr);
                         if (!black(f)) {
-                            const int32_t lm = medium_side(e, tl, sp.geo.z);
-                            const V ff = medium_tr(s, lm, ls.pos, sp.pos, rtr) * rad * f * fabsf(cosv(tl, sp.geo.z));
-                            const float bp = bsdf_pdf(b, tl, sp.wr);
+                            const V trv = (F & SF_MEDIA) ? medium_tr(s, medium_side(e, tl, sp.geo.z), ls.pos, sp.pos, rtr)
+                                                         : mk(1, 1, 1);
+                            const V ff = trv * rad * f * fabsf(cosv(tl, sp.geo.z));
+                            const float bp = bsdf_pdf<F>(b, tl, sp.wr);
                             contrib = coef * (ff / (ls.pdf + bp));
                             valid = true;
                             so = sp.pos; sd = tl; tmax = len;
@@ -2064,10 +2070,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                 const V rad = sky(s, dir);
                 if (!black(rad)) {
                     const V tl = normalize(lpos - sp.pos);
-                    const V f = bsdf_eval(b, tl, sp.wr);
+                    const V f = bsdf_eval<F>(b, tl, sp.wr);
                     if (!black(f)) {
                         const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
-                        const float bp = bsdf_pdf(b, tl, sp.wr);
+                        const float bp = bsdf_pdf<F>(b, tl, sp.wr);
                         contrib = coef * (ff / (SPHERE_PDF + bp));
                         valid = true;
                         // Scene::visible(A, B): [eps, |A-B| - eps] along normalize(B - A)
@@ -2085,21 +2091,21 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         // MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
         {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-            const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+            const BSmp bs = bsdf_sample<F>(b, sp.wr, u0, u1, u2, eps);
             if (!black(bs.f)) {
                 const V dir = normalize(bs.dir);
                 pb.ray_o[2 * p + 1] = f4(eps_offset(sp, dir, eps), 0.0f);
                 pb.ray_d[2 * p + 1] = f4(dir, INF_F);
                 pb.mis_a[p] = f4(coef, fabsf(dot(sp.geo.z, dir)));
                 pb.mis_b[p] = f4(bs.f, bs.pdf);
-                const int32_t mm = medium_side(e, bs.dir, sp.geo.z);
+                const int32_t mm = (F & SF_MEDIA) ? medium_side(e, bs.dir, sp.geo.z) : -1;
                 flags |= PF_MIS | (bs.delta ? PF_MIS_DELTA : 0u) | (uint32_t(mm + 1) << PF_MIS_MED_SHIFT);
                 pending = true;
                 push_mis = true;
             }
         }
         // continuation (path_tracing.cpp:150-169) and Russian roulette of the next depth
-        if (rc.guide && guidable(b.kind)) {
+        if ((F & SF_GUIDE) && rc.guide && guidable(b.kind)) {
             // one-sample mixture of the learned density (probability alpha) and the BSDF
             const float usel = rng.uniform();
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
@@ -2109,10 +2115,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             float pbsdf;
             if (usel < a) {
                 dir = guide_sample(G, u0, u1, u2);
-                f = bsdf_eval(b, dir, sp.wr);
-                pbsdf = bsdf_pdf(b, dir, sp.wr);
+                f = bsdf_eval<F>(b, dir, sp.wr);
+                pbsdf = bsdf_pdf<F>(b, dir, sp.wr);
             } else {
-                const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+                const BSmp bs = bsdf_sample<F>(b, sp.wr, u0, u1, u2, eps);
                 dir = normalize(bs.dir);
                 f = bs.f;
                 pbsdf = bs.pdf;
@@ -2149,7 +2155,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
             }
         } else {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
-            const BSmp bs = bsdf_sample(b, sp.wr, u0, u1, u2, eps);
+            const BSmp bs = bsdf_sample<F>(b, sp.wr, u0, u1, u2, eps);
             if (!black(bs.f) && !(bs.pdf < eps)) {
                 const float abscos = fabsf(cosv(sp.geo.z, bs.dir));
                 coef = coef * (bs.f * abscos / bs.pdf);
@@ -2324,7 +2330,10 @@ void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_
 }
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
-    hipLaunchKernelGGL(k_shade, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    // the lean variant serves scenes of Lambert / GGX / black surfaces in vacuum, unguided
+    const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA)) == 0 && !rc.guide;
+    if (lean) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
 }
 
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,

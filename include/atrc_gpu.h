@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 4
+#define AGX_ABI_VERSION 5
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -242,6 +242,7 @@ typedef struct agx_stats_t {
     double   guide_train_ms;   /* device time of gathering samples + training steps */
     uint64_t guide_samples;    /* training samples used */
     uint64_t guide_steps;      /* Adam steps taken by the context's network so far */
+    uint64_t guide_rows;       /* vertices the network evaluated */
     uint32_t node_bytes, prim_bytes, info_bytes; /* record sizes used for the byte counts */
     uint32_t reserved;
 } agx_stats_t;
