@@ -136,7 +136,7 @@ struct agx_ctx {
     int ensure_guide_wave(size_t n) {
         if (n <= g_wave_cap) return AGX_OK;
         free_guide_wave();
-        const size_t nr = n / GUIDE_REC_STRIDE + 1;
+        const size_t nr = guide_rec_paths(n);
         HIP_TRY(galloc(guide_wave_allocs, &pb.g_feat, n * 8));
         HIP_TRY(galloc(guide_wave_allocs, &pb.g_out, n * GUIDE_ROW));
         HIP_TRY(galloc(guide_wave_allocs, &pb.g_rec, nr * GUIDE_MAX_REC * GUIDE_REC_FLOATS));
@@ -477,6 +477,10 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     int64_t per_pass;
     if (wh <= int64_t(cap)) per_pass = wh * std::min<int64_t>(p->spp, int64_t(cap) / wh);
     else per_pass = int64_t(cap);
+    // a training render takes one sample per pixel per pass, so that the network
+    // learns after every sample layer
+    if (guide_train && wh <= int64_t(cap)) per_pass = wh;
+    rc.guide_rec_shift = guide_rec_shift(size_t(per_pass));
     int r = c->ensure_wave(size_t(per_pass), rc.L, wh);
     if (r != AGX_OK) return r;
     if (guide) {
@@ -530,6 +534,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     for (int64_t base = 0; base < total && !cancelled; base += per_pass) {
         rc.work_base = base;
         rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
+        rc.guide_rec_rot = uint32_t(uint64_t(base / per_pass) * 2654435761ull) & ((1u << rc.guide_rec_shift) - 1u);
         uint32_t *cin = c->counters, *cout = c->counters + 4;
         size_t a, b;
         if ((r = mark(&a))) return r;

@@ -39,17 +39,27 @@ struct PathBuffers {
     // path guiding (RenderConst::guide)
     float *g_feat;       // [N*8] network inputs of the vertices in q_in order
     float *g_out;        // [N*40] decoded mixture of those vertices (8 x (mu.xyz, kappa), 8 weights)
-    float *g_rec;        // [N/GUIDE_REC_STRIDE * GUIDE_MAX_REC * GUIDE_REC_FLOATS] training records
-    int2 *g_rec_state;   // [N/GUIDE_REC_STRIDE] (records made, record of the previous vertex or -1)
+    float *g_rec;        // [guide_rec_paths(N) * GUIDE_MAX_REC * GUIDE_REC_FLOATS] training records
+    int2 *g_rec_state;   // [guide_rec_paths(N)] (records made, record of the previous vertex or -1)
 };
 
-// Path guiding: every GUIDE_REC_STRIDE-th path keeps training records of its
+// Path guiding: one path in 2^guide_rec_shift keeps training records of its
 // first GUIDE_MAX_REC guided vertices: inputs[8], direction[3], mixture pdf,
-// throughput after the vertex[3], incident radiance estimate[3].
-constexpr int GUIDE_REC_STRIDE = 16;
+// throughput after the vertex[3], incident radiance estimate[3].  The stride
+// keeps at most GUIDE_REC_PATHS recording paths per pass (about 4 records
+// each fill a 65536-sample training batch).
+constexpr int GUIDE_REC_PATHS = 16384;
 constexpr int GUIDE_MAX_REC = 16;
 constexpr int GUIDE_REC_FLOATS = 18;
 constexpr int GUIDE_ROW = 40;
+// smallest shift that leaves at most GUIDE_REC_PATHS recording paths among n
+inline uint32_t guide_rec_shift(size_t n) {
+    uint32_t s = 0;
+    while ((n >> s) > size_t(GUIDE_REC_PATHS)) ++s;
+    return s;
+}
+// record slots needed by a pass of n paths at that shift
+inline size_t guide_rec_paths(size_t n) { return n ? ((n - 1) >> guide_rec_shift(n)) + 1 : 0; }
 
 enum : uint32_t {
     PF_CONT = 1u,        // continuation ray 2p is queued
@@ -76,6 +86,8 @@ struct RenderConst {
     int32_t guide;       // 1: guided continuation sampling
     float guide_alpha;   // probability of drawing from the learned density
     int32_t guide_record;  // 1: guided vertices of the recording paths keep training records
+    uint32_t guide_rec_shift;  // recording paths: (p + guide_rec_rot) % 2^shift == 0, record slot p >> shift
+    uint32_t guide_rec_rot;    // varies per pass so that every pixel records in turn
     int32_t pad2;
     float4 cam_pos, cam_x, cam_y, cam_z;  // camera frame
     float film_w, film_h, lens_r, focal;

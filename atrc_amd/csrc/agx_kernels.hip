@@ -1562,7 +1562,9 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         pb.q_out[qi] = uint32_t(p);
         pb.q_closest[ci] = uint32_t(2 * p);
     }
-    if (rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE] = make_int2(0, -1);
+    // the first path of every block of 2^shift clears the block's record slot
+    if (rc.guide_record && (p & ((1u << rc.guide_rec_shift) - 1u)) == 0)
+        pb.g_rec_state[p >> rc.guide_rec_shift] = make_int2(0, -1);
     pb.coef[p] = coef;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
@@ -1779,6 +1781,11 @@ __device__ __forceinline__ void put_shadow(const PathBuffers &pb, size_t slot, V
 }
 
 // ---------------------------------------------------------------- path guiding
+// One path of every block of 2^guide_rec_shift records (the one at offset
+// -guide_rec_rot in the block); its record slot is p >> guide_rec_shift.
+__device__ __forceinline__ bool guide_recording(const RenderConst &rc, uint32_t p) {
+    return ((p + rc.guide_rec_rot) & ((1u << rc.guide_rec_shift) - 1u)) == 0;
+}
 // Network inputs of a surface vertex, each in [0, 1]: position in the scene's
 // bounding sphere, shading normal, and the spherical angles of the direction
 // towards the previous vertex.
@@ -1795,8 +1802,13 @@ __device__ void guide_inputs(const DevScene &s, const Surf &sp, float *x) {
     x[6] = acosf(smin(1.0f, smax(-1.0f, w.z))) / PI_F;
     x[7] = (atan2f(w.y, w.x) + PI_F) / (2 * PI_F);
 }
+// Surfaces whose continuation may come from the learned density.  The one-sample
+// mixture needs the BSDF's pdf to be the density of its sampler; Disney's
+// pdf differs from its sampler (the diffuse pick overwritten, |sin phi_o| in
+// the specular term, material/disney.cpp), so Disney vertices keep plain BSDF
+// sampling and the image stays Atrc's.
 __device__ __forceinline__ bool guidable(int32_t kind) {
-    return kind == AGX_MAT_IDEAL_DIFFUSE || kind == AGX_MAT_METAL || kind == AGX_MAT_DISNEY;
+    return kind == AGX_MAT_IDEAL_DIFFUSE || kind == AGX_MAT_METAL;
 }
 // von Mises-Fisher lobe: density kappa / (2 pi (1 - e^-2kappa)) e^{kappa (mu.d - 1)}
 __device__ __forceinline__ float vmf_pdf(V mu, float k, V d) {
@@ -1851,11 +1863,11 @@ __global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffer
                                                       uint32_t *count, float *wsum) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t r = gid / GUIDE_MAX_REC, j = gid % GUIDE_MAX_REC;
-    const uint32_t n_rec_paths = uint32_t(rc.n_paths + GUIDE_REC_STRIDE - 1) / GUIDE_REC_STRIDE;
+    const uint32_t last_slot = uint32_t(rc.n_paths - 1) >> rc.guide_rec_shift;
     bool ok = false;
     float w = 0;
     const float *rec = nullptr;
-    if (r < n_rec_paths && int32_t(j) < pb.g_rec_state[r].x) {
+    if (r <= last_slot && int32_t(j) < pb.g_rec_state[r].x) {
         rec = pb.g_rec + (size_t(r) * GUIDE_MAX_REC + j) * GUIDE_REC_FLOATS;
         const float pdf = rec[11];
         w = lum3(mk(rec[15], rec[16], rec[17])) / pdf;
@@ -1898,11 +1910,11 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         }
         if (flags & PF_MIS) direct = direct + resolve_mis<F>(s, pb, p, flags, eps, rtr);
         L = L + direct;
-        if ((F & SF_GUIDE) && rc.guide_record && p % GUIDE_REC_STRIDE == 0 && !black(direct)) {
+        if ((F & SF_GUIDE) && rc.guide_record && guide_recording(rc, p) && !black(direct)) {
             // light reaching the previous vertex arrives at every earlier recorded
             // vertex along its sampled direction: incident radiance += direct / T
-            const int2 st = pb.g_rec_state[p / GUIDE_REC_STRIDE];
-            float *recs = pb.g_rec + size_t(p / GUIDE_REC_STRIDE) * GUIDE_MAX_REC * GUIDE_REC_FLOATS;
+            const int2 st = pb.g_rec_state[p >> rc.guide_rec_shift];
+            float *recs = pb.g_rec + size_t(p >> rc.guide_rec_shift) * GUIDE_MAX_REC * GUIDE_REC_FLOATS;
             for (int j = 0; j < st.x; ++j) {
                 if (j == st.y) continue;
                 float *rec = recs + j * GUIDE_REC_FLOATS;
@@ -1922,7 +1934,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     flags &= ~PF_CONT;
     // this vertex has made no training record (yet): its direct light will reach every earlier record
-    if ((F & SF_GUIDE) && rc.guide_record && p % GUIDE_REC_STRIDE == 0) pb.g_rec_state[p / GUIDE_REC_STRIDE].y = -1;
+    if ((F & SF_GUIDE) && rc.guide_record && guide_recording(rc, p)) pb.g_rec_state[p >> rc.guide_rec_shift].y = -1;
     V coef = mk(pb.coef[p]);
     int4 dep = pb.depth[p];
     const float4 h4 = pb.hit[2 * p];
@@ -2137,10 +2149,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         else coef = coef / rc.cont_prob;
                     }
                 }
-                if (cont && rc.guide_record && p % GUIDE_REC_STRIDE == 0) {
-                    int2 st = pb.g_rec_state[p / GUIDE_REC_STRIDE];
+                if (cont && rc.guide_record && guide_recording(rc, p)) {
+                    int2 st = pb.g_rec_state[p >> rc.guide_rec_shift];
                     if (st.x < GUIDE_MAX_REC) {
-                        float *rec = pb.g_rec + (size_t(p / GUIDE_REC_STRIDE) * GUIDE_MAX_REC + st.x) * GUIDE_REC_FLOATS;
+                        float *rec = pb.g_rec + (size_t(p >> rc.guide_rec_shift) * GUIDE_MAX_REC + st.x) * GUIDE_REC_FLOATS;
                         const float *x = pb.g_feat + size_t(tid) * 8;
                         for (int k = 0; k < 8; ++k) rec[k] = x[k];
                         rec[8] = dir.x; rec[9] = dir.y; rec[10] = dir.z;
@@ -2149,7 +2161,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         rec[15] = rec[16] = rec[17] = 0;
                         st.y = st.x;
                         st.x += 1;
-                        pb.g_rec_state[p / GUIDE_REC_STRIDE] = st;
+                        pb.g_rec_state[p >> rc.guide_rec_shift] = st;
                     }
                 }
             }
@@ -2344,7 +2356,7 @@ void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathB
 
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
                          uint32_t *count, float *wsum, hipStream_t st) {
-    const uint64_t n = uint64_t((rc.n_paths + GUIDE_REC_STRIDE - 1) / GUIDE_REC_STRIDE) * GUIDE_MAX_REC;
+    const uint64_t n = rc.n_paths > 0 ? ((uint64_t(rc.n_paths - 1) >> rc.guide_rec_shift) + 1) * GUIDE_MAX_REC : 0;
     if (!n) return;
     hipLaunchKernelGGL(k_guide_gather, dim3(grid_of(n, 256)), dim3(256), 0, st, rc, pb, samples, cap, count, wsum);
 }

@@ -204,7 +204,7 @@ typedef struct agx_render_params {
     int32_t collect_stats;     /* 1: count node visits / triangle tests (slower) */
     int32_t sample_begin;      /* first sample index rendered (progressive rendering), normally 0 */
     /* Path guiding (no counterpart in Atrc; 0 keeps Atrc's estimator). At a
-     * Lambert, GGX-metal or Disney vertex the continuation direction is drawn
+     * Lambert or GGX-metal vertex the continuation direction is drawn
      * from a one-sample mixture: with probability guide_alpha from a learned
      * density (8 von Mises-Fisher lobes predicted by a 4 x 64 MLP from the
      * vertex position, normal and view direction), else from the BSDF; the
@@ -212,7 +212,7 @@ typedef struct agx_render_params {
      * learning across renders. */
     int32_t guide;             /* 1: guided continuation sampling */
     float   guide_alpha;       /* in (0, 1); default 0.5 */
-    int32_t guide_train;       /* 1: train after every pass of this render */
+    int32_t guide_train;       /* 1: train after every pass of this render (passes of one sample per pixel) */
     int32_t guide_batch;       /* training samples per pass, at most 1 << 20 (default 65536) */
     int32_t guide_steps;       /* Adam steps per pass (default 4) */
     float   guide_lr;          /* Adam learning rate (default 1e-2) */
