@@ -558,11 +558,13 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             if (guide) {
                 // the network's mixture for every vertex about to be shaded
                 launch_guide_features(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
+                size_t ef, eg;
+                if ((r = mark(&ef))) return r;
                 launch_guide_infer(c->g_packed, c->g_bias, c->pb.g_feat, c->pb.g_out, cin + 2, uint32_t(rc.n_paths),
                                    c->stream);
-                size_t eg;
                 if ((r = mark(&eg))) return r;
-                timed.push_back({e1, eg, 3});
+                timed.push_back({e1, ef, 2});
+                timed.push_back({ef, eg, 3});
                 e1 = eg;
             }
             launch_trace_shadow(c->ds, c->pb, n_shadow_max, stats, c->dcnt + 1, c->stream);

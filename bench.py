@@ -22,6 +22,17 @@ WRITE_SIZE summary of this workload (profiles/).
 cpu_baseline: the CPU restatement of Atrc `pt` (FP32, Atrc's 32x32 tile
 scheduler, one worker per core) on a band of rows of the same frame at 1 spp,
 on this host.
+
+guided: equal-time relMSE of guided vs plain GPU PT on the door-gap room
+(10 x 10 x 4 Lambert room lit only through a 0.05-wide gap from a second room,
+500k clutter triangles, seed 7).  Plain and guided renders each run for
+``--guided-seconds`` of wall time in chunks of ``--guided-chunk`` spp (the
+guided run starts from a freshly initialised network and trains on every pass,
+so its warm-up counts against it); the reference is plain PT with a different
+seed for ``--guided-ref-factor`` times as long.  relMSE = mean over pixels and
+channels of (x - y)^2 / (y^2 + 1e-2) after scaling both images by 1 / mean(y)
+(the room is dark).  MLP utilisation = 61,440 FLOP per evaluated vertex / the
+inference kernel's device time, against the 2.5 PFLOP/s dense BF16 peak.
 """
 from __future__ import annotations
 
@@ -37,6 +48,10 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s; equal-time relMSE, guided vs plain GPU PT, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
+MLP_FORWARD_FLOP = 2 * (256 * 64 + 3 * 64 * 64 + 64 * 32)   # 61,440 per evaluated vertex
+MLP_TRAIN_FLOP = 3 * MLP_FORWARD_FLOP                       # forward + backward per sample and step
 
 
 def log(msg: str) -> None:
@@ -91,6 +106,81 @@ def cpu_baseline(desc, cam, tr, W, H, target_s: float):
     }
 
 
+def relmse_scaled(x, ref) -> float:
+    import numpy as np
+
+    k = 1.0 / max(float(ref.mean()), 1e-30)
+    return float(np.mean((x * k - ref * k) ** 2 / ((ref * k) ** 2 + 1e-2)))
+
+
+def guided_leg(ctx, args) -> dict:
+    """Equal-time relMSE, guided vs plain, on the door-gap room."""
+    import numpy as np
+
+    from atrc_amd import capi, scenes
+
+    t0 = time.time()
+    s, cam, tr = scenes.guided_room(n_clutter_tris=args.guided_tris, seed=7)
+    s.accel = capi.ACCEL_BVH4
+    desc, keep = s.to_desc()
+    ctx.upload_scene(desc)
+    W = H = args.guided_res
+    log(f"[bench] guided leg: door-gap room, {s.triangle_count()} triangles, {W}x{H} "
+        f"(built + uploaded in {time.time() - t0:.1f} s)")
+
+    def run(seconds: float, seed: int, guide: bool):
+        acc = np.zeros((H, W, 3), np.float64)
+        spp = 0
+        st_sum = {"guide_infer_ms": 0.0, "guide_train_ms": 0.0, "guide_rows": 0, "guide_samples": 0,
+                  "total_ms": 0.0, "rays": 0}
+        t_start = time.perf_counter()
+        while time.perf_counter() - t_start < seconds:
+            p = scenes.render_params(W, H, args.guided_chunk, cam, seed=seed, sample_begin=spp, guide=guide,
+                                     guide_reset=guide and spp == 0, **tr)
+            img = ctx.render(p, aovs=False)["image"]
+            acc += img.astype(np.float64) * args.guided_chunk
+            spp += args.guided_chunk
+            st = ctx.stats()
+            for k in ("guide_infer_ms", "guide_train_ms", "guide_rows", "guide_samples", "total_ms"):
+                st_sum[k] += st[k]
+            st_sum["rays"] += st["closest_rays"] + st["shadow_rays"]
+        wall = time.perf_counter() - t_start
+        return acc / spp, spp, wall, st_sum
+
+    ctx.render(scenes.render_params(W, H, 1, cam, seed=9, **tr), aovs=False)   # warm the kernels
+    ref, ref_spp, ref_wall, _ = run(args.guided_seconds * args.guided_ref_factor, 1001, False)
+    log(f"[bench] guided leg: reference plain {ref_spp} spp in {ref_wall:.1f} s")
+    plain, plain_spp, plain_wall, pst = run(args.guided_seconds, 2002, False)
+    guided, guided_spp, guided_wall, gst = run(args.guided_seconds, 3003, True)
+    ep, eg = relmse_scaled(plain, ref), relmse_scaled(guided, ref)
+    # time-normalised: relMSE ~ 1 / time for an unbiased estimator
+    ratio_t = (ep * plain_wall) / (eg * guided_wall) if eg > 0 else None
+    log(f"[bench] guided leg: plain {plain_spp} spp / {plain_wall:.2f} s relMSE {ep:.4e}; "
+        f"guided {guided_spp} spp / {guided_wall:.2f} s relMSE {eg:.4e}; ratio {ep / eg:.2f}")
+    infer_tflops = gst["guide_rows"] * MLP_FORWARD_FLOP / max(gst["guide_infer_ms"], 1e-9) / 1e9
+    train_tflops = (gst["guide_samples"] * args.guided_steps * MLP_TRAIN_FLOP
+                    / max(gst["guide_train_ms"], 1e-9) / 1e9)
+    return {
+        "workload": (f"door-gap room (10x10x4 Lambert room lit through a 0.05-wide gap, "
+                     f"{s.triangle_count()} triangles), {W}x{H}, max depth {tr['max_depth']}"),
+        "seconds": args.guided_seconds, "chunk_spp": args.guided_chunk,
+        "plain": {"spp": plain_spp, "wall_s": round(plain_wall, 3), "relmse": ep,
+                  "Mrays_per_s": pst["rays"] / plain_wall / 1e6},
+        "guided": {"spp": guided_spp, "wall_s": round(guided_wall, 3), "relmse": eg,
+                   "Mrays_per_s": gst["rays"] / guided_wall / 1e6},
+        "reference": {"spp": ref_spp, "wall_s": round(ref_wall, 3)},
+        "ratio": ep / eg if eg > 0 else None,
+        "ratio_time_normalised": ratio_t,
+        "mlp": {
+            "rows": gst["guide_rows"], "infer_ms": gst["guide_infer_ms"],
+            "infer_tflops": infer_tflops, "infer_frac_bf16_peak": infer_tflops / BF16_PEAK_TFLOPS,
+            "train_samples": gst["guide_samples"], "train_ms": gst["guide_train_ms"],
+            "train_tflops": train_tflops, "train_frac_fp32_peak": train_tflops / FP32_PEAK_TFLOPS,
+            "device_share": (gst["guide_infer_ms"] + gst["guide_train_ms"]) / max(gst["total_ms"], 1e-9),
+        },
+    }
+
+
 def pmc_traffic(kernel: str, workload_key: str):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 --pmc summary of
     this workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy kernel)."""
@@ -114,6 +204,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stats-steps", type=int, default=2)
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided vs plain equal-time leg")
+    ap.add_argument("--guided-res", type=int, default=512)
+    ap.add_argument("--guided-tris", type=int, default=500_000)
+    ap.add_argument("--guided-seconds", type=float, default=4.0)
+    ap.add_argument("--guided-chunk", type=int, default=4)
+    ap.add_argument("--guided-ref-factor", type=float, default=16.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=256,
                     help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()
@@ -188,9 +284,15 @@ def main() -> int:
         "device_ms": {k: st[k] for k in ("total_ms", "trace_closest_ms", "trace_shadow_ms", "shade_ms")},
         "roofline": roofline,
         "guided_vs_plain_relmse_ratio": None,
+        "guided": None,
         "copy_calibration": copy,
         "traversal_stack": "lds" if os.environ.get("AGX_TRAVERSAL_STACK") == "lds" else "private",
     }
+    if not args.no_guided:
+        args.guided_steps = 4   # Adam steps per pass (library default)
+        g = guided_leg(ctx, args)
+        out["guided"] = g
+        out["guided_vs_plain_relmse_ratio"] = g["ratio"]
     ctx.close()
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, FP32) ...")

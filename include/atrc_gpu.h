@@ -235,10 +235,10 @@ typedef struct agx_stats_t {
     double   total_ms;         /* device time of the whole render (HIP events) */
     double   trace_closest_ms; /* device time in the closest-hit kernel */
     double   trace_shadow_ms;  /* device time in the any-hit kernel */
-    double   shade_ms;         /* device time in raygen + shade + film kernels */
+    double   shade_ms;         /* device time in raygen + shade + film kernels (+ guiding features) */
     uint64_t trace_closest_launches;
     uint64_t trace_shadow_launches;
-    double   guide_infer_ms;   /* device time of the guiding network's features + inference */
+    double   guide_infer_ms;   /* device time of the guiding network's inference kernel */
     double   guide_train_ms;   /* device time of gathering samples + training steps */
     uint64_t guide_samples;    /* training samples used */
     uint64_t guide_steps;      /* Adam steps taken by the context's network so far */
