@@ -27,7 +27,7 @@ TEX3D_CONSTANT, TEX3D_GRID = range(2)
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
 MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE, MAT_DISNEY = range(7)
 ENV_NONE, ENV_NATIVE_SKY = 0, 1
-ACCEL_ATRC, ACCEL_BVH4 = 0, 1
+ACCEL_ATRC, ACCEL_BVH4, ACCEL_BVH4_WATERTIGHT = 0, 1, 2
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
 
 F3 = C.c_float * 3
@@ -314,7 +314,7 @@ def debug_scene_summary(desc: agx_scene_desc) -> dict:
 
 
 def debug_bvh4(desc: agx_scene_desc) -> tuple:
-    """(nodes (n, 8, 4) float32, prims (m, 3, 4) float32) of an ACCEL_BVH4 descriptor."""
+    """(nodes (n, 8, 4) float32, prims (m, 3, 4) float32) of an ACCEL_BVH4 or ACCEL_BVH4_WATERTIGHT descriptor."""
     lib = load_library()
     nn, npr = C.c_uint64(0), C.c_uint64(0)
     _check(lib.agx_debug_bvh4(C.byref(desc), C.byref(nn), C.byref(npr), None, None))

@@ -395,7 +395,7 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     ds.b4_prims = b4p;
     // a BVH4 traversal keeps at most 3 pending children per level
     ds.bvh4_large_stack = 3 * hs.b4_depth + 1 > BVH4_STACK_SMALL ? 1 : 0;
-    if (hs.accel == AGX_ACCEL_BVH4 && 3 * hs.b4_depth + 1 > BVH4_STACK_LARGE) {
+    if (hs.accel != AGX_ACCEL_ATRC && 3 * hs.b4_depth + 1 > BVH4_STACK_LARGE) {
         c->free_scene();
         return fail(AGX_E_LIMIT, "BVH4 depth " + std::to_string(hs.b4_depth) + " exceeds the traversal stack");
     }
@@ -494,7 +494,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
 
     const bool stats = p->collect_stats != 0;
     agx_stats_t st{};
-    st.node_bytes = c->accel == AGX_ACCEL_BVH4 ? 16 * BVH4_NODE_VEC4 : 32;
+    st.node_bytes = c->accel != AGX_ACCEL_ATRC ? 16 * BVH4_NODE_VEC4 : 32;
     st.prim_bytes = 48;
     st.info_bytes = 96;
     if (stats) HIP_TRY(hipMemsetAsync(c->dcnt, 0, 2 * sizeof(DevCounters), c->stream));
@@ -730,7 +730,7 @@ int agx_trace_rays(agx_ctx *c, const float *o, const float *d, const float *tmin
         if (v) std::memcpy(v, hv.data(), n * 4);
     }
     agx_stats_t st{};
-    st.node_bytes = c->accel == AGX_ACCEL_BVH4 ? 16 * BVH4_NODE_VEC4 : 32;
+    st.node_bytes = c->accel != AGX_ACCEL_ATRC ? 16 * BVH4_NODE_VEC4 : 32;
     st.prim_bytes = 48;
     st.info_bytes = 96;
     if (any_hit) st.shadow_rays = n; else st.closest_rays = n;
@@ -816,7 +816,8 @@ int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out) 
 int agx_debug_bvh4(const agx_scene_desc *desc, uint64_t *n_nodes, uint64_t *n_prims, float *nodes_out,
                    float *prims_out) {
     if (!desc || !n_nodes || !n_prims) return fail(AGX_E_INVALID_ARG, "null argument");
-    if (desc->accel != AGX_ACCEL_BVH4) return fail(AGX_E_INVALID_ARG, "descriptor does not ask for a BVH4");
+    if (desc->accel != AGX_ACCEL_BVH4 && desc->accel != AGX_ACCEL_BVH4_WATERTIGHT)
+        return fail(AGX_E_INVALID_ARG, "descriptor does not ask for a BVH4");
     HostScene hs;
     std::string err;
     const int rc = build_host_scene(*desc, &hs, &err);

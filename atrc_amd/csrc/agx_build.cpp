@@ -35,6 +35,7 @@ inline F3 unit(F3 a) { return divs(a, len3(a)); }
 inline F3 neg(F3 a) { return {-a.x, -a.y, -a.z}; }
 inline vec4 v4(F3 a, float w) { return vec4{a.x, a.y, a.z, w}; }
 inline float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+inline uint32_t f_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 
 struct Box {
     F3 lo{std::numeric_limits<float>::max(), std::numeric_limits<float>::max(), std::numeric_limits<float>::max()};
@@ -594,7 +595,8 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
     S.world = vec4{centre.x, centre.y, centre.z, 1.1f * diag / 2};
 
     S.accel = desc.accel;
-    if (desc.accel == AGX_ACCEL_BVH4) {
+    if (desc.accel == AGX_ACCEL_BVH4 || desc.accel == AGX_ACCEL_BVH4_WATERTIGHT) {
+        const bool watertight = desc.accel == AGX_ACCEL_BVH4_WATERTIGHT;
         // one reference per mesh triangle (in the mesh BVH's slot order) and per analytic entity
         std::vector<PrimRef> refs;
         std::vector<uint32_t> ref_ent, ref_slot;
@@ -641,7 +643,15 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
             const uint32_t r = b4.order[k];
             vec4 *o = &S.b4_prims[3 * k];
             const uint32_t slot = ref_slot[r];
-            if (slot != BVH4_ANALYTIC) {
+            if (slot != BVH4_ANALYTIC && watertight) {
+                // the mesh's own vertices, so that triangles sharing an edge see
+                // bit-identical endpoints
+                const uint32_t id = f_bits(S.blas_prims[3 * size_t(slot)].w);
+                const agx_vertex *t = desc.entities[ref_ent[r]].tris + 3 * size_t(id);
+                o[0] = vec4{t[0].pos[0], t[0].pos[1], t[0].pos[2], 0};
+                o[1] = vec4{t[1].pos[0], t[1].pos[1], t[1].pos[2], 0};
+                o[2] = vec4{t[2].pos[0], t[2].pos[1], t[2].pos[2], 0};
+            } else if (slot != BVH4_ANALYTIC) {
                 o[0] = S.blas_prims[3 * size_t(slot)];
                 o[1] = S.blas_prims[3 * size_t(slot) + 1];
                 o[2] = S.blas_prims[3 * size_t(slot) + 2];

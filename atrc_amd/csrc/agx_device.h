@@ -31,7 +31,9 @@ constexpr int MAX_LIGHTS = 16;
 // Child code: BVH4_EMPTY, an interior node index (bit 31 clear), or a leaf
 // BVH4_LEAF | (count - 1) << 27 | first primitive.  A primitive is 3 vec4:
 //   (A, entity), (B - A, slot), (C - A, 0); slot = index into blas_prims/blas_info
-//   of a mesh triangle, or BVH4_ANALYTIC for a whole analytic entity.
+//   of a mesh triangle, or BVH4_ANALYTIC for a whole analytic entity.  In the
+//   watertight mode a mesh triangle is (A, entity), (B, slot), (C, 0) with the
+//   mesh's own vertex coordinates.
 constexpr int BVH4_NODE_VEC4 = 8;
 constexpr uint32_t BVH4_EMPTY = 0xffffffffu;
 constexpr uint32_t BVH4_LEAF = 0x80000000u;

@@ -525,6 +525,61 @@ __device__ bool scene_any(const DevScene &s, V o, V d, float tmin, float tmax, C
     return false;
 }
 
+// ---------------------------------------------------------------- watertight triangle test
+// Woop, Benthin and Wald, "Watertight Ray/Triangle Intersection", JCGT 2(1),
+// 2013.  The ray is sheared so that it runs along +z through the origin;
+// each vertex is projected with the same per-ray arithmetic, so an edge shared
+// by two triangles gives both the same edge function with opposite sign (the
+// products commute exactly and no FMA contraction is allowed), and the
+// decision is consistent.  Edge functions that come out exactly zero are
+// recomputed in FP64 from the same projected coordinates.
+struct WtRay {
+    V o;
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+__device__ __forceinline__ float vcomp(V a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+__device__ __forceinline__ WtRay wt_ray(V o, V d) {
+    WtRay r;
+    r.o = o;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    r.kx = r.kz == 2 ? 0 : r.kz + 1;
+    r.ky = r.kx == 2 ? 0 : r.kx + 1;
+    const float dz = vcomp(d, r.kz);
+    if (dz < 0) { const int t = r.kx; r.kx = r.ky; r.ky = t; }  // keep the winding
+    r.sx = vcomp(d, r.kx) / dz;
+    r.sy = vcomp(d, r.ky) / dz;
+    r.sz = 1.0f / dz;
+    return r;
+}
+// hit: t and the barycentrics of B (u) and C (v), as the Moller-Trumbore test reports them
+__device__ __forceinline__ bool wt_test(const WtRay &r, V A, V B, V C, float tmin, float tmax, float *t, float *u,
+                                        float *v) {
+    const V a = A - r.o, b = B - r.o, c = C - r.o;
+    const float az = vcomp(a, r.kz), bz = vcomp(b, r.kz), cz = vcomp(c, r.kz);
+    const float ax = vcomp(a, r.kx) - r.sx * az, ay = vcomp(a, r.ky) - r.sy * az;
+    const float bx = vcomp(b, r.kx) - r.sx * bz, by = vcomp(b, r.ky) - r.sy * bz;
+    const float cx = vcomp(c, r.kx) - r.sx * cz, cy = vcomp(c, r.ky) - r.sy * cz;
+    float U = cx * by - cy * bx;
+    float W0 = ax * cy - ay * cx;  // weight of B
+    float W1 = bx * ay - by * ax;  // weight of C
+    if (U == 0.0f || W0 == 0.0f || W1 == 0.0f) {
+        U = float(double(cx) * double(by) - double(cy) * double(bx));
+        W0 = float(double(ax) * double(cy) - double(ay) * double(cx));
+        W1 = float(double(bx) * double(ay) - double(by) * double(ax));
+    }
+    if ((U < 0 || W0 < 0 || W1 < 0) && (U > 0 || W0 > 0 || W1 > 0)) return false;
+    const float det = U + W0 + W1;
+    if (det == 0.0f) return false;
+    const float T = U * (r.sz * az) + W0 * (r.sz * bz) + W1 * (r.sz * cz);
+    const float inv = 1.0f / det;
+    const float tt = T * inv;
+    if (!(tmin <= tt && tt <= tmax)) return false;
+    *t = tt; *u = W0 * inv; *v = W1 * inv;
+    return true;
+}
+
 // ---------------------------------------------------------------- BVH4 (production mode)
 // Slab test of one child box with Atrc's arithmetic, NaN axes ignored (fminf /
 // fmaxf) and the exit distance widened by 2^-21 relative, so that a ray meeting
@@ -560,9 +615,10 @@ __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
 
 // Closest hit over the BVH4: children visited near-first, pending children
 // kept with their entry distance and skipped once a closer hit is known.
-template <bool STATS, int CAP>
+template <bool STATS, int CAP, bool WT>
 __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
     const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
     const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
     uint32_t stk_k[CAP];
@@ -594,7 +650,9 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                     if (STATS) c.tests++;
                     const float4 p2 = P[3 * size_t(i) + 2];
                     float tt, uu, vv;
-                    if (mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv)) {
+                    const bool hit_tri = WT ? wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)
+                                            : mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv);
+                    if (hit_tri) {
                         tmax = tt;
                         h->t = tt; h->u = uu; h->v = vv; h->ent = ent; h->prim = int32_t(slot);
                         hit = true;
@@ -613,9 +671,10 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
     }
 }
 
-template <bool STATS, int CAP>
+template <bool STATS, int CAP, bool WT>
 __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+    const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
     const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
     const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
     uint32_t stk_k[CAP];
@@ -642,7 +701,12 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
                 if (slot != BVH4_ANALYTIC) {
                     if (STATS) c.tests++;
                     const float4 p2 = P[3 * size_t(i) + 2];
-                    if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) return true;
+                    if (WT) {
+                        float tt, uu, vv;
+                        if (wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)) return true;
+                    } else if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) {
+                        return true;
+                    }
                 } else if (analytic_any<STATS>(s.ents[__float_as_int(p0.w)], o, d, tmin, tmax, c)) {
                     return true;
                 }
@@ -654,8 +718,9 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
 }
 
 // Traversal variants: Atrc's structures with the private or the LDS stack, or
-// the BVH4 with the small or the large private stack.
-enum { TM_ATRC_PRIV = 0, TM_ATRC_LDS = 1, TM_B4_SMALL = 2, TM_B4_LARGE = 3 };
+// the BVH4 with the small or the large private stack, with Atrc's triangle
+// test or the watertight one.
+enum { TM_ATRC_PRIV = 0, TM_ATRC_LDS = 1, TM_B4_SMALL = 2, TM_B4_LARGE = 3, TM_B4W_SMALL = 4, TM_B4W_LARGE = 5 };
 
 template <bool STATS, int TM>
 __device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h,
@@ -666,10 +731,10 @@ __device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float
     } else if constexpr (TM == TM_ATRC_PRIV) {
         PrivStack stk;
         return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
-    } else if constexpr (TM == TM_B4_SMALL) {
-        return b4_closest<STATS, BVH4_STACK_SMALL>(s, o, d, tmin, tmax, h, c);
+    } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
+        return b4_closest<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, h, c);
     } else {
-        return b4_closest<STATS, BVH4_STACK_LARGE>(s, o, d, tmin, tmax, h, c);
+        return b4_closest<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, h, c);
     }
 }
 
@@ -682,10 +747,10 @@ __device__ __forceinline__ bool query_any(const DevScene &s, V o, V d, float tmi
     } else if constexpr (TM == TM_ATRC_PRIV) {
         PrivStack stk;
         return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
-    } else if constexpr (TM == TM_B4_SMALL) {
-        return b4_any<STATS, BVH4_STACK_SMALL>(s, o, d, tmin, tmax, c);
+    } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
+        return b4_any<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, c);
     } else {
-        return b4_any<STATS, BVH4_STACK_LARGE>(s, o, d, tmin, tmax, c);
+        return b4_any<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, c);
     }
 }
 
@@ -2323,6 +2388,9 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
         if (s.accel == AGX_ACCEL_BVH4) {                                                          \
             if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4_LARGE, grid, st, __VA_ARGS__);         \
             else AGX_LAUNCH_TM(K, TM_B4_SMALL, grid, st, __VA_ARGS__);                            \
+        } else if (s.accel == AGX_ACCEL_BVH4_WATERTIGHT) {                                        \
+            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, grid, st, __VA_ARGS__);        \
+            else AGX_LAUNCH_TM(K, TM_B4W_SMALL, grid, st, __VA_ARGS__);                           \
         } else if (s.lds_stack) {                                                                 \
             AGX_LAUNCH_TM(K, TM_ATRC_LDS, grid, st, __VA_ARGS__);                                 \
         } else {                                                                                  \

@@ -53,7 +53,9 @@ public:
         if (!scene.camera) throw ObjectConstructionException("scene has no camera");
         int rc;
         if (!ctx_ && (rc = agx_create(params_.device, &ctx_)) != AGX_OK) throw std::runtime_error(agx_error(rc));
-        if ((rc = agx_upload_scene(ctx_, &scene.desc())) != AGX_OK) throw std::runtime_error(agx_error(rc));
+        agx_scene_desc sd = scene.desc();
+        sd.accel = params_.accel;
+        if ((rc = agx_upload_scene(ctx_, &sd)) != AGX_OK) throw std::runtime_error(agx_error(rc));
         agx_render_params rp{};
         rp.width = filter.width;
         rp.height = filter.height;
@@ -67,6 +69,16 @@ public:
         rp.filter_radius = filter.filter->radius;
         rp.seed = params_.seed;
         rp.camera = scene.camera->desc;
+        if (params_.guide) {
+            rp.guide = 1;
+            rp.guide_alpha = params_.guide_alpha;
+            rp.guide_train = 1;
+            rp.guide_batch = params_.guide_batch;
+            rp.guide_steps = params_.guide_steps;
+            rp.guide_lr = params_.guide_lr;
+            rp.guide_reset = 1;
+            rp.guide_seed = params_.guide_seed;
+        }
         const size_t wh = size_t(filter.width) * size_t(filter.height);
         std::vector<float> rgb(wh * 3), alb(wh * 3), nor(wh * 3), den(wh);
         reporter.begin();
@@ -252,6 +264,17 @@ void register_renderers(CreatingContext &ctx) {
         r.specular_depth = p.child_int_or("specular_depth", 20);
         r.device = p.child_int_or("device", 0);
         r.seed = uint64_t(p.child_int_or("seed", 42));
+        const std::string accel = p.child_str_or("accel", "atrc");
+        if (accel == "atrc") r.accel = AGX_ACCEL_ATRC;
+        else if (accel == "bvh4") r.accel = AGX_ACCEL_BVH4;
+        else if (accel == "bvh4_watertight") r.accel = AGX_ACCEL_BVH4_WATERTIGHT;
+        else throw ObjectConstructionException("unknown accel: " + accel);
+        r.guide = p.child_int_or("guide", 0) != 0;
+        r.guide_alpha = p.child_real_or("guide_alpha", 0.5f);
+        r.guide_batch = p.child_int_or("guide_batch", 65536);
+        r.guide_steps = p.child_int_or("guide_steps", 4);
+        r.guide_lr = p.child_real_or("guide_lr", 1e-2f);
+        r.guide_seed = uint32_t(p.child_int_or("guide_seed", 0));
         return create_gpu_pt_renderer(r);
     });
     ctx.reporter.add_creator("stdout", [](const ConfigGroup &, CreatingContext &) {

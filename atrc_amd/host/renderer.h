@@ -68,6 +68,14 @@ struct PTRendererParams {
     int specular_depth = 20;
     int device = 0;          // HIP device ordinal (gpu_pt only)
     uint64_t seed = 42;      // PCG32 stream seed (gpu_pt only)
+    // gpu_pt only (no counterpart in Atrc's "pt"):
+    int accel = 0;           // AGX_ACCEL_*: "atrc" (default), "bvh4", "bvh4_watertight"
+    bool guide = false;      // path guiding (agx_render_params.guide*)
+    float guide_alpha = 0.5f;
+    int guide_batch = 65536;
+    int guide_steps = 4;
+    float guide_lr = 1e-2f;
+    uint32_t guide_seed = 0;
 };
 
 std::shared_ptr<Renderer> create_gpu_pt_renderer(const PTRendererParams &params);

@@ -72,8 +72,14 @@ enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
  *     primitive tests to Atrc's CPU traversal.
  *   AGX_ACCEL_BVH4: one 4-wide SAH BVH over all primitives, near-first
  *     traversal: identical hits (same primitive, t, u, v) except which of two
- *     primitives at exactly equal t wins; far fewer node visits. */
-enum { AGX_ACCEL_ATRC = 0, AGX_ACCEL_BVH4 = 1 };
+ *     primitives at exactly equal t wins; far fewer node visits.
+ *   AGX_ACCEL_BVH4_WATERTIGHT: the same BVH4 with the watertight ray/triangle
+ *     test of Woop, Benthin and Wald (JCGT 2(1), 2013) on mesh triangles, over
+ *     the meshes' own vertex coordinates: a ray through an edge or vertex
+ *     shared by two mesh triangles always hits one of them.  Away from edges
+ *     the hits are Atrc's up to rounding of t, u, v; analytic entities keep
+ *     Atrc's tests. */
+enum { AGX_ACCEL_ATRC = 0, AGX_ACCEL_BVH4 = 1, AGX_ACCEL_BVH4_WATERTIGHT = 2 };
 enum { AGX_MEDIUM_VOID = 0, AGX_MEDIUM_HOMOGENEOUS = 1, AGX_MEDIUM_HETEROGENEOUS = 2 };
 enum { AGX_TEX3D_CONSTANT = 0, AGX_TEX3D_GRID = 1 };
 enum { AGX_WRAP_CLAMP = 0, AGX_WRAP_REPEAT = 1, AGX_WRAP_MIRROR = 2 };

@@ -345,6 +345,55 @@ inline bool mt_closest(const Ray<T> &r, const V3<T> &A, const V3<T> &B_A, const 
     return true;
 }
 
+// Watertight ray/triangle test (Woop, Benthin and Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013), the GPU path's production
+// intersection mode (not an Atrc function).  The ray is sheared onto +z
+// through the origin; the three 2D edge functions of the projected vertices
+// decide the hit, and an edge function that is exactly zero is recomputed in
+// double from the same projected coordinates.  Returns t and the barycentrics
+// of B and C (the Moller-Trumbore u, v).
+template<class T> struct ShearedRay {
+    V3<T> o;
+    int kx = 0, ky = 1, kz = 2;
+    T sx = 0, sy = 0, sz = 1;
+    explicit ShearedRay(const Ray<T> &r) : o(r.o) {
+        const T ax = std::abs(r.d.x), ay = std::abs(r.d.y), az = std::abs(r.d.z);
+        kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+        kx = (kz + 1) % 3;
+        ky = (kx + 1) % 3;
+        if (r.d[kz] < 0) std::swap(kx, ky);
+        sx = r.d[kx] / r.d[kz];
+        sy = r.d[ky] / r.d[kz];
+        sz = T(1) / r.d[kz];
+    }
+};
+template<class T>
+inline bool watertight_closest(const ShearedRay<T> &w, T tmin, T tmax, const V3<T> &A, const V3<T> &B,
+                               const V3<T> &C, T *t_out, T *u_out, T *v_out) {
+    const V3<T> a = A - w.o, b = B - w.o, c = C - w.o;
+    const T ax = a[w.kx] - w.sx * a[w.kz], ay = a[w.ky] - w.sy * a[w.kz];
+    const T bx = b[w.kx] - w.sx * b[w.kz], by = b[w.ky] - w.sy * b[w.kz];
+    const T cx = c[w.kx] - w.sx * c[w.kz], cy = c[w.ky] - w.sy * c[w.kz];
+    T ea = cx * by - cy * bx;  // opposite A: weight of A
+    T eb = ax * cy - ay * cx;  // weight of B
+    T ec = bx * ay - by * ax;  // weight of C
+    if (ea == 0 || eb == 0 || ec == 0) {
+        ea = T(double(cx) * double(by) - double(cy) * double(bx));
+        eb = T(double(ax) * double(cy) - double(ay) * double(cx));
+        ec = T(double(bx) * double(ay) - double(by) * double(ax));
+    }
+    const bool neg = ea < 0 || eb < 0 || ec < 0, pos = ea > 0 || eb > 0 || ec > 0;
+    if (neg && pos) return false;
+    const T det = ea + eb + ec;
+    if (det == 0) return false;
+    const T tsc = ea * (w.sz * a[w.kz]) + eb * (w.sz * b[w.kz]) + ec * (w.sz * c[w.kz]);
+    const T inv = T(1) / det;
+    const T t = tsc * inv;
+    if (!(tmin <= t && t <= tmax)) return false;
+    *t_out = t; *u_out = eb * inv; *v_out = ec * inv;
+    return true;
+}
+
 template<class T> inline T triangle_area(const V3<T> &B_A, const V3<T> &C_A) { return cross(B_A, C_A).length() / 2; }
 
 // dpdu_as_ex (triangle_aux.h:76-88)

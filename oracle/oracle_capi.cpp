@@ -109,6 +109,37 @@ int oracle_render(void *p, const agx_render_params *rp, int threads, double *rgb
 }
 
 // PCG32 uniform stream of (seed, pixel, sample): n floats.
+// Closest hit over a triangle soup (9 floats per triangle: A, B, C), brute
+// force, with the watertight test (mode 1) or Atrc's Moller-Trumbore test on
+// (A, B - A, C - A) (mode 0).  A later triangle at an equal t replaces the
+// earlier one, as in Atrc's leaf loops.
+int oracle_triangle_soup_closest(const float *tris, uint32_t n_tris, const float *o, const float *d,
+                                 const float *tmin, const float *tmax, uint32_t n, int mode, int32_t *prim,
+                                 float *t, float *u, float *v) {
+    using V = oracle::V3<float>;
+    for (uint32_t i = 0; i < n; ++i) {
+        const oracle::Ray<float> r(V(o[3 * i], o[3 * i + 1], o[3 * i + 2]), V(d[3 * i], d[3 * i + 1], d[3 * i + 2]),
+                                   tmin[i], tmax[i]);
+        const oracle::ShearedRay<float> w(r);
+        float best = tmax[i];
+        prim[i] = -1; t[i] = -1; u[i] = 0; v[i] = 0;
+        for (uint32_t k = 0; k < n_tris; ++k) {
+            const float *q = tris + 9 * size_t(k);
+            const V A(q[0], q[1], q[2]), B(q[3], q[4], q[5]), C(q[6], q[7], q[8]);
+            float tt, uu, vv;
+            bool hit;
+            if (mode == 1) {
+                hit = oracle::watertight_closest(w, tmin[i], best, A, B, C, &tt, &uu, &vv);
+            } else {
+                const oracle::Ray<float> rr(r.o, r.d, tmin[i], best);
+                hit = oracle::mt_closest(rr, A, B - A, C - A, &tt, &uu, &vv);
+            }
+            if (hit) { best = tt; prim[i] = int32_t(k); t[i] = tt; u[i] = uu; v[i] = vv; }
+        }
+    }
+    return 0;
+}
+
 void oracle_pcg_stream(uint64_t seed, uint64_t pixel, uint64_t sample, uint32_t n, float *out) {
     oracle::Pcg32 r;
     r.seed_path(seed, pixel, sample);

@@ -34,6 +34,9 @@ def load() -> C.CDLL:
                                       P(C.c_uint64)]
     lib.oracle_render.argtypes = [C.c_void_p, C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                   P(C.c_double), P(C.c_uint64), C.c_int, C.c_int, P(C.c_double)]
+    lib.oracle_triangle_soup_closest.argtypes = [P(C.c_float), C.c_uint32, P(C.c_float), P(C.c_float),
+                                                 P(C.c_float), P(C.c_float), C.c_uint32, C.c_int, P(C.c_int32),
+                                                 P(C.c_float), P(C.c_float), P(C.c_float)]
     lib.oracle_pcg_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_float)]
     lib.oracle_pcg_raw.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, P(C.c_uint32)]
     lib.oracle_distribution.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, P(C.c_double)]
@@ -131,6 +134,24 @@ class OracleScene:
             raise RuntimeError(self.lib.oracle_last_error().decode())
         return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den, "seconds": secs.value,
                 "counters": dict(zip(COUNTER_NAMES, cnt.tolist())), "threads": th}
+
+
+def triangle_soup_closest(tris, o, d, tmin, tmax, watertight: bool):
+    """Brute-force closest hit over triangles (n, 3, 3) with the watertight test
+    or Atrc's Moller-Trumbore test.  Returns (prim, t, u, v); prim = -1 on a miss."""
+    lib = load()
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    o = np.ascontiguousarray(o, np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+    n = len(o)
+    tmin = np.ascontiguousarray(np.broadcast_to(np.asarray(tmin, np.float32), (n,)))
+    tmax = np.ascontiguousarray(np.broadcast_to(np.asarray(tmax, np.float32), (n,)))
+    prim = np.empty(n, np.int32)
+    t, u, v = (np.empty(n, np.float32) for _ in range(3))
+    lib.oracle_triangle_soup_closest(_f(tris), len(tris), _f(o), _f(d), _f(tmin), _f(tmax), n,
+                                     1 if watertight else 0, prim.ctypes.data_as(C.POINTER(C.c_int32)),
+                                     _f(t), _f(u), _f(v))
+    return prim, t, u, v
 
 
 def pcg_stream(seed: int, pixel: int, sample: int, n: int) -> np.ndarray:
