@@ -114,3 +114,29 @@ def test_unknown_accel_rejected():
     desc, keep = s.to_desc()
     with pytest.raises(capi.AgxError):
         capi.debug_scene_summary(desc)
+
+
+def test_watertight_records_hold_the_mesh_vertices():
+    """AGX_ACCEL_BVH4_WATERTIGHT: the same tree, mesh triangles as (A, B, C) with
+    the mesh's own vertex coordinates (shared edges stay bit-identical)."""
+    s, _, _ = scenes.procedural_spheres(n_spheres=6, subdiv=2, seed=4, extent=10.0)
+    s.accel = capi.ACCEL_BVH4
+    n4, p4 = capi.debug_bvh4(s.to_desc()[0])
+    s.accel = capi.ACCEL_BVH4_WATERTIGHT
+    desc, keep = s.to_desc()
+    nw, pw = capi.debug_bvh4(desc)
+    np.testing.assert_array_equal(n4, nw)
+    slot4, slotw = p4[:, 1, 3].view(np.uint32), pw[:, 1, 3].view(np.uint32)
+    np.testing.assert_array_equal(slot4, slotw)
+    np.testing.assert_array_equal(p4[:, 0, 3].view(np.int32), pw[:, 0, 3].view(np.int32))
+    mesh_ents = [i for i, e in enumerate(s.entities) if e.tris is not None]
+    ent = pw[:, 0, 3].view(np.int32)
+    for i in mesh_ents:
+        tris = s.entities[i].tris
+        ref_nodes, ref_prims, _ = capi.debug_triangle_bvh(tris)
+        orig = ref_prims.reshape(-1, 3, 4)[:, 0, 3].view(np.int32)   # original triangle id per slot
+        sel = np.where((slotw != ANALYTIC) & (ent == i))[0]
+        base = int(slotw[sel].min())
+        ids = orig[slotw[sel].astype(np.int64) - base]
+        verts = np.asarray(tris, np.float32)[:, :3].reshape(-1, 3, 3)
+        np.testing.assert_array_equal(pw[sel, :, :3], verts[ids])
