@@ -133,7 +133,7 @@ class agx_debug_summary(C.Structure):
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
-    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train",
+    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays",
 ]
 GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
 
@@ -170,6 +170,8 @@ def load_library() -> C.CDLL:
     lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
     lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
+    lib.agx_debug_replay_rays.argtypes = [C.c_void_p, P(agx_render_params), C.c_uint32, C.c_int, P(C.c_uint64),
+                                          P(C.c_double)]
     lib.agx_debug_bvh4.argtypes = [P(agx_scene_desc), P(C.c_uint64), P(C.c_uint64), P(C.c_float), P(C.c_float)]
     lib.agx_debug_guide_infer.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32, P(C.c_float)]
     lib.agx_debug_guide_train.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), P(C.c_float), P(C.c_float),
@@ -284,6 +286,15 @@ class Context:
         ms = C.c_double()
         _check(self.lib.agx_debug_copy_kernel(self.h, nbytes, C.byref(ms)))
         return ms.value
+
+    def replay_rays(self, params, max_rays: int = 1 << 26, reps: int = 3) -> dict:
+        """Renders `params` capturing every ray query, then replays the captured
+        closest-hit and any-hit sets with the traversal kernel alone."""
+        n = (C.c_uint64 * 2)()
+        ms = (C.c_double * 4)()
+        _check(self.lib.agx_debug_replay_rays(self.h, C.byref(params), max_rays, reps, n, ms))
+        return {"closest_rays": n[0], "shadow_rays": n[1], "replay_closest_ms": ms[0], "replay_shadow_ms": ms[1],
+                "render_closest_ms": ms[2], "render_shadow_ms": ms[3]}
 
     def stats(self) -> dict:
         s = agx_stats_t()

@@ -70,6 +70,10 @@ struct agx_ctx {
     float *h_resolve = nullptr;      // pinned staging of the planes the caller asked for
     size_t resolve_cap = 0;          // pixels both buffers hold
     static constexpr int RING = 64;
+    // ray capture of agx_debug_replay_rays (null outside it): [0] closest-hit, [1] any-hit rays
+    float4 *capture_o[2] = {nullptr, nullptr}, *capture_d[2] = {nullptr, nullptr};
+    uint32_t *capture_count = nullptr;  // device: 2 counters
+    uint32_t capture_cap = 0;
     // path guiding: network state (persists across renders) and its wave buffers
     static constexpr int GUIDE_TRAIN_BLOCKS = 256;
     bool guide_ready = false;
@@ -555,6 +559,12 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             if ((r = mark(&e0))) return r;
             launch_trace_closest(c->ds, c->pb, n_closest_max, stats, c->dcnt, c->stream);
             if ((r = mark(&e1))) return r;
+            if (c->capture_count) {
+                launch_capture_rays(c->pb, 0, c->capture_o[0], c->capture_d[0], c->capture_count, c->capture_cap,
+                                    n_closest_max, c->stream);
+                launch_capture_rays(c->pb, 1, c->capture_o[1], c->capture_d[1], c->capture_count + 1,
+                                    c->capture_cap, n_shadow_max, c->stream);
+            }
             if (guide) {
                 // the network's mixture for every vertex about to be shaded
                 launch_guide_features(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
@@ -781,6 +791,55 @@ int agx_debug_copy_kernel(agx_ctx *c, uint64_t bytes, double *ms) {
     if (e == hipSuccess && ms) *ms = elapsed(c->ev[0], c->ev[1]) / REPS;
     (void)hipFree(buf);
     if (e != hipSuccess) return fail(AGX_E_HIP, std::string("copy kernel: ") + hipGetErrorString(e));
+    return AGX_OK;
+}
+
+int agx_debug_replay_rays(agx_ctx *c, const agx_render_params *p, uint32_t max_rays, int reps, uint64_t out[2],
+                          double out_ms[4]) {
+    if (!c || !p || !out || !out_ms || max_rays == 0 || reps < 1) return fail(AGX_E_INVALID_ARG, "invalid argument");
+    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
+    HIP_TRY(hipSetDevice(c->device));
+    DevBuf rays, cnt, res;
+    HIP_TRY(hipMalloc(&rays.p, size_t(max_rays) * 4 * sizeof(float4)));
+    HIP_TRY(hipMalloc(&cnt.p, 2 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&res.p, size_t(max_rays) * 5 * sizeof(float)));
+    HIP_TRY(hipMemsetAsync(cnt.p, 0, 2 * sizeof(uint32_t), c->stream));
+    float4 *base = rays.get<float4>();
+    c->capture_o[0] = base;
+    c->capture_d[0] = base + max_rays;
+    c->capture_o[1] = base + 2 * size_t(max_rays);
+    c->capture_d[1] = base + 3 * size_t(max_rays);
+    c->capture_count = cnt.get<uint32_t>();
+    c->capture_cap = max_rays;
+    const int r = agx_render(c, p, nullptr, nullptr, nullptr, nullptr, nullptr);
+    c->capture_count = nullptr;
+    c->capture_o[0] = c->capture_o[1] = c->capture_d[0] = c->capture_d[1] = nullptr;
+    c->capture_cap = 0;
+    if (r != AGX_OK) return r;
+    const agx_stats_t st = c->stats;
+    uint32_t n[2];
+    HIP_TRY(hipMemcpy(n, cnt.p, sizeof(n), hipMemcpyDeviceToHost));
+    if (n[0] > max_rays || n[1] > max_rays)
+        return fail(AGX_E_LIMIT, "more rays than max_rays (" + std::to_string(std::max(n[0], n[1])) + ")");
+    int32_t *ent = res.get<int32_t>(), *prim = ent + max_rays;
+    float *t = reinterpret_cast<float *>(prim + max_rays), *u = t + max_rays, *v = u + max_rays;
+    for (int which = 0; which < 2; ++which) {
+        double ms = 0;
+        for (int k = 0; k <= reps; ++k) {
+            HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+            launch_trace_rays(c->ds, base + 2 * size_t(which) * max_rays, base + (2 * size_t(which) + 1) * max_rays,
+                              n[which], which, ent, prim, t, u, v, nullptr, c->stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+            HIP_TRY(hipEventSynchronize(c->ev[1]));
+            if (k > 0) ms += elapsed(c->ev[0], c->ev[1]);
+        }
+        out[which] = n[which];
+        out_ms[which] = ms / reps;
+    }
+    out_ms[2] = st.trace_closest_ms;
+    out_ms[3] = st.trace_shadow_ms;
+    c->stats = st;
     return AGX_OK;
 }
 

@@ -110,6 +110,9 @@ void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *sa
                          uint32_t *count, float *wsum, hipStream_t st);
 void launch_reset_counters(uint32_t *c, hipStream_t st);
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st);
+// rays of the closest-hit (which = 0) or any-hit (1) queue appended to o/d (at most cap)
+void launch_capture_rays(const PathBuffers &pb, int which, float4 *o, float4 *d, uint32_t *count, uint32_t cap,
+                         uint32_t max_n, hipStream_t st);
 void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st);
 void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st);
 void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,

@@ -2362,6 +2362,30 @@ __global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o,
     if (STATS) flush_counts(c, found, cnt);
 }
 
+// Copies the rays of the current closest-hit (which = 0) or any-hit (1) queue,
+// in queue order, into a capture buffer (ray replay measurements).
+__global__ void __launch_bounds__(256) k_capture_rays(PathBuffers pb, int which, float4 *o, float4 *d,
+                                                      uint32_t *count, uint32_t cap) {
+    const uint32_t n = pb.cnt_in[which];
+    if (blockIdx.x * blockDim.x >= n) return;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = tid < n;
+    const uint32_t idx = wave_append(count, live);
+    if (live && idx < cap) {
+        if (which == 0) {
+            const uint32_t ri = pb.q_closest[tid];
+            o[idx] = pb.ray_o[ri];
+            d[idx] = pb.ray_d[ri];
+        } else {
+            // shadow rays keep (o, t_max) (d, t_min); captured as (o, t_min) (d, t_max)
+            const uint32_t ri = pb.q_shadow[tid];
+            const float4 so = pb.sh_o[ri], sd = pb.sh_d[ri];
+            o[idx] = make_float4(so.x, so.y, so.z, sd.w);
+            d[idx] = make_float4(sd.x, sd.y, sd.z, so.w);
+        }
+    }
+}
+
 // plain streaming copy, used to calibrate HBM traffic counters
 __global__ void __launch_bounds__(256) k_copy_calibration(const float4 *__restrict__ src, float4 *__restrict__ dst,
                                                           uint64_t n) {
@@ -2447,6 +2471,11 @@ void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *a
                          hipStream_t st) {
     hipLaunchKernelGGL(k_film_resolve, dim3(grid_of(uint64_t(wh), 256)), dim3(256), 0, st, pb, wh, rgb, albedo,
                        normal, denoise);
+}
+void launch_capture_rays(const PathBuffers &pb, int which, float4 *o, float4 *d, uint32_t *count, uint32_t cap,
+                         uint32_t max_n, hipStream_t st) {
+    if (!max_n) return;
+    hipLaunchKernelGGL(k_capture_rays, dim3(grid_of(max_n, 256)), dim3(256), 0, st, pb, which, o, d, count, cap);
 }
 void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {

@@ -204,6 +204,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--stats-steps", type=int, default=2)
+    ap.add_argument("--no-replay", action="store_true", help="skip the traversal-only ray replay")
     ap.add_argument("--no-guided", action="store_true", help="skip the guided vs plain equal-time leg")
     ap.add_argument("--guided-res", type=int, default=512)
     ap.add_argument("--guided-tris", type=int, default=500_000)
@@ -288,6 +289,20 @@ def main() -> int:
         "copy_calibration": copy,
         "traversal_stack": "lds" if os.environ.get("AGX_TRAVERSAL_STACK") == "lds" else "private",
     }
+    if not args.no_replay:
+        # BVH-bound throughput of the same ray distribution: one frame's closest-hit and any-hit
+        # queries captured from the wavefront and traced again by the traversal kernel alone
+        rr = ctx.replay_rays(scenes.render_params(W, H, 1, cam, seed=4242, **tr))
+        n_r = rr["closest_rays"] + rr["shadow_rays"]
+        replay_mrays = n_r / ((rr["replay_closest_ms"] + rr["replay_shadow_ms"]) * 1e-3) / 1e6
+        out["replay"] = {
+            "rays": n_r, "closest_rays": rr["closest_rays"], "shadow_rays": rr["shadow_rays"],
+            "closest_ms": rr["replay_closest_ms"], "shadow_ms": rr["replay_shadow_ms"],
+            "render_closest_ms": rr["render_closest_ms"], "render_shadow_ms": rr["render_shadow_ms"],
+            "Mrays_per_s": replay_mrays, "pt_fraction": value / replay_mrays,
+        }
+        log(f"[bench] ray replay: {n_r} rays, traversal alone {replay_mrays:.1f} Mrays/s; "
+            f"PT reaches {value / replay_mrays:.1%} of it")
     if not args.no_guided:
         args.guided_steps = 4   # Adam steps per pass (library default)
         g = guided_leg(ctx, args)

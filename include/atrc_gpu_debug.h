@@ -58,6 +58,17 @@ int agx_debug_guide_train(agx_ctx *ctx, float *params, float *m, float *v, const
  * calibrating hardware traffic counters.  Returns its device time in *ms. */
 int agx_debug_copy_kernel(agx_ctx *ctx, uint64_t bytes, double *ms);
 
+/* Ray replay: renders `params` once (as agx_render, image discarded) while
+ * capturing every closest-hit and any-hit query the wavefront issues, in queue
+ * order (at most max_rays of each), then traces the captured sets again with
+ * the traversal kernel alone (no shading, no counters): the BVH-bound ray
+ * throughput of the same ray distribution.  out[0..1] = rays captured
+ * (closest, any-hit), out_ms[0..1] = average device time of one replay of
+ * each set (1 warm-up, then `reps` timed), out_ms[2..3] = the render's own
+ * closest-hit / any-hit kernel time.  AGX_E_LIMIT when a set overflows. */
+int agx_debug_replay_rays(agx_ctx *ctx, const agx_render_params *params, uint32_t max_rays, int reps,
+                          uint64_t out[2], double out_ms[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,3 +112,17 @@ def test_analytic_shapes_rays(gpu_ctx, accel):
     o, d = scenes.random_rays(((-2, -4, -2), (13, 6, 5)), 100_000, seed=12)
     n = len(o)
     _compare(gpu_ctx, s, o, d, np.zeros(n, np.float32), np.full(n, np.inf, np.float32), "analytic", accel)
+
+
+def test_ray_replay_captures_every_query(gpu_ctx):
+    """agx_debug_replay_rays captures exactly the queries the render issued."""
+    s, cam, tr = scenes.cornell_box()
+    s.accel = capi.ACCEL_BVH4
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    p = scenes.render_params(64, 64, 2, cam, seed=5, **tr)
+    rr = gpu_ctx.replay_rays(p)
+    st = gpu_ctx.stats()
+    log(f"[replay] cornell 64x64 2 spp: {rr}")
+    assert rr["closest_rays"] == st["closest_rays"] and rr["shadow_rays"] == st["shadow_rays"]
+    assert rr["replay_closest_ms"] > 0 and rr["replay_shadow_ms"] > 0
