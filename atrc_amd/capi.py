@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 5
+AGX_ABI_VERSION = 6
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -112,7 +112,7 @@ class agx_stats_t(C.Structure):
         ("total_ms", C.c_double), ("trace_closest_ms", C.c_double), ("trace_shadow_ms", C.c_double),
         ("shade_ms", C.c_double), ("trace_closest_launches", C.c_uint64), ("trace_shadow_launches", C.c_uint64),
         ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_samples", C.c_uint64),
-        ("guide_steps", C.c_uint64), ("guide_rows", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
+        ("guide_steps", C.c_uint64), ("guide_rows", C.c_uint64), ("guide_trained", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
     ]
 
     def as_dict(self) -> dict:

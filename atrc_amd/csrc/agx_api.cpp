@@ -81,6 +81,8 @@ struct agx_ctx {
     float *g_bias = nullptr, *g_samples = nullptr, *g_wsum = nullptr;
     __bf16 *g_packed = nullptr;
     uint32_t *g_count = nullptr;
+    uint32_t *g_ring = nullptr;      // sample ring (next slot, valid samples) over g_samples
+    uint32_t g_ring_cap = 0;         // capacity the ring state refers to (0: empty)
     uint32_t g_batch_cap = 0;
     uint64_t g_steps = 0;
     size_t g_wave_cap = 0;
@@ -161,7 +163,9 @@ struct agx_ctx {
             HIP_TRY(galloc(guide_allocs, &g_packed, G_PACK));
             HIP_TRY(galloc(guide_allocs, &g_count, 4));
             HIP_TRY(galloc(guide_allocs, &g_wsum, 4));
+            HIP_TRY(galloc(guide_allocs, &g_ring, 2));
         }
+        g_ring_cap = 0;
         std::vector<float> h(G_PARAMS);
         uint64_t x = 0x243F6A8885A308D3ull ^ uint64_t(seed);
         auto next = [&]() {  // splitmix64 -> uniform in [-1, 1)
@@ -186,10 +190,17 @@ struct agx_ctx {
         guide_ready = true;
         return ensure_guide_batch(batch);
     }
+    // sample storage for a ring of `batch` samples; the ring restarts empty when
+    // its capacity changes
     int ensure_guide_batch(uint32_t batch) {
-        if (batch <= g_batch_cap) return AGX_OK;
-        HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
-        g_batch_cap = batch;
+        if (batch > g_batch_cap) {
+            HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
+            g_batch_cap = batch;
+        }
+        if (g_ring_cap != batch) {
+            HIP_TRY(hipMemsetAsync(g_ring, 0, 2 * sizeof(uint32_t), stream));
+            g_ring_cap = batch;
+        }
         return AGX_OK;
     }
     int ensure_wave(size_t n, int L, int64_t wh) {
@@ -534,7 +545,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         return fail(AGX_E_UNSUPPORTED, "max_depth + specular_depth too large (at most " +
                                            std::to_string(agx_ctx::RING - 3) + ")");
     bool cancelled = false;
-    uint32_t guide_samples_h = 0;
+    uint32_t guide_samples_h = 0, guide_valid_h = 0;
     for (int64_t base = 0; base < total && !cancelled; base += per_pass) {
         rc.work_base = base;
         rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
@@ -607,10 +618,10 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             size_t ta, tb;
             if ((r = mark(&ta))) return r;
             HIP_TRY(hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
-            HIP_TRY(hipMemsetAsync(c->g_wsum, 0, sizeof(float), c->stream));
-            launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_wsum, c->stream);
+            launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_ring, c->stream);
+            launch_guide_ring_commit(c->g_ring, c->g_count, guide_batch, c->g_samples, c->g_wsum, c->stream);
             for (int k = 0; k < guide_steps; ++k) {
-                launch_guide_train(c->g_params, c->g_samples, c->g_count, guide_batch, c->g_wsum, c->g_partial,
+                launch_guide_train(c->g_params, c->g_samples, c->g_ring + 1, guide_batch, c->g_wsum, c->g_partial,
                                    agx_ctx::GUIDE_TRAIN_BLOCKS, c->stream);
                 launch_guide_reduce(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_grad, c->stream);
                 c->g_steps += 1;
@@ -619,12 +630,16 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             }
             launch_guide_pack(c->g_params, c->g_packed, c->g_bias, c->stream);
             HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipMemcpyAsync(&guide_valid_h, c->g_ring + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             if ((r = mark(&tb))) return r;
             timed.push_back({ta, tb, 4});
             b = tb;
         }
         HIP_TRY(hipEventSynchronize(c->pool[b]));
-        if (guide_train) st.guide_samples += std::min<uint32_t>(guide_samples_h, guide_batch);
+        if (guide_train) {
+            st.guide_samples += std::min<uint32_t>(guide_samples_h, guide_batch);
+            st.guide_trained += uint64_t(guide_valid_h) * uint64_t(guide_steps);
+        }
         // sum the ray queue sizes of this pass (every read has completed)
         for (size_t k = first_read; k < reads.size(); ++k) {
             const uint32_t *q = c->h_counters + 4 * reads[k].slot;

@@ -346,6 +346,30 @@ __global__ void __launch_bounds__(256) k_guide_train(const float *params, const 
 }
 
 // Sum of the per-block partial gradients in a fixed order.
+// one block: advance the ring, then a fixed-order sum of the valid weights
+__global__ void __launch_bounds__(256) k_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap,
+                                                           const float *samples, float *wsum) {
+    __shared__ float part[256];
+    __shared__ uint32_t valid_s;
+    if (threadIdx.x == 0) {
+        const uint32_t a = min(*added, cap);
+        ring[0] = (ring[0] + a) % cap;
+        valid_s = min(ring[1] + a, cap);
+        ring[1] = valid_s;
+    }
+    __syncthreads();
+    const uint32_t valid = valid_s;
+    float acc = 0.0f;
+    for (uint32_t i = threadIdx.x; i < valid; i += 256) acc += samples[size_t(i) * G_SAMPLE + G_IN + 3];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *wsum = part[0];
+}
+
 __global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int blocks, float *grad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G_PARAMS) return;
@@ -379,6 +403,11 @@ void launch_guide_train(const float *params, const float *samples, const uint32_
                         const float *wsum_dev, float *partial, int blocks, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(256), sizeof(TrainSmem), st, params, samples, n_dev, cap,
                        wsum_dev, partial);
+}
+
+void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
+                              float *wsum, hipStream_t st) {
+    hipLaunchKernelGGL(k_guide_ring_commit, dim3(1), dim3(256), 0, st, ring, added, cap, samples, wsum);
 }
 
 void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st) {

@@ -26,6 +26,11 @@ size_t guide_train_smem_bytes();
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
                         const float *wsum_dev, float *partial, int blocks, hipStream_t st);
 void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st);
+// sample ring: ring[0] = next write slot, ring[1] = valid samples (<= cap); a
+// gather appended *added samples from slot ring[0] on; advance the ring and
+// sum the weights of the valid samples into *wsum in a fixed order
+void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
+                              float *wsum, hipStream_t st);
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
                        int t, hipStream_t st);
 

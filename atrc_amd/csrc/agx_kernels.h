@@ -45,12 +45,14 @@ struct PathBuffers {
 
 // Path guiding: one path in 2^guide_rec_shift keeps training records of its
 // first GUIDE_MAX_REC guided vertices: inputs[8], direction[3], mixture pdf,
-// throughput after the vertex[3], incident radiance estimate[3].  The stride
-// keeps at most GUIDE_REC_PATHS recording paths per pass (about 4 records
-// each fill a 65536-sample training batch).
-constexpr int GUIDE_REC_PATHS = 16384;
-constexpr int GUIDE_MAX_REC = 16;
-constexpr int GUIDE_REC_FLOATS = 18;
+// throughput after the vertex[3], incident radiance estimate[3], luminance of
+// f |cos| of the sampled direction.  The stride keeps at most GUIDE_REC_PATHS
+// recording paths per pass (GUIDE_REC_PATHS * GUIDE_MAX_REC * 76 B = 160 MB):
+// light that reaches a vertex only through a narrow opening is found by few
+// paths, so the record budget is large.
+constexpr int GUIDE_REC_PATHS = 1 << 18;
+constexpr int GUIDE_MAX_REC = 8;
+constexpr int GUIDE_REC_FLOATS = 19;
 constexpr int GUIDE_ROW = 40;
 // smallest shift that leaves at most GUIDE_REC_PATHS recording paths among n
 inline uint32_t guide_rec_shift(size_t n) {
@@ -107,7 +109,7 @@ void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathB
 // training samples (inputs, direction, weight) from the records of finished paths;
 // appends at most cap samples, counts them in *count and sums their weights in *wsum
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
-                         uint32_t *count, float *wsum, hipStream_t st);
+                         uint32_t *count, const uint32_t *ring, hipStream_t st);
 void launch_reset_counters(uint32_t *c, hipStream_t st);
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st);
 // rays of the closest-hit (which = 0) or any-hit (1) queue appended to o/d (at most cap)

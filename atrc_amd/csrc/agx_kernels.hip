@@ -1922,10 +1922,13 @@ __global__ void __launch_bounds__(256) k_guide_features(DevScene s, RenderConst 
     dst[1] = make_float4(x[4], x[5], x[6], x[7]);
 }
 
-// Training samples from the records of the recording paths: weight = luminance
-// of the incident radiance estimate / density the direction was drawn with.
+// Training samples from the records of the recording paths, appended to the
+// sample ring from slot ring[0] on: weight = luminance of the incident radiance
+// estimate x luminance of f |cos| / density the direction was drawn with (the
+// product the sampling density should follow).  Records without light are
+// skipped: their weight is zero.
 __global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffers pb, float *samples, uint32_t cap,
-                                                      uint32_t *count, float *wsum) {
+                                                      uint32_t *count, const uint32_t *ring) {
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t r = gid / GUIDE_MAX_REC, j = gid % GUIDE_MAX_REC;
     const uint32_t last_slot = uint32_t(rc.n_paths - 1) >> rc.guide_rec_shift;
@@ -1935,15 +1938,14 @@ __global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffer
     if (r <= last_slot && int32_t(j) < pb.g_rec_state[r].x) {
         rec = pb.g_rec + (size_t(r) * GUIDE_MAX_REC + j) * GUIDE_REC_FLOATS;
         const float pdf = rec[11];
-        w = lum3(mk(rec[15], rec[16], rec[17])) / pdf;
+        w = lum3(mk(rec[15], rec[16], rec[17])) * rec[18] / pdf;
         ok = pdf > 0 && w > 0 && isfinite(w);
     }
     const uint32_t idx = wave_append(count, ok);
     if (ok && idx < cap) {
-        float *dst = samples + size_t(idx) * 12;
+        float *dst = samples + size_t((ring[0] + idx) % cap) * 12;
         for (int k = 0; k < 11; ++k) dst[k] = rec[k];
         dst[11] = w;
-        atomicAdd(wsum, w);
     }
 }
 
@@ -2224,6 +2226,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         rec[11] = pdf;
                         rec[12] = coef.x; rec[13] = coef.y; rec[14] = coef.z;
                         rec[15] = rec[16] = rec[17] = 0;
+                        rec[18] = lum3(f) * abscos;
                         st.y = st.x;
                         st.x += 1;
                         pb.g_rec_state[p >> rc.guide_rec_shift] = st;
@@ -2447,10 +2450,10 @@ void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathB
 }
 
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
-                         uint32_t *count, float *wsum, hipStream_t st) {
+                         uint32_t *count, const uint32_t *ring, hipStream_t st) {
     const uint64_t n = rc.n_paths > 0 ? ((uint64_t(rc.n_paths - 1) >> rc.guide_rec_shift) + 1) * GUIDE_MAX_REC : 0;
     if (!n) return;
-    hipLaunchKernelGGL(k_guide_gather, dim3(grid_of(n, 256)), dim3(256), 0, st, rc, pb, samples, cap, count, wsum);
+    hipLaunchKernelGGL(k_guide_gather, dim3(grid_of(n, 256)), dim3(256), 0, st, rc, pb, samples, cap, count, ring);
 }
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, st, src, dst, n);

@@ -132,7 +132,7 @@ def guided_leg(ctx, args) -> dict:
         acc = np.zeros((H, W, 3), np.float64)
         spp = 0
         st_sum = {"guide_infer_ms": 0.0, "guide_train_ms": 0.0, "guide_rows": 0, "guide_samples": 0,
-                  "total_ms": 0.0, "rays": 0}
+                  "guide_trained": 0, "total_ms": 0.0, "rays": 0}
         t_start = time.perf_counter()
         while time.perf_counter() - t_start < seconds:
             p = scenes.render_params(W, H, args.guided_chunk, cam, seed=seed, sample_begin=spp, guide=guide,
@@ -141,7 +141,7 @@ def guided_leg(ctx, args) -> dict:
             acc += img.astype(np.float64) * args.guided_chunk
             spp += args.guided_chunk
             st = ctx.stats()
-            for k in ("guide_infer_ms", "guide_train_ms", "guide_rows", "guide_samples", "total_ms"):
+            for k in ("guide_infer_ms", "guide_train_ms", "guide_rows", "guide_samples", "guide_trained", "total_ms"):
                 st_sum[k] += st[k]
             st_sum["rays"] += st["closest_rays"] + st["shadow_rays"]
         wall = time.perf_counter() - t_start
@@ -158,8 +158,7 @@ def guided_leg(ctx, args) -> dict:
     log(f"[bench] guided leg: plain {plain_spp} spp / {plain_wall:.2f} s relMSE {ep:.4e}; "
         f"guided {guided_spp} spp / {guided_wall:.2f} s relMSE {eg:.4e}; ratio {ep / eg:.2f}")
     infer_tflops = gst["guide_rows"] * MLP_FORWARD_FLOP / max(gst["guide_infer_ms"], 1e-9) / 1e9
-    train_tflops = (gst["guide_samples"] * args.guided_steps * MLP_TRAIN_FLOP
-                    / max(gst["guide_train_ms"], 1e-9) / 1e9)
+    train_tflops = gst["guide_trained"] * MLP_TRAIN_FLOP / max(gst["guide_train_ms"], 1e-9) / 1e9
     return {
         "workload": (f"door-gap room (10x10x4 Lambert room lit through a 0.05-wide gap, "
                      f"{s.triangle_count()} triangles), {W}x{H}, max depth {tr['max_depth']}"),
@@ -174,7 +173,8 @@ def guided_leg(ctx, args) -> dict:
         "mlp": {
             "rows": gst["guide_rows"], "infer_ms": gst["guide_infer_ms"],
             "infer_tflops": infer_tflops, "infer_frac_bf16_peak": infer_tflops / BF16_PEAK_TFLOPS,
-            "train_samples": gst["guide_samples"], "train_ms": gst["guide_train_ms"],
+            "train_samples": gst["guide_samples"], "trained_sample_steps": gst["guide_trained"],
+            "train_ms": gst["guide_train_ms"],
             "train_tflops": train_tflops, "train_frac_fp32_peak": train_tflops / FP32_PEAK_TFLOPS,
             "device_share": (gst["guide_infer_ms"] + gst["guide_train_ms"]) / max(gst["total_ms"], 1e-9),
         },
@@ -304,7 +304,6 @@ def main() -> int:
         log(f"[bench] ray replay: {n_r} rays, traversal alone {replay_mrays:.1f} Mrays/s; "
             f"PT reaches {value / replay_mrays:.1%} of it")
     if not args.no_guided:
-        args.guided_steps = 4   # Adam steps per pass (library default)
         g = guided_leg(ctx, args)
         out["guided"] = g
         out["guided_vs_plain_relmse_ratio"] = g["ratio"]

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 5
+#define AGX_ABI_VERSION 6
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -219,7 +219,8 @@ typedef struct agx_render_params {
     int32_t guide;             /* 1: guided continuation sampling */
     float   guide_alpha;       /* in (0, 1); default 0.5 */
     int32_t guide_train;       /* 1: train after every pass of this render (passes of one sample per pixel) */
-    int32_t guide_batch;       /* training samples per pass, at most 1 << 20 (default 65536) */
+    int32_t guide_batch;       /* size of the training sample ring: every Adam step trains on the
+                                  latest guide_batch samples; at most 1 << 20 (default 65536) */
     int32_t guide_steps;       /* Adam steps per pass (default 4) */
     float   guide_lr;          /* Adam learning rate (default 1e-2) */
     int32_t guide_reset;       /* 1: re-initialise the network from guide_seed first */
@@ -246,9 +247,10 @@ typedef struct agx_stats_t {
     uint64_t trace_shadow_launches;
     double   guide_infer_ms;   /* device time of the guiding network's inference kernel */
     double   guide_train_ms;   /* device time of gathering samples + training steps */
-    uint64_t guide_samples;    /* training samples used */
+    uint64_t guide_samples;    /* training samples gathered from the render's paths */
     uint64_t guide_steps;      /* Adam steps taken by the context's network so far */
     uint64_t guide_rows;       /* vertices the network evaluated */
+    uint64_t guide_trained;    /* samples passed through training (ring size x Adam steps) */
     uint32_t node_bytes, prim_bytes, info_bytes; /* record sizes used for the byte counts */
     uint32_t reserved;
 } agx_stats_t;
