@@ -130,10 +130,25 @@ class agx_debug_summary(C.Structure):
 
 
 # every symbol the public headers declare
+class agx_batch_item(C.Structure):
+    _fields_ = [
+        ("scene", C.POINTER(agx_scene_desc)), ("params", C.POINTER(agx_render_params)), ("rgb", C.POINTER(C.c_float)),
+        ("status", C.c_int32), ("worker", C.c_int32), ("rays", C.c_uint64), ("build_ms", C.c_double),
+        ("render_ms", C.c_double),
+    ]
+
+
+class agx_batch_report(C.Structure):
+    _fields_ = [
+        ("workers", C.c_int32), ("tuned", C.c_int32), ("wall_ms", C.c_double), ("rays", C.c_uint64),
+        ("tune_rays_per_s", C.c_double * 4),
+    ]
+
+
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
-    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays",
+    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays", "agx_render_batch",
 ]
 GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
 
@@ -170,6 +185,7 @@ def load_library() -> C.CDLL:
     lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
     lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
+    lib.agx_render_batch.argtypes = [C.c_void_p, P(agx_batch_item), C.c_uint32, C.c_int32, P(agx_batch_report)]
     lib.agx_debug_replay_rays.argtypes = [C.c_void_p, P(agx_render_params), C.c_uint32, C.c_int, P(C.c_uint64),
                                           P(C.c_double)]
     lib.agx_debug_bvh4.argtypes = [P(agx_scene_desc), P(C.c_uint64), P(C.c_uint64), P(C.c_float), P(C.c_float)]
@@ -244,6 +260,26 @@ class Context:
             _check(code)
         assert rgb.size == 3 * wh
         return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den, "cancelled": code == AGX_E_CANCELLED}
+
+    def render_batch(self, descs, params, workers: int = 0):
+        """agx_render_batch over (scene descriptor, render params) pairs: returns
+        (images, per-item records, report)."""
+        n = len(descs)
+        items = (agx_batch_item * n)()
+        images = []
+        for i, (d, p) in enumerate(zip(descs, params)):
+            img = np.zeros((p.height, p.width, 3), np.float32)
+            images.append(img)
+            items[i].scene = C.pointer(d)
+            items[i].params = C.pointer(p)
+            items[i].rgb = fptr(img)
+        rep = agx_batch_report()
+        _check(self.lib.agx_render_batch(self.h, items, n, workers, C.byref(rep)))
+        recs = [{"status": it.status, "worker": it.worker, "rays": it.rays, "build_ms": it.build_ms,
+                 "render_ms": it.render_ms} for it in items]
+        report = {"workers": rep.workers, "tuned": bool(rep.tuned), "wall_ms": rep.wall_ms, "rays": rep.rays,
+                  "tune_rays_per_s": list(rep.tune_rays_per_s)}
+        return images, recs, report
 
     def trace_rays(self, o: np.ndarray, d: np.ndarray, tmin: np.ndarray, tmax: np.ndarray, any_hit: bool = False):
         n = len(o)

@@ -7,7 +7,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/atrc_gpu.h"
@@ -70,6 +73,8 @@ struct agx_ctx {
     float *h_resolve = nullptr;      // pinned staging of the planes the caller asked for
     size_t resolve_cap = 0;          // pixels both buffers hold
     static constexpr int RING = 64;
+    // worker contexts of agx_render_batch (created on first use, same device)
+    std::vector<agx_ctx *> batch_workers;
     // ray capture of agx_debug_replay_rays (null outside it): [0] closest-hit, [1] any-hit rays
     float4 *capture_o[2] = {nullptr, nullptr}, *capture_d[2] = {nullptr, nullptr};
     uint32_t *capture_count = nullptr;  // device: 2 counters
@@ -342,6 +347,8 @@ int agx_create(int device_ordinal, agx_ctx **out) {
 
 void agx_destroy(agx_ctx *c) {
     if (!c) return;
+    for (agx_ctx *w : c->batch_workers) agx_destroy(w);
+    c->batch_workers.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->free_scene();
@@ -770,6 +777,133 @@ int agx_trace_rays(agx_ctx *c, const float *o, const float *d, const float *tmin
 int agx_get_stats(agx_ctx *c, agx_stats_t *out) {
     if (!c || !out) return fail(AGX_E_INVALID_ARG, "null argument");
     *out = c->stats;
+    return AGX_OK;
+}
+
+// ---------------------------------------------------------------- batched rendering
+}  // extern "C"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+}
+
+// Renders items[order[0..m)] with k worker contexts (each worker thread takes
+// the next item, builds + uploads it, renders it).  spp_override > 0 replaces
+// the items' spp (autotuner probes; images are then not written).  Returns the
+// first failing status; *first_err receives its message.
+int run_batch(agx_ctx *c, agx_batch_item *items, const std::vector<uint32_t> &order, int k, int spp_override,
+              uint64_t *rays_out, std::string *first_err) {
+    std::atomic<size_t> next{0};
+    std::atomic<uint64_t> rays{0};
+    std::mutex mu;
+    int status = AGX_OK;
+    auto work = [&](int wi) {
+        agx_ctx *w = c->batch_workers[size_t(wi)];
+        for (;;) {
+            const size_t j = next.fetch_add(1);
+            if (j >= order.size()) break;
+            agx_batch_item &it = items[order[j]];
+            agx_render_params p = *it.params;
+            if (spp_override > 0) p.spp = std::min(p.spp, spp_override);
+            const auto t0 = Clock::now();
+            int r = agx_upload_scene(w, it.scene);
+            const double b_ms = ms_since(t0);
+            const auto t1 = Clock::now();
+            if (r == AGX_OK) r = agx_render(w, &p, nullptr, spp_override > 0 ? nullptr : it.rgb, nullptr, nullptr, nullptr);
+            const double r_ms = ms_since(t1);
+            const uint64_t nr = r == AGX_OK ? w->stats.closest_rays + w->stats.shadow_rays : 0;
+            rays += nr;
+            if (spp_override <= 0) {
+                it.status = r;
+                it.worker = wi;
+                it.rays = nr;
+                it.build_ms = b_ms;
+                it.render_ms = r_ms;
+            }
+            if (r != AGX_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (status == AGX_OK) {
+                    status = r;
+                    *first_err = "batch item " + std::to_string(order[j]) + ": " + agx_last_error();
+                }
+            }
+        }
+    };
+    std::vector<std::thread> threads;
+    for (int wi = 1; wi < k; ++wi) threads.emplace_back(work, wi);
+    work(0);
+    for (auto &t : threads) t.join();
+    *rays_out = rays.load();
+    return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int agx_render_batch(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_t workers, agx_batch_report *report) {
+    if (!c || (n && !items)) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (workers < 0 || workers > 8) return fail(AGX_E_INVALID_ARG, "workers must lie in [0, 8]");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!items[i].scene || !items[i].params) return fail(AGX_E_INVALID_ARG, "batch item without scene or params");
+        if (items[i].params->guide) return fail(AGX_E_UNSUPPORTED, "guided rendering is not available in batches");
+        items[i].status = AGX_E_CANCELLED;
+        items[i].worker = -1;
+        items[i].rays = 0;
+        items[i].build_ms = items[i].render_ms = 0;
+    }
+    agx_batch_report rep{};
+    const int need = workers > 0 ? workers : 4;
+    while (int(c->batch_workers.size()) < need) {
+        agx_ctx *w = nullptr;
+        const int r = agx_create(c->device, &w);
+        if (r != AGX_OK) return r;
+        c->batch_workers.push_back(w);
+    }
+    std::string err;
+    int k = workers;
+    if (k == 0) {
+        // probe: the (up to) 4 cheapest items at 2 spp with 1, 2 and 4 workers;
+        // cost estimate = triangles + pixels * spp
+        std::vector<uint32_t> idx(n);
+        for (uint32_t i = 0; i < n; ++i) idx[i] = i;
+        auto cost = [&](uint32_t i) {
+            uint64_t tris = 0;
+            const agx_scene_desc &d = *items[i].scene;
+            for (uint32_t e = 0; e < d.entity_count; ++e) tris += d.entities[e].tri_count;
+            const agx_render_params &p = *items[i].params;
+            return double(tris) + double(p.width) * double(p.height) * double(p.spp);
+        };
+        std::sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cost(a) < cost(b); });
+        idx.resize(std::min<size_t>(idx.size(), 4));
+        double best = -1;
+        k = 1;
+        for (int kk : {1, 2, 4}) {
+            if (kk > 1 && size_t(kk) > idx.size()) break;
+            uint64_t rays = 0;
+            const auto t0 = Clock::now();
+            const int r = run_batch(c, items, idx, kk, 2, &rays, &err);
+            const double ms = ms_since(t0);
+            if (r != AGX_OK) return fail(r, err);
+            const double rate = rays / std::max(ms * 1e-3, 1e-9);
+            rep.tune_rays_per_s[kk - 1] = rate;
+            if (rate > best) { best = rate; k = kk; }
+        }
+        rep.tuned = 1;
+    }
+    std::vector<uint32_t> all(n);
+    for (uint32_t i = 0; i < n; ++i) all[i] = i;
+    uint64_t rays = 0;
+    const auto t0 = Clock::now();
+    const int r = run_batch(c, items, all, k, 0, &rays, &err);
+    rep.wall_ms = ms_since(t0);
+    rep.workers = k;
+    rep.rays = rays;
+    if (report) *report = rep;
+    if (r != AGX_OK) return fail(r, err);
     return AGX_OK;
 }
 

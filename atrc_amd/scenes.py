@@ -613,6 +613,58 @@ def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, ext
     return s, camera, trace
 
 
+def batch_scene(seed: int) -> tuple[Scene, dict, dict, int]:
+    """One scene of the batched workload: procedural_spheres with a seeded
+    triangle count log-uniform in [1e4, 2e6] (subdivision-4 spheres, or
+    subdivision 3 below 51,200 triangles) in a box that grows with the count,
+    a resolution drawn from {256, 512, 1024} squared, and 32 spp.  Returns
+    (scene, camera, trace, resolution)."""
+    rng = np.random.default_rng(seed)
+    tris = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
+    subdiv = 4 if tris >= 51_200 else 3
+    n = max(2, int(round(tris / (20 * 4 ** subdiv))))
+    extent = 50.0 * (n / 200.0) ** (1.0 / 3.0)
+    s, cam, tr = procedural_spheres(n_spheres=n, subdiv=subdiv, seed=seed, extent=extent)
+    k = extent / 50.0
+    cam = {"pos": (-25.0 * k, -35.0 * k, 45.0 * k), "dst": (25.0 * k, 25.0 * k, 15.0 * k), "up": (0, 0, 1),
+           "fov": 40.0}
+    res = int(rng.choice([256, 512, 1024]))
+    return s, cam, tr, res
+
+
+def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int = 11) -> tuple[Scene, dict, dict]:
+    """The door-gap room (guided_room) with its first room filled with
+    participating media: a heterogeneous medium on a seeded grid_n^3 fBm density
+    (4 octaves, max 2.0, albedo 0.9, HG g = 0.3) over the whole room, and a
+    homogeneous slab (sigma_t 0.5, albedo 0.9) between z = 2.0 and 2.5 bounded
+    by an invisible_surface box.  Every surface facing the first room has the
+    fog on that side (entity med_out, or med_in for the slab's box)."""
+    s, cam, tr = guided_room(n_clutter_tris=n_clutter_tris, seed=7)
+    fog = s.add_medium(Medium(kind=capi.MEDIUM_HETEROGENEOUS,
+                              local_to_world=(10.0, 0, 0, 0, 0, 10.0, 0, 0, 0, 0, 4.0, 0),
+                              density=Texture3D(grid=fbm_grid(grid_n, seed) * 2.0),
+                              albedo=Texture3D(value=(0.9, 0.9, 0.9)),
+                              g3=Texture3D(value=(0.3, 0.3, 0.3))))
+    slab = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.05, 0.05, 0.05),
+                               sigma_s=(0.45, 0.45, 0.45), g=0.0))
+    for e in s.entities:
+        if e.tris is not None:                       # clutter: outward normals, fog outside
+            e.med_out = fog
+            continue
+        if any(c != 0 for c in e.emit):              # the light, in the second room
+            continue
+        p = np.asarray(e.points, np.float64)
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        centre = p.mean(axis=0)
+        # inward-facing walls of the first room, and the dividing wall's side facing it
+        if centre[0] < 10.0 - 1e-6 or (abs(centre[0] - 10.0) < 1e-6 and n[0] < 0):
+            e.med_out = fog
+    inv = s.add_material(Material(kind=capi.MAT_INVISIBLE_SURFACE))
+    s.add(Entity(capi.GEOM_TRIANGLE_BVH, inv, tris=box_mesh((0.5, 0.5, 2.0), (9.5, 9.5, 2.5)), med_in=slab,
+                 med_out=fog))
+    return s, cam, dict(tr, max_depth=16)
+
+
 def random_rays(scene_bounds: tuple, n: int, seed: int, inside: bool = True):
     """Seeded rays: origins uniform in the box, directions uniform on the sphere."""
     rng = np.random.default_rng(seed)

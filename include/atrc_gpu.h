@@ -257,6 +257,26 @@ typedef struct agx_stats_t {
 
 typedef struct agx_ctx agx_ctx;
 
+/* One scene of a batch (agx_render_batch). */
+typedef struct agx_batch_item {
+    const agx_scene_desc *scene;
+    const agx_render_params *params;  /* guide must be 0 */
+    float   *rgb;              /* W*H*3 floats as agx_render writes them, or NULL */
+    int32_t status;            /* out: AGX_* of this item */
+    int32_t worker;            /* out: worker context that rendered it */
+    uint64_t rays;             /* out: closest-hit + any-hit queries */
+    double  build_ms;          /* out: host build + upload wall time */
+    double  render_ms;         /* out: render wall time */
+} agx_batch_item;
+
+typedef struct agx_batch_report {
+    int32_t workers;           /* scenes in flight at once */
+    int32_t tuned;             /* 1: chosen by the autotuner */
+    double  wall_ms;           /* whole batch, builds included */
+    uint64_t rays;
+    double  tune_rays_per_s[4];/* autotuner probe throughput with 1, 2, 3, 4 workers (0 = not tried) */
+} agx_batch_report;
+
 /* Version of this library's ABI (AGX_ABI_VERSION). */
 int  agx_abi_version(void);
 
@@ -290,6 +310,19 @@ int  agx_trace_rays(agx_ctx *ctx, const float *o, const float *d,
                     int32_t *entity_id, int32_t *prim_id, float *t, float *u, float *v);
 
 int  agx_get_stats(agx_ctx *ctx, agx_stats_t *out);
+
+/* Batched rendering (no counterpart in Atrc, which renders one scene per
+ * session): renders every item as agx_upload_scene + agx_render would, with
+ * `workers` scenes in flight at once, each on a worker context with its own
+ * HIP stream and buffers.  A worker builds its next scene's BVHs on the host
+ * while the others keep the GPU busy, and the kernels of different scenes
+ * overlap on the device (filling the tail of each wavefront).  workers = 0
+ * lets the autotuner choose 1, 2 or 4 from a short probe (the cheapest items
+ * at 2 spp).  Images are those agx_render gives for each item alone.  Returns
+ * AGX_OK when every item succeeded, else the first failure's status (each
+ * item's own status is in item.status). */
+int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t workers,
+                      agx_batch_report *report);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *agx_last_error(void);

@@ -39,6 +39,7 @@ def test_null_arguments_are_rejected():
     assert lib.agx_create(0, None) == capi.AGX_E_INVALID_ARG
     assert lib.agx_upload_scene(None, None) == capi.AGX_E_INVALID_ARG
     assert lib.agx_render(None, None, None, None, None, None, None) == capi.AGX_E_INVALID_ARG
+    assert lib.agx_render_batch(None, None, 0, 0, None) == capi.AGX_E_INVALID_ARG
     assert b"null" in lib.agx_last_error()
 
 

@@ -1,0 +1,62 @@
+"""Batched rendering (agx_render_batch; no counterpart in Atrc): several scenes
+in flight at once on worker contexts must give, item for item, the image that
+agx_render gives for that scene alone (same per-(pixel, sample) streams, same
+film order, so bit-identical), whatever the number of workers the autotuner
+picks."""
+import numpy as np
+import pytest
+
+from atrc_amd import capi, scenes
+
+from conftest import log
+
+pytestmark = pytest.mark.gpu
+
+
+def _items():
+    out = []
+    s, cam, tr = scenes.cornell_box()
+    out.append((s, scenes.render_params(64, 48, 8, cam, seed=3, **tr)))
+    for k, accel in enumerate((capi.ACCEL_BVH4, capi.ACCEL_ATRC, capi.ACCEL_BVH4_WATERTIGHT)):
+        s, cam, tr = scenes.procedural_spheres(n_spheres=6 + 4 * k, subdiv=3, seed=10 + k, extent=12.0)
+        s.accel = accel
+        cam = {"pos": (-8.0, -10.0, 14.0), "dst": (6.0, 6.0, 4.0), "fov": 45.0}
+        out.append((s, scenes.render_params(40 + 16 * k, 32 + 8 * k, 4 + 2 * k, cam, seed=20 + k, **tr)))
+    s, cam, tr = scenes.volumetric_cornell(grid_n=8)
+    out.append((s, scenes.render_params(32, 32, 6, cam, seed=7, **tr)))
+    return out
+
+
+@pytest.mark.parametrize("workers", [3, 0])
+def test_batch_matches_single_renders(gpu_ctx, workers):
+    items = _items()
+    descs, keep, params = [], [], []
+    for s, p in items:
+        d, k = s.to_desc()
+        descs.append(d)
+        keep.append(k)
+        params.append(p)
+    alone = []
+    for d, p in zip(descs, params):
+        gpu_ctx.upload_scene(d)
+        alone.append(gpu_ctx.render(p, aovs=False)["image"])
+    log(f"[batch] {len(items)} scenes, workers={workers}")
+    imgs, recs, rep = gpu_ctx.render_batch(descs, params, workers=workers)
+    log(f"  report {rep}")
+    for i, (a, b, r) in enumerate(zip(alone, imgs, recs)):
+        assert r["status"] == capi.AGX_OK and r["rays"] > 0
+        np.testing.assert_array_equal(a, b, err_msg=f"item {i}")
+    assert rep["workers"] >= 1 and rep["rays"] == sum(r["rays"] for r in recs)
+    if workers == 0:
+        assert rep["tuned"] and rep["tune_rays_per_s"][0] > 0
+    else:
+        assert rep["workers"] == workers and len({r["worker"] for r in recs}) > 1
+
+
+def test_batch_rejects_guided_items(gpu_ctx):
+    s, cam, tr = scenes.cornell_box()
+    d, keep = s.to_desc()
+    p = scenes.render_params(16, 16, 1, cam, guide=True, **tr)
+    with pytest.raises(capi.AgxError) as e:
+        gpu_ctx.render_batch([d], [p], workers=1)
+    assert e.value.code == capi.AGX_E_UNSUPPORTED
