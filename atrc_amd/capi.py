@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 6
+AGX_ABI_VERSION = 7
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -27,6 +27,7 @@ TEX3D_CONSTANT, TEX3D_GRID = range(2)
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
 MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE, MAT_DISNEY = range(7)
 ENV_NONE, ENV_NATIVE_SKY = 0, 1
+FILTER_BOX, FILTER_GAUSSIAN = 0, 1
 ACCEL_ATRC, ACCEL_BVH4, ACCEL_BVH4_WATERTIGHT = 0, 1, 2
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
 
@@ -99,7 +100,7 @@ class agx_render_params(C.Structure):
         ("eps", C.c_float), ("filter_radius", C.c_float), ("seed", C.c_uint64), ("camera", agx_camera_desc),
         ("collect_stats", C.c_int32), ("sample_begin", C.c_int32), ("guide", C.c_int32), ("guide_alpha", C.c_float),
         ("guide_train", C.c_int32), ("guide_batch", C.c_int32), ("guide_steps", C.c_int32), ("guide_lr", C.c_float),
-        ("guide_reset", C.c_int32), ("guide_seed", C.c_uint32),
+        ("guide_reset", C.c_int32), ("guide_seed", C.c_uint32), ("filter", C.c_int32), ("filter_alpha", C.c_float),
     ]
 
 
@@ -377,7 +378,8 @@ HOST_LIB_PATH = PKG / "libatrc_host.so"
 
 class atrc_host_session(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("eps", C.c_float), ("filter_radius", C.c_float),
-                ("camera", agx_camera_desc), ("session_count", C.c_int32)]
+                ("camera", agx_camera_desc), ("session_count", C.c_int32), ("filter", C.c_int32),
+                ("filter_alpha", C.c_float)]
 
 
 _host = None

@@ -459,9 +459,15 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
     if (p->max_depth < 1 || p->min_depth < 0 || !(p->cont_prob > 0) || !(p->eps > 0))
         return fail(AGX_E_INVALID_ARG, "invalid depth / cont_prob / eps");
-    const float fr = p->filter_radius - p->eps;
-    if (!(fr > 0)) return fail(AGX_E_INVALID_ARG, "invalid box filter radius");
-    if (fr > 0.5f) return fail(AGX_E_UNSUPPORTED, "box filter radius above 0.5 + eps is not supported by gpu_pt");
+    // BoxFilter radius_ = radius - EPS (box_filter.cpp:15-21); GaussianFilter keeps
+    // its radius and needs alpha > 0 (gaussian_filter.cpp:45-56)
+    if (p->filter != AGX_FILTER_BOX && p->filter != AGX_FILTER_GAUSSIAN)
+        return fail(AGX_E_INVALID_ARG, "unknown film filter");
+    const bool gaussian = p->filter == AGX_FILTER_GAUSSIAN;
+    const float fr = gaussian ? p->filter_radius : p->filter_radius - p->eps;
+    if (!(fr > 0)) return fail(AGX_E_INVALID_ARG, "invalid film filter radius");
+    if (gaussian && !(p->filter_alpha > 0)) return fail(AGX_E_INVALID_ARG, "invalid Gaussian filter alpha");
+    if (fr > 64.0f) return fail(AGX_E_LIMIT, "film filter radius above 64 pixels");
     HIP_TRY(hipSetDevice(c->device));
 
     RenderConst rc{};
@@ -473,6 +479,16 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     rc.cont_prob = p->cont_prob;
     rc.eps = p->eps;
     rc.filter_r = fr;
+    // a filter wider than half a pixel splats; sample_bound_of (framebuffer.h:36-47)
+    // adds floor-based borders around the image
+    rc.splat = (gaussian || fr > 0.5f) ? 1 : 0;
+    rc.filter_kind = p->filter;
+    rc.filter_alpha = gaussian ? p->filter_alpha : 0.0f;
+    rc.filter_exp = gaussian ? std::exp(-p->filter_alpha * fr * fr) : 0.0f;
+    rc.border0 = rc.splat ? -int32_t(std::floor(0.5f - fr)) : 0;
+    rc.border1 = rc.splat ? int32_t(std::floor(fr + 0.49999f)) : 0;
+    rc.EW = p->width + rc.border0 + rc.border1;
+    rc.EH = p->height + rc.border0 + rc.border1;
     rc.seed = p->seed;
     rc.sample_begin = p->sample_begin;
     rc.L = c->n_lights;
@@ -494,14 +510,16 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     }
 
     const int64_t wh = int64_t(p->width) * p->height;
-    const int64_t total = wh * p->spp;
+    // paths per sample layer: the image, or the image and its border when samples splat
+    const int64_t layer = rc.splat ? int64_t(rc.EW) * rc.EH : wh;
+    const int64_t total = layer * p->spp;
     const size_t cap = max_paths_in_flight();
     int64_t per_pass;
-    if (wh <= int64_t(cap)) per_pass = wh * std::min<int64_t>(p->spp, int64_t(cap) / wh);
-    else per_pass = int64_t(cap);
+    if (layer <= int64_t(cap)) per_pass = layer * std::min<int64_t>(p->spp, int64_t(cap) / layer);
+    else per_pass = rc.splat ? layer : int64_t(cap);  // splatting needs whole layers in a pass
     // a training render takes one sample per pixel per pass, so that the network
     // learns after every sample layer
-    if (guide_train && wh <= int64_t(cap)) per_pass = wh;
+    if (guide_train && (layer <= int64_t(cap) || rc.splat)) per_pass = layer;
     rc.guide_rec_shift = guide_rec_shift(size_t(per_pass));
     int r = c->ensure_wave(size_t(per_pass), rc.L, wh);
     if (r != AGX_OK) return r;

@@ -91,6 +91,14 @@ struct RenderConst {
     uint32_t guide_rec_shift;  // recording paths: (p + guide_rec_rot) % 2^shift == 0, record slot p >> shift
     uint32_t guide_rec_rot;    // varies per pass so that every pixel records in turn
     int32_t pad2;
+    // wide film filters (box wider than a pixel, Gaussian): every sample splats
+    // onto all pixels within filter_r; the sampled grid extends border0 pixels
+    // before and border1 pixels after the image on each axis (EW x EH pixels)
+    int32_t splat;
+    int32_t filter_kind;   // AGX_FILTER_*
+    int32_t border0, border1;
+    int32_t EW, EH;
+    float filter_alpha, filter_exp;  // Gaussian: alpha and e^{-alpha r^2}
     float4 cam_pos, cam_x, cam_y, cam_z;  // camera frame
     float film_w, film_h, lens_r, focal;
 };

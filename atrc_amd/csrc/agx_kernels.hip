@@ -1581,11 +1581,15 @@ __global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb
 __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, PathBuffers pb) {
     const int32_t p = int32_t(blockIdx.x * blockDim.x + threadIdx.x);
     if (p >= rc.n_paths) return;
-    const int64_t wh = int64_t(rc.W) * rc.H;
+    // work item = layer * (sampled pixels) + pixel; the sampled grid is the image,
+    // or the image with its filter border when samples splat
+    const int32_t gw = rc.splat ? rc.EW : rc.W, gh = rc.splat ? rc.EH : rc.H;
+    const int64_t wh = int64_t(gw) * gh;
     const int64_t w = rc.work_base + p;
     const int64_t pixel = w % wh;
     const int64_t sample = int64_t(rc.sample_begin) + w / wh;
-    const int32_t px = int32_t(pixel % rc.W), py = int32_t(pixel / rc.W);
+    const int32_t off = rc.splat ? rc.border0 : 0;
+    const int32_t px = int32_t(pixel % gw) - off, py = int32_t(pixel / gw) - off;
     Rng rng = rng_seed(rc.seed, path_seq((unsigned long long)pixel, (unsigned long long)sample));
     // PerPixelRenderer::render_grid (perpixel_renderer.cpp:27-38)
     const float fu = rng.uniform(), fv = rng.uniform();
@@ -1602,7 +1606,9 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     const V l = film - lens;
     const V d = normalize(cx * l.x + cy * l.y + cz * l.z);
     // box filter support: the sample reaches its own pixel only inside radius r
-    const bool keep = fabsf(float(px) + 0.5f - pxf) <= rc.filter_r && fabsf(float(py) + 0.5f - pyf) <= rc.filter_r;
+    // (a splatting filter weighs it for every pixel in k_film_splat)
+    const bool keep = rc.splat || (fabsf(float(px) + 0.5f - pxf) <= rc.filter_r &&
+                                   fabsf(float(py) + 0.5f - pyf) <= rc.filter_r);
     uint32_t flags = keep ? PF_KEEP : 0u;
     float4 coef = make_float4(1, 1, 1, 0);
     pb.radiance[p] = make_float4(0, 0, 0, 0);
@@ -2308,6 +2314,56 @@ __global__ void __launch_bounds__(256) k_film(RenderConst rc, PathBuffers pb) {
     }
 }
 
+// GaussianFilter::eval without its normalisation (it cancels in value / weight,
+// gaussian_filter.cpp:60-68)
+__device__ __forceinline__ float gauss1(float d, float alpha, float expv) {
+    return fmaxf(0.0f, expf(-alpha * d * d) - expv);
+}
+
+// Wide film filters: each image pixel gathers, in a fixed order (layer, then
+// sampled pixel row by row), every sample of this pass that lies within the
+// filter radius, weighted by the filter (FilmGrid::apply + apply_image_filter,
+// render_target.h:384-395, framebuffer.h:49-80).  A sample's film position is
+// re-derived from the first two numbers of its stream.
+__global__ void __launch_bounds__(256) k_film_splat(RenderConst rc, PathBuffers pb) {
+    const int64_t wh = int64_t(rc.W) * rc.H;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= wh) return;
+    const int32_t x = int32_t(i % rc.W), y = int32_t(i / rc.W);
+    const int64_t ewh = int64_t(rc.EW) * rc.EH;
+    const int64_t layers = rc.n_paths / ewh;
+    const int64_t first_sample = int64_t(rc.sample_begin) + rc.work_base / ewh;
+    const float r = rc.filter_r;
+    const int32_t reach = int32_t(floorf(r + 0.5f));
+    float *sum = pb.film_sum, *cmp = pb.film_comp;
+    float wsum = 0.0f;
+    for (int64_t j = 0; j < layers; ++j) {
+        for (int32_t qy = y - reach; qy <= y + reach; ++qy) {
+            if (qy < -rc.border0 || qy >= rc.H + rc.border1) continue;
+            for (int32_t qx = x - reach; qx <= x + reach; ++qx) {
+                if (qx < -rc.border0 || qx >= rc.W + rc.border1) continue;
+                const int64_t q = int64_t(qy + rc.border0) * rc.EW + (qx + rc.border0);
+                const int64_t slot = j * ewh + q;
+                const float4 v = pb.radiance[slot];
+                if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
+                Rng rs = rng_seed(rc.seed, path_seq((unsigned long long)q, (unsigned long long)(first_sample + j)));
+                const float sx = float(qx) + rs.uniform(), sy = float(qy) + rs.uniform();
+                const float xr = fabsf(float(x) + 0.5f - sx), yr = fabsf(float(y) + 0.5f - sy);
+                if (xr > r || yr > r) continue;
+                const float wgt = rc.filter_kind == AGX_FILTER_GAUSSIAN
+                                      ? gauss1(xr, rc.filter_alpha, rc.filter_exp) * gauss1(yr, rc.filter_alpha,
+                                                                                              rc.filter_exp)
+                                      : 1.0f;
+                const float4 a = pb.gb_albedo[slot], n = pb.gb_normal[slot];
+                const float vals[10] = {v.x, v.y, v.z, a.x, a.y, a.z, n.x, n.y, n.z, a.w};
+                for (int k = 0; k < 10; ++k) neumaier(&sum[k * wh + i], &cmp[k * wh + i], wgt * vals[k]);
+                wsum += wgt;
+            }
+        }
+    }
+    pb.film_count[i] += wsum;
+}
+
 __global__ void k_film_clear(PathBuffers pb, int64_t wh) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= wh) return;
@@ -2463,6 +2519,10 @@ void launch_reset_counters(uint32_t *c, hipStream_t st) {
 }
 void launch_film(const RenderConst &rc, const PathBuffers &pb, hipStream_t st) {
     const int64_t wh = int64_t(rc.W) * rc.H;
+    if (rc.splat) {
+        hipLaunchKernelGGL(k_film_splat, dim3(grid_of(uint64_t(wh), 256)), dim3(256), 0, st, rc, pb);
+        return;
+    }
     const int64_t cover = rc.n_paths < wh ? int64_t(rc.n_paths) : wh;
     if (cover <= 0) return;
     hipLaunchKernelGGL(k_film, dim3(grid_of(uint64_t(cover), 256)), dim3(256), 0, st, rc, pb);

@@ -640,6 +640,9 @@ CreatingContext::CreatingContext() {
         f->type = "gaussian";
         f->radius = p.child_real("radius");
         f->alpha = p.child_real("alpha");
+        // GaussianFilter::GaussianFilter (gaussian_filter.cpp:45-56)
+        if (f->radius <= 0) throw ObjectConstructionException("invalid radius value: " + std::to_string(f->radius));
+        if (f->alpha <= 0) throw ObjectConstructionException("invalid alpha value: " + std::to_string(f->alpha));
         return f;
     });
 

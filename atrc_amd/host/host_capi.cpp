@@ -67,6 +67,8 @@ int atrc_host_session_info(void *handle, int index, atrc_host_session *out) {
     out->filter_radius = s.film_filter ? s.film_filter->radius : 0.5f;
     out->camera = s.camera->desc;
     out->session_count = int32_t(h->sessions.size());
+    out->filter = s.film_filter && s.film_filter->type == "gaussian" ? AGX_FILTER_GAUSSIAN : AGX_FILTER_BOX;
+    out->filter_alpha = s.film_filter ? s.film_filter->alpha : 0.0f;
     return 0;
 }
 

@@ -48,8 +48,8 @@ public:
 
     RenderTarget render(FilmFilterApplier filter, Scene &scene, RendererInteractor &reporter) override {
         std::lock_guard<std::mutex> lk(mu_);
-        if (!filter.filter || filter.filter->type != "box")
-            throw ObjectConstructionException("gpu_pt supports the box film filter only");
+        if (!filter.filter || (filter.filter->type != "box" && filter.filter->type != "gaussian"))
+            throw ObjectConstructionException("gpu_pt supports the box and gaussian film filters");
         if (!scene.camera) throw ObjectConstructionException("scene has no camera");
         int rc;
         if (!ctx_ && (rc = agx_create(params_.device, &ctx_)) != AGX_OK) throw std::runtime_error(agx_error(rc));
@@ -67,6 +67,8 @@ public:
         rp.use_mis = 1;
         rp.eps = EPS();
         rp.filter_radius = filter.filter->radius;
+        rp.filter = filter.filter->type == "gaussian" ? AGX_FILTER_GAUSSIAN : AGX_FILTER_BOX;
+        rp.filter_alpha = filter.filter->alpha;
         rp.seed = params_.seed;
         rp.camera = scene.camera->desc;
         if (params_.guide) {

@@ -220,7 +220,8 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
                   cont_prob: float = 0.9, specular_depth: int = 20, eps: float = 3e-4, seed: int = 42,
                   filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0,
                   guide: bool = False, guide_alpha: float = 0.5, guide_train: bool = True, guide_batch: int = 65536,
-                  guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0):
+                  guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0,
+                  filter: str = "box", filter_alpha: float = 0.0):
     p = capi.agx_render_params()
     p.width, p.height, p.spp = width, height, spp
     p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
@@ -246,6 +247,8 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
     p.guide_lr = guide_lr
     p.guide_reset = 1 if guide_reset else 0
     p.guide_seed = guide_seed
+    p.filter = {"box": capi.FILTER_BOX, "gaussian": capi.FILTER_GAUSSIAN}[filter]
+    p.filter_alpha = filter_alpha
     return p
 
 
@@ -663,6 +666,19 @@ def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int 
     s.add(Entity(capi.GEOM_TRIANGLE_BVH, inv, tris=box_mesh((0.5, 0.5, 2.0), (9.5, 9.5, 2.5)), med_in=slab,
                  med_out=fog))
     return s, cam, dict(tr, max_depth=16)
+
+
+def half_plane_emitter() -> tuple[Scene, dict, dict]:
+    """A quad of radiance 1 covering the half-plane x >= 0 of the plane y = 0,
+    facing a pinhole camera at (0, -5, 0) that looks along +y: the edge runs
+    through the centre of the film, so with an even width it lies exactly on
+    the pixel boundary W/2 and a film filter turns it into its own integral."""
+    s = Scene(aggregate=capi.AGGREGATE_NATIVE)
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    s.add(Entity(capi.GEOM_QUAD, black, emit=(1.0, 1.0, 1.0),
+                 points=((0.0, 0.0, -100.0), (100.0, 0.0, -100.0), (100.0, 0.0, 100.0), (0.0, 0.0, 100.0))))
+    cam = {"pos": (0.0, -5.0, 0.0), "dst": (0.0, 0.0, 0.0), "up": (0, 0, 1), "fov": 40.0}
+    return s, cam, {"min_depth": 5, "max_depth": 1, "cont_prob": 0.9}
 
 
 def random_rays(scene_bounds: tuple, n: int, seed: int, inside: bool = True):

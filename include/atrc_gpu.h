@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 6
+#define AGX_ABI_VERSION 7
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -65,6 +65,7 @@ enum {
 };
 
 enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
+enum { AGX_FILTER_BOX = 0, AGX_FILTER_GAUSSIAN = 1 };
 enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
 /* Acceleration structure built at upload.
  *   AGX_ACCEL_ATRC: Atrc's own BVHs and traversal order (triangle_bvh.cpp,
@@ -204,7 +205,7 @@ typedef struct agx_render_params {
     float   cont_prob;         /* default 0.9 */
     int32_t use_mis;           /* must be 1 (trace_std) */
     float   eps;               /* global EPS, default 3e-4 */
-    float   filter_radius;     /* box filter radius as given (default 0.5) */
+    float   filter_radius;     /* film filter radius as given (default 0.5) */
     uint64_t seed;             /* per-(pixel, sample) PCG32 streams derive from it */
     agx_camera_desc camera;
     int32_t collect_stats;     /* 1: count node visits / triangle tests (slower) */
@@ -225,6 +226,14 @@ typedef struct agx_render_params {
     float   guide_lr;          /* Adam learning rate (default 1e-2) */
     int32_t guide_reset;       /* 1: re-initialise the network from guide_seed first */
     uint32_t guide_seed;
+    /* Film filter (film_filter_creators.cpp): AGX_FILTER_BOX (radius - eps; a
+     * radius up to 0.5 keeps every sample in its own pixel) or
+     * AGX_FILTER_GAUSSIAN (radius, alpha: weight (e^{-alpha x^2} - e^{-alpha r^2})
+     * (e^{-alpha y^2} - e^{-alpha r^2})).  Wider filters splat every sample
+     * onto all pixels within the radius, and pixels up to the radius outside
+     * the image are sampled as well (Atrc's sample_bound_of, framebuffer.h). */
+    int32_t filter;            /* AGX_FILTER_*, default box */
+    float   filter_alpha;      /* Gaussian alpha */
 } agx_render_params;
 
 /* Counters and timers of the last agx_render / agx_trace_rays call. */

@@ -19,6 +19,8 @@ typedef struct atrc_host_session {
     float filter_radius;
     agx_camera_desc camera;
     int32_t session_count;
+    int32_t filter;            /* AGX_FILTER_* */
+    float filter_alpha;        /* Gaussian alpha */
 } atrc_host_session;
 
 /* Parse JSON text ({"scene": ..., "rendering": ...}) and create the scene and

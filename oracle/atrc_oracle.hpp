@@ -2320,6 +2320,12 @@ struct RenderOut {
     double *rgb = nullptr, *albedo = nullptr, *normal = nullptr, *denoise = nullptr;
 };
 
+// GaussianFilter::eval without its normalisation factor, which cancels in
+// value / weight (film_filter/gaussian_filter.cpp:60-68)
+inline float gaussian_1d(float d, float alpha, float expv) {
+    return std::max(0.0f, float(std::exp(-alpha * d * d)) - expv);
+}
+
 template<class T>
 void render(const Scene<T> &scene, const agx_render_params &rp, int threads, const RenderOut &out,
             Counters *total, int row_begin = 0, int row_end = -1) {
@@ -2330,12 +2336,35 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
     tp.cont_prob = rp.cont_prob;
     const int W = rp.width, H = rp.height;
     if (row_end < 0) row_end = H;
-    const T radius = T(rp.filter_radius) - scene.eps;            // BoxFilter radius_ = r - EPS
-    if (!(radius > 0)) throw std::runtime_error("invalid box filter radius");
+    // BoxFilter radius_ = r - EPS (box_filter.cpp:15-21); GaussianFilter keeps r
+    const bool gaussian = rp.filter == AGX_FILTER_GAUSSIAN;
+    const float radius_f = gaussian ? rp.filter_radius : rp.filter_radius - float(scene.eps);
+    const T radius = T(radius_f);
+    if (!(radius > 0)) throw std::runtime_error("invalid film filter radius");
+    const float g_exp = gaussian ? float(std::exp(-rp.filter_alpha * radius_f * radius_f)) : 0.0f;
+    // Filters wider than half a pixel splat every sample onto the pixels of its
+    // tile within the radius, and each tile samples the pixels of
+    // sample_bound_of(radius, tile) (framebuffer.h:36-80; render_target.h:384-395).
+    // A sample's stream is keyed by its pixel in the whole sampled grid, so the
+    // samples that two tiles both take are the same paths.
+    const bool splat = gaussian || radius_f > 0.5f;
+    const int b0 = splat ? -int(std::floor(0.5f - radius_f)) : 0;
+    const int b1 = splat ? int(std::floor(radius_f + 0.49999f)) : 0;
+    const int EW = W + b0 + b1;
     const int tile = 32;
     const int tiles_x = (W + tile - 1) / tile, tiles_y = (row_end - row_begin + tile - 1) / tile;
     std::atomic<int> next{0};
     std::vector<Counters> cs(size_t(std::max(1, threads)));
+    auto write = [&](int px, int py, Kahan<T> *acc, T weight) {
+        const size_t o = size_t(py) * size_t(W) + size_t(px);
+        const double inv = weight > 0 ? 1.0 / double(weight) : 1.0;
+        for (int k = 0; k < 3; ++k) {
+            if (out.rgb) out.rgb[3 * o + k] = double(acc[k].get()) * inv;
+            if (out.albedo) out.albedo[3 * o + k] = double(acc[3 + k].get()) * inv;
+            if (out.normal) out.normal[3 * o + k] = double(acc[6 + k].get()) * inv;
+        }
+        if (out.denoise) out.denoise[o] = double(acc[9].get()) * inv;
+    };
     auto worker = [&](int tid) {
         Counters &c = cs[size_t(tid)];
         for (;;) {
@@ -2343,8 +2372,63 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
             if (task >= tiles_x * tiles_y) break;
             const int tx = task % tiles_x, ty = task / tiles_x;
             const int x0 = tx * tile, y0 = row_begin + ty * tile;
-            for (int py = y0; py < std::min(y0 + tile, row_end); ++py)
-                for (int px = x0; px < std::min(x0 + tile, W); ++px) {
+            const int x1 = std::min(x0 + tile, W), y1 = std::min(y0 + tile, row_end);
+            if (splat) {
+                const int tw = x1 - x0, th = y1 - y0;
+                std::vector<Kahan<T>> acc(size_t(tw) * th * 10);
+                std::vector<T> wsum(size_t(tw) * th, T(0));
+                const int lx = int(std::floor(float(x0) + 0.5f - radius_f));
+                const int ly = int(std::floor(float(y0) + 0.5f - radius_f));
+                const int hx = int(std::floor(float(x1 - 1) + 0.49999f + radius_f));
+                const int hy = int(std::floor(float(y1 - 1) + 0.49999f + radius_f));
+                for (int qy = ly; qy <= hy; ++qy)
+                    for (int qx = lx; qx <= hx; ++qx) {
+                        const uint64_t key = uint64_t(qy + b0) * uint64_t(EW) + uint64_t(qx + b0);
+                        for (int s = 0; s < rp.spp; ++s) {
+                            Sampler<T> sam, trs;
+                            sam.rng.seed_path(rp.seed, key, uint64_t(rp.sample_begin + s));
+                            trs.rng.seed_path(rp.seed ^ TR_STREAM_KEY, key, uint64_t(rp.sample_begin + s));
+                            const float fu = sam.rng.uniform(), fv = sam.rng.uniform();
+                            const float sx = float(qx) + fu, sy = float(qy) + fv;
+                            const T au = sam.u1(), av = sam.u1();
+                            const Ray<T> ray = cam.sample(T(sx / float(W)), T(sy / float(H)), au, av);
+                            const PixelResult<T> pr = trace_std(tp, scene, ray, sam, trs, &c);
+                            if (!pr.value.is_finite()) continue;
+                            // apply_image_filter over this tile's pixels
+                            const int xa = std::max(x0, int(std::ceil(sx - radius_f - 0.5f)));
+                            const int ya = std::max(y0, int(std::ceil(sy - radius_f - 0.5f)));
+                            const int xb = std::min(x1 - 1, int(std::floor(sx + radius_f - 0.5f)));
+                            const int yb = std::min(y1 - 1, int(std::floor(sy + radius_f - 0.5f)));
+                            for (int y = ya; y <= yb; ++y) {
+                                const float yr = std::abs(float(y) + 0.5f - sy);
+                                if (yr > radius_f) continue;
+                                for (int x = xa; x <= xb; ++x) {
+                                    const float xr = std::abs(float(x) + 0.5f - sx);
+                                    if (xr > radius_f) continue;
+                                    const T w = gaussian ? T(gaussian_1d(xr, rp.filter_alpha, g_exp) *
+                                                             gaussian_1d(yr, rp.filter_alpha, g_exp))
+                                                         : T(1);
+                                    const size_t o = size_t(y - y0) * size_t(tw) + size_t(x - x0);
+                                    Kahan<T> *a = &acc[10 * o];
+                                    for (int k = 0; k < 3; ++k) {
+                                        a[k].add(w * pr.value[k]); a[3 + k].add(w * pr.albedo[k]);
+                                        a[6 + k].add(w * pr.normal[k]);
+                                    }
+                                    a[9].add(w * pr.denoise);
+                                    wsum[o] += w;
+                                }
+                            }
+                        }
+                    }
+                for (int y = y0; y < y1; ++y)
+                    for (int x = x0; x < x1; ++x) {
+                        const size_t o = size_t(y - y0) * size_t(tw) + size_t(x - x0);
+                        write(x, y, &acc[10 * o], wsum[o]);
+                    }
+                continue;
+            }
+            for (int py = y0; py < y1; ++py)
+                for (int px = x0; px < x1; ++px) {
                     Kahan<T> acc[10];
                     T weight = 0;
                     const uint64_t pix = uint64_t(py) * uint64_t(W) + uint64_t(px);
@@ -2370,14 +2454,7 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                         }
                         acc[9].add(pr.denoise);
                     }
-                    const size_t o = size_t(py) * size_t(W) + size_t(px);
-                    const double inv = weight > 0 ? 1.0 / double(weight) : 1.0;
-                    for (int k = 0; k < 3; ++k) {
-                        if (out.rgb) out.rgb[3 * o + k] = double(acc[k].get()) * inv;
-                        if (out.albedo) out.albedo[3 * o + k] = double(acc[3 + k].get()) * inv;
-                        if (out.normal) out.normal[3 * o + k] = double(acc[6 + k].get()) * inv;
-                    }
-                    if (out.denoise) out.denoise[o] = double(acc[9].get()) * inv;
+                    write(px, py, acc, weight);
                 }
         }
     };

@@ -227,3 +227,15 @@ def test_disney_creator_defaults():
     m2 = h2.desc().materials[0]
     assert (m2.transmission_roughness, m2.ior) == pytest.approx((0.1, 1.33))
     np.testing.assert_allclose(list(m2.color2), [0.5, 0.6, 0.7], rtol=1e-6)
+
+
+def test_gaussian_film_filter_reaches_the_session():
+    text = json.loads(wrap([quad_json([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])]))
+    text["rendering"]["film_filter"] = {"type": "gaussian", "radius": 1.5, "alpha": 2}
+    h = capi.HostScene(json.dumps(text), ".")
+    s = h.session(0)
+    assert s.filter == capi.FILTER_GAUSSIAN and s.filter_radius == 1.5 and s.filter_alpha == 2.0
+    text["rendering"]["film_filter"] = {"type": "gaussian", "radius": 1.5, "alpha": -1}
+    with pytest.raises(RuntimeError) as ei:
+        capi.HostScene(json.dumps(text), ".")
+    assert "invalid alpha value" in str(ei.value)
