@@ -26,7 +26,8 @@ MEDIUM_VOID, MEDIUM_HOMOGENEOUS, MEDIUM_HETEROGENEOUS = range(3)
 TEX3D_CONSTANT, TEX3D_GRID = range(2)
 WRAP_CLAMP, WRAP_REPEAT, WRAP_MIRROR = range(3)
 MAT_IDEAL_DIFFUSE, MAT_METAL, MAT_GLASS, MAT_MIRROR, MAT_IDEAL_BLACK, MAT_INVISIBLE_SURFACE, MAT_DISNEY = range(7)
-ENV_NONE, ENV_NATIVE_SKY = 0, 1
+ENV_NONE, ENV_NATIVE_SKY, ENV_IBL = 0, 1, 2
+TEX2D_CONSTANT, TEX2D_IMAGE, TEX2D_CHECKER = 0, 1, 2
 FILTER_BOX, FILTER_GAUSSIAN = 0, 1
 ACCEL_ATRC, ACCEL_BVH4, ACCEL_BVH4_WATERTIGHT = 0, 1, 2
 AGGREGATE_NATIVE, AGGREGATE_BVH = 0, 1
@@ -73,8 +74,17 @@ class agx_medium_desc(C.Structure):
     ]
 
 
+class agx_texture2d_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("linear", C.c_int32), ("width", C.c_int32), ("height", C.c_int32),
+        ("data", C.POINTER(C.c_float)), ("value", F3), ("value2", F3), ("grid_count", C.c_int32),
+        ("transform", C.c_float * 6), ("wrap", C.c_int32 * 2), ("inv_gamma", C.c_float),
+    ]
+
+
 class agx_env_desc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("top", F3), ("bottom", F3), ("power", C.c_float)]
+    _fields_ = [("kind", C.c_int32), ("top", F3), ("bottom", F3), ("power", C.c_float), ("tex", agx_texture2d_desc),
+                ("no_importance_sampling", C.c_int32)]
 
 
 class agx_camera_desc(C.Structure):

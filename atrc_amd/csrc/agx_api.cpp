@@ -378,11 +378,16 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     DevMaterial *mats = nullptr;
     vec4 *bn = nullptr, *bp = nullptr, *bi = nullptr, *tn = nullptr, *b4n = nullptr, *b4p = nullptr;
     int32_t *tp = nullptr, *ai = nullptr;
-    float *ap = nullptr, *gr = nullptr;
+    float *ap = nullptr, *gr = nullptr, *etx = nullptr, *epr = nullptr, *eap = nullptr;
+    int32_t *eai = nullptr;
     DevMedium *md = nullptr;
     auto track = [&](void *p) { if (p) c->scene_allocs.push_back(p); };
     hipError_t e = hipSuccess;
     if (e == hipSuccess) { e = upload(&ents, hs.ents, c->stream); track(ents); }
+    if (e == hipSuccess) { e = upload(&etx, hs.env_texels, c->stream); track(etx); }
+    if (e == hipSuccess) { e = upload(&epr, hs.env_probs, c->stream); track(epr); }
+    if (e == hipSuccess) { e = upload(&eap, hs.env_alias_p, c->stream); track(eap); }
+    if (e == hipSuccess) { e = upload(&eai, hs.env_alias_i, c->stream); track(eai); }
     if (e == hipSuccess) { e = upload(&mats, hs.mats, c->stream); track(mats); }
     if (e == hipSuccess) { e = upload(&bn, hs.blas_nodes, c->stream); track(bn); }
     if (e == hipSuccess) { e = upload(&bp, hs.blas_prims, c->stream); track(bp); }
@@ -444,6 +449,13 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     for (int i = 0; i < MAX_LIGHTS; ++i) ds.lights[i] = i < ds.n_lights ? hs.lights[size_t(i)] : -2;
     ds.sky_top = hs.sky_top;
     ds.sky_bottom = hs.sky_bottom;
+    ds.env_tex = hs.env_tex;
+    ds.env_texels = etx;
+    ds.env_probs = epr;
+    ds.env_alias_p = eap;
+    ds.env_alias_i = eai;
+    ds.env_pw = hs.env_pw;
+    ds.env_ph = hs.env_ph;
     ds.world = hs.world;
     c->ds = ds;
     c->n_lights = ds.n_lights;

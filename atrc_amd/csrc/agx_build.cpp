@@ -269,6 +269,121 @@ void set_rows(vec4 *rows, const float *m) {
     for (int r = 0; r < 3; ++r) rows[r] = vec4{m[4 * r], m[4 * r + 1], m[4 * r + 2], m[4 * r + 3]};
 }
 
+// Texture2D lookup on the host, the same FP32 arithmetic as tex2d_env on the device
+float wrap2(int mode, float x) {
+    if (mode == AGX_WRAP_REPEAT) return std::min(1.0f, std::max(0.0f, x - std::floor(x)));
+    if (mode == AGX_WRAP_MIRROR) {
+        const int ip = int(x >= 0 ? x : std::floor(x));
+        const float v = (ip & 1) ? 1 - x + float(ip) : x - float(ip);
+        return std::min(1.0f, std::max(0.0f, v));
+    }
+    return std::min(1.0f, std::max(0.0f, x));
+}
+F3 tex2d_host(const DevTex2D &t, const std::vector<float> &texels, float u0, float v0) {
+    const float qu = t.xf[0].x * u0 + t.xf[0].y * v0 + t.xf[0].z;
+    const float qv = t.xf[1].x * u0 + t.xf[1].y * v0 + t.xf[1].z;
+    const float u = wrap2(t.wrap[0], qu), v = wrap2(t.wrap[1], qv);
+    auto texel = [&](int x, int y) {
+        const float *g = texels.data() + (size_t(y) * size_t(t.w) + size_t(x)) * 3;
+        return F3{g[0], g[1], g[2]};
+    };
+    F3 r;
+    if (t.kind == AGX_TEX2D_CONSTANT) {
+        r = F3{t.value.x, t.value.y, t.value.z};
+    } else if (t.kind == AGX_TEX2D_CHECKER) {
+        const float gs = 1.0f / float(t.grid_count);
+        const int ui = int(std::min(1.0f, std::max(0.0f, u)) / gs), vi = int(std::min(1.0f, std::max(0.0f, v)) / gs);
+        r = ((ui + vi) & 1) ? F3{t.value.x, t.value.y, t.value.z} : F3{t.value2.x, t.value2.y, t.value2.z};
+    } else if (!t.linear) {
+        r = texel(std::min(t.w - 1, int(u * float(t.w))), std::min(t.h - 1, int(v * float(t.h))));
+    } else {
+        const float fu = u * float(t.w) - 0.5f, fv = v * float(t.h) - 0.5f;
+        const float bu = std::floor(fu), bv = std::floor(fv);
+        const float du = fu - bu, dv = fv - bv;
+        const int x0 = std::min(t.w - 1, std::max(0, int(bu))), x1 = std::min(t.w - 1, std::max(0, int(bu) + 1));
+        const int y0 = std::min(t.h - 1, std::max(0, int(bv))), y1 = std::min(t.h - 1, std::max(0, int(bv) + 1));
+        auto lerp = [](F3 a, F3 b, float k) { return add(a, mul(sub(b, a), k)); };
+        r = lerp(lerp(texel(x0, y0), texel(x1, y0), du), lerp(texel(x0, y1), texel(x1, y1), du), dv);
+    }
+    if (t.inv_gamma != 1.0f) r = F3{std::pow(r.x, t.inv_gamma), std::pow(r.y, t.inv_gamma), std::pow(r.z, t.inv_gamma)};
+    return r;
+}
+
+// IBL + EnvironmentLightSampler construction (ibl.cpp:24-60, env_sampler.h:17-110):
+// patches of at most 200 x 200 over the texture, each weighted by the summed
+// luminance of the texels around it (one texel margin) times its solid angle,
+// normalised, then an alias table.  no_importance_sampling samples a constant
+// 1x1 texture instead: one patch, uniform over the sphere.
+int build_ibl(const agx_env_desc &env, HostScene *S, std::string *err) {
+    const agx_texture2d_desc &t = env.tex;
+    DevTex2D &o = S->env_tex;
+    o = DevTex2D{};
+    o.kind = t.kind;
+    o.linear = t.linear;
+    o.grid_count = t.grid_count;
+    o.inv_gamma = t.inv_gamma > 0 ? t.inv_gamma : 1.0f;
+    o.wrap[0] = t.wrap[0];
+    o.wrap[1] = t.wrap[1];
+    o.value = vec4{t.value[0], t.value[1], t.value[2], 0};
+    o.value2 = vec4{t.value2[0], t.value2[1], t.value2[2], 0};
+    o.xf[0] = vec4{t.transform[0], t.transform[1], t.transform[2], 0};
+    o.xf[1] = vec4{t.transform[3], t.transform[4], t.transform[5], 0};
+    for (int k = 0; k < 2; ++k)
+        if (t.wrap[k] < AGX_WRAP_CLAMP || t.wrap[k] > AGX_WRAP_MIRROR) { *err = "ibl: invalid wrap mode"; return AGX_E_INVALID_ARG; }
+    if (t.kind == AGX_TEX2D_IMAGE) {
+        if (!t.data || t.width <= 0 || t.height <= 0 || int64_t(t.width) * t.height > (int64_t(1) << 28)) {
+            *err = "ibl: invalid image texture";
+            return AGX_E_INVALID_ARG;
+        }
+        o.w = t.width;
+        o.h = t.height;
+        S->env_texels.assign(t.data, t.data + size_t(t.width) * size_t(t.height) * 3);
+    } else if (t.kind == AGX_TEX2D_CONSTANT || t.kind == AGX_TEX2D_CHECKER) {
+        if (t.kind == AGX_TEX2D_CHECKER && t.grid_count <= 0) { *err = "ibl: invalid grid_count value"; return AGX_E_INVALID_ARG; }
+        o.w = o.h = 1;
+        S->env_texels.assign(3, 0.0f);
+    } else {
+        *err = "ibl: unknown texture kind";
+        return AGX_E_INVALID_ARG;
+    }
+    const bool uniform = env.no_importance_sampling != 0;
+    const int width = uniform ? 1 : o.w, height = uniform ? 1 : o.h;
+    const int nw = std::min(width, 200), nh = std::min(height, 200);
+    S->env_pw = nw;
+    S->env_ph = nh;
+    S->env_probs.assign(size_t(nw) * nh, 0.0f);
+    const float pi = 3.14159265358979323846f;
+    float lum_sum = 0;
+    for (int y = 0; y < nh; ++y) {
+        const float y0 = float(y) / float(nh), y1 = float(y + 1) / float(nh);
+        const int ys = std::max(0, int(std::floor(y0 * float(height)) - 1));
+        const int yl = std::min(height - 1, int(std::floor(y1 * float(height)) + 1));
+        for (int x = 0; x < nw; ++x) {
+            const float x0 = float(x) / float(nw), x1 = float(x + 1) / float(nw);
+            const int xs = std::max(0, int(std::floor(x0 * float(width)) - 1));
+            const int xl = std::min(width - 1, int(std::floor(x1 * float(width)) + 1));
+            float lum = 0;
+            for (int yy = ys; yy <= yl; ++yy)
+                for (int xx = xs; xx <= xl; ++xx) {
+                    const float su = (float(xx) + 0.5f) / float(width), sv = (float(yy) + 0.5f) / float(height);
+                    const F3 c = uniform ? F3{1, 1, 1} : tex2d_host(o, S->env_texels, su, sv);
+                    lum += 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+                }
+            const float area = std::abs(2 * pi * (x1 - x0) * (std::cos(pi * y1) - std::cos(pi * y0)));
+            S->env_probs[size_t(y) * nw + x] = lum * area;
+            lum_sum += lum * area;
+        }
+    }
+    if (lum_sum > 0)
+        for (float &p : S->env_probs) p *= 1 / lum_sum;
+    else
+        std::fill(S->env_probs.begin(), S->env_probs.end(), 1.0f / float(S->env_probs.size()));
+    S->env_alias_p.clear();
+    S->env_alias_i.clear();
+    build_alias(S->env_probs, &S->env_alias_p, &S->env_alias_i);
+    return AGX_OK;
+}
+
 F3 apply_point(const float *m, F3 p) {
     return {m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
             m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]};
@@ -568,7 +683,14 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
 
     // light list: environment first, then emissive entities (scene_default.cpp:66-82)
     if (desc.env.kind != AGX_ENV_NONE) {
-        if (desc.env.kind != AGX_ENV_NATIVE_SKY) { *err = "unsupported environment light"; return AGX_E_UNSUPPORTED; }
+        if (desc.env.kind != AGX_ENV_NATIVE_SKY && desc.env.kind != AGX_ENV_IBL) {
+            *err = "unsupported environment light";
+            return AGX_E_UNSUPPORTED;
+        }
+        if (desc.env.kind == AGX_ENV_IBL) {
+            const int r = build_ibl(desc.env, &S, err);
+            if (r != AGX_OK) return r;
+        }
         S.lights.push_back(-1);
     }
     for (uint32_t i = 0; i < desc.entity_count; ++i) {

@@ -26,6 +26,11 @@ struct HostScene {
     int32_t aggregate = AGX_AGGREGATE_NATIVE;
     int32_t env_kind = AGX_ENV_NONE;
     vec4 sky_top{0, 0, 0, 0}, sky_bottom{0, 0, 0, 0}, world{0, 0, 0, 1};
+    // ibl environment: texture, its texels, and the patch sampler
+    DevTex2D env_tex{};
+    std::vector<float> env_texels, env_probs, env_alias_p;
+    std::vector<int32_t> env_alias_i;
+    int32_t env_pw = 0, env_ph = 0;
     int max_blas_depth = 0;
     int max_tlas_depth = 0;
     uint64_t total_tris = 0;

@@ -78,6 +78,16 @@ struct DevTex3D {
     vec4 xf[3];            // rows of the uvw transform
 };
 
+// Texture2D (constant / hdr image / checker board) of an ibl environment
+struct DevTex2D {
+    int32_t kind, linear, w, h;
+    int32_t wrap[2];
+    int32_t grid_count;
+    float inv_gamma;
+    vec4 value, value2;
+    vec4 xf[2];            // rows of the uv transform (2x3)
+};
+
 struct DevMedium {
     int32_t kind;          // AGX_MEDIUM_*
     int32_t max_scat;
@@ -129,6 +139,14 @@ struct DevScene {
     const vec4 *b4_prims;
     int32_t lights[MAX_LIGHTS];
     vec4 sky_top, sky_bottom;
+    // ibl environment (AGX_ENV_IBL): texture, and the importance sampler over
+    // env_pw x env_ph patches (normalised patch probabilities + alias table)
+    DevTex2D env_tex;
+    const float *env_texels;
+    const float *env_probs;
+    const float *env_alias_p;
+    const int32_t *env_alias_i;
+    int32_t env_pw, env_ph;
     vec4 world;            // (centre.xyz, radius)
 };
 

@@ -1654,6 +1654,100 @@ __device__ __forceinline__ float wrap3(int mode, float x) {
     }
     return smin(1.0f, smax(0.0f, x));
 }
+// ---------------------------------------------------------------- Texture2D and the ibl environment
+// Texture2D::sample_spectrum (core/texture2d.h:104-125): uv transform, wrap per
+// axis, the texture (constant2d.cpp; checker_board.cpp:14-24; hdr.cpp:16-36 with
+// nearest or bilinear filtering over texel centres), then ^inv_gamma
+__device__ __forceinline__ V texel2(const DevScene &s, int x, int y) {
+    const float *g = s.env_texels + (size_t(y) * size_t(s.env_tex.w) + size_t(x)) * 3;
+    return mk(g[0], g[1], g[2]);
+}
+__device__ V tex2d_env(const DevScene &s, float u0, float v0) {
+    const DevTex2D &t = s.env_tex;
+    const float qu = t.xf[0].x * u0 + t.xf[0].y * v0 + t.xf[0].z;
+    const float qv = t.xf[1].x * u0 + t.xf[1].y * v0 + t.xf[1].z;
+    const float u = wrap3(t.wrap[0], qu), v = wrap3(t.wrap[1], qv);
+    V r;
+    if (t.kind == AGX_TEX2D_CONSTANT) {
+        r = mk(t.value);
+    } else if (t.kind == AGX_TEX2D_CHECKER) {
+        const float gs = 1.0f / float(t.grid_count);
+        const int ui = int(smin(1.0f, smax(0.0f, u)) / gs), vi = int(smin(1.0f, smax(0.0f, v)) / gs);
+        r = ((ui + vi) & 1) ? mk(t.value) : mk(t.value2);
+    } else if (!t.linear) {
+        const int x = min(t.w - 1, int(u * float(t.w))), y = min(t.h - 1, int(v * float(t.h)));
+        r = texel2(s, x, y);
+    } else {
+        const float fu = u * float(t.w) - 0.5f, fv = v * float(t.h) - 0.5f;
+        const float bu = floorf(fu), bv = floorf(fv);
+        const float du = fu - bu, dv = fv - bv;
+        const int x0 = min(t.w - 1, max(0, int(bu))), x1 = min(t.w - 1, max(0, int(bu) + 1));
+        const int y0 = min(t.h - 1, max(0, int(bv))), y1 = min(t.h - 1, max(0, int(bv) + 1));
+        auto lerp = [](V a, V b, float k) { return a + (b - a) * k; };
+        r = lerp(lerp(texel2(s, x0, y0), texel2(s, x1, y0), du), lerp(texel2(s, x0, y1), texel2(s, x1, y1), du), dv);
+    }
+    if (t.inv_gamma != 1.0f) r = mk(powf(r.x, t.inv_gamma), powf(r.y, t.inv_gamma), powf(r.z, t.inv_gamma));
+    return r;
+}
+// spherical angles of a unit direction: phi = atan2(y, x) in [0, 2 pi), theta = acos(z)
+__device__ __forceinline__ float dir_phi(V d) {
+    const float p = atan2f(d.y, d.x);
+    return p < 0 ? p + 2 * PI_F : p;
+}
+// radiance arriving from direction d (native_sky.cpp:13-20; IBL::radiance, ibl.cpp:160-170)
+__device__ V env_radiance(const DevScene &s, V d) {
+    if (s.env_kind == AGX_ENV_NATIVE_SKY) return sky(s, d);
+    if (s.env_kind != AGX_ENV_IBL) return mk(0, 0, 0);
+    const V dir = normalize(d);
+    const float theta = acosf(smin(1.0f, smax(-1.0f, dir.z)));
+    const float u = smin(1.0f, smax(0.0f, dir_phi(dir) / (2 * PI_F)));
+    const float v = smin(1.0f, smax(0.0f, theta / PI_F));
+    return tex2d_env(s, u, v);
+}
+// solid-angle pdf of one patch (EnvironmentLightSampler, env_sampler.h:125-183)
+__device__ __forceinline__ float env_patch_pdf(const DevScene &s, int px, int py, float *cvmin, float *cvmax,
+                                               float *u0, float *u1) {
+    *u0 = float(px) / float(s.env_pw);
+    *u1 = float(px + 1) / float(s.env_pw);
+    const float v0 = float(py) / float(s.env_ph), v1 = float(py + 1) / float(s.env_ph);
+    const float c1 = cosf(PI_F * v1), c0 = cosf(PI_F * v0);
+    *cvmin = smin(c1, c0);
+    *cvmax = smax(c1, c0);
+    const float in_patch = 1.0f / (2 * PI_F * ((*u1 - *u0) * (*cvmax - *cvmin)));
+    return s.env_probs[py * s.env_pw + px] * in_patch;
+}
+__device__ float env_pdf(const DevScene &s, V d) {
+    if (s.env_kind != AGX_ENV_IBL) return SPHERE_PDF;
+    const V dir = normalize(d);
+    const float theta = acosf(smin(1.0f, smax(-1.0f, dir.z)));
+    const float u = dir_phi(dir) / (2 * PI_F), v = theta / PI_F;
+    const int px = min(s.env_pw - 1, max(0, int(floorf(u * float(s.env_pw)))));
+    const int py = min(s.env_ph - 1, max(0, int(floorf(v * float(s.env_ph)))));
+    float cvmin, cvmax, u0, u1;
+    return env_patch_pdf(s, px, py, &cvmin, &cvmax, &u0, &u1);
+}
+// direction towards the environment and its pdf (NativeSky: uniform sphere;
+// IBL: alias-sampled patch, then uniform in cos theta and phi within it)
+__device__ V env_sample(const DevScene &s, float su, float sv, float sw, float *pdf) {
+    if (s.env_kind != AGX_ENV_IBL) {
+        *pdf = SPHERE_PDF;
+        return uniform_sphere(su, sv);
+    }
+    const int32_t n = s.env_pw * s.env_ph;
+    const float nu = su * float(n);
+    int32_t idx = int32_t(nu);
+    if (idx >= n) idx = n - 1;
+    const float frac = nu - float(idx);
+    const int32_t pick = frac < s.env_alias_p[idx] ? idx : s.env_alias_i[idx];
+    const int px = pick % s.env_pw, py = pick / s.env_pw;
+    float cvmin, cvmax, u0, u1;
+    *pdf = env_patch_pdf(s, px, py, &cvmin, &cvmax, &u0, &u1);
+    const float ct = cvmin + sw * (cvmax - cvmin);
+    const float st = sqrtf(smax(0.0f, 1 - ct * ct));
+    const float phi = 2 * PI_F * (u0 * (1 - sv) + u1 * sv);
+    return mk(st * cosf(phi), st * sinf(phi), ct);
+}
+
 __device__ __forceinline__ V texel3(const DevScene &s, const DevTex3D &t, int x, int y, int z) {
     const float *g = s.grids + t.offset + ((size_t(z) * size_t(t.h) + size_t(y)) * size_t(t.w) + size_t(x)) * size_t(t.ch);
     if (t.ch == 1) return mk(g[0], g[0], g[0]);
@@ -1825,11 +1919,11 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     const V o = mk(pb.ray_o[2 * p + 1]), d = mk(pb.ray_d[2 * p + 1]);
     if (ent < 0) {
         if (s.env_kind == AGX_ENV_NONE) return mk(0, 0, 0);
-        const V lr = sky(s, d);
+        const V lr = env_radiance(s, d);
         if (black(lr)) return mk(0, 0, 0);
         const V f = lr * bf * absdot;
         if (delta) return coef * (f / pdf);
-        return coef * (f / (pdf + SPHERE_PDF));
+        return coef * (f / (pdf + env_pdf(s, d)));
     }
     const DevEntity &e = s.ents[ent];
     if (e.light < 0) return mk(0, 0, 0);
@@ -2014,7 +2108,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
     if (ent < 0) {  // path_tracing.cpp:41-49
-        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * sky(s, rd);
+        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance(s, rd);
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
         pb.rng_tr[p] = rtr.state;
@@ -2150,16 +2244,17 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                     }
                 }
             } else {
-                const V dir = uniform_sphere(s5[0], s5[1]);
+                float epdf;
+                const V dir = env_sample(s, s5[0], s5[1], s5[2], &epdf);
                 const V lpos = env_emit_pos(s, sp.pos, dir);
-                const V rad = sky(s, dir);
+                const V rad = env_radiance(s, dir);
                 if (!black(rad)) {
                     const V tl = normalize(lpos - sp.pos);
                     const V f = bsdf_eval<F>(b, tl, sp.wr);
                     if (!black(f)) {
                         const V ff = rad * f * fabsf(cosv(tl, sp.geo.z));
                         const float bp = bsdf_pdf<F>(b, tl, sp.wr);
-                        contrib = coef * (ff / (SPHERE_PDF + bp));
+                        contrib = coef * (ff / (epdf + bp));
                         valid = true;
                         // Scene::visible(A, B): [eps, |A-B| - eps] along normalize(B - A)
                         so = sp.pos; sd = normalize(lpos - sp.pos); tmax = length(sp.pos - lpos) - eps;

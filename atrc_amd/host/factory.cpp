@@ -218,8 +218,145 @@ namespace {
 
 void copy3(float *dst, const Vec3 &v) { dst[0] = v.x; dst[1] = v.y; dst[2] = v.z; }
 
+// material parameters on this path are constant textures
+Vec3 constant_texel(const Texture2D &t) {
+    if (t.type != "constant")
+        throw ObjectConstructionException("texture type '" + t.type + "' is not supported by gpu_pt materials");
+    return t.texel;
+}
+
 Vec3 texture_value(CreatingContext &ctx, const ConfigGroup &params, const std::string &name) {
-    return ctx.create<Texture2D>(params.child_group(name))->texel;
+    return constant_texel(*ctx.create<Texture2D>(params.child_group(name)));
+}
+
+// to_transform2 (utility/config.cpp:136-190): translate / rotate / scale in the uv
+// plane, kept as a Transform3 that leaves the third coordinate alone; an array
+// is the product of its elements
+Transform3 to_transform2(const ConfigNode &node) {
+    if (node.is_array()) {
+        const auto &arr = node.as_array();
+        Transform3 r;
+        for (size_t i = 0; i < arr.size(); ++i) r = r * to_transform2(arr.at(i));
+        return r;
+    }
+    const auto &g = node.as_group();
+    const std::string &type = g.child_str("type");
+    if (ends_with(type, "//")) return Transform3();
+    auto vec2 = [&](const std::string &name) {
+        const auto &a = g.child_array(name);
+        if (a.size() == 1) return std::pair<double, double>(a.at_real(0), a.at_real(0));
+        if (a.size() == 2) return std::pair<double, double>(a.at_real(0), a.at_real(1));
+        throw ConfigException("invalid array size for vec2 (" + std::to_string(a.size()) + ")");
+    };
+    if (type == "translate") {
+        const auto o = vec2("offset");
+        return Transform3::translate(o.first, o.second, 0);
+    }
+    if (type == "rotate") {
+        double rad;
+        if (auto p = g.find_child("rad")) rad = p->as_value().as_real();
+        else rad = g.child_real("deg") / 180 * 3.14159265358979323846f;
+        return Transform3::rotate(0, 0, 1, rad);
+    }
+    if (type == "scale") {
+        if (g.child("ratio").is_array()) {
+            const auto s = vec2("ratio");
+            return Transform3::scale(s.first, s.second, 1);
+        }
+        const double s = g.child_real("ratio");
+        return Transform3::scale(s, s, 1);
+    }
+    throw ConfigException("unknown transform2 type (type = " + type + ")");
+}
+
+// Texture2DCommonParams (core/texture2d.h:12-50): inv_v, inv_u, swap_uv, then the
+// user transform (applied first); wrap modes and inv_gamma
+void tex2d_common(const ConfigGroup &p, agx_texture2d_desc *d) {
+    Transform3 ret;
+    if (p.child_int_or("inv_v", 0)) ret = ret * (Transform3::translate(0, 1, 0) * Transform3::scale(1, -1, 1));
+    if (p.child_int_or("inv_u", 0)) ret = ret * (Transform3::translate(1, 0, 0) * Transform3::scale(-1, 1, 1));
+    if (p.child_int_or("swap_uv", 0)) {
+        Transform3 s;
+        s.m[0][0] = 0; s.m[0][1] = 1; s.m[1][0] = 1; s.m[1][1] = 0;
+        ret = ret * s;
+    }
+    if (auto n = p.find_child("transform")) ret = ret * to_transform2(*n);
+    const float row[6] = {float(ret.m[0][0]), float(ret.m[0][1]), float(ret.m[0][3]),
+                          float(ret.m[1][0]), float(ret.m[1][1]), float(ret.m[1][3])};
+    for (int i = 0; i < 6; ++i) d->transform[i] = row[i];
+    const auto wrap = [](const std::string &w, const char *axis) {
+        if (w == "clamp") return int32_t(AGX_WRAP_CLAMP);
+        if (w == "repeat") return int32_t(AGX_WRAP_REPEAT);
+        if (w == "mirror") return int32_t(AGX_WRAP_MIRROR);
+        throw ObjectConstructionException(std::string("invalid wrap_") + axis + " value: " + w +
+                                          " (expect clamp/repeat/mirror)");
+    };
+    d->wrap[0] = wrap(p.child_str_or("wrap_u", "clamp"), "u");
+    d->wrap[1] = wrap(p.child_str_or("wrap_v", "clamp"), "v");
+    d->inv_gamma = p.child_real_or("inv_gamma", 1);
+}
+
+// Radiance RGBE (.hdr) reader: "#?RADIANCE"/"#?RGBE" header, "-Y H +X W"
+// resolution, flat or new-style run-length scanlines; texel = mantissa *
+// 2^(e - 136), rows top to bottom as stored
+std::shared_ptr<std::vector<float>> load_hdr(const std::string &filename, int *w, int *h) {
+    std::ifstream f(filename, std::ios::binary);
+    if (!f) throw ObjectConstructionException("failed to load texture from " + filename);
+    std::string line;
+    std::getline(f, line);
+    if (line.rfind("#?", 0) != 0) throw ObjectConstructionException("not a radiance hdr file: " + filename);
+    for (;;) {
+        if (!std::getline(f, line)) throw ObjectConstructionException("truncated hdr header: " + filename);
+        if (line.empty() || line == "\r") break;
+        if (line.rfind("FORMAT=", 0) == 0 && line.find("32-bit_rle_rgbe") == std::string::npos)
+            throw ObjectConstructionException("unsupported hdr format: " + line);
+    }
+    std::getline(f, line);
+    char ya[3] = {0}, xa[3] = {0};
+    int H = 0, W = 0;
+    if (std::sscanf(line.c_str(), "%2s %d %2s %d", ya, &H, xa, &W) != 4 || std::string(ya) != "-Y" ||
+        std::string(xa) != "+X" || W <= 0 || H <= 0 || int64_t(W) * H > (int64_t(1) << 28))
+        throw ObjectConstructionException("unsupported hdr resolution line: " + line);
+    auto out = std::make_shared<std::vector<float>>(size_t(W) * size_t(H) * 3);
+    std::vector<unsigned char> scan(size_t(W) * 4);
+    auto byte = [&]() {
+        const int c = f.get();
+        if (c == EOF) throw ObjectConstructionException("truncated hdr data: " + filename);
+        return static_cast<unsigned char>(c);
+    };
+    for (int y = 0; y < H; ++y) {
+        unsigned char head[4] = {byte(), byte(), byte(), byte()};
+        if (W >= 8 && W < 32768 && head[0] == 2 && head[1] == 2 && ((head[2] << 8) | head[3]) == W) {
+            for (int c = 0; c < 4; ++c)
+                for (int x = 0; x < W;) {
+                    int n = byte();
+                    if (n > 128) {
+                        n -= 128;
+                        const unsigned char v = byte();
+                        if (x + n > W) throw ObjectConstructionException("bad hdr run length: " + filename);
+                        for (int k = 0; k < n; ++k) scan[size_t(x++) * 4 + size_t(c)] = v;
+                    } else {
+                        if (n == 0 || x + n > W) throw ObjectConstructionException("bad hdr run length: " + filename);
+                        for (int k = 0; k < n; ++k) scan[size_t(x++) * 4 + size_t(c)] = byte();
+                    }
+                }
+        } else {
+            for (int c = 0; c < 4; ++c) scan[size_t(c)] = head[c];
+            for (int x = 1; x < W; ++x)
+                for (int c = 0; c < 4; ++c) scan[size_t(x) * 4 + size_t(c)] = byte();
+        }
+        for (int x = 0; x < W; ++x) {
+            const unsigned char *p = &scan[size_t(x) * 4];
+            const float s = p[3] ? std::ldexp(1.0f, int(p[3]) - 136) : 0.0f;
+            float *o = out->data() + (size_t(y) * size_t(W) + size_t(x)) * 3;
+            o[0] = float(p[0]) * s;
+            o[1] = float(p[1]) * s;
+            o[2] = float(p[2]) * s;
+        }
+    }
+    *w = W;
+    *h = H;
+    return out;
 }
 
 void set_transforms(agx_entity_desc &d, const Transform3 &l2w) {
@@ -304,7 +441,7 @@ std::shared_ptr<std::vector<float>> load_texels(const std::string &filename, boo
 // helper::child_texture_or_constant (factory/include/agz/factory/help.h:11-19): the
 // constant texel of child `name`, or `def` when the child is absent
 Vec3 texture_value_or(CreatingContext &ctx, const ConfigGroup &params, const std::string &name, float def) {
-    if (auto n = params.find_child_group(name)) return ctx.create<Texture2D>(*n)->texel;
+    if (auto n = params.find_child_group(name)) return constant_texel(*ctx.create<Texture2D>(*n));
     return Vec3{def, def, def};
 }
 
@@ -316,13 +453,45 @@ void reject_normal_map(const ConfigGroup &params) {
 }  // namespace
 
 CreatingContext::CreatingContext() {
-    // texture2d: only constant textures reach the GPU path (texture2d_creators.cpp:50-65)
+    // texture2d (texture2d_creators.cpp:10-166): constant, hdr and checker_board
+    // (materials take constant textures only; hdr / checker feed the ibl light)
     texture2d.add_creator("constant", [](const ConfigGroup &p, CreatingContext &) {
         auto t = std::make_shared<Texture2D>();
         t->texel = p.child_spectrum("texel");
+        tex2d_common(p, &t->desc);
+        t->desc.kind = AGX_TEX2D_CONSTANT;
+        copy3(t->desc.value, t->texel);
         return t;
     });
-    for (const char *unsupported : {"image", "hdr", "checker_board", "solid_image"})
+    texture2d.add_creator("checker_board", [](const ConfigGroup &p, CreatingContext &) {
+        auto t = std::make_shared<Texture2D>();
+        t->type = "checker_board";
+        tex2d_common(p, &t->desc);
+        t->desc.kind = AGX_TEX2D_CHECKER;
+        t->desc.grid_count = p.child_int("grid_count");
+        if (t->desc.grid_count <= 0)
+            throw ObjectConstructionException("invalid grid_count value: " + std::to_string(t->desc.grid_count));
+        copy3(t->desc.value, p.child_spectrum("color1"));
+        copy3(t->desc.value2, p.child_spectrum("color2"));
+        return t;
+    });
+    texture2d.add_creator("hdr", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto t = std::make_shared<Texture2D>();
+        t->type = "hdr";
+        tex2d_common(p, &t->desc);
+        const std::string sample = p.child_str_or("sample", "linear");
+        if (sample != "linear" && sample != "nearest") throw ObjectConstructionException("invalid sample method");
+        int w = 0, h = 0;
+        const std::string file = ctx.path_mapper ? ctx.path_mapper->map(p.child_str("filename")) : p.child_str("filename");
+        t->image = load_hdr(file, &w, &h);
+        t->desc.kind = AGX_TEX2D_IMAGE;
+        t->desc.linear = sample == "linear" ? 1 : 0;
+        t->desc.width = w;
+        t->desc.height = h;
+        t->desc.data = t->image->data();
+        return t;
+    });
+    for (const char *unsupported : {"image", "solid_image"})
         texture2d.add_creator(unsupported, [n = std::string(unsupported)](const ConfigGroup &, CreatingContext &)
                                                -> std::shared_ptr<Texture2D> {
             throw ObjectConstructionException("texture type '" + n + "' is not supported by gpu_pt");
@@ -583,8 +752,15 @@ CreatingContext::CreatingContext() {
         l->desc.power = p.child_real_or("power", -1);
         return l;
     });
-    envir_light.add_creator("ibl", [](const ConfigGroup &, CreatingContext &) -> std::shared_ptr<EnvirLight> {
-        throw ObjectConstructionException("environment light 'ibl' is not supported by gpu_pt yet");
+    // IBLCreator (envir_light_creators.cpp:11-31)
+    envir_light.add_creator("ibl", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto l = std::make_shared<EnvirLight>();
+        l->desc.kind = AGX_ENV_IBL;
+        l->tex = ctx.create<Texture2D>(p.child_group("tex"));
+        l->desc.tex = l->tex->desc;
+        l->desc.no_importance_sampling = p.child_int_or("no_importance_sampling", 0) != 0 ? 1 : 0;
+        l->desc.power = p.child_real_or("power", -1);
+        return l;
     });
 
     // aggregate (aggregate_creators.cpp)

@@ -22,7 +22,10 @@ public:
 
 // ---------------------------------------------------------------- scene objects
 struct Texture2D {
-    Vec3 texel;                      // "constant" textures only
+    std::string type = "constant";  // "constant", "hdr" or "checker_board"
+    Vec3 texel;                      // "constant" texel
+    agx_texture2d_desc desc{};       // every kind; data points into image (hdr)
+    std::shared_ptr<std::vector<float>> image;
 };
 struct Geometry {
     agx_entity_desc proto{};         // geometry part of the entity descriptor
@@ -48,6 +51,7 @@ struct Entity {
 };
 struct EnvirLight {
     agx_env_desc desc{};
+    std::shared_ptr<Texture2D> tex;  // ibl: owns the texels desc.tex points to
 };
 struct Aggregate {
     int32_t kind = AGX_AGGREGATE_NATIVE;

@@ -118,7 +118,10 @@ class Entity:
 class Scene:
     entities: list = field(default_factory=list)
     materials: list = field(default_factory=list)
-    env: dict | None = None              # {"top": (r,g,b), "bottom": (r,g,b)}
+    # native_sky {"top": (r,g,b), "bottom": (r,g,b)}, or ibl {"image": (H, W, 3) float32
+    # | "texel": (r,g,b) | "checker": (grid_count, color1, color2), optional "linear",
+    # "wrap": (u, v), "transform": 2x3 row-major, "inv_gamma", "no_importance_sampling"}
+    env: dict | None = None
     aggregate: int = capi.AGGREGATE_NATIVE
     max_leaf_size: int = 5
     media: list = field(default_factory=list)
@@ -189,11 +192,35 @@ class Scene:
         desc.entities = C.cast(ents, C.POINTER(capi.agx_entity_desc))
         desc.material_count = len(self.materials)
         desc.materials = C.cast(mats, C.POINTER(capi.agx_material_desc))
-        if self.env is not None:
+        if self.env is not None and "top" in self.env:
             desc.env.kind = capi.ENV_NATIVE_SKY
             desc.env.top[:] = [float(x) for x in self.env["top"]]
             desc.env.bottom[:] = [float(x) for x in self.env["bottom"]]
             desc.env.power = -1.0
+        elif self.env is not None:
+            e, t = self.env, desc.env.tex
+            desc.env.kind = capi.ENV_IBL
+            desc.env.power = -1.0
+            desc.env.no_importance_sampling = 1 if e.get("no_importance_sampling") else 0
+            if "image" in e:
+                img = np.ascontiguousarray(e["image"], np.float32)
+                keep.append(img)
+                t.kind = capi.TEX2D_IMAGE
+                t.height, t.width = img.shape[0], img.shape[1]
+                t.data = img.ctypes.data_as(C.POINTER(C.c_float))
+            elif "checker" in e:
+                n, c1, c2 = e["checker"]
+                t.kind, t.grid_count = capi.TEX2D_CHECKER, int(n)
+                t.value[:] = [float(x) for x in c1]
+                t.value2[:] = [float(x) for x in c2]
+            else:
+                t.kind = capi.TEX2D_CONSTANT
+                t.value[:] = [float(x) for x in e["texel"]]
+            t.linear = 0 if e.get("linear") is False else 1
+            wrap = {"clamp": capi.WRAP_CLAMP, "repeat": capi.WRAP_REPEAT, "mirror": capi.WRAP_MIRROR}
+            t.wrap[:] = [wrap[w] for w in e.get("wrap", ("clamp", "clamp"))]
+            t.transform[:] = [float(x) for x in e.get("transform", (1, 0, 0, 0, 1, 0))]
+            t.inv_gamma = float(e.get("inv_gamma", 1.0))
         desc.aggregate = self.aggregate
         desc.max_leaf_size = self.max_leaf_size
         meds = (capi.agx_medium_desc * max(1, len(self.media)))()

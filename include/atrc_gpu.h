@@ -64,7 +64,8 @@ enum {
                                   anisotropic, ior, and the disney_* fields */
 };
 
-enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1 };
+enum { AGX_ENV_NONE = 0, AGX_ENV_NATIVE_SKY = 1, AGX_ENV_IBL = 2 };
+enum { AGX_TEX2D_CONSTANT = 0, AGX_TEX2D_IMAGE = 1, AGX_TEX2D_CHECKER = 2 };
 enum { AGX_FILTER_BOX = 0, AGX_FILTER_GAUSSIAN = 1 };
 enum { AGX_AGGREGATE_NATIVE = 0, AGX_AGGREGATE_BVH = 1 };
 /* Acceleration structure built at upload.
@@ -163,11 +164,30 @@ typedef struct agx_material_desc {
     float   clearcoat_gloss;
 } agx_material_desc;
 
+/* Atrc Texture2D (src/tracer/include/agz/tracer/core/texture2d.h): "constant",
+ * "hdr" (RGB float texels) and "checker_board".  A lookup at uv applies
+ * `transform` (Texture2DCommonParams::full_transform, inv_u / inv_v / swap_uv
+ * folded in), the wrap of each axis, then the texture, then ^inv_gamma. */
+typedef struct agx_texture2d_desc {
+    int32_t kind;              /* AGX_TEX2D_* */
+    int32_t linear;            /* image: 1 = bilinear over texel centres, 0 = nearest */
+    int32_t width, height;     /* image */
+    const float *data;         /* image: height*width*3 floats, x fastest, row 0 at v = 0 */
+    float   value[3];          /* constant texel; checker color1 */
+    float   value2[3];         /* checker color2 */
+    int32_t grid_count;        /* checker */
+    float   transform[6];      /* row-major 2x3 */
+    int32_t wrap[2];           /* AGX_WRAP_* for u, v */
+    float   inv_gamma;
+} agx_texture2d_desc;
+
 typedef struct agx_env_desc {
     int32_t kind;              /* AGX_ENV_* */
-    float   top[3];
+    float   top[3];            /* native_sky */
     float   bottom[3];
     float   power;             /* <= 0 = derived */
+    agx_texture2d_desc tex;    /* ibl (envir_light/ibl.cpp) */
+    int32_t no_importance_sampling; /* ibl: 1 = uniform sphere directions */
 } agx_env_desc;
 
 /* Thin-lens camera ("thin_lens"; "pinhole" maps onto it).  src/tracer/src/core/camera/thin_lens.cpp */

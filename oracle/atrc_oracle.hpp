@@ -1602,6 +1602,163 @@ template<class T> struct Material {
 
 // Texture3D lookup: transform, per-axis wrap, nearest or trilinear over texel
 // centres (agz-utils linear_sample3d / nearest_sample3d: unpinned), ^inv_gamma.
+// Texture2D (core/texture2d.h:90-125; texture2d/constant2d.cpp, checker_board.cpp:14-24,
+// hdr.cpp:16-36): uv transform, wrap per axis, texture, ^inv_gamma.  The
+// bilinear filter over texel centres stands for agz-utils linear_sample2d
+// (unpinned, as its 3D counterpart).
+template<class T> struct Texture2D {
+    int kind = AGX_TEX2D_CONSTANT;
+    bool linear = true;
+    int w = 1, h = 1, grid = 1;
+    std::vector<float> data;
+    V3<T> value, value2;
+    T xf[6] = {1, 0, 0, 0, 1, 0};
+    int wrap[2] = {0, 0};
+    T inv_gamma = 1;
+    void init(const agx_texture2d_desc &t) {
+        kind = t.kind;
+        linear = t.linear != 0;
+        value = V3<T>(T(t.value[0]), T(t.value[1]), T(t.value[2]));
+        value2 = V3<T>(T(t.value2[0]), T(t.value2[1]), T(t.value2[2]));
+        for (int i = 0; i < 6; ++i) xf[i] = T(t.transform[i]);
+        wrap[0] = t.wrap[0];
+        wrap[1] = t.wrap[1];
+        inv_gamma = T(t.inv_gamma > 0 ? t.inv_gamma : 1);
+        grid = t.grid_count;
+        if (kind == AGX_TEX2D_IMAGE) {
+            if (t.width <= 0 || t.height <= 0 || !t.data) throw std::runtime_error("invalid 2d texture");
+            w = t.width; h = t.height;
+            data.assign(t.data, t.data + size_t(w) * size_t(h) * 3);
+        } else if (kind == AGX_TEX2D_CHECKER) {
+            if (grid <= 0) throw std::runtime_error("invalid grid_count value");
+        } else if (kind != AGX_TEX2D_CONSTANT) {
+            throw std::runtime_error("unknown 2d texture");
+        }
+    }
+    static T wrapf(int mode, T x) {
+        if (mode == AGX_WRAP_REPEAT) return std::min(T(1), std::max(T(0), x - std::floor(x)));
+        if (mode == AGX_WRAP_MIRROR) {
+            const int ip = int(x >= 0 ? x : std::floor(x));
+            const T v = (ip & 1) ? 1 - x + T(ip) : x - T(ip);
+            return std::min(T(1), std::max(T(0), v));
+        }
+        return std::min(T(1), std::max(T(0), x));
+    }
+    int width() const { return kind == AGX_TEX2D_IMAGE ? w : 1; }
+    int height() const { return kind == AGX_TEX2D_IMAGE ? h : 1; }
+    V3<T> texel(int x, int y) const {
+        const size_t o = (size_t(y) * size_t(w) + size_t(x)) * 3;
+        return V3<T>(T(data[o]), T(data[o + 1]), T(data[o + 2]));
+    }
+    V3<T> sample(T u0, T v0) const {
+        const T u = wrapf(wrap[0], xf[0] * u0 + xf[1] * v0 + xf[2]);
+        const T v = wrapf(wrap[1], xf[3] * u0 + xf[4] * v0 + xf[5]);
+        V3<T> r;
+        if (kind == AGX_TEX2D_CONSTANT) {
+            r = value;
+        } else if (kind == AGX_TEX2D_CHECKER) {
+            const T gs = T(1) / T(grid);
+            const int ui = int(std::min(T(1), std::max(T(0), u)) / gs), vi = int(std::min(T(1), std::max(T(0), v)) / gs);
+            r = ((ui + vi) & 1) ? value : value2;
+        } else if (!linear) {
+            r = texel(std::min(w - 1, int(u * T(w))), std::min(h - 1, int(v * T(h))));
+        } else {
+            const T fu = u * T(w) - T(0.5), fv = v * T(h) - T(0.5);
+            const T bu = std::floor(fu), bv = std::floor(fv);
+            const T du = fu - bu, dv = fv - bv;
+            const int x0 = std::min(w - 1, std::max(0, int(bu))), x1 = std::min(w - 1, std::max(0, int(bu) + 1));
+            const int y0 = std::min(h - 1, std::max(0, int(bv))), y1 = std::min(h - 1, std::max(0, int(bv) + 1));
+            auto lerp = [](const V3<T> &a, const V3<T> &b, T k) { return a + (b - a) * k; };
+            r = lerp(lerp(texel(x0, y0), texel(x1, y0), du), lerp(texel(x0, y1), texel(x1, y1), du), dv);
+        }
+        if (inv_gamma != 1) r = V3<T>(std::pow(r.x, inv_gamma), std::pow(r.y, inv_gamma), std::pow(r.z, inv_gamma));
+        return r;
+    }
+};
+
+// IBL + EnvironmentLightSampler (envir_light/ibl.cpp:24-60, 64-80, 160-170;
+// env_sampler.h:17-183): radiance by spherical (phi, theta) -> (u, v); patches
+// of at most 200 x 200 weighted by luminance (one texel margin) x solid angle,
+// an alias table over them, then uniform cos(theta) and phi inside the patch.
+// no_importance_sampling samples a constant 1x1 texture: one patch = uniform sphere.
+template<class T> struct IblLight {
+    Texture2D<T> tex;
+    int pw = 1, ph = 1;
+    std::vector<T> probs;
+    AliasTable<T> alias;
+    static T phi_of(const V3<T> &d) {  // local_angle::phi: atan2(y, x) in [0, 2 pi)
+        const T p = std::atan2(d.y, d.x);
+        return p < 0 ? p + 2 * PI<T> : p;
+    }
+    void init(const agx_env_desc &e) {
+        tex.init(e.tex);
+        const bool uniform = e.no_importance_sampling != 0;
+        const int width = uniform ? 1 : tex.width(), height = uniform ? 1 : tex.height();
+        pw = std::min(width, 200);
+        ph = std::min(height, 200);
+        probs.assign(size_t(pw) * ph, T(0));
+        T lum_sum = 0;
+        for (int y = 0; y < ph; ++y) {
+            const T y0 = T(y) / T(ph), y1 = T(y + 1) / T(ph);
+            const int ys = std::max(0, int(std::floor(y0 * T(height)) - 1));
+            const int yl = std::min(height - 1, int(std::floor(y1 * T(height)) + 1));
+            for (int x = 0; x < pw; ++x) {
+                const T x0 = T(x) / T(pw), x1 = T(x + 1) / T(pw);
+                const int xs = std::max(0, int(std::floor(x0 * T(width)) - 1));
+                const int xl = std::min(width - 1, int(std::floor(x1 * T(width)) + 1));
+                T lum = 0;
+                for (int yy = ys; yy <= yl; ++yy)
+                    for (int xx = xs; xx <= xl; ++xx) {
+                        const V3<T> c = uniform ? V3<T>(1)
+                                                : tex.sample((T(xx) + T(0.5)) / T(width), (T(yy) + T(0.5)) / T(height));
+                        lum += T(0.2126) * c.x + T(0.7152) * c.y + T(0.0722) * c.z;
+                    }
+                const T area = std::abs(2 * PI<T> * (x1 - x0) * (std::cos(PI<T> * y1) - std::cos(PI<T> * y0)));
+                probs[size_t(y) * pw + x] = lum * area;
+                lum_sum += lum * area;
+            }
+        }
+        if (lum_sum > 0) for (T &p : probs) p *= 1 / lum_sum;
+        else for (T &p : probs) p = T(1) / T(probs.size());
+        alias.build(probs);
+    }
+    V3<T> radiance(const V3<T> &d) const {
+        const V3<T> dir = d.normalize();
+        const T theta = std::acos(std::min(T(1), std::max(T(-1), dir.z)));
+        const T u = std::min(T(1), std::max(T(0), phi_of(dir) / (2 * PI<T>)));
+        const T v = std::min(T(1), std::max(T(0), theta / PI<T>));
+        return tex.sample(u, v);
+    }
+    T patch_pdf(int px, int py, T *cvmin, T *cvmax, T *u0, T *u1) const {
+        *u0 = T(px) / T(pw);
+        *u1 = T(px + 1) / T(pw);
+        const T v0 = T(py) / T(ph), v1 = T(py + 1) / T(ph);
+        const T c1 = std::cos(PI<T> * v1), c0 = std::cos(PI<T> * v0);
+        *cvmin = std::min(c1, c0);
+        *cvmax = std::max(c1, c0);
+        return probs[size_t(py) * pw + px] / (2 * PI<T> * ((*u1 - *u0) * (*cvmax - *cvmin)));
+    }
+    T pdf(const V3<T> &d) const {
+        const V3<T> dir = d.normalize();
+        const T theta = std::acos(std::min(T(1), std::max(T(-1), dir.z)));
+        const T u = phi_of(dir) / (2 * PI<T>), v = theta / PI<T>;
+        const int px = std::min(pw - 1, std::max(0, int(std::floor(u * T(pw)))));
+        const int py = std::min(ph - 1, std::max(0, int(std::floor(v * T(ph)))));
+        T a, b, c, e;
+        return patch_pdf(px, py, &a, &b, &c, &e);
+    }
+    V3<T> sample(T su, T sv, T sw, T *pdf) const {
+        const int32_t pick = alias.sample(su);
+        const int px = pick % pw, py = pick / pw;
+        T cvmin, cvmax, u0, u1;
+        *pdf = patch_pdf(px, py, &cvmin, &cvmax, &u0, &u1);
+        const T ct = cvmin + sw * (cvmax - cvmin);
+        const T st = std::sqrt(std::max(T(0), 1 - ct * ct));
+        const T phi = 2 * PI<T> * (u0 * (1 - sv) + u1 * sv);
+        return V3<T>(st * std::cos(phi), st * std::sin(phi), ct);
+    }
+};
+
 template<class T> struct Texture3D {
     int kind = AGX_TEX3D_CONSTANT;
     bool linear = true;
@@ -1867,6 +2024,7 @@ template<class T> struct Scene {
     }
     int env_kind = AGX_ENV_NONE;
     V3<T> sky_top, sky_bottom;
+    IblLight<T> ibl;
     T world_radius = 1;
     V3<T> world_centre;
     std::vector<int32_t> lights;  // -1 = environment, else entity index (scene_default.cpp:66-82)
@@ -1906,6 +2064,9 @@ template<class T> struct Scene {
         env_kind = d.env.kind;
         sky_top = V3<T>(T(d.env.top[0]), T(d.env.top[1]), T(d.env.top[2]));
         sky_bottom = V3<T>(T(d.env.bottom[0]), T(d.env.bottom[1]), T(d.env.bottom[2]));
+        if (env_kind == AGX_ENV_IBL) ibl.init(d.env);
+        else if (env_kind != AGX_ENV_NONE && env_kind != AGX_ENV_NATIVE_SKY)
+            throw std::runtime_error("unknown environment light");
         lights.clear();
         if (env_kind != AGX_ENV_NONE) lights.push_back(-1);
         for (uint32_t i = 0; i < d.entity_count; ++i) if (entities[i].is_light) lights.push_back(int32_t(i));
@@ -2012,11 +2173,20 @@ template<class T> struct Scene {
         return !any(Ray<T>(A, (B - A).normalize(), eps, dis - eps), c);
     }
 
-    // NativeSky (native_sky.cpp:13-20, 71-105)
+    // NativeSky (native_sky.cpp:13-20, 71-105) or IBL
     V3<T> sky_radiance(const V3<T> &d) const {
+        if (env_kind == AGX_ENV_IBL) return ibl.radiance(d);
         const T ct = std::min(T(1), std::max(T(-1), d.normalize().z));
         const T s = (ct + 1) / 2;
         return sky_top * s + sky_bottom * (1 - s);
+    }
+    T env_pdf(const V3<T> &d) const {
+        return env_kind == AGX_ENV_IBL ? ibl.pdf(d) : uniform_on_sphere_pdf<T>;
+    }
+    V3<T> env_sample(T su, T sv, T sw, T *pdf) const {
+        if (env_kind == AGX_ENV_IBL) return ibl.sample(su, sv, sw, pdf);
+        *pdf = uniform_on_sphere_pdf<T>;
+        return uniform_on_sphere(su, sv);
     }
     V3<T> env_emit_pos(const V3<T> &ref, const V3<T> &ref_to_light) const {
         // point where the ray ref + t d meets the plane through centre + R d, normal d
@@ -2183,8 +2353,9 @@ V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> 
         if (scene.any(Ray<T>(inct.pos, to_light, eps, len), c)) return V3<T>(0);
         return ff / (ls.pdf + bpdf);
     }
-    // native sky: uniform sphere direction, emit position on the bounding plane
-    const V3<T> dir = uniform_on_sphere(s5[0], s5[1]);
+    // environment: native sky (uniform sphere) or IBL (patch sampler); emit position on the bounding plane
+    T epdf;
+    const V3<T> dir = scene.env_sample(s5[0], s5[1], s5[2], &epdf);
     const V3<T> pos = scene.env_emit_pos(inct.pos, dir);
     const V3<T> rad = scene.sky_radiance(dir);
     if (rad.is_black()) return V3<T>(0);
@@ -2194,7 +2365,7 @@ V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> 
     const V3<T> ff = rad * f * std::abs(cos_between(to_light, inct.geometry_coord.z));
     const T bpdf = bsdf.pdf(to_light, inct.wr);
     if (!scene.visible(inct.pos, pos, c)) return V3<T>(0);
-    return ff / (uniform_on_sphere_pdf<T> + bpdf);
+    return ff / (epdf + bpdf);
 }
 
 // mis_sample_bsdf (direct_illum.cpp:130-192)
@@ -2215,7 +2386,7 @@ V3<T> mis_sample_bsdf(const Scene<T> &scene, const EntityHit<T> &inct, const BSD
         if (lr.is_black()) return V3<T>(0);
         const V3<T> f = lr * bs.f * std::abs(dot(inct.geometry_coord.z, nr.d));
         if (bs.is_delta) return f / bs.pdf;
-        return f / (bs.pdf + uniform_on_sphere_pdf<T>);
+        return f / (bs.pdf + scene.env_pdf(nr.d));
     }
     if (!scene.entities[size_t(h.entity)].is_light) return V3<T>(0);
     const V3<T> lrad = scene.area_radiance(h.entity, h.geometry_coord.z, h.wr);
@@ -2259,7 +2430,7 @@ V3<T> mis_sample_bsdf_medium(const Scene<T> &scene, const MediumScatter<T> &ms, 
         if (scene.env_kind == AGX_ENV_NONE) return V3<T>(0);
         const V3<T> lf = scene.sky_radiance(nr.d);
         if (lf.is_black()) return V3<T>(0);
-        return lf * V3<T>(f) / (f + uniform_on_sphere_pdf<T>);
+        return lf * V3<T>(f) / (f + scene.env_pdf(nr.d));
     }
     if (!scene.entities[size_t(h.entity)].is_light) return V3<T>(0);
     const V3<T> lf = scene.area_radiance(h.entity, h.geometry_coord.z, h.wr);

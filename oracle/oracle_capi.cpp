@@ -256,6 +256,33 @@ int oracle_tex3d(void *h, int mi, int which, const double *uvw, uint32_t n, doub
     return H->precision == 64 ? run(H->sd) : run(H->sf);
 }
 
+// Environment light of the scene: mode 0 radiance(d) -> rgb; 1 pdf(d) -> 1 value;
+// 2 sample(u, v, w given in in[3i..3i+2]) -> dir[3], pdf.  4 doubles out per item.
+int oracle_env(void *h, int mode, const double *in, uint32_t n, double *out) {
+    using namespace oracle;
+    auto *H = static_cast<Holder *>(h);
+    auto run = [&](const auto &sc) -> int {
+        using T = typename std::decay_t<decltype(sc)>::Real;
+        if (sc.env_kind == AGX_ENV_NONE) { g_err = "scene has no environment light"; return -1; }
+        for (uint32_t i = 0; i < n; ++i) {
+            const V3<T> a{T(in[3 * i]), T(in[3 * i + 1]), T(in[3 * i + 2])};
+            double *o = out + 4 * size_t(i);
+            if (mode == 0) {
+                const V3<T> r = sc.sky_radiance(a);
+                o[0] = double(r.x); o[1] = double(r.y); o[2] = double(r.z); o[3] = 0;
+            } else if (mode == 1) {
+                o[0] = double(sc.env_pdf(a)); o[1] = o[2] = o[3] = 0;
+            } else {
+                T pdf;
+                const V3<T> d = sc.env_sample(a.x, a.y, a.z, &pdf);
+                o[0] = double(d.x); o[1] = double(d.y); o[2] = double(d.z); o[3] = double(pdf);
+            }
+        }
+        return 0;
+    };
+    return H->precision == 64 ? run(H->sd) : run(H->sf);
+}
+
 // Henyey-Greenstein: mode 0 eval(wi, wo) -> 1 value; 1 sample(wo, u, v) -> dir[3], f.
 int oracle_hg(double g, int mode, const double *wi, const double *wo, double u, double v, double *out) {
     using namespace oracle;

@@ -45,6 +45,7 @@ def load() -> C.CDLL:
     lib.oracle_medium.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), P(C.c_double), C.c_int, C.c_uint64,
                                   C.c_uint32, P(C.c_double)]
     lib.oracle_tex3d.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), C.c_uint32, P(C.c_double)]
+    lib.oracle_env.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_uint32, P(C.c_double)]
     lib.oracle_hg.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               P(C.c_double)]
     _lib = lib
@@ -117,6 +118,16 @@ class OracleScene:
         if self.lib.oracle_tex3d(self.h, mi, w, _d(uvw), len(uvw), _d(out)) != 0:
             raise RuntimeError(self.lib.oracle_last_error().decode())
         return out
+
+    def env(self, mode: str, x) -> np.ndarray:
+        """Environment light: mode "radiance" / "pdf" on directions (n, 3), or
+        "sample" on uniforms (n, 3) -> (n, 4) = direction, pdf."""
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+        out = np.zeros((len(x), 4), np.float64)
+        m = {"radiance": 0, "pdf": 1, "sample": 2}[mode]
+        if self.lib.oracle_env(self.h, m, _d(x), len(x), _d(out)) != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return out[:, :3] if m == 0 else (out[:, 0] if m == 1 else out)
 
     def render(self, params, threads: int | None = None, rows: tuple | None = None, aovs: bool = True):
         W, H = params.width, params.height

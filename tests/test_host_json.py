@@ -239,3 +239,71 @@ def test_gaussian_film_filter_reaches_the_session():
     with pytest.raises(RuntimeError) as ei:
         capi.HostScene(json.dumps(text), ".")
     assert "invalid alpha value" in str(ei.value)
+
+
+def write_hdr(path, mant, expo, rle):
+    """Radiance RGBE file from integer mantissas (h, w, 3) and exponents (h, w)."""
+    h, w, _ = mant.shape
+    out = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode())
+    for y in range(h):
+        px = np.concatenate([mant[y], expo[y][:, None]], axis=1).astype(np.uint8)
+        if not rle:
+            out += px.tobytes()
+            continue
+        out += bytes([2, 2, w >> 8, w & 255])
+        for c in range(4):
+            ch = px[:, c]
+            x = 0
+            while x < w:
+                run = 1
+                while x + run < w and run < 127 and ch[x + run] == ch[x]:
+                    run += 1
+                if run >= 3:
+                    out += bytes([128 + run, int(ch[x])])
+                    x += run
+                else:
+                    n = min(128, w - x)
+                    # literal block up to the next run of 3
+                    k = 0
+                    while k < n and not (x + k + 2 < w and ch[x + k] == ch[x + k + 1] == ch[x + k + 2]):
+                        k += 1
+                    k = max(1, k)
+                    out += bytes([k]) + bytes(int(v) for v in ch[x:x + k])
+                    x += k
+    path.write_bytes(bytes(out))
+
+
+@pytest.mark.parametrize("rle", [False, True])
+def test_ibl_hdr_texture_is_read_exactly(tmp_path, rle):
+    rng = np.random.default_rng(3)
+    h, w = 5, 24
+    mant = rng.integers(0, 256, (h, w, 3))
+    mant[..., 0] = rng.integers(128, 256, (h, w))
+    mant[1, 3:9] = (200, 100, 50)          # runs for the encoder
+    expo = rng.integers(120, 140, (h, w))
+    write_hdr(tmp_path / "env.hdr", mant, expo, rle)
+    expect = (mant * np.ldexp(1.0, expo - 136)[..., None]).astype(np.float32)
+    text = json.loads(wrap([quad_json([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])]))
+    text["scene"]["env"] = {"type": "ibl", "tex": {"type": "hdr", "filename": "${scene-directory}/env.hdr",
+                                                    "sample": "nearest", "wrap_u": "repeat"},
+                            "no_importance_sampling": 1}
+    hs = capi.HostScene(json.dumps(text), str(tmp_path))
+    env = hs.desc().env
+    assert env.kind == capi.ENV_IBL and env.no_importance_sampling == 1
+    t = env.tex
+    assert (t.kind, t.width, t.height, t.linear) == (capi.TEX2D_IMAGE, w, h, 0)
+    assert list(t.wrap) == [capi.WRAP_REPEAT, capi.WRAP_CLAMP]
+    got = np.ctypeslib.as_array(t.data, shape=(h * w * 3,)).reshape(h, w, 3)
+    np.testing.assert_array_equal(got, expect)
+
+
+def test_ibl_checker_and_transform2():
+    text = json.loads(wrap([quad_json([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])]))
+    text["scene"]["env"] = {"type": "ibl", "tex": {
+        "type": "checker_board", "grid_count": 4, "color1": [1, 0, 0], "color2": [0, 0, 1], "inv_u": 1,
+        "transform": [{"type": "translate", "offset": [0.25, 0]}, {"type": "scale", "ratio": 2}]}}
+    hs = capi.HostScene(json.dumps(text), ".")
+    t = hs.desc().env.tex
+    assert t.kind == capi.TEX2D_CHECKER and t.grid_count == 4
+    # inv_u (u -> 1 - u) applied after the user transform: translate(0.25) * scale(2): u -> 2u + 0.25
+    np.testing.assert_allclose(list(t.transform), [-2, 0, 0.75, 0, 2, 0], atol=1e-6)
