@@ -26,10 +26,13 @@ on this host.
 guided: equal-time relMSE of guided vs plain GPU PT on the door-gap room
 (10 x 10 x 4 Lambert room lit only through a 0.05-wide gap from a second room,
 500k clutter triangles, seed 7).  Plain and guided renders each run for
-``--guided-seconds`` of wall time in chunks of ``--guided-chunk`` spp (the
-guided run starts from a freshly initialised network and trains on every pass,
-so its warm-up counts against it); the reference is plain PT with a different
-seed for ``--guided-ref-factor`` times as long.  relMSE = mean over pixels and
+``--guided-seconds`` of wall time in chunks of ``--guided-chunk`` spp.  The
+guided run starts from a freshly initialised network, trains on every pass for
+the first ``--guided-train-frac`` of the time (one sample per pixel per pass,
+``--guided-steps`` Adam steps on a ring of ``--guided-batch`` samples), then
+renders with the network frozen; every sample, warm-up included, goes into its
+image.  The reference is plain PT with a different seed for
+``--guided-ref-factor`` times as long.  relMSE = mean over pixels and
 channels of (x - y)^2 / (y^2 + 1e-2) after scaling both images by 1 / mean(y)
 (the room is dark).  MLP utilisation = 61,440 FLOP per evaluated vertex / the
 inference kernel's device time, against the 2.5 PFLOP/s dense BF16 peak.
@@ -134,15 +137,17 @@ def guided_leg(ctx, args) -> dict:
     log(f"[bench] guided leg: door-gap room, {s.triangle_count()} triangles, {W}x{H} "
         f"(built + uploaded in {time.time() - t0:.1f} s)")
 
-    def run(seconds: float, seed: int, guide: bool):
+    def run(seconds: float, seed: int, guide: bool, train_s: float = 0.0):
         acc = np.zeros((H, W, 3), np.float64)
         spp = 0
         st_sum = {"guide_infer_ms": 0.0, "guide_train_ms": 0.0, "guide_rows": 0, "guide_samples": 0,
                   "guide_trained": 0, "total_ms": 0.0, "rays": 0}
         t_start = time.perf_counter()
         while time.perf_counter() - t_start < seconds:
+            train = guide and (spp == 0 or time.perf_counter() - t_start < train_s)
             p = scenes.render_params(W, H, args.guided_chunk, cam, seed=seed, sample_begin=spp, guide=guide,
-                                     guide_reset=guide and spp == 0, **tr)
+                                     guide_reset=guide and spp == 0, guide_train=train,
+                                     guide_steps=args.guided_steps, guide_batch=args.guided_batch, **tr)
             img = ctx.render(p, aovs=False)["image"]
             acc += img.astype(np.float64) * args.guided_chunk
             spp += args.guided_chunk
@@ -157,7 +162,8 @@ def guided_leg(ctx, args) -> dict:
     ref, ref_spp, ref_wall, _ = run(args.guided_seconds * args.guided_ref_factor, 1001, False)
     log(f"[bench] guided leg: reference plain {ref_spp} spp in {ref_wall:.1f} s")
     plain, plain_spp, plain_wall, pst = run(args.guided_seconds, 2002, False)
-    guided, guided_spp, guided_wall, gst = run(args.guided_seconds, 3003, True)
+    guided, guided_spp, guided_wall, gst = run(args.guided_seconds, 3003, True,
+                                               train_s=args.guided_train_frac * args.guided_seconds)
     ep, eg = relmse_scaled(plain, ref), relmse_scaled(guided, ref)
     # time-normalised: relMSE ~ 1 / time for an unbiased estimator
     ratio_t = (ep * plain_wall) / (eg * guided_wall) if eg > 0 else None
@@ -169,6 +175,8 @@ def guided_leg(ctx, args) -> dict:
         "workload": (f"door-gap room (10x10x4 Lambert room lit through a 0.05-wide gap, "
                      f"{s.triangle_count()} triangles), {W}x{H}, max depth {tr['max_depth']}"),
         "seconds": args.guided_seconds, "chunk_spp": args.guided_chunk,
+        "train_frac": args.guided_train_frac, "adam_steps_per_pass": args.guided_steps,
+        "sample_ring": args.guided_batch,
         "plain": {"spp": plain_spp, "wall_s": round(plain_wall, 3), "relmse": ep,
                   "Mrays_per_s": pst["rays"] / plain_wall / 1e6},
         "guided": {"spp": guided_spp, "wall_s": round(guided_wall, 3), "relmse": eg,
@@ -272,6 +280,9 @@ def main() -> int:
     ap.add_argument("--guided-seconds", type=float, default=4.0)
     ap.add_argument("--guided-chunk", type=int, default=4)
     ap.add_argument("--guided-ref-factor", type=float, default=16.0)
+    ap.add_argument("--guided-train-frac", type=float, default=0.3)
+    ap.add_argument("--guided-steps", type=int, default=1)
+    ap.add_argument("--guided-batch", type=int, default=32768)
     ap.add_argument("--copy-calibration-mb", type=int, default=256,
                     help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()

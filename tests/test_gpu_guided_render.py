@@ -45,17 +45,17 @@ def test_guiding_reduces_error_through_a_door_gap(gpu_ctx):
     s.accel = 1
     desc, keep = s.to_desc()
     gpu_ctx.upload_scene(desc)
-    W = H = 64
-    log("[guided] door-gap room 64x64: reference plain 16384 spp")
-    ref = gpu_ctx.render(scenes.render_params(W, H, 16384, cam, seed=1, **tr), aovs=False)["image"].astype(np.float64)
+    W = H = 128
+    log(f"[guided] door-gap room {W}x{H}: reference plain 4096 spp")
+    ref = gpu_ctx.render(scenes.render_params(W, H, 4096, cam, seed=1, **tr), aovs=False)["image"].astype(np.float64)
     plain = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=2, **tr), aovs=False)["image"].astype(np.float64)
-    # train on a warm-up render, then render the measured image (training continues)
+    # train on a warm-up render, then render the measured image with the network frozen
     gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=3, guide=True, guide_reset=True, **tr), aovs=False)
-    guided = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=4, guide=True, **tr),
-                            aovs=False)["image"].astype(np.float64)
     st = gpu_ctx.stats()
-    log(f"  network: {st['guide_steps']} Adam steps, {st['guide_samples']} samples in the measured render")
+    log(f"  network: {st['guide_steps']} Adam steps, {st['guide_samples']} samples gathered in the warm-up")
+    guided = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=4, guide=True, guide_train=False, **tr),
+                            aovs=False)["image"].astype(np.float64)
     ep, eg = relmse_scaled(plain, ref), relmse_scaled(guided, ref)
     log(f"  relMSE at 256 spp: plain {ep:.3e}, guided {eg:.3e} (ratio {ep / eg:.2f})")
     assert np.isfinite(guided).all()
-    assert eg < ep
+    assert eg * 1.5 < ep
