@@ -2063,7 +2063,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     Rng rng;
     rng.state = pb.rng[p];
     {
-        const int64_t wh = int64_t(rc.W) * rc.H;
+        // the stream of (sampled pixel, sample), as k_raygen keys it (the sampled
+        // grid includes the filter border when samples splat)
+        const int64_t wh = rc.splat ? int64_t(rc.EW) * rc.EH : int64_t(rc.W) * rc.H;
         const int64_t w = rc.work_base + int64_t(p);
         rng.inc = (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
         rtr.inc = rng.inc;

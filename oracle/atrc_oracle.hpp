@@ -2491,6 +2491,28 @@ struct RenderOut {
     double *rgb = nullptr, *albedo = nullptr, *normal = nullptr, *denoise = nullptr;
 };
 
+// One camera sample of pixel (qx, qy) with stream key `key` (the sampled-grid
+// index): its film position and PixelResult (the per-sample unit of render()).
+template<class T>
+PixelResult<T> trace_sample(const Scene<T> &scene, const agx_render_params &rp, int qx, int qy, uint64_t key,
+                            uint64_t sample, float *sx, float *sy) {
+    Camera<T> cam;
+    cam.init(rp.camera);
+    TraceParams tp;
+    tp.min_depth = rp.min_depth; tp.max_depth = rp.max_depth; tp.specular_depth = rp.specular_depth;
+    tp.cont_prob = rp.cont_prob;
+    Sampler<T> sam, trs;
+    sam.rng.seed_path(rp.seed, key, sample);
+    trs.rng.seed_path(rp.seed ^ TR_STREAM_KEY, key, sample);
+    const float fu = sam.rng.uniform(), fv = sam.rng.uniform();
+    *sx = float(qx) + fu;
+    *sy = float(qy) + fv;
+    const T au = sam.u1(), av = sam.u1();
+    const Ray<T> ray = cam.sample(T(*sx / float(rp.width)), T(*sy / float(rp.height)), au, av);
+    Counters c;
+    return trace_std(tp, scene, ray, sam, trs, &c);
+}
+
 // GaussianFilter::eval without its normalisation factor, which cancels in
 // value / weight (film_filter/gaussian_filter.cpp:60-68)
 inline float gaussian_1d(float d, float alpha, float expv) {

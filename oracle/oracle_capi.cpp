@@ -108,6 +108,24 @@ int oracle_render(void *p, const agx_render_params *rp, int threads, double *rgb
     }
 }
 
+// One camera sample (trace_sample): out = value[3], albedo[3], normal[3], denoise, sx, sy.
+int oracle_trace_sample(void *p, const agx_render_params *rp, int qx, int qy, uint64_t key, uint64_t sample,
+                        double *out) {
+    auto *h = static_cast<Holder *>(p);
+    if (!h || !rp || !out) { g_err = "null argument"; return 1; }
+    auto run = [&](auto &sc) {
+        sc.eps = decltype(sc.eps)(rp->eps);
+        float sx, sy;
+        const auto r = oracle::trace_sample(sc, *rp, qx, qy, key, sample, &sx, &sy);
+        for (int k = 0; k < 3; ++k) {
+            out[k] = double(r.value[k]); out[3 + k] = double(r.albedo[k]); out[6 + k] = double(r.normal[k]);
+        }
+        out[9] = double(r.denoise); out[10] = sx; out[11] = sy;
+    };
+    if (h->precision == 64) run(h->sd); else run(h->sf);
+    return 0;
+}
+
 // PCG32 uniform stream of (seed, pixel, sample): n floats.
 // Closest hit over a triangle soup (9 floats per triangle: A, B, C), brute
 // force, with the watertight test (mode 1) or Atrc's Moller-Trumbore test on

@@ -46,6 +46,8 @@ def load() -> C.CDLL:
                                   C.c_uint32, P(C.c_double)]
     lib.oracle_tex3d.argtypes = [C.c_void_p, C.c_int, C.c_int, P(C.c_double), C.c_uint32, P(C.c_double)]
     lib.oracle_env.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_uint32, P(C.c_double)]
+    lib.oracle_trace_sample.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint64,
+                                        P(C.c_double)]
     lib.oracle_hg.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               P(C.c_double)]
     _lib = lib
@@ -116,6 +118,13 @@ class OracleScene:
         out = np.zeros((len(uvw), 3), np.float64)
         w = {"density": 0, "albedo": 1, "g": 2}[which]
         if self.lib.oracle_tex3d(self.h, mi, w, _d(uvw), len(uvw), _d(out)) != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+        return out
+
+    def trace_sample(self, params, qx: int, qy: int, key: int, sample: int) -> np.ndarray:
+        """One camera sample: value[3], albedo[3], normal[3], denoise, film x, film y."""
+        out = np.zeros(12, np.float64)
+        if self.lib.oracle_trace_sample(self.h, C.byref(params), qx, qy, key, sample, _d(out)) != 0:
             raise RuntimeError(self.lib.oracle_last_error().decode())
         return out
 
