@@ -160,6 +160,7 @@ EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
     "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays", "agx_render_batch",
+    "agx_debug_guide_grad_mfma",
 ]
 GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
 
@@ -196,6 +197,8 @@ def load_library() -> C.CDLL:
     lib.agx_debug_triangle_bvh.argtypes = [P(agx_vertex), C.c_uint32, P(C.c_uint32), P(C.c_uint32), fp, fp]
     lib.agx_debug_scene_summary.argtypes = [P(agx_scene_desc), P(agx_debug_summary)]
     lib.agx_debug_copy_kernel.argtypes = [C.c_void_p, C.c_uint64, P(C.c_double)]
+    lib.agx_debug_guide_grad_mfma.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32, C.c_float,
+                                              P(C.c_float)]
     lib.agx_render_batch.argtypes = [C.c_void_p, P(agx_batch_item), C.c_uint32, C.c_int32, P(agx_batch_report)]
     lib.agx_debug_replay_rays.argtypes = [C.c_void_p, P(agx_render_params), C.c_uint32, C.c_int, P(C.c_uint64),
                                           P(C.c_double)]
@@ -327,6 +330,15 @@ class Context:
         _check(self.lib.agx_debug_guide_train(self.h, fptr(params), fptr(m), fptr(v), fptr(samples), len(samples),
                                               wscale, lr, t, fptr(g)))
         return params, m, v, g
+
+    def guide_grad_mfma(self, params, samples, wscale: float) -> np.ndarray:
+        """Gradient of the training loss from the matrix-core training path."""
+        params = np.ascontiguousarray(params, np.float32)
+        samples = np.ascontiguousarray(samples, np.float32)
+        assert samples.ndim == 2 and samples.shape[1] == 12
+        g = np.zeros(GUIDE_PARAMS, np.float32)
+        _check(self.lib.agx_debug_guide_grad_mfma(self.h, fptr(params), fptr(samples), len(samples), wscale, fptr(g)))
+        return g
 
     def copy_kernel(self, nbytes: int) -> float:
         """One plain copy kernel of nbytes (traffic-counter calibration); device ms."""

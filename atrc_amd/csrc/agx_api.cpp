@@ -87,6 +87,7 @@ struct agx_ctx {
     __bf16 *g_packed = nullptr;
     uint32_t *g_count = nullptr;
     uint32_t *g_ring = nullptr;      // sample ring (next slot, valid samples) over g_samples
+    __bf16 *g_acts = nullptr, *g_dy = nullptr;  // matrix-core training: activations, output gradients
     uint32_t g_ring_cap = 0;         // capacity the ring state refers to (0: empty)
     uint32_t g_batch_cap = 0;
     uint64_t g_steps = 0;
@@ -200,6 +201,8 @@ struct agx_ctx {
     int ensure_guide_batch(uint32_t batch) {
         if (batch > g_batch_cap) {
             HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
+            HIP_TRY(galloc(guide_allocs, &g_acts, size_t(batch) * 4 * G_HID));
+            HIP_TRY(galloc(guide_allocs, &g_dy, size_t(batch) * 2 * G_HID));
             g_batch_cap = batch;
         }
         if (g_ring_cap != batch) {
@@ -658,14 +661,16 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_ring, c->stream);
             launch_guide_ring_commit(c->g_ring, c->g_count, guide_batch, c->g_samples, c->g_wsum, c->stream);
             for (int k = 0; k < guide_steps; ++k) {
-                launch_guide_train(c->g_params, c->g_samples, c->g_ring + 1, guide_batch, c->g_wsum, c->g_partial,
-                                   agx_ctx::GUIDE_TRAIN_BLOCKS, c->stream);
+                // matrix-core step: forward with the current BF16 weights, backward, reduce, Adam, repack
+                launch_guide_train_mfma(c->g_params, c->g_packed, c->g_bias, c->g_samples, c->g_ring + 1, guide_batch,
+                                        c->g_wsum, c->g_acts, c->g_dy, c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS,
+                                        c->stream);
                 launch_guide_reduce(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_grad, c->stream);
                 c->g_steps += 1;
                 launch_guide_adam(c->g_params, c->g_m, c->g_v, c->g_grad, guide_lr, 0.9f, 0.99f, 1e-8f,
                                   int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream);
+                launch_guide_pack(c->g_params, c->g_packed, c->g_bias, c->stream);
             }
-            launch_guide_pack(c->g_params, c->g_packed, c->g_bias, c->stream);
             HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipMemcpyAsync(&guide_valid_h, c->g_ring + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             if ((r = mark(&tb))) return r;
@@ -1130,6 +1135,42 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
     HIP_TRY(hipMemcpyAsync(params, dp.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(m, dm.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(v, dv.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return AGX_OK;
+}
+
+int agx_debug_guide_grad_mfma(agx_ctx *c, const float *params, const float *samples, uint32_t n, float wscale,
+                              float *grad_out) {
+    if (!c || !params || !grad_out || (n && !samples)) return fail(AGX_E_INVALID_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipError_t e = guide_prepare(c->device); e != hipSuccess)
+        return fail(AGX_E_UNSUPPORTED, std::string("guiding kernels: ") + hipGetErrorString(e));
+    constexpr int BLOCKS = 64;
+    const uint32_t cap = std::max<uint32_t>(n, 1);
+    DevBuf dp, dpack, dbias, ds, dacts, ddy, dpart, dg, dscal;
+    HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dpack.p, G_PACK * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&dbias.p, G_BIASES * sizeof(float)));
+    HIP_TRY(hipMalloc(&ds.p, size_t(cap) * (G_IN + 4) * sizeof(float)));
+    HIP_TRY(hipMalloc(&dacts.p, size_t(cap) * 4 * G_HID * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&ddy.p, size_t(cap) * 2 * G_HID * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dg.p, G_PARAMS * sizeof(float)));
+    HIP_TRY(hipMalloc(&dscal.p, 2 * sizeof(float)));
+    const uint32_t nn = n;
+    const float ws = wscale > 0.0f ? 1.0f / wscale : 0.0f;
+    HIP_TRY(hipMemcpyAsync(dscal.p, &nn, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(static_cast<char *>(dscal.p) + 4, &ws, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dp.p, params, G_PARAMS * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    if (n) HIP_TRY(hipMemcpyAsync(ds.p, samples, size_t(n) * (G_IN + 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    launch_guide_pack(dp.get<float>(), dpack.get<__bf16>(), dbias.get<float>(), c->stream);
+    launch_guide_train_mfma(dp.get<float>(), dpack.get<__bf16>(), dbias.get<float>(), ds.get<float>(),
+                            dscal.get<uint32_t>(), cap, dscal.get<float>() + 1, dacts.get<__bf16>(), ddy.get<__bf16>(),
+                            dpart.get<float>(), BLOCKS, c->stream);
+    launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(grad_out, dg.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return AGX_OK;
 }

@@ -4,6 +4,7 @@
 // Layout constants are in agx_guide.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "agx_guide.h"
@@ -345,6 +346,212 @@ __global__ void __launch_bounds__(256) k_guide_train(const float *params, const 
     }
 }
 
+// ---------------------------------------------------------------- matrix-core training
+// The same loss and gradient as k_guide_train, layer by layer on the matrix
+// cores: BF16 operands (weights, activations, output gradients), FP32
+// accumulation and FP32 master weights.  One step =
+//   k_gtrain_forward: forward of every sample (activations H1..H4 to HBM) and
+//     d loss / d outputs (dY5);
+//   k_gtrain_backward<L>, L = 4..0: dW_L = H_L^T dY_{L+1} and db_L = sum dY_{L+1}
+//     per block over its samples (partial[block]), and dY_L = (dY_{L+1} W_L^T)
+//     masked by ReLU'(H_L) for the layer below;
+//   then the fixed-order reduction over blocks and Adam, as before.
+// Activations and gradients: bf16 [sample][64] rows (dY5 uses 32 of them).
+constexpr int GT_CHUNK = 32;                     // samples per MFMA k-step in dW
+constexpr int GT_PAD = 8;                        // bf16 padding of LDS rows
+
+__global__ void __launch_bounds__(256) k_gtrain_forward(const __bf16 *packed, const float *bias, const float *samples,
+                                                        const uint32_t *n_dev, uint32_t cap, const float *wsum_dev,
+                                                        __bf16 *acts, __bf16 *dy5) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    GuideSmem &sm = *reinterpret_cast<GuideSmem *>(smem_raw);
+    const uint32_t n = min(*n_dev, cap);
+    const float wsum = *wsum_dev;
+    const float wscale = wsum > 0.0f ? 1.0f / wsum : 0.0f;
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    if (uint32_t(blockIdx.x) * 64u >= n) return;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(packed);
+        uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
+        for (int i = tid; i < G_PACK / 8; i += 256) dst[i] = src[i];
+        for (int i = tid; i < G_BIASES; i += 256) sm.b[i] = bias[i];
+    }
+    __syncthreads();
+    const int r = lane % 16, g = lane / 16;
+    for (uint32_t tile = blockIdx.x * 4u + uint32_t(wave); tile * 16u < n; tile += gridDim.x * 4u) {
+        const uint32_t row = tile * 16u + uint32_t(r);
+        const bool valid = row < n;
+        float x[G_IN];
+        #pragma unroll
+        for (int k = 0; k < G_IN; ++k) {
+            const float v = valid ? samples[size_t(row) * G_SAMPLE + k] : 0.0f;
+            x[k] = fminf(1.0f, fmaxf(0.0f, v));
+        }
+        f32x4 acc[4];
+        #pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int k = 0; k < G_IN; ++k) {
+            bf16x8 a;
+            #pragma unroll
+            for (int i = 0; i < 8; ++i) a[i] = __bf16(one_blob(x[k], 8 * g + i));
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bf16x8 *w = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(0) + ((k * 4 + t) * 64 + lane) * 8]);
+                acc[t] = mfma(a, *w, acc[t]);
+            }
+        }
+        #pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int col = 16 * t + r;
+            const float b0 = sm.b[g_bias_off(0) + col];
+            #pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = acc[t][j] + b0;
+                sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
+            }
+        }
+        wave_sync();
+        // H_l (input of layer l) for l = 1..4 goes to HBM before layer l overwrites it
+        auto store_act = [&](int l) {
+            const uint32_t base = tile * 16u;
+            for (int e = lane; e < 16 * G_HID / 8; e += 64) {
+                const int rr = e / (G_HID / 8), c8 = e % (G_HID / 8);
+                if (base + uint32_t(rr) < n)
+                    *reinterpret_cast<uint4 *>(acts + (size_t(l - 1) * cap + base + rr) * G_HID + 8 * c8) =
+                        *reinterpret_cast<const uint4 *>(&sm.act[wave][rr][8 * c8]);
+            }
+        };
+        store_act(1);
+        hidden_layer<1>(sm, wave, lane);
+        store_act(2);
+        hidden_layer<2>(sm, wave, lane);
+        store_act(3);
+        hidden_layer<3>(sm, wave, lane);
+        store_act(4);
+        hidden_layer<4>(sm, wave, lane);
+        // d loss / d outputs of row r (one lane per row)
+        if (g == 0 && valid) {
+            const float *smp = samples + size_t(row) * G_SAMPLE;
+            float gr[G_OUT];
+            const float w = smp[G_IN + 3] * wscale;
+            if (w != 0.0f) mixture_grad(&sm.ost[wave][r][0], smp + G_IN, w, gr);
+            else for (int j = 0; j < G_OUT; ++j) gr[j] = 0.0f;
+            for (int j = 0; j < G_OUT; ++j) dy5[size_t(row) * G_HID + j] = __bf16(gr[j]);
+        }
+        wave_sync();
+    }
+}
+
+template <int L>
+struct GtBwdSmem {
+    static constexpr int IN = g_in(L), OUT = g_out(L);
+    __bf16 w[IN][OUT + GT_PAD];                  // W_L row-major (input feature, output)
+    __bf16 ht[IN][GT_CHUNK + GT_PAD];            // input activations, feature-major
+    __bf16 hs[GT_CHUNK][(L > 0 ? IN : 8) + GT_PAD];  // input activations, sample-major (ReLU mask)
+    __bf16 ds[GT_CHUNK][OUT + GT_PAD];           // dY_{L+1}, sample-major
+    __bf16 dt[OUT][GT_CHUNK + GT_PAD];           // dY_{L+1}, feature-major
+};
+
+template <int L>
+__global__ void __launch_bounds__(256) k_gtrain_backward(const float *params, const float *samples,
+                                                         const uint32_t *n_dev, uint32_t cap, const __bf16 *acts,
+                                                         const __bf16 *dy_in, __bf16 *dy_out, float *partial) {
+    using S = GtBwdSmem<L>;
+    constexpr int IN = S::IN, OUT = S::OUT;
+    constexpr int MT = IN / 16, NT = OUT / 16;            // dW tiles
+    constexpr int DW_PER_WAVE = (MT * NT) / 4;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    S &sm = *reinterpret_cast<S *>(smem_raw);
+    const uint32_t n = min(*n_dev, cap);
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int r = lane % 16, g = lane / 16;
+    for (int i = tid; i < IN * OUT; i += 256) sm.w[i / OUT][i % OUT] = __bf16(params[g_w_off(L) + i]);
+    // this block's samples: [s_begin, s_end), whole chunks
+    const uint32_t chunks = (n + GT_CHUNK - 1) / GT_CHUNK;
+    const uint32_t per = (chunks + gridDim.x - 1) / gridDim.x;
+    const uint32_t c_begin = blockIdx.x * per, c_end = min(chunks, c_begin + per);
+    f32x4 dw[DW_PER_WAVE];
+    #pragma unroll
+    for (int t = 0; t < DW_PER_WAVE; ++t) dw[t] = f32x4{0, 0, 0, 0};
+    float db = 0.0f;
+    for (uint32_t c = c_begin; c < c_end; ++c) {
+        const uint32_t s0 = c * GT_CHUNK;
+        __syncthreads();
+        // stage dY_{L+1} (both layouts) and the input activations
+        for (int e = tid; e < GT_CHUNK * OUT; e += 256) {
+            const int sI = e / OUT, o = e % OUT;
+            const __bf16 v = s0 + sI < n ? dy_in[size_t(s0 + sI) * G_HID + o] : __bf16(0.0f);
+            sm.ds[sI][o] = v;
+            sm.dt[o][sI] = v;
+        }
+        if constexpr (L == 0) {
+            for (int e = tid; e < GT_CHUNK * IN; e += 256) {
+                const int sI = e / IN, f = e % IN;
+                float v = 0.0f;
+                if (s0 + sI < n) {
+                    const float xin = fminf(1.0f, fmaxf(0.0f, samples[size_t(s0 + sI) * G_SAMPLE + f / G_BINS]));
+                    v = one_blob(xin, f % G_BINS);
+                }
+                sm.ht[f][sI] = __bf16(v);
+            }
+        } else {
+            for (int e = tid; e < GT_CHUNK * IN; e += 256) {
+                const int sI = e / IN, f = e % IN;
+                const __bf16 v = s0 + sI < n ? acts[(size_t(L - 1) * cap + s0 + sI) * G_HID + f] : __bf16(0.0f);
+                sm.ht[f][sI] = v;
+                sm.hs[sI][f] = v;
+            }
+        }
+        __syncthreads();
+        // bias gradient: thread o sums its column over the chunk
+        if (tid < OUT)
+            for (int sI = 0; sI < GT_CHUNK; ++sI) db += float(sm.ds[sI][tid]);
+        // dW tiles of this wave: A = H^T (feature rows, sample k), B = dY (sample k, output cols)
+        #pragma unroll
+        for (int t = 0; t < DW_PER_WAVE; ++t) {
+            const int tile = wave * DW_PER_WAVE + t, mt = tile / NT, nt = tile % NT;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sm.ht[16 * mt + r][8 * g]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sm.dt[16 * nt + r][8 * g]);
+            dw[t] = mfma(a, b, dw[t]);
+        }
+        if constexpr (L > 0) {
+            // dY_L = dY_{L+1} W_L^T masked by ReLU'(H_L): 2 x MT tiles of (16 samples x 16 features)
+            constexpr int DH_TILES = 2 * MT, DH_PER_WAVE = DH_TILES / 4;
+            #pragma unroll
+            for (int t = 0; t < DH_PER_WAVE; ++t) {
+                const int tile = wave * DH_PER_WAVE + t, ms = tile / MT, ni = tile % MT;
+                f32x4 acc = f32x4{0, 0, 0, 0};
+                #pragma unroll
+                for (int kk = 0; kk < OUT / 32; ++kk) {
+                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sm.ds[16 * ms + r][32 * kk + 8 * g]);
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sm.w[16 * ni + r][32 * kk + 8 * g]);
+                    acc = mfma(a, b, acc);
+                }
+                const int f = 16 * ni + r;
+                #pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int sI = 16 * ms + 4 * g + j;
+                    if (s0 + sI < n) {
+                        const float h = float(sm.hs[sI][f]);
+                        dy_out[size_t(s0 + sI) * G_HID + f] = __bf16(h > 0.0f ? acc[j] : 0.0f);
+                    }
+                }
+            }
+        }
+    }
+    // this block's partial gradient of layer L (zeros when it had no samples)
+    float *part = partial + size_t(blockIdx.x) * G_PARAMS;
+    #pragma unroll
+    for (int t = 0; t < DW_PER_WAVE; ++t) {
+        const int tile = wave * DW_PER_WAVE + t, mt = tile / NT, nt = tile % NT;
+        const int col = 16 * nt + r;
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) part[g_w_off(L) + (16 * mt + 4 * g + j) * OUT + col] = dw[t][j];
+    }
+    if (tid < OUT) part[g_b_off(L) + tid] = db;
+}
+
 // Sum of the per-block partial gradients in a fixed order.
 // one block: advance the ring, then a fixed-order sum of the valid weights
 __global__ void __launch_bounds__(256) k_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap,
@@ -405,6 +612,26 @@ void launch_guide_train(const float *params, const float *samples, const uint32_
                        wsum_dev, partial);
 }
 
+void launch_guide_train_mfma(const float *params, const __bf16 *packed, const float *bias, const float *samples,
+                             const uint32_t *n_dev, uint32_t cap, const float *wsum_dev, __bf16 *acts, __bf16 *dy,
+                             float *partial, int blocks, hipStream_t st) {
+    // forward: enough persistent blocks to cover the ring's 16-row tiles
+    const uint32_t tiles = (cap + 63) / 64;
+    hipLaunchKernelGGL(k_gtrain_forward, dim3(std::min<uint32_t>(tiles, 512u)), dim3(256), sizeof(GuideSmem), st,
+                       packed, bias, samples, n_dev, cap, wsum_dev, acts, dy);
+    __bf16 *dy_a = dy, *dy_b = dy + size_t(cap) * G_HID;
+    hipLaunchKernelGGL(k_gtrain_backward<4>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<4>), st, params, samples, n_dev,
+                       cap, acts, dy_a, dy_b, partial);
+    hipLaunchKernelGGL(k_gtrain_backward<3>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<3>), st, params, samples, n_dev,
+                       cap, acts, dy_b, dy_a, partial);
+    hipLaunchKernelGGL(k_gtrain_backward<2>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<2>), st, params, samples, n_dev,
+                       cap, acts, dy_a, dy_b, partial);
+    hipLaunchKernelGGL(k_gtrain_backward<1>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<1>), st, params, samples, n_dev,
+                       cap, acts, dy_b, dy_a, partial);
+    hipLaunchKernelGGL(k_gtrain_backward<0>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<0>), st, params, samples, n_dev,
+                       cap, acts, dy_a, nullptr, partial);
+}
+
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
                               float *wsum, hipStream_t st) {
     hipLaunchKernelGGL(k_guide_ring_commit, dim3(1), dim3(256), 0, st, ring, added, cap, samples, wsum);
@@ -432,8 +659,15 @@ hipError_t guide_prepare(int device) {
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GuideSmem)));
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
+    if (e != hipSuccess) return e;
+    if (size_t(lds) < sizeof(GuideSmem) || size_t(lds) < sizeof(GtBwdSmem<0>)) return hipErrorInvalidConfiguration;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_forward),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GuideSmem)));
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_backward<0>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GtBwdSmem<0>)));
 }
 
 void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,

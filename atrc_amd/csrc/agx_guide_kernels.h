@@ -26,6 +26,12 @@ size_t guide_train_smem_bytes();
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
                         const float *wsum_dev, float *partial, int blocks, hipStream_t st);
 void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st);
+// the same gradient on the matrix cores (BF16 operands, FP32 accumulation): forward
+// of the ring's samples into acts (4 x cap x 64 bf16), backward through dy
+// (2 x cap x 64 bf16); blocks x G_PARAMS partial gradients
+void launch_guide_train_mfma(const float *params, const __bf16 *packed, const float *bias, const float *samples,
+                             const uint32_t *n_dev, uint32_t cap, const float *wsum_dev, __bf16 *acts, __bf16 *dy,
+                             float *partial, int blocks, hipStream_t st);
 // sample ring: ring[0] = next write slot, ring[1] = valid samples (<= cap); a
 // gather appended *added samples from slot ring[0] on; advance the ring and
 // sum the weights of the valid samples into *wsum in a fixed order

@@ -2590,9 +2590,11 @@ void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_
 }
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
-    // the lean variant serves scenes of Lambert / GGX / black surfaces in vacuum, unguided
-    const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA)) == 0 && !rc.guide;
-    if (lean) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    // lean variants serve scenes of Lambert / GGX / black surfaces in vacuum,
+    // unguided or guided; every other scene takes the full kernel
+    const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA)) == 0;
+    if (lean && !rc.guide) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    else if (lean) hipLaunchKernelGGL(k_shade<SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
 }
 

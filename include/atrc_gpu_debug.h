@@ -53,6 +53,11 @@ int agx_debug_guide_infer(agx_ctx *ctx, const float *params, const float *inputs
 int agx_debug_guide_train(agx_ctx *ctx, float *params, float *m, float *v, const float *samples, uint32_t n,
                           float wscale, float lr, int t, float *grad_out);
 
+/* The same gradient from the matrix-core training path that renders use (BF16
+ * operands, FP32 accumulation); no parameter update. */
+int agx_debug_guide_grad_mfma(agx_ctx *ctx, const float *params, const float *samples, uint32_t n, float wscale,
+                              float *grad_out);
+
 /* Run one plain device-to-device copy kernel (k_copy_calibration) of `bytes`
  * bytes on the context's stream and wait for it: a known-size reference for
  * calibrating hardware traffic counters.  Returns its device time in *ms. */
