@@ -431,6 +431,7 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     }
     c->accel = hs.accel;
     ds.features = hs.media.empty() ? 0u : SF_MEDIA;
+    if (hs.env_kind == AGX_ENV_IBL) ds.features |= SF_IBL;
     for (const DevMaterial &m : hs.mats) {
         if (m.kind == AGX_MAT_DISNEY) ds.features |= SF_DISNEY;
         if (m.kind == AGX_MAT_GLASS || m.kind == AGX_MAT_MIRROR || m.kind == AGX_MAT_INVISIBLE_SURFACE)

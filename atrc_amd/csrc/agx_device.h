@@ -157,7 +157,8 @@ enum : uint32_t {
     SF_DISNEY = 2u,   // Disney BSDF
     SF_DELTA = 4u,    // glass, mirror, invisible_surface
     SF_GUIDE = 8u,    // path guiding
-    SF_ALL = 15u,
+    SF_IBL = 16u,     // image-based environment light
+    SF_ALL = 31u,
 };
 
 // per-launch counters (collect_stats)

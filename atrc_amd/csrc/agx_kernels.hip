@@ -1694,10 +1694,12 @@ __device__ __forceinline__ float dir_phi(V d) {
     const float p = atan2f(d.y, d.x);
     return p < 0 ? p + 2 * PI_F : p;
 }
-// radiance arriving from direction d (native_sky.cpp:13-20; IBL::radiance, ibl.cpp:160-170)
+// radiance arriving from direction d (native_sky.cpp:13-20; IBL::radiance, ibl.cpp:160-170);
+// kernels compiled without SF_IBL serve scenes without an ibl
+template <uint32_t F = SF_ALL>
 __device__ V env_radiance(const DevScene &s, V d) {
     if (s.env_kind == AGX_ENV_NATIVE_SKY) return sky(s, d);
-    if (s.env_kind != AGX_ENV_IBL) return mk(0, 0, 0);
+    if (!(F & SF_IBL) || s.env_kind != AGX_ENV_IBL) return mk(0, 0, 0);
     const V dir = normalize(d);
     const float theta = acosf(smin(1.0f, smax(-1.0f, dir.z)));
     const float u = smin(1.0f, smax(0.0f, dir_phi(dir) / (2 * PI_F)));
@@ -1716,8 +1718,9 @@ __device__ __forceinline__ float env_patch_pdf(const DevScene &s, int px, int py
     const float in_patch = 1.0f / (2 * PI_F * ((*u1 - *u0) * (*cvmax - *cvmin)));
     return s.env_probs[py * s.env_pw + px] * in_patch;
 }
+template <uint32_t F = SF_ALL>
 __device__ float env_pdf(const DevScene &s, V d) {
-    if (s.env_kind != AGX_ENV_IBL) return SPHERE_PDF;
+    if (!(F & SF_IBL) || s.env_kind != AGX_ENV_IBL) return SPHERE_PDF;
     const V dir = normalize(d);
     const float theta = acosf(smin(1.0f, smax(-1.0f, dir.z)));
     const float u = dir_phi(dir) / (2 * PI_F), v = theta / PI_F;
@@ -1728,8 +1731,9 @@ __device__ float env_pdf(const DevScene &s, V d) {
 }
 // direction towards the environment and its pdf (NativeSky: uniform sphere;
 // IBL: alias-sampled patch, then uniform in cos theta and phi within it)
+template <uint32_t F = SF_ALL>
 __device__ V env_sample(const DevScene &s, float su, float sv, float sw, float *pdf) {
-    if (s.env_kind != AGX_ENV_IBL) {
+    if (!(F & SF_IBL) || s.env_kind != AGX_ENV_IBL) {
         *pdf = SPHERE_PDF;
         return uniform_sphere(su, sv);
     }
@@ -1919,11 +1923,11 @@ __device__ V resolve_mis(const DevScene &s, const PathBuffers &pb, uint32_t p, u
     const V o = mk(pb.ray_o[2 * p + 1]), d = mk(pb.ray_d[2 * p + 1]);
     if (ent < 0) {
         if (s.env_kind == AGX_ENV_NONE) return mk(0, 0, 0);
-        const V lr = env_radiance(s, d);
+        const V lr = env_radiance<F>(s, d);
         if (black(lr)) return mk(0, 0, 0);
         const V f = lr * bf * absdot;
         if (delta) return coef * (f / pdf);
-        return coef * (f / (pdf + env_pdf(s, d)));
+        return coef * (f / (pdf + env_pdf<F>(s, d)));
     }
     const DevEntity &e = s.ents[ent];
     if (e.light < 0) return mk(0, 0, 0);
@@ -2110,7 +2114,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
     if (ent < 0) {  // path_tracing.cpp:41-49
-        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance(s, rd);
+        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance<F>(s, rd);
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
         pb.rng_tr[p] = rtr.state;
@@ -2247,9 +2251,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                 }
             } else {
                 float epdf;
-                const V dir = env_sample(s, s5[0], s5[1], s5[2], &epdf);
+                const V dir = env_sample<F>(s, s5[0], s5[1], s5[2], &epdf);
                 const V lpos = env_emit_pos(s, sp.pos, dir);
-                const V rad = env_radiance(s, dir);
+                const V rad = env_radiance<F>(s, dir);
                 if (!black(rad)) {
                     const V tl = normalize(lpos - sp.pos);
                     const V f = bsdf_eval<F>(b, tl, sp.wr);
@@ -2592,7 +2596,7 @@ void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &p
     if (!max_n) return;
     // lean variants serve scenes of Lambert / GGX / black surfaces in vacuum,
     // unguided or guided; every other scene takes the full kernel
-    const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA)) == 0;
+    const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA | SF_IBL)) == 0;
     if (lean && !rc.guide) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else if (lean) hipLaunchKernelGGL(k_shade<SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
