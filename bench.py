@@ -35,7 +35,9 @@ image.  The reference is plain PT with a different seed for
 ``--guided-ref-factor`` times as long.  relMSE = mean over pixels and
 channels of (x - y)^2 / (y^2 + 1e-2) after scaling both images by 1 / mean(y)
 (the room is dark).  MLP utilisation = 61,440 FLOP per evaluated vertex / the
-inference kernel's device time, against the 2.5 PFLOP/s dense BF16 peak.
+inference kernel's device time, and 3 x 61,440 FLOP per trained sample and
+step / the training device time (gather, matrix-core forward and backward,
+reduction, Adam), both against the 2.5 PFLOP/s dense BF16 peak.
 """
 from __future__ import annotations
 
@@ -189,7 +191,7 @@ def guided_leg(ctx, args) -> dict:
             "infer_tflops": infer_tflops, "infer_frac_bf16_peak": infer_tflops / BF16_PEAK_TFLOPS,
             "train_samples": gst["guide_samples"], "trained_sample_steps": gst["guide_trained"],
             "train_ms": gst["guide_train_ms"],
-            "train_tflops": train_tflops, "train_frac_fp32_peak": train_tflops / FP32_PEAK_TFLOPS,
+            "train_tflops": train_tflops, "train_frac_bf16_peak": train_tflops / BF16_PEAK_TFLOPS,
             "device_share": (gst["guide_infer_ms"] + gst["guide_train_ms"]) / max(gst["total_ms"], 1e-9),
         },
     }
@@ -281,8 +283,8 @@ def main() -> int:
     ap.add_argument("--guided-chunk", type=int, default=4)
     ap.add_argument("--guided-ref-factor", type=float, default=16.0)
     ap.add_argument("--guided-train-frac", type=float, default=0.3)
-    ap.add_argument("--guided-steps", type=int, default=1)
-    ap.add_argument("--guided-batch", type=int, default=32768)
+    ap.add_argument("--guided-steps", type=int, default=8)
+    ap.add_argument("--guided-batch", type=int, default=65536)
     ap.add_argument("--copy-calibration-mb", type=int, default=256,
                     help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()
