@@ -156,6 +156,23 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     return base + rank;
 }
 
+// Two predicates appended to one queue with one atomic per wave: the lanes with
+// `a` take the first slots, then the lanes with `b`.
+__device__ __forceinline__ void wave_append2(uint32_t *counter, bool a, bool b, uint32_t *ia, uint32_t *ib) {
+    const unsigned long long ma = __ballot(a), mb = __ballot(b);
+    const uint32_t lane = __lane_id();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const uint32_t na = uint32_t(__popcll(ma));
+    uint32_t base = 0;
+    if (ma | mb) {
+        const int leader = __ffsll((long long)(ma | mb)) - 1;
+        if (int(lane) == leader) base = atomicAdd(counter, na + uint32_t(__popcll(mb)));
+        base = __shfl(base, leader);
+    }
+    *ia = base + uint32_t(__popcll(ma & below));
+    *ib = base + na + uint32_t(__popcll(mb & below));
+}
+
 // ---------------------------------------------------------------- sampling distributions
 __device__ void unit_disk(float u, float v, float *ox, float *oy) {  // Shirley-Chiu concentric map
     const float sx = 2 * u - 1, sy = 2 * v - 1;
@@ -1627,11 +1644,23 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         pb.ray_o[2 * p] = f4(o, 0.0f);
         pb.ray_d[2 * p] = f4(d, INF_F);
     }
-    const uint32_t qi = wave_append(&pb.cnt_out[2], launch);
-    const uint32_t ci = wave_append(&pb.cnt_out[0], launch);
-    if (launch) {
-        pb.q_out[qi] = uint32_t(p);
-        pb.q_closest[ci] = uint32_t(2 * p);
+    if (rc.min_depth >= 1 && rc.max_depth >= 1) {
+        // no roulette at depth 1 and a first segment to trace: every camera path is
+        // queued, at its own index, so no counter is contended; path 0 writes
+        // the queue sizes
+        pb.q_out[p] = uint32_t(p);
+        pb.q_closest[p] = uint32_t(2 * p);
+        if (p == 0) {
+            pb.cnt_out[2] = uint32_t(rc.n_paths);
+            pb.cnt_out[0] = uint32_t(rc.n_paths);
+        }
+    } else {
+        const uint32_t qi = wave_append(&pb.cnt_out[2], launch);
+        const uint32_t ci = wave_append(&pb.cnt_out[0], launch);
+        if (launch) {
+            pb.q_out[qi] = uint32_t(p);
+            pb.q_closest[ci] = uint32_t(2 * p);
+        }
     }
     // the first path of every block of 2^shift clears the block's record slot
     if (rc.guide_record && (p & ((1u << rc.guide_rec_shift) - 1u)) == 0)
@@ -2365,9 +2394,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     if (cont) flags |= PF_CONT;
     // queue the rays of this vertex (wave-aggregated appends)
     {
-        const uint32_t ci = wave_append(&pb.cnt_out[0], cont);
+        uint32_t ci, mi_;
+        wave_append2(&pb.cnt_out[0], cont, push_mis, &ci, &mi_);
         if (cont) pb.q_closest[ci] = 2 * p;
-        const uint32_t mi_ = wave_append(&pb.cnt_out[0], push_mis);
         if (push_mis) pb.q_closest[mi_] = 2 * p + 1;
         for (int32_t l = 0; l < rc.L; ++l) {
             const bool v = (shadow_mask >> l) & 1u;
