@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 7
+AGX_ABI_VERSION = 8
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -124,6 +124,7 @@ class agx_stats_t(C.Structure):
         ("shade_ms", C.c_double), ("trace_closest_launches", C.c_uint64), ("trace_shadow_launches", C.c_uint64),
         ("guide_infer_ms", C.c_double), ("guide_train_ms", C.c_double), ("guide_samples", C.c_uint64),
         ("guide_steps", C.c_uint64), ("guide_rows", C.c_uint64), ("guide_trained", C.c_uint64), ("node_bytes", C.c_uint32), ("prim_bytes", C.c_uint32), ("info_bytes", C.c_uint32), ("reserved", C.c_uint32),
+        ("host_ms", C.c_double), ("readback_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -261,10 +262,19 @@ class Context:
     def upload_scene(self, desc: "agx_scene_desc") -> None:
         _check(self.lib.agx_upload_scene(self.h, C.byref(desc)))
 
-    def render(self, params: agx_render_params, cancel: C.c_int32 | None = None, aovs: bool = True) -> dict:
-        """agx_render; aovs=False reads back the image only (albedo / normal / denoise stay None)."""
+    def render(self, params: agx_render_params, cancel: C.c_int32 | None = None, aovs: bool = True,
+               out: np.ndarray | None = None) -> dict:
+        """agx_render; aovs=False reads back the image only (albedo / normal / denoise stay None).
+        out: a C-contiguous float32 (height, width, 3) array the image is written into
+        (a render loop reuses it instead of mapping fresh host pages every call)."""
         wh = params.width * params.height
-        rgb = np.zeros((params.height, params.width, 3), np.float32)
+        if out is not None:
+            if (out.dtype != np.float32 or out.shape != (params.height, params.width, 3)
+                    or not out.flags["C_CONTIGUOUS"]):
+                raise ValueError("out must be a C-contiguous float32 array of shape (height, width, 3)")
+            rgb = out
+        else:
+            rgb = np.zeros((params.height, params.width, 3), np.float32)
         alb = np.zeros_like(rgb) if aovs else None
         nor = np.zeros_like(rgb) if aovs else None
         den = np.zeros((params.height, params.width), np.float32) if aovs else None

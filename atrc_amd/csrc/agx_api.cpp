@@ -481,6 +481,7 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
 int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *cancel_flag, float *rgb,
                float *albedo, float *normal, float *denoise) {
     if (!c || !p) return fail(AGX_E_INVALID_ARG, "null argument");
+    const auto t_call = std::chrono::steady_clock::now();
     if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0) return fail(AGX_E_INVALID_ARG, "width, height and spp must be positive");
     if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
@@ -726,6 +727,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     }
     st.guide_steps = c->g_steps;
     c->stats = st;
+    const auto t_device_end = std::chrono::steady_clock::now();
     // normalise into host outputs
     if (rgb || albedo || normal || denoise) {
         // device planes and pinned staging persist across renders (no allocation per call)
@@ -754,6 +756,9 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
         for (const Plane &pl : planes)
             if (pl.user) std::memcpy(pl.user, hout + pl.off * size_t(wh), pl.n * size_t(wh) * sizeof(float));
     }
+    const auto t_end = std::chrono::steady_clock::now();
+    c->stats.host_ms = std::chrono::duration<double, std::milli>(t_end - t_call).count();
+    c->stats.readback_ms = std::chrono::duration<double, std::milli>(t_end - t_device_end).count();
     if (cancelled) return fail(AGX_E_CANCELLED, "render cancelled");
     return AGX_OK;
 }

@@ -1556,9 +1556,11 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
-// TM = traversal variant (TM_*).
+// TM = traversal variant (TM_*).  Eight waves per SIMD: the traversal is bound by
+// the latency of its node fetches, and the default allocation (66 VGPRs) left
+// room for seven; eight fit in 64 without spilling.
 template <bool STATS, int TM>
-__global__ void __launch_bounds__(256) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[0];
     if (blockIdx.x * blockDim.x >= n) return;  // whole block past the queue
     AGX_LDS_COLUMN(TM);

@@ -289,10 +289,11 @@ def main() -> int:
                     help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is described in "
-                          "DESIGN.md and not implemented", "n_gpus": args.gpus}), flush=True)
+        print(json.dumps({"error": "runs on one GPU only (--gpus 1); sharding the render over several "
+                          "GPUs is not implemented", "n_gpus": args.gpus}), flush=True)
         return 2
 
+    import numpy as np
     from atrc_amd import capi, scenes
 
     if args.config == "batched":
@@ -313,14 +314,16 @@ def main() -> int:
         ms = ctx.copy_kernel(nbytes)
         copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GB/s": 2 * nbytes / (ms * 1e-3) / 1e9}
         log(f"[bench] copy calibration kernel: {nbytes >> 20} MiB in {ms:.3f} ms ({copy['GB/s']:.0f} GB/s r+w)")
+    # one host image buffer for every render, as a render loop keeps it
+    img = np.empty((H, W, 3), np.float32)
     # warmup
     if args.warmup > 0:
-        ctx.render(scenes.render_params(W, H, args.warmup, cam, seed=42, **tr))
+        ctx.render(scenes.render_params(W, H, args.warmup, cam, seed=42, **tr), aovs=False, out=img)
         log(f"[bench] warmup {args.warmup} spp done")
     # counting pass: per-ray algorithmic BVH bytes
     st_steps = max(1, min(args.stats_steps, args.steps))
     ctx.render(scenes.render_params(W, H, st_steps, cam, seed=42, collect_stats=True,
-                                    sample_begin=args.warmup, **tr))
+                                    sample_begin=args.warmup, **tr), aovs=False, out=img)
     cs = ctx.stats()
     bytes_per_closest = cs["closest_bvh_bytes"] / max(1, cs["closest_rays"])
     bytes_per_shadow = cs["shadow_bvh_bytes"] / max(1, cs["shadow_rays"])
@@ -330,14 +333,15 @@ def main() -> int:
     # timed region: the library synchronises its stream before returning
     p = scenes.render_params(W, H, args.steps, cam, seed=42, sample_begin=args.warmup + st_steps, **tr)
     t0 = time.perf_counter()
-    ctx.render(p, aovs=False)   # the image comes back to the host; the AOV planes are not asked for
+    ctx.render(p, aovs=False, out=img)   # the image comes back to the host; the AOV planes are not asked for
     t1 = time.perf_counter()
     st = ctx.stats()
     wall = t1 - t0
     rays = st["closest_rays"] + st["shadow_rays"]
     value = rays / wall / 1e6
     log(f"[bench] timed {args.steps} spp: {wall:.3f} s wall, device {st['total_ms']:.1f} ms, "
-        f"closest {st['trace_closest_ms']:.1f} ms, shadow {st['trace_shadow_ms']:.1f} ms, shade {st['shade_ms']:.1f} ms")
+        f"closest {st['trace_closest_ms']:.1f} ms, shadow {st['trace_shadow_ms']:.1f} ms, shade {st['shade_ms']:.1f} ms; "
+        f"host call {st['host_ms']:.1f} ms of which image readback {st['readback_ms']:.1f} ms")
 
     closest_bytes = bytes_per_closest * st["closest_rays"]
     achieved = closest_bytes / (st["trace_closest_ms"] * 1e-3) / 1e9
@@ -359,6 +363,7 @@ def main() -> int:
         "rays": {"closest": st["closest_rays"], "shadow": st["shadow_rays"], "paths": st["paths"]},
         "samples_per_s": st["paths"] / wall,
         "device_ms": {k: st[k] for k in ("total_ms", "trace_closest_ms", "trace_shadow_ms", "shade_ms")},
+        "host_ms": {"call": round(st["host_ms"], 3), "image_readback": round(st["readback_ms"], 3)},
         "roofline": roofline,
         "guided_vs_plain_relmse_ratio": None,
         "guided": None,

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 7
+#define AGX_ABI_VERSION 8
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -282,6 +282,9 @@ typedef struct agx_stats_t {
     uint64_t guide_trained;    /* samples passed through training (ring size x Adam steps) */
     uint32_t node_bytes, prim_bytes, info_bytes; /* record sizes used for the byte counts */
     uint32_t reserved;
+    double   host_ms;          /* wall time of the agx_render call */
+    double   readback_ms;      /* wall time from the end of the device work to the return
+                                  (film resolve, copy to the host outputs) */
 } agx_stats_t;
 
 typedef struct agx_ctx agx_ctx;
