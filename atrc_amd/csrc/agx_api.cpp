@@ -63,8 +63,13 @@ struct agx_ctx {
     // wavefront buffers
     PathBuffers pb{};
     std::vector<void *> wave_allocs;
+    uint4 *qc_blk = nullptr;  // per-block counts / offsets of the queue compaction
     size_t cap_paths = 0;
     int cap_L = -1;
+    // paths per wavefront pass, chosen once per light count (the free-memory
+    // query takes milliseconds, and the choice stays stable from call to call)
+    size_t pass_paths = 0;
+    int pass_paths_L = -1;
     int64_t cap_wh = 0;
     uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
@@ -130,6 +135,7 @@ struct agx_ctx {
         free_guide_wave();
         pb = PathBuffers{};
         counters = nullptr;
+        qc_blk = nullptr;
         cap_paths = 0;
         cap_L = -1;
         cap_wh = 0;
@@ -238,6 +244,7 @@ struct agx_ctx {
         HIP_TRY(walloc(&pb.q_in, n));
         HIP_TRY(walloc(&pb.q_out, n));
         HIP_TRY(walloc(&counters, 8));
+        HIP_TRY(walloc(&qc_blk, queue_compact_blocks(uint32_t(n))));
         HIP_TRY(walloc(&pb.film_sum, size_t(10 * wh)));
         HIP_TRY(walloc(&pb.film_comp, size_t(10 * wh)));
         HIP_TRY(walloc(&pb.film_count, size_t(wh)));
@@ -541,7 +548,11 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     // paths per sample layer: the image, or the image and its border when samples splat
     const int64_t layer = rc.splat ? int64_t(rc.EW) * rc.EH : wh;
     const int64_t total = layer * p->spp;
-    const size_t cap = max_paths_in_flight(rc.L);
+    if (c->pass_paths_L != rc.L) {
+        c->pass_paths = max_paths_in_flight(rc.L);
+        c->pass_paths_L = rc.L;
+    }
+    const size_t cap = c->pass_paths;
     int64_t per_pass;
     if (layer <= int64_t(cap)) per_pass = layer * std::min<int64_t>(p->spp, int64_t(cap) / layer);
     else per_pass = rc.splat ? layer : int64_t(cap);  // splatting needs whole layers in a pass
@@ -645,6 +656,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             if ((r = mark(&e2))) return r;
             launch_reset_counters(cout, c->stream);
             launch_shade(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
+            launch_queue_compact(c->pb, uint32_t(rc.n_paths), rc.L, c->qc_blk, c->stream);
             if ((r = mark(&e3))) return r;
             timed.push_back({e0, e1, 0});
             timed.push_back({e1, e2, 1});

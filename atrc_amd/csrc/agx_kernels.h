@@ -111,6 +111,10 @@ void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max
 void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st);
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st);
+// queues of the next bounce from the paths k_shade shaded (at most max_n of them);
+// blk holds queue_compact_blocks(max_n) entries
+uint32_t queue_compact_blocks(uint32_t max_n);
+void launch_queue_compact(const PathBuffers &pb, uint32_t max_n, int L, uint4 *blk, hipStream_t st);
 // network inputs of the vertices about to be shaded (q_in order)
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
                            hipStream_t st);

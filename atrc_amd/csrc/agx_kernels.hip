@@ -156,23 +156,6 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     return base + rank;
 }
 
-// Two predicates appended to one queue with one atomic per wave: the lanes with
-// `a` take the first slots, then the lanes with `b`.
-__device__ __forceinline__ void wave_append2(uint32_t *counter, bool a, bool b, uint32_t *ia, uint32_t *ib) {
-    const unsigned long long ma = __ballot(a), mb = __ballot(b);
-    const uint32_t lane = __lane_id();
-    const unsigned long long below = (1ull << lane) - 1ull;
-    const uint32_t na = uint32_t(__popcll(ma));
-    uint32_t base = 0;
-    if (ma | mb) {
-        const int leader = __ffsll((long long)(ma | mb)) - 1;
-        if (int(lane) == leader) base = atomicAdd(counter, na + uint32_t(__popcll(mb)));
-        base = __shfl(base, leader);
-    }
-    *ia = base + uint32_t(__popcll(ma & below));
-    *ib = base + na + uint32_t(__popcll(mb & below));
-}
-
 // ---------------------------------------------------------------- sampling distributions
 __device__ void unit_disk(float u, float v, float *ox, float *oy) {  // Shirley-Chiu concentric map
     const float sx = 2 * u - 1, sy = 2 * v - 1;
@@ -2394,26 +2377,150 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     if (pending) flags |= PF_PENDING;
     pb.radiance[p] = f4(L, 0);
     if (cont) flags |= PF_CONT;
-    // queue the rays of this vertex (wave-aggregated appends)
-    {
-        uint32_t ci, mi_;
-        wave_append2(&pb.cnt_out[0], cont, push_mis, &ci, &mi_);
-        if (cont) pb.q_closest[ci] = 2 * p;
-        if (push_mis) pb.q_closest[mi_] = 2 * p + 1;
-        for (int32_t l = 0; l < rc.L; ++l) {
-            const bool v = (shadow_mask >> l) & 1u;
-            const uint32_t si = wave_append(&pb.cnt_out[1], v);
-            if (v) pb.q_shadow[si] = uint32_t(size_t(p) * rc.L + l);
-        }
-        const bool again = (flags & (PF_CONT | PF_PENDING)) != 0;
-        const uint32_t qi = wave_append(&pb.cnt_out[2], again);
-        if (again) pb.q_out[qi] = p;
-    }
+    // the rays of this vertex: 2p (PF_CONT), 2p + 1 (PF_MIS) and the shadow rays of
+    // the lights in shadow_mask; k_queue_* build the queues from these after the launch
+    (void)push_mis;
+    dep.w = int32_t(shadow_mask);
     pb.coef[p] = f4(coef, 0);
     pb.depth[p] = dep;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
     pb.rng_tr[p] = rtr.state;
+}
+
+// ---------------------------------------------------------------- queue compaction
+// The queues of the next bounce (closest-hit rays, shadow rays, paths to shade)
+// are built from the flags k_shade leaves on every path it shaded, in the order
+// of the shade queue: per-block counts, one block's exclusive scan of them, and a
+// scatter.  No atomics (a device-scope atomic on one counter serialises every
+// wave of a launch behind it), and the queues come out in path order, the same
+// in every run.
+constexpr uint32_t QC_THREADS = 256, QC_ITEMS = 8, QC_SPAN = QC_THREADS * QC_ITEMS;
+static_assert(MAX_LIGHTS < 32, "the shadow mask of a path is one 32-bit word");
+
+struct QPred { uint32_t cont, mis, mask, again; };
+__device__ __forceinline__ QPred queue_pred(const PathBuffers &pb, uint32_t p, uint32_t L) {
+    const uint32_t f = pb.flags[p];
+    QPred q;
+    q.cont = (f & PF_CONT) ? 1u : 0u;
+    q.mis = ((f & PF_MIS) && (f & PF_PENDING)) ? 1u : 0u;
+    q.again = (f & (PF_CONT | PF_PENDING)) ? 1u : 0u;
+    q.mask = (f & PF_PENDING) ? uint32_t(pb.depth[p].w) & ((1u << L) - 1u) : 0u;  // L <= MAX_LIGHTS
+    return q;
+}
+// exclusive prefix sum over the wave's lanes; *total = the wave's sum
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, uint32_t *total) {
+    const int lane = int(__lane_id());
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+// blk[b] = (closest rays, shadow rays, paths) of shade-queue entries [b * QC_SPAN, (b + 1) * QC_SPAN)
+__global__ void __launch_bounds__(QC_THREADS) k_queue_count(PathBuffers pb, uint32_t L, uint4 *blk) {
+    const uint32_t n = pb.cnt_in[2];
+    const uint32_t base = blockIdx.x * QC_SPAN;
+    if (base >= n) return;
+    uint32_t c0 = 0, c1 = 0, c2 = 0;
+    for (uint32_t k = 0; k < QC_ITEMS; ++k) {
+        const uint32_t tid = base + k * QC_THREADS + threadIdx.x;
+        if (tid < n) {
+            const QPred q = queue_pred(pb, pb.q_in[tid], L);
+            c0 += q.cont + q.mis;
+            c1 += uint32_t(__popc(q.mask));
+            c2 += q.again;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        c0 += __shfl_xor(c0, off);
+        c1 += __shfl_xor(c1, off);
+        c2 += __shfl_xor(c2, off);
+    }
+    __shared__ uint32_t red[3][QC_THREADS / 64];
+    const uint32_t w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = c0; red[1][w] = c1; red[2][w] = c2; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint4 t = make_uint4(0, 0, 0, 0);
+        for (uint32_t j = 0; j < QC_THREADS / 64; ++j) { t.x += red[0][j]; t.y += red[1][j]; t.z += red[2][j]; }
+        blk[blockIdx.x] = t;
+    }
+}
+
+// one block: blk[b] <- exclusive prefix of the block counts; cnt_out[0..2] <- totals
+__global__ void __launch_bounds__(1024) k_queue_scan(PathBuffers pb, uint4 *blk) {
+    const uint32_t n = pb.cnt_in[2];
+    const uint32_t nb = (n + QC_SPAN - 1) / QC_SPAN;
+    __shared__ uint32_t wsum[3][16];
+    __shared__ uint32_t carry[3];
+    if (threadIdx.x < 3) carry[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint4 v = i < nb ? blk[i] : make_uint4(0, 0, 0, 0);
+        uint32_t s0, s1, s2;
+        const uint32_t e0 = wave_excl_scan(v.x, &s0), e1 = wave_excl_scan(v.y, &s1), e2 = wave_excl_scan(v.z, &s2);
+        if ((threadIdx.x & 63) == 0) { wsum[0][w] = s0; wsum[1][w] = s1; wsum[2][w] = s2; }
+        __syncthreads();
+        uint32_t o0 = carry[0], o1 = carry[1], o2 = carry[2];
+        for (uint32_t j = 0; j < w; ++j) { o0 += wsum[0][j]; o1 += wsum[1][j]; o2 += wsum[2][j]; }
+        if (i < nb) blk[i] = make_uint4(o0 + e0, o1 + e1, o2 + e2, 0);
+        __syncthreads();  // every thread has read carry and wsum
+        if (threadIdx.x < 3)
+            for (uint32_t j = 0; j < 16; ++j) carry[threadIdx.x] += wsum[threadIdx.x][j];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pb.cnt_out[0] = carry[0];
+        pb.cnt_out[1] = carry[1];
+        pb.cnt_out[2] = carry[2];
+    }
+}
+
+// writes the entries of every shaded path at its block's offset + the rank of
+// its entries in the block (queue order; a path's 2p before its 2p + 1)
+__global__ void __launch_bounds__(QC_THREADS) k_queue_scatter(PathBuffers pb, uint32_t L, const uint4 *blk) {
+    const uint32_t n = pb.cnt_in[2];
+    const uint32_t base = blockIdx.x * QC_SPAN;
+    if (base >= n) return;
+    __shared__ uint32_t wsum[2][3][QC_THREADS / 64];
+    const uint4 o = blk[blockIdx.x];
+    uint32_t run0 = o.x, run1 = o.y, run2 = o.z;
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t k = 0; k < QC_ITEMS; ++k) {
+        const uint32_t tid = base + k * QC_THREADS + threadIdx.x;
+        QPred q{0u, 0u, 0u, 0u};
+        uint32_t p = 0;
+        if (tid < n) {
+            p = pb.q_in[tid];
+            q = queue_pred(pb, p, L);
+        }
+        uint32_t s0, s1, s2;
+        const uint32_t e0 = wave_excl_scan(q.cont + q.mis, &s0);
+        const uint32_t e1 = wave_excl_scan(uint32_t(__popc(q.mask)), &s1);
+        const uint32_t e2 = wave_excl_scan(q.again, &s2);
+        // double-buffered by k: the next write to this half follows another barrier
+        uint32_t (*ws)[QC_THREADS / 64] = wsum[k & 1];
+        if ((threadIdx.x & 63) == 0) { ws[0][w] = s0; ws[1][w] = s1; ws[2][w] = s2; }
+        __syncthreads();
+        uint32_t b0 = run0, b1 = run1, b2 = run2;
+        for (uint32_t j = 0; j < QC_THREADS / 64; ++j) {
+            const uint32_t a0 = ws[0][j], a1 = ws[1][j], a2 = ws[2][j];
+            if (j < w) { b0 += a0; b1 += a1; b2 += a2; }
+            run0 += a0; run1 += a1; run2 += a2;
+        }
+        if (q.cont) pb.q_closest[b0 + e0] = 2 * p;
+        if (q.mis) pb.q_closest[b0 + e0 + q.cont] = 2 * p + 1;
+        uint32_t si = b1 + e1;
+        for (uint32_t l = 0; l < L; ++l)
+            if ((q.mask >> l) & 1u) pb.q_shadow[si++] = p * L + l;
+        if (q.again) pb.q_out[b2 + e2] = p;
+    }
 }
 
 // accumulate the pass into the film in sample order, Neumaier-compensated
@@ -2631,6 +2738,16 @@ void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &p
     if (lean && !rc.guide) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else if (lean) hipLaunchKernelGGL(k_shade<SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+}
+
+uint32_t queue_compact_blocks(uint32_t max_n) { return (max_n + QC_SPAN - 1) / QC_SPAN + 1; }
+
+void launch_queue_compact(const PathBuffers &pb, uint32_t max_n, int L, uint4 *blk, hipStream_t st) {
+    const uint32_t nb = (max_n + QC_SPAN - 1) / QC_SPAN;
+    const uint32_t nl = uint32_t(L > 0 ? L : 0);
+    if (nb) hipLaunchKernelGGL(k_queue_count, dim3(nb), dim3(QC_THREADS), 0, st, pb, nl, blk);
+    hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, st, pb, blk);
+    if (nb) hipLaunchKernelGGL(k_queue_scatter, dim3(nb), dim3(QC_THREADS), 0, st, pb, nl, blk);
 }
 
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
