@@ -316,14 +316,18 @@ def main() -> int:
         log(f"[bench] copy calibration kernel: {nbytes >> 20} MiB in {ms:.3f} ms ({copy['GB/s']:.0f} GB/s r+w)")
     # one host image buffer for every render, as a render loop keeps it
     img = np.empty((H, W, 3), np.float32)
-    # warmup
+    # warmup: renders of the timed configuration (same pass size, so the wavefront
+    # buffers the timed render uses are allocated before it), on other samples
+    for i in range(args.warmup):
+        ctx.render(scenes.render_params(W, H, args.steps, cam, seed=42, sample_begin=i * args.steps, **tr),
+                   aovs=False, out=img)
     if args.warmup > 0:
-        ctx.render(scenes.render_params(W, H, args.warmup, cam, seed=42, **tr), aovs=False, out=img)
-        log(f"[bench] warmup {args.warmup} spp done")
+        log(f"[bench] warmup: {args.warmup} renders of {args.steps} spp done")
+    warm = args.warmup * args.steps
     # counting pass: per-ray algorithmic BVH bytes
     st_steps = max(1, min(args.stats_steps, args.steps))
     ctx.render(scenes.render_params(W, H, st_steps, cam, seed=42, collect_stats=True,
-                                    sample_begin=args.warmup, **tr), aovs=False, out=img)
+                                    sample_begin=warm, **tr), aovs=False, out=img)
     cs = ctx.stats()
     bytes_per_closest = cs["closest_bvh_bytes"] / max(1, cs["closest_rays"])
     bytes_per_shadow = cs["shadow_bvh_bytes"] / max(1, cs["shadow_rays"])
@@ -331,7 +335,7 @@ def main() -> int:
         f"{cs['closest_node_visits'] / max(1, cs['closest_rays']):.1f} node tests/closest ray")
 
     # timed region: the library synchronises its stream before returning
-    p = scenes.render_params(W, H, args.steps, cam, seed=42, sample_begin=args.warmup + st_steps, **tr)
+    p = scenes.render_params(W, H, args.steps, cam, seed=42, sample_begin=warm + st_steps, **tr)
     t0 = time.perf_counter()
     ctx.render(p, aovs=False, out=img)   # the image comes back to the host; the AOV planes are not asked for
     t1 = time.perf_counter()
