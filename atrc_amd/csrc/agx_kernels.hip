@@ -621,8 +621,7 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
     const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
     const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
     const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
-    uint32_t stk_k[CAP];
-    float stk_t[CAP];
+    uint2 stk[CAP];  // (node code, entry distance bits): one 8-byte scratch access per push / pop
     int sp = 0;
     uint32_t code = 0;
     bool hit = false;
@@ -634,9 +633,9 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
             b4_children(N + size_t(code) * BVH4_NODE_VEC4, o, inv, tmin, tmax, t, k);
             cswap(t, k, 0, 1); cswap(t, k, 2, 3); cswap(t, k, 0, 2); cswap(t, k, 1, 3); cswap(t, k, 1, 2);
             if (t[0] != INF_F) {
-                if (t[3] != INF_F && sp < CAP) { stk_k[sp] = k[3]; stk_t[sp] = t[3]; ++sp; }
-                if (t[2] != INF_F && sp < CAP) { stk_k[sp] = k[2]; stk_t[sp] = t[2]; ++sp; }
-                if (t[1] != INF_F && sp < CAP) { stk_k[sp] = k[1]; stk_t[sp] = t[1]; ++sp; }
+                if (t[3] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[3], __float_as_uint(t[3]));
+                if (t[2] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[2], __float_as_uint(t[2]));
+                if (t[1] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[1], __float_as_uint(t[1]));
                 code = k[0];
                 continue;
             }
@@ -665,8 +664,8 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
         // next pending child that may still hold a closer hit
         for (;;) {
             if (sp == 0) return hit;
-            --sp;
-            if (stk_t[sp] <= tmax) { code = stk_k[sp]; break; }
+            const uint2 e = stk[--sp];
+            if (__uint_as_float(e.y) <= tmax) { code = e.x; break; }
         }
     }
 }
@@ -2539,7 +2538,15 @@ __global__ void __launch_bounds__(256) k_film(RenderConst rc, PathBuffers pb) {
     if (i >= cover) return;
     const int64_t pixel = (rc.work_base + i) % wh;
     const int64_t layers = rc.n_paths >= wh ? int64_t(rc.n_paths) / wh : 1;
-    float *sum = pb.film_sum, *cmp = pb.film_comp;
+    // the pixel's sums stay in registers over the layers of the pass and are
+    // written back once (same operations in the same order as per-layer updates)
+    float sum[10], cmp[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        sum[k] = pb.film_sum[k * wh + pixel];
+        cmp[k] = pb.film_comp[k * wh + pixel];
+    }
+    float count = pb.film_count[pixel];
     for (int64_t j = 0; j < layers; ++j) {
         const int64_t slot = j * wh + i;
         if (slot >= rc.n_paths) break;
@@ -2548,9 +2555,16 @@ __global__ void __launch_bounds__(256) k_film(RenderConst rc, PathBuffers pb) {
         if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
         const float4 a = pb.gb_albedo[slot], n = pb.gb_normal[slot];
         const float vals[10] = {v.x, v.y, v.z, a.x, a.y, a.z, n.x, n.y, n.z, a.w};
-        for (int k = 0; k < 10; ++k) neumaier(&sum[k * wh + pixel], &cmp[k * wh + pixel], vals[k]);
-        pb.film_count[pixel] += 1.0f;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) neumaier(&sum[k], &cmp[k], vals[k]);
+        count += 1.0f;
     }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        pb.film_sum[k * wh + pixel] = sum[k];
+        pb.film_comp[k * wh + pixel] = cmp[k];
+    }
+    pb.film_count[pixel] = count;
 }
 
 // GaussianFilter::eval without its normalisation (it cancels in value / weight,
