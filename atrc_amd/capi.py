@@ -394,11 +394,11 @@ def debug_scene_summary(desc: agx_scene_desc) -> dict:
 
 
 def debug_bvh4(desc: agx_scene_desc) -> tuple:
-    """(nodes (n, 8, 4) float32, prims (m, 3, 4) float32) of an ACCEL_BVH4 or ACCEL_BVH4_WATERTIGHT descriptor."""
+    """(nodes (n, 4, 4) float32, prims (m, 3, 4) float32) of an ACCEL_BVH4 or ACCEL_BVH4_WATERTIGHT descriptor."""
     lib = load_library()
     nn, npr = C.c_uint64(0), C.c_uint64(0)
     _check(lib.agx_debug_bvh4(C.byref(desc), C.byref(nn), C.byref(npr), None, None))
-    nodes = np.zeros((nn.value, 8, 4), np.float32)
+    nodes = np.zeros((nn.value, 4, 4), np.float32)
     prims = np.zeros((npr.value, 3, 4), np.float32)
     _check(lib.agx_debug_bvh4(C.byref(desc), C.byref(nn), C.byref(npr), fptr(nodes), fptr(prims)))
     return nodes, prims

@@ -210,6 +210,74 @@ float bitsf(uint32_t u) {
 
 // Widens a box by a few ulps so that the rounded slab test of a ray that
 // meets a primitive exactly on its bound never culls it.
+// A child bound as the traversal decodes it: origin + byte x power-of-two step,
+// FP32 multiply then add (no contraction), exactly as b4_children evaluates it.
+float q_decode(float o, uint32_t q, float step) {
+    const float m = float(q) * step;
+    return o + m;
+}
+
+// Writes one 64-byte node: per axis the children's union low corner and the
+// smallest power-of-two step with which byte 255 reaches the union's high
+// corner; every child's bounds as bytes whose decoded value encloses the exact
+// bound (q = 0 decodes to the origin, q = 255 to the union's top or beyond, so
+// the adjustments stop).
+void quantize_node(const Box boxes[4], const uint32_t codes[4], vec4 *n) {
+    float o[3], step[3];
+    uint32_t ebits = 0;
+    for (int a = 0; a < 3; ++a) {
+        float lo = std::numeric_limits<float>::infinity(), hi = -lo;
+        for (int k = 0; k < 4; ++k) {
+            if (codes[k] == BVH4_EMPTY) continue;
+            lo = std::min(lo, boxes[k].lo[a]);
+            hi = std::max(hi, boxes[k].hi[a]);
+        }
+        if (!(lo <= hi)) lo = hi = 0.0f;
+        int e = -126;
+        if (hi > lo) {
+            int ex = 0;
+            (void)std::frexp(double(hi - lo) / 255.0, &ex);
+            e = std::max(-126, ex - 1);
+        }
+        while (e < 127 && q_decode(lo, 255u, std::ldexp(1.0f, e)) < hi) ++e;
+        o[a] = lo;
+        step[a] = std::ldexp(1.0f, e);
+        ebits |= uint32_t(e + 127) << (8 * a);
+    }
+    uint32_t qw[6] = {0, 0, 0, 0, 0, 0};  // lo.x hi.x lo.y hi.y lo.z hi.z; byte k = child k
+    for (int k = 0; k < 4; ++k) {
+        if (codes[k] == BVH4_EMPTY) continue;
+        for (int a = 0; a < 3; ++a) {
+            const double rl = (double(boxes[k].lo[a]) - double(o[a])) / double(step[a]);
+            const double rh = (double(boxes[k].hi[a]) - double(o[a])) / double(step[a]);
+            uint32_t ql = uint32_t(std::clamp(std::floor(rl), 0.0, 255.0));
+            uint32_t qh = uint32_t(std::clamp(std::ceil(rh), 0.0, 255.0));
+            while (ql > 0 && q_decode(o[a], ql, step[a]) > boxes[k].lo[a]) --ql;
+            while (qh < 255 && q_decode(o[a], qh, step[a]) < boxes[k].hi[a]) ++qh;
+            qw[2 * a] |= ql << (8 * k);
+            qw[2 * a + 1] |= qh << (8 * k);
+        }
+    }
+    n[0] = vec4{o[0], o[1], o[2], bitsf(ebits)};
+    n[1] = vec4{bitsf(codes[0]), bitsf(codes[1]), bitsf(codes[2]), bitsf(codes[3])};
+    n[2] = vec4{bitsf(qw[0]), bitsf(qw[1]), bitsf(qw[2]), bitsf(qw[3])};
+    n[3] = vec4{bitsf(qw[4]), bitsf(qw[5]), 0.0f, 0.0f};
+}
+
+// decoded child boxes and codes of a node (as the traversal sees them)
+void decode_node(const vec4 *n, Box boxes[4], uint32_t codes[4]) {
+    const uint32_t eb = fbits(n[0].w);
+    const float o[3] = {n[0].x, n[0].y, n[0].z};
+    const uint32_t qw[6] = {fbits(n[2].x), fbits(n[2].y), fbits(n[2].z), fbits(n[2].w), fbits(n[3].x), fbits(n[3].y)};
+    codes[0] = fbits(n[1].x); codes[1] = fbits(n[1].y); codes[2] = fbits(n[1].z); codes[3] = fbits(n[1].w);
+    for (int k = 0; k < 4; ++k)
+        for (int a = 0; a < 3; ++a) {
+            const float step = std::ldexp(1.0f, int((eb >> (8 * a)) & 0xffu) - 127);
+            boxes[k].lo[a] = q_decode(o[a], (qw[2 * a] >> (8 * k)) & 0xffu, step);
+            boxes[k].hi[a] = q_decode(o[a], (qw[2 * a + 1] >> (8 * k)) & 0xffu, step);
+        }
+}
+
 void pad(Box *b) {
     for (int k = 0; k < 3; ++k) {
         const float m = std::max(std::fabs(b->lo[k]), std::fabs(b->hi[k]));
@@ -269,17 +337,7 @@ struct Collapser {
                 codes[k] = emit(kids[k], depth + 1);
             }
         }
-        vec4 *n = &out->nodes[size_t(idx) * BVH4_NODE_VEC4];
-        for (int k = 0; k < 4; ++k) {
-            const bool used = codes[k] != BVH4_EMPTY;
-            (&n[0].x)[k] = used ? boxes[k].lo[0] : 0;
-            (&n[1].x)[k] = used ? boxes[k].hi[0] : 0;
-            (&n[2].x)[k] = used ? boxes[k].lo[1] : 0;
-            (&n[3].x)[k] = used ? boxes[k].hi[1] : 0;
-            (&n[4].x)[k] = used ? boxes[k].lo[2] : 0;
-            (&n[5].x)[k] = used ? boxes[k].hi[2] : 0;
-            (&n[6].x)[k] = bitsf(codes[k]);
-        }
+        quantize_node(boxes, codes, &out->nodes[size_t(idx) * BVH4_NODE_VEC4]);
         return idx;
     }
 };
@@ -305,15 +363,13 @@ void build_bvh4(const std::vector<PrimRef> &refs, int max_leaf, Bvh4 *out) {
     double cost = 0;
     const size_t nn = out->nodes.size() / BVH4_NODE_VEC4;
     for (size_t i = 0; i < nn; ++i) {
-        const vec4 *n = &out->nodes[i * BVH4_NODE_VEC4];
+        Box boxes[4];
+        uint32_t codes[4];
+        decode_node(&out->nodes[i * BVH4_NODE_VEC4], boxes, codes);
         for (int k = 0; k < 4; ++k) {
-            const uint32_t code = fbits((&n[6].x)[k]);
+            const uint32_t code = codes[k];
             if (code == BVH4_EMPTY) continue;
-            Box bx;
-            bx.lo[0] = (&n[0].x)[k]; bx.hi[0] = (&n[1].x)[k];
-            bx.lo[1] = (&n[2].x)[k]; bx.hi[1] = (&n[3].x)[k];
-            bx.lo[2] = (&n[4].x)[k]; bx.hi[2] = (&n[5].x)[k];
-            const double a = double(bx.area()) / root_area;
+            const double a = double(boxes[k].area()) / root_area;
             cost += (code & BVH4_LEAF) ? a * double(((code >> 27) & 15u) + 1) : a * C_TRAV;
         }
     }

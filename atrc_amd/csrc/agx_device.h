@@ -25,16 +25,19 @@ struct alignas(16) vec4 {
 constexpr uint32_t INTERIOR = 0xffffffffu;   // BLAS node: start == INTERIOR marks an interior node
 constexpr uint32_t TLAS_LEAF = 0x80000000u;  // TLAS node: left word has this bit for leaves
 constexpr int MAX_LIGHTS = 16;
-// BVH4 (production acceleration mode): a node is 8 vec4 = 128 B:
-//   [0] lo.x of children 0..3, [1] hi.x, [2] lo.y, [3] hi.y, [4] lo.z, [5] hi.z,
-//   [6] child codes (uint bits), [7] unused.
+// BVH4 (production acceleration mode): a node is 4 vec4 = 64 B:
+//   [0] (origin.xyz, step exponents: byte a = e_a + 127, step_a = 2^e_a)
+//   [1] child codes (uint bits)
+//   [2] (lo.x, hi.x, lo.y, hi.y), [3] (lo.z, hi.z, 0, 0): each a uint with byte k
+//       = bound of child k; bound = origin + float(byte) * step (FP32 multiply,
+//       then add), always enclosing the child's exact box.
 // Child code: BVH4_EMPTY, an interior node index (bit 31 clear), or a leaf
 // BVH4_LEAF | (count - 1) << 27 | first primitive.  A primitive is 3 vec4:
 //   (A, entity), (B - A, slot), (C - A, 0); slot = index into blas_prims/blas_info
 //   of a mesh triangle, or BVH4_ANALYTIC for a whole analytic entity.  In the
 //   watertight mode a mesh triangle is (A, entity), (B, slot), (C, 0) with the
 //   mesh's own vertex coordinates.
-constexpr int BVH4_NODE_VEC4 = 8;
+constexpr int BVH4_NODE_VEC4 = 4;
 constexpr uint32_t BVH4_EMPTY = 0xffffffffu;
 constexpr uint32_t BVH4_LEAF = 0x80000000u;
 constexpr uint32_t BVH4_FIRST_MASK = 0x07ffffffu;

@@ -595,15 +595,29 @@ __device__ __forceinline__ float b4_slab(float lx, float hx, float ly, float hy,
     return t0 <= t1 * 1.000000477f ? t0 : INF_F;
 }
 
+// child bound j of a quantized node: origin + byte j of w times the step
+__device__ __forceinline__ float b4_q(float org, uint32_t w, int j, float step) {
+    const float m = float((w >> (8 * j)) & 0xffu) * step;
+    return org + m;
+}
+
+// 64-byte node: decode the four children's (conservative) boxes and slab-test them
 __device__ __forceinline__ void b4_children(const float4 *n, V o, V inv, float tmin, float tmax, float t[4],
                                             uint32_t k[4]) {
-    const float4 lx = n[0], hx = n[1], ly = n[2], hy = n[3], lz = n[4], hz = n[5], cc = n[6];
+    const float4 h = n[0], cc = n[1], qa = n[2], qb = n[3];
+    const uint32_t eb = __float_as_uint(h.w);
+    const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
+                sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+    const uint32_t wlx = __float_as_uint(qa.x), whx = __float_as_uint(qa.y), wly = __float_as_uint(qa.z),
+                   why = __float_as_uint(qa.w), wlz = __float_as_uint(qb.x), whz = __float_as_uint(qb.y);
     k[0] = __float_as_uint(cc.x); k[1] = __float_as_uint(cc.y);
     k[2] = __float_as_uint(cc.z); k[3] = __float_as_uint(cc.w);
-    t[0] = k[0] != BVH4_EMPTY ? b4_slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, o, inv, tmin, tmax) : INF_F;
-    t[1] = k[1] != BVH4_EMPTY ? b4_slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, o, inv, tmin, tmax) : INF_F;
-    t[2] = k[2] != BVH4_EMPTY ? b4_slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, o, inv, tmin, tmax) : INF_F;
-    t[3] = k[3] != BVH4_EMPTY ? b4_slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, o, inv, tmin, tmax) : INF_F;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        t[j] = k[j] != BVH4_EMPTY
+                   ? b4_slab(b4_q(h.x, wlx, j, sx), b4_q(h.x, whx, j, sx), b4_q(h.y, wly, j, sy),
+                             b4_q(h.y, why, j, sy), b4_q(h.z, wlz, j, sz), b4_q(h.z, whz, j, sz), o, inv, tmin, tmax)
+                   : INF_F;
 }
 
 __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {

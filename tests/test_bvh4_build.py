@@ -1,6 +1,7 @@
 """Structure of the production BVH4 (AGX_ACCEL_BVH4) built at upload, checked
 on the CPU through agx_debug_bvh4: a tree over every primitive exactly once,
-leaves of at most 4 primitives, every child box enclosing its subtree, and
+leaves of at most 4 primitives, every child box -- decoded from its bytes as
+the traversal decodes it, in FP32 -- enclosing its subtree, and
 primitive records that carry the mesh triangles of Atrc's own triangle BVH
 (src/tracer/src/core/geometry/triangle_bvh.cpp:24-27, 225-304) unchanged.
 """
@@ -14,8 +15,42 @@ LEAF = 0x80000000
 ANALYTIC = 0xFFFFFFFF
 
 
+def _decode(nodes):
+    """Child boxes of 64-byte nodes as b4_children decodes them: origin +
+    float(byte) * 2^e, an FP32 multiply then an FP32 add."""
+    u = nodes.view(np.uint32)
+    eb = u[:, 0, 3]
+    org = nodes[:, 0, :3]
+    words = [u[:, 2, 0], u[:, 2, 1], u[:, 2, 2], u[:, 2, 3], u[:, 3, 0], u[:, 3, 1]]
+    lo = np.zeros((len(nodes), 4, 3), np.float32)
+    hi = np.zeros((len(nodes), 4, 3), np.float32)
+    for a in range(3):
+        step = (((eb >> np.uint32(8 * a)) & np.uint32(0xFF)) << np.uint32(23)).astype(np.uint32).view(np.float32)
+        for k in range(4):
+            ql = ((words[2 * a] >> np.uint32(8 * k)) & np.uint32(0xFF)).astype(np.float32)
+            qh = ((words[2 * a + 1] >> np.uint32(8 * k)) & np.uint32(0xFF)).astype(np.float32)
+            lo[:, k, a] = org[:, a] + ql * step
+            hi[:, k, a] = org[:, a] + qh * step
+    return u[:, 1, :], lo, hi
+
+
+def test_decode_is_fp32_multiply_then_add():
+    # a hand-made node: origin (1, -2, 0.5), steps 2^-3, 2^0, 2^-8; child 2 bytes chosen
+    n = np.zeros((1, 4, 4), np.float32)
+    u = n.view(np.uint32)
+    n[0, 0, :3] = [1.0, -2.0, 0.5]
+    u[0, 0, 3] = (124) | (127 << 8) | (119 << 16)
+    u[0, 1, :] = [0xFFFFFFFF, 0xFFFFFFFF, 5, 0xFFFFFFFF]
+    u[0, 2, :] = [8 << 16, 255 << 16, 3 << 16, 4 << 16]
+    u[0, 3, :2] = [0, 128 << 16]
+    codes, lo, hi = _decode(n)
+    assert codes[0, 2] == 5
+    np.testing.assert_array_equal(lo[0, 2], [2.0, 1.0, 0.5])
+    np.testing.assert_array_equal(hi[0, 2], [1.0 + 255 / 8, 2.0, 1.0])
+
+
 def _walk(nodes, prims):
-    codes = nodes[:, 6, :].view(np.uint32)
+    codes, dlo, dhi = _decode(nodes)
     seen_nodes = np.zeros(len(nodes), np.int32)
     seen_prims = np.zeros(len(prims), np.int32)
     leaf_sizes = []
@@ -43,8 +78,8 @@ def _walk(nodes, prims):
             c = int(codes[n, k])
             if c == EMPTY:
                 continue
-            box_lo = np.array([nodes[n, 0, k], nodes[n, 2, k], nodes[n, 4, k]], np.float64)
-            box_hi = np.array([nodes[n, 1, k], nodes[n, 3, k], nodes[n, 5, k]], np.float64)
+            box_lo = dlo[n, k].astype(np.float64)
+            box_hi = dhi[n, k].astype(np.float64)
             if c & LEAF:
                 first, cnt = c & 0x07FFFFFF, ((c >> 27) & 15) + 1
                 leaf_sizes.append(cnt)
