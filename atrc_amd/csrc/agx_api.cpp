@@ -70,6 +70,7 @@ struct agx_ctx {
     // query takes milliseconds, and the choice stays stable from call to call)
     size_t pass_paths = 0;
     int pass_paths_L = -1;
+    size_t pass_paths_limit = 0;  // 0: none; batch workers, which render side by side, use 2^22
     int64_t cap_wh = 0;
     uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
@@ -550,6 +551,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     const int64_t total = layer * p->spp;
     if (c->pass_paths_L != rc.L) {
         c->pass_paths = max_paths_in_flight(rc.L);
+        if (c->pass_paths_limit) c->pass_paths = std::min(c->pass_paths, c->pass_paths_limit);
         c->pass_paths_L = rc.L;
     }
     const size_t cap = c->pass_paths;
@@ -925,6 +927,9 @@ int agx_render_batch(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_t work
         agx_ctx *w = nullptr;
         const int r = agx_create(c->device, &w);
         if (r != AGX_OK) return r;
+        // several scenes in flight share the chip: passes of 4 M paths keep each
+        // worker's buffers small and its allocations few as scene sizes vary
+        w->pass_paths_limit = size_t(1) << 22;
         c->batch_workers.push_back(w);
     }
     std::string err;
