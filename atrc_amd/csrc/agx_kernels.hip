@@ -1552,15 +1552,31 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
+// The dispatcher deals blocks to the eight XCDs round-robin by block index.
+// The traversal kernels map them so that XCD x takes the x-th contiguous
+// eighth of the queue: queues are in path (pixel) order, so an XCD's rays come
+// from one band of the image and share BVH nodes in that XCD's own L2.  The
+// logical block of this physical block for a queue of n rays, or -1; the grid
+// holds at least ceil(n / 256) + 7 blocks, so every logical block is covered once.
+__device__ __forceinline__ int32_t xcd_block(uint32_t n) {
+    const uint32_t nb = (n + 255u) / 256u;
+    const uint32_t per = (nb + 7u) / 8u;
+    const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3;
+    if (j >= per) return -1;
+    const uint32_t lb = x * per + j;
+    return lb < nb ? int32_t(lb) : -1;
+}
+
 // TM = traversal variant (TM_*).  Eight waves per SIMD: the traversal is bound by
 // the latency of its node fetches, and the default allocation (66 VGPRs) left
 // room for seven; eight fit in 64 without spilling.
 template <bool STATS, int TM>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[0];
-    if (blockIdx.x * blockDim.x >= n) return;  // whole block past the queue
+    const int32_t lb = xcd_block(n);
+    if (lb < 0) return;  // whole block past the queue
     AGX_LDS_COLUMN(TM);
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
     if (tid < n) {
@@ -1579,9 +1595,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 template <bool STATS, int TM>
 __global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[1];
-    if (blockIdx.x * blockDim.x >= n) return;
+    const int32_t lb = xcd_block(n);
+    if (lb < 0) return;
     AGX_LDS_COLUMN(TM);
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool occ = false;
     if (tid < n) {
@@ -2751,12 +2768,12 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
 void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                           hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, 256)), st, s, pb, cnt);
+    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, 256) + 7), st, s, pb, cnt);
 }
 void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, 256)), st, s, pb, cnt);
+    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, 256) + 7), st, s, pb, cnt);
 }
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
