@@ -92,6 +92,7 @@ struct agx_ctx {
     float *g_bias = nullptr, *g_samples = nullptr, *g_wsum = nullptr;
     __bf16 *g_packed = nullptr;
     uint32_t *g_count = nullptr;
+    uint32_t *g_gather_blk = nullptr;  // per-block counts / offsets of the sample gather
     uint32_t *g_ring = nullptr;      // sample ring (next slot, valid samples) over g_samples
     __bf16 *g_acts = nullptr, *g_dy = nullptr;  // matrix-core training: activations, output gradients
     uint32_t g_ring_cap = 0;         // capacity the ring state refers to (0: empty)
@@ -175,6 +176,7 @@ struct agx_ctx {
             HIP_TRY(galloc(guide_allocs, &g_bias, G_BIASES));
             HIP_TRY(galloc(guide_allocs, &g_packed, G_PACK));
             HIP_TRY(galloc(guide_allocs, &g_count, 4));
+            HIP_TRY(galloc(guide_allocs, &g_gather_blk, GUIDE_GATHER_BLOCKS));
             HIP_TRY(galloc(guide_allocs, &g_wsum, 4));
             HIP_TRY(galloc(guide_allocs, &g_ring, 2));
         }
@@ -685,7 +687,8 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             size_t ta, tb;
             if ((r = mark(&ta))) return r;
             HIP_TRY(hipMemsetAsync(c->g_count, 0, sizeof(uint32_t), c->stream));
-            launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_ring, c->stream);
+            launch_guide_gather(rc, c->pb, c->g_samples, guide_batch, c->g_count, c->g_ring, c->g_gather_blk,
+                                c->stream);
             launch_guide_ring_commit(c->g_ring, c->g_count, guide_batch, c->g_samples, c->g_wsum, c->stream);
             for (int k = 0; k < guide_steps; ++k) {
                 // matrix-core step: forward with the current BF16 weights, backward, reduce, Adam, repack

@@ -121,7 +121,9 @@ void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathB
 // training samples (inputs, direction, weight) from the records of finished paths;
 // appends at most cap samples, counts them in *count and sums their weights in *wsum
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
-                         uint32_t *count, const uint32_t *ring, hipStream_t st);
+                         uint32_t *count, const uint32_t *ring, uint32_t *blk, hipStream_t st);
+// blocks of the sample gather (blk of launch_guide_gather holds this many entries)
+constexpr uint32_t GUIDE_GATHER_BLOCKS = uint32_t(GUIDE_REC_PATHS) * GUIDE_MAX_REC / 256 + 1;
 void launch_reset_counters(uint32_t *c, hipStream_t st);
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st);
 // rays of the closest-hit (which = 0) or any-hit (1) queue appended to o/d (at most cap)

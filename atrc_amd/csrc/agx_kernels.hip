@@ -18,6 +18,8 @@
 // one written here, which is Atrc's.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstdint>
 
 #include "../../include/atrc_gpu.h"
@@ -1644,8 +1646,6 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     uint32_t flags = keep ? PF_KEEP : 0u;
     float4 coef = make_float4(1, 1, 1, 0);
     pb.radiance[p] = make_float4(0, 0, 0, 0);
-    pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
-    pb.gb_normal[p] = make_float4(0, 0, 0, 0);
     pb.depth[p] = make_int4(1, 1, 0, 0);
     pb.rng_tr[p] = rng_seed(rc.seed ^ TR_STREAM_KEY, path_seq((unsigned long long)pixel, (unsigned long long)sample)).state;
     bool alive = true;
@@ -1654,6 +1654,10 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
         else coef = make_float4(coef.x / rc.cont_prob, coef.y / rc.cont_prob, coef.z / rc.cont_prob, 0);
     }
     const bool launch = alive && 1 <= rc.max_depth;
+    if (!launch) {  // a launched path gets its G-buffer from k_shade at depth 1 (hit or escape)
+        pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
+        pb.gb_normal[p] = make_float4(0, 0, 0, 0);
+    }
     if (launch) {
         flags |= PF_CONT;
         pb.ray_o[2 * p] = f4(o, 0.0f);
@@ -2070,33 +2074,6 @@ __global__ void __launch_bounds__(256) k_guide_features(DevScene s, RenderConst 
     dst[1] = make_float4(x[4], x[5], x[6], x[7]);
 }
 
-// Training samples from the records of the recording paths, appended to the
-// sample ring from slot ring[0] on: weight = luminance of the incident radiance
-// estimate x luminance of f |cos| / density the direction was drawn with (the
-// product the sampling density should follow).  Records without light are
-// skipped: their weight is zero.
-__global__ void __launch_bounds__(256) k_guide_gather(RenderConst rc, PathBuffers pb, float *samples, uint32_t cap,
-                                                      uint32_t *count, const uint32_t *ring) {
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t r = gid / GUIDE_MAX_REC, j = gid % GUIDE_MAX_REC;
-    const uint32_t last_slot = uint32_t(rc.n_paths - 1) >> rc.guide_rec_shift;
-    bool ok = false;
-    float w = 0;
-    const float *rec = nullptr;
-    if (r <= last_slot && int32_t(j) < pb.g_rec_state[r].x) {
-        rec = pb.g_rec + (size_t(r) * GUIDE_MAX_REC + j) * GUIDE_REC_FLOATS;
-        const float pdf = rec[11];
-        w = lum3(mk(rec[15], rec[16], rec[17])) * rec[18] / pdf;
-        ok = pdf > 0 && w > 0 && isfinite(w);
-    }
-    const uint32_t idx = wave_append(count, ok);
-    if (ok && idx < cap) {
-        float *dst = samples + size_t((ring[0] + idx) % cap) * 12;
-        for (int k = 0; k < 11; ++k) dst[k] = rec[k];
-        dst[11] = w;
-    }
-}
-
 template <uint32_t F>
 __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathBuffers pb) {
     const uint32_t n = pb.cnt_in[2];
@@ -2106,8 +2083,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const float eps = rc.eps;
     uint32_t flags = pb.flags[p];
     V L = mk(pb.radiance[p]);
+    // the transmittance stream is drawn only in media: vacuum variants neither
+    // load nor store it
     Rng rtr;
-    rtr.state = pb.rng_tr[p];
+    rtr.state = (F & SF_MEDIA) ? pb.rng_tr[p] : 0ull;
     Rng rng;
     rng.state = pb.rng[p];
     {
@@ -2146,7 +2125,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     if (!(flags & PF_CONT)) {
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
-        pb.rng_tr[p] = rtr.state;
+        if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
         return;
     }
     flags &= ~PF_CONT;
@@ -2159,9 +2138,13 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
     if (ent < 0) {  // path_tracing.cpp:41-49
         if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance<F>(s, rd);
+        if (dep.x == 1) {  // the camera ray escaped: empty G-buffer (k_raygen leaves it to this branch)
+            pb.gb_normal[p] = make_float4(0, 0, 0, 0);
+            pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
+        }
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
-        pb.rng_tr[p] = rtr.state;
+        if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
         return;
     }
     HitRec h;
@@ -2415,7 +2398,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     pb.depth[p] = dep;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
-    pb.rng_tr[p] = rtr.state;
+    if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
 }
 
 // ---------------------------------------------------------------- queue compaction
@@ -2550,6 +2533,85 @@ __global__ void __launch_bounds__(QC_THREADS) k_queue_scatter(PathBuffers pb, ui
         for (uint32_t l = 0; l < L; ++l)
             if ((q.mask >> l) & 1u) pb.q_shadow[si++] = p * L + l;
         if (q.again) pb.q_out[b2 + e2] = p;
+    }
+}
+
+// ---------------------------------------------------------------- training samples
+// Training samples from the records of the recording paths, written to the
+// sample ring from slot ring[0] on, in record order (count, scan, scatter: the
+// same samples reach the same slots in every run): weight = luminance of the
+// incident radiance estimate x luminance of f |cos| / density the direction was
+// drawn with (the product the sampling density should follow).  Records without
+// light are skipped: their weight is zero.
+__device__ __forceinline__ bool gather_ok(const RenderConst &rc, const PathBuffers &pb, uint32_t gid,
+                                          const float **rec_out, float *w_out) {
+    const uint32_t r = gid / GUIDE_MAX_REC, j = gid % GUIDE_MAX_REC;
+    const uint32_t last_slot = uint32_t(rc.n_paths - 1) >> rc.guide_rec_shift;
+    if (r > last_slot || int32_t(j) >= pb.g_rec_state[r].x) return false;
+    const float *rec = pb.g_rec + (size_t(r) * GUIDE_MAX_REC + j) * GUIDE_REC_FLOATS;
+    const float pdf = rec[11];
+    const float w = lum3(mk(rec[15], rec[16], rec[17])) * rec[18] / pdf;
+    *rec_out = rec;
+    *w_out = w;
+    return pdf > 0 && w > 0 && isfinite(w);
+}
+
+__global__ void __launch_bounds__(256) k_guide_gather_count(RenderConst rc, PathBuffers pb, uint32_t n, uint32_t *blk) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const float *rec = nullptr;
+    float w = 0;
+    const bool ok = gid < n && gather_ok(rc, pb, gid, &rec, &w);
+    __shared__ uint32_t ws[4];
+    const uint32_t c = uint32_t(__popcll(__ballot(ok)));
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// one block: blk[0..nb) <- its exclusive prefix sum, *total <- the sum
+__global__ void __launch_bounds__(1024) k_scan_counts(uint32_t *blk, uint32_t nb, uint32_t *total) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t w = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < nb ? blk[i] : 0u;
+        uint32_t sw;
+        const uint32_t e = wave_excl_scan(v, &sw);
+        if ((threadIdx.x & 63) == 0) wsum[w] = sw;
+        __syncthreads();
+        uint32_t o = carry;
+        for (uint32_t j = 0; j < w; ++j) o += wsum[j];
+        if (i < nb) blk[i] = o + e;
+        __syncthreads();  // every thread has read carry and wsum
+        if (threadIdx.x == 0)
+            for (uint32_t j = 0; j < 16; ++j) carry += wsum[j];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) k_guide_gather_scatter(RenderConst rc, PathBuffers pb, float *samples,
+                                                              uint32_t cap, uint32_t n, const uint32_t *blk,
+                                                              const uint32_t *ring) {
+    const uint32_t gid = blockIdx.x * 256u + threadIdx.x;
+    const float *rec = nullptr;
+    float w = 0;
+    const bool ok = gid < n && gather_ok(rc, pb, gid, &rec, &w);
+    const unsigned long long m = __ballot(ok);
+    const uint32_t lane = __lane_id();
+    __shared__ uint32_t ws[4];
+    if (lane == 0) ws[threadIdx.x >> 6] = uint32_t(__popcll(m));
+    __syncthreads();
+    uint32_t idx = blk[blockIdx.x];
+    for (uint32_t j = 0; j < (threadIdx.x >> 6); ++j) idx += ws[j];
+    idx += uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+    if (ok && idx < cap) {
+        float *dst = samples + size_t((ring[0] + idx) % cap) * 12;
+        for (int k = 0; k < 11; ++k) dst[k] = rec[k];
+        dst[11] = w;
     }
 }
 
@@ -2802,10 +2864,16 @@ void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathB
 }
 
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,
-                         uint32_t *count, const uint32_t *ring, hipStream_t st) {
-    const uint64_t n = rc.n_paths > 0 ? ((uint64_t(rc.n_paths - 1) >> rc.guide_rec_shift) + 1) * GUIDE_MAX_REC : 0;
-    if (!n) return;
-    hipLaunchKernelGGL(k_guide_gather, dim3(grid_of(n, 256)), dim3(256), 0, st, rc, pb, samples, cap, count, ring);
+                         uint32_t *count, const uint32_t *ring, uint32_t *blk, hipStream_t st) {
+    uint64_t n = rc.n_paths > 0 ? ((uint64_t(rc.n_paths - 1) >> rc.guide_rec_shift) + 1) * GUIDE_MAX_REC : 0;
+    // the recording stride keeps at most GUIDE_REC_PATHS record slots (blk holds their blocks)
+    n = std::min<uint64_t>(n, uint64_t(GUIDE_REC_PATHS) * GUIDE_MAX_REC);
+    const uint32_t nb = grid_of(n, 256);
+    if (nb) hipLaunchKernelGGL(k_guide_gather_count, dim3(nb), dim3(256), 0, st, rc, pb, uint32_t(n), blk);
+    hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, blk, nb, count);
+    if (nb)
+        hipLaunchKernelGGL(k_guide_gather_scatter, dim3(nb), dim3(256), 0, st, rc, pb, samples, cap, uint32_t(n), blk,
+                           ring);
 }
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, st, src, dst, n);
