@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 8
+AGX_ABI_VERSION = 9
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -160,7 +160,7 @@ class agx_batch_report(C.Structure):
 EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
-    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays", "agx_render_batch",
+    "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays", "agx_render_batch", "agx_render_batch_graph",
     "agx_debug_guide_grad_mfma",
 ]
 GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
@@ -201,6 +201,8 @@ def load_library() -> C.CDLL:
     lib.agx_debug_guide_grad_mfma.argtypes = [C.c_void_p, P(C.c_float), P(C.c_float), C.c_uint32, C.c_float,
                                               P(C.c_float)]
     lib.agx_render_batch.argtypes = [C.c_void_p, P(agx_batch_item), C.c_uint32, C.c_int32, P(agx_batch_report)]
+    lib.agx_render_batch_graph.argtypes = [C.c_void_p, P(agx_batch_item), C.c_uint32, C.c_int32,
+                                           P(agx_batch_report)]
     lib.agx_debug_replay_rays.argtypes = [C.c_void_p, P(agx_render_params), C.c_uint32, C.c_int, P(C.c_uint64),
                                           P(C.c_double)]
     lib.agx_debug_bvh4.argtypes = [P(agx_scene_desc), P(C.c_uint64), P(C.c_uint64), P(C.c_float), P(C.c_float)]
@@ -285,7 +287,7 @@ class Context:
         assert rgb.size == 3 * wh
         return {"image": rgb, "albedo": alb, "normal": nor, "denoise": den, "cancelled": code == AGX_E_CANCELLED}
 
-    def render_batch(self, descs, params, workers: int = 0):
+    def render_batch(self, descs, params, workers: int = 0, graph: bool = False):
         """agx_render_batch over (scene descriptor, render params) pairs: returns
         (images, per-item records, report)."""
         n = len(descs)
@@ -298,7 +300,8 @@ class Context:
             items[i].params = C.pointer(p)
             items[i].rgb = fptr(img)
         rep = agx_batch_report()
-        _check(self.lib.agx_render_batch(self.h, items, n, workers, C.byref(rep)))
+        fn = self.lib.agx_render_batch_graph if graph else self.lib.agx_render_batch
+        _check(fn(self.h, items, n, workers, C.byref(rep)))
         recs = [{"status": it.status, "worker": it.worker, "rays": it.rays, "build_ms": it.build_ms,
                  "render_ms": it.render_ms} for it in items]
         report = {"workers": rep.workers, "tuned": bool(rep.tuned), "wall_ms": rep.wall_ms, "rays": rep.rays,

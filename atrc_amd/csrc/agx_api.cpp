@@ -325,6 +325,50 @@ size_t max_paths_in_flight(int lights) {
     return cap;
 }
 
+// Validation of a render's parameters and its RenderConst (image, depths,
+// filter borders, camera); shared by agx_render and the graph-captured batch.
+int fill_render_const(agx_ctx *c, const agx_render_params *p, RenderConst *out) {
+    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
+    if (p->width <= 0 || p->height <= 0 || p->spp <= 0) return fail(AGX_E_INVALID_ARG, "width, height and spp must be positive");
+    if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
+    if (p->max_depth < 1 || p->min_depth < 0 || !(p->cont_prob > 0) || !(p->eps > 0))
+        return fail(AGX_E_INVALID_ARG, "invalid depth / cont_prob / eps");
+    // BoxFilter radius_ = radius - EPS (box_filter.cpp:15-21); GaussianFilter keeps
+    // its radius and needs alpha > 0 (gaussian_filter.cpp:45-56)
+    if (p->filter != AGX_FILTER_BOX && p->filter != AGX_FILTER_GAUSSIAN)
+        return fail(AGX_E_INVALID_ARG, "unknown film filter");
+    const bool gaussian = p->filter == AGX_FILTER_GAUSSIAN;
+    const float fr = gaussian ? p->filter_radius : p->filter_radius - p->eps;
+    if (!(fr > 0)) return fail(AGX_E_INVALID_ARG, "invalid film filter radius");
+    if (gaussian && !(p->filter_alpha > 0)) return fail(AGX_E_INVALID_ARG, "invalid Gaussian filter alpha");
+    if (fr > 64.0f) return fail(AGX_E_LIMIT, "film filter radius above 64 pixels");
+    HIP_TRY(hipSetDevice(c->device));
+    RenderConst &rc = *out;
+    rc.W = p->width;
+    rc.H = p->height;
+    rc.min_depth = p->min_depth;
+    rc.max_depth = p->max_depth;
+    rc.specular_depth = p->specular_depth;
+    rc.cont_prob = p->cont_prob;
+    rc.eps = p->eps;
+    rc.filter_r = fr;
+    // a filter wider than half a pixel splats; sample_bound_of (framebuffer.h:36-47)
+    // adds floor-based borders around the image
+    rc.splat = (gaussian || fr > 0.5f) ? 1 : 0;
+    rc.filter_kind = p->filter;
+    rc.filter_alpha = gaussian ? p->filter_alpha : 0.0f;
+    rc.filter_exp = gaussian ? std::exp(-p->filter_alpha * fr * fr) : 0.0f;
+    rc.border0 = rc.splat ? -int32_t(std::floor(0.5f - fr)) : 0;
+    rc.border1 = rc.splat ? int32_t(std::floor(fr + 0.49999f)) : 0;
+    rc.EW = p->width + rc.border0 + rc.border1;
+    rc.EH = p->height + rc.border0 + rc.border1;
+    rc.seed = p->seed;
+    rc.sample_begin = p->sample_begin;
+    rc.L = c->n_lights;
+    camera_frame(p->camera, &rc);
+    return AGX_OK;
+}
+
 // device buffer freed at scope exit (debug entry points only)
 struct DevBuf {
     void *p = nullptr;
@@ -492,45 +536,8 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
                float *albedo, float *normal, float *denoise) {
     if (!c || !p) return fail(AGX_E_INVALID_ARG, "null argument");
     const auto t_call = std::chrono::steady_clock::now();
-    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
-    if (p->width <= 0 || p->height <= 0 || p->spp <= 0) return fail(AGX_E_INVALID_ARG, "width, height and spp must be positive");
-    if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
-    if (p->max_depth < 1 || p->min_depth < 0 || !(p->cont_prob > 0) || !(p->eps > 0))
-        return fail(AGX_E_INVALID_ARG, "invalid depth / cont_prob / eps");
-    // BoxFilter radius_ = radius - EPS (box_filter.cpp:15-21); GaussianFilter keeps
-    // its radius and needs alpha > 0 (gaussian_filter.cpp:45-56)
-    if (p->filter != AGX_FILTER_BOX && p->filter != AGX_FILTER_GAUSSIAN)
-        return fail(AGX_E_INVALID_ARG, "unknown film filter");
-    const bool gaussian = p->filter == AGX_FILTER_GAUSSIAN;
-    const float fr = gaussian ? p->filter_radius : p->filter_radius - p->eps;
-    if (!(fr > 0)) return fail(AGX_E_INVALID_ARG, "invalid film filter radius");
-    if (gaussian && !(p->filter_alpha > 0)) return fail(AGX_E_INVALID_ARG, "invalid Gaussian filter alpha");
-    if (fr > 64.0f) return fail(AGX_E_LIMIT, "film filter radius above 64 pixels");
-    HIP_TRY(hipSetDevice(c->device));
-
     RenderConst rc{};
-    rc.W = p->width;
-    rc.H = p->height;
-    rc.min_depth = p->min_depth;
-    rc.max_depth = p->max_depth;
-    rc.specular_depth = p->specular_depth;
-    rc.cont_prob = p->cont_prob;
-    rc.eps = p->eps;
-    rc.filter_r = fr;
-    // a filter wider than half a pixel splats; sample_bound_of (framebuffer.h:36-47)
-    // adds floor-based borders around the image
-    rc.splat = (gaussian || fr > 0.5f) ? 1 : 0;
-    rc.filter_kind = p->filter;
-    rc.filter_alpha = gaussian ? p->filter_alpha : 0.0f;
-    rc.filter_exp = gaussian ? std::exp(-p->filter_alpha * fr * fr) : 0.0f;
-    rc.border0 = rc.splat ? -int32_t(std::floor(0.5f - fr)) : 0;
-    rc.border1 = rc.splat ? int32_t(std::floor(fr + 0.49999f)) : 0;
-    rc.EW = p->width + rc.border0 + rc.border1;
-    rc.EH = p->height + rc.border0 + rc.border1;
-    rc.seed = p->seed;
-    rc.sample_begin = p->sample_begin;
-    rc.L = c->n_lights;
-    camera_frame(p->camera, &rc);
+    if (int r0 = fill_render_const(c, p, &rc); r0 != AGX_OK) return r0;
     const bool guide = p->guide != 0;
     const bool guide_train = guide && p->guide_train != 0;
     const uint32_t guide_batch = p->guide_batch > 0 ? uint32_t(p->guide_batch) : 65536u;
@@ -909,6 +916,91 @@ int run_batch(agx_ctx *c, agx_batch_item *items, const std::vector<uint32_t> &or
     return status;
 }
 
+
+// ---------------------------------------------------------------- graph-captured batch
+// Paths per wavefront pass of a render without guiding (agx_render's sizing).
+int64_t plain_pass_size(agx_ctx *c, const RenderConst &rc, int64_t layer, int spp) {
+    if (c->pass_paths_L != rc.L) {
+        c->pass_paths = max_paths_in_flight(rc.L);
+        if (c->pass_paths_limit) c->pass_paths = std::min(c->pass_paths, c->pass_paths_limit);
+        c->pass_paths_L = rc.L;
+    }
+    const int64_t cap = int64_t(c->pass_paths);
+    if (layer <= cap) return layer * std::min<int64_t>(spp, cap / layer);
+    return rc.splat ? layer : cap;
+}
+
+// Allocations a captured render needs (path buffers, resolve planes), made
+// before the capture: nothing may allocate inside it.
+int graph_prepare(agx_ctx *c, const agx_render_params *p) {
+    RenderConst rc{};
+    if (int r = fill_render_const(c, p, &rc); r != AGX_OK) return r;
+    const int64_t wh = int64_t(p->width) * p->height;
+    const int64_t layer = rc.splat ? int64_t(rc.EW) * rc.EH : wh;
+    const int64_t per_pass = plain_pass_size(c, rc, layer, p->spp);
+    if (int r = c->ensure_wave(size_t(per_pass), rc.L, wh); r != AGX_OK) return r;
+    if (size_t(wh) > c->resolve_cap) {
+        if (c->d_resolve) (void)hipFree(c->d_resolve);
+        if (c->h_resolve) (void)hipHostFree(c->h_resolve);
+        c->d_resolve = nullptr;
+        c->h_resolve = nullptr;
+        c->resolve_cap = 0;
+        HIP_TRY(hipMalloc(reinterpret_cast<void **>(&c->d_resolve), size_t(wh) * 10 * sizeof(float)));
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&c->h_resolve), size_t(wh) * 10 * sizeof(float),
+                              hipHostMallocDefault));
+        c->resolve_cap = size_t(wh);
+    }
+    return AGX_OK;
+}
+
+// One scene's whole render enqueued on a stream with no host wait, for capture:
+// every bounce up to the depth bound is launched (a kernel whose queue is empty
+// exits at once), the rays of each bounce are summed on the device into *rays,
+// and the image is resolved into the context's device planes.  No memset, no
+// allocation, no host synchronisation.
+int enqueue_render(agx_ctx *c, const agx_render_params *p, hipStream_t st, unsigned long long *rays) {
+    RenderConst rc{};
+    if (int r = fill_render_const(c, p, &rc); r != AGX_OK) return r;
+    const int64_t wh = int64_t(p->width) * p->height;
+    const int64_t layer = rc.splat ? int64_t(rc.EW) * rc.EH : wh;
+    const int64_t total = layer * p->spp;
+    const int64_t per_pass = plain_pass_size(c, rc, layer, p->spp);
+    if (size_t(per_pass) > c->cap_paths || std::max(rc.L, 1) > c->cap_L || wh > c->cap_wh ||
+        size_t(wh) > c->resolve_cap)
+        return fail(AGX_E_INVALID_ARG, "graph batch: render buffers not prepared");
+    rc.guide_rec_shift = guide_rec_shift(size_t(per_pass));
+    const uint32_t n_closest_max = uint32_t(2 * per_pass), n_shadow_max = uint32_t(per_pass * std::max(rc.L, 1));
+    const int max_bounces = p->max_depth + std::max(p->specular_depth, 0) + 2;
+    PathBuffers pb = c->pb;
+    launch_film_clear(pb, wh, st);
+    for (int64_t base = 0; base < total; base += per_pass) {
+        rc.work_base = base;
+        rc.n_paths = int32_t(std::min<int64_t>(per_pass, total - base));
+        uint32_t *cin = c->counters, *cout = c->counters + 4;
+        launch_reset_counters(cout, st);
+        pb.cnt_out = cout;
+        launch_raygen(c->ds, rc, pb, st);
+        std::swap(cin, cout);
+        std::swap(pb.q_in, pb.q_out);
+        for (int bounce = 0; bounce < max_bounces; ++bounce) {
+            pb.cnt_in = cin;
+            pb.cnt_out = cout;
+            launch_count_rays(cin, rays, st);
+            launch_trace_closest(c->ds, pb, n_closest_max, false, c->dcnt, st);
+            launch_trace_shadow(c->ds, pb, n_shadow_max, false, c->dcnt + 1, st);
+            launch_reset_counters(cout, st);
+            launch_shade(c->ds, rc, pb, uint32_t(rc.n_paths), st);
+            launch_queue_compact(pb, uint32_t(rc.n_paths), rc.L, c->qc_blk, st);
+            std::swap(cin, cout);
+            std::swap(pb.q_in, pb.q_out);
+        }
+        launch_film(rc, pb, st);
+    }
+    float *dout = c->d_resolve;
+    launch_film_resolve(pb, wh, dout, dout + 3 * wh, dout + 6 * wh, dout + 9 * wh, st);
+    return AGX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -976,6 +1068,145 @@ int agx_render_batch(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_t work
     rep.rays = rays;
     if (report) *report = rep;
     if (r != AGX_OK) return fail(r, err);
+    return AGX_OK;
+}
+
+int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_t branches,
+                           agx_batch_report *report) {
+    if (!c || (n && !items)) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (branches < 0 || branches > 8) return fail(AGX_E_INVALID_ARG, "branches must lie in [0, 8]");
+    const int k = branches > 0 ? branches : 4;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!items[i].scene || !items[i].params) return fail(AGX_E_INVALID_ARG, "batch item without scene or params");
+        if (items[i].params->guide) return fail(AGX_E_UNSUPPORTED, "guided rendering is not available in batches");
+        items[i].status = AGX_E_CANCELLED;
+        items[i].worker = -1;
+        items[i].rays = 0;
+        items[i].build_ms = items[i].render_ms = 0;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    const auto t0 = Clock::now();
+    // everything the graph touches, released on every exit
+    struct Held {
+        std::vector<agx_ctx *> ctxs;
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> events;
+        unsigned long long *rays = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        ~Held() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            for (hipEvent_t e : events) (void)hipEventDestroy(e);
+            for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+            if (rays) (void)hipFree(rays);
+            for (agx_ctx *w : ctxs) if (w) agx_destroy(w);
+        }
+    } held;
+    // one context per scene: its scene and buffers stay resident for the graph;
+    // host BVH builds and uploads run on k threads
+    held.ctxs.assign(n, nullptr);
+    std::atomic<uint32_t> next{0};
+    std::mutex mu;
+    int status = AGX_OK;
+    std::string err;
+    auto build = [&]() {
+        for (;;) {
+            const uint32_t i = next.fetch_add(1);
+            if (i >= n) break;
+            const auto tb = Clock::now();
+            agx_ctx *w = nullptr;
+            int r = agx_create(c->device, &w);
+            if (r == AGX_OK) {
+                held.ctxs[i] = w;
+                // scenes render side by side: passes of at most 4 M paths each
+                w->pass_paths_limit = size_t(1) << 22;
+                r = agx_upload_scene(w, items[i].scene);
+            }
+            if (r == AGX_OK) r = graph_prepare(w, items[i].params);
+            items[i].build_ms = ms_since(tb);
+            items[i].status = r;
+            if (r != AGX_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (status == AGX_OK) {
+                    status = r;
+                    err = "batch item " + std::to_string(i) + ": " + agx_last_error();
+                }
+            }
+        }
+    };
+    {
+        std::vector<std::thread> th;
+        for (int t = 1; t < k; ++t) th.emplace_back(build);
+        build();
+        for (auto &t : th) t.join();
+    }
+    if (status != AGX_OK) return fail(status, err);
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&held.rays), std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+    // branches: the costliest scene first onto the least loaded branch
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    auto cost = [&](uint32_t i) {
+        const agx_render_params *p = items[i].params;
+        return double(p->width) * p->height * p->spp + 4.0 * double(items[i].scene->entity_count);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
+    std::vector<std::vector<uint32_t>> lanes(static_cast<size_t>(k));
+    std::vector<double> load(static_cast<size_t>(k), 0.0);
+    for (uint32_t i : order) {
+        const size_t b = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+        lanes[b].push_back(i);
+        load[b] += cost(i);
+        items[i].worker = int32_t(b);
+    }
+    held.streams.assign(size_t(k), nullptr);
+    for (auto &st : held.streams) HIP_TRY(hipStreamCreate(&st));
+    held.events.assign(size_t(k) + 1, nullptr);
+    for (auto &e : held.events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // capture: fork the branches off the context's stream, enqueue each branch's
+    // scenes one after another, join
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+    int cap_status = AGX_OK;
+    hipError_t he = hipEventRecord(held.events[0], c->stream);
+    for (int b = 0; b < k && he == hipSuccess; ++b) he = hipStreamWaitEvent(held.streams[size_t(b)], held.events[0], 0);
+    for (int b = 0; b < k && he == hipSuccess && cap_status == AGX_OK; ++b)
+        for (uint32_t i : lanes[size_t(b)]) {
+            cap_status = enqueue_render(held.ctxs[i], items[i].params, held.streams[size_t(b)], held.rays + i);
+            if (cap_status != AGX_OK) break;
+        }
+    for (int b = 0; b < k && he == hipSuccess; ++b) {
+        he = hipEventRecord(held.events[size_t(b) + 1], held.streams[size_t(b)]);
+        if (he == hipSuccess) he = hipStreamWaitEvent(c->stream, held.events[size_t(b) + 1], 0);
+    }
+    const hipError_t he_end = hipStreamEndCapture(c->stream, &held.graph);
+    if (cap_status != AGX_OK) return cap_status;  // the message is set
+    if (he != hipSuccess) return fail(AGX_E_HIP, std::string("graph capture: ") + hipGetErrorString(he));
+    if (he_end != hipSuccess) return fail(AGX_E_HIP, std::string("graph capture: ") + hipGetErrorString(he_end));
+    HIP_TRY(hipGraphInstantiate(&held.exec, held.graph, nullptr, nullptr, 0));
+    const auto tr = Clock::now();
+    HIP_TRY(hipGraphLaunch(held.exec, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double render_ms = ms_since(tr);
+    std::vector<unsigned long long> hr(std::max<size_t>(n, 1), 0ull);
+    HIP_TRY(hipMemcpy(hr.data(), held.rays, hr.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    uint64_t total_rays = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const agx_render_params *p = items[i].params;
+        const size_t wh = size_t(p->width) * size_t(p->height);
+        if (items[i].rgb)
+            HIP_TRY(hipMemcpy(items[i].rgb, held.ctxs[i]->d_resolve, 3 * wh * sizeof(float), hipMemcpyDeviceToHost));
+        items[i].rays = hr[i];
+        items[i].render_ms = render_ms;
+        items[i].status = AGX_OK;
+        total_rays += hr[i];
+    }
+    agx_batch_report rep{};
+    rep.workers = k;
+    rep.tuned = 0;
+    rep.wall_ms = ms_since(t0);
+    rep.rays = total_rays;
+    if (report) *report = rep;
     return AGX_OK;
 }
 

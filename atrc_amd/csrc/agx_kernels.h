@@ -114,6 +114,8 @@ void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &p
 // queues of the next bounce from the paths k_shade shaded (at most max_n of them);
 // blk holds queue_compact_blocks(max_n) entries
 uint32_t queue_compact_blocks(uint32_t max_n);
+// *acc += cin[0] + cin[1] (rays of the coming bounce), on the stream
+void launch_count_rays(const uint32_t *cin, unsigned long long *acc, hipStream_t st);
 void launch_queue_compact(const PathBuffers &pb, uint32_t max_n, int L, uint4 *blk, hipStream_t st);
 // network inputs of the vertices about to be shaded (q_in order)
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,

@@ -2875,6 +2875,14 @@ void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *sa
         hipLaunchKernelGGL(k_guide_gather_scatter, dim3(nb), dim3(256), 0, st, rc, pb, samples, cap, uint32_t(n), blk,
                            ring);
 }
+// rays traced by the bounce about to run (closest-hit + shadow queue sizes),
+// summed on the device; one thread, stream-ordered
+__global__ void k_count_rays(const uint32_t *cin, unsigned long long *acc) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *acc += (unsigned long long)cin[0] + (unsigned long long)cin[1];
+}
+void launch_count_rays(const uint32_t *cin, unsigned long long *acc, hipStream_t st) {
+    hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, st, cin, acc);
+}
 void launch_copy_calibration(const float4 *src, float4 *dst, uint64_t n, hipStream_t st) {
     hipLaunchKernelGGL(k_copy_calibration, dim3(2048), dim3(256), 0, st, src, dst, n);
 }

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 8
+#define AGX_ABI_VERSION 9
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -355,6 +355,21 @@ int  agx_get_stats(agx_ctx *ctx, agx_stats_t *out);
  * item's own status is in item.status). */
 int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t workers,
                       agx_batch_report *report);
+
+/* Batched rendering in one graph launch (no counterpart in Atrc): every
+ * item's scene is built and uploaded first (on `branches` host threads),
+ * each into a context of its own with its path buffers allocated, then the
+ * renders of all items are captured into one hipGraph and launched once.
+ * The graph has `branches` parallel branches (0 = 4); the costliest scenes
+ * are dealt first, each onto the least loaded branch, and a branch renders
+ * its scenes one after another.  A captured render launches every bounce
+ * up to the depth bound (kernels of an empty queue exit at once) and counts
+ * its rays on the device.  Images are bit-identical to agx_render's for
+ * each item alone.  item.worker = the item's branch; item.render_ms = the
+ * graph's run time.  Memory: every scene and its buffers are resident at
+ * once (passes of at most 2^22 paths per scene). */
+int  agx_render_batch_graph(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t branches,
+                            agx_batch_report *report);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char *agx_last_error(void);

@@ -60,3 +60,36 @@ def test_batch_rejects_guided_items(gpu_ctx):
     with pytest.raises(capi.AgxError) as e:
         gpu_ctx.render_batch([d], [p], workers=1)
     assert e.value.code == capi.AGX_E_UNSUPPORTED
+
+
+def test_batch_graph_matches_single_renders(gpu_ctx):
+    """agx_render_batch_graph: all items captured into one hipGraph and launched
+    once.  Launched twice, with eager renders between the launches: every item of
+    every launch is bit-identical to agx_render alone, and its ray count is the
+    one the eager batch reports."""
+    items = _items()
+    s, cam, tr = scenes.cornell_box()
+    items.append((s, scenes.render_params(40, 40, 4, cam, seed=5, filter="gaussian", filter_radius=1.5,
+                                          filter_alpha=2.0, **tr)))
+    descs, keep, params = [], [], []
+    for s, p in items:
+        d, k = s.to_desc()
+        descs.append(d)
+        keep.append(k)
+        params.append(p)
+    alone = []
+    for d, p in zip(descs, params):
+        gpu_ctx.upload_scene(d)
+        alone.append(gpu_ctx.render(p, aovs=False)["image"])
+    _, eager_recs, _ = gpu_ctx.render_batch(descs, params, workers=2)
+    for launch in range(2):
+        imgs, recs, rep = gpu_ctx.render_batch(descs, params, workers=3, graph=True)
+        log(f"[batch graph] launch {launch}: {len(items)} scenes, report {rep}")
+        for i, (a, b, r, e) in enumerate(zip(alone, imgs, recs, eager_recs)):
+            assert r["status"] == capi.AGX_OK
+            np.testing.assert_array_equal(a, b, err_msg=f"launch {launch} item {i}")
+            assert r["rays"] == e["rays"], (launch, i, r["rays"], e["rays"])
+        assert rep["workers"] == 3 and rep["rays"] == sum(r["rays"] for r in recs)
+        # eager work between the launches
+        gpu_ctx.upload_scene(descs[0])
+        np.testing.assert_array_equal(gpu_ctx.render(params[0], aovs=False)["image"], alone[0])
