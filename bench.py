@@ -202,6 +202,7 @@ def batched_main(args) -> int:
     resolution from {256, 512, 1024} squared, 32 spp each) rendered by one
     agx_render_batch call; value = all ray queries / wall time of the call
     (host BVH builds and uploads included)."""
+    import numpy as np
     from atrc_amd import capi, scenes
 
     t0 = time.time()
@@ -231,6 +232,25 @@ def batched_main(args) -> int:
         raise RuntimeError(f"batched items failed: {bad}")
     log(f"[bench] batched: {rep['rays']} rays in {wall:.2f} s with {rep['workers']} workers "
         f"(autotuner probe rays/s by workers: {rep['tune_rays_per_s']})")
+    # the same batch captured into one hipGraph (scenes built first, then one launch)
+    graph = None
+    if not args.no_batch_graph:
+        ctx = capi.Context(0)
+        t0 = time.perf_counter()
+        gimgs, grecs, grep_ = ctx.render_batch(descs, params, workers=4, graph=True)
+        gwall = time.perf_counter() - t0
+        ctx.close()
+        same = all(np.array_equal(a, b) for a, b in zip(imgs, gimgs))
+        graph = {"branches": grep_["workers"], "wall_s": round(gwall, 4), "rays": grep_["rays"],
+                 "Mrays_per_s": round(grep_["rays"] / gwall / 1e6, 3),
+                 "build_ms_max": max(r["build_ms"] for r in grecs),
+                 "graph_run_ms": grecs[0]["render_ms"] if grecs else 0.0,
+                 "Mrays_per_s_graph_run": round(grep_["rays"] / max(grecs[0]["render_ms"], 1e-9) / 1e3, 3)
+                 if grecs else 0.0,
+                 "images_identical_to_worker_batch": bool(same)}
+        log(f"[bench] batched graph: {grep_['rays']} rays in {gwall:.2f} s "
+            f"(graph run {graph['graph_run_ms']:.0f} ms = {graph['Mrays_per_s_graph_run']:.0f} Mrays/s; "
+            f"images identical to the worker batch: {same})")
     out = {
         "metric": METRIC, "value": round(rep["rays"] / wall / 1e6, 3), "unit": "Mrays/s", "n_gpus": 1,
         "steps": 1, "warmup": 0, "ms_per_step": round(wall * 1e3, 3), "higher_is_better": True,
@@ -243,6 +263,7 @@ def batched_main(args) -> int:
         "batch": {"workers": rep["workers"], "tuned": rep["tuned"], "tune_rays_per_s": rep["tune_rays_per_s"],
                   "build_ms_sum": sum(r["build_ms"] for r in recs),
                   "render_ms_sum": sum(r["render_ms"] for r in recs)},
+        "batch_graph": graph,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
@@ -285,6 +306,8 @@ def main() -> int:
     ap.add_argument("--guided-train-frac", type=float, default=0.3)
     ap.add_argument("--guided-steps", type=int, default=8)
     ap.add_argument("--guided-batch", type=int, default=65536)
+    ap.add_argument("--no-batch-graph", action="store_true",
+                    help="batched config: skip the hipGraph-captured run of the same batch")
     ap.add_argument("--copy-calibration-mb", type=int, default=256,
                     help="run one plain copy kernel of this size first (traffic-counter reference), 0 = off")
     args = ap.parse_args()
