@@ -41,6 +41,8 @@ def test_null_arguments_are_rejected():
     assert lib.agx_render(None, None, None, None, None, None, None) == capi.AGX_E_INVALID_ARG
     assert lib.agx_render_batch(None, None, 0, 0, None) == capi.AGX_E_INVALID_ARG
     assert b"null" in lib.agx_last_error()
+    assert lib.agx_render_batch_graph(None, None, 0, 0, None) == capi.AGX_E_INVALID_ARG
+    assert b"null" in lib.agx_last_error()
 
 
 def test_descriptor_validation():
