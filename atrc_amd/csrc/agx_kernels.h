@@ -69,6 +69,8 @@ enum : uint32_t {
     PF_PENDING = 4u,     // NEE / MIS contributions await resolution
     PF_KEEP = 8u,        // the camera sample lands inside the box filter
     PF_MIS_DELTA = 16u,  // the MIS-BSDF sample was a delta lobe
+    PF_FRESH = 32u,      // camera path not shaded yet: radiance 0, depth (1, 1, 0, 0) and the
+                         // depth-1 throughput are implied (k_raygen does not store them)
     PF_MIS_MED_SHIFT = 16u,  // bits 16..31: 1 + medium index the MIS-BSDF ray travels through (0 = void)
 };
 // key of the transmittance stream (initstate = splitmix(seed ^ key ^ splitmix(seq)))

@@ -1644,22 +1644,20 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     const bool keep = rc.splat || (fabsf(float(px) + 0.5f - pxf) <= rc.filter_r &&
                                    fabsf(float(py) + 0.5f - pyf) <= rc.filter_r);
     uint32_t flags = keep ? PF_KEEP : 0u;
-    float4 coef = make_float4(1, 1, 1, 0);
-    pb.radiance[p] = make_float4(0, 0, 0, 0);
-    pb.depth[p] = make_int4(1, 1, 0, 0);
-    pb.rng_tr[p] = rng_seed(rc.seed ^ TR_STREAM_KEY, path_seq((unsigned long long)pixel, (unsigned long long)sample)).state;
+    if (s.features & SF_MEDIA)  // the transmittance stream is drawn only in media
+        pb.rng_tr[p] = rng_seed(rc.seed ^ TR_STREAM_KEY, path_seq((unsigned long long)pixel, (unsigned long long)sample)).state;
     bool alive = true;
-    if (1 > rc.min_depth) {  // path_tracing.cpp:30-35 at depth 1
+    if (1 > rc.min_depth) {  // path_tracing.cpp:30-35 at depth 1 (a survivor's throughput is 1 / cont_prob)
         if (rng.uniform() > rc.cont_prob) alive = false;
-        else coef = make_float4(coef.x / rc.cont_prob, coef.y / rc.cont_prob, coef.z / rc.cont_prob, 0);
     }
     const bool launch = alive && 1 <= rc.max_depth;
-    if (!launch) {  // a launched path gets its G-buffer from k_shade at depth 1 (hit or escape)
+    if (!launch) {  // a launched path gets its G-buffer and radiance from k_shade at depth 1
         pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
         pb.gb_normal[p] = make_float4(0, 0, 0, 0);
+        pb.radiance[p] = make_float4(0, 0, 0, 0);
     }
     if (launch) {
-        flags |= PF_CONT;
+        flags |= PF_CONT | PF_FRESH;
         pb.ray_o[2 * p] = f4(o, 0.0f);
         pb.ray_d[2 * p] = f4(d, INF_F);
     }
@@ -1684,7 +1682,6 @@ __global__ void __launch_bounds__(256) k_raygen(DevScene s, RenderConst rc, Path
     // the first path of every block of 2^shift clears the block's record slot
     if (rc.guide_record && (p & ((1u << rc.guide_rec_shift) - 1u)) == 0)
         pb.g_rec_state[p >> rc.guide_rec_shift] = make_int2(0, -1);
-    pb.coef[p] = coef;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
 }
@@ -2082,11 +2079,15 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const uint32_t p = pb.q_in[tid];
     const float eps = rc.eps;
     uint32_t flags = pb.flags[p];
-    V L = mk(pb.radiance[p]);
+    // a fresh camera path: nothing stored yet but its flags, rays and stream
+    const bool fresh = (flags & PF_FRESH) != 0;
+    flags &= ~PF_FRESH;
+    V L = fresh ? mk(0, 0, 0) : mk(pb.radiance[p]);
     // the transmittance stream is drawn only in media: vacuum variants neither
     // load nor store it
     Rng rtr;
-    rtr.state = (F & SF_MEDIA) ? pb.rng_tr[p] : 0ull;
+    const bool media = (F & SF_MEDIA) && (s.features & SF_MEDIA);
+    rtr.state = media ? pb.rng_tr[p] : 0ull;
     Rng rng;
     rng.state = pb.rng[p];
     {
@@ -2125,14 +2126,22 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     if (!(flags & PF_CONT)) {
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
-        if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
+        if (media) pb.rng_tr[p] = rtr.state;
         return;
     }
     flags &= ~PF_CONT;
     // this vertex has made no training record (yet): its direct light will reach every earlier record
     if ((F & SF_GUIDE) && rc.guide_record && guide_recording(rc, p)) pb.g_rec_state[p >> rc.guide_rec_shift].y = -1;
-    V coef = mk(pb.coef[p]);
-    int4 dep = pb.depth[p];
+    V coef;
+    int4 dep;
+    if (fresh) {  // k_raygen's depth-1 state: throughput 1, or 1 / cont_prob after the depth-1 roulette
+        const float c0 = (1 > rc.min_depth) ? 1.0f / rc.cont_prob : 1.0f;
+        coef = mk(c0, c0, c0);
+        dep = make_int4(1, 1, 0, 0);
+    } else {
+        coef = mk(pb.coef[p]);
+        dep = pb.depth[p];
+    }
     const float4 h4 = pb.hit[2 * p];
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
@@ -2144,7 +2153,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         }
         pb.radiance[p] = f4(L, 0);
         pb.flags[p] = flags;
-        if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
+        if (media) pb.rng_tr[p] = rtr.state;
         return;
     }
     HitRec h;
@@ -2398,7 +2407,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     pb.depth[p] = dep;
     pb.flags[p] = flags;
     pb.rng[p] = rng.state;
-    if (F & SF_MEDIA) pb.rng_tr[p] = rtr.state;
+    if (media) pb.rng_tr[p] = rtr.state;
 }
 
 // ---------------------------------------------------------------- queue compaction
