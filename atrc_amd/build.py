@@ -4,7 +4,7 @@
   ``include/atrc_gpu.h`` (hipcc, ``--offload-arch=gfx950``).
 * ``atrc_amd/libatrc_host.so`` and ``atrc_amd/atrc_cli`` — the C++ host: Atrc
   JSON config, factory and the ``gpu_pt`` renderer (g++; loads libatrc_gpu.so).
-* ``oracle/liboracle.so`` — the CPU restatement used by tests as the checker.
+* ``oracle/liboracle.so`` — the CPU restatement the tests compare the GPU path with.
 
 Every build compiles from scratch (no incremental skipping), so ``build()``
 is also the "does it build" check.

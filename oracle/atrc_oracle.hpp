@@ -4,8 +4,8 @@
 // the scene queries beneath it, templated on the real type: float is the FP32
 // restatement that the GPU path is checked against ray for ray and path for
 // path; double is the FP64 high-spp reference.  Only tests, the smoke check and
-// the CPU-baseline leg of the benchmark use it, as the checker; the product
-// (atrc_amd/) never links it.
+// the CPU-baseline leg of the benchmark use it, as the reference results are
+// compared with; the product (atrc_amd/) never links it.
 //
 // Every function cites the Atrc file:line it follows (paths relative to
 // src/tracer/ unless they start with src/).  Where Atrc calls its agz-utils
