@@ -2691,7 +2691,7 @@ __global__ void __launch_bounds__(256) k_film_splat(RenderConst rc, PathBuffers 
     const float r = rc.filter_r;
     const int32_t reach = int32_t(floorf(r + 0.5f));
     float *sum = pb.film_sum, *cmp = pb.film_comp;
-    float wsum = 0.0f;
+    float wsum = pb.film_count[i];  // running weight sum, one addition per sample in sample order
     for (int64_t j = 0; j < layers; ++j) {
         for (int32_t qy = y - reach; qy <= y + reach; ++qy) {
             if (qy < -rc.border0 || qy >= rc.H + rc.border1) continue;
@@ -2716,7 +2716,7 @@ __global__ void __launch_bounds__(256) k_film_splat(RenderConst rc, PathBuffers 
             }
         }
     }
-    pb.film_count[i] += wsum;
+    pb.film_count[i] = wsum;
 }
 
 __global__ void k_film_clear(PathBuffers pb, int64_t wh) {
