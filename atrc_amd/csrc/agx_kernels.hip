@@ -2851,8 +2851,13 @@ void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &p
     // lean variants serve scenes of Lambert / GGX / black surfaces in vacuum,
     // unguided or guided; every other scene takes the full kernel
     const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA | SF_IBL)) == 0;
+    // media and delta surfaces without Disney, IBL or guiding (the volumetric
+    // scenes): 199 VGPRs and no scratch against the full variant's spills
+    const bool media_delta = (s.features & (SF_DISNEY | SF_IBL)) == 0 && !rc.guide;
     if (lean && !rc.guide) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else if (lean) hipLaunchKernelGGL(k_shade<SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    else if (media_delta)
+        hipLaunchKernelGGL(k_shade<SF_MEDIA | SF_DELTA>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
 }
 
