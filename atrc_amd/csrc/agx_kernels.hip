@@ -1554,6 +1554,9 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
+// threads per block of the traversal kernels (the LDS stack columns hold 256)
+constexpr unsigned TRACE_BLOCK = 256;
+static_assert(TRACE_BLOCK <= 256 && TRACE_BLOCK % 64 == 0, "traversal block: whole waves, at most the LDS columns");
 // The dispatcher deals blocks to the eight XCDs round-robin by block index.
 // The traversal kernels map them so that XCD x takes the x-th contiguous
 // eighth of the queue: queues are in path (pixel) order, so an XCD's rays come
@@ -1561,7 +1564,7 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 // logical block of this physical block for a queue of n rays, or -1; the grid
 // holds at least ceil(n / 256) + 7 blocks, so every logical block is covered once.
 __device__ __forceinline__ int32_t xcd_block(uint32_t n) {
-    const uint32_t nb = (n + 255u) / 256u;
+    const uint32_t nb = (n + TRACE_BLOCK - 1u) / TRACE_BLOCK;
     const uint32_t per = (nb + 7u) / 8u;
     const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3;
     if (j >= per) return -1;
@@ -1573,7 +1576,7 @@ __device__ __forceinline__ int32_t xcd_block(uint32_t n) {
 // the latency of its node fetches, and the default allocation (66 VGPRs) left
 // room for seven; eight fit in 64 without spilling.
 template <bool STATS, int TM>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
+__global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[0];
     const int32_t lb = xcd_block(n);
     if (lb < 0) return;  // whole block past the queue
@@ -1595,7 +1598,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))
 }
 
 template <bool STATS, int TM>
-__global__ void __launch_bounds__(256) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[1];
     const int32_t lb = xcd_block(n);
     if (lb < 0) return;
@@ -2746,7 +2749,7 @@ __global__ void k_reset_counters(uint32_t *c) {
 }
 
 template <bool STATS, int TM>
-__global__ void __launch_bounds__(256) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
+__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
                                                     int any_hit, int32_t *ent, int32_t *prim, float *t, float *u,
                                                     float *v, DevCounters *cnt) {
     AGX_LDS_COLUMN(TM);
@@ -2818,8 +2821,8 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
 }
 #define AGX_LAUNCH_TM(K, TM, grid, st, ...)                                                        \
     do {                                                                                          \
-        if (stats) hipLaunchKernelGGL((K<true, TM>), grid, dim3(256), 0, st, __VA_ARGS__);         \
-        else hipLaunchKernelGGL((K<false, TM>), grid, dim3(256), 0, st, __VA_ARGS__);              \
+        if (stats) hipLaunchKernelGGL((K<true, TM>), grid, dim3(TRACE_BLOCK), 0, st, __VA_ARGS__); \
+        else hipLaunchKernelGGL((K<false, TM>), grid, dim3(TRACE_BLOCK), 0, st, __VA_ARGS__);      \
     } while (0)
 #define AGX_LAUNCH_STACKED(K, grid, st, ...)                                                        \
     do {                                                                                          \
@@ -2839,12 +2842,12 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
 void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                           hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, 256) + 7), st, s, pb, cnt);
+    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, TRACE_BLOCK) + 7), st, s, pb, cnt);
 }
 void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, 256) + 7), st, s, pb, cnt);
+    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, TRACE_BLOCK) + 7), st, s, pb, cnt);
 }
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
@@ -2930,7 +2933,7 @@ void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {
     if (!n) return;
     const bool stats = cnt != nullptr;
-    AGX_LAUNCH_STACKED(k_trace_rays, dim3(grid_of(n, 256)), st, s, o, d, n, any_hit, ent, prim, t, u, v, cnt);
+    AGX_LAUNCH_STACKED(k_trace_rays, dim3(grid_of(n, TRACE_BLOCK)), st, s, o, d, n, any_hit, ent, prim, t, u, v, cnt);
 }
 
 }  // namespace agx
