@@ -45,6 +45,12 @@ constexpr int BVH4_MAX_LEAF = 8;
 constexpr uint32_t BVH4_ANALYTIC = 0xffffffffu;
 constexpr int BVH4_STACK_SMALL = 64;   // traversal stack entries; the large variant
 constexpr int BVH4_STACK_LARGE = 192;  // serves trees deeper than (64 - 1) / 3 levels
+// The newest entries of a lane's BVH4 stack live in an LDS ring, older ones in
+// private memory: 8 (code, entry distance) pairs of the closest-hit traversal
+// (16 KiB per 256-thread block, so eight waves per SIMD still fit the CU's
+// 160 KiB) and 16 codes of the any-hit traversal.
+constexpr int BVH4_RING_CLOSEST = 8;
+constexpr int BVH4_RING_ANY = 16;
 constexpr int BLAS_STACK = 96;               // >= max BLAS depth + 2 (checked at upload)
 constexpr int TLAS_STACK = 64;
 constexpr int LDS_STACK = 32;                // LDS stack column per thread (TLAS + BLAS entries)

@@ -248,6 +248,10 @@ struct HitRec {
     int32_t ent, prim;
 };
 
+// threads per block of the traversal kernels (the Atrc-mode LDS stack columns hold 256)
+constexpr unsigned TRACE_BLOCK = 256;
+static_assert(TRACE_BLOCK <= 256 && TRACE_BLOCK % 64 == 0, "traversal block: whole waves, at most the LDS columns");
+
 // Traversal stacks.  TLAS and BLAS share one stack: a BLAS traversal uses the
 // entries above the TLAS entries pending at its entity.  LdsStack keeps one
 // column per thread in LDS ([depth][thread], conflict-free at equal depth);
@@ -629,15 +633,45 @@ __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
     }
 }
 
+// Per-lane traversal stack whose newest S entries live in an LDS ring and
+// whose older entries spill to a private array.  The ring is a
+// [slot][TRACE_BLOCK] array, so the lanes of a wave touch consecutive words at
+// any mix of depths and never conflict on a bank; an entry reaches private
+// memory only while the stack is deeper than S.  A push onto a full ring moves
+// the oldest ring entry (the same slot) out to the spill array, and a pop that
+// leaves the stack at depth >= S brings it back, so the ring always holds the
+// top min(depth, S) entries.  Each lane touches only its own column: no
+// barrier is needed.  (Free functions over separate locals, so that the depth
+// and the column address stay in registers.)
+template <int S, class E>
+__device__ __forceinline__ void ring_push(E *col, E *spill, int &sp, E e) {
+    const int slot = sp & (S - 1);
+    if (sp >= S) spill[sp - S] = col[slot * int(TRACE_BLOCK)];
+    col[slot * int(TRACE_BLOCK)] = e;
+    ++sp;
+}
+template <int S, class E>
+__device__ __forceinline__ E ring_pop(E *col, const E *spill, int &sp) {
+    --sp;
+    const int slot = sp & (S - 1);
+    const E e = col[slot * int(TRACE_BLOCK)];
+    if (sp >= S) col[slot * int(TRACE_BLOCK)] = spill[sp - S];
+    return e;
+}
+static_assert((BVH4_RING_CLOSEST & (BVH4_RING_CLOSEST - 1)) == 0 && (BVH4_RING_ANY & (BVH4_RING_ANY - 1)) == 0,
+              "ring sizes are powers of two");
+
 // Closest hit over the BVH4: children visited near-first, pending children
 // kept with their entry distance and skipped once a closer hit is known.
+// ring = this lane's column of a [BVH4_RING_CLOSEST][TRACE_BLOCK] LDS array.
 template <bool STATS, int CAP, bool WT>
-__device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c) {
+__device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c,
+                           uint2 *ring) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
     const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
     const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
     const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
-    uint2 stk[CAP];  // (node code, entry distance bits): one 8-byte scratch access per push / pop
+    uint2 spill[CAP - BVH4_RING_CLOSEST];  // (node code, entry distance bits)
     int sp = 0;
     uint32_t code = 0;
     bool hit = false;
@@ -649,9 +683,12 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
             b4_children(N + size_t(code) * BVH4_NODE_VEC4, o, inv, tmin, tmax, t, k);
             cswap(t, k, 0, 1); cswap(t, k, 2, 3); cswap(t, k, 0, 2); cswap(t, k, 1, 3); cswap(t, k, 1, 2);
             if (t[0] != INF_F) {
-                if (t[3] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[3], __float_as_uint(t[3]));
-                if (t[2] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[2], __float_as_uint(t[2]));
-                if (t[1] != INF_F && sp < CAP) stk[sp++] = make_uint2(k[1], __float_as_uint(t[1]));
+                if (t[3] != INF_F && sp < CAP)
+                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[3], __float_as_uint(t[3])));
+                if (t[2] != INF_F && sp < CAP)
+                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[2], __float_as_uint(t[2])));
+                if (t[1] != INF_F && sp < CAP)
+                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[1], __float_as_uint(t[1])));
                 code = k[0];
                 continue;
             }
@@ -680,19 +717,21 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
         // next pending child that may still hold a closer hit
         for (;;) {
             if (sp == 0) return hit;
-            const uint2 e = stk[--sp];
+            const uint2 e = ring_pop<BVH4_RING_CLOSEST>(ring, spill, sp);
             if (__uint_as_float(e.y) <= tmax) { code = e.x; break; }
         }
     }
 }
 
+// Any hit over the BVH4; ring = this lane's column of a
+// [BVH4_RING_ANY][TRACE_BLOCK] LDS array.
 template <bool STATS, int CAP, bool WT>
-__device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c) {
+__device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c, uint32_t *ring) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
     const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
     const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
     const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
-    uint32_t stk_k[CAP];
+    uint32_t spill[CAP - BVH4_RING_ANY];
     int sp = 0;
     uint32_t code = 0;
     for (;;) {
@@ -705,7 +744,7 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
             for (int j = 0; j < 4; ++j) {
                 if (t[j] == INF_F) continue;
                 if (next == BVH4_EMPTY) next = k[j];
-                else if (sp < CAP) stk_k[sp++] = k[j];
+                else if (sp < CAP) ring_push<BVH4_RING_ANY>(ring, spill, sp, k[j]);
             }
             if (next != BVH4_EMPTY) { code = next; continue; }
         } else {
@@ -728,51 +767,64 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
             }
         }
         if (sp == 0) return false;
-        code = stk_k[--sp];
+        code = ring_pop<BVH4_RING_ANY>(ring, spill, sp);
     }
 }
 
 // Traversal variants: Atrc's structures with the private or the LDS stack, or
-// the BVH4 with the small or the large private stack, with Atrc's triangle
-// test or the watertight one.
+// the BVH4 with the small or the large stack (LDS ring + private spill), with
+// Atrc's triangle test or the watertight one.
 enum { TM_ATRC_PRIV = 0, TM_ATRC_LDS = 1, TM_B4_SMALL = 2, TM_B4_LARGE = 3, TM_B4W_SMALL = 4, TM_B4W_LARGE = 5 };
 
+// LDS words per thread of a traversal kernel: the Atrc-mode LDS column, or
+// the BVH4 rings (the closest-hit ring of (code, distance) pairs and the
+// any-hit ring of codes share the space; k_trace_rays runs either).
+constexpr int lds_stack_words(int tm) {
+    return tm == TM_ATRC_LDS ? LDS_STACK
+                             : (tm >= TM_B4_SMALL ? (2 * BVH4_RING_CLOSEST > BVH4_RING_ANY ? 2 * BVH4_RING_CLOSEST
+                                                                                          : BVH4_RING_ANY)
+                                                  : 1);
+}
+
+// lds = the block's [lds_stack_words(TM)][TRACE_BLOCK] word array
 template <bool STATS, int TM>
 __device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h,
-                                              Counts &c, uint32_t *lds_col) {
+                                              Counts &c, uint32_t *lds) {
     if constexpr (TM == TM_ATRC_LDS) {
-        LdsStack stk{lds_col, 0};
+        LdsStack stk{lds + threadIdx.x, 0};
         return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
     } else if constexpr (TM == TM_ATRC_PRIV) {
         PrivStack stk;
         return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
     } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
-        return b4_closest<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, h, c);
+        return b4_closest<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, h, c,
+                                                                        reinterpret_cast<uint2 *>(lds) + threadIdx.x);
     } else {
-        return b4_closest<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, h, c);
+        return b4_closest<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, h, c,
+                                                                        reinterpret_cast<uint2 *>(lds) + threadIdx.x);
     }
 }
 
 template <bool STATS, int TM>
 __device__ __forceinline__ bool query_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c,
-                                          uint32_t *lds_col) {
+                                          uint32_t *lds) {
     if constexpr (TM == TM_ATRC_LDS) {
-        LdsStack stk{lds_col, 0};
+        LdsStack stk{lds + threadIdx.x, 0};
         return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
     } else if constexpr (TM == TM_ATRC_PRIV) {
         PrivStack stk;
         return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
     } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
-        return b4_any<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, c);
+        return b4_any<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
     } else {
-        return b4_any<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, c);
+        return b4_any<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
     }
 }
 
-// LDS stack columns exist only in the TM_ATRC_LDS instantiations
-#define AGX_LDS_COLUMN(TM)                                                  \
-    __shared__ uint32_t lds_stack_[(TM) == TM_ATRC_LDS ? LDS_STACK : 1][256]; \
-    uint32_t *lds_col = &lds_stack_[0][threadIdx.x]
+// The block's traversal stack memory in LDS (one word when the variant has none)
+#define AGX_LDS_COLUMN(TM)                                                                   \
+    __shared__ __align__(16) uint32_t lds_stack_[lds_stack_words(TM) * int(TRACE_BLOCK)]; \
+    uint32_t *lds_col = lds_stack_
 
 // ---------------------------------------------------------------- surface reconstruction
 struct Surf {
@@ -1554,9 +1606,6 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 }
 
 // ---------------------------------------------------------------- kernels
-// threads per block of the traversal kernels (the LDS stack columns hold 256)
-constexpr unsigned TRACE_BLOCK = 256;
-static_assert(TRACE_BLOCK <= 256 && TRACE_BLOCK % 64 == 0, "traversal block: whole waves, at most the LDS columns");
 // The dispatcher deals blocks to the eight XCDs round-robin by block index.
 // The traversal kernels map them so that XCD x takes the x-th contiguous
 // eighth of the queue: queues are in path (pixel) order, so an XCD's rays come
