@@ -356,6 +356,8 @@ int fill_render_const(agx_ctx *c, const agx_render_params *p, RenderConst *out) 
     // adds floor-based borders around the image
     rc.splat = (gaussian || fr > 0.5f) ? 1 : 0;
     rc.filter_kind = p->filter;
+    if (p->task_grid_size < 0) return fail(AGX_E_INVALID_ARG, "invalid task_grid_size");
+    rc.tile = p->task_grid_size > 0 ? p->task_grid_size : 32;
     rc.filter_alpha = gaussian ? p->filter_alpha : 0.0f;
     rc.filter_exp = gaussian ? std::exp(-p->filter_alpha * fr * fr) : 0.0f;
     rc.border0 = rc.splat ? -int32_t(std::floor(0.5f - fr)) : 0;

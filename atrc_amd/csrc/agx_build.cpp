@@ -782,7 +782,7 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
             }
             o[0].w = bits_f(ref_ent[r]);
             o[1].w = bits_f(slot);
-            o[2].w = 0;
+            o[2].w = 0;  // the entity's tie rank, set once the aggregate is built
         }
         S.b4_depth = b4.depth;
         S.b4_leaves = b4.leaves;
@@ -808,6 +808,16 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         *err = "unknown aggregate kind";
         return AGX_E_INVALID_ARG;
     }
+    // Atrc's aggregates test entities in a fixed order (the native aggregate in
+    // JSON order, native_aggregate.cpp:35-44; the entity BVH leaf by leaf, left
+    // subtree first, entity_bvh.cpp:150-170) and keep the last hit whose t is
+    // within the inclusive bound, so of two entities hit at exactly the same t
+    // the later one wins.  The BVH4 traversal reproduces this with the rank.
+    for (uint32_t i = 0; i < S.ents.size(); ++i) S.ents[i].tie_rank = int32_t(i);
+    if (desc.aggregate == AGX_AGGREGATE_BVH)
+        for (size_t k = 0; k < S.tlas_prims.size(); ++k) S.ents[S.tlas_prims[k]].tie_rank = int32_t(k);
+    for (size_t k = 0; k + 2 < S.b4_prims.size(); k += 3)
+        S.b4_prims[k + 2].w = bits_f(uint32_t(S.ents[f_bits(S.b4_prims[k].w)].tie_rank));
     if (S.max_blas_depth + 2 > BLAS_STACK || S.max_tlas_depth + 2 > TLAS_STACK) {
         *err = "BVH depth exceeds the traversal stack (blas " + std::to_string(S.max_blas_depth) + ", tlas " +
                std::to_string(S.max_tlas_depth) + ")";

@@ -33,10 +33,11 @@ constexpr int MAX_LIGHTS = 16;
 //       then add), always enclosing the child's exact box.
 // Child code: BVH4_EMPTY, an interior node index (bit 31 clear), or a leaf
 // BVH4_LEAF | (count - 1) << 27 | first primitive.  A primitive is 3 vec4:
-//   (A, entity), (B - A, slot), (C - A, 0); slot = index into blas_prims/blas_info
-//   of a mesh triangle, or BVH4_ANALYTIC for a whole analytic entity.  In the
-//   watertight mode a mesh triangle is (A, entity), (B, slot), (C, 0) with the
-//   mesh's own vertex coordinates.
+//   (A, entity), (B - A, slot), (C - A, tie rank); slot = index into
+//   blas_prims/blas_info of a mesh triangle, or BVH4_ANALYTIC for a whole
+//   analytic entity; tie rank = DevEntity::tie_rank of the entity.  In the
+//   watertight mode a mesh triangle is (A, entity), (B, slot), (C, tie rank)
+//   with the mesh's own vertex coordinates.
 constexpr int BVH4_NODE_VEC4 = 4;
 constexpr uint32_t BVH4_EMPTY = 0xffffffffu;
 constexpr uint32_t BVH4_LEAF = 0x80000000u;
@@ -61,7 +62,9 @@ struct DevEntity {
     int32_t light;         // -1 or index into the light list
     int32_t flags;         // bit 0: no_denoise
     int32_t med_in, med_out;  // media on the -z / +z side of the geometric normal, -1 = void
-    int32_t pad_m0, pad_m1;
+    int32_t tie_rank;      // order in which Atrc's aggregate visits the entity: on an exactly equal
+                           // hit distance the later-visited entity wins (native: index; BVH: leaf order)
+    int32_t pad_m1;
     vec4 a;                // quad/triangle: a.xyz ; sphere/disk: (radius, radius^2, world_radius, ratio)
     vec4 b_a, c_a, d_a;    // edges (quad: world space; triangle: local space)
     vec4 x_abc, x_acd, z;  // geometric frame vectors

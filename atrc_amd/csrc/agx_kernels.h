@@ -94,9 +94,12 @@ struct RenderConst {
     uint32_t guide_rec_rot;    // varies per pass so that every pixel records in turn
     int32_t pad2;
     // wide film filters (box wider than a pixel, Gaussian): every sample splats
-    // onto all pixels within filter_r; the sampled grid extends border0 pixels
-    // before and border1 pixels after the image on each axis (EW x EH pixels)
+    // onto the pixels of its render tile (tile x tile pixels, Atrc's
+    // task_grid_size) as Atrc's apply_image_filter reaches them; the sampled
+    // grid extends border0 pixels before and border1 pixels after the image on
+    // each axis (EW x EH pixels)
     int32_t splat;
+    int32_t tile;
     int32_t filter_kind;   // AGX_FILTER_*
     int32_t border0, border1;
     int32_t EW, EH;

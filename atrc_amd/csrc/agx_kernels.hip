@@ -674,7 +674,7 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
     uint2 spill[CAP - BVH4_RING_CLOSEST];  // (node code, entry distance bits)
     int sp = 0;
     uint32_t code = 0;
-    bool hit = false;
+    int32_t best_rank = -1;  // tie rank of the entity of the current hit, -1 = no hit yet
     for (;;) {
         if (!(code & BVH4_LEAF)) {
             if (STATS) c.visits++;
@@ -704,19 +704,29 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                     float tt, uu, vv;
                     const bool hit_tri = WT ? wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)
                                             : mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv);
-                    if (hit_tri) {
+                    // an exact tie with another entity's hit goes to the entity that
+                    // Atrc's aggregate tests later (its tie rank)
+                    const int32_t rank = __float_as_int(p2.w);
+                    if (hit_tri && (tt < tmax || ent == h->ent || rank > best_rank)) {
                         tmax = tt;
                         h->t = tt; h->u = uu; h->v = vv; h->ent = ent; h->prim = int32_t(slot);
-                        hit = true;
+                        best_rank = rank;
                     }
-                } else if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tmax, h, c)) {
-                    hit = true;
+                } else {
+                    // a lower-ranked entity must hit strictly closer: its test runs
+                    // against the next float below the current hit distance
+                    const int32_t rank = __float_as_int(P[3 * size_t(i) + 2].w);
+                    float tb = rank < best_rank ? __uint_as_float(__float_as_uint(tmax) - 1u) : tmax;
+                    if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tb, h, c)) {
+                        tmax = tb;
+                        best_rank = rank;
+                    }
                 }
             }
         }
         // next pending child that may still hold a closer hit
         for (;;) {
-            if (sp == 0) return hit;
+            if (sp == 0) return best_rank >= 0;
             const uint2 e = ring_pop<BVH4_RING_CLOSEST>(ring, spill, sp);
             if (__uint_as_float(e.y) <= tmax) { code = e.x; break; }
         }
@@ -2732,11 +2742,30 @@ __device__ __forceinline__ float gauss1(float d, float alpha, float expv) {
 // filter radius, weighted by the filter (FilmGrid::apply + apply_image_filter,
 // render_target.h:384-395, framebuffer.h:49-80).  A sample's film position is
 // re-derived from the first two numbers of its stream.
+// The distance Atrc's apply_image_filter (framebuffer.h:49-80) hands the film
+// filter when a sample at s (one axis) reaches pixel x of a film grid whose
+// pixel range on this axis is [lo, hi], or -1 if it does not reach it: the
+// grid's first pixel in reach is a = max(lo, ceil(s - r - 0.5)), its distance
+// |a + 0.5 - s| grows by one per pixel after it (so it is not the distance to
+// the pixel's centre), and a pixel whose running distance exceeds r is
+// skipped.  The additions run in the same order as Atrc's loop.
+__device__ __forceinline__ float atrc_filter_rel(int32_t x, int32_t lo, int32_t hi, float r, float s) {
+    const int32_t a = max(lo, int32_t(ceilf(s - r - 0.5f)));
+    const int32_t b = min(hi, int32_t(floorf(s + r - 0.5f)));
+    if (x < a || x > b) return -1.0f;
+    float d = fabsf(float(a) + 0.5f - s);
+    for (int32_t k = a; k < x; ++k) d += 1.0f;
+    return d > r ? -1.0f : d;
+}
+
 __global__ void __launch_bounds__(256) k_film_splat(RenderConst rc, PathBuffers pb) {
     const int64_t wh = int64_t(rc.W) * rc.H;
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= wh) return;
     const int32_t x = int32_t(i % rc.W), y = int32_t(i / rc.W);
+    // the pixel's render tile [tx0, tx1] x [ty0, ty1]: its film grid
+    const int32_t tx0 = x - x % rc.tile, ty0 = y - y % rc.tile;
+    const int32_t tx1 = min(tx0 + rc.tile, rc.W) - 1, ty1 = min(ty0 + rc.tile, rc.H) - 1;
     const int64_t ewh = int64_t(rc.EW) * rc.EH;
     const int64_t layers = rc.n_paths / ewh;
     const int64_t first_sample = int64_t(rc.sample_begin) + rc.work_base / ewh;
@@ -2755,8 +2784,8 @@ __global__ void __launch_bounds__(256) k_film_splat(RenderConst rc, PathBuffers 
                 if (!(isfinite(v.x) && isfinite(v.y) && isfinite(v.z))) continue;
                 Rng rs = rng_seed(rc.seed, path_seq((unsigned long long)q, (unsigned long long)(first_sample + j)));
                 const float sx = float(qx) + rs.uniform(), sy = float(qy) + rs.uniform();
-                const float xr = fabsf(float(x) + 0.5f - sx), yr = fabsf(float(y) + 0.5f - sy);
-                if (xr > r || yr > r) continue;
+                const float xr = atrc_filter_rel(x, tx0, tx1, r, sx), yr = atrc_filter_rel(y, ty0, ty1, r, sy);
+                if (xr < 0.0f || yr < 0.0f) continue;
                 const float wgt = rc.filter_kind == AGX_FILTER_GAUSSIAN
                                       ? gauss1(xr, rc.filter_alpha, rc.filter_exp) * gauss1(yr, rc.filter_alpha,
                                                                                               rc.filter_exp)

@@ -69,6 +69,7 @@ public:
         rp.filter_radius = filter.filter->radius;
         rp.filter = filter.filter->type == "gaussian" ? AGX_FILTER_GAUSSIAN : AGX_FILTER_BOX;
         rp.filter_alpha = filter.filter->alpha;
+        rp.task_grid_size = params_.task_grid_size;
         rp.seed = params_.seed;
         rp.camera = scene.camera->desc;
         if (params_.guide) {

@@ -248,7 +248,7 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
                   filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0,
                   guide: bool = False, guide_alpha: float = 0.5, guide_train: bool = True, guide_batch: int = 65536,
                   guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0,
-                  filter: str = "box", filter_alpha: float = 0.0):
+                  filter: str = "box", filter_alpha: float = 0.0, task_grid_size: int = 32):
     p = capi.agx_render_params()
     p.width, p.height, p.spp = width, height, spp
     p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
@@ -276,6 +276,7 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
     p.guide_seed = guide_seed
     p.filter = {"box": capi.FILTER_BOX, "gaussian": capi.FILTER_GAUSSIAN}[filter]
     p.filter_alpha = filter_alpha
+    p.task_grid_size = task_grid_size
     return p
 
 
@@ -396,20 +397,75 @@ def cornell_box() -> tuple[Scene, dict, dict]:
     return s, camera, trace
 
 
+def cornell_delta(disney: bool = False) -> tuple[Scene, dict, dict]:
+    """The Cornell-like scene with delta surfaces and curved emitters, seen
+    through a thin lens: a glass sphere (ior 1.5, tinted refraction), a
+    mirror quad on the back wall, a small sphere light and a disk light
+    beside the ceiling quad light; the camera has lens radius 0.08 focused
+    on the glass sphere.  disney=True adds a Disney sphere (metallic +
+    clearcoat), so the full shading kernel takes the delta branches too."""
+    s, cam, tr = cornell_box()
+    glass = s.add_material(Material(kind=capi.MAT_GLASS, color=(1.0, 1.0, 1.0), color2=(0.8, 0.9, 1.0), ior=1.5))
+    mirror = s.add_material(Material(kind=capi.MAT_MIRROR, color=(0.9, 0.9, 0.9), eta=(0.2, 0.2, 0.2),
+                                     k=(3.0, 3.0, 3.0)))
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    s.add(Entity(capi.GEOM_SPHERE, glass, radius=0.7, local_to_world=(1, 0, 0, 1.6, 0, 1, 0, 1.4, 0, 0, 1, 2.4)))
+    y = 5.55 - 0.02
+    s.add(Entity(capi.GEOM_QUAD, mirror, points=((3.2, y, 0.6), (4.8, y, 0.6), (4.8, y, 2.6), (3.2, y, 2.6))))
+    s.add(Entity(capi.GEOM_SPHERE, black, emit=(6.0, 8.0, 10.0), radius=0.18,
+                 local_to_world=(1, 0, 0, 0.9, 0, 1, 0, 4.2, 0, 0, 1, 4.6)))
+    # disk facing down (local +z turned to -z) below the ceiling near the right wall
+    s.add(Entity(capi.GEOM_DISK, black, emit=(3.0, 6.0, 3.0), radius=0.35,
+                 local_to_world=(1, 0, 0, 4.6, 0, -1, 0, 1.2, 0, 0, -1, 5.3)))
+    if disney:
+        d = s.add_material(Material(kind=capi.MAT_DISNEY, color=(0.9, 0.6, 0.3), metallic=0.7, roughness=0.3,
+                                    clearcoat=0.8, clearcoat_gloss=0.7))
+        s.add(Entity(capi.GEOM_SPHERE, d, radius=0.5, local_to_world=(1, 0, 0, 4.0, 0, 1, 0, 1.4, 0, 0, 1, 0.5)))
+    cam = dict(cam, lens_radius=0.08, focal_distance=9.0)
+    return s, cam, dict(tr, max_depth=6, specular_depth=8)
+
+
+def _lattice_hash(ix, iy, iz, seed: int) -> np.ndarray:
+    """Value of the noise lattice at integer points (in [-1, 1])."""
+    x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+    x = (x ^ (x >> 13)) * 1274126177
+    x = x ^ (x >> 16)
+    return (x & 0xFFFF).astype(np.float64) / 32767.5 - 1.0
+
+
+def _value_noise_grid(c: np.ndarray, seed: int) -> np.ndarray:
+    """_value_noise over the tensor grid c x c x c (indexed [z, y, x]): the same
+    trilinear lattice noise, evaluated one axis at a time."""
+    pi = np.floor(c).astype(np.int64)
+    f = c - pi
+    w = f * f * (3 - 2 * f)
+    lo = int(pi.min())
+    k = np.arange(lo, int(pi.max()) + 2)
+    L = _lattice_hash(k[:, None, None], k[None, :, None], k[None, None, :], seed)   # [ix, iy, iz]
+    idx = pi - lo
+    wts = np.stack([1 - w, w], axis=1)                                              # (n, 2)
+
+    def along(a, axis):
+        a = np.moveaxis(a, axis, -1)
+        out = a[..., idx] * wts[:, 0] + a[..., idx + 1] * wts[:, 1]
+        return np.moveaxis(out, -1, axis)
+
+    g = along(along(along(L, 0), 1), 2)      # [x, y, z]
+    return np.ascontiguousarray(g.transpose(2, 1, 0))
+
+
 def fbm_grid(n: int, seed: int, octaves: int = 4, base: float = 4.0) -> np.ndarray:
     """Seeded n^3 fBm density in [0, 1] (value noise, 'octaves' octaves, lowest frequency
     'base' cells across the cube), indexed [z, y, x]."""
     c = (np.arange(n, dtype=np.float64) + 0.5) / n
-    z, y, x = np.meshgrid(c, c, c, indexing="ij")
-    p = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
-    acc = np.zeros(len(p))
+    acc = np.zeros((n, n, n))
     amp, freq, norm = 1.0, base, 0.0
     for o in range(octaves):
-        acc += amp * _value_noise(p * freq, seed + 17 * o)
+        acc += amp * _value_noise_grid(c * freq, seed + 17 * o)
         norm += amp
         amp *= 0.5
         freq *= 2.0
-    return np.clip(0.5 + 0.5 * acc / norm * 1.6, 0.0, 1.0).reshape(n, n, n).astype(np.float32)
+    return np.clip(0.5 + 0.5 * acc / norm * 1.6, 0.0, 1.0).astype(np.float32)
 
 
 def _box_frame(lo, hi, rot_deg: float) -> tuple:
@@ -706,6 +762,81 @@ def half_plane_emitter() -> tuple[Scene, dict, dict]:
                  points=((0.0, 0.0, -100.0), (100.0, 0.0, -100.0), (100.0, 0.0, 100.0), (0.0, 0.0, 100.0))))
     cam = {"pos": (0.0, -5.0, 0.0), "dst": (0.0, 0.0, 0.0), "up": (0, 0, 1), "fov": 40.0}
     return s, cam, {"min_depth": 5, "max_depth": 1, "cont_prob": 0.9}
+
+
+def _inward_shell(radius: float, subdiv: int) -> np.ndarray:
+    """Icosphere of the given radius with inward winding and flat inward normals
+    (geometric = shading normal), as a (3n, 8) mesh."""
+    pos, faces = icosphere(subdiv)
+    pos = pos * radius
+    faces = faces[:, ::-1].copy()
+    a, b, c = pos[faces[:, 0]], pos[faces[:, 1]], pos[faces[:, 2]]
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    tris = mesh_from_indexed(pos, np.zeros_like(pos), faces)
+    tris[:, 3:6] = np.repeat(n, 3, axis=0)
+    return tris
+
+
+def emitting_enclosure(kind: str = "quads", rho: float = 0.5, emit: float = 1.0,
+                       max_depth: int = 24) -> tuple[Scene, dict, dict, float]:
+    """Closed emitting Lambert enclosure (surface furnace): every wall is a diffuse
+    area light of radiance `emit` with a Lambert albedo `rho`.  The camera inside
+    sees L = emit * (1 - rho^(D+1)) / (1 - rho) at every pixel for max depth D
+    (emission at depth 1, then rho^k from the direct light at the k-th vertex).
+    kind = "quads": six inward quads of a 4 x 3 x 2.5 box (quad light sampling);
+    "mesh": an inward icosphere of radius 2 (triangle-mesh light sampling).
+    Returns (scene, camera, trace settings, expected radiance)."""
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    m = s.add_material(Material(color=(rho, rho, rho)))
+    e = (emit, emit, emit)
+    if kind == "quads":
+        n0 = len(s.entities)
+        _room_quads(s, (0.0, 0.0, 0.0), (4.0, 3.0, 2.5), m)
+        for ent in s.entities[n0:]:
+            ent.emit = e
+        cam = {"pos": (1.0, 0.8, 1.0), "dst": (3.0, 2.2, 1.5), "up": (0, 0, 1), "fov": 70.0}
+    else:
+        s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, emit=e, tris=_inward_shell(2.0, 2)))
+        cam = {"pos": (0.2, -0.3, 0.1), "dst": (1.0, 1.0, 0.5), "up": (0, 0, 1), "fov": 70.0}
+    expected = emit * (1.0 - rho ** (max_depth + 1)) / (1.0 - rho)
+    return s, cam, {"min_depth": 3, "max_depth": max_depth, "cont_prob": 0.8}, expected
+
+
+def light_in_shell(kind: str = "sphere", rho: float = 0.6, emit: float = 3.0, shell_radius: float = 4.0,
+                   subdiv: int = 6) -> tuple[Scene, dict, dict, float]:
+    """Direct light (max depth 1) on an inward Lambert shell of radius D from an
+    emitter whose irradiance is the same at every shell point, so that every pixel
+    that sees the shell has one closed-form value:
+    * "sphere": a sphere light of radius R = 1 at the centre; E = pi L (R/D)^2,
+      radiance rho L (R/D)^2.
+    * "disk": a disk light facing -z whose rim lies on the sphere at height
+      z0 = D/2.  By the sphere's configuration-factor theorem (from a point on a
+      sphere, any area on the sphere has form factor A / (4 pi D^2)) and because
+      the disk and the spherical cap above it share their boundary, every shell
+      point below the disk sees it with form factor (D - z0) / (2 D): radiance
+      rho L (D - z0) / (2 D).
+    The shell's flat faces lie within 5e-5 D of the sphere at subdivision 6.
+    Returns (scene, camera, trace settings, expected radiance)."""
+    s = Scene(aggregate=capi.AGGREGATE_BVH)
+    m = s.add_material(Material(color=(rho, rho, rho)))
+    black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+    D = shell_radius
+    s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, tris=_inward_shell(D, subdiv)))
+    e = (emit, emit, emit)
+    if kind == "sphere":
+        R = 1.0
+        s.add(Entity(capi.GEOM_SPHERE, black, emit=e, radius=R))
+        expected = rho * emit * (R / D) ** 2
+        cam = {"pos": (0.0, -1.6, -0.4), "dst": (0.3, -4.0, -1.0), "up": (0, 0, 1), "fov": 60.0}
+    else:
+        z0 = D / 2
+        a = math.sqrt(D * D - z0 * z0)
+        # rotation by pi about x: the disk's local +z normal points down
+        s.add(Entity(capi.GEOM_DISK, black, emit=e, radius=a, local_to_world=(1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1, z0)))
+        expected = rho * emit * (D - z0) / (2 * D)
+        cam = {"pos": (0.0, 0.0, 0.5), "dst": (0.4, 0.3, -3.0), "up": (0, 1, 0), "fov": 70.0}
+    return s, cam, {"min_depth": 5, "max_depth": 1, "cont_prob": 0.9}, expected
 
 
 def random_rays(scene_bounds: tuple, n: int, seed: int, inside: bool = True):

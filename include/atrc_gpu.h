@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 9
+#define AGX_ABI_VERSION 10
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -254,6 +254,13 @@ typedef struct agx_render_params {
      * the image are sampled as well (Atrc's sample_bound_of, framebuffer.h). */
     int32_t filter;            /* AGX_FILTER_*, default box */
     float   filter_alpha;      /* Gaussian alpha */
+    /* Atrc's render tile size (PTRendererParams::task_grid_size, default 32;
+     * 0 = 32).  Atrc's wide filters splat a sample only within the film grid
+     * of the tile that drew it, starting from the tile's first pixel in reach
+     * and passing the filter a distance that grows by one per pixel
+     * (apply_image_filter, framebuffer.h:49-80); the GPU film does the same,
+     * so wide-filter images depend on the tile size as Atrc's do. */
+    int32_t task_grid_size;
 } agx_render_params;
 
 /* Counters and timers of the last agx_render / agx_trace_rays call. */

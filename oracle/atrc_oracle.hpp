@@ -2519,6 +2519,28 @@ inline float gaussian_1d(float d, float alpha, float expv) {
     return std::max(0.0f, float(std::exp(-alpha * d * d)) - expv);
 }
 
+// One axis of apply_image_filter (framebuffer.h:49-80) for a sample at s and a
+// film grid whose pixel range on this axis is [lo, hi] (a render tile, Atrc's
+// task_grid_size): the first pixel is max(lo, ceil(s - r - 0.5)), the last
+// min(hi, floor(s + r - 0.5)), and the distance handed to the filter starts at
+// |first + 0.5 - s| and grows by 1 per pixel (AGZ_SCOPE_EXIT{ rel += 1; }),
+// pixels whose running distance exceeds r being skipped.  (The running
+// distance is not the distance to the pixel's centre; Atrc's wide filters
+// weigh samples by it, and so does this restatement.)  Writes the pixels and
+// distances, returns their count.
+constexpr int AXIS_CAP = 160;  // > 2 r + 1 for the largest radius accepted (64)
+inline int atrc_filter_axis(int lo, int hi, float r, float s, int *pix, float *rel, int cap) {
+    const int a = std::max(lo, int(std::ceil(s - r - 0.5f)));
+    const int b = std::min(hi, int(std::floor(s + r - 0.5f)));
+    float d = std::abs(float(a) + 0.5f - s);
+    int n = 0;
+    for (int x = a; x <= b; ++x) {
+        if (!(d > r) && n < cap) { pix[n] = x; rel[n] = d; ++n; }
+        d += 1.0f;
+    }
+    return n;
+}
+
 template<class T>
 void render(const Scene<T> &scene, const agx_render_params &rp, int threads, const RenderOut &out,
             Counters *total, int row_begin = 0, int row_end = -1) {
@@ -2544,8 +2566,11 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
     const int b0 = splat ? -int(std::floor(0.5f - radius_f)) : 0;
     const int b1 = splat ? int(std::floor(radius_f + 0.49999f)) : 0;
     const int EW = W + b0 + b1;
-    const int tile = 32;
-    const int tiles_x = (W + tile - 1) / tile, tiles_y = (row_end - row_begin + tile - 1) / tile;
+    // Atrc's render tiles (task_grid_size), aligned to the image: a wide filter
+    // splats a sample only within the film grid of the tile that drew it
+    const int tile = rp.task_grid_size > 0 ? rp.task_grid_size : 32;
+    const int ty_first = row_begin / tile;
+    const int tiles_x = (W + tile - 1) / tile, tiles_y = (row_end - 1) / tile - ty_first + 1;
     std::atomic<int> next{0};
     std::vector<Counters> cs(size_t(std::max(1, threads)));
     auto write = [&](int px, int py, Kahan<T> *acc, T weight) {
@@ -2563,17 +2588,19 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
         for (;;) {
             const int task = next.fetch_add(1);
             if (task >= tiles_x * tiles_y) break;
-            const int tx = task % tiles_x, ty = task / tiles_x;
-            const int x0 = tx * tile, y0 = row_begin + ty * tile;
-            const int x1 = std::min(x0 + tile, W), y1 = std::min(y0 + tile, row_end);
+            const int tx = task % tiles_x, ty = ty_first + task / tiles_x;
+            // the tile's pixels [x0, x1) x [ty0, ty1); rows [y0, y1) of it are rendered
+            const int x0 = tx * tile, ty0 = ty * tile;
+            const int x1 = std::min(x0 + tile, W), ty1 = std::min(ty0 + tile, H);
+            const int y0 = std::max(ty0, row_begin), y1 = std::min(ty1, row_end);
             if (splat) {
-                const int tw = x1 - x0, th = y1 - y0;
+                const int tw = x1 - x0, th = ty1 - ty0;
                 std::vector<Kahan<T>> acc(size_t(tw) * th * 10);
                 std::vector<T> wsum(size_t(tw) * th, T(0));
                 const int lx = int(std::floor(float(x0) + 0.5f - radius_f));
-                const int ly = int(std::floor(float(y0) + 0.5f - radius_f));
+                const int ly = int(std::floor(float(ty0) + 0.5f - radius_f));
                 const int hx = int(std::floor(float(x1 - 1) + 0.49999f + radius_f));
-                const int hy = int(std::floor(float(y1 - 1) + 0.49999f + radius_f));
+                const int hy = int(std::floor(float(ty1 - 1) + 0.49999f + radius_f));
                 for (int qy = ly; qy <= hy; ++qy)
                     for (int qx = lx; qx <= hx; ++qx) {
                         const uint64_t key = uint64_t(qy + b0) * uint64_t(EW) + uint64_t(qx + b0);
@@ -2587,21 +2614,19 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                             const Ray<T> ray = cam.sample(T(sx / float(W)), T(sy / float(H)), au, av);
                             const PixelResult<T> pr = trace_std(tp, scene, ray, sam, trs, &c);
                             if (!pr.value.is_finite()) continue;
-                            // apply_image_filter over this tile's pixels
-                            const int xa = std::max(x0, int(std::ceil(sx - radius_f - 0.5f)));
-                            const int ya = std::max(y0, int(std::ceil(sy - radius_f - 0.5f)));
-                            const int xb = std::min(x1 - 1, int(std::floor(sx + radius_f - 0.5f)));
-                            const int yb = std::min(y1 - 1, int(std::floor(sy + radius_f - 0.5f)));
-                            for (int y = ya; y <= yb; ++y) {
-                                const float yr = std::abs(float(y) + 0.5f - sy);
-                                if (yr > radius_f) continue;
-                                for (int x = xa; x <= xb; ++x) {
-                                    const float xr = std::abs(float(x) + 0.5f - sx);
-                                    if (xr > radius_f) continue;
-                                    const T w = gaussian ? T(gaussian_1d(xr, rp.filter_alpha, g_exp) *
-                                                             gaussian_1d(yr, rp.filter_alpha, g_exp))
+                            // apply_image_filter over this tile's film grid
+                            int px_[AXIS_CAP], py_[AXIS_CAP];
+                            float rx_[AXIS_CAP], ry_[AXIS_CAP];
+                            const int nx = atrc_filter_axis(x0, x1 - 1, radius_f, sx, px_, rx_, AXIS_CAP);
+                            const int ny = atrc_filter_axis(ty0, ty1 - 1, radius_f, sy, py_, ry_, AXIS_CAP);
+                            for (int jy = 0; jy < ny; ++jy) {
+                                const int y = py_[jy];
+                                for (int jx = 0; jx < nx; ++jx) {
+                                    const int x = px_[jx];
+                                    const T w = gaussian ? T(gaussian_1d(rx_[jx], rp.filter_alpha, g_exp) *
+                                                             gaussian_1d(ry_[jy], rp.filter_alpha, g_exp))
                                                          : T(1);
-                                    const size_t o = size_t(y - y0) * size_t(tw) + size_t(x - x0);
+                                    const size_t o = size_t(y - ty0) * size_t(tw) + size_t(x - x0);
                                     Kahan<T> *a = &acc[10 * o];
                                     for (int k = 0; k < 3; ++k) {
                                         a[k].add(w * pr.value[k]); a[3 + k].add(w * pr.albedo[k]);
@@ -2615,7 +2640,7 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                     }
                 for (int y = y0; y < y1; ++y)
                     for (int x = x0; x < x1; ++x) {
-                        const size_t o = size_t(y - y0) * size_t(tw) + size_t(x - x0);
+                        const size_t o = size_t(y - ty0) * size_t(tw) + size_t(x - x0);
                         write(x, y, &acc[10 * o], wsum[o]);
                     }
                 continue;

@@ -318,4 +318,15 @@ int oracle_hg(double g, int mode, const double *wi, const double *wo, double u, 
     return 0;
 }
 
+// One axis of Atrc's apply_image_filter (framebuffer.h:49-80) for a sample at s
+// in a film grid with pixel range [lo, hi]: the pixels reached and the
+// distances passed to the filter.  Returns the count (at most cap).
+int oracle_filter_axis(int lo, int hi, float r, float s, int32_t *pix, float *rel, int cap) {
+    int p[oracle::AXIS_CAP];
+    float d[oracle::AXIS_CAP];
+    const int n = oracle::atrc_filter_axis(lo, hi, r, s, p, d, std::min(cap, oracle::AXIS_CAP));
+    for (int i = 0; i < n; ++i) { pix[i] = p[i]; rel[i] = d[i]; }
+    return n;
+}
+
 }  // extern "C"

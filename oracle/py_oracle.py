@@ -50,6 +50,7 @@ def load() -> C.CDLL:
                                         P(C.c_double)]
     lib.oracle_hg.argtypes = [C.c_double, C.c_int, P(C.c_double), P(C.c_double), C.c_double, C.c_double,
                               P(C.c_double)]
+    lib.oracle_filter_axis.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, P(C.c_int32), P(C.c_float), C.c_int]
     _lib = lib
     return lib
 
@@ -212,3 +213,12 @@ def hg_sample(g: float, wo, u: float, v: float):
     z = np.zeros(3, np.float64)
     load().oracle_hg(g, 1, _d(z), _d(np.asarray(wo, np.float64)), u, v, _d(out))
     return out[:3], float(out[3])
+
+
+def filter_axis(lo: int, hi: int, r: float, s: float) -> list:
+    """Pixels of one axis that Atrc's apply_image_filter reaches from a sample at s
+    in a film grid with pixel range [lo, hi], with the distances it passes."""
+    pix = np.zeros(256, np.int32)
+    rel = np.zeros(256, np.float32)
+    n = load().oracle_filter_axis(lo, hi, r, s, pix.ctypes.data_as(C.POINTER(C.c_int32)), _f(rel), 256)
+    return [(int(pix[i]), float(rel[i])) for i in range(n)]

@@ -1,6 +1,7 @@
-"""Wide film filters on the gfx950 path (k_film_splat: each pixel gathers the
-samples of its neighbourhood in a fixed order) against the oracle, which
-splats as Atrc's tiles do (tests/test_oracle_filter.py pins it):
+"""Wide film filters on the gfx950 path (k_film_splat: each pixel gathers, in a
+fixed order, the samples of its neighbourhood that Atrc's apply_image_filter
+would splat onto it from the pixel's own render tile) against the oracle,
+which splats as Atrc's tiles do (tests/test_oracle_filter.py pins it):
 
 * path for path at 1 spp on the Cornell scene, Gaussian and wide box;
 * the half-plane emitter's closed form (the filter's integral across the edge).
@@ -18,13 +19,15 @@ from test_oracle_filter import filter_profile
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kind,r,alpha", [("gaussian", 1.5, 2.0), ("box", 1.2, 0.0)])
-def test_filter_path_parity(gpu_ctx, kind, r, alpha):
+@pytest.mark.parametrize("kind,r,alpha,tile", [("gaussian", 1.5, 2.0, 32), ("box", 1.2, 0.0, 32),
+                                               ("gaussian", 2.3, 0.7, 8)])
+def test_filter_path_parity(gpu_ctx, kind, r, alpha, tile):
     s, cam, tr = scenes.cornell_box()
     desc, keep = s.to_desc()
     gpu_ctx.upload_scene(desc)
-    p = scenes.render_params(64, 48, 2, cam, seed=77, filter=kind, filter_radius=r, filter_alpha=alpha, **tr)
-    log(f"[filter] cornell 64x48 2 spp, {kind} r={r}")
+    p = scenes.render_params(64, 48, 2, cam, seed=77, filter=kind, filter_radius=r, filter_alpha=alpha,
+                             task_grid_size=tile, **tr)
+    log(f"[filter] cornell 64x48 2 spp, {kind} r={r}, tiles of {tile}")
     g = gpu_ctx.render(p)
     o = po.OracleScene(desc, 32).render(p, threads=8)
     gi, oi = g["image"].astype(np.float64), o["image"]
@@ -43,9 +46,7 @@ def test_filter_edge_closed_form(gpu_ctx, kind, r, alpha, W):
     rr = r + (3e-4 if kind == "box" else 0.0)
     p = scenes.render_params(W, 6, 2048, cam, seed=5, filter=kind, filter_radius=rr, filter_alpha=alpha, **tr)
     img = gpu_ctx.render(p, aovs=False)["image"][..., 0].astype(np.float64)
-    expect = filter_profile(kind, r, alpha, W)
-    if img[:, 0].mean() > img[:, -1].mean():
-        expect = expect[::-1]
+    expect = filter_profile(kind, r, alpha, W, lit_right=img[:, -1].mean() > img[:, 0].mean())
     log(f"[filter] half-plane {kind} r={r} W={W}: max |row - closed form| "
         f"{np.abs(img.mean(axis=0) - expect).max():.4f}")
     np.testing.assert_allclose(img.mean(axis=0), expect, atol=0.01)

@@ -40,8 +40,10 @@ def _path_parity(gpu_ctx, scene, camera, trace, W, H, label, min_frac=0.98):
     frac = float((rel <= 1e-3).mean())
     log(f"  per-path agreement {frac:.5f}; relMSE(gpu, oracle) {relmse(gi, ri):.3e}")
     assert frac >= min_frac
-    # G-buffer at depth 1 (path_tracing.cpp:54-60) agrees as well
+    # G-buffer at depth 1 (path_tracing.cpp:54-60) agrees as well: albedo,
+    # shading normal and the denoise flag
     np.testing.assert_allclose(g["albedo"], r["albedo"], atol=1e-4)
+    np.testing.assert_allclose(g["normal"], r["normal"], atol=1e-4)
     np.testing.assert_allclose(g["denoise"], r["denoise"], atol=1e-6)
     return frac
 
@@ -59,6 +61,32 @@ def test_spheres_path_parity(gpu_ctx, accel):
     s.accel = accel
     cam = {"pos": (-12.0, -16.0, 24.0), "dst": (10.0, 10.0, 8.0), "fov": 40.0}
     _path_parity(gpu_ctx, s, cam, tr, 128, 96, f"spheres24 (Lambert + GGX metal + sky, accel {accel})")
+
+
+@pytest.mark.parametrize("disney", [False, True])
+@pytest.mark.parametrize("accel", [capi.ACCEL_ATRC, capi.ACCEL_BVH4])
+def test_delta_lens_path_parity(gpu_ctx, accel, disney):
+    """Glass sphere, mirror quad, sphere and disk lights, thin lens (radius 0.08):
+    the delta branch of MIS (f / pdf without the light pdf, direct_illum.cpp:
+    130-192), the specular depth extension (path_tracing.cpp:150-169), glass
+    and mirror sampling and the lens (thin_lens.cpp:74-94).  disney=True
+    also puts a Disney sphere in, which runs the full shading kernel."""
+    s, cam, tr = scenes.cornell_delta(disney)
+    s.accel = accel
+    _path_parity(gpu_ctx, s, cam, tr, 96, 96, f"cornell delta/lens (disney {disney}, accel {accel})")
+
+
+def test_delta_lens_converged(gpu_ctx):
+    s, cam, tr = scenes.cornell_delta(True)
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    W = H = 32
+    log("[render parity] cornell delta/lens converged 32x32, GPU 2048 spp vs FP64 oracle 2048 spp")
+    g = gpu_ctx.render(scenes.render_params(W, H, 2048, cam, seed=98, **tr))["image"].astype(np.float64)
+    r = po.OracleScene(desc, 64).render(scenes.render_params(W, H, 2048, cam, seed=8, **tr), threads=8)["image"]
+    e = relmse(g, r)
+    log(f"  relMSE {e:.3e}")
+    assert e < 5e-3
 
 
 def test_cornell_converged(gpu_ctx):

@@ -1,15 +1,23 @@
 """Wide film filters in the oracle (box wider than a pixel, Gaussian;
-film_filter/*.cpp, framebuffer.h:36-80, render_target.h:384-395): every
-sample splats onto all pixels within the radius and pixels up to the radius
-outside the image are sampled too.  Pinned by closed forms:
+film_filter/*.cpp, framebuffer.h:36-80, render_target.h:384-395).
 
-* a straight emitter edge through the film centre becomes the filter's own
-  integral: pixel value = integral of the filter weight over the lit side of
-  [c - r, c + r] / integral over the whole interval (c = pixel centre);
-* a uniform field stays uniform (value / weight), at the image borders too;
-* a box filter of radius <= 0.5 + eps keeps every sample in its own pixel
-  (the unchanged default path).
+Atrc splats a sample only within the film grid of the render tile that drew
+it (task_grid_size x task_grid_size pixels).  apply_image_filter starts at the
+grid's first pixel in reach, a = max(lo, ceil(s - r - 0.5)), hands the filter
+the distance |a + 0.5 - s| and adds 1 to it for every later pixel, skipping
+pixels whose running distance exceeds r.  Away from a tile's first pixel a
+sample therefore reaches exactly one pixel, ceil(s - r - 0.5), about r - 0.5
+pixels before it.  Pinned by:
+
+* known answers of one axis, derived by hand from that loop;
+* closed forms of whole renders: a straight emitter edge becomes
+  (integral of weight x lit) / (integral of weight) over the sample positions
+  that reach each pixel under that rule; a uniform field stays uniform (value
+  / weight), at the image borders too; a box filter of radius <= 0.5 + eps
+  keeps every sample in its own pixel (the unchanged default path).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -17,18 +25,44 @@ from atrc_amd import scenes
 from oracle import py_oracle as po
 
 
-def filter_profile(kind: str, r: float, alpha: float, W: int):
-    """Expected row of the half-plane image: fraction of filter mass on x >= W/2."""
-    t = np.linspace(-r, r, 200001)
-    if kind == "box":
-        g = np.ones_like(t)
-    else:
-        g = np.maximum(0.0, np.exp(-alpha * t * t) - np.exp(-alpha * r * r))
+@pytest.mark.parametrize("lo,hi,r,s,expect", [
+    (0, 31, 1.5, 5.3, [(4, 0.8)]),              # true distances would also reach 5 (0.2) and 6 (1.2)
+    (8, 11, 1.5, 8.2, [(8, 0.3), (9, 1.3)]),    # a = the tile's first pixel
+    (4, 7, 1.5, 8.2, [(7, 0.7)]),               # the same sample in the tile before
+    (0, 31, 2.3, 10.9, [(9, 1.4)]),
+    (0, 31, 1.0, 3.25, [(2, 0.75)]),
+    (0, 3, 2.3, -1.6, [(0, 2.1)]),              # a border sample reaches the image's first pixel
+])
+def test_filter_axis_known_answers(lo, hi, r, s, expect):
+    got = po.filter_axis(lo, hi, np.float32(r), np.float32(s))
+    assert [p for p, _ in got] == [p for p, _ in expect]
+    np.testing.assert_allclose([d for _, d in got], [d for _, d in expect], atol=2e-6)
+
+
+def axis_rule(x: int, lo: int, hi: int, r: float, s: np.ndarray):
+    """(reached, distance) of pixel x for samples at s in a grid with range [lo, hi]."""
+    a = np.maximum(lo, np.ceil(s - r - 0.5))
+    b = np.minimum(hi, np.floor(s + r - 0.5))
+    d = np.abs(a + 0.5 - s) + (x - a)
+    return (a <= x) & (x <= b) & (d <= r), d
+
+
+def filter_profile(kind: str, r: float, alpha: float, W: int, tile: int = 32, lit_right: bool = True):
+    """Expected row of the half-plane image (edge at x = W / 2, lit side x >= W / 2,
+    or x < W / 2 when not lit_right):
+    for each pixel, the filter weight over the sample positions that reach it
+    (tile of the pixel, running distance) times lit, over the weight alone."""
+    b0, b1 = -math.floor(0.5 - r), math.floor(r + 0.49999)
+    s = np.linspace(-b0, W + b1, 400001)[:-1] + 0.5 * (W + b0 + b1) / 400000
     out = []
-    for i in range(W):
-        c = i + 0.5
-        lit = (c + t) >= W / 2
-        out.append(np.trapezoid(g * lit, t) / np.trapezoid(g, t))
+    for x in range(W):
+        lo = x - x % tile
+        hi = min(lo + tile, W) - 1
+        ok, d = axis_rule(x, lo, hi, r, s)
+        g = np.ones_like(s) if kind == "box" else np.maximum(0.0, np.exp(-alpha * d * d) - np.exp(-alpha * r * r))
+        g = g * ok
+        lit = (s >= W / 2) if lit_right else (s < W / 2)
+        out.append(np.sum(g * lit) / np.sum(g))
     return np.array(out)
 
 
@@ -46,9 +80,8 @@ def render_half_plane(kind, r, alpha, W=16, H=6, spp=512, precision=64):
                                             ("gaussian", 2.3, 0.7, 4)])
 def test_edge_becomes_the_filter_integral(kind, r, alpha, W):
     img = render_half_plane(kind, r + (3e-4 if kind == "box" else 0.0), alpha, W=W)
-    expect = filter_profile(kind, r, alpha, W)
-    if img[:, 0].mean() > img[:, -1].mean():   # the film x axis runs against world x
-        expect = expect[::-1]
+    # which side of the film the lit half lands on follows the camera convention
+    expect = filter_profile(kind, r, alpha, W, lit_right=img[:, -1].mean() > img[:, 0].mean())
     np.testing.assert_allclose(img.mean(axis=0), expect, atol=0.02)
     # rows agree with each other (the edge is vertical)
     assert np.abs(img - img.mean(axis=0)).max() < 0.06
@@ -77,12 +110,14 @@ def test_narrow_box_keeps_samples_in_their_pixel():
 def test_tile_splat_equals_per_pixel_gather():
     """Atrc's tile splatting (what the oracle restates) and the per-pixel gather
     (what the GPU kernel does) give the same image: every pixel sums, over the
-    samples of the sampled grid within the radius, weight x value / weight."""
+    samples of the sampled grid that reach it from its own tile (tiles of 4
+    pixels here), weight x value / weight."""
     import math
     s, cam, tr = scenes.cornell_box()
     desc, keep = s.to_desc()
-    W, H, spp, r, alpha = 10, 7, 2, 1.5, 2.0
-    p = scenes.render_params(W, H, spp, cam, seed=77, filter="gaussian", filter_radius=r, filter_alpha=alpha, **tr)
+    W, H, spp, r, alpha, tile = 10, 7, 2, 1.5, 2.0, 4
+    p = scenes.render_params(W, H, spp, cam, seed=77, filter="gaussian", filter_radius=r, filter_alpha=alpha,
+                             task_grid_size=tile, **tr)
     o = po.OracleScene(desc, 32)
     img = o.render(p, threads=8)["image"]
     b0, b1 = -math.floor(0.5 - r), math.floor(r + 0.49999)
@@ -99,12 +134,12 @@ def test_tile_splat_equals_per_pixel_gather():
             for (qx, qy, k), v in smp.items():
                 if abs(qx - x) > reach or abs(qy - y) > reach or not np.isfinite(v[:3]).all():
                     continue
-                xr = abs(np.float32(x) + np.float32(0.5) - np.float32(v[10]))
-                yr = abs(np.float32(y) + np.float32(0.5) - np.float32(v[11]))
-                if xr > r or yr > r:
+                okx, xr = axis_rule(x, x - x % tile, min(x - x % tile + tile, W) - 1, r, np.float64(v[10]))
+                oky, yr = axis_rule(y, y - y % tile, min(y - y % tile + tile, H) - 1, r, np.float64(v[11]))
+                if not (okx and oky):
                     continue
                 w = g1(xr) * g1(yr)
                 acc += w * v[:3]
                 ws += w
             emu[y, x] = acc / ws if ws > 0 else acc
-    np.testing.assert_allclose(emu, img, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(emu, img, rtol=2e-5, atol=1e-7)
