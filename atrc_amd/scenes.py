@@ -883,7 +883,7 @@ def _transform_json(l2w) -> list:
 
 def to_atrc_json(scene: Scene, camera: dict, trace: dict, width: int, height: int, spp: int, mesh_dir,
                  renderer: str = "gpu_pt", seed: int = 42, post_processors: list | None = None,
-                 eps: float | None = None) -> str:
+                 eps: float | None = None, renderer_extra: dict | None = None) -> str:
     """Atrc JSON ({"scene", "rendering"}) for a Python-built scene; meshes are
     written as .bm files into mesh_dir and referenced via ${scene-directory}."""
     import json
@@ -921,7 +921,8 @@ def to_atrc_json(scene: Scene, camera: dict, trace: dict, width: int, height: in
     cam = {"type": "thin_lens", "pos": [float(x) for x in camera["pos"]], "dst": [float(x) for x in camera["dst"]],
            "up": [float(x) for x in camera.get("up", (0, 0, 1))], "fov": float(camera.get("fov", 40.0))}
     rend = {"type": renderer, "spp": spp, "min_depth": trace.get("min_depth", 5),
-            "max_depth": trace.get("max_depth", 10), "cont_prob": trace.get("cont_prob", 0.9), "seed": seed}
+            "max_depth": trace.get("max_depth", 10), "cont_prob": trace.get("cont_prob", 0.9), "seed": seed,
+            **(renderer_extra or {})}
     rendering = {"width": width, "height": height, "camera": cam, "renderer": rend, "reporter": {"type": "noout"}}
     if post_processors:
         rendering["post_processors"] = post_processors
