@@ -162,8 +162,9 @@ EXPORTS = [
     "agx_abi_version", "agx_create", "agx_upload_scene", "agx_render", "agx_trace_rays", "agx_get_stats",
     "agx_last_error", "agx_destroy", "agx_debug_triangle_bvh", "agx_debug_scene_summary", "agx_debug_copy_kernel",
     "agx_debug_bvh4", "agx_debug_guide_infer", "agx_debug_guide_train", "agx_debug_replay_rays", "agx_render_batch", "agx_render_batch_graph",
-    "agx_debug_guide_grad_mfma",
+    "agx_debug_guide_grad_mfma", "agx_trace_rays_noalloc",
 ]
+NOALLOC_MAX_RAYS = 131072   # AGX_NOALLOC_MAX_RAYS
 GUIDE_PARAMS = 31008   # AGX_GUIDE_PARAMS
 
 
@@ -192,6 +193,7 @@ def load_library() -> C.CDLL:
     lib.agx_upload_scene.argtypes = [C.c_void_p, P(agx_scene_desc)]
     lib.agx_render.argtypes = [C.c_void_p, P(agx_render_params), P(C.c_int32), fp, fp, fp, fp]
     lib.agx_trace_rays.argtypes = [C.c_void_p, fp, fp, fp, fp, C.c_uint32, C.c_int32, ip, ip, fp, fp, fp]
+    lib.agx_trace_rays_noalloc.argtypes = [C.c_void_p, fp, fp, fp, fp, C.c_uint32, C.c_int32, ip, ip, fp, fp, fp]
     lib.agx_get_stats.argtypes = [C.c_void_p, P(agx_stats_t)]
     lib.agx_last_error.restype = C.c_char_p
     lib.agx_destroy.argtypes = [C.c_void_p]
@@ -309,7 +311,9 @@ class Context:
                   "tune_rays_per_s": list(rep.tune_rays_per_s)}
         return images, recs, report
 
-    def trace_rays(self, o: np.ndarray, d: np.ndarray, tmin: np.ndarray, tmax: np.ndarray, any_hit: bool = False):
+    def trace_rays(self, o: np.ndarray, d: np.ndarray, tmin: np.ndarray, tmax: np.ndarray, any_hit: bool = False,
+                   noalloc: bool = False):
+        """Ray queries (agx_trace_rays, or agx_trace_rays_noalloc when noalloc)."""
         n = len(o)
         o = np.ascontiguousarray(o, np.float32)
         d = np.ascontiguousarray(d, np.float32)
@@ -320,8 +324,9 @@ class Context:
         t = np.zeros(n, np.float32)
         u = np.zeros(n, np.float32)
         v = np.zeros(n, np.float32)
-        _check(self.lib.agx_trace_rays(self.h, fptr(o), fptr(d), fptr(tmin), fptr(tmax), n, 1 if any_hit else 0,
-                                       iptr(ent), iptr(prim), fptr(t), fptr(u), fptr(v)))
+        fn = self.lib.agx_trace_rays_noalloc if noalloc else self.lib.agx_trace_rays
+        _check(fn(self.h, fptr(o), fptr(d), fptr(tmin), fptr(tmax), n, 1 if any_hit else 0,
+                  iptr(ent), iptr(prim), fptr(t), fptr(u), fptr(v)))
         return {"entity": ent, "prim": prim, "t": t, "u": u, "v": v}
 
     def guide_infer(self, params: np.ndarray, inputs: np.ndarray) -> np.ndarray:

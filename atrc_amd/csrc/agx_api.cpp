@@ -411,6 +411,11 @@ int agx_create(int device_ordinal, agx_ctx **out) {
             agx_destroy(c);
             return fail(AGX_E_HIP, "hipEventCreate failed");
         }
+    // the staging array of agx_trace_rays_noalloc, resolved once per device
+    if (const hipError_t e = noalloc_scratch_init(device_ordinal); e != hipSuccess) {
+        agx_destroy(c);
+        return fail(AGX_E_HIP, std::string("staging array address: ") + hipGetErrorString(e));
+    }
     *out = c;
     return AGX_OK;
 }
@@ -839,6 +844,54 @@ int agx_trace_rays(agx_ctx *c, const float *o, const float *d, const float *tmin
         if (u) std::memcpy(u, hu.data(), n * 4);
         if (v) std::memcpy(v, hv.data(), n * 4);
     }
+    agx_stats_t st{};
+    st.node_bytes = c->accel != AGX_ACCEL_ATRC ? 16 * BVH4_NODE_VEC4 : 32;
+    st.prim_bytes = 48;
+    st.info_bytes = 96;
+    if (any_hit) st.shadow_rays = n; else st.closest_rays = n;
+    fill_traversal_stats(hc, &st);
+    st.total_ms = elapsed(c->ev[0], c->ev[1]);
+    if (any_hit) { st.trace_shadow_ms = st.total_ms; st.trace_shadow_launches = 1; }
+    else { st.trace_closest_ms = st.total_ms; st.trace_closest_launches = 1; }
+    c->stats = st;
+    return AGX_OK;
+}
+
+static_assert(AGX_NOALLOC_MAX_RAYS == NOALLOC_MAX_RAYS, "staging capacity");
+
+int agx_trace_rays_noalloc(agx_ctx *c, const float *o, const float *d, const float *tmin, const float *tmax,
+                           uint32_t n, int32_t any_hit, int32_t *entity_id, int32_t *prim_id, float *t, float *u,
+                           float *v) {
+    if (!c || (n && (!o || !d || !tmin || !tmax || !entity_id))) return fail(AGX_E_INVALID_ARG, "null argument");
+    if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
+    if (n > NOALLOC_MAX_RAYS)
+        return fail(AGX_E_LIMIT, "agx_trace_rays_noalloc takes at most " + std::to_string(NOALLOC_MAX_RAYS) +
+                                     " rays per call (" + std::to_string(n) + " given)");
+    if (!n) return AGX_OK;
+    HIP_TRY(hipSetDevice(c->device));
+    char *base = static_cast<char *>(noalloc_scratch(c->device));
+    if (!base) return fail(AGX_E_HIP, "staging array not resolved on this device");
+    const size_t M = NOALLOC_MAX_RAYS;
+    auto word = [&](size_t k) { return base + 4 * M * k; };  // layout: launch_trace_rays_staged
+    hipError_t e = hipMemcpyAsync(word(8), o, size_t(n) * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(word(11), d, size_t(n) * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(word(14), tmin, size_t(n) * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(word(15), tmax, size_t(n) * 4, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[0], c->stream);
+    if (e == hipSuccess) launch_trace_rays_staged(c->ds, base, n, any_hit, c->dcnt, c->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(entity_id, word(16), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && !any_hit) {
+        if (prim_id) e = hipMemcpyAsync(prim_id, word(17), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && t) e = hipMemcpyAsync(t, word(18), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && u) e = hipMemcpyAsync(u, word(19), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && v) e = hipMemcpyAsync(v, word(20), size_t(n) * 4, hipMemcpyDeviceToHost, c->stream);
+    }
+    DevCounters hc[2]{};
+    if (e == hipSuccess) e = hipMemcpyAsync(hc, c->dcnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(AGX_E_HIP, std::string("trace_rays_noalloc: ") + hipGetErrorString(e));
     agx_stats_t st{};
     st.node_bytes = c->accel != AGX_ACCEL_ATRC ? 16 * BVH4_NODE_VEC4 : 32;
     st.prim_bytes = 48;

@@ -141,6 +141,19 @@ void launch_film_clear(const PathBuffers &pb, int64_t wh, hipStream_t st);
 void launch_film_resolve(const PathBuffers &pb, int64_t wh, float *rgb, float *albedo, float *normal, float *denoise,
                          hipStream_t st);
 // parity hook: rays in device memory, results to device memory
+// Staging of agx_trace_rays_noalloc: a file-scope device array of
+// NOALLOC_MAX_RAYS rays (84 B each: packed rays, raw inputs, outputs) that the
+// library never allocates or frees.  noalloc_scratch_init resolves its address
+// on the current device (once per device; call with that device current);
+// noalloc_scratch returns it, or nullptr before init.
+constexpr uint32_t NOALLOC_MAX_RAYS = 1u << 17;
+constexpr size_t NOALLOC_BYTES_PER_RAY = 84;
+hipError_t noalloc_scratch_init(int device);
+void *noalloc_scratch(int device);
+// trace_rays over raw host-layout inputs staged in the scratch: packs
+// (o, tmin) / (d, tmax), clears cnt with a kernel, then traces
+void launch_trace_rays_staged(const DevScene &s, void *scratch, uint32_t n, int any_hit, DevCounters *cnt,
+                              hipStream_t st);
 void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st);
 

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include <cstdint>
 
@@ -3007,6 +3008,68 @@ void launch_capture_rays(const PathBuffers &pb, int which, float4 *o, float4 *d,
     if (!max_n) return;
     hipLaunchKernelGGL(k_capture_rays, dim3(grid_of(max_n, 256)), dim3(256), 0, st, pb, which, o, d, count, cap);
 }
+// ---------------------------------------------------------------- staging without allocation
+// The staging buffer of agx_trace_rays_noalloc.  Layout for n rays (n <=
+// NOALLOC_MAX_RAYS), offsets in units of NOALLOC_MAX_RAYS 4-byte words:
+//   [0, 4) packed (o, tmin) float4   [4, 8) packed (d, tmax) float4
+//   [8, 11) o (3 floats per ray)     [11, 14) d   [14] tmin   [15] tmax
+//   [16] entity  [17] primitive  [18] t  [19] u  [20] v
+// Every GPU holds its own copy until the process ends; host code uses the
+// address hipGetSymbolAddress gives on that device, never the array's name.
+static __device__ __align__(256) uint32_t scratch[21 * size_t(NOALLOC_MAX_RAYS)];
+static_assert(21 * 4 == NOALLOC_BYTES_PER_RAY, "staging layout");
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+__global__ void __launch_bounds__(256) k_pack_rays(uint32_t *base, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t M = NOALLOC_MAX_RAYS;
+    const float *o = reinterpret_cast<const float *>(base + 8 * M), *d = reinterpret_cast<const float *>(base + 11 * M);
+    const float *tmin = reinterpret_cast<const float *>(base + 14 * M), *tmax = reinterpret_cast<const float *>(base + 15 * M);
+    float4 *po = reinterpret_cast<float4 *>(base), *pd = reinterpret_cast<float4 *>(base + 4 * M);
+    po[i] = make_float4(o[3 * i], o[3 * i + 1], o[3 * i + 2], tmin[i]);
+    pd[i] = make_float4(d[3 * i], d[3 * i + 1], d[3 * i + 2], tmax[i]);
+}
+
+__global__ void k_zero_counters(DevCounters *c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = DevCounters{0, 0, 0, 0};
+}
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+void *noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+void launch_trace_rays_staged(const DevScene &s, void *scratch_ptr, uint32_t n, int any_hit, DevCounters *cnt,
+                              hipStream_t st) {
+    if (!n) return;
+    uint32_t *base = static_cast<uint32_t *>(scratch_ptr);
+    const size_t M = NOALLOC_MAX_RAYS;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, cnt);
+    hipLaunchKernelGGL(k_pack_rays, dim3(grid_of(n, 256)), dim3(256), 0, st, base, n);
+    launch_trace_rays(s, reinterpret_cast<const float4 *>(base), reinterpret_cast<const float4 *>(base + 4 * M), n,
+                      any_hit, reinterpret_cast<int32_t *>(base + 16 * M), reinterpret_cast<int32_t *>(base + 17 * M),
+                      reinterpret_cast<float *>(base + 18 * M), reinterpret_cast<float *>(base + 19 * M),
+                      reinterpret_cast<float *>(base + 20 * M), cnt + (any_hit ? 1 : 0), st);
+}
+
 void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint32_t n, int any_hit, int32_t *ent,
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {
     if (!n) return;

@@ -348,6 +348,20 @@ int  agx_trace_rays(agx_ctx *ctx, const float *o, const float *d,
                     const float *tmin, const float *tmax, uint32_t n, int32_t any_hit,
                     int32_t *entity_id, int32_t *prim_id, float *t, float *u, float *v);
 
+/* agx_trace_rays without any device allocation (no counterpart in Atrc): for
+ * callers that cannot allocate device memory, such as serving loops and
+ * harnesses that forbid hipMalloc.  Same arguments, results and statistics as
+ * agx_trace_rays; rays and hits are staged in a fixed device array that the
+ * library holds on every GPU it runs on until the process ends
+ * (AGX_NOALLOC_MAX_RAYS rays, 84 bytes each, about 11 MB), so n above
+ * AGX_NOALLOC_MAX_RAYS returns AGX_E_LIMIT.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (from several threads or
+ * contexts) race, and must be serialised by the caller. */
+#define AGX_NOALLOC_MAX_RAYS 131072
+int  agx_trace_rays_noalloc(agx_ctx *ctx, const float *o, const float *d,
+                            const float *tmin, const float *tmax, uint32_t n, int32_t any_hit,
+                            int32_t *entity_id, int32_t *prim_id, float *t, float *u, float *v);
+
 int  agx_get_stats(agx_ctx *ctx, agx_stats_t *out);
 
 /* Batched rendering (no counterpart in Atrc, which renders one scene per
