@@ -662,13 +662,21 @@ def disney_spheres() -> tuple[Scene, dict, dict]:
     return s, cam, {"min_depth": 3, "max_depth": 8, "cont_prob": 0.9}
 
 
+SUN_ANGULAR_RADIUS = math.radians(0.265)   # the sun's 0.53 degree disk
+
+
 def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, extent: float = 50.0,
-                       ground: bool = True) -> tuple[Scene, dict, dict]:
+                       ground: bool = True, sun: bool = False) -> tuple[Scene, dict, dict]:
     """Seeded procedural mesh scene: n icospheres (20*4^subdiv triangles each) of
     radius U[0.2, 2] in a [0, extent]^3 box, radially displaced by value noise of
     amplitude 0.1 r, on a ground quad; half Lambert (albedo U[0.1, 0.9]), half
     GGX conductor (roughness U[0.05, 0.6], gold or copper); native_sky.
-    200 spheres at subdivision 4 give 1,024,000 triangles."""
+    200 spheres at subdivision 4 give 1,024,000 triangles.
+
+    sun=True adds the sun as Atrc can express it: a distant emissive disk (Atrc
+    has no directional light) 40 scene extents away at elevation 55 degrees,
+    facing the scene, subtending 0.53 degrees, with radiance chosen so that its
+    irradiance on the ground is about three times the sky's."""
     rng = np.random.default_rng(seed)
     base_v, faces = icosphere(subdiv)
     s = Scene(aggregate=capi.AGGREGATE_BVH)
@@ -694,6 +702,22 @@ def procedural_spheres(n_spheres: int = 200, subdiv: int = 4, seed: int = 1, ext
         lo, hi, z = -100.0, extent + 100.0, -2.5
         s.add(Entity(capi.GEOM_QUAD, g, points=((lo, lo, z), (hi, lo, z), (hi, hi, z), (lo, hi, z))))
     s.env = {"top": (0.9, 0.95, 1.2), "bottom": (0.25, 0.22, 0.2)}
+    if sun:
+        el, az = math.radians(55.0), math.radians(30.0)
+        to_sun = np.array([math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)])
+        dist = 40.0 * extent
+        centre = np.full(3, extent / 2) + dist * to_sun
+        # disk frame: local +z (its emitting side) turned towards the scene
+        z = -to_sun
+        x = np.cross(np.array([0.0, 0.0, 1.0]), z)
+        x /= np.linalg.norm(x)
+        y = np.cross(z, x)
+        l2w = np.concatenate([np.stack([x, y, z], axis=1), centre[:, None]], axis=1)
+        omega = math.pi * math.tan(SUN_ANGULAR_RADIUS) ** 2
+        radiance = 3.0 * math.pi / (omega * math.sin(el))
+        black = s.add_material(Material(kind=capi.MAT_IDEAL_BLACK))
+        s.add(Entity(capi.GEOM_DISK, black, emit=(radiance, 0.95 * radiance, 0.85 * radiance),
+                     radius=dist * math.tan(SUN_ANGULAR_RADIUS), local_to_world=tuple(float(v) for v in l2w.ravel())))
     camera = {"pos": (-25.0, -35.0, 45.0), "dst": (25.0, 25.0, 15.0), "up": (0, 0, 1), "fov": 40.0}
     trace = {"min_depth": 5, "max_depth": 8, "cont_prob": 0.9}
     return s, camera, trace

@@ -17,13 +17,17 @@ LIB_PATH = HERE / "liboracle.so"
 _lib = None
 
 
-def load() -> C.CDLL:
+def load(path: Path | None = None) -> C.CDLL:
+    """The oracle library (liboracle.so), or the library at `path` (for
+    instance a -march=native build of the same source, which the module then
+    uses from here on)."""
     global _lib
-    if _lib is not None:
+    if _lib is not None and path is None:
         return _lib
-    if not LIB_PATH.exists():
-        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-    lib = C.CDLL(str(LIB_PATH))
+    lib_path = Path(path) if path is not None else LIB_PATH
+    if not lib_path.exists():
+        raise RuntimeError(f"{lib_path} missing: run `make -C oracle`")
+    lib = C.CDLL(str(lib_path))
     P = C.POINTER
     lib.oracle_last_error.restype = C.c_char_p
     lib.oracle_scene_create.restype = C.c_void_p
