@@ -705,15 +705,15 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
                                 c->stream);
             launch_guide_ring_commit(c->g_ring, c->g_count, guide_batch, c->g_samples, c->g_wsum, c->stream);
             for (int k = 0; k < guide_steps; ++k) {
-                // matrix-core step: forward with the current BF16 weights, backward, reduce, Adam, repack
+                // matrix-core step: forward with the current BF16 weights, backward, reduce, Adam
+                // (which also repacks the BF16 weights)
                 launch_guide_train_mfma(c->g_params, c->g_packed, c->g_bias, c->g_samples, c->g_ring + 1, guide_batch,
                                         c->g_wsum, c->g_acts, c->g_dy, c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS,
                                         c->stream);
                 launch_guide_reduce(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_grad, c->stream);
                 c->g_steps += 1;
                 launch_guide_adam(c->g_params, c->g_m, c->g_v, c->g_grad, guide_lr, 0.9f, 0.99f, 1e-8f,
-                                  int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream);
-                launch_guide_pack(c->g_params, c->g_packed, c->g_bias, c->stream);
+                                  int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream, c->g_packed, c->g_bias);
             }
             HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipMemcpyAsync(&guide_valid_h, c->g_ring + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1451,8 +1451,26 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
     launch_guide_train(dp.get<float>(), ds.get<float>(), dscal.get<uint32_t>(), n, dscal.get<float>() + 1,
                        dpart.get<float>(), BLOCKS, c->stream);
     launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
-    if (lr != 0.0f) launch_guide_adam(dp.get<float>(), dm.get<float>(), dv.get<float>(), dg.get<float>(), lr, 0.9f,
-                                      0.99f, 1e-8f, t, c->stream);
+    DevBuf dpk, dbi, dpk2, dbi2;
+    if (lr != 0.0f) {
+        // the renders' Adam also repacks the BF16 weights: check it against the packing kernel
+        HIP_TRY(hipMalloc(&dpk.p, G_PACK * sizeof(__bf16)));
+        HIP_TRY(hipMalloc(&dbi.p, G_BIASES * sizeof(float)));
+        HIP_TRY(hipMalloc(&dpk2.p, G_PACK * sizeof(__bf16)));
+        HIP_TRY(hipMalloc(&dbi2.p, G_BIASES * sizeof(float)));
+        launch_guide_adam(dp.get<float>(), dm.get<float>(), dv.get<float>(), dg.get<float>(), lr, 0.9f, 0.99f, 1e-8f,
+                          t, c->stream, dpk.get<__bf16>(), dbi.get<float>());
+        launch_guide_pack(dp.get<float>(), dpk2.get<__bf16>(), dbi2.get<float>(), c->stream);
+        std::vector<uint16_t> a(G_PACK), b(G_PACK);
+        std::vector<float> ba(G_BIASES), bb(G_BIASES);
+        HIP_TRY(hipMemcpyAsync(a.data(), dpk.p, G_PACK * 2, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(b.data(), dpk2.p, G_PACK * 2, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(ba.data(), dbi.p, G_BIASES * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(bb.data(), dbi2.p, G_BIASES * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (a != b || std::memcmp(ba.data(), bb.data(), G_BIASES * 4) != 0)
+            return fail(AGX_E_HIP, "Adam's repacked weights differ from the packing kernel's");
+    }
     HIP_TRY(hipGetLastError());
     if (grad_out) HIP_TRY(hipMemcpyAsync(grad_out, dg.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(params, dp.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -577,18 +577,37 @@ __global__ void __launch_bounds__(256) k_guide_ring_commit(uint32_t *ring, const
     if (threadIdx.x == 0) *wsum = part[0];
 }
 
-__global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int blocks, float *grad) {
+// The fixed-order sum runs in two passes so that every CU takes part: pass 1
+// adds each group of `per` consecutive block partials into the group's first
+// partial (GR_GROUPS x ceil(params / 256) blocks of independent loads), pass 2
+// adds the group sums in group order.
+constexpr int GR_GROUPS = 16;
+
+__global__ void __launch_bounds__(256) k_guide_reduce_groups(float *partial, int blocks, int per) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b0 = int(blockIdx.y) * per;
+    if (i >= G_PARAMS || b0 >= blocks) return;
+    const int b1 = min(blocks, b0 + per);
+    float g = partial[size_t(b0) * G_PARAMS + i];
+    for (int b = b0 + 1; b < b1; ++b) g += partial[size_t(b) * G_PARAMS + i];
+    partial[size_t(b0) * G_PARAMS + i] = g;
+}
+
+__global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int blocks, int per, float *grad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G_PARAMS) return;
     float g = 0;
-    for (int b = 0; b < blocks; ++b) g += partial[size_t(b) * G_PARAMS + i];
+    for (int b = 0; b < blocks; b += per) g += partial[size_t(b) * G_PARAMS + i];
     grad[i] = g;
 }
 
 // One Adam step on the FP32 master parameters: m, v moments; lr; bias
-// corrections for step t.
+// corrections for step t.  When packed is given, the updated parameter is
+// also written where k_guide_pack puts it (BF16 weight in fragment order, or
+// FP32 bias), so the next forward pass needs no separate packing launch.
 __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, float *v, const float *grad, float lr,
-                                                    float b1, float b2, float eps, int t) {
+                                                    float b1, float b2, float eps, int t, __bf16 *packed,
+                                                    float *bias) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G_PARAMS) return;
     const float g = grad[i];
@@ -598,7 +617,20 @@ __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, flo
     v[i] = vi;
     const float mh = mi / (1 - powf(b1, float(t)));
     const float vh = vi / (1 - powf(b2, float(t)));
-    params[i] -= lr * mh / (sqrtf(vh) + eps);
+    const float p = params[i] - lr * mh / (sqrtf(vh) + eps);
+    params[i] = p;
+    if (!packed) return;
+    int l = 0;
+    while (l + 1 < G_LAYERS && i >= g_w_off(l + 1)) ++l;
+    const int local = i - g_w_off(l);
+    if (local >= g_in(l) * g_out(l)) {  // bias
+        bias[g_bias_off(l) + local - g_in(l) * g_out(l)] = p;
+        return;
+    }
+    // inverse of k_guide_pack's index map
+    const int k = local / g_out(l), n = local % g_out(l), T = g_out(l) / 16;
+    const int sI = k / 32, tI = n / 16, lane = ((k % 32) / 8) * 16 + n % 16;
+    packed[g_pack_off(l) + ((sI * T + tI) * 64 + lane) * 8 + k % 8] = __bf16(p);
 }
 
 }  // namespace
@@ -637,14 +669,17 @@ void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t ca
     hipLaunchKernelGGL(k_guide_ring_commit, dim3(1), dim3(256), 0, st, ring, added, cap, samples, wsum);
 }
 
-void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_reduce, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, grad);
+void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st) {
+    const int per = (blocks + GR_GROUPS - 1) / GR_GROUPS;
+    hipLaunchKernelGGL(k_guide_reduce_groups, dim3((G_PARAMS + 255) / 256, GR_GROUPS), dim3(256), 0, st, partial,
+                       blocks, per);
+    hipLaunchKernelGGL(k_guide_reduce, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, per, grad);
 }
 
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
-                       int t, hipStream_t st) {
+                       int t, hipStream_t st, __bf16 *packed, float *bias) {
     hipLaunchKernelGGL(k_guide_adam, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, params, m, v, grad, lr, b1, b2,
-                       eps, t);
+                       eps, t, packed, bias);
 }
 
 void launch_guide_pack(const float *params, __bf16 *packed, float *bias, hipStream_t st) {

@@ -25,7 +25,8 @@ void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in
 size_t guide_train_smem_bytes();
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
                         const float *wsum_dev, float *partial, int blocks, hipStream_t st);
-void launch_guide_reduce(const float *partial, int blocks, float *grad, hipStream_t st);
+// sums the per-block partial gradients in a fixed order (overwrites partial)
+void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st);
 // the same gradient on the matrix cores (BF16 operands, FP32 accumulation): forward
 // of the ring's samples into acts (4 x cap x 64 bf16), backward through dy
 // (2 x cap x 64 bf16); blocks x G_PARAMS partial gradients
@@ -37,7 +38,9 @@ void launch_guide_train_mfma(const float *params, const __bf16 *packed, const fl
 // sum the weights of the valid samples into *wsum in a fixed order
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
                               float *wsum, hipStream_t st);
+// packed / bias (may be NULL): also refresh the BF16 weights and biases that
+// launch_guide_pack would produce from the updated parameters
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
-                       int t, hipStream_t st);
+                       int t, hipStream_t st, __bf16 *packed = nullptr, float *bias = nullptr);
 
 }  // namespace agx

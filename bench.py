@@ -537,9 +537,9 @@ def main() -> int:
     ap.add_argument("--no-guided", action="store_true", help="skip the guided vs plain equal-time leg")
     ap.add_argument("--guided-res", type=int, default=1024)
     ap.add_argument("--guided-tris", type=int, default=500_000)
-    ap.add_argument("--guided-seconds", type=float, default=4.0)
+    ap.add_argument("--guided-seconds", type=float, default=10.0)
     ap.add_argument("--guided-chunk", type=int, default=4)
-    ap.add_argument("--guided-ref-factor", type=float, default=16.0)
+    ap.add_argument("--guided-ref-factor", type=float, default=8.0)
     ap.add_argument("--guided-train-frac", type=float, default=0.3)
     ap.add_argument("--guided-steps", type=int, default=8)
     ap.add_argument("--guided-batch", type=int, default=65536)
