@@ -330,7 +330,6 @@ size_t max_paths_in_flight(int lights) {
 int fill_render_const(agx_ctx *c, const agx_render_params *p, RenderConst *out) {
     if (!c->has_scene) return fail(AGX_E_NO_SCENE, "no scene uploaded");
     if (p->width <= 0 || p->height <= 0 || p->spp <= 0) return fail(AGX_E_INVALID_ARG, "width, height and spp must be positive");
-    if (p->use_mis == 0) return fail(AGX_E_UNSUPPORTED, "gpu_pt implements trace_std (use_mis = 1) only");
     if (p->max_depth < 1 || p->min_depth < 0 || !(p->cont_prob > 0) || !(p->eps > 0))
         return fail(AGX_E_INVALID_ARG, "invalid depth / cont_prob / eps");
     // BoxFilter radius_ = radius - EPS (box_filter.cpp:15-21); GaussianFilter keeps
@@ -355,6 +354,7 @@ int fill_render_const(agx_ctx *c, const agx_render_params *p, RenderConst *out) 
     // a filter wider than half a pixel splats; sample_bound_of (framebuffer.h:36-47)
     // adds floor-based borders around the image
     rc.splat = (gaussian || fr > 0.5f) ? 1 : 0;
+    rc.nomis = p->use_mis == 0 ? 1 : 0;  // PathTracingRenderer: trace_std or trace_nomis (pt.cpp:29-31)
     rc.filter_kind = p->filter;
     if (p->task_grid_size < 0) return fail(AGX_E_INVALID_ARG, "invalid task_grid_size");
     rc.tile = p->task_grid_size > 0 ? p->task_grid_size : 32;
@@ -551,6 +551,7 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     const int guide_steps = p->guide_steps > 0 ? p->guide_steps : 4;
     const float guide_lr = p->guide_lr > 0 ? p->guide_lr : 1e-2f;
     if (guide) {
+        if (rc.nomis) return fail(AGX_E_UNSUPPORTED, "path guiding runs with trace_std (use_mis = 1) only");
         const float a = p->guide_alpha > 0 ? p->guide_alpha : 0.5f;
         if (!(a > 0 && a < 1)) return fail(AGX_E_INVALID_ARG, "guide_alpha must lie in (0, 1)");
         if (guide_batch > (1u << 20)) return fail(AGX_E_INVALID_ARG, "guide_batch above 1 << 20");

@@ -100,6 +100,7 @@ struct RenderConst {
     // each axis (EW x EH pixels)
     int32_t splat;
     int32_t tile;
+    int32_t nomis;         // 1: trace_nomis (use_mis = 0): no light sampling, emission at every hit
     int32_t filter_kind;   // AGX_FILTER_*
     int32_t border0, border1;
     int32_t EW, EH;

@@ -2208,8 +2208,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const float4 h4 = pb.hit[2 * p];
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
-    if (ent < 0) {  // path_tracing.cpp:41-49
-        if (dep.x == 1 && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance<F>(s, rd);
+    if (ent < 0) {  // path_tracing.cpp:41-49 (trace_nomis, :250-256: at every depth)
+        if ((dep.x == 1 || rc.nomis) && s.env_kind != AGX_ENV_NONE) L = L + coef * env_radiance<F>(s, rd);
         if (dep.x == 1) {  // the camera ray escaped: empty G-buffer (k_raygen leaves it to this branch)
             pb.gb_normal[p] = make_float4(0, 0, 0, 0);
             pb.gb_albedo[p] = make_float4(0, 0, 0, 1);
@@ -2244,7 +2244,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                 scattered = true;
                 dep.z += 1;
                 // NEE at the medium point: area lights only; the environment draws nothing
-                for (int32_t l = 0; l < rc.L; ++l) {
+                // (trace_nomis samples no light)
+                for (int32_t l = 0; l < (rc.nomis ? 0 : rc.L); ++l) {
                     const size_t slot = size_t(p) * rc.L + l;
                     const int32_t li = s.lights[l];
                     pb.nee[slot] = make_float4(0, 0, 0, 0);
@@ -2266,7 +2267,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                     put_shadow(pb, slot, ls.pos, normalize(ms.pos - ls.pos), eps, dis - eps);
                     shadow_mask |= 1u << l;
                 }
-                {  // MIS-BSDF ray with the phase function (direct_illum.cpp:194-253)
+                if (!rc.nomis) {  // MIS-BSDF ray with the phase function (direct_illum.cpp:194-253)
                     const float u0 = rng.uniform(), u1 = rng.uniform();
                     (void)rng.uniform();
                     V dir;
@@ -2283,15 +2284,16 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         push_mis = true;
                     }
                 }
-                {  // phase-function continuation (path_tracing.cpp:99-108)
+                {  // phase-function continuation (path_tracing.cpp:99-108; trace_nomis :285-296
+                   // samples around the surface hit's wr, tests only f and keeps the direction as sampled)
                     const float u0 = rng.uniform(), u1 = rng.uniform();
                     (void)rng.uniform();
                     V dir;
                     float f;
-                    hg_sample(ms.g, ms.wr, u0, u1, &dir, &f);
-                    if (f != 0.0f && !(f < eps)) {
+                    hg_sample(ms.g, rc.nomis ? sp.wr : ms.wr, u0, u1, &dir, &f);
+                    if (f != 0.0f && (rc.nomis || !(f < eps))) {
                         pb.ray_o[2 * p] = f4(ms.pos, 0.0f);
-                        pb.ray_d[2 * p] = f4(normalize(dir), INF_F);
+                        pb.ray_d[2 * p] = f4(rc.nomis ? dir : normalize(dir), INF_F);
                         coef = coef * (mk(f, f, f) / f);
                         dep.x += 1;
                         if (dep.x <= rc.max_depth) {
@@ -2316,10 +2318,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     }
     if (!scattered) {
         dep.z = 0;
-        if (dep.x == 1 && e.light >= 0)  // :123-130
+        if ((dep.x == 1 || rc.nomis) && e.light >= 0)  // :123-130 (trace_nomis, :313-318: at every depth)
             if (dot(sp.geo.z, sp.wr) > 0) L = L + coef * mk(e.emit);
-        // NEE over every light (mis_sample_light, direct_illum.cpp:11-42, 72-99)
-        for (int32_t l = 0; l < rc.L; ++l) {
+        // NEE over every light (mis_sample_light, direct_illum.cpp:11-42, 72-99); none in trace_nomis
+        for (int32_t l = 0; l < (rc.nomis ? 0 : rc.L); ++l) {
             float s5[5];
             for (int k = 0; k < 5; ++k) s5[k] = rng.uniform();
             const size_t slot = size_t(p) * rc.L + l;
@@ -2373,8 +2375,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                 shadow_mask |= 1u << l;
             }
         }
-        // MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147)
-        {
+        // MIS-BSDF ray (mis_sample_bsdf, direct_illum.cpp:130-147); none in trace_nomis
+        if (!rc.nomis) {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
             const BSmp bs = bsdf_sample<F>(b, sp.wr, u0, u1, u2, eps);
             if (!black(bs.f)) {
@@ -2442,11 +2444,12 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         } else {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
             const BSmp bs = bsdf_sample<F>(b, sp.wr, u0, u1, u2, eps);
-            if (!black(bs.f) && !(bs.pdf < eps)) {
+            // trace_nomis (:320-327) rejects only a black f and keeps the sampled direction
+            if (!black(bs.f) && (rc.nomis || !(bs.pdf < eps))) {
                 const float abscos = fabsf(cosv(sp.geo.z, bs.dir));
                 coef = coef * (bs.f * abscos / bs.pdf);
                 pb.ray_o[2 * p] = f4(eps_offset(sp, bs.dir, eps), 0.0f);
-                pb.ray_d[2 * p] = f4(normalize(bs.dir), INF_F);
+                pb.ray_d[2 * p] = f4(rc.nomis ? bs.dir : normalize(bs.dir), INF_F);
                 if (bs.delta && dep.x >= 2 && dep.y <= rc.specular_depth) { dep.x -= 1; dep.y += 1; }
                 dep.x += 1;
                 if (dep.x <= rc.max_depth) {

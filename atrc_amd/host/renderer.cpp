@@ -40,7 +40,6 @@ class GpuPathTracingRenderer : public Renderer {
 public:
     explicit GpuPathTracingRenderer(const PTRendererParams &p) : params_(p) {
         if (p.spp <= 0) throw ObjectConstructionException("invalid spp value");
-        if (!p.use_mis) throw ObjectConstructionException("gpu_pt implements trace_std (use_mis = 1) only");
     }
     ~GpuPathTracingRenderer() override {
         if (ctx_) agx_destroy(ctx_);
@@ -64,7 +63,7 @@ public:
         rp.max_depth = params_.max_depth;
         rp.specular_depth = params_.specular_depth;
         rp.cont_prob = params_.cont_prob;
-        rp.use_mis = 1;
+        rp.use_mis = params_.use_mis ? 1 : 0;
         rp.eps = EPS();
         rp.filter_radius = filter.filter->radius;
         rp.filter = filter.filter->type == "gaussian" ? AGX_FILTER_GAUSSIAN : AGX_FILTER_BOX;

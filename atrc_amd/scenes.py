@@ -248,12 +248,12 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
                   filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0,
                   guide: bool = False, guide_alpha: float = 0.5, guide_train: bool = True, guide_batch: int = 65536,
                   guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0,
-                  filter: str = "box", filter_alpha: float = 0.0, task_grid_size: int = 32):
+                  filter: str = "box", filter_alpha: float = 0.0, task_grid_size: int = 32, use_mis: bool = True):
     p = capi.agx_render_params()
     p.width, p.height, p.spp = width, height, spp
     p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
     p.cont_prob = cont_prob
-    p.use_mis = 1
+    p.use_mis = 1 if use_mis else 0
     p.eps = eps
     p.filter_radius = filter_radius
     p.seed = seed

@@ -223,7 +223,9 @@ typedef struct agx_render_params {
     int32_t max_depth;         /* default 10 */
     int32_t specular_depth;    /* default 20 */
     float   cont_prob;         /* default 0.9 */
-    int32_t use_mis;           /* must be 1 (trace_std) */
+    int32_t use_mis;           /* 1: trace_std (NEE with MIS); 0: trace_nomis (no light sampling;
+                                  emission and the environment added wherever a path meets them;
+                                  path_tracing.cpp:227-381); default 1 */
     float   eps;               /* global EPS, default 3e-4 */
     float   filter_radius;     /* film filter radius as given (default 0.5) */
     uint64_t seed;             /* per-(pixel, sample) PCG32 streams derive from it */

@@ -2231,7 +2231,13 @@ template<class T> struct PixelResult {
     T denoise = 1;
 };
 
-struct TraceParams { int min_depth = 5, max_depth = 10, specular_depth = 20; float cont_prob = 0.9f; };
+// PTRendererParams that reach the trace functions; use_mis selects trace_std
+// (1) or trace_nomis (0) as PathTracingRenderer does (pt.cpp:29-31)
+struct TraceParams {
+    int min_depth = 5, max_depth = 10, specular_depth = 20;
+    float cont_prob = 0.9f;
+    bool use_mis = true;
+};
 
 template<class T>
 V3<T> mis_sample_light(const Scene<T> &scene, int32_t light, const EntityHit<T> &inct, const BSDF<T> &bsdf,
@@ -2329,6 +2335,76 @@ PixelResult<T> trace_std(const TraceParams &p, const Scene<T> &scene, Ray<T> r, 
         if (bs.is_delta && depth >= 2 && s_depth <= p.specular_depth) { --depth; ++s_depth; }
     }
     return pixel;
+}
+
+// render::trace_nomis (path_tracing.cpp:227-381): no light sampling; the
+// environment is added wherever a path escapes and an area light's emission
+// wherever a path meets it.  Atrc's differences from trace_std are kept: the
+// phase function samples around the surface hit's wr (path_tracing.cpp:290-291),
+// a BSDF or phase sample is rejected only when f is black (no pdf < EPS test),
+// and the next ray keeps the sampled direction as it is (Ray does not normalise).
+template<class T>
+PixelResult<T> trace_nomis(const TraceParams &p, const Scene<T> &scene, Ray<T> r, Sampler<T> &sam, Counters *c) {
+    const T eps = scene.eps;
+    V3<T> coef(1);
+    PixelResult<T> pixel;
+    int scattering_count = 0;
+    for (int depth = 1, s_depth = 1; depth <= p.max_depth; ++depth) {
+        if (depth > p.min_depth) {                                          // :241-246
+            if (sam.u1() > T(p.cont_prob)) return pixel;
+            coef /= T(p.cont_prob);
+        }
+        EntityHit<T> inct;
+        if (!scene.closest(r, &inct, c)) {                                  // :250-256
+            if (scene.env_kind != AGX_ENV_NONE) pixel.value += coef * scene.sky_radiance(r.d);
+            return pixel;
+        }
+        const Entity<T> &ent = scene.entities[size_t(inct.entity)];
+        V3<T> shading_normal;
+        const BSDF<T> bsdf = scene.materials[size_t(ent.material)].shade(inct, &shading_normal);
+        if (depth == 1) {                                                   // :260-266
+            pixel.normal = shading_normal;
+            pixel.albedo = bsdf.albedo;
+            if (ent.no_denoise) pixel.denoise = 0;
+        }
+        const Medium<T> &med = scene.medium_of(inct, inct.wr);              // :270-307
+        if (scattering_count < med.max_scat) {
+            const MediumScatter<T> ms = med.sample_scattering(r.o, inct.pos, sam, scattering_count > 0);
+            coef *= ms.throughput;
+            if (ms.happened) {
+                ++scattering_count;
+                const T u0 = sam.u1(), u1 = sam.u1();
+                (void)sam.u1();
+                V3<T> dir;
+                T f;
+                ms.phase.sample(inct.wr, u0, u1, &dir, &f);
+                if (f == 0) return pixel;
+                coef *= V3<T>(f) / f;
+                r = Ray<T>(ms.pos, dir);
+                continue;
+            }
+        } else {
+            coef *= med.ab(r.o, inct.pos, sam);
+        }
+        scattering_count = 0;
+        if (ent.is_light)                                                   // :313-318
+            pixel.value += coef * scene.area_radiance(inct.entity, inct.geometry_coord.z, inct.wr);
+        T s3[3];                                                            // :320-337
+        for (int k = 0; k < 3; ++k) s3[k] = sam.u1();
+        const BSDFSample<T> bs = bsdf.sample(inct.wr, s3[0], s3[1], s3[2], eps);
+        if (bs.f.is_black()) return pixel;
+        const T abscos = std::abs(cos_between(inct.geometry_coord.z, bs.dir));
+        coef *= bs.f * abscos / bs.pdf;
+        r = Ray<T>(inct.eps_offset(bs.dir, eps), bs.dir);
+        if (bs.is_delta && depth >= 2 && s_depth <= p.specular_depth) { --depth; ++s_depth; }
+    }
+    return pixel;
+}
+
+template<class T>
+PixelResult<T> trace_path(const TraceParams &p, const Scene<T> &scene, const Ray<T> &r, Sampler<T> &sam,
+                          Sampler<T> &trs, Counters *c) {
+    return p.use_mis ? trace_std(p, scene, r, sam, trs, c) : trace_nomis(p, scene, r, sam, c);
 }
 
 // mis_sample_area_light / mis_sample_envir_light (direct_illum.cpp:11-42, 72-99)
@@ -2501,6 +2577,7 @@ PixelResult<T> trace_sample(const Scene<T> &scene, const agx_render_params &rp, 
     TraceParams tp;
     tp.min_depth = rp.min_depth; tp.max_depth = rp.max_depth; tp.specular_depth = rp.specular_depth;
     tp.cont_prob = rp.cont_prob;
+    tp.use_mis = rp.use_mis != 0;
     Sampler<T> sam, trs;
     sam.rng.seed_path(rp.seed, key, sample);
     trs.rng.seed_path(rp.seed ^ TR_STREAM_KEY, key, sample);
@@ -2510,7 +2587,7 @@ PixelResult<T> trace_sample(const Scene<T> &scene, const agx_render_params &rp, 
     const T au = sam.u1(), av = sam.u1();
     const Ray<T> ray = cam.sample(T(*sx / float(rp.width)), T(*sy / float(rp.height)), au, av);
     Counters c;
-    return trace_std(tp, scene, ray, sam, trs, &c);
+    return trace_path(tp, scene, ray, sam, trs, &c);
 }
 
 // GaussianFilter::eval without its normalisation factor, which cancels in
@@ -2549,6 +2626,7 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
     TraceParams tp;
     tp.min_depth = rp.min_depth; tp.max_depth = rp.max_depth; tp.specular_depth = rp.specular_depth;
     tp.cont_prob = rp.cont_prob;
+    tp.use_mis = rp.use_mis != 0;
     const int W = rp.width, H = rp.height;
     if (row_end < 0) row_end = H;
     // BoxFilter radius_ = r - EPS (box_filter.cpp:15-21); GaussianFilter keeps r
@@ -2612,7 +2690,7 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                             const float sx = float(qx) + fu, sy = float(qy) + fv;
                             const T au = sam.u1(), av = sam.u1();
                             const Ray<T> ray = cam.sample(T(sx / float(W)), T(sy / float(H)), au, av);
-                            const PixelResult<T> pr = trace_std(tp, scene, ray, sam, trs, &c);
+                            const PixelResult<T> pr = trace_path(tp, scene, ray, sam, trs, &c);
                             if (!pr.value.is_finite()) continue;
                             // apply_image_filter over this tile's film grid
                             int px_[AXIS_CAP], py_[AXIS_CAP];
@@ -2659,7 +2737,7 @@ void render(const Scene<T> &scene, const agx_render_params &rp, int threads, con
                         const float pxf = float(px) + fu, pyf = float(py) + fv;
                         const T au = sam.u1(), av = sam.u1();
                         const Ray<T> ray = cam.sample(T(pxf / float(W)), T(pyf / float(H)), au, av);
-                        const PixelResult<T> pr = trace_std(tp, scene, ray, sam, trs, &c);
+                        const PixelResult<T> pr = trace_path(tp, scene, ray, sam, trs, &c);
                         if (!pr.value.is_finite()) continue;
                         // apply_image_filter with box radius r: a sample reaches only its
                         // own pixel, and only if |px + 0.5 - sample| <= r on both axes

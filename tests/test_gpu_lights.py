@@ -62,3 +62,19 @@ def test_path_parity(gpu_ctx, which, kind):
     np.testing.assert_allclose(g["albedo"], r["albedo"], atol=1e-4)
     np.testing.assert_allclose(g["normal"], r["normal"], atol=1e-4)
     np.testing.assert_allclose(g["denoise"], r["denoise"], atol=1e-6)
+
+
+@pytest.mark.parametrize("kind", ["quads", "mesh"])
+def test_furnace_without_mis(gpu_ctx, kind):
+    """trace_nomis on the surface furnaces: L (1 - rho^D) / (1 - rho) at max depth D."""
+    D = 4
+    s, cam, tr, _ = scenes.emitting_enclosure(kind, max_depth=D)
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    expected = (1.0 - 0.5 ** D) / 0.5
+    img = gpu_ctx.render(scenes.render_params(128, 128, 512, cam, seed=33, use_mis=False, **tr))["image"]
+    img = img.astype(np.float64)
+    m = float(img.mean())
+    se = float(img.std()) / np.sqrt(img.size)
+    log(f"[lights] trace_nomis furnace/{kind}: mean {m:.6f} expected {expected:.6f}")
+    assert abs(m - expected) <= 4 * se + 3e-4 * expected

@@ -27,10 +27,10 @@ def relmse(x, y):
     return float(np.mean((x - y) ** 2 / (y ** 2 + 1e-2)))
 
 
-def _path_parity(gpu_ctx, scene, camera, trace, W, H, label, min_frac=0.98):
+def _path_parity(gpu_ctx, scene, camera, trace, W, H, label, min_frac=0.98, **extra):
     desc, keep = scene.to_desc()
     gpu_ctx.upload_scene(desc)
-    p = scenes.render_params(W, H, 1, camera, seed=1234, **trace)
+    p = scenes.render_params(W, H, 1, camera, seed=1234, **trace, **extra)
     log(f"[render parity] {label} {W}x{H} 1 spp")
     g = gpu_ctx.render(p)
     r = po.OracleScene(desc, 32).render(p, threads=8)
@@ -87,6 +87,20 @@ def test_delta_lens_converged(gpu_ctx):
     e = relmse(g, r)
     log(f"  relMSE {e:.3e}")
     assert e < 5e-3
+
+
+@pytest.mark.parametrize("which", ["cornell", "delta", "volumetric"])
+def test_trace_nomis_path_parity(gpu_ctx, which):
+    """use_mis = 0: trace_nomis (path_tracing.cpp:227-381) on a plain, a
+    delta/lens/curved-light and a media scene."""
+    if which == "cornell":
+        s, cam, tr = scenes.cornell_box()
+    elif which == "delta":
+        s, cam, tr = scenes.cornell_delta(True)
+    else:
+        s, cam, tr = scenes.volumetric_cornell(grid_n=16)
+    s.accel = capi.ACCEL_BVH4
+    _path_parity(gpu_ctx, s, cam, tr, 96, 96, f"trace_nomis {which}", use_mis=False)
 
 
 def test_cornell_converged(gpu_ctx):

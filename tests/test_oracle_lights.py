@@ -56,3 +56,37 @@ def test_furnace_tolerance_resolves_one_bounce():
     _check(r["image"], expected)
     with pytest.raises(AssertionError):
         _check(r["image"], expected - 0.5 ** 6)
+
+
+@pytest.mark.parametrize("kind", ["quads", "mesh"])
+def test_surface_furnace_without_mis(kind):
+    """trace_nomis (use_mis = 0, path_tracing.cpp:227-381) adds emission at every
+    hit and samples no light: the last vertex adds no direct light, so the
+    enclosure reads L (1 - rho^D) / (1 - rho), one term fewer than trace_std."""
+    D = 4
+    s, cam, tr, expected_std = scenes.emitting_enclosure(kind, max_depth=D)
+    desc, keep = s.to_desc()
+    r = po.OracleScene(desc, 64).render(scenes.render_params(40, 40, 256, cam, seed=24, use_mis=False, **tr), threads=8)
+    rho = 0.5
+    expected = (1.0 - rho ** D) / (1.0 - rho)
+    _check(r["image"], expected)
+    with pytest.raises(AssertionError):
+        _check(r["image"], expected_std)
+
+
+def test_white_furnace_without_mis():
+    """A Lambert sphere (albedo rho) under a uniform sky of radiance 1 reads rho
+    with trace_nomis too: one bounce, then the sky."""
+    from atrc_amd import capi
+    rho = 0.7
+    s = scenes.Scene(aggregate=capi.AGGREGATE_NATIVE)
+    m = s.add_material(scenes.Material(color=(rho, rho, rho)))
+    s.add(scenes.Entity(capi.GEOM_SPHERE, m, radius=1.0))
+    s.env = {"top": (1.0, 1.0, 1.0), "bottom": (1.0, 1.0, 1.0)}
+    desc, keep = s.to_desc()
+    cam = {"pos": (0.0, -4.0, 0.0), "dst": (0.0, 0.0, 0.0), "fov": 60.0}
+    img = po.OracleScene(desc, 64).render(scenes.render_params(32, 32, 64, cam, min_depth=3, max_depth=10, seed=5,
+                                                               use_mis=False), threads=8)["image"]
+    yy, xx = np.mgrid[0:32, 0:32]
+    inner = (xx - 15.5) ** 2 + (yy - 15.5) ** 2 < 5 ** 2
+    np.testing.assert_allclose(img[inner], rho, rtol=1e-5)
