@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 10
+AGX_ABI_VERSION = 11
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -45,6 +45,7 @@ class agx_entity_desc(C.Structure):
         ("emit_radiance", F3), ("power", C.c_float), ("no_denoise", C.c_int32), ("tri_count", C.c_uint32),
         ("tris", C.POINTER(agx_vertex)), ("p", (C.c_float * 3) * 4), ("t", (C.c_float * 2) * 4),
         ("radius", C.c_float), ("local_to_world", C.c_float * 12), ("world_to_local", C.c_float * 12),
+        ("double_sided", C.c_int32),
     ]
 
 

@@ -551,7 +551,7 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         d.kind = e.geometry;
         d.material = e.material;
         d.light = -1;
-        d.flags = e.no_denoise ? 1 : 0;
+        d.flags = (e.no_denoise ? EF_NO_DENOISE : 0) | (e.double_sided ? EF_DOUBLE_SIDED : 0);
         if (e.material < 0 || uint32_t(e.material) >= desc.material_count) {
             *err = "entity " + std::to_string(i) + ": material index out of range";
             return AGX_E_INVALID_ARG;

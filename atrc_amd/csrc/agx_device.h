@@ -56,11 +56,15 @@ constexpr int BLAS_STACK = 96;               // >= max BLAS depth + 2 (checked a
 constexpr int TLAS_STACK = 64;
 constexpr int LDS_STACK = 32;                // LDS stack column per thread (TLAS + BLAS entries)
 
+// DevEntity::flags
+constexpr int32_t EF_NO_DENOISE = 1;
+constexpr int32_t EF_DOUBLE_SIDED = 2;  // double_sided.cpp: back-side hits turn the frames around
+
 struct DevEntity {
     int32_t kind;          // AGX_GEOM_*
     int32_t material;
     int32_t light;         // -1 or index into the light list
-    int32_t flags;         // bit 0: no_denoise
+    int32_t flags;         // EF_*
     int32_t med_in, med_out;  // media on the -z / +z side of the geometric normal, -1 = void
     int32_t tie_rank;      // order in which Atrc's aggregate visits the entity: on an exactly equal
                            // hit distance the later-visited entity wins (native: index; BVH: leaf order)

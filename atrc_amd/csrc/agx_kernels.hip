@@ -884,7 +884,7 @@ __device__ void mesh_frame(const DevScene &s, uint32_t slot, float bu, float bv,
     sp->user = rotate_to_new_z(sp->geo, user_z);
 }
 
-__device__ Surf surface_at(const DevScene &s, const HitRec &h, V o, V d, float eps) {
+__device__ Surf surface_at_raw(const DevScene &s, const HitRec &h, V o, V d, float eps) {
     const DevEntity &e = s.ents[h.ent];
     Surf sp;
     switch (e.kind) {
@@ -949,6 +949,20 @@ __device__ Surf surface_at(const DevScene &s, const HitRec &h, V o, V d, float e
     }
 }
 
+__device__ __forceinline__ Frame flip(const Frame &f) { return Frame{-f.x, -f.y, -f.z}; }
+
+// The surface point of a hit; a double-sided entity hit from the back turns its
+// geometry and user frames around (DoubleSidedGeometry::closest_intersection,
+// double_sided.cpp:26-38)
+__device__ __forceinline__ Surf surface_at(const DevScene &s, const HitRec &h, V o, V d, float eps) {
+    Surf sp = surface_at_raw(s, h, o, d, eps);
+    if ((s.ents[h.ent].flags & EF_DOUBLE_SIDED) && dot(sp.geo.z, sp.wr) < 0) {
+        sp.geo = flip(sp.geo);
+        sp.user = flip(sp.user);
+    }
+    return sp;
+}
+
 __device__ __forceinline__ V eps_offset(const Surf &sp, V dir, float eps) {  // intersection.h:17-22
     if (dot(dir, sp.geo.z) > 0) return sp.pos + sp.geo.z * eps;
     return sp.pos - sp.geo.z * eps;
@@ -979,10 +993,12 @@ __device__ float geom_pdf(const DevEntity &e, V ref) {
     return 1 / e.emit.w;
 }
 
-// GeometryToDiffuseLight::pdf (geometry_to_diffuse_light.cpp:75-90)
+// GeometryToDiffuseLight::pdf (geometry_to_diffuse_light.cpp:75-90); a
+// double-sided geometry halves its pdf (double_sided.cpp:89-99)
 __device__ float area_light_pdf(const DevEntity &e, V ref, V pos, V nor) {
     if (dot(nor, ref - pos) <= 0) return 0;
-    const float ap = geom_pdf(e, ref);
+    float ap = geom_pdf(e, ref);
+    if (e.flags & EF_DOUBLE_SIDED) ap = ap * 0.5f;
     const V s2r = ref - pos;
     const float dist2 = dot(s2r, s2r);
     const float ac = fabsf(cosv(nor, s2r));
@@ -1001,6 +1017,11 @@ __device__ LightSmp area_light_sample(const DevScene &s, const DevEntity &e, V r
     ls.pdf = 0;
     V pos, z;
     float pdf_area;
+    // a double-sided geometry picks its side with the first number and reuses
+    // it, rescaled, for the wrapped geometry (double_sided.cpp:66-87)
+    const bool two = (e.flags & EF_DOUBLE_SIDED) != 0;
+    const bool back = two && su < 0.5f;
+    if (two) su = back ? 2 * su : 2 * (su - 0.5f);
     switch (e.kind) {
     case AGX_GEOM_QUAD: {  // quad.cpp:80-105
         const float sq = sqrtf(su);
@@ -1067,6 +1088,10 @@ __device__ LightSmp area_light_sample(const DevScene &s, const DevEntity &e, V r
         pdf_area = 1 / e.emit.w;
         break;
     }
+    }
+    if (two) {
+        pdf_area = pdf_area * 0.5f;
+        if (back) z = -z;
     }
     if (dot(z, ref - pos) <= 0) return ls;
     const V s2r = ref - pos;
@@ -2226,7 +2251,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     const Bsdf b = make_bsdf(s.mats[e.material], sp, eps);
     if (dep.x == 1) {  // :54-60
         pb.gb_normal[p] = f4(sp.user.z, 0);
-        pb.gb_albedo[p] = f4(b.albedo, (e.flags & 1) ? 0.0f : 1.0f);
+        pb.gb_albedo[p] = f4(b.albedo, (e.flags & EF_NO_DENOISE) ? 0.0f : 1.0f);
     }
     bool pending = false;
     bool cont = false;

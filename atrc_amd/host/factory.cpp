@@ -555,11 +555,51 @@ CreatingContext::CreatingContext() {
             set_transforms(g->proto, Transform3());
             return g;
         });
-    for (const char *unsupported : {"double_sided", "transform_wrapper"})
-        geometry.add_creator(unsupported, [n = std::string(unsupported)](const ConfigGroup &, CreatingContext &)
-                                              -> std::shared_ptr<Geometry> {
-            throw ObjectConstructionException("geometry type '" + n + "' is not supported by gpu_pt yet");
-        });
+    // DoubleSidedGeometry (double_sided.cpp:22-100): the wrapped geometry with the
+    // double_sided flag; the library turns back-side frames around and halves
+    // the light pdf
+    geometry.add_creator("double_sided", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto in = ctx.create<Geometry>(p.child_group("internal"));
+        if (in->proto.double_sided)
+            throw ObjectConstructionException("double_sided of a double_sided geometry is not supported by gpu_pt");
+        auto g = std::make_shared<Geometry>(*in);
+        g->proto.double_sided = 1;
+        return g;
+    });
+    // TransformWrapper (transform_wrapper.cpp:5-140): the wrapper's transform is
+    // composed into the wrapped geometry at load (quad corners and mesh
+    // vertices move, the local-to-world of triangle / sphere / disk is
+    // premultiplied).  Atrc intersects in the wrapper's local space instead, so
+    // hits agree up to rounding; its light pdf scales by the z axis' length
+    // squared, which equals the exact area ratio for uniform scales.
+    geometry.add_creator("transform_wrapper", [](const ConfigGroup &p, CreatingContext &ctx) {
+        auto in = ctx.create<Geometry>(p.child_group("internal"));
+        const Transform3 T = p.child_transform3("transform");
+        auto g = std::make_shared<Geometry>(*in);
+        switch (g->proto.geometry) {
+        case AGX_GEOM_QUAD:
+            for (int k = 0; k < 4; ++k)
+                copy3(g->proto.p[k], T.apply_to_point(Vec3{g->proto.p[k][0], g->proto.p[k][1], g->proto.p[k][2]}));
+            break;
+        case AGX_GEOM_TRIANGLE_BVH: {
+            auto tris = std::make_shared<std::vector<agx_vertex>>(*in->mesh);
+            for (agx_vertex &v : *tris) {
+                copy3(v.pos, T.apply_to_point(Vec3{v.pos[0], v.pos[1], v.pos[2]}));
+                copy3(v.nor, T.apply_to_vector(Vec3{v.nor[0], v.nor[1], v.nor[2]}));
+            }
+            g->mesh = tris;
+            break;
+        }
+        default: {
+            Transform3 l2w;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 4; ++j) l2w.m[i][j] = g->proto.local_to_world[4 * i + j];
+            set_transforms(g->proto, T * l2w);
+            break;
+        }
+        }
+        return g;
+    });
 
     // material (material_creators.cpp)
     material.add_creator("ideal_diffuse", [](const ConfigGroup &p, CreatingContext &ctx) {

@@ -110,6 +110,7 @@ class Entity:
     radius: float = 1.0
     local_to_world: tuple = IDENTITY     # row-major 3x4
     no_denoise: bool = False
+    double_sided: bool = False           # Atrc "double_sided" around the geometry
     med_in: int = -1                     # index into Scene.media, -1 = void
     med_out: int = -1
 
@@ -155,6 +156,7 @@ class Scene:
             d.emit_radiance[:] = [float(x) for x in e.emit]
             d.power = -1.0
             d.no_denoise = 1 if e.no_denoise else 0
+            d.double_sided = 1 if e.double_sided else 0
             if e.tris is not None:
                 arr = np.ascontiguousarray(e.tris, np.float32)
                 keep.append(arr)
@@ -803,7 +805,7 @@ def _inward_shell(radius: float, subdiv: int) -> np.ndarray:
 
 
 def emitting_enclosure(kind: str = "quads", rho: float = 0.5, emit: float = 1.0,
-                       max_depth: int = 24) -> tuple[Scene, dict, dict, float]:
+                       max_depth: int = 24, outward_double_sided: bool = False) -> tuple[Scene, dict, dict, float]:
     """Closed emitting Lambert enclosure (surface furnace): every wall is a diffuse
     area light of radiance `emit` with a Lambert albedo `rho`.  The camera inside
     sees L = emit * (1 - rho^(D+1)) / (1 - rho) at every pixel for max depth D
@@ -823,18 +825,30 @@ def emitting_enclosure(kind: str = "quads", rho: float = 0.5, emit: float = 1.0,
     else:
         s.add(Entity(capi.GEOM_TRIANGLE_BVH, m, emit=e, tris=_inward_shell(2.0, 2)))
         cam = {"pos": (0.2, -0.3, 0.1), "dst": (1.0, 1.0, 0.5), "up": (0, 0, 1), "fov": 70.0}
+    if outward_double_sided:
+        # the same walls facing outwards, made double-sided: the inside sees
+        # their back sides, which shade and emit like front sides
+        for ent in s.entities:
+            ent.double_sided = True
+            if ent.tris is not None:
+                t = ent.tris.reshape(-1, 3, 8)[:, ::-1].reshape(-1, 8).copy()
+                t[:, 3:6] *= -1
+                ent.tris = t
+            else:
+                ent.points = tuple(reversed(ent.points))
     expected = emit * (1.0 - rho ** (max_depth + 1)) / (1.0 - rho)
     return s, cam, {"min_depth": 3, "max_depth": max_depth, "cont_prob": 0.8}, expected
 
 
 def light_in_shell(kind: str = "sphere", rho: float = 0.6, emit: float = 3.0, shell_radius: float = 4.0,
-                   subdiv: int = 6) -> tuple[Scene, dict, dict, float]:
+                   subdiv: int = 6, double_sided: bool = False, face_up: bool = False) -> tuple[Scene, dict, dict, float]:
     """Direct light (max depth 1) on an inward Lambert shell of radius D from an
     emitter whose irradiance is the same at every shell point, so that every pixel
     that sees the shell has one closed-form value:
     * "sphere": a sphere light of radius R = 1 at the centre; E = pi L (R/D)^2,
       radiance rho L (R/D)^2.
-    * "disk": a disk light facing -z whose rim lies on the sphere at height
+    * "disk": a disk light facing -z (or +z with face_up, then seen from below
+      only through double_sided) whose rim lies on the sphere at height
       z0 = D/2.  By the sphere's configuration-factor theorem (from a point on a
       sphere, any area on the sphere has form factor A / (4 pi D^2)) and because
       the disk and the spherical cap above it share their boundary, every shell
@@ -850,14 +864,15 @@ def light_in_shell(kind: str = "sphere", rho: float = 0.6, emit: float = 3.0, sh
     e = (emit, emit, emit)
     if kind == "sphere":
         R = 1.0
-        s.add(Entity(capi.GEOM_SPHERE, black, emit=e, radius=R))
+        s.add(Entity(capi.GEOM_SPHERE, black, emit=e, radius=R, double_sided=double_sided))
         expected = rho * emit * (R / D) ** 2
         cam = {"pos": (0.0, -1.6, -0.4), "dst": (0.3, -4.0, -1.0), "up": (0, 0, 1), "fov": 60.0}
     else:
         z0 = D / 2
         a = math.sqrt(D * D - z0 * z0)
         # rotation by pi about x: the disk's local +z normal points down
-        s.add(Entity(capi.GEOM_DISK, black, emit=e, radius=a, local_to_world=(1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1, z0)))
+        l2w = (1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, z0) if face_up else (1, 0, 0, 0, 0, -1, 0, 0, 0, 0, -1, z0)
+        s.add(Entity(capi.GEOM_DISK, black, emit=e, radius=a, local_to_world=l2w, double_sided=double_sided))
         expected = rho * emit * (D - z0) / (2 * D)
         cam = {"pos": (0.0, 0.0, 0.5), "dst": (0.4, 0.3, -3.0), "up": (0, 1, 0), "fov": 70.0}
     return s, cam, {"min_depth": 5, "max_depth": 1, "cont_prob": 0.9}, expected
@@ -930,6 +945,8 @@ def to_atrc_json(scene: Scene, camera: dict, trace: dict, width: int, height: in
         else:
             g = {"type": "sphere" if e.geometry == capi.GEOM_SPHERE else "disk", "radius": float(e.radius),
                  "transform": _transform_json(e.local_to_world)}
+        if e.double_sided:
+            g = {"type": "double_sided", "internal": g}
         ent = {"type": "geometric", "geometry": g, "material": _material_json(scene.materials[e.material])}
         if any(e.emit):
             ent["emit_radiance"] = [float(x) for x in e.emit]

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 10
+#define AGX_ABI_VERSION 11
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -109,6 +109,10 @@ typedef struct agx_entity_desc {
     float   radius;            /* sphere, disk */
     float   local_to_world[12];/* row-major 3x4 (triangle, sphere, disk) */
     float   world_to_local[12];/* its inverse, row-major 3x4 */
+    /* "double_sided" (double_sided.cpp:22-100): a hit from the back turns the
+     * geometry's frames around, so the surface shades and emits on both sides;
+     * as a light it samples either side with probability 1/2 (pdf halved). */
+    int32_t double_sided;
 } agx_entity_desc;
 
 /* Atrc Texture3D: "constant3d" or "image3d" (src/tracer/include/agz/tracer/core/texture3d.h,

@@ -674,6 +674,7 @@ template<class T> struct TriangleBVH {
 
 template<class T> struct Geometry {
     int kind = AGX_GEOM_QUAD;
+    bool double_sided = false;  // DoubleSidedGeometry around this geometry (double_sided.cpp)
     // quad (core/geometry/quad.cpp): world-space a, edges; triangle (triangle.cpp): local
     V3<T> a, b_a, c_a, d_a;
     V2<T> t_a, t_b_a, t_c_a, t_d_a;
@@ -704,6 +705,7 @@ template<class T> struct Geometry {
 
     void init(const agx_entity_desc &e) {
         kind = e.geometry;
+        double_sided = e.double_sided != 0;
         auto P = [&](int i) { return V3<T>(T(e.p[i][0]), T(e.p[i][1]), T(e.p[i][2])); };
         auto TT = [&](int i) { return V2<T>(T(e.t[i][0]), T(e.t[i][1])); };
         l2w = Affine<T>::from(e.local_to_world);
@@ -822,7 +824,19 @@ template<class T> struct Geometry {
         sp->user_coord = sp->geometry_coord;
     }
 
+    // DoubleSidedGeometry::closest_intersection (double_sided.cpp:26-38): a hit on
+    // the back side turns both frames around
     bool closest(const Ray<T> &r, GeometryHit<T> *h, T eps, Counters *c) const {
+        if (!closest_one_sided(r, h, eps, c)) return false;
+        if (double_sided && dot(h->geometry_coord.z, h->wr) < 0) {
+            h->geometry_coord = flip(h->geometry_coord);
+            h->user_coord = flip(h->user_coord);
+        }
+        return true;
+    }
+    static Coord<T> flip(const Coord<T> &f) { return Coord<T>(-f.x, -f.y, -f.z); }
+
+    bool closest_one_sided(const Ray<T> &r, GeometryHit<T> *h, T eps, Counters *c) const {
         switch (kind) {
         case AGX_GEOM_QUAD: {                                   // quad.cpp:30-58
             T t, u, v;
@@ -904,8 +918,23 @@ template<class T> struct Geometry {
         }
     }
 
-    // Geometry::sample(ref, pdf, sam) per kind; pdf w.r.t. area
+    // DoubleSidedGeometry::sample(ref, pdf, sam) (double_sided.cpp:66-87): the
+    // first number picks the side and is rescaled for the wrapped geometry;
+    // the pdf halves and a back-side sample turns its frames around
     SurfacePoint<T> sample(const V3<T> &ref, T *pdf, T su, T sv, T sw, T eps) const {
+        if (!double_sided) return sample_one_sided(ref, pdf, su, sv, sw, eps);
+        const bool back = su < T(0.5);
+        SurfacePoint<T> sp = sample_one_sided(ref, pdf, back ? 2 * su : 2 * (su - T(0.5)), sv, sw, eps);
+        *pdf = *pdf * T(0.5);
+        if (back) {
+            sp.geometry_coord = flip(sp.geometry_coord);
+            sp.user_coord = flip(sp.user_coord);
+        }
+        return sp;
+    }
+
+    // Geometry::sample(ref, pdf, sam) per kind; pdf w.r.t. area
+    SurfacePoint<T> sample_one_sided(const V3<T> &ref, T *pdf, T su, T sv, T sw, T eps) const {
         SurfacePoint<T> sp;
         switch (kind) {
         case AGX_GEOM_QUAD: {                                   // quad.cpp:80-105
@@ -968,8 +997,13 @@ template<class T> struct Geometry {
         }
     }
 
-    // Geometry::pdf(ref, pos), area measure
+    // Geometry::pdf(ref, pos), area measure (halved for a double-sided geometry,
+    // double_sided.cpp:89-99)
     T pdf(const V3<T> &ref) const {
+        const T p = pdf_one_sided(ref);
+        return double_sided ? p * T(0.5) : p;
+    }
+    T pdf_one_sided(const V3<T> &ref) const {
         if (kind == AGX_GEOM_SPHERE) {                          // sphere.cpp:131-143
             const V3<T> lref = w2l.point(ref);
             const T d = lref.length();

@@ -71,3 +71,33 @@ def test_descriptor_validation():
     d.abi_version = 999
     with pytest.raises(capi.AgxError):
         capi.debug_scene_summary(d)
+
+
+def test_ctypes_structures_match_the_c_headers(tmp_path):
+    """Every ctypes mirror in atrc_amd/capi.py has the size and field offsets
+    that the C compiler gives the structure in include/*.h."""
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("no C compiler")
+    structs = [getattr(capi, n) for n in dir(capi)
+               if isinstance(getattr(capi, n), type) and issubclass(getattr(capi, n), C.Structure)
+               and getattr(capi, n) is not C.Structure]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "atrc_gpu.h"', '#include "atrc_gpu_debug.h"',
+             '#include "atrc_host.h"', 'int main(void) {']
+    for st in structs:
+        lines.append(f'printf("{st.__name__} size %zu\\n", sizeof({st.__name__}));')
+        for name, _ in st._fields_:
+            lines.append(f'printf("{st.__name__} {name} %zu\\n", offsetof({st.__name__}, {name}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{ROOT / 'include'}", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for st in structs:
+        assert got[(st.__name__, "size")] == C.sizeof(st), st.__name__
+        for name, _ in st._fields_:
+            assert got[(st.__name__, name)] == getattr(st, name).offset, (st.__name__, name)

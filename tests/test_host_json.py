@@ -307,3 +307,42 @@ def test_ibl_checker_and_transform2():
     assert t.kind == capi.TEX2D_CHECKER and t.grid_count == 4
     # inv_u (u -> 1 - u) applied after the user transform: translate(0.25) * scale(2): u -> 2u + 0.25
     np.testing.assert_allclose(list(t.transform), [-2, 0, 0.75, 0, 2, 0], atol=1e-6)
+
+
+def test_double_sided_and_transform_wrapper(tmp_path):
+    """double_sided (double_sided.cpp) sets the entity's double_sided flag around
+    any geometry; transform_wrapper (transform_wrapper.cpp) is composed into
+    the wrapped geometry: quad corners and mesh vertices are moved, a sphere's
+    local-to-world is premultiplied."""
+    tr = [{"type": "translate", "offset": [0, 0, 2]}, {"type": "rotate_z", "deg": 90}]
+    quad = {"type": "quad", "A": [1, 0, 0], "B": [1, 1, 0], "C": [0, 1, 0], "D": [0, 0, 0]}
+    sphere = {"type": "sphere", "radius": 0.5, "transform": [{"type": "translate", "offset": [3, 0, 0]}]}
+    tri = np.array([[0, 0, 0, 0, 0, 1, 0, 0], [1, 0, 0, 0, 0, 1, 0, 0], [0, 1, 0, 0, 0, 1, 0, 0]], np.float32)
+    scenes.write_bin_mesh(tmp_path / "t.bm", tri)
+    mesh = {"type": "triangle_bvh", "filename": "${scene-directory}/t.bm"}
+    mat = {"type": "ideal_diffuse", "albedo": {"type": "constant", "texel": [0.5]}}
+    ents = [
+        {"type": "geometric", "geometry": {"type": "double_sided", "internal": quad}, "material": mat},
+        {"type": "geometric", "geometry": {"type": "transform_wrapper", "internal": quad, "transform": tr},
+         "material": mat},
+        {"type": "geometric", "geometry": {"type": "transform_wrapper", "transform": tr,
+                                           "internal": {"type": "double_sided", "internal": sphere}},
+         "material": mat},
+        {"type": "geometric", "geometry": {"type": "transform_wrapper", "internal": mesh, "transform": tr},
+         "material": mat},
+    ]
+    h = capi.HostScene(wrap(ents), str(tmp_path))
+    d = h.desc()
+    assert [entity(d, i).double_sided for i in range(4)] == [1, 0, 1, 0]
+    rot = lambda p: (-p[1], p[0], p[2] + 2)       # rotate_z(90) first, then translate (0, 0, 2)
+    np.testing.assert_allclose(list(entity(d, 1).p[0]), rot((1, 0, 0)), atol=1e-6)
+    np.testing.assert_allclose(list(entity(d, 1).p[2]), rot((0, 1, 0)), atol=1e-6)
+    l2w = np.array(list(entity(d, 2).local_to_world)).reshape(3, 4)
+    np.testing.assert_allclose(l2w @ [0, 0, 0, 1], rot((3, 0, 0)), atol=1e-6)
+    np.testing.assert_allclose(l2w[:, :3], [[0, -1, 0], [1, 0, 0], [0, 0, 1]], atol=1e-6)
+    m = entity(d, 3)
+    v = np.array([[m.tris[k].pos[j] for j in range(3)] for k in range(3)])
+    np.testing.assert_allclose(v, [rot(p) for p in tri[:, :3]], atol=1e-6)
+    with pytest.raises(RuntimeError, match="double_sided"):
+        capi.HostScene(wrap([{"type": "geometric", "material": mat, "geometry": {
+            "type": "double_sided", "internal": {"type": "double_sided", "internal": quad}}}]), ".")

@@ -90,3 +90,28 @@ def test_white_furnace_without_mis():
     yy, xx = np.mgrid[0:32, 0:32]
     inner = (xx - 15.5) ** 2 + (yy - 15.5) ** 2 < 5 ** 2
     np.testing.assert_allclose(img[inner], rho, rtol=1e-5)
+
+
+def test_double_sided_disk_light_from_its_back():
+    """double_sided (double_sided.cpp): the shell-disk closed form holds for a
+    disk that faces away from the receivers when it is double-sided (its back
+    side emits; side choice by the first number, pdf halved), and the same
+    disk one-sided lights nothing."""
+    s, cam, tr, expected = scenes.light_in_shell("disk", double_sided=True, face_up=True)
+    desc, keep = s.to_desc()
+    r = po.OracleScene(desc, 64).render(scenes.render_params(48, 48, 256, cam, seed=25, **tr), threads=8)
+    _check(r["image"], expected)
+    s, cam, tr, _ = scenes.light_in_shell("disk", double_sided=False, face_up=True)
+    desc, keep = s.to_desc()
+    r = po.OracleScene(desc, 64).render(scenes.render_params(24, 24, 16, cam, seed=25, **tr), threads=8)
+    assert r["image"].max() == 0.0
+
+
+@pytest.mark.parametrize("kind", ["quads", "mesh"])
+def test_double_sided_furnace_from_the_back(kind):
+    """Walls facing outwards but double-sided shade and emit on their back
+    sides: the surface furnace reads the same closed form."""
+    s, cam, tr, expected = scenes.emitting_enclosure(kind, outward_double_sided=True)
+    desc, keep = s.to_desc()
+    r = po.OracleScene(desc, 64).render(scenes.render_params(40, 40, 128, cam, seed=26, **tr), threads=8)
+    _check(r["image"], expected)
