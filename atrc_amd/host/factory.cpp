@@ -83,6 +83,67 @@ std::string PathMapper::map(const std::string &s) const {  // BasicPathMapper::m
 }
 
 // ---------------------------------------------------------------- meshes
+// STL triangle soup (binary: 80-byte header, uint32 count, then per triangle
+// a facet normal, three vertices and a uint16 attribute; ASCII: "solid" ...
+// "facet normal" / "vertex" lines).  Each corner takes the facet normal, or
+// (B - A) x (C - A) when the file's normal is zero; uv = 0.  (Atrc reads STL
+// through agz-utils' mesh loader, which is not in the source tree: its choice
+// of corner normals is not pinned.)
+std::vector<agx_vertex> load_stl(const std::string &filename) {
+    std::ifstream in(filename, std::ios::binary);
+    if (!in) throw ObjectConstructionException("failed to open mesh file: " + filename);
+    std::string bytes((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    std::vector<agx_vertex> out;
+    auto emit = [&out](const float n[3], const float v[9]) {
+        float nn[3] = {n[0], n[1], n[2]};
+        if (nn[0] == 0 && nn[1] == 0 && nn[2] == 0) {
+            const float ux = v[3] - v[0], uy = v[4] - v[1], uz = v[5] - v[2];
+            const float vx = v[6] - v[0], vy = v[7] - v[1], vz = v[8] - v[2];
+            nn[0] = uy * vz - uz * vy; nn[1] = uz * vx - ux * vz; nn[2] = ux * vy - uy * vx;
+        }
+        for (int k = 0; k < 3; ++k) {
+            agx_vertex a{};
+            for (int j = 0; j < 3; ++j) { a.pos[j] = v[3 * k + j]; a.nor[j] = nn[j]; }
+            out.push_back(a);
+        }
+    };
+    // binary when the size matches the triangle count (ASCII files may also start with "solid")
+    if (bytes.size() >= 84) {
+        uint32_t n = 0;
+        std::memcpy(&n, bytes.data() + 80, 4);
+        if (bytes.size() == 84 + size_t(n) * 50) {
+            for (uint32_t i = 0; i < n; ++i) {
+                float f[12];
+                std::memcpy(f, bytes.data() + 84 + size_t(i) * 50, sizeof(f));
+                emit(f, f + 3);
+            }
+            if (out.empty()) throw ObjectConstructionException("empty mesh: " + filename);
+            return out;
+        }
+    }
+    std::istringstream ss(bytes);
+    std::string tok;
+    float n[3] = {0, 0, 0}, v[9];
+    int nv = 0;
+    while (ss >> tok) {
+        if (tok == "facet") {
+            std::string normal;
+            ss >> normal >> n[0] >> n[1] >> n[2];
+            nv = 0;
+        } else if (tok == "vertex") {
+            if (nv >= 3) throw ObjectConstructionException("invalid STL facet in " + filename);
+            ss >> v[3 * nv] >> v[3 * nv + 1] >> v[3 * nv + 2];
+            ++nv;
+        } else if (tok == "endfacet") {
+            if (nv != 3) throw ObjectConstructionException("invalid STL facet in " + filename);
+            emit(n, v);
+        }
+    }
+    if (!ss.eof()) throw ObjectConstructionException("invalid STL file: " + filename);
+    if (out.empty()) throw ObjectConstructionException("empty mesh: " + filename);
+    return out;
+}
+
 std::vector<agx_vertex> load_triangle_mesh(const std::string &filename) {
     std::vector<agx_vertex> out;
     if (ends_with(filename, ".bm")) {  // bin_mesh.cpp:7-27
@@ -96,6 +157,7 @@ std::vector<agx_vertex> load_triangle_mesh(const std::string &filename) {
         if (!in) throw ObjectConstructionException("truncated binary mesh: " + filename);
         return out;
     }
+    if (ends_with(filename, ".stl")) return load_stl(filename);
     if (!ends_with(filename, ".obj")) throw ObjectConstructionException("unsupported mesh file format: " + filename);
     std::ifstream in(filename);
     if (!in) throw ObjectConstructionException("failed to open mesh file: " + filename);

@@ -346,3 +346,42 @@ def test_double_sided_and_transform_wrapper(tmp_path):
     with pytest.raises(RuntimeError, match="double_sided"):
         capi.HostScene(wrap([{"type": "geometric", "material": mat, "geometry": {
             "type": "double_sided", "internal": {"type": "double_sided", "internal": quad}}}]), ".")
+
+
+def test_stl_meshes_binary_and_ascii(tmp_path):
+    """triangle_bvh from STL files (§ mesh formats): binary and ASCII STL give the
+    same world-space triangles, each corner carrying the facet normal (or the
+    cross product when the file's normal is zero)."""
+    import struct
+    rng = np.random.default_rng(3)
+    v = rng.uniform(-1, 1, (4, 3, 3)).astype(np.float32)
+    normals = np.zeros((4, 3), np.float32)
+    normals[0] = (0, 0, 1)                              # a stated normal is kept as it is
+    with open(tmp_path / "m.stl", "wb") as f:
+        f.write(b"solid binary header".ljust(80, b" "))
+        f.write(struct.pack("<I", len(v)))
+        for n, t in zip(normals, v):
+            f.write(struct.pack("<12fH", *n, *t.reshape(-1), 0))
+    lines = ["solid m"]
+    for n, t in zip(normals, v):
+        lines.append(f"facet normal {n[0]:.9g} {n[1]:.9g} {n[2]:.9g}")
+        lines.append(" outer loop")
+        lines += [f"  vertex {p[0]:.9g} {p[1]:.9g} {p[2]:.9g}" for p in t]
+        lines += [" endloop", "endfacet"]
+    lines.append("endsolid m")
+    (tmp_path / "a.stl").write_text("\n".join(lines) + "\n")
+    mat = {"type": "ideal_diffuse", "albedo": {"type": "constant", "texel": [0.5]}}
+    ents = [{"type": "geometric", "material": mat,
+             "geometry": {"type": "triangle_bvh", "filename": "${scene-directory}/" + f}} for f in ("m.stl", "a.stl")]
+    h = capi.HostScene(wrap(ents), str(tmp_path))   # keeps the descriptor's memory alive
+    d = h.desc()
+    for i in range(2):
+        e = entity(d, i)
+        assert e.tri_count == 4
+        pos = np.array([[e.tris[k].pos[j] for j in range(3)] for k in range(12)], np.float32).reshape(4, 3, 3)
+        nor = np.array([[e.tris[k].nor[j] for j in range(3)] for k in range(12)], np.float32).reshape(4, 3, 3)
+        np.testing.assert_array_equal(pos, v)
+        np.testing.assert_array_equal(nor[0], np.tile([0, 0, 1], (3, 1)))
+        for t in range(1, 4):
+            c = np.cross(v[t, 1] - v[t, 0], v[t, 2] - v[t, 0])
+            np.testing.assert_allclose(nor[t], np.tile(c, (3, 1)), rtol=1e-5, atol=1e-6)
