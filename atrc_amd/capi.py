@@ -140,6 +140,7 @@ class agx_debug_summary(C.Structure):
         ("device_bytes", C.c_uint64), ("world_centre_x", C.c_float), ("world_centre_y", C.c_float),
         ("world_centre_z", C.c_float), ("world_radius", C.c_float), ("bvh4_nodes", C.c_uint64),
         ("bvh4_leaves", C.c_uint64), ("bvh4_depth", C.c_uint32), ("pad_", C.c_uint32), ("bvh4_sah", C.c_double),
+        ("grid_texels_f32", C.c_uint64), ("grid_texels_f16", C.c_uint64),
     ]
 
 

@@ -99,6 +99,7 @@ struct agx_ctx {
     uint32_t g_batch_cap = 0;
     uint64_t g_steps = 0;
     size_t g_wave_cap = 0;
+    size_t g_ms_cap = 0;
     std::vector<void *> guide_allocs, guide_wave_allocs;
     hipEvent_t ev[8]{};
     std::vector<hipEvent_t> pool;    // timing / readback events, reused across renders
@@ -129,7 +130,9 @@ struct agx_ctx {
         guide_wave_allocs.clear();
         pb.g_feat = pb.g_out = pb.g_rec = nullptr;
         pb.g_rec_state = nullptr;
+        pb.g_ms = nullptr;
         g_wave_cap = 0;
+        g_ms_cap = 0;
     }
     void free_wave() {
         for (void *p : wave_allocs) if (p) (void)hipFree(p);
@@ -162,6 +165,13 @@ struct agx_ctx {
         HIP_TRY(galloc(guide_wave_allocs, &pb.g_rec, nr * GUIDE_MAX_REC * GUIDE_REC_FLOATS));
         HIP_TRY(galloc(guide_wave_allocs, &pb.g_rec_state, nr));
         g_wave_cap = n;
+        return AGX_OK;
+    }
+    // the medium samples of guided renders in media (k_medium_pass)
+    int ensure_guide_media(size_t n) {
+        if (n <= g_ms_cap) return AGX_OK;
+        HIP_TRY(galloc(guide_wave_allocs, &pb.g_ms, 3 * n));
+        g_ms_cap = n;
         return AGX_OK;
     }
     // network state; parameters initialised uniform in +-1/sqrt(fan-in) (weights)
@@ -473,6 +483,8 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     if (e == hipSuccess) { e = upload(&ai, hs.alias_idx, c->stream); track(ai); }
     if (e == hipSuccess) { e = upload(&md, hs.media, c->stream); track(md); }
     if (e == hipSuccess) { e = upload(&gr, hs.grids, c->stream); track(gr); }
+    uint16_t *gr16 = nullptr;
+    if (e == hipSuccess) { e = upload(&gr16, hs.grids16, c->stream); track(gr16); }
     if (e == hipSuccess) { e = upload(&b4n, hs.b4_nodes, c->stream); track(b4n); }
     if (e == hipSuccess) { e = upload(&b4p, hs.b4_prims, c->stream); track(b4p); }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -492,6 +504,7 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     ds.alias_idx = ai;
     ds.media = md;
     ds.grids = gr;
+    ds.grids16 = gr16;
     ds.accel = hs.accel;
     ds.b4_nodes = b4n;
     ds.b4_prims = b4p;
@@ -583,6 +596,9 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     if (r != AGX_OK) return r;
     if (guide) {
         if ((r = c->ensure_guide_wave(size_t(per_pass))) != AGX_OK) return r;
+        // media: scattering points get their own network evaluation (k_medium_pass)
+        rc.guide_media = (c->ds.features & SF_MEDIA) ? 1 : 0;
+        if (rc.guide_media && (r = c->ensure_guide_media(size_t(per_pass))) != AGX_OK) return r;
         if (!c->guide_ready || p->guide_reset) {
             if ((r = c->guide_init(p->guide_seed, guide_batch)) != AGX_OK) return r;
         } else if ((r = c->ensure_guide_batch(guide_batch)) != AGX_OK) {
@@ -660,7 +676,9 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
                                     c->capture_cap, n_shadow_max, c->stream);
             }
             if (guide) {
-                // the network's mixture for every vertex about to be shaded
+                // the network's mixture for every vertex about to be shaded (in media,
+                // at the scattering point the vertex's medium sample reaches)
+                if (rc.guide_media) launch_medium_pass(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
                 launch_guide_features(c->ds, rc, c->pb, uint32_t(rc.n_paths), c->stream);
                 size_t ef, eg;
                 if ((r = mark(&ef))) return r;
@@ -1377,6 +1395,9 @@ int agx_debug_scene_summary(const agx_scene_desc *desc, agx_debug_summary *out) 
     out->bvh4_depth = uint32_t(hs.b4_depth);
     out->bvh4_sah = hs.b4_sah;
     out->device_bytes += (hs.b4_nodes.size() + hs.b4_prims.size()) * sizeof(vec4);
+    out->grid_texels_f32 = hs.grids.size();
+    out->grid_texels_f16 = hs.grids16.size();
+    out->device_bytes += hs.grids.size() * 4 + hs.grids16.size() * 2;
     return AGX_OK;
 }
 

@@ -18,6 +18,29 @@
 
 namespace agx {
 
+// IEEE binary16 bits of f when f converts to FP16 without rounding (zeros,
+// normal and subnormal FP16 values), else false.
+bool half_exact(float f, uint16_t *out) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    const uint32_t sign = (b >> 16) & 0x8000u, ex = (b >> 23) & 0xffu, man = b & 0x7fffffu;
+    if (ex == 0 && man == 0) { *out = uint16_t(sign); return true; }
+    if (ex == 255 || ex == 0) return false;  // inf / NaN, FP32 subnormals (far below FP16's range)
+    const int e = int(ex) - 127;
+    if (e > 15) return false;
+    if (e >= -14) {
+        if (man & 0x1fffu) return false;
+        *out = uint16_t(sign | (uint32_t(e + 15) << 10) | (man >> 13));
+        return true;
+    }
+    if (e < -24) return false;
+    const uint32_t full = 0x800000u | man;
+    const int shift = -1 - e;  // FP16 subnormal: value = m 2^-24
+    if (full & ((1u << shift) - 1u)) return false;
+    *out = uint16_t(sign | (full >> shift));
+    return true;
+}
+
 namespace {
 
 struct F3 {
@@ -518,9 +541,20 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
                     (t.channels != 1 && t.channels != 3))
                     return false;
                 o->w = t.width; o->h = t.height; o->d = t.depth; o->ch = t.channels;
-                o->offset = uint32_t(S.grids.size());
                 const size_t n = size_t(t.width) * size_t(t.height) * size_t(t.depth) * size_t(t.channels);
-                S.grids.insert(S.grids.end(), t.data, t.data + n);
+                // a grid whose every texel FP16 holds exactly is stored in half the
+                // bytes; lookups widen it back to the same FP32 values
+                std::vector<uint16_t> h(n);
+                bool exact = true;
+                for (size_t k = 0; k < n && exact; ++k) exact = half_exact(t.data[k], &h[k]);
+                o->f16 = exact ? 1 : 0;
+                if (exact) {
+                    o->offset = uint32_t(S.grids16.size());
+                    S.grids16.insert(S.grids16.end(), h.begin(), h.end());
+                } else {
+                    o->offset = uint32_t(S.grids.size());
+                    S.grids.insert(S.grids.end(), t.data, t.data + n);
+                }
                 if (max_real) {
                     float mx = -std::numeric_limits<float>::max();
                     for (size_t k = 0; k < n; k += size_t(t.channels)) mx = std::max(mx, t.data[k]);
@@ -541,7 +575,10 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         }
         S.media.push_back(dm);
     }
-    if (S.grids.size() > size_t(0xffffffffu)) { *err = "3d grids too large"; return AGX_E_LIMIT; }
+    if (S.grids.size() > size_t(0xffffffffu) || S.grids16.size() > size_t(0xffffffffu)) {
+        *err = "3d grids too large";
+        return AGX_E_LIMIT;
+    }
 
     std::vector<Box> bounds(desc.entity_count);
     for (uint32_t i = 0; i < desc.entity_count; ++i) {

@@ -18,6 +18,7 @@ struct HostScene {
     std::vector<int32_t> lights;
     std::vector<DevMedium> media;
     std::vector<float> grids;
+    std::vector<uint16_t> grids16;
     int32_t accel = AGX_ACCEL_ATRC;
     std::vector<vec4> b4_nodes, b4_prims;  // AGX_ACCEL_BVH4
     int b4_depth = 0;

@@ -86,10 +86,10 @@ static_assert(sizeof(DevEntity) == 320, "DevEntity layout");
 struct DevTex3D {
     int32_t kind, linear, w, h;
     int32_t d, ch;
-    uint32_t offset;       // first float of the grid in DevScene::grids
+    uint32_t offset;       // first texel of the grid in DevScene::grids (or grids16)
     float inv_gamma;
     int32_t wrap[3];
-    int32_t pad;
+    int32_t f16;           // 1: texels stored as FP16 in DevScene::grids16 (every texel exactly representable)
     vec4 value;
     vec4 xf[3];            // rows of the uvw transform
 };
@@ -143,6 +143,7 @@ struct DevScene {
     const int32_t *alias_idx;
     const DevMedium *media;
     const float *grids;
+    const uint16_t *grids16;  // FP16 texels of the grids whose values FP16 holds exactly
     int32_t n_ent;
     int32_t aggregate;     // AGX_AGGREGATE_*
     int32_t n_lights;

@@ -41,6 +41,8 @@ struct PathBuffers {
     float *g_out;        // [N*40] decoded mixture of those vertices (8 x (mu.xyz, kappa), 8 weights)
     float *g_rec;        // [guide_rec_paths(N) * GUIDE_MAX_REC * GUIDE_REC_FLOATS] training records
     int2 *g_rec_state;   // [guide_rec_paths(N)] (records made, record of the previous vertex or -1)
+    float4 *g_ms;        // [3N] guided renders in media: the medium sample of the vertex about to be
+                         // shaded, taken before inference: (pos.xyz, g), (throughput.xyz, happened), (wr.xyz, 0)
 };
 
 // Path guiding: one path in 2^guide_rec_shift keeps training records of its
@@ -71,6 +73,7 @@ enum : uint32_t {
     PF_MIS_DELTA = 16u,  // the MIS-BSDF sample was a delta lobe
     PF_FRESH = 32u,      // camera path not shaded yet: radiance 0, depth (1, 1, 0, 0) and the
                          // depth-1 throughput are implied (k_raygen does not store them)
+    PF_MS = 64u,         // k_medium_pass sampled this vertex's medium segment into g_ms
     PF_MIS_MED_SHIFT = 16u,  // bits 16..31: 1 + medium index the MIS-BSDF ray travels through (0 = void)
 };
 // key of the transmittance stream (initstate = splitmix(seed ^ key ^ splitmix(seq)))
@@ -88,6 +91,7 @@ struct RenderConst {
     int32_t sample_begin;
     int32_t L;           // number of lights
     int32_t guide;       // 1: guided continuation sampling
+    int32_t guide_media; // 1: guided render of a scene with media (k_medium_pass runs before inference)
     float guide_alpha;   // probability of drawing from the learned density
     int32_t guide_record;  // 1: guided vertices of the recording paths keep training records
     uint32_t guide_rec_shift;  // recording paths: (p + guide_rec_rot) % 2^shift == 0, record slot p >> shift
@@ -126,6 +130,11 @@ void launch_queue_compact(const PathBuffers &pb, uint32_t max_n, int L, uint4 *b
 // network inputs of the vertices about to be shaded (q_in order)
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
                            hipStream_t st);
+// guided renders in media: samples the medium segment of every vertex about to
+// be shaded (so that a scattering point can get its own network evaluation);
+// k_shade then uses the stored sample
+void launch_medium_pass(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
+                        hipStream_t st);
 // training samples (inputs, direction, weight) from the records of finished paths;
 // appends at most cap samples, counts them in *count and sums their weights in *wsum
 void launch_guide_gather(const RenderConst &rc, const PathBuffers &pb, float *samples, uint32_t cap,

@@ -1886,7 +1886,16 @@ __device__ V env_sample(const DevScene &s, float su, float sv, float sw, float *
 }
 
 __device__ __forceinline__ V texel3(const DevScene &s, const DevTex3D &t, int x, int y, int z) {
-    const float *g = s.grids + t.offset + ((size_t(z) * size_t(t.h) + size_t(y)) * size_t(t.w) + size_t(x)) * size_t(t.ch);
+    const size_t i = t.offset + ((size_t(z) * size_t(t.h) + size_t(y)) * size_t(t.w) + size_t(x)) * size_t(t.ch);
+    if (t.f16) {  // FP16 texels widen exactly to the FP32 values they were built from
+        const _Float16 *g = reinterpret_cast<const _Float16 *>(s.grids16) + i;
+        if (t.ch == 1) {
+            const float v = float(g[0]);
+            return mk(v, v, v);
+        }
+        return mk(float(g[0]), float(g[1]), float(g[2]));
+    }
+    const float *g = s.grids + i;
     if (t.ch == 1) return mk(g[0], g[0], g[0]);
     return mk(g[0], g[1], g[2]);
 }
@@ -2104,6 +2113,26 @@ __device__ void guide_inputs(const DevScene &s, const Surf &sp, float *x) {
     x[6] = acosf(smin(1.0f, smax(-1.0f, w.z))) / PI_F;
     x[7] = (atan2f(w.y, w.x) + PI_F) / (2 * PI_F);
 }
+// A training record of a guided vertex of a recording path: its inputs, the
+// sampled direction, the mixture pdf, the throughput after the vertex and
+// lum(f) |cos| (or the phase value at a medium point) of the sampled direction.
+__device__ void guide_record(const PathBuffers &pb, const RenderConst &rc, uint32_t p, uint32_t tid, V dir, float pdf,
+                             V coef, float flum) {
+    int2 st = pb.g_rec_state[p >> rc.guide_rec_shift];
+    if (st.x >= GUIDE_MAX_REC) return;
+    float *rec = pb.g_rec + (size_t(p >> rc.guide_rec_shift) * GUIDE_MAX_REC + st.x) * GUIDE_REC_FLOATS;
+    const float *x = pb.g_feat + size_t(tid) * 8;
+    for (int k = 0; k < 8; ++k) rec[k] = x[k];
+    rec[8] = dir.x; rec[9] = dir.y; rec[10] = dir.z;
+    rec[11] = pdf;
+    rec[12] = coef.x; rec[13] = coef.y; rec[14] = coef.z;
+    rec[15] = rec[16] = rec[17] = 0;
+    rec[18] = flum;
+    st.y = st.x;
+    st.x += 1;
+    pb.g_rec_state[p >> rc.guide_rec_shift] = st;
+}
+
 // Surfaces whose continuation may come from the learned density.  The one-sample
 // mixture needs the BSDF's pdf to be the density of its sampler; Disney's
 // pdf differs from its sampler (the diffuse pick overwritten, |sin phi_o| in
@@ -2140,6 +2169,66 @@ __device__ V guide_sample(const float *G, float u0, float u1, float u2) {
 }
 __device__ __forceinline__ float lum3(V c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 
+// the increment of path p's streams: its (sampled pixel, sample), as k_raygen
+// keys it (the sampled grid includes the filter border when samples splat)
+__device__ __forceinline__ unsigned long long path_inc(const RenderConst &rc, uint32_t p) {
+    const int64_t wh = rc.splat ? int64_t(rc.EW) * rc.EH : int64_t(rc.W) * rc.H;
+    const int64_t w = rc.work_base + int64_t(p);
+    return (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
+}
+
+// Network inputs of a medium scattering point: position and direction as at a
+// surface, and the normal inputs at 0.5, the encoding of a zero normal, which
+// no surface vertex has.
+__device__ void guide_inputs_medium(const DevScene &s, V pos, V wr, float *x) {
+    const float inv = 0.5f / s.world.w;
+    x[0] = (pos.x - s.world.x) * inv + 0.5f;
+    x[1] = (pos.y - s.world.y) * inv + 0.5f;
+    x[2] = (pos.z - s.world.z) * inv + 0.5f;
+    x[3] = x[4] = x[5] = 0.5f;
+    const V w = normalize(wr);
+    x[6] = acosf(smin(1.0f, smax(-1.0f, w.z))) / PI_F;
+    x[7] = (atan2f(w.y, w.x) + PI_F) / (2 * PI_F);
+}
+
+// Guided renders in media: the medium segment of each vertex about to be
+// shaded is sampled here, before inference, so that a scattering point gets a
+// network evaluation of its own.  The draws are the ones k_shade would make
+// first at this vertex (Medium::sample_scattering, path_tracing.cpp:64-76);
+// k_shade takes the stored sample instead of drawing again.
+__global__ void __launch_bounds__(256) k_medium_pass(DevScene s, RenderConst rc, PathBuffers pb) {
+    const uint32_t n = pb.cnt_in[2];
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= n) return;
+    const uint32_t p = pb.q_in[tid];
+    uint32_t flags = pb.flags[p] & ~PF_MS;
+    const float4 h4 = pb.hit[2 * p];
+    const int32_t ent = __float_as_int(h4.w);
+    if ((flags & PF_CONT) && ent >= 0) {
+        const int4 dep = (flags & PF_FRESH) ? make_int4(1, 1, 0, 0) : pb.depth[p];
+        HitRec h;
+        h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
+        const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
+        const Surf sp = surface_at(s, h, ro, rd, rc.eps);
+        const DevEntity &e = s.ents[ent];
+        const int32_t mi = medium_side(e, sp.wr, sp.geo.z);
+        if (mi >= 0 && dep.z < s.media[mi].max_scat) {
+            Rng rng, rtr;
+            rng.state = pb.rng[p];
+            rtr.state = pb.rng_tr[p];
+            rng.inc = rtr.inc = path_inc(rc, p);
+            const MScat ms = medium_sample(s, s.media[mi], ro, sp.pos, rng, rtr, dep.z > 0, rc.eps);
+            pb.g_ms[3 * size_t(p)] = f4(ms.pos, ms.g);
+            pb.g_ms[3 * size_t(p) + 1] = f4(ms.thr, ms.happened ? 1.0f : 0.0f);
+            pb.g_ms[3 * size_t(p) + 2] = f4(ms.wr, 0.0f);
+            pb.rng[p] = rng.state;
+            pb.rng_tr[p] = rtr.state;
+            flags |= PF_MS;
+        }
+    }
+    pb.flags[p] = flags;
+}
+
 __global__ void __launch_bounds__(256) k_guide_features(DevScene s, RenderConst rc, PathBuffers pb) {
     const uint32_t n = pb.cnt_in[2];
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2148,7 +2237,10 @@ __global__ void __launch_bounds__(256) k_guide_features(DevScene s, RenderConst 
     float x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const float4 h4 = pb.hit[2 * p];
     const int32_t ent = __float_as_int(h4.w);
-    if ((pb.flags[p] & PF_CONT) && ent >= 0) {
+    const uint32_t flags = pb.flags[p];
+    if ((flags & PF_MS) && pb.g_ms[3 * size_t(p) + 1].w != 0.0f) {
+        guide_inputs_medium(s, mk(pb.g_ms[3 * size_t(p)]), mk(pb.g_ms[3 * size_t(p) + 2]), x);
+    } else if ((flags & PF_CONT) && ent >= 0) {
         HitRec h;
         h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
         const Surf sp = surface_at(s, h, mk(pb.ray_o[2 * p]), mk(pb.ray_d[2 * p]), rc.eps);
@@ -2178,14 +2270,10 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     rtr.state = media ? pb.rng_tr[p] : 0ull;
     Rng rng;
     rng.state = pb.rng[p];
-    {
-        // the stream of (sampled pixel, sample), as k_raygen keys it (the sampled
-        // grid includes the filter border when samples splat)
-        const int64_t wh = rc.splat ? int64_t(rc.EW) * rc.EH : int64_t(rc.W) * rc.H;
-        const int64_t w = rc.work_base + int64_t(p);
-        rng.inc = (path_seq((unsigned long long)(w % wh), (unsigned long long)(rc.sample_begin + w / wh)) << 1u) | 1u;
-        rtr.inc = rng.inc;
-    }
+    rng.inc = rtr.inc = path_inc(rc, p);
+    // the medium sample k_medium_pass took for this vertex (guided renders in media)
+    const bool ms_done = (F & SF_GUIDE) && (F & SF_MEDIA) && (flags & PF_MS);
+    flags &= ~PF_MS;
     // 1. direct illumination of the previous vertex (path_tracing.cpp:85-97, 134-146)
     if (flags & PF_PENDING) {
         V direct = mk(0, 0, 0);
@@ -2263,7 +2351,13 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     if ((F & SF_MEDIA) && mi >= 0) {
         const DevMedium &m = s.media[mi];
         if (dep.z < m.max_scat) {
-            const MScat ms = medium_sample(s, m, ro, sp.pos, rng, rtr, dep.z > 0, eps);
+            MScat ms;
+            if (ms_done) {
+                const float4 a = pb.g_ms[3 * size_t(p)], b4 = pb.g_ms[3 * size_t(p) + 1], c4 = pb.g_ms[3 * size_t(p) + 2];
+                ms.pos = mk(a); ms.g = a.w; ms.thr = mk(b4); ms.happened = b4.w != 0.0f; ms.wr = mk(c4);
+            } else {
+                ms = medium_sample(s, m, ro, sp.pos, rng, rtr, dep.z > 0, eps);
+            }
             coef = coef * ms.thr;
             if (ms.happened) {
                 scattered = true;
@@ -2309,7 +2403,39 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         push_mis = true;
                     }
                 }
-                {  // phase-function continuation (path_tracing.cpp:99-108; trace_nomis :285-296
+                if ((F & SF_GUIDE) && rc.guide) {
+                    // guided: one-sample mixture of the learned density (probability
+                    // alpha) and the phase function, weighted by the mixture density
+                    const float usel = rng.uniform();
+                    const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
+                    const float *G = pb.g_out + size_t(tid) * GUIDE_ROW;
+                    const float a = rc.guide_alpha;
+                    V dir;
+                    float fp;
+                    if (usel < a) {
+                        dir = guide_sample(G, u0, u1, u2);
+                        fp = hg_eval(ms.g, dir, ms.wr);
+                    } else {
+                        hg_sample(ms.g, ms.wr, u0, u1, &dir, &fp);
+                        dir = normalize(dir);
+                    }
+                    const float pdf = a * guide_pdf(G, dir) + (1 - a) * fp;  // the phase function's pdf is its value
+                    if (fp != 0.0f && !(pdf < eps) && isfinite(pdf)) {
+                        pb.ray_o[2 * p] = f4(ms.pos, 0.0f);
+                        pb.ray_d[2 * p] = f4(dir, INF_F);
+                        coef = coef * (fp / pdf);
+                        dep.x += 1;
+                        if (dep.x <= rc.max_depth) {
+                            cont = true;
+                            if (dep.x > rc.min_depth) {
+                                if (rng.uniform() > rc.cont_prob) cont = false;
+                                else coef = coef / rc.cont_prob;
+                            }
+                        }
+                        if (cont && rc.guide_record && guide_recording(rc, p))
+                            guide_record(pb, rc, p, tid, dir, pdf, coef, fp);
+                    }
+                } else {  // phase-function continuation (path_tracing.cpp:99-108; trace_nomis :285-296
                    // samples around the surface hit's wr, tests only f and keeps the direction as sampled)
                     const float u0 = rng.uniform(), u1 = rng.uniform();
                     (void)rng.uniform();
@@ -2449,22 +2575,8 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         else coef = coef / rc.cont_prob;
                     }
                 }
-                if (cont && rc.guide_record && guide_recording(rc, p)) {
-                    int2 st = pb.g_rec_state[p >> rc.guide_rec_shift];
-                    if (st.x < GUIDE_MAX_REC) {
-                        float *rec = pb.g_rec + (size_t(p >> rc.guide_rec_shift) * GUIDE_MAX_REC + st.x) * GUIDE_REC_FLOATS;
-                        const float *x = pb.g_feat + size_t(tid) * 8;
-                        for (int k = 0; k < 8; ++k) rec[k] = x[k];
-                        rec[8] = dir.x; rec[9] = dir.y; rec[10] = dir.z;
-                        rec[11] = pdf;
-                        rec[12] = coef.x; rec[13] = coef.y; rec[14] = coef.z;
-                        rec[15] = rec[16] = rec[17] = 0;
-                        rec[18] = lum3(f) * abscos;
-                        st.y = st.x;
-                        st.x += 1;
-                        pb.g_rec_state[p >> rc.guide_rec_shift] = st;
-                    }
-                }
+                if (cont && rc.guide_record && guide_recording(rc, p))
+                    guide_record(pb, rc, p, tid, dir, pdf, coef, lum3(f) * abscos);
             }
         } else {
             const float u0 = rng.uniform(), u1 = rng.uniform(), u2 = rng.uniform();
@@ -2963,11 +3075,14 @@ void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &p
     const bool lean = (s.features & (SF_MEDIA | SF_DISNEY | SF_DELTA | SF_IBL)) == 0;
     // media and delta surfaces without Disney, IBL or guiding (the volumetric
     // scenes): 199 VGPRs and no scratch against the full variant's spills
-    const bool media_delta = (s.features & (SF_DISNEY | SF_IBL)) == 0 && !rc.guide;
+    const bool media_delta = (s.features & (SF_DISNEY | SF_IBL)) == 0;
     if (lean && !rc.guide) hipLaunchKernelGGL(k_shade<0u>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
     else if (lean) hipLaunchKernelGGL(k_shade<SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
-    else if (media_delta)
+    else if (media_delta && !rc.guide)
         hipLaunchKernelGGL(k_shade<SF_MEDIA | SF_DELTA>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
+    else if (media_delta)  // guided volumetric scenes
+        hipLaunchKernelGGL(k_shade<SF_MEDIA | SF_DELTA | SF_GUIDE>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s,
+                           rc, pb);
     else hipLaunchKernelGGL(k_shade<SF_ALL>, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
 }
 
@@ -2979,6 +3094,12 @@ void launch_queue_compact(const PathBuffers &pb, uint32_t max_n, int L, uint4 *b
     if (nb) hipLaunchKernelGGL(k_queue_count, dim3(nb), dim3(QC_THREADS), 0, st, pb, nl, blk);
     hipLaunchKernelGGL(k_queue_scan, dim3(1), dim3(1024), 0, st, pb, blk);
     if (nb) hipLaunchKernelGGL(k_queue_scatter, dim3(nb), dim3(QC_THREADS), 0, st, pb, nl, blk);
+}
+
+void launch_medium_pass(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,
+                        hipStream_t st) {
+    if (!max_n) return;
+    hipLaunchKernelGGL(k_medium_pass, dim3(grid_of(max_n, 256)), dim3(256), 0, st, s, rc, pb);
 }
 
 void launch_guide_features(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n,

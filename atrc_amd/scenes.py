@@ -747,14 +747,16 @@ def batch_scene(seed: int) -> tuple[Scene, dict, dict, int]:
 def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int = 11) -> tuple[Scene, dict, dict]:
     """The door-gap room (guided_room) with its first room filled with
     participating media: a heterogeneous medium on a seeded grid_n^3 fBm density
-    (4 octaves, max 2.0, albedo 0.9, HG g = 0.3) over the whole room, and a
+    (4 octaves, max 2.0, FP16 values, albedo 0.9, HG g = 0.3) over the whole room, and a
     homogeneous slab (sigma_t 0.5, albedo 0.9) between z = 2.0 and 2.5 bounded
     by an invisible_surface box.  Every surface facing the first room has the
     fog on that side (entity med_out, or med_in for the slab's box)."""
     s, cam, tr = guided_room(n_clutter_tris=n_clutter_tris, seed=7)
+    # densities rounded to FP16 values: the library stores such a grid in FP16
+    density = (fbm_grid(grid_n, seed) * 2.0).astype(np.float16).astype(np.float32)
     fog = s.add_medium(Medium(kind=capi.MEDIUM_HETEROGENEOUS,
                               local_to_world=(10.0, 0, 0, 0, 0, 10.0, 0, 0, 0, 0, 4.0, 0),
-                              density=Texture3D(grid=fbm_grid(grid_n, seed) * 2.0),
+                              density=Texture3D(grid=density),
                               albedo=Texture3D(value=(0.9, 0.9, 0.9)),
                               g3=Texture3D(value=(0.3, 0.3, 0.3))))
     slab = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.05, 0.05, 0.05),

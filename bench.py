@@ -39,7 +39,9 @@ Further legs (each can be switched off):
 * batched: 64 seeded scenes (1e4..2e6 triangles, 256^2..1024^2,
   32 spp) in one hipGraph launch (agx_render_batch_graph), host builds first.
 * guided: equal-time relMSE of guided vs plain GPU PT on the
-  door-gap room (500k clutter triangles) at 1024x1024.  Plain and guided runs
+  door-gap room (500k clutter triangles) at 1024x1024; guided_volumetric
+  (configs[4]): the same comparison in the fog-filled room (FP16 density
+  grid, depth 16), where scattering points get their own network evaluation.  Plain and guided runs
   each take --guided-seconds of wall time in chunks of --guided-chunk spp; the
   guided run starts from a fresh network, trains on every pass for the first
   --guided-train-frac of its time, then renders with the network frozen, and
@@ -430,19 +432,26 @@ def leg_batched(args) -> dict:
     return out
 
 
-def leg_guided(ctx, args) -> dict:
-    """equal-time relMSE, guided vs plain, on the door-gap room."""
+def leg_guided(ctx, args, volumetric: bool = False) -> dict:
+    """configs[3] (door-gap room) or configs[4] (the room filled with media):
+    equal-time relMSE, guided vs plain."""
     import numpy as np
     from atrc_amd import capi, scenes
 
     t0 = time.time()
-    s, cam, tr = scenes.guided_room(n_clutter_tris=args.guided_tris, seed=7)
+    if volumetric:
+        s, cam, tr = scenes.volumetric_room(n_clutter_tris=args.guided_tris, grid_n=args.grid)
+        seconds, ref_factor = args.vol_guided_seconds, args.vol_guided_ref_factor
+    else:
+        s, cam, tr = scenes.guided_room(n_clutter_tris=args.guided_tris, seed=7)
+        seconds, ref_factor = args.guided_seconds, args.guided_ref_factor
     s.accel = capi.ACCEL_BVH4
     desc, keep = s.to_desc()
     ctx.upload_scene(desc)
     W = H = args.guided_res
-    log(f"[bench] guided leg: door-gap room, {s.triangle_count()} triangles, {W}x{H} "
-        f"(built + uploaded in {time.time() - t0:.1f} s)")
+    label = "guided volumetric leg" if volumetric else "guided leg"
+    log(f"[bench] {label}: {'fog-filled ' if volumetric else ''}door-gap room, {s.triangle_count()} triangles, "
+        f"{W}x{H} (built + uploaded in {time.time() - t0:.1f} s)")
     img = np.empty((H, W, 3), np.float32)
 
     def run(seconds: float, seed: int, guide: bool, train_s: float = 0.0):
@@ -466,27 +475,29 @@ def leg_guided(ctx, args) -> dict:
         return acc / max(spp, 1), spp, time.perf_counter() - t_start, st_sum
 
     ctx.render(scenes.render_params(W, H, 1, cam, seed=9, **tr), aovs=False, out=img)   # warm the kernels
-    half = args.guided_seconds * args.guided_ref_factor / 2
+    half = seconds * ref_factor / 2
     ref_a, spp_a, wall_a, _ = run(half, 1001, False)
     ref_b, spp_b, wall_b, _ = run(half, 1002, False)
     ref = (ref_a * spp_a + ref_b * spp_b) / (spp_a + spp_b)
     # the reference's own noise: var(ref) = E|a - b|^2 / 4 for equal halves
     ref_noise = relmse_scaled(ref_a, ref_b) / 4.0 * (4.0 * spp_a * spp_b / (spp_a + spp_b) ** 2)
-    log(f"[bench] guided leg: reference plain {spp_a + spp_b} spp in {wall_a + wall_b:.1f} s "
+    log(f"[bench] {label}: reference plain {spp_a + spp_b} spp in {wall_a + wall_b:.1f} s "
         f"(own relMSE about {ref_noise:.3e})")
-    plain, plain_spp, plain_wall, pst = run(args.guided_seconds, 2002, False)
-    guided, guided_spp, guided_wall, gst = run(args.guided_seconds, 3003, True,
-                                               train_s=args.guided_train_frac * args.guided_seconds)
+    plain, plain_spp, plain_wall, pst = run(seconds, 2002, False)
+    guided, guided_spp, guided_wall, gst = run(seconds, 3003, True, train_s=args.guided_train_frac * seconds)
     ep_raw, eg_raw = relmse_scaled(plain, ref), relmse_scaled(guided, ref)
     ep, eg = max(ep_raw - ref_noise, 1e-30), max(eg_raw - ref_noise, 1e-30)
-    log(f"[bench] guided leg: plain {plain_spp} spp / {plain_wall:.2f} s relMSE {ep:.4e}; guided {guided_spp} spp / "
+    log(f"[bench] {label}: plain {plain_spp} spp / {plain_wall:.2f} s relMSE {ep:.4e}; guided {guided_spp} spp / "
         f"{guided_wall:.2f} s relMSE {eg:.4e}; ratio {ep / eg:.2f} (uncorrected {ep_raw / eg_raw:.2f})")
     infer_tflops = gst["guide_rows"] * MLP_FORWARD_FLOP / max(gst["guide_infer_ms"], 1e-9) / 1e9
     train_tflops = gst["guide_trained"] * MLP_TRAIN_FLOP / max(gst["guide_train_ms"], 1e-9) / 1e9
     return {
         "workload": (f"door-gap room (10x10x4 Lambert room lit through a 0.05-wide gap, "
-                     f"{s.triangle_count()} triangles), {W}x{H}, max depth {tr['max_depth']}, BVH4"),
-        "seconds": args.guided_seconds, "chunk_spp": args.guided_chunk,
+                     f"{s.triangle_count()} triangles)"
+                     + (f" filled with a {args.grid}^3 fBm medium (FP16 grid, sigma_t max 2, albedo 0.9, g 0.3) "
+                        f"and a homogeneous slab (sigma_t 0.5); network inputs at medium vertices" if volumetric else "")
+                     + f", {W}x{H}, max depth {tr['max_depth']}, BVH4"),
+        "seconds": seconds, "chunk_spp": args.guided_chunk,
         "train_frac": args.guided_train_frac, "adam_steps_per_pass": args.guided_steps,
         "sample_ring": args.guided_batch,
         "plain": {"spp": plain_spp, "wall_s": round(plain_wall, 3), "relmse": ep, "relmse_uncorrected": ep_raw,
@@ -543,6 +554,9 @@ def main() -> int:
     ap.add_argument("--guided-train-frac", type=float, default=0.3)
     ap.add_argument("--guided-steps", type=int, default=8)
     ap.add_argument("--guided-batch", type=int, default=65536)
+    ap.add_argument("--no-guided-volumetric", action="store_true", help="skip the guided vs plain leg in media")
+    ap.add_argument("--vol-guided-seconds", type=float, default=6.0)
+    ap.add_argument("--vol-guided-ref-factor", type=float, default=6.0)
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the further legs (profiling); prints no headline line")
     args = ap.parse_args()
@@ -566,6 +580,8 @@ def main() -> int:
         legs["volumetric"] = leg_volumetric(ctx, args)
     if not args.no_guided:
         legs["guided"] = leg_guided(ctx, args)
+    if not args.no_guided_volumetric:
+        legs["guided_volumetric"] = leg_guided(ctx, args, volumetric=True)
     ctx.close()
     if not args.no_batched:
         legs["batched"] = leg_batched(args)

@@ -24,6 +24,9 @@ typedef struct agx_debug_summary {
     uint64_t bvh4_nodes, bvh4_leaves;
     uint32_t bvh4_depth, pad_;
     double bvh4_sah;
+    /* 3D texture texels stored as FP32, and as FP16 (grids whose every texel
+     * FP16 holds exactly) */
+    uint64_t grid_texels_f32, grid_texels_f16;
 } agx_debug_summary;
 
 /* Atrc triangle BVH of one mesh.  nodes_out (may be NULL): node_count * 8

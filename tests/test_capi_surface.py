@@ -101,3 +101,23 @@ def test_ctypes_structures_match_the_c_headers(tmp_path):
         assert got[(st.__name__, "size")] == C.sizeof(st), st.__name__
         for name, _ in st._fields_:
             assert got[(st.__name__, name)] == getattr(st, name).offset, (st.__name__, name)
+
+
+def test_fp16_grids_only_when_exact():
+    """A heterogeneous medium's grid is stored in FP16 exactly when FP16 holds
+    every texel (FP16 zeros, normals and subnormals); one texel off the FP16
+    grid keeps the whole grid FP32."""
+    rng = np.random.default_rng(5)
+    exact = rng.uniform(0, 2, (8, 8, 8)).astype(np.float16).astype(np.float32)
+    exact[0, 0, :4] = [0.0, -0.0, 2.0 ** -24, 3 * 2.0 ** -20]   # zero, signed zero, FP16 subnormals
+    off = exact.copy()
+    off[3, 4, 5] = np.float32(1.0) + np.float32(2.0 ** -20)      # needs more than FP16's 10 mantissa bits
+    tiny = exact.copy()
+    tiny[1, 1, 1] = np.float32(2.0 ** -25)                       # below FP16's smallest subnormal
+    for grid, f16 in ((exact, True), (off, False), (tiny, False)):
+        s, cam, tr = scenes.absorbing_slab()
+        s.add_medium(scenes.Medium(kind=capi.MEDIUM_HETEROGENEOUS, density=scenes.Texture3D(grid=grid)))
+        desc, keep = s.to_desc()
+        sm = capi.debug_scene_summary(desc)
+        # density grid only (albedo and g are constant textures)
+        assert (sm["grid_texels_f16"], sm["grid_texels_f32"]) == ((512, 0) if f16 else (0, 512))
