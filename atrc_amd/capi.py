@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 11
+AGX_ABI_VERSION = 12
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -112,7 +112,7 @@ class agx_render_params(C.Structure):
         ("collect_stats", C.c_int32), ("sample_begin", C.c_int32), ("guide", C.c_int32), ("guide_alpha", C.c_float),
         ("guide_train", C.c_int32), ("guide_batch", C.c_int32), ("guide_steps", C.c_int32), ("guide_lr", C.c_float),
         ("guide_reset", C.c_int32), ("guide_seed", C.c_uint32), ("filter", C.c_int32), ("filter_alpha", C.c_float),
-        ("task_grid_size", C.c_int32),
+        ("task_grid_size", C.c_int32), ("guide_medium_depth", C.c_int32),
     ]
 
 

@@ -596,8 +596,10 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
     if (r != AGX_OK) return r;
     if (guide) {
         if ((r = c->ensure_guide_wave(size_t(per_pass))) != AGX_OK) return r;
-        // media: scattering points get their own network evaluation (k_medium_pass)
-        rc.guide_media = (c->ds.features & SF_MEDIA) ? 1 : 0;
+        // media: with guiding at medium points, scattering points get their own
+        // network evaluation (k_medium_pass)
+        rc.guide_medium_depth = p->guide_medium_depth > 0 ? p->guide_medium_depth : 0;
+        rc.guide_media = ((c->ds.features & SF_MEDIA) && rc.guide_medium_depth > 0) ? 1 : 0;
         if (rc.guide_media && (r = c->ensure_guide_media(size_t(per_pass))) != AGX_OK) return r;
         if (!c->guide_ready || p->guide_reset) {
             if ((r = c->guide_init(p->guide_seed, guide_batch)) != AGX_OK) return r;

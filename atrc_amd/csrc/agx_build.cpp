@@ -779,6 +779,21 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
                     ref_ent.push_back(i);
                     ref_slot.push_back(slot);
                 }
+            } else if (watertight && d.kind == AGX_GEOM_QUAD) {
+                // the quad's two triangles over its own corners: quads that share
+                // an edge then see bit-identical endpoints
+                const float(*P)[3] = desc.entities[i].p;
+                const int halves[2][3] = {{0, 1, 2}, {0, 2, 3}};
+                for (int hf = 0; hf < 2; ++hf) {
+                    PrimRef r;
+                    for (int a = 0; a < 3; ++a) {
+                        r.lo[a] = std::min({P[halves[hf][0]][a], P[halves[hf][1]][a], P[halves[hf][2]][a]});
+                        r.hi[a] = std::max({P[halves[hf][0]][a], P[halves[hf][1]][a], P[halves[hf][2]][a]});
+                    }
+                    refs.push_back(r);
+                    ref_ent.push_back(i);
+                    ref_slot.push_back(hf == 0 ? BVH4_QUAD_ABC : BVH4_QUAD_ACD);
+                }
             } else {
                 PrimRef r;
                 r.lo[0] = bounds[i].lo.x; r.lo[1] = bounds[i].lo.y; r.lo[2] = bounds[i].lo.z;
@@ -802,7 +817,13 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
             const uint32_t r = b4.order[k];
             vec4 *o = &S.b4_prims[3 * k];
             const uint32_t slot = ref_slot[r];
-            if (slot != BVH4_ANALYTIC && watertight) {
+            if (slot == BVH4_QUAD_ABC || slot == BVH4_QUAD_ACD) {
+                const float(*P)[3] = desc.entities[ref_ent[r]].p;
+                const int v1 = slot == BVH4_QUAD_ABC ? 1 : 2, v2 = slot == BVH4_QUAD_ABC ? 2 : 3;
+                o[0] = vec4{P[0][0], P[0][1], P[0][2], 0};
+                o[1] = vec4{P[v1][0], P[v1][1], P[v1][2], 0};
+                o[2] = vec4{P[v2][0], P[v2][1], P[v2][2], 0};
+            } else if (slot != BVH4_ANALYTIC && watertight) {
                 // the mesh's own vertices, so that triangles sharing an edge see
                 // bit-identical endpoints
                 const uint32_t id = f_bits(S.blas_prims[3 * size_t(slot)].w);

@@ -37,13 +37,18 @@ constexpr int MAX_LIGHTS = 16;
 //   blas_prims/blas_info of a mesh triangle, or BVH4_ANALYTIC for a whole
 //   analytic entity; tie rank = DevEntity::tie_rank of the entity.  In the
 //   watertight mode a mesh triangle is (A, entity), (B, slot), (C, tie rank)
-//   with the mesh's own vertex coordinates.
+//   with the mesh's own vertex coordinates, and a quad is two primitives
+//   (A, entity), (B, BVH4_QUAD_ABC), (C, rank) and (A, entity), (C, BVH4_QUAD_ACD), (D, rank).
 constexpr int BVH4_NODE_VEC4 = 4;
 constexpr uint32_t BVH4_EMPTY = 0xffffffffu;
 constexpr uint32_t BVH4_LEAF = 0x80000000u;
 constexpr uint32_t BVH4_FIRST_MASK = 0x07ffffffu;
 constexpr int BVH4_MAX_LEAF = 8;
 constexpr uint32_t BVH4_ANALYTIC = 0xffffffffu;
+// watertight mode: a quad enters the BVH4 as its two triangles over the
+// descriptor's own corners, (A, B, C) and (A, C, D)
+constexpr uint32_t BVH4_QUAD_ABC = 0xfffffffeu;
+constexpr uint32_t BVH4_QUAD_ACD = 0xfffffffdu;
 constexpr int BVH4_STACK_SMALL = 64;   // traversal stack entries; the large variant
 constexpr int BVH4_STACK_LARGE = 192;  // serves trees deeper than (64 - 1) / 3 levels
 // The newest entries of a lane's BVH4 stack live in an LDS ring, older ones in

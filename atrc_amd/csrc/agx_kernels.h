@@ -92,6 +92,7 @@ struct RenderConst {
     int32_t L;           // number of lights
     int32_t guide;       // 1: guided continuation sampling
     int32_t guide_media; // 1: guided render of a scene with media (k_medium_pass runs before inference)
+    int32_t guide_medium_depth;  // medium scattering points at path depth <= this are guided
     float guide_alpha;   // probability of drawing from the learned density
     int32_t guide_record;  // 1: guided vertices of the recording paths keep training records
     uint32_t guide_rec_shift;  // recording paths: (p + guide_rec_rot) % 2^shift == 0, record slot p >> shift

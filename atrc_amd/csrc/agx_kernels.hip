@@ -699,7 +699,8 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                 const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
                 const int32_t ent = __float_as_int(p0.w);
                 const uint32_t slot = __float_as_uint(p1.w);
-                if (slot != BVH4_ANALYTIC) {
+                if (slot < BVH4_QUAD_ACD || (WT && slot != BVH4_ANALYTIC)) {
+                    // a mesh triangle, or (watertight mode) one triangle of a quad
                     if (STATS) c.tests++;
                     const float4 p2 = P[3 * size_t(i) + 2];
                     float tt, uu, vv;
@@ -710,7 +711,8 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                     const int32_t rank = __float_as_int(p2.w);
                     if (hit_tri && (tt < tmax || ent == h->ent || rank > best_rank)) {
                         tmax = tt;
-                        h->t = tt; h->u = uu; h->v = vv; h->ent = ent; h->prim = int32_t(slot);
+                        h->t = tt; h->u = uu; h->v = vv; h->ent = ent;
+                        h->prim = (WT && slot >= BVH4_QUAD_ACD) ? (slot == BVH4_QUAD_ABC ? 0 : 1) : int32_t(slot);
                         best_rank = rank;
                     }
                 } else {
@@ -763,7 +765,7 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
             for (uint32_t i = first; i < first + cnt; ++i) {
                 const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
                 const uint32_t slot = __float_as_uint(p1.w);
-                if (slot != BVH4_ANALYTIC) {
+                if (slot < BVH4_QUAD_ACD || (WT && slot != BVH4_ANALYTIC)) {
                     if (STATS) c.tests++;
                     const float4 p2 = P[3 * size_t(i) + 2];
                     if (WT) {
@@ -2403,7 +2405,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
                         push_mis = true;
                     }
                 }
-                if ((F & SF_GUIDE) && rc.guide) {
+                if ((F & SF_GUIDE) && rc.guide && dep.x <= rc.guide_medium_depth) {
                     // guided: one-sample mixture of the learned density (probability
                     // alpha) and the phase function, weighted by the mixture density
                     const float usel = rng.uniform();

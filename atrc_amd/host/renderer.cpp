@@ -80,6 +80,7 @@ public:
             rp.guide_lr = params_.guide_lr;
             rp.guide_reset = 1;
             rp.guide_seed = params_.guide_seed;
+            rp.guide_medium_depth = params_.guide_medium_depth;
         }
         const size_t wh = size_t(filter.width) * size_t(filter.height);
         std::vector<float> rgb(wh * 3), alb(wh * 3), nor(wh * 3), den(wh);
@@ -277,6 +278,7 @@ void register_renderers(CreatingContext &ctx) {
         r.guide_steps = p.child_int_or("guide_steps", 4);
         r.guide_lr = p.child_real_or("guide_lr", 1e-2f);
         r.guide_seed = uint32_t(p.child_int_or("guide_seed", 0));
+        r.guide_medium_depth = p.child_int_or("guide_medium_depth", 0);
         return create_gpu_pt_renderer(r);
     });
     ctx.reporter.add_creator("stdout", [](const ConfigGroup &, CreatingContext &) {

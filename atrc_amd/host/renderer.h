@@ -76,6 +76,7 @@ struct PTRendererParams {
     int guide_steps = 4;
     float guide_lr = 1e-2f;
     uint32_t guide_seed = 0;
+    int guide_medium_depth = 0;  // medium scattering points guided up to this depth (0: none)
 };
 
 std::shared_ptr<Renderer> create_gpu_pt_renderer(const PTRendererParams &params);

@@ -250,7 +250,8 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
                   filter_radius: float = 0.5, collect_stats: bool = False, sample_begin: int = 0,
                   guide: bool = False, guide_alpha: float = 0.5, guide_train: bool = True, guide_batch: int = 65536,
                   guide_steps: int = 4, guide_lr: float = 1e-2, guide_reset: bool = False, guide_seed: int = 0,
-                  filter: str = "box", filter_alpha: float = 0.0, task_grid_size: int = 32, use_mis: bool = True):
+                  filter: str = "box", filter_alpha: float = 0.0, task_grid_size: int = 32, use_mis: bool = True,
+                  guide_medium_depth: int = -1):
     p = capi.agx_render_params()
     p.width, p.height, p.spp = width, height, spp
     p.min_depth, p.max_depth, p.specular_depth = min_depth, max_depth, specular_depth
@@ -279,6 +280,7 @@ def render_params(width: int, height: int, spp: int, camera: dict, *, min_depth:
     p.filter = {"box": capi.FILTER_BOX, "gaussian": capi.FILTER_GAUSSIAN}[filter]
     p.filter_alpha = filter_alpha
     p.task_grid_size = task_grid_size
+    p.guide_medium_depth = guide_medium_depth
     return p
 
 
@@ -744,23 +746,24 @@ def batch_scene(seed: int) -> tuple[Scene, dict, dict, int]:
     return s, cam, tr, res
 
 
-def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int = 11) -> tuple[Scene, dict, dict]:
+def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int = 11, density_max: float = 2.0,
+                    slab: bool = True) -> tuple[Scene, dict, dict]:
     """The door-gap room (guided_room) with its first room filled with
     participating media: a heterogeneous medium on a seeded grid_n^3 fBm density
-    (4 octaves, max 2.0, FP16 values, albedo 0.9, HG g = 0.3) over the whole room, and a
-    homogeneous slab (sigma_t 0.5, albedo 0.9) between z = 2.0 and 2.5 bounded
+    (4 octaves, max density_max, FP16 values, albedo 0.9, HG g = 0.3) over the whole room, and
+    (with slab) a homogeneous slab (sigma_t 0.5, albedo 0.9) between z = 2.0 and 2.5 bounded
     by an invisible_surface box.  Every surface facing the first room has the
     fog on that side (entity med_out, or med_in for the slab's box)."""
     s, cam, tr = guided_room(n_clutter_tris=n_clutter_tris, seed=7)
     # densities rounded to FP16 values: the library stores such a grid in FP16
-    density = (fbm_grid(grid_n, seed) * 2.0).astype(np.float16).astype(np.float32)
+    density = (fbm_grid(grid_n, seed) * density_max).astype(np.float16).astype(np.float32)
     fog = s.add_medium(Medium(kind=capi.MEDIUM_HETEROGENEOUS,
                               local_to_world=(10.0, 0, 0, 0, 0, 10.0, 0, 0, 0, 0, 4.0, 0),
                               density=Texture3D(grid=density),
                               albedo=Texture3D(value=(0.9, 0.9, 0.9)),
                               g3=Texture3D(value=(0.3, 0.3, 0.3))))
-    slab = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.05, 0.05, 0.05),
-                               sigma_s=(0.45, 0.45, 0.45), g=0.0))
+    slab_medium = s.add_medium(Medium(kind=capi.MEDIUM_HOMOGENEOUS, sigma_a=(0.05, 0.05, 0.05),
+                                      sigma_s=(0.45, 0.45, 0.45), g=0.0)) if slab else -1
     for e in s.entities:
         if e.tris is not None:                       # clutter: outward normals, fog outside
             e.med_out = fog
@@ -773,9 +776,10 @@ def volumetric_room(n_clutter_tris: int = 500_000, grid_n: int = 256, seed: int 
         # inward-facing walls of the first room, and the dividing wall's side facing it
         if centre[0] < 10.0 - 1e-6 or (abs(centre[0] - 10.0) < 1e-6 and n[0] < 0):
             e.med_out = fog
-    inv = s.add_material(Material(kind=capi.MAT_INVISIBLE_SURFACE))
-    s.add(Entity(capi.GEOM_TRIANGLE_BVH, inv, tris=box_mesh((0.5, 0.5, 2.0), (9.5, 9.5, 2.5)), med_in=slab,
-                 med_out=fog))
+    if slab:
+        inv = s.add_material(Material(kind=capi.MAT_INVISIBLE_SURFACE))
+        s.add(Entity(capi.GEOM_TRIANGLE_BVH, inv, tris=box_mesh((0.5, 0.5, 2.0), (9.5, 9.5, 2.5)),
+                     med_in=slab_medium, med_out=fog))
     return s, cam, dict(tr, max_depth=16)
 
 

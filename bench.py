@@ -41,7 +41,9 @@ Further legs (each can be switched off):
 * guided: equal-time relMSE of guided vs plain GPU PT on the
   door-gap room (500k clutter triangles) at 1024x1024; guided_volumetric
   (configs[4]): the same comparison in the fog-filled room (FP16 density
-  grid, depth 16), where scattering points get their own network evaluation.  Plain and guided runs
+  grid, depth 16) with the learned density at probability --vol-guided-alpha;
+  --vol-guided-medium-depth > 0 also guides scattering points up to that
+  depth, each with its own network evaluation.  Plain and guided runs
   each take --guided-seconds of wall time in chunks of --guided-chunk spp; the
   guided run starts from a fresh network, trains on every pass for the first
   --guided-train-frac of its time, then renders with the network frozen, and
@@ -442,7 +444,9 @@ def leg_guided(ctx, args, volumetric: bool = False) -> dict:
     if volumetric:
         s, cam, tr = scenes.volumetric_room(n_clutter_tris=args.guided_tris, grid_n=args.grid)
         seconds, ref_factor = args.vol_guided_seconds, args.vol_guided_ref_factor
+        gopts = {"guide_alpha": args.vol_guided_alpha, "guide_medium_depth": args.vol_guided_medium_depth}
     else:
+        gopts = {}
         s, cam, tr = scenes.guided_room(n_clutter_tris=args.guided_tris, seed=7)
         seconds, ref_factor = args.guided_seconds, args.guided_ref_factor
     s.accel = capi.ACCEL_BVH4
@@ -464,7 +468,8 @@ def leg_guided(ctx, args, volumetric: bool = False) -> dict:
             train = guide and (spp == 0 or time.perf_counter() - t_start < train_s)
             p = scenes.render_params(W, H, args.guided_chunk, cam, seed=seed, sample_begin=spp, guide=guide,
                                      guide_reset=guide and spp == 0, guide_train=train,
-                                     guide_steps=args.guided_steps, guide_batch=args.guided_batch, **tr)
+                                     guide_steps=args.guided_steps, guide_batch=args.guided_batch,
+                                     **(gopts if guide else {}), **tr)
             ctx.render(p, aovs=False, out=img)
             acc += img.astype(np.float64) * args.guided_chunk
             spp += args.guided_chunk
@@ -495,11 +500,13 @@ def leg_guided(ctx, args, volumetric: bool = False) -> dict:
         "workload": (f"door-gap room (10x10x4 Lambert room lit through a 0.05-wide gap, "
                      f"{s.triangle_count()} triangles)"
                      + (f" filled with a {args.grid}^3 fBm medium (FP16 grid, sigma_t max 2, albedo 0.9, g 0.3) "
-                        f"and a homogeneous slab (sigma_t 0.5); network inputs at medium vertices" if volumetric else "")
+                        f"and a homogeneous slab (sigma_t 0.5); mixture alpha {args.vol_guided_alpha}, medium points "
+                        f"guided to depth {args.vol_guided_medium_depth}" if volumetric else "")
                      + f", {W}x{H}, max depth {tr['max_depth']}, BVH4"),
         "seconds": seconds, "chunk_spp": args.guided_chunk,
         "train_frac": args.guided_train_frac, "adam_steps_per_pass": args.guided_steps,
-        "sample_ring": args.guided_batch,
+        "sample_ring": args.guided_batch, "guide_alpha": gopts.get("guide_alpha", 0.5),
+        "guide_medium_depth": gopts.get("guide_medium_depth", 0),
         "plain": {"spp": plain_spp, "wall_s": round(plain_wall, 3), "relmse": ep, "relmse_uncorrected": ep_raw,
                   "Mrays_per_s": pst["rays"] / plain_wall / 1e6},
         "guided": {"spp": guided_spp, "wall_s": round(guided_wall, 3), "relmse": eg, "relmse_uncorrected": eg_raw,
@@ -557,6 +564,10 @@ def main() -> int:
     ap.add_argument("--no-guided-volumetric", action="store_true", help="skip the guided vs plain leg in media")
     ap.add_argument("--vol-guided-seconds", type=float, default=6.0)
     ap.add_argument("--vol-guided-ref-factor", type=float, default=6.0)
+    ap.add_argument("--vol-guided-alpha", type=float, default=0.25,
+                    help="probability of the learned density in the fog room's mixture")
+    ap.add_argument("--vol-guided-medium-depth", type=int, default=0,
+                    help="medium scattering points guided up to this path depth (0: surfaces only)")
     ap.add_argument("--skip-headline", action="store_true",
                     help="run only the further legs (profiling); prints no headline line")
     args = ap.parse_args()

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 11
+#define AGX_ABI_VERSION 12
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -267,6 +267,16 @@ typedef struct agx_render_params {
      * (apply_image_filter, framebuffer.h:49-80); the GPU film does the same,
      * so wide-filter images depend on the tile size as Atrc's do. */
     int32_t task_grid_size;
+    /* Path guiding in media: medium scattering points at path depth <= this
+     * take the mixture of the learned density and the phase function; deeper
+     * ones sample the phase function alone (its weight f / pdf is exactly 1,
+     * while the mixture's varies, and the variance of a product of many such
+     * weights grows with the scattering count).  0 or negative (default) = no
+     * guiding at medium points: in optically thick media, where the light
+     * reaching a scattering point is nearly isotropic, the mixture only adds
+     * variance and medium records pull the shared network away from the
+     * surface distribution. */
+    int32_t guide_medium_depth;
 } agx_render_params;
 
 /* Counters and timers of the last agx_render / agx_trace_rays call. */

@@ -154,7 +154,7 @@ def test_unknown_accel_rejected():
 def test_watertight_records_hold_the_mesh_vertices():
     """AGX_ACCEL_BVH4_WATERTIGHT: the same tree, mesh triangles as (A, B, C) with
     the mesh's own vertex coordinates (shared edges stay bit-identical)."""
-    s, _, _ = scenes.procedural_spheres(n_spheres=6, subdiv=2, seed=4, extent=10.0)
+    s, _, _ = scenes.procedural_spheres(n_spheres=6, subdiv=2, seed=4, extent=10.0, ground=False)
     s.accel = capi.ACCEL_BVH4
     n4, p4 = capi.debug_bvh4(s.to_desc()[0])
     s.accel = capi.ACCEL_BVH4_WATERTIGHT
@@ -175,3 +175,25 @@ def test_watertight_records_hold_the_mesh_vertices():
         ids = orig[slotw[sel].astype(np.int64) - base]
         verts = np.asarray(tris, np.float32)[:, :3].reshape(-1, 3, 3)
         np.testing.assert_array_equal(pw[sel, :, :3], verts[ids])
+
+
+def test_watertight_quads_are_two_triangles_over_their_corners():
+    """AGX_ACCEL_BVH4_WATERTIGHT: every quad is two primitives, (A, B, C) and
+    (A, C, D) over the descriptor's own corners, so quads sharing an edge
+    share bit-identical endpoints; spheres stay single analytic primitives."""
+    s = scenes.Scene(aggregate=capi.AGGREGATE_NATIVE)
+    m = s.add_material(scenes.Material())
+    scenes._room_quads(s, (0.1, 0.2, 0.3), (1.7, 2.9, 1.3), m)
+    s.add(scenes.Entity(capi.GEOM_SPHERE, m, radius=0.3, local_to_world=(1, 0, 0, 0.9, 0, 1, 0, 1.5, 0, 0, 1, 0.8)))
+    s.accel = capi.ACCEL_BVH4_WATERTIGHT
+    desc, keep = s.to_desc()
+    nodes, prims = capi.debug_bvh4(desc)
+    slot = prims[:, 1, 3].view(np.uint32)
+    ent = prims[:, 0, 3].view(np.int32)
+    assert (slot == ANALYTIC).sum() == 1 and ent[slot == ANALYTIC][0] == 6
+    for i, e in enumerate(s.entities[:6]):
+        P = np.asarray(e.points, np.float32)
+        for code, (b, c) in ((0xFFFFFFFE, (1, 2)), (0xFFFFFFFD, (2, 3))):
+            k = np.nonzero((ent == i) & (slot == code))[0]
+            assert len(k) == 1
+            np.testing.assert_array_equal(prims[k[0], :, :3], P[[0, b, c]])

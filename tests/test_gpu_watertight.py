@@ -107,3 +107,53 @@ def test_watertight_render_path_parity(gpu_ctx):
     s, cam, tr = scenes.cornell_box()
     s.accel = capi.ACCEL_BVH4_WATERTIGHT
     _path_parity(gpu_ctx, s, cam, tr, 128, 128, "cornell, watertight BVH4", min_frac=0.97)
+
+
+def _quad_box(accel):
+    """A closed box of six inward quads; adjacent quads share their corners
+    exactly (corner tuples reused), with coordinates that are not dyadic."""
+    lo, hi = (-0.9, -1.6, 0.15), (1.6, 1.3, 1.85)
+    s = scenes.Scene(aggregate=capi.AGGREGATE_BVH)
+    m = s.add_material(scenes.Material(color=(0.5, 0.5, 0.5)))
+    scenes._room_quads(s, lo, hi, m)
+    s.accel = accel
+    corners = sorted({tuple(np.float32(c) for c in p) for e in s.entities for p in e.points})
+    index = {c: k for k, c in enumerate(corners)}
+    pos = np.array(corners, np.float32)
+    faces = []
+    for e in s.entities:
+        a, b, c, d = (index[tuple(np.float32(x) for x in p)] for p in e.points)
+        faces += [(a, b, c), (a, c, d)]
+    return s, pos, np.array(faces), pos[np.array(faces)]
+
+
+@pytest.mark.parametrize("accel", [capi.ACCEL_BVH4_WATERTIGHT, capi.ACCEL_BVH4])
+def test_quad_box_edges(gpu_ctx, accel):
+    """Watertight mode on quads: rays from inside aimed exactly at the box's
+    corners, shared edges and quad diagonals never escape, and every hit is the
+    watertight test of the quad triangle it reports, bit for bit; Atrc's
+    Moller-Trumbore quads (the BVH4 mode) let some of the same rays through."""
+    s, pos, faces, tris = _quad_box(accel)
+    desc, keep = s.to_desc()
+    gpu_ctx.upload_scene(desc)
+    o, d = _edge_rays(pos, faces, seed=31, n_origins=40)
+    n = len(o)
+    g = gpu_ctx.trace_rays(o, d, np.zeros(n, np.float32), np.full(n, np.inf, np.float32))
+    a = gpu_ctx.trace_rays(o, d, np.zeros(n, np.float32), np.full(n, np.inf, np.float32), any_hit=True)
+    escaped, any_escaped = int((g["entity"] < 0).sum()), int((a["entity"] < 0).sum())
+    log(f"[watertight] quad box, accel {accel}: {n} corner / edge / diagonal rays from inside, "
+        f"escaped {escaped} (any hit {any_escaped})")
+    if accel == capi.ACCEL_BVH4:
+        assert escaped > 0
+        return
+    assert escaped == 0 and any_escaped == 0
+    # each hit: the watertight test of the reported half (prim 0 = ABC, 1 = ACD) of the reported quad
+    tri_index = 2 * g["entity"] + g["prim"]
+    for i in range(0, n, 7):
+        r = po.triangle_soup_closest(tris[tri_index[i]:tri_index[i] + 1], o[i:i + 1], d[i:i + 1], 0, np.inf,
+                                     watertight=True)
+        assert r[0][0] == 0
+        assert (r[1][0], r[2][0], r[3][0]) == (g["t"][i], g["u"][i], g["v"][i])
+    # and the closest hit over all twelve triangles has the same distance
+    prim, t, u, v = po.triangle_soup_closest(tris, o, d, 0, np.inf, watertight=True)
+    np.testing.assert_array_equal(t, g["t"])
