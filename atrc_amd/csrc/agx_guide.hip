@@ -43,8 +43,21 @@ __device__ __forceinline__ float one_blob(float v, int b) {
 
 __device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
+// Input k of a 32-wide k-step held as element i by lane group g (lanes 16 g ..
+// 16 g + 15) of a fragment.  Layer 0 takes its inputs in natural order (the
+// one-blob features are generated in registers in any order).  Layers 1..4
+// take them in the order an MFMA output leaves them: output tile 2 s' (rows
+// 0..15 of the k-step) gives lane group g rows 4 g .. 4 g + 3, output tile
+// 2 s' + 1 (rows 16..31) rows 16 + 4 g .. 16 + 4 g + 3, so a layer's output
+// fragments are the next layer's input fragments without any data movement.
+__host__ __device__ constexpr int g_kperm(int l, int g, int i) {
+    return l == 0 ? 8 * g + i : (i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4));
+}
+
 // FP32 master parameters -> BF16 fragment-ordered weights + FP32 biases.
-// packed[off(l) + ((s * T + t) * 64 + lane) * 8 + i] = W_l[32 s + 8 (lane / 16) + i][16 t + lane % 16]
+// packed[off(l) + ((s * T + t) * 64 + lane) * 8 + i] = W_l[32 s + g_kperm(l, lane / 16, i)][16 t + lane % 16]:
+// the B fragment of x W (rows x inputs times inputs x outputs) and, identically,
+// the A fragment of W^T x^T (outputs x inputs times inputs x rows).
 __global__ void __launch_bounds__(256) k_guide_pack(const float *params, __bf16 *packed, float *bias) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid < G_PACK) {
@@ -54,7 +67,7 @@ __global__ void __launch_bounds__(256) k_guide_pack(const float *params, __bf16 
         const int i = local & 7, lane = (local >> 3) & 63, frag = local >> 9;
         const int T = g_out(l) / 16;
         const int s = frag / T, t = frag % T;
-        const int k = 32 * s + 8 * (lane / 16) + i, n = 16 * t + (lane % 16);
+        const int k = 32 * s + g_kperm(l, lane / 16, i), n = 16 * t + (lane % 16);
         packed[gid] = __bf16(params[g_w_off(l) + k * g_out(l) + n]);
     }
     if (gid < G_BIASES) {
@@ -74,12 +87,21 @@ struct GuideSmem {
 // Hidden layer l (64 -> 64 or 64 -> 32) for the 16 rows staged in act[wave]:
 // reads the activations, then (after the wave has read them) overwrites them
 // with ReLU(x W + b), or writes the raw outputs to ost for the last layer.
+// The A fragment (16 rows x 32 inputs) of k-step s from a staged activation
+// row, in the packed weights' input order (g_kperm).
+__device__ __forceinline__ bf16x8 act_frag(const __bf16 *row, int s, int g) {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(row + 32 * s + 4 * g);
+    const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(row + 32 * s + 16 + 4 * g);
+    return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
 template <int L>
 __device__ __forceinline__ void hidden_layer(GuideSmem &sm, int wave, int lane) {
     constexpr int T = g_out(L) / 16;
     const int r = lane % 16, g = lane / 16;
-    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(&sm.act[wave][r][8 * g]);
-    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&sm.act[wave][r][32 + 8 * g]);
+    const bf16x8 a0 = act_frag(&sm.act[wave][r][0], 0, g);
+    const bf16x8 a1 = act_frag(&sm.act[wave][r][0], 1, g);
     f32x4 acc[T];
     #pragma unroll
     for (int t = 0; t < T; ++t) {
@@ -105,88 +127,210 @@ __device__ __forceinline__ void hidden_layer(GuideSmem &sm, int wave, int lane) 
 }
 
 // Batched inference: row i = in[8 i .. 8 i + 7] -> out[40 i ..]: 8 lobes of
-// (mu.xyz, kappa) then 8 normalised lobe weights.  Persistent blocks; each
-// wave takes 16-row tiles.
-__global__ void __launch_bounds__(256) k_guide_infer(const __bf16 *packed, const float *bias, const float *in,
+// (mu.xyz, kappa) then 8 normalised lobe weights.
+//
+// Each layer is computed transposed, Y^T = W^T X^T: the packed weights are the
+// A operand (16 outputs x 32 inputs), the activations the B operand (32 inputs
+// x 16 rows), and an output fragment (16 outputs x 16 rows: lane l holds row
+// l % 16, outputs 4 (l / 16) .. + 3) becomes, after bias, ReLU and BF16
+// rounding in registers, the input fragment of the next layer in the packed
+// input order (g_kperm).  Activations never leave registers.  A wave works on
+// IR 16-row sub-tiles at once, so every weight fragment read from LDS feeds IR
+// MFMAs.  Persistent blocks of IW waves; all BF16 weights and the biases stay
+// in LDS for the block's lifetime (61 KB: two blocks per CU, so blocks of 8
+// waves give each SIMD 4 waves).
+
+constexpr int IW = 8;  // waves per inference block
+
+struct InferSmem {
+    __bf16 w[G_PACK];
+    float b[G_BIASES];
+};
+
+// wz is zero, laundered once per tile: the weights in LDS do not change over
+// the tile loop, and without it the compiler hoists every fragment load out of
+// the loop and spills the 240 registers they take
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr float LOG2E_G = 1.44269504088896340736f;
+constexpr float INV_E_G = 0.36787944117144233f;
+
+// (lo, hi) rounded to a BF16 pair (nearest even, one v_cvt_pk_bf16_f32)
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+}
+// ReLU of a BF16 pair: a set sign bit reads as a negative int16, and max with 0
+// clears it (rounding then ReLU equals ReLU then rounding)
+__device__ __forceinline__ uint32_t relu_pair(float lo, float hi) {
+    const s16x2 v = __builtin_bit_cast(s16x2, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, (s16x2){0, 0}));
+}
+
+// One-blob bins 8 g .. 8 g + 7 of x as a B fragment.  With d_i = 32 x - (8 g +
+// i) - 1/2, bin i is exp(-d_i^2 / 2); neighbours follow by ratios:
+// bin i + 1 = bin i * exp(d_i - 1/2) and bin i - 1 = bin i * exp(-d_i - 1/2),
+// and each further step multiplies the ratio by 1/e.  The recurrence starts at
+// bin 4, so the bins near the Gaussian's centre carry the rounding of an
+// exponential of a small argument, as a direct evaluation would: three
+// exponentials per eight bins instead of eight.
+__device__ __forceinline__ bf16x8 one_blob8(float x, int g) {
+    const float d = x * float(G_BINS) - (float(8 * g) + 4.5f);
+    float v[8];
+    v[4] = __builtin_amdgcn_exp2f(d * d * (-0.5f * LOG2E_G));
+    float up = __builtin_amdgcn_exp2f((d - 0.5f) * LOG2E_G);
+    float dn = __builtin_amdgcn_exp2f((-d - 0.5f) * LOG2E_G);
+    v[5] = v[4] * up;
+    up = up * INV_E_G;
+    v[6] = v[5] * up;
+    up = up * INV_E_G;
+    v[7] = v[6] * up;
+    v[3] = v[4] * dn;
+    dn = dn * INV_E_G;
+    v[2] = v[3] * dn;
+    dn = dn * INV_E_G;
+    v[1] = v[2] * dn;
+    dn = dn * INV_E_G;
+    v[0] = v[1] * dn;
+    const u32x4 u = {bf16_pair(v[0], v[1]), bf16_pair(v[2], v[3]), bf16_pair(v[4], v[5]), bf16_pair(v[6], v[7])};
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ float sigmoid_fast(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
+
+__device__ __forceinline__ bf16x8 wfrag(const InferSmem &sm, int l, int s, int t, int lane, int wz) {
+    return *reinterpret_cast<const bf16x8 *>(&sm.w[wz + g_pack_off(l) + ((s * (g_out(l) / 16) + t) * 64 + lane) * 8]);
+}
+
+// ReLU and BF16 rounding of a 64-output layer's fragments (bias already in
+// the accumulators) -> the next layer's two k-step fragments of each sub-tile
+template <int IR>
+__device__ __forceinline__ void relu_frags(const f32x4 (&acc)[IR][4], bf16x8 (&h)[IR][2]) {
+    #pragma unroll
+    for (int q = 0; q < IR; ++q) {
+        #pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const f32x4 a = acc[q][2 * s], b = acc[q][2 * s + 1];
+            const u32x4 u = {relu_pair(a[0], a[1]), relu_pair(a[2], a[3]), relu_pair(b[0], b[1]), relu_pair(b[2], b[3])};
+            h[q][s] = __builtin_bit_cast(bf16x8, u);
+        }
+    }
+}
+
+// 64 -> T * 16 outputs for the IR sub-tiles; the first k-step's accumulator
+// input is the bias (lane group g: outputs 16 t + 4 g .. + 3)
+template <int L, int IR>
+__device__ __forceinline__ void chained_layer(const InferSmem &sm, int lane, int wz, const bf16x8 (&h)[IR][2],
+                                              f32x4 (&acc)[IR][4]) {
+    constexpr int T = g_out(L) / 16;
+    f32x4 b[T];
+    #pragma unroll
+    for (int t = 0; t < T; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&sm.b[wz + g_bias_off(L) + 16 * t + 4 * (lane / 16)]);
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        #pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const bf16x8 w = wfrag(sm, L, s, t, lane, wz);
+            #pragma unroll
+            for (int q = 0; q < IR; ++q) acc[q][t] = mfma(w, h[q][s], s == 0 ? b[t] : acc[q][t]);
+        }
+    }
+}
+
+template <int IR>  // 16-row sub-tiles per wave
+__global__ void __launch_bounds__(IW * 64, 4) k_guide_infer(const __bf16 *packed, const float *bias, const float *in,
                                                      float *out, const uint32_t *n_ptr, uint32_t n_fixed) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    GuideSmem &sm = *reinterpret_cast<GuideSmem *>(smem_raw);
+    InferSmem &sm = *reinterpret_cast<InferSmem *>(smem_raw);
     const uint32_t n = n_ptr ? *n_ptr : n_fixed;
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    if (uint32_t(blockIdx.x) * 64u >= n) return;  // whole block past the rows (block-uniform)
+    constexpr uint32_t ROWS = 16u * IR;  // rows of one wave tile
+    if (uint32_t(blockIdx.x) * IW * ROWS >= n) return;  // whole block past the rows (block-uniform)
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(packed);
         uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
-        for (int i = tid; i < G_PACK / 8; i += 256) dst[i] = src[i];
-        for (int i = tid; i < G_BIASES; i += 256) sm.b[i] = bias[i];
+        for (int i = tid; i < G_PACK / 8; i += IW * 64) dst[i] = src[i];
+        for (int i = tid; i < G_BIASES; i += IW * 64) sm.b[i] = bias[i];
     }
     __syncthreads();
     const int r = lane % 16, g = lane / 16;
-    for (uint32_t tile = blockIdx.x * 4u + uint32_t(wave); tile * 16u < n; tile += gridDim.x * 4u) {
-        const uint32_t row = tile * 16u + uint32_t(r);
-        const bool valid = row < n;
-        float x[G_IN];
-        #pragma unroll
-        for (int s = 0; s < G_IN; ++s) {
-            const float v = valid ? in[size_t(row) * G_IN + s] : 0.0f;
-            x[s] = fminf(1.0f, fmaxf(0.0f, v));
+    for (uint32_t tile = blockIdx.x * IW + uint32_t(wave); tile * ROWS < n; tile += gridDim.x * IW) {
+        int wz = 0;
+        asm volatile("" : "+s"(wz));
+        // layer 0: k-step s covers the 32 bins of input s; lane group g makes bins 8 g .. 8 g + 7;
+        // the first k-step's accumulator input is the bias
+        f32x4 acc[IR][4];
+        {
+            f32x4 b[4];
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&sm.b[wz + g_bias_off(0) + 16 * t + 4 * g]);
+            #pragma unroll
+            for (int q = 0; q < IR; ++q)
+                #pragma unroll
+                for (int t = 0; t < 4; ++t) acc[q][t] = b[t];
         }
-        // layer 0: k-step s covers the 32 bins of input s
-        f32x4 acc[4];
-        #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
-        #pragma unroll
+        #pragma unroll 2
         for (int s = 0; s < G_IN; ++s) {
-            bf16x8 a;
+            float x[IR];
             #pragma unroll
-            for (int i = 0; i < 8; ++i) a[i] = __bf16(one_blob(x[s], 8 * g + i));
+            for (int q = 0; q < IR; ++q) {
+                const uint32_t row = tile * ROWS + 16u * q + uint32_t(r);
+                x[q] = row < n ? fminf(1.0f, fmaxf(0.0f, in[size_t(row) * G_IN + s])) : 0.0f;
+            }
+            bf16x8 w[4];
             #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bf16x8 *w = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(0) + ((s * 4 + t) * 64 + lane) * 8]);
-                acc[t] = mfma(a, *w, acc[t]);
+            for (int t = 0; t < 4; ++t) w[t] = wfrag(sm, 0, s, t, lane, wz);
+            #pragma unroll
+            for (int q = 0; q < IR; ++q) {
+                const bf16x8 f = one_blob8(x[q], g);
+                #pragma unroll
+                for (int t = 0; t < 4; ++t) acc[q][t] = mfma(w[t], f, acc[q][t]);
             }
         }
+        bf16x8 h[IR][2];
+        relu_frags<IR>(acc, h);
+        chained_layer<1, IR>(sm, lane, wz, h, acc);
+        relu_frags<IR>(acc, h);
+        chained_layer<2, IR>(sm, lane, wz, h, acc);
+        relu_frags<IR>(acc, h);
+        chained_layer<3, IR>(sm, lane, wz, h, acc);
+        relu_frags<IR>(acc, h);
+        chained_layer<4, IR>(sm, lane, wz, h, acc);
+        // decode: output tile t of lane group g holds lobe 4 t + g of row r, components
+        // (weight logit, theta, phi, log kappa)
         #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int col = 16 * t + r;
-            const float b = sm.b[g_bias_off(0) + col];
+        for (int q = 0; q < IR; ++q) {
+            const uint32_t row = tile * ROWS + 16u * q + uint32_t(r);
+            const bool valid = row < n;
+            float logit[2];
             #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = acc[t][j] + b;
-                sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
+            for (int t = 0; t < 2; ++t) {
+                const f32x4 o = acc[q][t];
+                logit[t] = o[0];
+                const float th = PI_G * sigmoid_fast(o[1]), ph = 2.0f * PI_G * sigmoid_fast(o[2]);
+                const float kappa = __expf(fminf(G_LOGK_MAX, fmaxf(G_LOGK_MIN, o[3])));
+                float st, ct, sp, cp;
+                __sincosf(th, &st, &ct);
+                __sincosf(ph, &sp, &cp);
+                if (valid)
+                    *reinterpret_cast<float4 *>(out + size_t(row) * G_ROW_OUT + 4 * (4 * t + g)) =
+                        make_float4(st * cp, st * sp, ct, kappa);
             }
-        }
-        wave_sync();
-        hidden_layer<1>(sm, wave, lane);
-        hidden_layer<2>(sm, wave, lane);
-        hidden_layer<3>(sm, wave, lane);
-        hidden_layer<4>(sm, wave, lane);
-        // decode: lane (r, g) decodes lobes 2g and 2g + 1 of row r
-        #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const int lobe = 2 * g + q;
-            const float *o = &sm.ost[wave][r][4 * lobe];
-            const float th = PI_G * sigmoid(o[1]), ph = 2.0f * PI_G * sigmoid(o[2]);
-            const float kappa = __expf(fminf(G_LOGK_MAX, fmaxf(G_LOGK_MIN, o[3])));
-            float st, ct, sp, cp;
-            __sincosf(th, &st, &ct);
-            __sincosf(ph, &sp, &cp);
+            // softmax of the 8 weight logits over the row's four lane groups
+            float mx = fmaxf(logit[0], logit[1]);
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float e0 = __expf(logit[0] - mx), e1 = __expf(logit[1] - mx);
+            float sum = e0 + e1;
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
+            const float inv = __builtin_amdgcn_rcpf(sum);
             if (valid) {
-                float4 *dst = reinterpret_cast<float4 *>(out + size_t(row) * G_ROW_OUT + 4 * lobe);
-                *dst = make_float4(st * cp, st * sp, ct, kappa);
+                out[size_t(row) * G_ROW_OUT + 4 * G_LOBES + g] = e0 * inv;
+                out[size_t(row) * G_ROW_OUT + 4 * G_LOBES + 4 + g] = e1 * inv;
             }
         }
-        if (g == 0 && valid) {  // softmax of the 8 weight logits
-            float mx = -3.0e38f;
-            #pragma unroll
-            for (int j = 0; j < G_LOBES; ++j) mx = fmaxf(mx, sm.ost[wave][r][4 * j]);
-            float e[G_LOBES], sum = 0;
-            #pragma unroll
-            for (int j = 0; j < G_LOBES; ++j) { e[j] = __expf(sm.ost[wave][r][4 * j] - mx); sum += e[j]; }
-            #pragma unroll
-            for (int j = 0; j < G_LOBES; ++j) out[size_t(row) * G_ROW_OUT + 4 * G_LOBES + j] = e[j] / sum;
-        }
-        wave_sync();
     }
 }
 
@@ -627,15 +771,17 @@ __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, flo
         bias[g_bias_off(l) + local - g_in(l) * g_out(l)] = p;
         return;
     }
-    // inverse of k_guide_pack's index map
+    // inverse of k_guide_pack's index map (g_kperm)
     const int k = local / g_out(l), n = local % g_out(l), T = g_out(l) / 16;
-    const int sI = k / 32, tI = n / 16, lane = ((k % 32) / 8) * 16 + n % 16;
-    packed[g_pack_off(l) + ((sI * T + tI) * 64 + lane) * 8 + k % 8] = __bf16(p);
+    const int sI = k / 32, tI = n / 16, kk = k % 32;
+    const int grp = l == 0 ? kk / 8 : (kk % 16) / 4;
+    const int elem = l == 0 ? kk % 8 : (kk < 16 ? 0 : 4) + kk % 4;
+    packed[g_pack_off(l) + ((sI * T + tI) * 64 + grp * 16 + n % 16) * 8 + elem] = __bf16(p);
 }
 
 }  // namespace
 
-size_t guide_smem_bytes() { return sizeof(GuideSmem); }
+size_t guide_smem_bytes() { return sizeof(InferSmem) > sizeof(GuideSmem) ? sizeof(InferSmem) : sizeof(GuideSmem); }
 size_t guide_train_smem_bytes() { return sizeof(TrainSmem); }
 
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
@@ -691,8 +837,11 @@ hipError_t guide_prepare(int device) {
     hipError_t e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) return e;
     if (size_t(lds) < sizeof(GuideSmem) || size_t(lds) < sizeof(TrainSmem)) return hipErrorInvalidConfiguration;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GuideSmem)));
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer<1>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer<4>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
@@ -708,11 +857,18 @@ hipError_t guide_prepare(int device) {
 void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,
                         uint32_t n_max, hipStream_t st) {
     if (!n_max) return;
-    // persistent: at most 2 blocks per CU resident (LDS), enough to cover the rows
-    const uint32_t tiles = (n_max + 63) / 64;
-    const uint32_t grid = tiles < 1024u ? tiles : 1024u;
-    hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(256), sizeof(GuideSmem), st, packed, bias, in, out, n_dev,
-                       n_max);
+    // persistent: 2 blocks per CU resident (LDS), enough to cover the rows.  Waves
+    // take 64-row tiles (each weight fragment feeds 4 MFMAs) once every CU gets
+    // at least one such tile per wave; smaller batches take 16-row tiles.
+    if (n_max >= 512u * IW * 64u) {
+        const uint32_t blocks = (n_max + IW * 64 - 1) / (IW * 64);
+        hipLaunchKernelGGL(k_guide_infer<4>, dim3(blocks < 512u ? blocks : 512u), dim3(IW * 64), sizeof(InferSmem), st,
+                           packed, bias, in, out, n_dev, n_max);
+    } else {
+        const uint32_t blocks = (n_max + IW * 16 - 1) / (IW * 16);
+        hipLaunchKernelGGL(k_guide_infer<1>, dim3(blocks < 512u ? blocks : 512u), dim3(IW * 64), sizeof(InferSmem), st,
+                           packed, bias, in, out, n_dev, n_max);
+    }
 }
 
 }  // namespace agx

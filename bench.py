@@ -1,6 +1,6 @@
 """Benchmark of the gfx950 path tracer (one MI355X).
 
-Headline workload (configs[1] of the benchmark definition): plain GPU PT+NEE
+Headline workload: plain GPU PT+NEE
 on the seeded 1M-triangle procedural mesh scene -- 200 displaced icospheres
 (1,024,000 FP32 triangles, half Lambert / half GGX conductor) on a ground
 quad under a native_sky and the sun, 1920x1080, 64 spp, max depth 8 (Atrc
@@ -31,16 +31,18 @@ build made on this host, and a one-thread run; nproc and the CPU model are
 reported.
 
 Further legs (each can be switched off):
-* cornell (configs[0]): Cornell-like scene, 256x256, 16 spp, depth 5, on the
+* watertight: the headline frame again in the watertight BVH4 mode
+  (Woop-Benthin-Wald triangle and quad tests over the same tree).
+* cornell: Cornell-like scene, 256x256, 16 spp, depth 5, on the
   GPU and on the CPU oracle, timed, each with its relMSE against a 4096-spp
   GPU reference.
-* volumetric (configs[4], plain PT): the door-gap room filled with a 256^3
+* volumetric (plain PT): the door-gap room filled with a 256^3
   fBm heterogeneous medium and a homogeneous slab, 1024x1024, depth 16.
-* batched (configs[2]): 64 seeded scenes (1e4..2e6 triangles, 256^2..1024^2,
+* batched: 64 seeded scenes (1e4..2e6 triangles, 256^2..1024^2,
   32 spp) in one hipGraph launch (agx_render_batch_graph), host builds first.
-* guided (configs[3]): equal-time relMSE of guided vs plain GPU PT on the
-  door-gap room (500k clutter triangles) at 1024x1024; guided_volumetric
-  (configs[4]): the same comparison in the fog-filled room (FP16 density
+* guided: equal-time relMSE of guided vs plain GPU PT on the
+  door-gap room (500k clutter triangles) at 1024x1024; guided_volumetric:
+  the same comparison in the fog-filled room (FP16 density
   grid, depth 16) with the learned density at probability --vol-guided-alpha;
   --vol-guided-medium-depth > 0 also guides scattering points up to that
   depth, each with its own network evaluation.  Plain and guided runs
@@ -142,7 +144,8 @@ def build_headline(args):
         workload = ("plain GPU PT+NEE, 1M-triangle procedural mesh scene (200 displaced icospheres, "
                     "Lambert + GGX conductor, ground quad, native_sky + sun as a 0.53-degree emissive disk), "
                     f"1920x1080, {args.spp} spp per frame, depth 8")
-    s.accel = capi.ACCEL_BVH4 if args.accel == "bvh4" else capi.ACCEL_ATRC
+    s.accel = {"bvh4": capi.ACCEL_BVH4, "atrc": capi.ACCEL_ATRC,
+               "bvh4_watertight": capi.ACCEL_BVH4_WATERTIGHT}[args.accel]
     return s, cam, tr, W, H, workload
 
 
@@ -258,6 +261,39 @@ def headline(ctx, args):
     return out, (desc, keep, cam, tr, W, H)
 
 
+def leg_watertight(ctx, args, head: dict, desc, cam, tr, W: int, H: int) -> dict:
+    """The headline frame again with the watertight BVH4 mode (Woop-Benthin-Wald
+    triangle and quad tests over the same tree): its cost against the default
+    Moller-Trumbore mode."""
+    import numpy as np
+    from atrc_amd import capi, scenes
+
+    desc.accel = capi.ACCEL_BVH4_WATERTIGHT
+    t0 = time.time()
+    ctx.upload_scene(desc)
+    build_s = time.time() - t0
+    img = np.empty((H, W, 3), np.float32)
+    ctx.render(scenes.render_params(W, H, args.spp, cam, seed=42, **tr), aovs=False, out=img)
+    rays = closest_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(args.watertight_frames):
+        ctx.render(scenes.render_params(W, H, args.spp, cam, seed=42, sample_begin=(k + 1) * args.spp, **tr),
+                   aovs=False, out=img)
+        st = ctx.stats()
+        rays += st["closest_rays"] + st["shadow_rays"]
+        closest_ms += st["trace_closest_ms"]
+    wall = time.perf_counter() - t0
+    desc.accel = capi.ACCEL_BVH4
+    value = rays / wall / 1e6
+    log(f"[bench] watertight BVH4: {args.watertight_frames} frames of {args.spp} spp, {value:.1f} Mrays/s "
+        f"({value / head['value']:.3f} of the default mode), closest hit {closest_ms / args.watertight_frames:.1f} "
+        f"ms/frame")
+    return {"workload": "the headline scene and frame, accel bvh4_watertight", "frames": args.watertight_frames,
+            "Mrays_per_s": value, "ms_per_frame": wall * 1e3 / args.watertight_frames,
+            "closest_ms_per_frame": closest_ms / args.watertight_frames, "upload_s": build_s,
+            "vs_default_mode": value / head["value"]}
+
+
 # ----------------------------------------------------------------------------- CPU baseline
 
 def build_native_oracle():
@@ -328,7 +364,7 @@ def cpu_baseline(desc, cam, tr, W, H, args) -> dict:
 # ----------------------------------------------------------------------------- further legs
 
 def leg_cornell(ctx, args) -> dict:
-    """the Cornell-like scene at 256x256, 16 spp, on the GPU and on the CPU oracle."""
+    """The Cornell-like scene at 256x256, 16 spp, on the GPU and on the CPU oracle."""
     import numpy as np
     from atrc_amd import capi, scenes
     from oracle import py_oracle as po
@@ -367,7 +403,7 @@ def leg_cornell(ctx, args) -> dict:
 
 
 def leg_volumetric(ctx, args) -> dict:
-    """configs[4] (plain PT): the door-gap room filled with media, 1024x1024, depth 16."""
+    """Plain PT in the door-gap room filled with media, 1024x1024, depth 16."""
     import numpy as np
     from atrc_amd import capi, scenes
 
@@ -396,7 +432,7 @@ def leg_volumetric(ctx, args) -> dict:
 
 
 def leg_batched(args) -> dict:
-    """seeded scenes of uneven size rendered in one hipGraph launch."""
+    """Seeded scenes of uneven size rendered in one hipGraph launch."""
     from atrc_amd import capi, scenes
 
     t0 = time.time()
@@ -435,7 +471,7 @@ def leg_batched(args) -> dict:
 
 
 def leg_guided(ctx, args, volumetric: bool = False) -> dict:
-    """configs[3] (door-gap room) or configs[4] (the room filled with media):
+    """The door-gap room, or the same room filled with media:
     equal-time relMSE, guided vs plain."""
     import numpy as np
     from atrc_amd import capi, scenes
@@ -536,8 +572,12 @@ def main() -> int:
     ap.add_argument("--spp", type=int, default=64, help="samples per pixel of one frame (one step)")
     ap.add_argument("--config", default="mesh1m", choices=["mesh1m", "cornell", "volumetric"],
                     help="headline workload (default: the 1M-triangle scene)")
-    ap.add_argument("--accel", default="bvh4", choices=["bvh4", "atrc"],
-                    help="acceleration structure: bvh4 (SAH BVH4, production) or atrc (Atrc's own BVHs)")
+    ap.add_argument("--accel", default="bvh4", choices=["bvh4", "atrc", "bvh4_watertight"],
+                    help="acceleration structure: bvh4 (SAH BVH4, production), atrc (Atrc's own BVHs) or "
+                         "bvh4_watertight (BVH4 with watertight triangle and quad tests)")
+    ap.add_argument("--no-watertight", action="store_true",
+                    help="skip the leg that renders the headline scene in the watertight BVH4 mode")
+    ap.add_argument("--watertight-frames", type=int, default=3)
     ap.add_argument("--grid", type=int, default=256, help="density grid resolution of the volumetric scene")
     ap.add_argument("--stats-spp", type=int, default=2)
     ap.add_argument("--copy-calibration-mb", type=int, default=256,
@@ -585,6 +625,8 @@ def main() -> int:
     else:
         out, (desc, keep, cam, tr, W, H) = headline(ctx, args)
     legs = {}
+    if desc is not None and not args.no_watertight and args.accel == "bvh4":
+        legs["watertight"] = leg_watertight(ctx, args, out, desc, cam, tr, W, H)
     if not args.no_cornell:
         legs["cornell"] = leg_cornell(ctx, args)
     if not args.no_volumetric:

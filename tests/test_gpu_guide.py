@@ -75,7 +75,7 @@ def ref_infer(p, x):
 def test_guide_inference_matches_reference(gpu_ctx):
     p = init_params(1) * 3   # larger weights: non-trivial activations
     rng = np.random.default_rng(2)
-    for n in (1, 17, 1000, 70001):
+    for n in (1, 17, 1000, 70001, 300007):   # 300007 takes the 64-row-tile kernel (batches of >= 262144 rows)
         x = rng.random((n, 8)).astype(np.float32)
         log(f"[guide] inference n={n}")
         g = gpu_ctx.guide_infer(p, x)
