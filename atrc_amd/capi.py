@@ -16,7 +16,7 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 GPU_LIB_PATH = PKG / "libatrc_gpu.so"
 
-AGX_ABI_VERSION = 12
+AGX_ABI_VERSION = 13
 
 AGX_OK, AGX_E_INVALID_ARG, AGX_E_HIP, AGX_E_NO_SCENE, AGX_E_UNSUPPORTED, AGX_E_CANCELLED, \
     AGX_E_OUT_OF_MEMORY, AGX_E_LIMIT = range(8)
@@ -156,7 +156,9 @@ class agx_batch_item(C.Structure):
 class agx_batch_report(C.Structure):
     _fields_ = [
         ("workers", C.c_int32), ("tuned", C.c_int32), ("wall_ms", C.c_double), ("rays", C.c_uint64),
-        ("tune_rays_per_s", C.c_double * 4),
+        ("tune_rays_per_s", C.c_double * 4), ("pass_paths_log2", C.c_int32), ("trace_block", C.c_int32),
+        ("groups", C.c_int32), ("build_threads", C.c_int32), ("build_wall_ms", C.c_double),
+        ("render_ms", C.c_double), ("tune_configs", C.c_int32), ("reserved", C.c_int32),
     ]
 
 
@@ -310,7 +312,9 @@ class Context:
         recs = [{"status": it.status, "worker": it.worker, "rays": it.rays, "build_ms": it.build_ms,
                  "render_ms": it.render_ms} for it in items]
         report = {"workers": rep.workers, "tuned": bool(rep.tuned), "wall_ms": rep.wall_ms, "rays": rep.rays,
-                  "tune_rays_per_s": list(rep.tune_rays_per_s)}
+                  "tune_rays_per_s": list(rep.tune_rays_per_s), "pass_paths_log2": rep.pass_paths_log2,
+                  "trace_block": rep.trace_block, "groups": rep.groups, "build_threads": rep.build_threads,
+                  "build_wall_ms": rep.build_wall_ms, "render_ms": rep.render_ms, "tune_configs": rep.tune_configs}
         return images, recs, report
 
     def trace_rays(self, o: np.ndarray, d: np.ndarray, tmin: np.ndarray, tmax: np.ndarray, any_hit: bool = False,

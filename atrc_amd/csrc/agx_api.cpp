@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -71,6 +72,7 @@ struct agx_ctx {
     size_t pass_paths = 0;
     int pass_paths_L = -1;
     size_t pass_paths_limit = 0;  // 0: none; batch workers, which render side by side, use 2^22
+    int trace_block = 256;        // threads per block of the BVH4 traversal launches (128 or 256)
     int64_t cap_wh = 0;
     uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
@@ -506,6 +508,10 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     ds.grids = gr;
     ds.grids16 = gr16;
     ds.accel = hs.accel;
+    // AGX_TRACE_BLOCK=128 selects 128-thread traversal blocks for contexts left at the default
+    ds.trace_block = c->trace_block;
+    if (const char *e = std::getenv("AGX_TRACE_BLOCK"))
+        if (std::string(e) == "128" && c->trace_block == 256) ds.trace_block = 128;
     ds.b4_nodes = b4n;
     ds.b4_prims = b4p;
     // a BVH4 traversal keeps at most 3 pending children per level
@@ -1151,7 +1157,6 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
                            agx_batch_report *report) {
     if (!c || (n && !items)) return fail(AGX_E_INVALID_ARG, "null argument");
     if (branches < 0 || branches > 8) return fail(AGX_E_INVALID_ARG, "branches must lie in [0, 8]");
-    const int k = branches > 0 ? branches : 4;
     for (uint32_t i = 0; i < n; ++i) {
         if (!items[i].scene || !items[i].params) return fail(AGX_E_INVALID_ARG, "batch item without scene or params");
         if (items[i].params->guide) return fail(AGX_E_UNSUPPORTED, "guided rendering is not available in batches");
@@ -1162,126 +1167,277 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     }
     HIP_TRY(hipSetDevice(c->device));
     const auto t0 = Clock::now();
-    // everything the graph touches, released on every exit
+    agx_batch_report rep{};
+    // cost estimate of an item: paths it traces plus a BVH term
+    auto cost = [&](uint32_t i) {
+        const agx_render_params *p = items[i].params;
+        uint64_t tris = 0;
+        for (uint32_t e = 0; e < items[i].scene->entity_count; ++e) tris += items[i].scene->entities[e].tri_count;
+        return double(p->width) * p->height * p->spp + 4.0 * double(items[i].scene->entity_count) + double(tris);
+    };
+    // cheapest first: the first group builds quickly and the GPU starts while the
+    // costlier scenes are still being built
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) < cost(b); });
+    const uint32_t group_size = 8;
+    const uint32_t n_groups = (n + group_size - 1) / group_size;
+
+    // everything the graphs touch, released on every exit
     struct Held {
         std::vector<agx_ctx *> ctxs;
         std::vector<hipStream_t> streams;
-        std::vector<hipEvent_t> events;
+        std::vector<hipEvent_t> fork, join, start, done;
         unsigned long long *rays = nullptr;
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        std::vector<hipGraphExec_t> execs;
+        std::vector<std::thread> builders;
+        std::atomic<bool> stop{false};
         ~Held() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-            for (hipEvent_t e : events) (void)hipEventDestroy(e);
-            for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+            stop = true;
+            for (auto &t : builders) if (t.joinable()) t.join();
+            for (hipGraphExec_t e : execs) if (e) (void)hipGraphExecDestroy(e);
+            for (auto *v : {&fork, &join, &start, &done})
+                for (hipEvent_t e : *v) if (e) (void)hipEventDestroy(e);
+            for (hipStream_t st : streams) (void)hipStreamDestroy(st);
             if (rays) (void)hipFree(rays);
             for (agx_ctx *w : ctxs) if (w) agx_destroy(w);
         }
     } held;
-    // one context per scene: its scene and buffers stay resident for the graph;
-    // host BVH builds and uploads run on k threads
     held.ctxs.assign(n, nullptr);
-    std::atomic<uint32_t> next{0};
+    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&held.rays), std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+    HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+    HIP_TRY(hipDeviceSynchronize());
+
+    // host builds (BVH + upload) on every host core, in group order
+    std::vector<std::atomic<int>> built(n);  // 0 pending, 1 built, 2 failed
+    for (auto &b : built) b = 0;
     std::mutex mu;
-    int status = AGX_OK;
-    std::string err;
+    std::condition_variable cv;
+    std::atomic<uint32_t> next{0};
+    std::atomic<int64_t> build_end_ns{0};
+    const int n_threads = int(std::max(1u, std::min({std::thread::hardware_concurrency(), 16u, std::max(n, 1u)})));
+    std::string build_err;
     auto build = [&]() {
         for (;;) {
-            const uint32_t i = next.fetch_add(1);
-            if (i >= n) break;
+            if (held.stop) break;
+            const uint32_t j = next.fetch_add(1);
+            if (j >= n) break;
+            const uint32_t i = order[j];
             const auto tb = Clock::now();
             agx_ctx *w = nullptr;
             int r = agx_create(c->device, &w);
             if (r == AGX_OK) {
                 held.ctxs[i] = w;
-                // scenes render side by side: passes of at most 4 M paths each
-                w->pass_paths_limit = size_t(1) << 22;
                 r = agx_upload_scene(w, items[i].scene);
             }
-            if (r == AGX_OK) r = graph_prepare(w, items[i].params);
             items[i].build_ms = ms_since(tb);
             items[i].status = r;
-            if (r != AGX_OK) {
+            {
                 std::lock_guard<std::mutex> lk(mu);
-                if (status == AGX_OK) {
-                    status = r;
-                    err = "batch item " + std::to_string(i) + ": " + agx_last_error();
-                }
+                if (r != AGX_OK && build_err.empty()) build_err = "batch item " + std::to_string(i) + ": " + agx_last_error();
+                built[i] = r == AGX_OK ? 1 : 2;
+                build_end_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t0).count();
             }
+            cv.notify_all();
         }
     };
-    {
-        std::vector<std::thread> th;
-        for (int t = 1; t < k; ++t) th.emplace_back(build);
-        build();
-        for (auto &t : th) t.join();
-    }
-    if (status != AGX_OK) return fail(status, err);
-    HIP_TRY(hipMalloc(reinterpret_cast<void **>(&held.rays), std::max<size_t>(n, 1) * sizeof(unsigned long long)));
-    HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
-    // branches: the costliest scene first onto the least loaded branch
-    std::vector<uint32_t> order(n);
-    for (uint32_t i = 0; i < n; ++i) order[i] = i;
-    auto cost = [&](uint32_t i) {
-        const agx_render_params *p = items[i].params;
-        return double(p->width) * p->height * p->spp + 4.0 * double(items[i].scene->entity_count);
+    for (int t = 0; t < n_threads; ++t) held.builders.emplace_back(build);
+    auto wait_group = [&](uint32_t g) -> int {
+        std::unique_lock<std::mutex> lk(mu);
+        const uint32_t a = g * group_size, b = std::min(n, a + group_size);
+        cv.wait(lk, [&] {
+            for (uint32_t j = a; j < b; ++j) if (built[order[j]] == 0) return false;
+            return true;
+        });
+        for (uint32_t j = a; j < b; ++j)
+            if (built[order[j]] == 2) return fail(items[order[j]].status, build_err);
+        return AGX_OK;
     };
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) > cost(b); });
-    std::vector<std::vector<uint32_t>> lanes(static_cast<size_t>(k));
-    std::vector<double> load(static_cast<size_t>(k), 0.0);
-    for (uint32_t i : order) {
-        const size_t b = size_t(std::min_element(load.begin(), load.end()) - load.begin());
-        lanes[b].push_back(i);
-        load[b] += cost(i);
-        items[i].worker = int32_t(b);
-    }
-    held.streams.assign(size_t(k), nullptr);
-    for (auto &st : held.streams) HIP_TRY(hipStreamCreate(&st));
-    held.events.assign(size_t(k) + 1, nullptr);
-    for (auto &e : held.events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // capture: fork the branches off the context's stream, enqueue each branch's
-    // scenes one after another, join
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-    int cap_status = AGX_OK;
-    hipError_t he = hipEventRecord(held.events[0], c->stream);
-    for (int b = 0; b < k && he == hipSuccess; ++b) he = hipStreamWaitEvent(held.streams[size_t(b)], held.events[0], 0);
-    for (int b = 0; b < k && he == hipSuccess && cap_status == AGX_OK; ++b)
-        for (uint32_t i : lanes[size_t(b)]) {
-            cap_status = enqueue_render(held.ctxs[i], items[i].params, held.streams[size_t(b)], held.rays + i);
-            if (cap_status != AGX_OK) break;
+
+    // per-branch streams and events (branches at most 8)
+    held.streams.assign(8, nullptr);
+    for (auto &st : held.streams) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    held.fork.assign(n_groups + 64, nullptr);
+    held.join.assign(size_t(n_groups + 64) * 8, nullptr);
+    held.start.assign(n_groups + 64, nullptr);
+    held.done.assign(n_groups + 64, nullptr);
+    for (auto *v : {&held.fork, &held.join})
+        for (auto &e : *v) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto *v : {&held.start, &held.done})
+        for (auto &e : *v) HIP_TRY(hipEventCreate(&e));
+    size_t ev = 0;  // next free event slot
+
+    struct Config {
+        int branches;
+        int pass_log2;
+        int trace_block;
+    };
+    // Capture the renders of items sel with configuration cfg into one graph
+    // (costliest first, each onto the least loaded branch; a branch renders its
+    // scenes one after another), launch it on the context's stream between
+    // timing events.  spp_override > 0 renders the items at that spp (probes).
+    auto launch = [&](const std::vector<uint32_t> &sel, const Config &cfg, int spp_override, size_t *slot) -> int {
+        std::vector<agx_render_params> prm(sel.size());
+        for (size_t q = 0; q < sel.size(); ++q) {
+            agx_ctx *w = held.ctxs[sel[q]];
+            prm[q] = *items[sel[q]].params;
+            if (spp_override > 0) prm[q].spp = std::min(prm[q].spp, spp_override);
+            if (w->pass_paths_limit != (size_t(1) << cfg.pass_log2)) {
+                w->pass_paths_limit = size_t(1) << cfg.pass_log2;
+                w->pass_paths_L = -1;
+            }
+            w->ds.trace_block = cfg.trace_block;
+            if (int r = graph_prepare(w, &prm[q]); r != AGX_OK) return r;
         }
-    for (int b = 0; b < k && he == hipSuccess; ++b) {
-        he = hipEventRecord(held.events[size_t(b) + 1], held.streams[size_t(b)]);
-        if (he == hipSuccess) he = hipStreamWaitEvent(c->stream, held.events[size_t(b) + 1], 0);
+        std::vector<size_t> by_cost(sel.size());
+        for (size_t q = 0; q < sel.size(); ++q) by_cost[q] = q;
+        std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) { return cost(sel[a]) > cost(sel[b]); });
+        const int k = cfg.branches;
+        std::vector<std::vector<size_t>> lanes(static_cast<size_t>(k));
+        std::vector<double> load(static_cast<size_t>(k), 0.0);
+        for (size_t q : by_cost) {
+            const size_t b = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+            lanes[b].push_back(q);
+            load[b] += cost(sel[q]);
+            if (spp_override <= 0) items[sel[q]].worker = int32_t(b);
+        }
+        if (ev >= held.start.size()) return fail(AGX_E_LIMIT, "graph batch: out of events");
+        const size_t e = ev++;
+        *slot = e;
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        int cap_status = AGX_OK;
+        hipError_t he = hipEventRecord(held.fork[e], c->stream);
+        for (int b = 0; b < k && he == hipSuccess; ++b) he = hipStreamWaitEvent(held.streams[size_t(b)], held.fork[e], 0);
+        for (int b = 0; b < k && he == hipSuccess && cap_status == AGX_OK; ++b)
+            for (size_t q : lanes[size_t(b)]) {
+                cap_status = enqueue_render(held.ctxs[sel[q]], &prm[q], held.streams[size_t(b)], held.rays + sel[q]);
+                if (cap_status != AGX_OK) break;
+            }
+        for (int b = 0; b < k && he == hipSuccess; ++b) {
+            he = hipEventRecord(held.join[e * 8 + size_t(b)], held.streams[size_t(b)]);
+            if (he == hipSuccess) he = hipStreamWaitEvent(c->stream, held.join[e * 8 + size_t(b)], 0);
+        }
+        const hipError_t he_end = hipStreamEndCapture(c->stream, &graph);
+        if (cap_status != AGX_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return cap_status;  // the message is set
+        }
+        if (he != hipSuccess || he_end != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return fail(AGX_E_HIP, std::string("graph capture: ") + hipGetErrorString(he != hipSuccess ? he : he_end));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t hi = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (hi != hipSuccess) return fail(AGX_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(hi));
+        held.execs.push_back(exec);
+        HIP_TRY(hipEventRecord(held.start[e], c->stream));
+        HIP_TRY(hipGraphLaunch(exec, c->stream));
+        HIP_TRY(hipEventRecord(held.done[e], c->stream));
+        return AGX_OK;
+    };
+    auto group_items = [&](uint32_t g) {
+        std::vector<uint32_t> v;
+        for (uint32_t j = g * group_size; j < std::min(n, (g + 1) * group_size); ++j) v.push_back(order[j]);
+        return v;
+    };
+
+    Config cfg{branches > 0 ? branches : 4, 22, 256};
+    if (branches == 0 && n > 0) {
+        // autotuner: coordinate descent over branches, pass size and traversal block
+        // size, each configuration timed on the first group's scenes at 2 spp
+        // (device time between events); later groups build meanwhile
+        if (int r = wait_group(0); r != AGX_OK) return r;
+        const std::vector<uint32_t> probe = group_items(0);
+        auto time_cfg = [&](const Config &k, double *rate) -> int {
+            HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+            size_t slot = 0;
+            if (int r = launch(probe, k, 2, &slot); r != AGX_OK) return r;
+            HIP_TRY(hipEventSynchronize(held.done[slot]));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, held.start[slot], held.done[slot]));
+            std::vector<unsigned long long> hr(n, 0ull);
+            HIP_TRY(hipMemcpy(hr.data(), held.rays, n * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+            uint64_t rays = 0;
+            for (uint32_t i : probe) rays += hr[i];
+            *rate = double(rays) / std::max(double(ms) * 1e-3, 1e-9);
+            ++rep.tune_configs;
+            return AGX_OK;
+        };
+        double best = -1;
+        // warm-up launch (first-use costs), not compared
+        if (int r = time_cfg(cfg, &best); r != AGX_OK) return r;
+        best = -1;
+        for (int b : {2, 4, 8}) {
+            double rate = 0;
+            const Config k{b, cfg.pass_log2, cfg.trace_block};
+            if (int r = time_cfg(k, &rate); r != AGX_OK) return r;
+            rep.tune_rays_per_s[b == 2 ? 0 : (b == 4 ? 1 : 2)] = rate;
+            if (rate > best) { best = rate; cfg.branches = b; }
+        }
+        for (int pl : {21, 23}) {
+            double rate = 0;
+            const Config k{cfg.branches, pl, cfg.trace_block};
+            if (int r = time_cfg(k, &rate); r != AGX_OK) return r;
+            if (rate > best * 1.02) { best = rate; cfg.pass_log2 = pl; }
+        }
+        {
+            double rate = 0;
+            const Config k{cfg.branches, cfg.pass_log2, 128};
+            if (int r = time_cfg(k, &rate); r != AGX_OK) return r;
+            rep.tune_rays_per_s[3] = rate;
+            if (rate > best * 1.02) { best = rate; cfg.trace_block = 128; }
+        }
+        rep.tuned = 1;
+        HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
+        HIP_TRY(hipDeviceSynchronize());
     }
-    const hipError_t he_end = hipStreamEndCapture(c->stream, &held.graph);
-    if (cap_status != AGX_OK) return cap_status;  // the message is set
-    if (he != hipSuccess) return fail(AGX_E_HIP, std::string("graph capture: ") + hipGetErrorString(he));
-    if (he_end != hipSuccess) return fail(AGX_E_HIP, std::string("graph capture: ") + hipGetErrorString(he_end));
-    HIP_TRY(hipGraphInstantiate(&held.exec, held.graph, nullptr, nullptr, 0));
+
+    // the batch: group after group, each captured and launched as soon as its
+    // scenes are built; a finished group's images are copied out and its
+    // contexts released while later groups render
     const auto tr = Clock::now();
-    HIP_TRY(hipGraphLaunch(held.exec, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    const double render_ms = ms_since(tr);
-    std::vector<unsigned long long> hr(std::max<size_t>(n, 1), 0ull);
-    HIP_TRY(hipMemcpy(hr.data(), held.rays, hr.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    uint64_t total_rays = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        const agx_render_params *p = items[i].params;
-        const size_t wh = size_t(p->width) * size_t(p->height);
-        if (items[i].rgb)
-            HIP_TRY(hipMemcpy(items[i].rgb, held.ctxs[i]->d_resolve, 3 * wh * sizeof(float), hipMemcpyDeviceToHost));
-        items[i].rays = hr[i];
-        items[i].render_ms = render_ms;
-        items[i].status = AGX_OK;
-        total_rays += hr[i];
+    std::vector<size_t> slots(n_groups, 0);
+    std::vector<uint64_t> hr(std::max<size_t>(n, 1), 0);
+    auto finish_group = [&](uint32_t g) -> int {
+        HIP_TRY(hipEventSynchronize(held.done[slots[g]]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, held.start[slots[g]], held.done[slots[g]]));
+        for (uint32_t i : group_items(g)) {
+            unsigned long long r = 0;
+            HIP_TRY(hipMemcpy(&r, held.rays + i, sizeof(r), hipMemcpyDeviceToHost));
+            const agx_render_params *p = items[i].params;
+            const size_t wh = size_t(p->width) * size_t(p->height);
+            if (items[i].rgb)
+                HIP_TRY(hipMemcpy(items[i].rgb, held.ctxs[i]->d_resolve, 3 * wh * sizeof(float), hipMemcpyDeviceToHost));
+            items[i].rays = r;
+            items[i].render_ms = ms;
+            items[i].status = AGX_OK;
+            rep.rays += r;
+            agx_destroy(held.ctxs[i]);
+            held.ctxs[i] = nullptr;
+        }
+        return AGX_OK;
+    };
+    for (uint32_t g = 0; g < n_groups; ++g) {
+        if (int r = wait_group(g); r != AGX_OK) return r;
+        if (int r = launch(group_items(g), cfg, 0, &slots[g]); r != AGX_OK) return r;
+        if (g >= 1)
+            if (int r = finish_group(g - 1); r != AGX_OK) return r;
     }
-    agx_batch_report rep{};
-    rep.workers = k;
-    rep.tuned = 0;
+    if (n_groups)
+        if (int r = finish_group(n_groups - 1); r != AGX_OK) return r;
+    rep.render_ms = ms_since(tr);
+    for (auto &t : held.builders) t.join();
+    held.builders.clear();
+    rep.workers = cfg.branches;
+    rep.pass_paths_log2 = cfg.pass_log2;
+    rep.trace_block = cfg.trace_block;
+    rep.groups = int32_t(n_groups);
+    rep.build_threads = n_threads;
+    rep.build_wall_ms = double(build_end_ns.load()) * 1e-6;
     rep.wall_ms = ms_since(t0);
-    rep.rays = total_rays;
     if (report) *report = rep;
     return AGX_OK;
 }

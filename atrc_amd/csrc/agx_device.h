@@ -157,6 +157,7 @@ struct DevScene {
     int32_t accel;         // AGX_ACCEL_*
     int32_t bvh4_large_stack;  // 1: the BVH4 is too deep for BVH4_STACK_SMALL
     uint32_t features;     // SF_* the scene uses (media, Disney, delta materials)
+    int32_t trace_block;   // threads per block of the BVH4 traversal kernels (128 or 256; host-side choice)
     const vec4 *b4_nodes;
     const vec4 *b4_prims;
     int32_t lights[MAX_LIGHTS];

@@ -636,7 +636,7 @@ __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
 
 // Per-lane traversal stack whose newest S entries live in an LDS ring and
 // whose older entries spill to a private array.  The ring is a
-// [slot][TRACE_BLOCK] array, so the lanes of a wave touch consecutive words at
+// [slot][TB] array (TB = threads per block), so the lanes of a wave touch consecutive words at
 // any mix of depths and never conflict on a bank; an entry reaches private
 // memory only while the stack is deeper than S.  A push onto a full ring moves
 // the oldest ring entry (the same slot) out to the spill array, and a pop that
@@ -644,19 +644,19 @@ __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
 // top min(depth, S) entries.  Each lane touches only its own column: no
 // barrier is needed.  (Free functions over separate locals, so that the depth
 // and the column address stay in registers.)
-template <int S, class E>
+template <int S, int TB, class E>
 __device__ __forceinline__ void ring_push(E *col, E *spill, int &sp, E e) {
     const int slot = sp & (S - 1);
-    if (sp >= S) spill[sp - S] = col[slot * int(TRACE_BLOCK)];
-    col[slot * int(TRACE_BLOCK)] = e;
+    if (sp >= S) spill[sp - S] = col[slot * TB];
+    col[slot * TB] = e;
     ++sp;
 }
-template <int S, class E>
+template <int S, int TB, class E>
 __device__ __forceinline__ E ring_pop(E *col, const E *spill, int &sp) {
     --sp;
     const int slot = sp & (S - 1);
-    const E e = col[slot * int(TRACE_BLOCK)];
-    if (sp >= S) col[slot * int(TRACE_BLOCK)] = spill[sp - S];
+    const E e = col[slot * TB];
+    if (sp >= S) col[slot * TB] = spill[sp - S];
     return e;
 }
 static_assert((BVH4_RING_CLOSEST & (BVH4_RING_CLOSEST - 1)) == 0 && (BVH4_RING_ANY & (BVH4_RING_ANY - 1)) == 0,
@@ -664,8 +664,8 @@ static_assert((BVH4_RING_CLOSEST & (BVH4_RING_CLOSEST - 1)) == 0 && (BVH4_RING_A
 
 // Closest hit over the BVH4: children visited near-first, pending children
 // kept with their entry distance and skipped once a closer hit is known.
-// ring = this lane's column of a [BVH4_RING_CLOSEST][TRACE_BLOCK] LDS array.
-template <bool STATS, int CAP, bool WT>
+// ring = this lane's column of a [BVH4_RING_CLOSEST][TB] LDS array.
+template <bool STATS, int CAP, bool WT, int TB>
 __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c,
                            uint2 *ring) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
@@ -685,11 +685,11 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
             cswap(t, k, 0, 1); cswap(t, k, 2, 3); cswap(t, k, 0, 2); cswap(t, k, 1, 3); cswap(t, k, 1, 2);
             if (t[0] != INF_F) {
                 if (t[3] != INF_F && sp < CAP)
-                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[3], __float_as_uint(t[3])));
+                    ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[3], __float_as_uint(t[3])));
                 if (t[2] != INF_F && sp < CAP)
-                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[2], __float_as_uint(t[2])));
+                    ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[2], __float_as_uint(t[2])));
                 if (t[1] != INF_F && sp < CAP)
-                    ring_push<BVH4_RING_CLOSEST>(ring, spill, sp, make_uint2(k[1], __float_as_uint(t[1])));
+                    ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[1], __float_as_uint(t[1])));
                 code = k[0];
                 continue;
             }
@@ -730,15 +730,15 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
         // next pending child that may still hold a closer hit
         for (;;) {
             if (sp == 0) return best_rank >= 0;
-            const uint2 e = ring_pop<BVH4_RING_CLOSEST>(ring, spill, sp);
+            const uint2 e = ring_pop<BVH4_RING_CLOSEST, TB>(ring, spill, sp);
             if (__uint_as_float(e.y) <= tmax) { code = e.x; break; }
         }
     }
 }
 
 // Any hit over the BVH4; ring = this lane's column of a
-// [BVH4_RING_ANY][TRACE_BLOCK] LDS array.
-template <bool STATS, int CAP, bool WT>
+// [BVH4_RING_ANY][TB] LDS array.
+template <bool STATS, int CAP, bool WT, int TB>
 __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c, uint32_t *ring) {
     const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
     const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
@@ -757,7 +757,7 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
             for (int j = 0; j < 4; ++j) {
                 if (t[j] == INF_F) continue;
                 if (next == BVH4_EMPTY) next = k[j];
-                else if (sp < CAP) ring_push<BVH4_RING_ANY>(ring, spill, sp, k[j]);
+                else if (sp < CAP) ring_push<BVH4_RING_ANY, TB>(ring, spill, sp, k[j]);
             }
             if (next != BVH4_EMPTY) { code = next; continue; }
         } else {
@@ -780,7 +780,7 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
             }
         }
         if (sp == 0) return false;
-        code = ring_pop<BVH4_RING_ANY>(ring, spill, sp);
+        code = ring_pop<BVH4_RING_ANY, TB>(ring, spill, sp);
     }
 }
 
@@ -799,8 +799,9 @@ constexpr int lds_stack_words(int tm) {
                                                   : 1);
 }
 
-// lds = the block's [lds_stack_words(TM)][TRACE_BLOCK] word array
-template <bool STATS, int TM>
+// lds = the block's [lds_stack_words(TM)][TB] word array (the Atrc-mode LDS
+// column needs TB = 256)
+template <bool STATS, int TM, int TB>
 __device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h,
                                               Counts &c, uint32_t *lds) {
     if constexpr (TM == TM_ATRC_LDS) {
@@ -810,15 +811,15 @@ __device__ __forceinline__ bool query_closest(const DevScene &s, V o, V d, float
         PrivStack stk;
         return scene_closest<STATS>(s, o, d, tmin, tmax, h, c, stk);
     } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
-        return b4_closest<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, h, c,
+        return b4_closest<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL, TB>(s, o, d, tmin, tmax, h, c,
                                                                         reinterpret_cast<uint2 *>(lds) + threadIdx.x);
     } else {
-        return b4_closest<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, h, c,
+        return b4_closest<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE, TB>(s, o, d, tmin, tmax, h, c,
                                                                         reinterpret_cast<uint2 *>(lds) + threadIdx.x);
     }
 }
 
-template <bool STATS, int TM>
+template <bool STATS, int TM, int TB>
 __device__ __forceinline__ bool query_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c,
                                           uint32_t *lds) {
     if constexpr (TM == TM_ATRC_LDS) {
@@ -828,15 +829,15 @@ __device__ __forceinline__ bool query_any(const DevScene &s, V o, V d, float tmi
         PrivStack stk;
         return scene_any<STATS>(s, o, d, tmin, tmax, c, stk);
     } else if constexpr (TM == TM_B4_SMALL || TM == TM_B4W_SMALL) {
-        return b4_any<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
+        return b4_any<STATS, BVH4_STACK_SMALL, TM == TM_B4W_SMALL, TB>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
     } else {
-        return b4_any<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
+        return b4_any<STATS, BVH4_STACK_LARGE, TM == TM_B4W_LARGE, TB>(s, o, d, tmin, tmax, c, lds + threadIdx.x);
     }
 }
 
 // The block's traversal stack memory in LDS (one word when the variant has none)
-#define AGX_LDS_COLUMN(TM)                                                                   \
-    __shared__ __align__(16) uint32_t lds_stack_[lds_stack_words(TM) * int(TRACE_BLOCK)]; \
+#define AGX_LDS_COLUMN(TM, TB)                                                 \
+    __shared__ __align__(16) uint32_t lds_stack_[lds_stack_words(TM) * (TB)]; \
     uint32_t *lds_col = lds_stack_
 
 // ---------------------------------------------------------------- surface reconstruction
@@ -1649,9 +1650,10 @@ __device__ __forceinline__ void flush_counts(const Counts &c, bool ok, DevCounte
 // eighth of the queue: queues are in path (pixel) order, so an XCD's rays come
 // from one band of the image and share BVH nodes in that XCD's own L2.  The
 // logical block of this physical block for a queue of n rays, or -1; the grid
-// holds at least ceil(n / 256) + 7 blocks, so every logical block is covered once.
+// holds at least ceil(n / TB) + 7 blocks, so every logical block is covered once.
+template <int TB>
 __device__ __forceinline__ int32_t xcd_block(uint32_t n) {
-    const uint32_t nb = (n + TRACE_BLOCK - 1u) / TRACE_BLOCK;
+    const uint32_t nb = (n + TB - 1u) / TB;
     const uint32_t per = (nb + 7u) / 8u;
     const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3;
     if (j >= per) return -1;
@@ -1662,12 +1664,12 @@ __device__ __forceinline__ int32_t xcd_block(uint32_t n) {
 // TM = traversal variant (TM_*).  Eight waves per SIMD: the traversal is bound by
 // the latency of its node fetches, and the default allocation (66 VGPRs) left
 // room for seven; eight fit in 64 without spilling.
-template <bool STATS, int TM>
-__global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
+template <bool STATS, int TM, int TB>
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8))) k_trace_closest(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[0];
-    const int32_t lb = xcd_block(n);
+    const int32_t lb = xcd_block<TB>(n);
     if (lb < 0) return;  // whole block past the queue
-    AGX_LDS_COLUMN(TM);
+    AGX_LDS_COLUMN(TM, TB);
     const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
@@ -1676,7 +1678,7 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
         const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
         HitRec h;
         h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-        found = query_closest<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
+        found = query_closest<STATS, TM, TB>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
         if (!found) h.ent = -1;
         pb.hit[ri] = make_float4(h.t, h.u, h.v, __int_as_float(h.ent));
         pb.hit_prim[ri] = h.prim;
@@ -1684,19 +1686,19 @@ __global__ void __launch_bounds__(TRACE_BLOCK) __attribute__((amdgpu_waves_per_e
     if (STATS) flush_counts(c, found, cnt);
 }
 
-template <bool STATS, int TM>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
+template <bool STATS, int TM, int TB>
+__global__ void __launch_bounds__(TB) k_trace_shadow(DevScene s, PathBuffers pb, DevCounters *cnt) {
     const uint32_t n = pb.cnt_in[1];
-    const int32_t lb = xcd_block(n);
+    const int32_t lb = xcd_block<TB>(n);
     if (lb < 0) return;
-    AGX_LDS_COLUMN(TM);
+    AGX_LDS_COLUMN(TM, TB);
     const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool occ = false;
     if (tid < n) {
         const uint32_t ri = pb.q_shadow[tid];
         const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
-        occ = query_any<STATS, TM>(s, mk(o4), mk(d4), d4.w, o4.w, c, lds_col);
+        occ = query_any<STATS, TM, TB>(s, mk(o4), mk(d4), d4.w, o4.w, c, lds_col);
         pb.occluded[ri] = occ ? 1 : 0;
     }
     if (STATS) flush_counts(c, false, cnt);
@@ -2969,24 +2971,24 @@ __global__ void k_reset_counters(uint32_t *c) {
     if (threadIdx.x < 4) c[threadIdx.x] = 0;
 }
 
-template <bool STATS, int TM>
-__global__ void __launch_bounds__(TRACE_BLOCK) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
+template <bool STATS, int TM, int TB>
+__global__ void __launch_bounds__(TB) k_trace_rays(DevScene s, const float4 *o, const float4 *d, uint32_t n,
                                                     int any_hit, int32_t *ent, int32_t *prim, float *t, float *u,
                                                     float *v, DevCounters *cnt) {
-    AGX_LDS_COLUMN(TM);
+    AGX_LDS_COLUMN(TM, TB);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
     if (i < n) {
         const float4 o4 = o[i], d4 = d[i];
         if (any_hit) {
-            found = query_any<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, c, lds_col);
+            found = query_any<STATS, TM, TB>(s, mk(o4), mk(d4), o4.w, d4.w, c, lds_col);
             ent[i] = found ? 1 : -1;
             found = false;
         } else {
             HitRec h;
             h.ent = -1; h.prim = -1; h.t = -1; h.u = 0; h.v = 0;
-            found = query_closest<STATS, TM>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
+            found = query_closest<STATS, TM, TB>(s, mk(o4), mk(d4), o4.w, d4.w, &h, c, lds_col);
             if (found) {
                 const DevEntity &e = s.ents[h.ent];
                 int32_t pid = h.prim;
@@ -3040,35 +3042,48 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
     if (rc.n_paths <= 0) return;
     hipLaunchKernelGGL(k_raygen, dim3(grid_of(uint64_t(rc.n_paths), 256)), dim3(256), 0, st, s, rc, pb);
 }
-#define AGX_LAUNCH_TM(K, TM, grid, st, ...)                                                        \
+#define AGX_LAUNCH_TM(K, TM, TB, grid, st, ...)                                                  \
     do {                                                                                          \
-        if (stats) hipLaunchKernelGGL((K<true, TM>), grid, dim3(TRACE_BLOCK), 0, st, __VA_ARGS__); \
-        else hipLaunchKernelGGL((K<false, TM>), grid, dim3(TRACE_BLOCK), 0, st, __VA_ARGS__);      \
+        if (stats) hipLaunchKernelGGL((K<true, TM, TB>), grid, dim3(TB), 0, st, __VA_ARGS__);     \
+        else hipLaunchKernelGGL((K<false, TM, TB>), grid, dim3(TB), 0, st, __VA_ARGS__);          \
     } while (0)
-#define AGX_LAUNCH_STACKED(K, grid, st, ...)                                                        \
+// tb = threads per block: 128 or 256 in the BVH4 (Moller-Trumbore) mode, 256 otherwise
+#define AGX_LAUNCH_STACKED(K, tb, grid, st, ...)                                                  \
     do {                                                                                          \
         if (s.accel == AGX_ACCEL_BVH4) {                                                          \
-            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4_LARGE, grid, st, __VA_ARGS__);         \
-            else AGX_LAUNCH_TM(K, TM_B4_SMALL, grid, st, __VA_ARGS__);                            \
+            if ((tb) == 128) {                                                                    \
+                if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4_LARGE, 128, grid, st, __VA_ARGS__);\
+                else AGX_LAUNCH_TM(K, TM_B4_SMALL, 128, grid, st, __VA_ARGS__);                   \
+            } else {                                                                              \
+                if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4_LARGE, 256, grid, st, __VA_ARGS__);\
+                else AGX_LAUNCH_TM(K, TM_B4_SMALL, 256, grid, st, __VA_ARGS__);                   \
+            }                                                                                     \
         } else if (s.accel == AGX_ACCEL_BVH4_WATERTIGHT) {                                        \
-            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, grid, st, __VA_ARGS__);        \
-            else AGX_LAUNCH_TM(K, TM_B4W_SMALL, grid, st, __VA_ARGS__);                           \
+            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, 256, grid, st, __VA_ARGS__);   \
+            else AGX_LAUNCH_TM(K, TM_B4W_SMALL, 256, grid, st, __VA_ARGS__);                      \
         } else if (s.lds_stack) {                                                                 \
-            AGX_LAUNCH_TM(K, TM_ATRC_LDS, grid, st, __VA_ARGS__);                                 \
+            AGX_LAUNCH_TM(K, TM_ATRC_LDS, 256, grid, st, __VA_ARGS__);                            \
         } else {                                                                                  \
-            AGX_LAUNCH_TM(K, TM_ATRC_PRIV, grid, st, __VA_ARGS__);                                \
+            AGX_LAUNCH_TM(K, TM_ATRC_PRIV, 256, grid, st, __VA_ARGS__);                           \
         }                                                                                         \
     } while (0)
+
+// threads per block of the wavefront's traversal launches for this scene
+inline unsigned trace_block_of(const DevScene &s) {
+    return (s.accel == AGX_ACCEL_BVH4 && s.trace_block == 128) ? 128u : TRACE_BLOCK;
+}
 
 void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                           hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_closest, dim3(grid_of(max_n, TRACE_BLOCK) + 7), st, s, pb, cnt);
+    const unsigned tb = trace_block_of(s);
+    AGX_LAUNCH_STACKED(k_trace_closest, tb, dim3(grid_of(max_n, tb) + 7), st, s, pb, cnt);
 }
 void launch_trace_shadow(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,
                          hipStream_t st) {
     if (!max_n) return;
-    AGX_LAUNCH_STACKED(k_trace_shadow, dim3(grid_of(max_n, TRACE_BLOCK) + 7), st, s, pb, cnt);
+    const unsigned tb = trace_block_of(s);
+    AGX_LAUNCH_STACKED(k_trace_shadow, tb, dim3(grid_of(max_n, tb) + 7), st, s, pb, cnt);
 }
 void launch_shade(const DevScene &s, const RenderConst &rc, const PathBuffers &pb, uint32_t max_n, hipStream_t st) {
     if (!max_n) return;
@@ -3225,7 +3240,7 @@ void launch_trace_rays(const DevScene &s, const float4 *o, const float4 *d, uint
                        int32_t *prim, float *t, float *u, float *v, DevCounters *cnt, hipStream_t st) {
     if (!n) return;
     const bool stats = cnt != nullptr;
-    AGX_LAUNCH_STACKED(k_trace_rays, dim3(grid_of(n, TRACE_BLOCK)), st, s, o, d, n, any_hit, ent, prim, t, u, v, cnt);
+    AGX_LAUNCH_STACKED(k_trace_rays, TRACE_BLOCK, dim3(grid_of(n, TRACE_BLOCK)), st, s, o, d, n, any_hit, ent, prim, t, u, v, cnt);
 }
 
 }  // namespace agx

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define AGX_ABI_VERSION 12
+#define AGX_ABI_VERSION 13
 
 /* ---- status codes ---------------------------------------------------------------- */
 enum {
@@ -325,11 +325,23 @@ typedef struct agx_batch_item {
 } agx_batch_item;
 
 typedef struct agx_batch_report {
-    int32_t workers;           /* scenes in flight at once */
+    int32_t workers;           /* scenes in flight at once (agx_render_batch), parallel graph branches
+                                  (agx_render_batch_graph) */
     int32_t tuned;             /* 1: chosen by the autotuner */
     double  wall_ms;           /* whole batch, builds included */
     uint64_t rays;
-    double  tune_rays_per_s[4];/* autotuner probe throughput with 1, 2, 3, 4 workers (0 = not tried) */
+    double  tune_rays_per_s[4];/* autotuner probe throughput: agx_render_batch with 1, 2, 3, 4 workers;
+                                  agx_render_batch_graph with 2, 4, 8 branches, then 128-thread
+                                  traversal blocks (0 = not tried) */
+    /* agx_render_batch_graph only */
+    int32_t pass_paths_log2;   /* wavefront pass size of each scene: at most 2^this paths */
+    int32_t trace_block;       /* threads per block of the BVH4 traversal launches */
+    int32_t groups;            /* graphs launched (one per group of up to 8 scenes) */
+    int32_t build_threads;     /* host threads building BVHs */
+    double  build_wall_ms;     /* call start to the last scene built */
+    double  render_ms;         /* first graph launch to the last image copied out */
+    int32_t tune_configs;      /* configurations the autotuner timed */
+    int32_t reserved;
 } agx_batch_report;
 
 /* Version of this library's ABI (AGX_ABI_VERSION). */
@@ -393,18 +405,24 @@ int  agx_get_stats(agx_ctx *ctx, agx_stats_t *out);
 int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t workers,
                       agx_batch_report *report);
 
-/* Batched rendering in one graph launch (no counterpart in Atrc): every
- * item's scene is built and uploaded first (on `branches` host threads),
- * each into a context of its own with its path buffers allocated, then the
- * renders of all items are captured into one hipGraph and launched once.
- * The graph has `branches` parallel branches (0 = 4); the costliest scenes
- * are dealt first, each onto the least loaded branch, and a branch renders
- * its scenes one after another.  A captured render launches every bounce
- * up to the depth bound (kernels of an empty queue exit at once) and counts
- * its rays on the device.  Images are bit-identical to agx_render's for
- * each item alone.  item.worker = the item's branch; item.render_ms = the
- * graph's run time.  Memory: every scene and its buffers are resident at
- * once (passes of at most 2^22 paths per scene). */
+/* Batched rendering in hipGraph launches (no counterpart in Atrc).  Items are
+ * taken cheapest first (pixels x spp + triangles) in groups of 8.  Host
+ * threads (one per core, at most 16) build and upload the scenes, each into
+ * a context of its own, in that order; as soon as a group is built, its
+ * renders are captured into one hipGraph with `branches` parallel branches
+ * (the costliest scene first onto the least loaded branch; a branch renders
+ * its scenes one after another) and launched, so later scenes build while
+ * earlier groups render.  A finished group's images are copied out and its
+ * contexts released while the next group renders.  A captured render
+ * launches every bounce up to the depth bound (kernels of an empty queue
+ * exit at once) and counts its rays on the device.  branches = 0: an
+ * autotuner times the first group at 2 spp for 2, 4 and 8 branches, passes
+ * of 2^21 / 2^22 / 2^23 paths and 128- / 256-thread traversal blocks
+ * (coordinate descent; later groups build meanwhile) and renders the batch
+ * with the fastest; otherwise `branches` branches, passes of at most 2^22
+ * paths and 256-thread blocks.  Images are bit-identical to agx_render's for
+ * each item alone.  item.worker = the item's branch; item.render_ms = its
+ * group's graph time. */
 int  agx_render_batch_graph(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t branches,
                             agx_batch_report *report);
 

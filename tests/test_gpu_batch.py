@@ -93,3 +93,54 @@ def test_batch_graph_matches_single_renders(gpu_ctx):
         # eager work between the launches
         gpu_ctx.upload_scene(descs[0])
         np.testing.assert_array_equal(gpu_ctx.render(params[0], aovs=False)["image"], alone[0])
+
+
+def test_batch_graph_autotuned_and_grouped(gpu_ctx):
+    """agx_render_batch_graph with branches = 0: the autotuner times branch
+    counts, pass sizes and traversal block sizes on the first group of 8, then
+    the batch (here 11 items: two groups, built while the first renders) gives
+    every item the image agx_render gives it alone."""
+    items = _items()
+    s, cam, tr = scenes.cornell_box()
+    for k in range(11 - len(items)):
+        items.append((s, scenes.render_params(24 + 4 * k, 24, 2 + k % 3, cam, seed=60 + k, **tr)))
+    descs, keep, params = [], [], []
+    for sc, p in items:
+        d, kk = sc.to_desc()
+        descs.append(d)
+        keep.append(kk)
+        params.append(p)
+    alone = []
+    for d, p in zip(descs, params):
+        gpu_ctx.upload_scene(d)
+        alone.append(gpu_ctx.render(p, aovs=False)["image"])
+    imgs, recs, rep = gpu_ctx.render_batch(descs, params, workers=0, graph=True)
+    log(f"[batch graph] tuned: {rep}")
+    assert rep["tuned"] and rep["tune_configs"] >= 6
+    assert rep["workers"] in (2, 4, 8) and rep["pass_paths_log2"] in (21, 22, 23) and rep["trace_block"] in (128, 256)
+    assert rep["groups"] == 2 and rep["build_threads"] >= 1
+    for i, (a, b, r) in enumerate(zip(alone, imgs, recs)):
+        assert r["status"] == capi.AGX_OK
+        np.testing.assert_array_equal(a, b, err_msg=f"item {i}")
+    assert rep["rays"] == sum(r["rays"] for r in recs)
+
+
+def test_traversal_block_128_matches_256(gpu_ctx, monkeypatch):
+    """128-thread traversal blocks (an autotuner choice; AGX_TRACE_BLOCK=128 at
+    upload selects them) give the image of the default 256-thread blocks bit
+    for bit (BVH4 mode), with the same ray counts."""
+    s, cam, tr = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=5)
+    s.accel = capi.ACCEL_BVH4
+    d, keep = s.to_desc()
+    p = scenes.render_params(64, 48, 4, cam, seed=9, **tr)
+    gpu_ctx.upload_scene(d)
+    a = gpu_ctx.render(p, aovs=False)["image"]
+    sa = gpu_ctx.stats()
+    monkeypatch.setenv("AGX_TRACE_BLOCK", "128")
+    gpu_ctx.upload_scene(d)
+    b = gpu_ctx.render(p, aovs=False)["image"]
+    sb = gpu_ctx.stats()
+    monkeypatch.delenv("AGX_TRACE_BLOCK")
+    gpu_ctx.upload_scene(d)
+    np.testing.assert_array_equal(a, b)
+    assert (sa["closest_rays"], sa["shadow_rays"]) == (sb["closest_rays"], sb["shadow_rays"])
