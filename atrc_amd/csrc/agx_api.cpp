@@ -96,7 +96,7 @@ struct agx_ctx {
     uint32_t *g_count = nullptr;
     uint32_t *g_gather_blk = nullptr;  // per-block counts / offsets of the sample gather
     uint32_t *g_ring = nullptr;      // sample ring (next slot, valid samples) over g_samples
-    __bf16 *g_acts = nullptr, *g_dy = nullptr;  // matrix-core training: activations, output gradients
+    __bf16 *g_acts = nullptr, *g_dy = nullptr;  // matrix-core training: H_1..H_4, dY_1..dY_5 (per chunk, feature-major)
     uint32_t g_ring_cap = 0;         // capacity the ring state refers to (0: empty)
     uint32_t g_batch_cap = 0;
     uint64_t g_steps = 0;
@@ -186,7 +186,7 @@ struct agx_ctx {
             HIP_TRY(galloc(guide_allocs, &g_grad, G_PARAMS));
             HIP_TRY(galloc(guide_allocs, &g_partial, size_t(GUIDE_TRAIN_BLOCKS) * G_PARAMS));
             HIP_TRY(galloc(guide_allocs, &g_bias, G_BIASES));
-            HIP_TRY(galloc(guide_allocs, &g_packed, G_PACK));
+            HIP_TRY(galloc(guide_allocs, &g_packed, G_PACK_ALL));
             HIP_TRY(galloc(guide_allocs, &g_count, 4));
             HIP_TRY(galloc(guide_allocs, &g_gather_blk, GUIDE_GATHER_BLOCKS));
             HIP_TRY(galloc(guide_allocs, &g_wsum, 4));
@@ -222,8 +222,8 @@ struct agx_ctx {
     int ensure_guide_batch(uint32_t batch) {
         if (batch > g_batch_cap) {
             HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
-            HIP_TRY(galloc(guide_allocs, &g_acts, size_t(batch) * 4 * G_HID));
-            HIP_TRY(galloc(guide_allocs, &g_dy, size_t(batch) * 2 * G_HID));
+            HIP_TRY(galloc(guide_allocs, &g_acts, g_train_h_elems(batch)));
+            HIP_TRY(galloc(guide_allocs, &g_dy, g_train_dy_elems(batch)));
             g_batch_cap = batch;
         }
         if (g_ring_cap != batch) {
@@ -737,10 +737,10 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
                 launch_guide_train_mfma(c->g_params, c->g_packed, c->g_bias, c->g_samples, c->g_ring + 1, guide_batch,
                                         c->g_wsum, c->g_acts, c->g_dy, c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS,
                                         c->stream);
-                launch_guide_reduce(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_grad, c->stream);
                 c->g_steps += 1;
-                launch_guide_adam(c->g_params, c->g_m, c->g_v, c->g_grad, guide_lr, 0.9f, 0.99f, 1e-8f,
-                                  int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream, c->g_packed, c->g_bias);
+                launch_guide_reduce_adam(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_params, c->g_m, c->g_v,
+                                         guide_lr, 0.9f, 0.99f, 1e-8f, int(std::min<uint64_t>(c->g_steps, 1u << 30)),
+                                         c->stream, c->g_packed, c->g_bias);
             }
             HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipMemcpyAsync(&guide_valid_h, c->g_ring + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1585,7 +1585,7 @@ int agx_debug_guide_infer(agx_ctx *c, const float *params, const float *inputs, 
                                            std::to_string(guide_smem_bytes()) + " B needed)");
     DevBuf dp, dpack, dbias, din, dout;
     HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
-    HIP_TRY(hipMalloc(&dpack.p, G_PACK * 2));
+    HIP_TRY(hipMalloc(&dpack.p, G_PACK_ALL * 2));
     HIP_TRY(hipMalloc(&dbias.p, G_BIASES * sizeof(float)));
     HIP_TRY(hipMalloc(&din.p, size_t(n) * G_IN * sizeof(float)));
     HIP_TRY(hipMalloc(&dout.p, size_t(n) * G_ROW_OUT * sizeof(float)));
@@ -1634,17 +1634,17 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
     DevBuf dpk, dbi, dpk2, dbi2;
     if (lr != 0.0f) {
         // the renders' Adam also repacks the BF16 weights: check it against the packing kernel
-        HIP_TRY(hipMalloc(&dpk.p, G_PACK * sizeof(__bf16)));
+        HIP_TRY(hipMalloc(&dpk.p, G_PACK_ALL * sizeof(__bf16)));
         HIP_TRY(hipMalloc(&dbi.p, G_BIASES * sizeof(float)));
-        HIP_TRY(hipMalloc(&dpk2.p, G_PACK * sizeof(__bf16)));
+        HIP_TRY(hipMalloc(&dpk2.p, G_PACK_ALL * sizeof(__bf16)));
         HIP_TRY(hipMalloc(&dbi2.p, G_BIASES * sizeof(float)));
         launch_guide_adam(dp.get<float>(), dm.get<float>(), dv.get<float>(), dg.get<float>(), lr, 0.9f, 0.99f, 1e-8f,
                           t, c->stream, dpk.get<__bf16>(), dbi.get<float>());
         launch_guide_pack(dp.get<float>(), dpk2.get<__bf16>(), dbi2.get<float>(), c->stream);
-        std::vector<uint16_t> a(G_PACK), b(G_PACK);
+        std::vector<uint16_t> a(G_PACK_ALL), b(G_PACK_ALL);
         std::vector<float> ba(G_BIASES), bb(G_BIASES);
-        HIP_TRY(hipMemcpyAsync(a.data(), dpk.p, G_PACK * 2, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(b.data(), dpk2.p, G_PACK * 2, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(a.data(), dpk.p, G_PACK_ALL * 2, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(b.data(), dpk2.p, G_PACK_ALL * 2, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipMemcpyAsync(ba.data(), dbi.p, G_BIASES * 4, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipMemcpyAsync(bb.data(), dbi2.p, G_BIASES * 4, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1670,11 +1670,11 @@ int agx_debug_guide_grad_mfma(agx_ctx *c, const float *params, const float *samp
     const uint32_t cap = std::max<uint32_t>(n, 1);
     DevBuf dp, dpack, dbias, ds, dacts, ddy, dpart, dg, dscal;
     HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
-    HIP_TRY(hipMalloc(&dpack.p, G_PACK * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&dpack.p, G_PACK_ALL * sizeof(__bf16)));
     HIP_TRY(hipMalloc(&dbias.p, G_BIASES * sizeof(float)));
     HIP_TRY(hipMalloc(&ds.p, size_t(cap) * (G_IN + 4) * sizeof(float)));
-    HIP_TRY(hipMalloc(&dacts.p, size_t(cap) * 4 * G_HID * sizeof(__bf16)));
-    HIP_TRY(hipMalloc(&ddy.p, size_t(cap) * 2 * G_HID * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&dacts.p, g_train_h_elems(cap) * sizeof(__bf16)));
+    HIP_TRY(hipMalloc(&ddy.p, g_train_dy_elems(cap) * sizeof(__bf16)));
     HIP_TRY(hipMalloc(&dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dg.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dscal.p, 2 * sizeof(float)));

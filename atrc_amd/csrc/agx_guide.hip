@@ -17,8 +17,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr float PI_G = 3.14159265358979323846f;
-constexpr int ACT_STRIDE = G_HID + 8;      // bf16 per staged activation row (16-byte aligned rows)
-constexpr int OST_STRIDE = G_OUT + 1;      // floats per staged output row
 
 // v_mfma_f32_16x16x32_bf16 operand layouts (wave64):
 //   A (16 x 32): lane l holds row l % 16, k = 8 * (l / 16) + i, i = 0..7
@@ -70,60 +68,23 @@ __global__ void __launch_bounds__(256) k_guide_pack(const float *params, __bf16 
         const int k = 32 * s + g_kperm(l, lane / 16, i), n = 16 * t + (lane % 16);
         packed[gid] = __bf16(params[g_w_off(l) + k * g_out(l) + n]);
     }
+    if (gid < G_PACK_BWD) {
+        // packed[G_PACK + off_bwd(l) + ((m * S + s) * 64 + lane) * 8 + i] =
+        //     W_l[16 m + lane % 16][32 s + g_kperm(l, lane / 16, i)], S = outputs / 32
+        int l = 1;
+        while (l + 1 < G_LAYERS && gid >= g_pack_bwd_off(l + 1)) ++l;
+        const int local = gid - g_pack_bwd_off(l);
+        const int i = local & 7, lane = (local >> 3) & 63, frag = local >> 9;
+        const int S = g_out(l) / 32;
+        const int m = frag / S, sk = frag % S;
+        const int k = 16 * m + lane % 16, n = 32 * sk + g_kperm(l, lane / 16, i);
+        packed[G_PACK + gid] = __bf16(params[g_w_off(l) + k * g_out(l) + n]);
+    }
     if (gid < G_BIASES) {
         int l = 0;
         while (l + 1 < G_LAYERS && gid >= g_bias_off(l + 1)) ++l;
         bias[gid] = params[g_b_off(l) + gid - g_bias_off(l)];
     }
-}
-
-struct GuideSmem {
-    __bf16 w[G_PACK];
-    float b[G_BIASES];
-    __bf16 act[4][16][ACT_STRIDE];
-    float ost[4][16][OST_STRIDE];
-};
-
-// Hidden layer l (64 -> 64 or 64 -> 32) for the 16 rows staged in act[wave]:
-// reads the activations, then (after the wave has read them) overwrites them
-// with ReLU(x W + b), or writes the raw outputs to ost for the last layer.
-// The A fragment (16 rows x 32 inputs) of k-step s from a staged activation
-// row, in the packed weights' input order (g_kperm).
-__device__ __forceinline__ bf16x8 act_frag(const __bf16 *row, int s, int g) {
-    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-    const bf16x4 lo = *reinterpret_cast<const bf16x4 *>(row + 32 * s + 4 * g);
-    const bf16x4 hi = *reinterpret_cast<const bf16x4 *>(row + 32 * s + 16 + 4 * g);
-    return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-}
-
-template <int L>
-__device__ __forceinline__ void hidden_layer(GuideSmem &sm, int wave, int lane) {
-    constexpr int T = g_out(L) / 16;
-    const int r = lane % 16, g = lane / 16;
-    const bf16x8 a0 = act_frag(&sm.act[wave][r][0], 0, g);
-    const bf16x8 a1 = act_frag(&sm.act[wave][r][0], 1, g);
-    f32x4 acc[T];
-    #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        acc[t] = f32x4{0, 0, 0, 0};
-        const bf16x8 *w0 = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(L) + ((0 * T + t) * 64 + lane) * 8]);
-        const bf16x8 *w1 = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(L) + ((1 * T + t) * 64 + lane) * 8]);
-        acc[t] = mfma(a0, *w0, acc[t]);
-        acc[t] = mfma(a1, *w1, acc[t]);
-    }
-    wave_sync();
-    #pragma unroll
-    for (int t = 0; t < T; ++t) {
-        const int col = 16 * t + r;
-        const float bias = sm.b[g_bias_off(L) + col];
-        #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float v = acc[t][j] + bias;
-            if (L == G_LAYERS - 1) sm.ost[wave][4 * g + j][col] = v;
-            else sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
-        }
-    }
-    wave_sync();
 }
 
 // Batched inference: row i = in[8 i .. 8 i + 7] -> out[40 i ..]: 8 lobes of
@@ -147,9 +108,6 @@ struct InferSmem {
     float b[G_BIASES];
 };
 
-// wz is zero, laundered once per tile: the weights in LDS do not change over
-// the tile loop, and without it the compiler hoists every fragment load out of
-// the loop and spills the 240 registers they take
 typedef short s16x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -199,8 +157,9 @@ __device__ __forceinline__ bf16x8 one_blob8(float x, int g) {
 
 __device__ __forceinline__ float sigmoid_fast(float x) { return __builtin_amdgcn_rcpf(1.0f + __expf(-x)); }
 
-__device__ __forceinline__ bf16x8 wfrag(const InferSmem &sm, int l, int s, int t, int lane, int wz) {
-    return *reinterpret_cast<const bf16x8 *>(&sm.w[wz + g_pack_off(l) + ((s * (g_out(l) / 16) + t) * 64 + lane) * 8]);
+// Forward weight fragment (A operand of W^T x^T) of layer l, k-step s, output tile t.
+__device__ __forceinline__ bf16x8 wfrag(const __bf16 *w, int l, int s, int t, int lane) {
+    return *reinterpret_cast<const bf16x8 *>(&w[g_pack_off(l) + ((s * (g_out(l) / 16) + t) * 64 + lane) * 8]);
 }
 
 // ReLU and BF16 rounding of a 64-output layer's fragments (bias already in
@@ -221,17 +180,17 @@ __device__ __forceinline__ void relu_frags(const f32x4 (&acc)[IR][4], bf16x8 (&h
 // 64 -> T * 16 outputs for the IR sub-tiles; the first k-step's accumulator
 // input is the bias (lane group g: outputs 16 t + 4 g .. + 3)
 template <int L, int IR>
-__device__ __forceinline__ void chained_layer(const InferSmem &sm, int lane, int wz, const bf16x8 (&h)[IR][2],
+__device__ __forceinline__ void chained_layer(const __bf16 *W, const float *B, int lane, const bf16x8 (&h)[IR][2],
                                               f32x4 (&acc)[IR][4]) {
     constexpr int T = g_out(L) / 16;
     f32x4 b[T];
     #pragma unroll
-    for (int t = 0; t < T; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&sm.b[wz + g_bias_off(L) + 16 * t + 4 * (lane / 16)]);
+    for (int t = 0; t < T; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&B[g_bias_off(L) + 16 * t + 4 * (lane / 16)]);
     #pragma unroll
     for (int s = 0; s < 2; ++s) {
         #pragma unroll
         for (int t = 0; t < T; ++t) {
-            const bf16x8 w = wfrag(sm, L, s, t, lane, wz);
+            const bf16x8 w = wfrag(W, L, s, t, lane);
             #pragma unroll
             for (int q = 0; q < IR; ++q) acc[q][t] = mfma(w, h[q][s], s == 0 ? b[t] : acc[q][t]);
         }
@@ -256,15 +215,20 @@ __global__ void __launch_bounds__(IW * 64, 4) k_guide_infer(const __bf16 *packed
     __syncthreads();
     const int r = lane % 16, g = lane / 16;
     for (uint32_t tile = blockIdx.x * IW + uint32_t(wave); tile * ROWS < n; tile += gridDim.x * IW) {
+        // wz is zero, laundered once per tile: the weights in LDS do not change over
+        // the tile loop, and without it the compiler hoists every fragment load out
+        // of the loop and spills the 240 registers they take
         int wz = 0;
         asm volatile("" : "+s"(wz));
+        const __bf16 *W = sm.w + wz;
+        const float *B = sm.b + wz;
         // layer 0: k-step s covers the 32 bins of input s; lane group g makes bins 8 g .. 8 g + 7;
         // the first k-step's accumulator input is the bias
         f32x4 acc[IR][4];
         {
             f32x4 b[4];
             #pragma unroll
-            for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&sm.b[wz + g_bias_off(0) + 16 * t + 4 * g]);
+            for (int t = 0; t < 4; ++t) b[t] = *reinterpret_cast<const f32x4 *>(&B[g_bias_off(0) + 16 * t + 4 * g]);
             #pragma unroll
             for (int q = 0; q < IR; ++q)
                 #pragma unroll
@@ -280,7 +244,7 @@ __global__ void __launch_bounds__(IW * 64, 4) k_guide_infer(const __bf16 *packed
             }
             bf16x8 w[4];
             #pragma unroll
-            for (int t = 0; t < 4; ++t) w[t] = wfrag(sm, 0, s, t, lane, wz);
+            for (int t = 0; t < 4; ++t) w[t] = wfrag(W, 0, s, t, lane);
             #pragma unroll
             for (int q = 0; q < IR; ++q) {
                 const bf16x8 f = one_blob8(x[q], g);
@@ -290,13 +254,13 @@ __global__ void __launch_bounds__(IW * 64, 4) k_guide_infer(const __bf16 *packed
         }
         bf16x8 h[IR][2];
         relu_frags<IR>(acc, h);
-        chained_layer<1, IR>(sm, lane, wz, h, acc);
+        chained_layer<1, IR>(W, B, lane, h, acc);
         relu_frags<IR>(acc, h);
-        chained_layer<2, IR>(sm, lane, wz, h, acc);
+        chained_layer<2, IR>(W, B, lane, h, acc);
         relu_frags<IR>(acc, h);
-        chained_layer<3, IR>(sm, lane, wz, h, acc);
+        chained_layer<3, IR>(W, B, lane, h, acc);
         relu_frags<IR>(acc, h);
-        chained_layer<4, IR>(sm, lane, wz, h, acc);
+        chained_layer<4, IR>(W, B, lane, h, acc);
         // decode: output tile t of lane group g holds lobe 4 t + g of row r, components
         // (weight logit, theta, phi, log kappa)
         #pragma unroll
@@ -491,216 +455,325 @@ __global__ void __launch_bounds__(256) k_guide_train(const float *params, const 
 }
 
 // ---------------------------------------------------------------- matrix-core training
-// The same loss and gradient as k_guide_train, layer by layer on the matrix
-// cores: BF16 operands (weights, activations, output gradients), FP32
-// accumulation and FP32 master weights.  One step =
-//   k_gtrain_forward: forward of every sample (activations H1..H4 to HBM) and
-//     d loss / d outputs (dY5);
-//   k_gtrain_backward<L>, L = 4..0: dW_L = H_L^T dY_{L+1} and db_L = sum dY_{L+1}
-//     per block over its samples (partial[block]), and dY_L = (dY_{L+1} W_L^T)
-//     masked by ReLU'(H_L) for the layer below;
-//   then the fixed-order reduction over blocks and Adam, as before.
-// Activations and gradients: bf16 [sample][64] rows (dY5 uses 32 of them).
-constexpr int GT_CHUNK = 32;                     // samples per MFMA k-step in dW
-constexpr int GT_PAD = 8;                        // bf16 padding of LDS rows
+// One Adam step over the sample ring takes three kernels and no host wait:
+//   k_gtrain_fb: per 16-sample tile, the forward pass through all five layers
+//     and the back-propagation dY_5 -> dY_1 with activations and gradients
+//     kept in registers (the forward's chaining, with the weights packed again
+//     as the A operand of W_l dY^T); it writes H_1..H_4 and dY_1..dY_5 BF16,
+//     feature-major per chunk of 32 samples, through a per-wave LDS transpose;
+//   k_gtrain_dw: dW_l = H_l^T dY_(l+1) and db_l for every layer, split over
+//     the chunks: each block sums its chunks' products (one MFMA per 16 x 16
+//     tile and chunk, 16-byte operand loads; layer 0's one-blob features
+//     rebuilt in LDS per chunk) and writes its partial gradient;
+//   then the fixed-order reduction over blocks and Adam.
+// dY_(l+1) = d loss / d (H_l W_l + b_l); dY_l = (dY_(l+1) W_l^T) masked by
+// ReLU'(H_l), H_0 = the one-blob features.
+constexpr int TW = 16;  // waves per k_gtrain_fb block
 
-__global__ void __launch_bounds__(256) k_gtrain_forward(const __bf16 *packed, const float *bias, const float *samples,
-                                                        const uint32_t *n_dev, uint32_t cap, const float *wsum_dev,
-                                                        __bf16 *acts, __bf16 *dy5) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    GuideSmem &sm = *reinterpret_cast<GuideSmem *>(smem_raw);
-    const uint32_t n = min(*n_dev, cap);
-    const float wsum = *wsum_dev;
-    const float wscale = wsum > 0.0f ? 1.0f / wsum : 0.0f;
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    if (uint32_t(blockIdx.x) * 64u >= n) return;
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(packed);
-        uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
-        for (int i = tid; i < G_PACK / 8; i += 256) dst[i] = src[i];
-        for (int i = tid; i < G_BIASES; i += 256) sm.b[i] = bias[i];
-    }
-    __syncthreads();
+struct FbSmem {
+    __bf16 w[G_PACK_ALL];      // forward fragments, then the back-propagation fragments
+    float b[G_BIASES];
+    __bf16 stage[TW][G_HID][16];  // per-wave transpose: [feature][sample of the tile]
+};
+
+// Back-propagation weight fragment (A operand of W_l dY^T): inputs 16 m .., k-step s over outputs.
+__device__ __forceinline__ bf16x8 wbfrag(const __bf16 *w, int l, int m, int s, int lane) {
+    return *reinterpret_cast<const bf16x8 *>(&w[G_PACK + g_pack_bwd_off(l) + ((m * (g_out(l) / 32) + s) * 64 + lane) * 8]);
+}
+
+// A 64-feature tile held as two B fragments (sample lane % 16, features in
+// g_kperm order) -> dst[f * 32 + k0 + sample], f = 0..63, through the wave's
+// staging area; rows_valid = 0 skips the global stores.
+__device__ __forceinline__ void store_tile64(__bf16 (*st)[16], const bf16x8 (&h)[2], int lane, __bf16 *dst, int k0) {
     const int r = lane % 16, g = lane / 16;
-    for (uint32_t tile = blockIdx.x * 4u + uint32_t(wave); tile * 16u < n; tile += gridDim.x * 4u) {
-        const uint32_t row = tile * 16u + uint32_t(r);
-        const bool valid = row < n;
-        float x[G_IN];
+    #pragma unroll
+    for (int s = 0; s < 2; ++s)
         #pragma unroll
-        for (int k = 0; k < G_IN; ++k) {
-            const float v = valid ? samples[size_t(row) * G_SAMPLE + k] : 0.0f;
-            x[k] = fminf(1.0f, fmaxf(0.0f, v));
-        }
-        f32x4 acc[4];
-        #pragma unroll
-        for (int t = 0; t < 4; ++t) acc[t] = f32x4{0, 0, 0, 0};
-        #pragma unroll
-        for (int k = 0; k < G_IN; ++k) {
-            bf16x8 a;
-            #pragma unroll
-            for (int i = 0; i < 8; ++i) a[i] = __bf16(one_blob(x[k], 8 * g + i));
-            #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const bf16x8 *w = reinterpret_cast<const bf16x8 *>(&sm.w[g_pack_off(0) + ((k * 4 + t) * 64 + lane) * 8]);
-                acc[t] = mfma(a, *w, acc[t]);
-            }
-        }
-        #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int col = 16 * t + r;
-            const float b0 = sm.b[g_bias_off(0) + col];
-            #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = acc[t][j] + b0;
-                sm.act[wave][4 * g + j][col] = __bf16(v > 0.0f ? v : 0.0f);
-            }
-        }
-        wave_sync();
-        // H_l (input of layer l) for l = 1..4 goes to HBM before layer l overwrites it
-        auto store_act = [&](int l) {
-            const uint32_t base = tile * 16u;
-            for (int e = lane; e < 16 * G_HID / 8; e += 64) {
-                const int rr = e / (G_HID / 8), c8 = e % (G_HID / 8);
-                if (base + uint32_t(rr) < n)
-                    *reinterpret_cast<uint4 *>(acts + (size_t(l - 1) * cap + base + rr) * G_HID + 8 * c8) =
-                        *reinterpret_cast<const uint4 *>(&sm.act[wave][rr][8 * c8]);
-            }
-        };
-        store_act(1);
-        hidden_layer<1>(sm, wave, lane);
-        store_act(2);
-        hidden_layer<2>(sm, wave, lane);
-        store_act(3);
-        hidden_layer<3>(sm, wave, lane);
-        store_act(4);
-        hidden_layer<4>(sm, wave, lane);
-        // d loss / d outputs of row r (one lane per row)
-        if (g == 0 && valid) {
-            const float *smp = samples + size_t(row) * G_SAMPLE;
-            float gr[G_OUT];
-            const float w = smp[G_IN + 3] * wscale;
-            if (w != 0.0f) mixture_grad(&sm.ost[wave][r][0], smp + G_IN, w, gr);
-            else for (int j = 0; j < G_OUT; ++j) gr[j] = 0.0f;
-            for (int j = 0; j < G_OUT; ++j) dy5[size_t(row) * G_HID + j] = __bf16(gr[j]);
-        }
-        wave_sync();
+        for (int i = 0; i < 8; ++i) st[32 * s + g_kperm(1, g, i)][r] = h[s][i];
+    wave_sync();
+    #pragma unroll
+    for (int q = lane; q < 128; q += 64) {
+        const int f = q / 2, hh = q % 2;
+        *reinterpret_cast<uint4 *>(dst + f * G_CHUNK + k0 + 8 * hh) = *reinterpret_cast<const uint4 *>(&st[f][8 * hh]);
+    }
+    wave_sync();
+}
+
+// d(-w log q(dir)) / d(raw outputs) with the 8 lobes of a row spread over its
+// four lanes (lane group g holds lobes g and 4 + g: o[t] = outputs 4 (4 t + g)
+// .. + 3); the sums and maxima over lobes are taken across the four lanes.
+// The per-lobe terms are mixture_grad's, with the hardware exponential,
+// logarithm, sine, cosine and reciprocal, and log kappa taken from the clamped
+// output itself.
+__device__ void mixture_grad_lanes(const f32x4 (&o)[2], const float *dir, float w, float (&gr)[2][4]) {
+    float lm = fmaxf(o[0][0], o[1][0]);
+    lm = fmaxf(lm, __shfl_xor(lm, 16));
+    lm = fmaxf(lm, __shfl_xor(lm, 32));
+    float wj[2];
+    wj[0] = __expf(o[0][0] - lm);
+    wj[1] = __expf(o[1][0] - lm);
+    float ws = wj[0] + wj[1];
+    ws += __shfl_xor(ws, 16);
+    ws += __shfl_xor(ws, 32);
+    const float inv_ws = __builtin_amdgcn_rcpf(ws);
+    float lt[2], kap[2], cosm[2], s1[2], s2[2], mdt[2], mdp[2], om[2];
+    #pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        wj[t] = wj[t] * inv_ws;
+        s1[t] = sigmoid_fast(o[t][1]);
+        s2[t] = sigmoid_fast(o[t][2]);
+        const float th = PI_G * s1[t], ph = 2.0f * PI_G * s2[t];
+        const float lk = fminf(G_LOGK_MAX, fmaxf(G_LOGK_MIN, o[t][3]));
+        kap[t] = __expf(lk);
+        float st, ct, sp, cp;
+        __sincosf(th, &st, &ct);
+        __sincosf(ph, &sp, &cp);
+        cosm[t] = st * cp * dir[0] + st * sp * dir[1] + ct * dir[2];
+        mdt[t] = ct * cp * dir[0] + ct * sp * dir[1] - st * dir[2];
+        mdp[t] = -st * sp * dir[0] + st * cp * dir[1];
+        om[t] = 1.0f - __expf(-2.0f * kap[t]);  // 1 - e^(-2 kappa) >= 0.035 for kappa >= e^-4
+        lt[t] = __logf(fmaxf(wj[t], 1e-30f)) + (lk - 1.8378770664093453f - __logf(om[t])) + kap[t] * (cosm[t] - 1.0f);
+    }
+    float lmax = fmaxf(lt[0], lt[1]);
+    lmax = fmaxf(lmax, __shfl_xor(lmax, 16));
+    lmax = fmaxf(lmax, __shfl_xor(lmax, 32));
+    float qs = __expf(lt[0] - lmax) + __expf(lt[1] - lmax);
+    qs += __shfl_xor(qs, 16);
+    qs += __shfl_xor(qs, 32);
+    const float logq = lmax + __logf(qs);
+    #pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const float r = __expf(lt[t] - logq);
+        gr[t][0] = -w * (r - wj[t]);
+        gr[t][1] = -w * r * kap[t] * mdt[t] * PI_G * s1[t] * (1.0f - s1[t]);
+        gr[t][2] = -w * r * kap[t] * mdp[t] * 2.0f * PI_G * s2[t] * (1.0f - s2[t]);
+        const float lk_raw = o[t][3];
+        const bool free_k = lk_raw > G_LOGK_MIN && lk_raw < G_LOGK_MAX;
+        const float dlogv = __builtin_amdgcn_rcpf(kap[t]) - 2.0f * (1.0f - om[t]) * __builtin_amdgcn_rcpf(om[t]) +
+                            (cosm[t] - 1.0f);
+        gr[t][3] = free_k ? -w * r * dlogv * kap[t] : 0.0f;
     }
 }
 
+// dY_l^T = W_l dY_(l+1)^T masked by ReLU'(H_l): dy_in = dY_(l+1) as B fragments
+// (K = outputs of layer l in g_kperm order), h = H_l as B fragments; out = dY_l
+// as B fragments of the layer below.
 template <int L>
-struct GtBwdSmem {
-    static constexpr int IN = g_in(L), OUT = g_out(L);
-    __bf16 w[IN][OUT + GT_PAD];                  // W_L row-major (input feature, output)
-    __bf16 ht[IN][GT_CHUNK + GT_PAD];            // input activations, feature-major
-    __bf16 hs[GT_CHUNK][(L > 0 ? IN : 8) + GT_PAD];  // input activations, sample-major (ReLU mask)
-    __bf16 ds[GT_CHUNK][OUT + GT_PAD];           // dY_{L+1}, sample-major
-    __bf16 dt[OUT][GT_CHUNK + GT_PAD];           // dY_{L+1}, feature-major
-};
+__device__ __forceinline__ void bwd_layer(const __bf16 *W, int lane, const bf16x8 (&dy_in)[2], const bf16x8 (&h)[2],
+                                          bf16x8 (&out)[2]) {
+    constexpr int S = g_out(L) / 32;
+    f32x4 acc[4];
+    #pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        acc[m] = f32x4{0, 0, 0, 0};
+        #pragma unroll
+        for (int s = 0; s < S; ++s) acc[m] = mfma(wbfrag(W, L, m, s, lane), dy_in[s], acc[m]);
+    }
+    // output tile m holds inputs 16 m + 4 g + j: element (m % 2) * 4 + j of h[m / 2]
+    #pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        float v[8];
+        #pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[j] = float(h[s][j]) > 0.0f ? acc[2 * s][j] : 0.0f;
+            v[4 + j] = float(h[s][4 + j]) > 0.0f ? acc[2 * s + 1][j] : 0.0f;
+        }
+        const u32x4 u = {bf16_pair(v[0], v[1]), bf16_pair(v[2], v[3]), bf16_pair(v[4], v[5]), bf16_pair(v[6], v[7])};
+        out[s] = __builtin_bit_cast(bf16x8, u);
+    }
+}
 
-template <int L>
-__global__ void __launch_bounds__(256) k_gtrain_backward(const float *params, const float *samples,
-                                                         const uint32_t *n_dev, uint32_t cap, const __bf16 *acts,
-                                                         const __bf16 *dy_in, __bf16 *dy_out, float *partial) {
-    using S = GtBwdSmem<L>;
-    constexpr int IN = S::IN, OUT = S::OUT;
-    constexpr int MT = IN / 16, NT = OUT / 16;            // dW tiles
-    constexpr int DW_PER_WAVE = (MT * NT) / 4;
+__global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, const float *bias, const float *samples,
+                                                       const uint32_t *n_dev, uint32_t cap, const float *wsum_dev,
+                                                       __bf16 *hT, __bf16 *dT) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    S &sm = *reinterpret_cast<S *>(smem_raw);
+    FbSmem &sm = *reinterpret_cast<FbSmem *>(smem_raw);
     const uint32_t n = min(*n_dev, cap);
+    const uint32_t tiles = 2u * uint32_t((n + G_CHUNK - 1) / G_CHUNK);  // whole chunks: rows past n get w = 0
+    const float wsum = *wsum_dev;
+    const float wscale = wsum > 0.0f ? 1.0f / wsum : 0.0f;
+    const size_t lstride = g_chunks(cap) * G_HID * G_CHUNK;  // one layer of hT / dT
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    if (uint32_t(blockIdx.x) * TW >= tiles) return;  // whole block past the tiles (block-uniform)
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(packed);
+        uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
+        for (int i = tid; i < G_PACK_ALL / 8; i += TW * 64) dst[i] = src[i];
+        for (int i = tid; i < G_BIASES; i += TW * 64) sm.b[i] = bias[i];
+    }
+    __syncthreads();
     const int r = lane % 16, g = lane / 16;
-    for (int i = tid; i < IN * OUT; i += 256) sm.w[i / OUT][i % OUT] = __bf16(params[g_w_off(L) + i]);
-    // this block's samples: [s_begin, s_end), whole chunks
-    const uint32_t chunks = (n + GT_CHUNK - 1) / GT_CHUNK;
+    __bf16 (*st)[16] = sm.stage[wave];
+    for (uint32_t tile = blockIdx.x * TW + uint32_t(wave); tile < tiles; tile += gridDim.x * TW) {
+        int wz = 0;
+        asm volatile("" : "+s"(wz));
+        const __bf16 *W = sm.w + wz;
+        const float *B = sm.b + wz;
+        const uint32_t row = tile * 16u + uint32_t(r);
+        const bool valid = row < n;
+        const size_t cbase = size_t(tile / 2) * G_HID * G_CHUNK;
+        const int k0 = 16 * int(tile % 2);
+        // forward
+        f32x4 acc[1][4];
+        #pragma unroll
+        for (int t = 0; t < 4; ++t) acc[0][t] = *reinterpret_cast<const f32x4 *>(&B[g_bias_off(0) + 16 * t + 4 * g]);
+        #pragma unroll 2
+        for (int s = 0; s < G_IN; ++s) {
+            const float x = valid ? fminf(1.0f, fmaxf(0.0f, samples[size_t(row) * G_SAMPLE + s])) : 0.0f;
+            const bf16x8 f = one_blob8(x, g);
+            #pragma unroll
+            for (int t = 0; t < 4; ++t) acc[0][t] = mfma(wfrag(W, 0, s, t, lane), f, acc[0][t]);
+        }
+        bf16x8 h1[1][2], h2[1][2], h3[1][2], h4[1][2];
+        relu_frags<1>(acc, h1);
+        store_tile64(st, h1[0], lane, hT + 0 * lstride + cbase, k0);
+        chained_layer<1, 1>(W, B, lane, h1, acc);
+        relu_frags<1>(acc, h2);
+        store_tile64(st, h2[0], lane, hT + 1 * lstride + cbase, k0);
+        chained_layer<2, 1>(W, B, lane, h2, acc);
+        relu_frags<1>(acc, h3);
+        store_tile64(st, h3[0], lane, hT + 2 * lstride + cbase, k0);
+        chained_layer<3, 1>(W, B, lane, h3, acc);
+        relu_frags<1>(acc, h4);
+        store_tile64(st, h4[0], lane, hT + 3 * lstride + cbase, k0);
+        chained_layer<4, 1>(W, B, lane, h4, acc);
+        // d loss / d outputs: lobes g and 4 + g of row r
+        float dir[3] = {0.0f, 0.0f, 1.0f};
+        float w = 0.0f;
+        if (valid) {
+            const float *smp = samples + size_t(row) * G_SAMPLE;
+            dir[0] = smp[G_IN]; dir[1] = smp[G_IN + 1]; dir[2] = smp[G_IN + 2];
+            w = smp[G_IN + 3] * wscale;
+        }
+        const f32x4 o[2] = {acc[0][0], acc[0][1]};
+        float gr[2][4];
+        mixture_grad_lanes(o, dir, w, gr);
+        if (w == 0.0f)
+            #pragma unroll
+            for (int t = 0; t < 2; ++t)
+                #pragma unroll
+                for (int c = 0; c < 4; ++c) gr[t][c] = 0.0f;
+        bf16x8 dy5[2];
+        {
+            const u32x4 u = {bf16_pair(gr[0][0], gr[0][1]), bf16_pair(gr[0][2], gr[0][3]), bf16_pair(gr[1][0], gr[1][1]),
+                             bf16_pair(gr[1][2], gr[1][3])};
+            dy5[0] = __builtin_bit_cast(bf16x8, u);
+            dy5[1] = dy5[0];  // unused: the output layer has one k-step
+            // dY_5: rows 16 t + 4 g + c of the chunk's 32-feature block
+            #pragma unroll
+            for (int i = 0; i < 8; ++i) st[g_kperm(1, g, i)][r] = dy5[0][i];
+            wave_sync();
+            const int f = lane / 2, hh = lane % 2;
+            *reinterpret_cast<uint4 *>(dT + 4 * lstride + cbase + f * G_CHUNK + k0 + 8 * hh) =
+                *reinterpret_cast<const uint4 *>(&st[f][8 * hh]);
+            wave_sync();
+        }
+        // back-propagation
+        bf16x8 d4[2], d3[2], d2[2], d1[2];
+        bwd_layer<4>(W, lane, dy5, h4[0], d4);
+        store_tile64(st, d4, lane, dT + 3 * lstride + cbase, k0);
+        bwd_layer<3>(W, lane, d4, h3[0], d3);
+        store_tile64(st, d3, lane, dT + 2 * lstride + cbase, k0);
+        bwd_layer<2>(W, lane, d3, h2[0], d2);
+        store_tile64(st, d2, lane, dT + 1 * lstride + cbase, k0);
+        bwd_layer<1>(W, lane, d2, h1[0], d1);
+        store_tile64(st, d1, lane, dT + 0 * lstride + cbase, k0);
+    }
+}
+
+// dW / db of every layer over the block's chunks.  Tiles of 16 inputs x 16
+// outputs, one MFMA per tile and chunk: waves 0..3 take input tiles 4 w ..
+// 4 w + 3 of layer 0 against its four output tiles; wave 4 + l' takes all of
+// layer l' + 1 (4 x 4 tiles, layer 4: 4 x 2), so each operand fragment is
+// loaded once per chunk.  db_l (the column sums of dY_(l+1) over the samples)
+// comes from MFMAs with an all-ones A operand (wave 0 for layer 0).
+constexpr int DW_WAVES = 8;
+constexpr int DW_BATCH = 8;  // chunks whose one-blob features are in LDS at once
+
+__global__ void __launch_bounds__(DW_WAVES * 64) k_gtrain_dw(const float *samples, const uint32_t *n_dev, uint32_t cap,
+                                                            const __bf16 *hT, const __bf16 *dT, float *partial) {
+    // one-blob features of up to DW_BATCH chunks, feature-major (128 KB)
+    __shared__ __align__(16) __bf16 x0[DW_BATCH][G_FEAT][G_CHUNK];
+    const uint32_t n = min(*n_dev, cap);
+    const uint32_t chunks = (n + G_CHUNK - 1) / G_CHUNK;
+    const size_t lstride = g_chunks(cap) * G_HID * G_CHUNK;
     const uint32_t per = (chunks + gridDim.x - 1) / gridDim.x;
     const uint32_t c_begin = blockIdx.x * per, c_end = min(chunks, c_begin + per);
-    f32x4 dw[DW_PER_WAVE];
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int r = lane % 16, g = lane / 16;
+    // this wave's layer and tile grid
+    const int L = wave < 4 ? 0 : wave - 3;
+    const int NT = g_out(L) / 16;                 // output tiles (4, or 2 for layer 4)
+    bf16x8 ones;
     #pragma unroll
-    for (int t = 0; t < DW_PER_WAVE; ++t) dw[t] = f32x4{0, 0, 0, 0};
-    float db = 0.0f;
-    for (uint32_t c = c_begin; c < c_end; ++c) {
-        const uint32_t s0 = c * GT_CHUNK;
-        __syncthreads();
-        // stage dY_{L+1} (both layouts) and the input activations
-        for (int e = tid; e < GT_CHUNK * OUT; e += 256) {
-            const int sI = e / OUT, o = e % OUT;
-            const __bf16 v = s0 + sI < n ? dy_in[size_t(s0 + sI) * G_HID + o] : __bf16(0.0f);
-            sm.ds[sI][o] = v;
-            sm.dt[o][sI] = v;
-        }
-        if constexpr (L == 0) {
-            for (int e = tid; e < GT_CHUNK * IN; e += 256) {
-                const int sI = e / IN, f = e % IN;
-                float v = 0.0f;
-                if (s0 + sI < n) {
-                    const float xin = fminf(1.0f, fmaxf(0.0f, samples[size_t(s0 + sI) * G_SAMPLE + f / G_BINS]));
-                    v = one_blob(xin, f % G_BINS);
-                }
-                sm.ht[f][sI] = __bf16(v);
-            }
-        } else {
-            for (int e = tid; e < GT_CHUNK * IN; e += 256) {
-                const int sI = e / IN, f = e % IN;
-                const __bf16 v = s0 + sI < n ? acts[(size_t(L - 1) * cap + s0 + sI) * G_HID + f] : __bf16(0.0f);
-                sm.ht[f][sI] = v;
-                sm.hs[sI][f] = v;
+    for (int i = 0; i < 8; ++i) ones[i] = __bf16(1.0f);
+    f32x4 acc[16], accb[4];
+    #pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = f32x4{0, 0, 0, 0};
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) accb[j] = f32x4{0, 0, 0, 0};
+    for (uint32_t cb = c_begin; cb < c_end; cb += DW_BATCH) {
+        const uint32_t nb = min(uint32_t(DW_BATCH), c_end - cb);
+        __syncthreads();  // the previous batch's features are read
+        // item (sample k, input s, bins 8 part ..): rows past n take input 0 (their dY is 0)
+        for (int it = tid; it < G_CHUNK * G_IN * 4; it += DW_WAVES * 64) {
+            const int k = it % G_CHUNK, s = (it / G_CHUNK) % G_IN, part = it / (G_CHUNK * G_IN);
+            for (uint32_t q = 0; q < nb; ++q) {
+                const uint32_t row = (cb + q) * G_CHUNK + uint32_t(k);
+                const float x = row < n ? fminf(1.0f, fmaxf(0.0f, samples[size_t(row) * G_SAMPLE + s])) : 0.0f;
+                const bf16x8 v = one_blob8(x, part);
+                #pragma unroll
+                for (int i = 0; i < 8; ++i) x0[q][32 * s + 8 * part + i][k] = v[i];
             }
         }
         __syncthreads();
-        // bias gradient: thread o sums its column over the chunk
-        if (tid < OUT)
-            for (int sI = 0; sI < GT_CHUNK; ++sI) db += float(sm.ds[sI][tid]);
-        // dW tiles of this wave: A = H^T (feature rows, sample k), B = dY (sample k, output cols)
-        #pragma unroll
-        for (int t = 0; t < DW_PER_WAVE; ++t) {
-            const int tile = wave * DW_PER_WAVE + t, mt = tile / NT, nt = tile % NT;
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sm.ht[16 * mt + r][8 * g]);
-            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sm.dt[16 * nt + r][8 * g]);
-            dw[t] = mfma(a, b, dw[t]);
-        }
-        if constexpr (L > 0) {
-            // dY_L = dY_{L+1} W_L^T masked by ReLU'(H_L): 2 x MT tiles of (16 samples x 16 features)
-            constexpr int DH_TILES = 2 * MT, DH_PER_WAVE = DH_TILES / 4;
+        for (uint32_t q = 0; q < nb; ++q) {
+            const size_t cbase = size_t(cb + q) * G_HID * G_CHUNK;
+            bf16x8 b[4], a[4];
             #pragma unroll
-            for (int t = 0; t < DH_PER_WAVE; ++t) {
-                const int tile = wave * DH_PER_WAVE + t, ms = tile / MT, ni = tile % MT;
-                f32x4 acc = f32x4{0, 0, 0, 0};
+            for (int nt = 0; nt < 4; ++nt)
+                if (nt < NT)
+                    b[nt] = *reinterpret_cast<const bf16x8 *>(dT + size_t(L) * lstride + cbase + (16 * nt + r) * G_CHUNK + 8 * g);
+            #pragma unroll
+            for (int m = 0; m < 4; ++m)
+                a[m] = wave < 4 ? *reinterpret_cast<const bf16x8 *>(&x0[q][16 * (4 * wave + m) + r][8 * g])
+                                : *reinterpret_cast<const bf16x8 *>(hT + size_t(L - 1) * lstride + cbase +
+                                                                    (16 * m + r) * G_CHUNK + 8 * g);
+            #pragma unroll
+            for (int m = 0; m < 4; ++m)
                 #pragma unroll
-                for (int kk = 0; kk < OUT / 32; ++kk) {
-                    const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&sm.ds[16 * ms + r][32 * kk + 8 * g]);
-                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sm.w[16 * ni + r][32 * kk + 8 * g]);
-                    acc = mfma(a, b, acc);
-                }
-                const int f = 16 * ni + r;
+                for (int nt = 0; nt < 4; ++nt)
+                    if (nt < NT) acc[4 * m + nt] = mfma(a[m], b[nt], acc[4 * m + nt]);
+            if (wave >= 3) {  // bias sums: wave 3 for layer 0, waves 4..7 for their layer
                 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int sI = 16 * ms + 4 * g + j;
-                    if (s0 + sI < n) {
-                        const float h = float(sm.hs[sI][f]);
-                        dy_out[size_t(s0 + sI) * G_HID + f] = __bf16(h > 0.0f ? acc[j] : 0.0f);
-                    }
-                }
+                for (int nt = 0; nt < 4; ++nt)
+                    if (nt < NT) accb[nt] = mfma(ones, b[nt], accb[nt]);
             }
         }
     }
-    // this block's partial gradient of layer L (zeros when it had no samples)
+    // this block's partial gradient (zeros when it had no chunks): tile rows 4 g + j, column r
     float *part = partial + size_t(blockIdx.x) * G_PARAMS;
+    const int m0 = wave < 4 ? 4 * wave : 0;
     #pragma unroll
-    for (int t = 0; t < DW_PER_WAVE; ++t) {
-        const int tile = wave * DW_PER_WAVE + t, mt = tile / NT, nt = tile % NT;
-        const int col = 16 * nt + r;
+    for (int m = 0; m < 4; ++m)
         #pragma unroll
-        for (int j = 0; j < 4; ++j) part[g_w_off(L) + (16 * mt + 4 * g + j) * OUT + col] = dw[t][j];
+        for (int nt = 0; nt < 4; ++nt)
+            if (nt < NT)
+                #pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    part[g_w_off(L) + (16 * (m0 + m) + 4 * g + j) * g_out(L) + 16 * nt + r] = acc[4 * m + nt][j];
+    if (wave >= 3 && g == 0) {  // every row of accb holds the column sums
+        #pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            if (nt < NT) part[g_b_off(L) + 16 * nt + r] = accb[nt][0];
     }
-    if (tid < OUT) part[g_b_off(L) + tid] = db;
 }
 
 // Sum of the per-block partial gradients in a fixed order.
 // one block: advance the ring, then a fixed-order sum of the valid weights
-__global__ void __launch_bounds__(256) k_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap,
-                                                           const float *samples, float *wsum) {
-    __shared__ float part[256];
+__global__ void __launch_bounds__(1024) k_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap,
+                                                            const float *samples, float *wsum) {
+    __shared__ float part[1024];
     __shared__ uint32_t valid_s;
     if (threadIdx.x == 0) {
         const uint32_t a = min(*added, cap);
@@ -710,11 +783,18 @@ __global__ void __launch_bounds__(256) k_guide_ring_commit(uint32_t *ring, const
     }
     __syncthreads();
     const uint32_t valid = valid_s;
-    float acc = 0.0f;
-    for (uint32_t i = threadIdx.x; i < valid; i += 256) acc += samples[size_t(i) * G_SAMPLE + G_IN + 3];
-    part[threadIdx.x] = acc;
+    // eight independent loads in flight per thread, added in a fixed order
+    float a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = threadIdx.x; i < valid; i += 8 * 1024) {
+        #pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t k = i + uint32_t(u) * 1024u;
+            if (k < valid) a[u] += samples[size_t(k) * G_SAMPLE + G_IN + 3];
+        }
+    }
+    part[threadIdx.x] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (int(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
         __syncthreads();
     }
@@ -745,16 +825,28 @@ __global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int 
     grad[i] = g;
 }
 
+// Adam step of parameter i (k_guide_adam's update and repacking)
+__device__ __forceinline__ void adam_update(int i, float g, float *params, float *m, float *v, float lr, float b1,
+                                            float b2, float eps, int t, __bf16 *packed, float *bias);
+
+// The second pass of the fixed-order reduction fused with Adam: gradient i =
+// the group sums of k_guide_reduce_groups added in group order.
+__global__ void __launch_bounds__(256) k_guide_reduce_adam(const float *partial, int blocks, int per, float *params,
+                                                           float *m, float *v, float lr, float b1, float b2, float eps,
+                                                           int t, __bf16 *packed, float *bias) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G_PARAMS) return;
+    float g = 0;
+    for (int b = 0; b < blocks; b += per) g += partial[size_t(b) * G_PARAMS + i];
+    adam_update(i, g, params, m, v, lr, b1, b2, eps, t, packed, bias);
+}
+
 // One Adam step on the FP32 master parameters: m, v moments; lr; bias
 // corrections for step t.  When packed is given, the updated parameter is
 // also written where k_guide_pack puts it (BF16 weight in fragment order, or
 // FP32 bias), so the next forward pass needs no separate packing launch.
-__global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, float *v, const float *grad, float lr,
-                                                    float b1, float b2, float eps, int t, __bf16 *packed,
-                                                    float *bias) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G_PARAMS) return;
-    const float g = grad[i];
+__device__ __forceinline__ void adam_update(int i, float g, float *params, float *m, float *v, float lr, float b1,
+                                            float b2, float eps, int t, __bf16 *packed, float *bias) {
     const float mi = b1 * m[i] + (1 - b1) * g;
     const float vi = b2 * v[i] + (1 - b2) * g * g;
     m[i] = mi;
@@ -777,11 +869,24 @@ __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, flo
     const int grp = l == 0 ? kk / 8 : (kk % 16) / 4;
     const int elem = l == 0 ? kk % 8 : (kk < 16 ? 0 : 4) + kk % 4;
     packed[g_pack_off(l) + ((sI * T + tI) * 64 + grp * 16 + n % 16) * 8 + elem] = __bf16(p);
+    if (l > 0) {  // the back-propagation fragments (k_guide_pack's second map)
+        const int m = k / 16, sk = n / 32, nn = n % 32;
+        const int gb = (nn % 16) / 4, eb = (nn < 16 ? 0 : 4) + nn % 4;
+        packed[G_PACK + g_pack_bwd_off(l) + ((m * (g_out(l) / 32) + sk) * 64 + gb * 16 + k % 16) * 8 + eb] = __bf16(p);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, float *v, const float *grad, float lr,
+                                                    float b1, float b2, float eps, int t, __bf16 *packed,
+                                                    float *bias) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G_PARAMS) return;
+    adam_update(i, grad[i], params, m, v, lr, b1, b2, eps, t, packed, bias);
 }
 
 }  // namespace
 
-size_t guide_smem_bytes() { return sizeof(InferSmem) > sizeof(GuideSmem) ? sizeof(InferSmem) : sizeof(GuideSmem); }
+size_t guide_smem_bytes() { return sizeof(InferSmem) > sizeof(FbSmem) ? sizeof(InferSmem) : sizeof(FbSmem); }
 size_t guide_train_smem_bytes() { return sizeof(TrainSmem); }
 
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
@@ -793,26 +898,18 @@ void launch_guide_train(const float *params, const float *samples, const uint32_
 void launch_guide_train_mfma(const float *params, const __bf16 *packed, const float *bias, const float *samples,
                              const uint32_t *n_dev, uint32_t cap, const float *wsum_dev, __bf16 *acts, __bf16 *dy,
                              float *partial, int blocks, hipStream_t st) {
-    // forward: enough persistent blocks to cover the ring's 16-row tiles
-    const uint32_t tiles = (cap + 63) / 64;
-    hipLaunchKernelGGL(k_gtrain_forward, dim3(std::min<uint32_t>(tiles, 512u)), dim3(256), sizeof(GuideSmem), st,
+    (void)params;
+    // forward + back-propagation: one 16-sample tile per wave, blocks of TW waves
+    const uint32_t tiles = uint32_t(2 * g_chunks(cap));
+    const uint32_t fb_blocks = (tiles + TW - 1) / TW;
+    hipLaunchKernelGGL(k_gtrain_fb, dim3(fb_blocks < 256u ? fb_blocks : 256u), dim3(TW * 64), sizeof(FbSmem), st,
                        packed, bias, samples, n_dev, cap, wsum_dev, acts, dy);
-    __bf16 *dy_a = dy, *dy_b = dy + size_t(cap) * G_HID;
-    hipLaunchKernelGGL(k_gtrain_backward<4>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<4>), st, params, samples, n_dev,
-                       cap, acts, dy_a, dy_b, partial);
-    hipLaunchKernelGGL(k_gtrain_backward<3>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<3>), st, params, samples, n_dev,
-                       cap, acts, dy_b, dy_a, partial);
-    hipLaunchKernelGGL(k_gtrain_backward<2>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<2>), st, params, samples, n_dev,
-                       cap, acts, dy_a, dy_b, partial);
-    hipLaunchKernelGGL(k_gtrain_backward<1>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<1>), st, params, samples, n_dev,
-                       cap, acts, dy_b, dy_a, partial);
-    hipLaunchKernelGGL(k_gtrain_backward<0>, dim3(blocks), dim3(256), sizeof(GtBwdSmem<0>), st, params, samples, n_dev,
-                       cap, acts, dy_a, nullptr, partial);
+    hipLaunchKernelGGL(k_gtrain_dw, dim3(blocks), dim3(DW_WAVES * 64), 0, st, samples, n_dev, cap, acts, dy, partial);
 }
 
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
                               float *wsum, hipStream_t st) {
-    hipLaunchKernelGGL(k_guide_ring_commit, dim3(1), dim3(256), 0, st, ring, added, cap, samples, wsum);
+    hipLaunchKernelGGL(k_guide_ring_commit, dim3(1), dim3(1024), 0, st, ring, added, cap, samples, wsum);
 }
 
 void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st) {
@@ -820,6 +917,15 @@ void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st
     hipLaunchKernelGGL(k_guide_reduce_groups, dim3((G_PARAMS + 255) / 256, GR_GROUPS), dim3(256), 0, st, partial,
                        blocks, per);
     hipLaunchKernelGGL(k_guide_reduce, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, per, grad);
+}
+
+void launch_guide_reduce_adam(float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
+                              float b2, float eps, int t, hipStream_t st, __bf16 *packed, float *bias) {
+    const int per = (blocks + GR_GROUPS - 1) / GR_GROUPS;
+    hipLaunchKernelGGL(k_guide_reduce_groups, dim3((G_PARAMS + 255) / 256, GR_GROUPS), dim3(256), 0, st, partial,
+                       blocks, per);
+    hipLaunchKernelGGL(k_guide_reduce_adam, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, per,
+                       params, m, v, lr, b1, b2, eps, t, packed, bias);
 }
 
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
@@ -836,22 +942,17 @@ hipError_t guide_prepare(int device) {
     int lds = 0;
     hipError_t e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) return e;
-    if (size_t(lds) < sizeof(GuideSmem) || size_t(lds) < sizeof(TrainSmem)) return hipErrorInvalidConfiguration;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer<1>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_infer<4>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
-    if (e != hipSuccess) return e;
+    if (size_t(lds) < sizeof(InferSmem) || size_t(lds) < sizeof(TrainSmem) || size_t(lds) < sizeof(FbSmem))
+        return hipErrorInvalidConfiguration;
+    for (const void *k : {reinterpret_cast<const void *>(k_guide_infer<1>), reinterpret_cast<const void *>(k_guide_infer<4>)}) {
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
+        if (e != hipSuccess) return e;
+    }
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
     if (e != hipSuccess) return e;
-    if (size_t(lds) < sizeof(GuideSmem) || size_t(lds) < sizeof(GtBwdSmem<0>)) return hipErrorInvalidConfiguration;
-    e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_forward),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GuideSmem)));
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_backward<0>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(GtBwdSmem<0>)));
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_fb), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               int(sizeof(FbSmem)));
 }
 
 void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,

@@ -38,6 +38,10 @@ void launch_guide_train_mfma(const float *params, const __bf16 *packed, const fl
 // sum the weights of the valid samples into *wsum in a fixed order
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
                               float *wsum, hipStream_t st);
+// the fixed-order sum of the per-block partial gradients fused with one Adam
+// step (packed / bias as below)
+void launch_guide_reduce_adam(float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
+                              float b2, float eps, int t, hipStream_t st, __bf16 *packed, float *bias);
 // packed / bias (may be NULL): also refresh the BF16 weights and biases that
 // launch_guide_pack would produce from the updated parameters
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,

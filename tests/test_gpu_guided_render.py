@@ -41,6 +41,11 @@ def relmse_scaled(x, ref):
 
 
 def test_guiding_reduces_error_through_a_door_gap(gpu_ctx):
+    """The network trained on a warm-up render (Adam lr 1e-2) and then frozen
+    cuts the error at equal sample count.  The gain of one online-trained
+    network varies with its initialisation (measured 1.2x to 4.8x over six
+    seeds on this 128x128 warm-up), so three networks are trained and the
+    median gain must exceed 1.5x."""
     s, cam, tr = scenes.guided_room(n_clutter_tris=50_000, seed=7)
     s.accel = 1
     desc, keep = s.to_desc()
@@ -49,16 +54,21 @@ def test_guiding_reduces_error_through_a_door_gap(gpu_ctx):
     log(f"[guided] door-gap room {W}x{H}: reference plain 4096 spp")
     ref = gpu_ctx.render(scenes.render_params(W, H, 4096, cam, seed=1, **tr), aovs=False)["image"].astype(np.float64)
     plain = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=2, **tr), aovs=False)["image"].astype(np.float64)
-    # train on a warm-up render, then render the measured image with the network frozen
-    gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=3, guide=True, guide_reset=True, **tr), aovs=False)
-    st = gpu_ctx.stats()
-    log(f"  network: {st['guide_steps']} Adam steps, {st['guide_samples']} samples gathered in the warm-up")
-    guided = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=4, guide=True, guide_train=False, **tr),
-                            aovs=False)["image"].astype(np.float64)
-    ep, eg = relmse_scaled(plain, ref), relmse_scaled(guided, ref)
-    log(f"  relMSE at 256 spp: plain {ep:.3e}, guided {eg:.3e} (ratio {ep / eg:.2f})")
-    assert np.isfinite(guided).all()
-    assert eg * 1.5 < ep
+    ep = relmse_scaled(plain, ref)
+    ratios = []
+    for gs in range(3):
+        # train on a warm-up render, then render the measured image with the network frozen
+        gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=3, guide=True, guide_reset=True, guide_seed=gs, **tr),
+                       aovs=False)
+        st = gpu_ctx.stats()
+        guided = gpu_ctx.render(scenes.render_params(W, H, 256, cam, seed=4, guide=True, guide_train=False, **tr),
+                                aovs=False)["image"].astype(np.float64)
+        assert np.isfinite(guided).all()
+        eg = relmse_scaled(guided, ref)
+        ratios.append(ep / eg)
+        log(f"  network seed {gs}: {st['guide_steps']} Adam steps, {st['guide_samples']} samples in the warm-up; "
+            f"relMSE at 256 spp: plain {ep:.3e}, guided {eg:.3e} (ratio {ep / eg:.2f})")
+    assert float(np.median(ratios)) > 1.5
 
 
 def test_guided_media_converges_to_the_oracle(gpu_ctx):
