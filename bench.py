@@ -600,7 +600,7 @@ def main() -> int:
     ap.add_argument("--no-replay", action="store_true", help="skip the traversal-only ray replay")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-native-cpu", action="store_true", help="CPU baseline: skip the -march=native build")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cornell", action="store_true")
     ap.add_argument("--no-volumetric", action="store_true")
     ap.add_argument("--vol-spp", type=int, default=16)
