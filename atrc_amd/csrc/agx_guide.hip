@@ -106,6 +106,7 @@ constexpr int IW = 8;  // waves per inference block
 struct InferSmem {
     __bf16 w[G_PACK];
     float b[G_BIASES];
+    float x[IW][64 * G_IN];  // each wave's current tile inputs (at most 64 rows), row-major
 };
 
 typedef short s16x2 __attribute__((ext_vector_type(2)));
@@ -234,14 +235,22 @@ __global__ void __launch_bounds__(IW * 64, 4) k_guide_infer(const __bf16 *packed
                 #pragma unroll
                 for (int t = 0; t < 4; ++t) acc[q][t] = b[t];
         }
+        // the tile's inputs: 16 * IR rows of 8 floats, staged in LDS with 16-byte loads
+        float *xs = sm.x[wave];
+        wave_sync();  // the previous tile's reads of xs are done
+        #pragma unroll
+        for (int v = lane; v < 16 * IR * G_IN / 4; v += 64) {
+            const uint32_t row = tile * ROWS + uint32_t(v / 2);
+            const float4 a = row < n ? *reinterpret_cast<const float4 *>(in + size_t(row) * G_IN + 4 * (v % 2))
+                                     : make_float4(0, 0, 0, 0);
+            *reinterpret_cast<float4 *>(xs + 4 * v) = a;
+        }
+        wave_sync();
         #pragma unroll 2
         for (int s = 0; s < G_IN; ++s) {
             float x[IR];
             #pragma unroll
-            for (int q = 0; q < IR; ++q) {
-                const uint32_t row = tile * ROWS + 16u * q + uint32_t(r);
-                x[q] = row < n ? fminf(1.0f, fmaxf(0.0f, in[size_t(row) * G_IN + s])) : 0.0f;
-            }
+            for (int q = 0; q < IR; ++q) x[q] = fminf(1.0f, fmaxf(0.0f, xs[(16 * q + r) * G_IN + s]));
             bf16x8 w[4];
             #pragma unroll
             for (int t = 0; t < 4; ++t) w[t] = wfrag(W, 0, s, t, lane);
@@ -474,6 +483,7 @@ struct FbSmem {
     __bf16 w[G_PACK_ALL];      // forward fragments, then the back-propagation fragments
     float b[G_BIASES];
     __bf16 stage[TW][G_HID][16];  // per-wave transpose: [feature][sample of the tile]
+    float smp[TW][16 * G_SAMPLE];  // each wave's tile of samples, row-major
 };
 
 // Back-propagation weight fragment (A operand of W_l dY^T): inputs 16 m .., k-step s over outputs.
@@ -613,13 +623,25 @@ __global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, con
         const bool valid = row < n;
         const size_t cbase = size_t(tile / 2) * G_HID * G_CHUNK;
         const int k0 = 16 * int(tile % 2);
+        // the tile's 16 samples (12 floats each), staged in LDS with 16-byte loads;
+        // rows past n read as zeros
+        float *xs = sm.smp[wave];
+        wave_sync();  // the previous tile's reads of xs are done
+        if (lane < 16 * G_SAMPLE / 4) {
+            const uint32_t rr = tile * 16u + uint32_t(lane / (G_SAMPLE / 4));
+            const float4 a = rr < n ? *reinterpret_cast<const float4 *>(samples + size_t(tile) * 16 * G_SAMPLE + 4 * lane)
+                                    : make_float4(0, 0, 0, 0);
+            *reinterpret_cast<float4 *>(xs + 4 * lane) = a;
+        }
+        wave_sync();
+        const float *sx = xs + r * G_SAMPLE;
         // forward
         f32x4 acc[1][4];
         #pragma unroll
         for (int t = 0; t < 4; ++t) acc[0][t] = *reinterpret_cast<const f32x4 *>(&B[g_bias_off(0) + 16 * t + 4 * g]);
         #pragma unroll 2
         for (int s = 0; s < G_IN; ++s) {
-            const float x = valid ? fminf(1.0f, fmaxf(0.0f, samples[size_t(row) * G_SAMPLE + s])) : 0.0f;
+            const float x = fminf(1.0f, fmaxf(0.0f, sx[s]));
             const bf16x8 f = one_blob8(x, g);
             #pragma unroll
             for (int t = 0; t < 4; ++t) acc[0][t] = mfma(wfrag(W, 0, s, t, lane), f, acc[0][t]);
@@ -641,9 +663,8 @@ __global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, con
         float dir[3] = {0.0f, 0.0f, 1.0f};
         float w = 0.0f;
         if (valid) {
-            const float *smp = samples + size_t(row) * G_SAMPLE;
-            dir[0] = smp[G_IN]; dir[1] = smp[G_IN + 1]; dir[2] = smp[G_IN + 2];
-            w = smp[G_IN + 3] * wscale;
+            dir[0] = sx[G_IN]; dir[1] = sx[G_IN + 1]; dir[2] = sx[G_IN + 2];
+            w = sx[G_IN + 3] * wscale;
         }
         const f32x4 o[2] = {acc[0][0], acc[0][1]};
         float gr[2][4];
