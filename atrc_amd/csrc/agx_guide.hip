@@ -39,8 +39,6 @@ __device__ __forceinline__ float one_blob(float v, int b) {
     return __expf(-0.5f * x * x);
 }
 
-__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
-
 // Input k of a 32-wide k-step held as element i by lane group g (lanes 16 g ..
 // 16 g + 15) of a fragment.  Layer 0 takes its inputs in natural order (the
 // one-blob features are generated in registers in any order).  Layers 1..4
