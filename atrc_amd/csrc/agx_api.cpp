@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <atomic>
@@ -451,12 +452,11 @@ void agx_destroy(agx_ctx *c) {
     delete c;
 }
 
-int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
-    if (!c || !desc) return fail(AGX_E_INVALID_ARG, "null argument");
-    HostScene hs;
-    std::string err;
-    const int rc = build_host_scene(*desc, &hs, &err);
-    if (rc != AGX_OK) return fail(rc, err);
+}  // extern "C"
+
+// The device half of agx_upload_scene: copy a built host scene into the
+// context (replacing its scene) and fill its DevScene.
+static int upload_host_scene(agx_ctx *c, const HostScene &hs) {
     HIP_TRY(hipSetDevice(c->device));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->free_scene();
@@ -556,6 +556,17 @@ int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
     c->n_lights = ds.n_lights;
     c->has_scene = true;
     return AGX_OK;
+}
+
+extern "C" {
+
+int agx_upload_scene(agx_ctx *c, const agx_scene_desc *desc) {
+    if (!c || !desc) return fail(AGX_E_INVALID_ARG, "null argument");
+    HostScene hs;
+    std::string err;
+    const int rc = build_host_scene(*desc, &hs, &err);
+    if (rc != AGX_OK) return fail(rc, err);
+    return upload_host_scene(c, hs);
 }
 
 int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *cancel_flag, float *rgb,
@@ -947,6 +958,14 @@ using Clock = std::chrono::steady_clock;
 double ms_since(Clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
+// AGX_BATCH_TRACE=1: the graph batch prints the time of each phase to stderr
+bool batch_trace() {
+    static const bool on = [] {
+        const char *e = std::getenv("AGX_BATCH_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
 
 // Renders items[order[0..m)] with k worker contexts (each worker thread takes
 // the next item, builds + uploads it, renders it).  spp_override > 0 replaces
@@ -1052,7 +1071,10 @@ int enqueue_render(agx_ctx *c, const agx_render_params *p, hipStream_t st, unsig
         return fail(AGX_E_INVALID_ARG, "graph batch: render buffers not prepared");
     rc.guide_rec_shift = guide_rec_shift(size_t(per_pass));
     const uint32_t n_closest_max = uint32_t(2 * per_pass), n_shadow_max = uint32_t(per_pass * std::max(rc.L, 1));
-    const int max_bounces = p->max_depth + std::max(p->specular_depth, 0) + 2;
+    // without delta surfaces no bounce extends the depth (path_tracing.cpp:155-162),
+    // so the specular allowance adds only empty launches
+    const int spec = (c->ds.features & SF_DELTA) ? std::max(p->specular_depth, 0) : 0;
+    const int max_bounces = p->max_depth + spec + 2;
     PathBuffers pb = c->pb;
     launch_film_clear(pb, wh, st);
     for (int64_t base = 0; base < total; base += per_pass) {
@@ -1187,7 +1209,9 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     struct Held {
         std::vector<agx_ctx *> ctxs;
         std::vector<hipStream_t> streams;
-        std::vector<hipEvent_t> fork, join, start, done;
+        std::vector<hipEvent_t> fork, join, start, done, copied;
+        float *stage[2] = {nullptr, nullptr};  // pinned host images of two groups
+        unsigned long long *rays_pin[2] = {nullptr, nullptr};  // pinned ray counters of two groups
         unsigned long long *rays = nullptr;
         std::vector<hipGraphExec_t> execs;
         std::vector<std::thread> builders;
@@ -1196,26 +1220,46 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
             stop = true;
             for (auto &t : builders) if (t.joinable()) t.join();
             for (hipGraphExec_t e : execs) if (e) (void)hipGraphExecDestroy(e);
-            for (auto *v : {&fork, &join, &start, &done})
+            for (float *st : stage) if (st) (void)hipHostFree(st);
+            for (unsigned long long *rp : rays_pin) if (rp) (void)hipHostFree(rp);
+            for (auto *v : {&fork, &join, &start, &done, &copied})
                 for (hipEvent_t e : *v) if (e) (void)hipEventDestroy(e);
             for (hipStream_t st : streams) (void)hipStreamDestroy(st);
             if (rays) (void)hipFree(rays);
             for (agx_ctx *w : ctxs) if (w) agx_destroy(w);
         }
     } held;
-    held.ctxs.assign(n, nullptr);
+    // SETS sets of group_size contexts, used by the groups in turn: a group's
+    // contexts take the scenes of group g + SETS once it has finished, so path
+    // buffers are allocated once and grow with the scenes instead of being
+    // allocated and freed per scene
+    std::vector<uint32_t> pos(n);
+    for (uint32_t j = 0; j < n; ++j) pos[order[j]] = j;
+    constexpr uint32_t SETS = 3;
+    const uint32_t pool_size = std::min(n, SETS * group_size);
+    held.ctxs.assign(pool_size, nullptr);
+    for (auto &w : held.ctxs)
+        if (int r = agx_create(c->device, &w); r != AGX_OK) return r;
+    auto ctx_of = [&](uint32_t i) {
+        return held.ctxs[((pos[i] / group_size) % SETS) * group_size + pos[i] % group_size];
+    };
     HIP_TRY(hipMalloc(reinterpret_cast<void **>(&held.rays), std::max<size_t>(n, 1) * sizeof(unsigned long long)));
     HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
     HIP_TRY(hipDeviceSynchronize());
 
-    // host builds (BVH + upload) on every host core, in group order
+    // host builds on every host core, in group order; a built scene is uploaded
+    // into its pooled context once the group that last used that context has
+    // finished (groups finish in order)
     std::vector<std::atomic<int>> built(n);  // 0 pending, 1 built, 2 failed
     for (auto &b : built) b = 0;
     std::mutex mu;
     std::condition_variable cv;
     std::atomic<uint32_t> next{0};
     std::atomic<int64_t> build_end_ns{0};
-    const int n_threads = int(std::max(1u, std::min({std::thread::hardware_concurrency(), 16u, std::max(n, 1u)})));
+    int64_t finished_upto = -1;  // highest finished group (guarded by mu)
+    // one builder per core (the main thread mostly waits on events)
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int n_threads = int(std::max(1u, std::min({hw, 16u, std::max(n, 1u)})));
     std::string build_err;
     auto build = [&]() {
         for (;;) {
@@ -1224,17 +1268,24 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
             if (j >= n) break;
             const uint32_t i = order[j];
             const auto tb = Clock::now();
-            agx_ctx *w = nullptr;
-            int r = agx_create(c->device, &w);
+            HostScene hs;
+            std::string err;
+            int r = build_host_scene(*items[i].scene, &hs, &err);
             if (r == AGX_OK) {
-                held.ctxs[i] = w;
-                r = agx_upload_scene(w, items[i].scene);
+                const int64_t g = int64_t(j / group_size);
+                std::unique_lock<std::mutex> lk(mu);
+                while (!held.stop && g >= int64_t(SETS) && finished_upto < g - int64_t(SETS))
+                    cv.wait_for(lk, std::chrono::milliseconds(5));
+                lk.unlock();
+                if (held.stop) break;
+                r = upload_host_scene(ctx_of(i), hs);
+                if (r != AGX_OK) err = agx_last_error();
             }
             items[i].build_ms = ms_since(tb);
             items[i].status = r;
             {
                 std::lock_guard<std::mutex> lk(mu);
-                if (r != AGX_OK && build_err.empty()) build_err = "batch item " + std::to_string(i) + ": " + agx_last_error();
+                if (r != AGX_OK && build_err.empty()) build_err = "batch item " + std::to_string(i) + ": " + err;
                 built[i] = r == AGX_OK ? 1 : 2;
                 build_end_ns = std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - t0).count();
             }
@@ -1261,10 +1312,28 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     held.join.assign(size_t(n_groups + 64) * 8, nullptr);
     held.start.assign(n_groups + 64, nullptr);
     held.done.assign(n_groups + 64, nullptr);
+    held.copied.assign(n_groups + 64, nullptr);
     for (auto *v : {&held.fork, &held.join})
         for (auto &e : *v) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (auto *v : {&held.start, &held.done})
         for (auto &e : *v) HIP_TRY(hipEventCreate(&e));
+    for (auto &e : held.copied) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // pinned staging for the images of two groups: a group's images are copied
+    // out on the stream right after its graph, and to the callers' buffers on
+    // the host once the group is done
+    size_t stage_floats = 0;
+    for (uint32_t g = 0; g < n_groups; ++g) {
+        size_t f = 0;
+        for (uint32_t j = g * group_size; j < std::min(n, (g + 1) * group_size); ++j)
+            f += 3 * size_t(items[order[j]].params->width) * size_t(items[order[j]].params->height);
+        stage_floats = std::max(stage_floats, f);
+    }
+    for (float *&st : held.stage)
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&st), std::max<size_t>(stage_floats, 1) * sizeof(float),
+                              hipHostMallocDefault));
+    for (unsigned long long *&rp : held.rays_pin)
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&rp), std::max<size_t>(n, 1) * sizeof(unsigned long long),
+                              hipHostMallocDefault));
     size_t ev = 0;  // next free event slot
 
     struct Config {
@@ -1279,7 +1348,7 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     auto launch = [&](const std::vector<uint32_t> &sel, const Config &cfg, int spp_override, size_t *slot) -> int {
         std::vector<agx_render_params> prm(sel.size());
         for (size_t q = 0; q < sel.size(); ++q) {
-            agx_ctx *w = held.ctxs[sel[q]];
+            agx_ctx *w = ctx_of(sel[q]);
             prm[q] = *items[sel[q]].params;
             if (spp_override > 0) prm[q].spp = std::min(prm[q].spp, spp_override);
             if (w->pass_paths_limit != (size_t(1) << cfg.pass_log2)) {
@@ -1311,7 +1380,7 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
         for (int b = 0; b < k && he == hipSuccess; ++b) he = hipStreamWaitEvent(held.streams[size_t(b)], held.fork[e], 0);
         for (int b = 0; b < k && he == hipSuccess && cap_status == AGX_OK; ++b)
             for (size_t q : lanes[size_t(b)]) {
-                cap_status = enqueue_render(held.ctxs[sel[q]], &prm[q], held.streams[size_t(b)], held.rays + sel[q]);
+                cap_status = enqueue_render(ctx_of(sel[q]), &prm[q], held.streams[size_t(b)], held.rays + sel[q]);
                 if (cap_status != AGX_OK) break;
             }
         for (int b = 0; b < k && he == hipSuccess; ++b) {
@@ -1343,12 +1412,16 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
         return v;
     };
 
+    auto trace = [&](const char *what, uint32_t g) {
+        if (batch_trace()) std::fprintf(stderr, "[batch graph] %8.1f ms  %s %u\n", ms_since(t0), what, g);
+    };
     Config cfg{branches > 0 ? branches : 4, 22, 256};
     if (branches == 0 && n > 0) {
         // autotuner: coordinate descent over branches, pass size and traversal block
         // size, each configuration timed on the first group's scenes at 2 spp
         // (device time between events); later groups build meanwhile
         if (int r = wait_group(0); r != AGX_OK) return r;
+        trace("first group built", 0);
         const std::vector<uint32_t> probe = group_items(0);
         auto time_cfg = [&](const Config &k, double *rate) -> int {
             HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));
@@ -1363,6 +1436,7 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
             for (uint32_t i : probe) rays += hr[i];
             *rate = double(rays) / std::max(double(ms) * 1e-3, 1e-9);
             ++rep.tune_configs;
+            trace("probe timed", uint32_t(rep.tune_configs));
             return AGX_OK;
         };
         double best = -1;
@@ -1400,34 +1474,70 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     const auto tr = Clock::now();
     std::vector<size_t> slots(n_groups, 0);
     std::vector<uint64_t> hr(std::max<size_t>(n, 1), 0);
+    // a launched group's images and ray counts to pinned host memory, on the stream
+    auto copy_out = [&](uint32_t g) -> int {
+        size_t off = 0;
+        for (uint32_t i : group_items(g)) {
+            const size_t f = 3 * size_t(items[i].params->width) * size_t(items[i].params->height);
+            HIP_TRY(hipMemcpyAsync(held.stage[g % 2] + off, ctx_of(i)->d_resolve, f * sizeof(float),
+                                   hipMemcpyDeviceToHost, c->stream));
+            off += f;
+        }
+        HIP_TRY(hipMemcpyAsync(held.rays_pin[g % 2], held.rays, size_t(n) * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipEventRecord(held.copied[slots[g]], c->stream));
+        return AGX_OK;
+    };
     auto finish_group = [&](uint32_t g) -> int {
         HIP_TRY(hipEventSynchronize(held.done[slots[g]]));
+        trace("group done on the device", g);
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, held.start[slots[g]], held.done[slots[g]]));
+        HIP_TRY(hipEventSynchronize(held.copied[slots[g]]));
+        {   // the images to the callers' buffers, one thread per image (their first
+            // touch of fresh pages dominates the copy)
+            std::vector<std::thread> cp;
+            size_t off = 0;
+            for (uint32_t i : group_items(g)) {
+                const size_t f = 3 * size_t(items[i].params->width) * size_t(items[i].params->height);
+                if (items[i].rgb)
+                    cp.emplace_back([dst = items[i].rgb, src = held.stage[g % 2] + off, f]() {
+                        std::memcpy(dst, src, f * sizeof(float));
+                    });
+                off += f;
+            }
+            for (auto &t : cp) t.join();
+        }
         for (uint32_t i : group_items(g)) {
-            unsigned long long r = 0;
-            HIP_TRY(hipMemcpy(&r, held.rays + i, sizeof(r), hipMemcpyDeviceToHost));
-            const agx_render_params *p = items[i].params;
-            const size_t wh = size_t(p->width) * size_t(p->height);
-            if (items[i].rgb)
-                HIP_TRY(hipMemcpy(items[i].rgb, held.ctxs[i]->d_resolve, 3 * wh * sizeof(float), hipMemcpyDeviceToHost));
+            const unsigned long long r = held.rays_pin[g % 2][i];
             items[i].rays = r;
             items[i].render_ms = ms;
             items[i].status = AGX_OK;
             rep.rays += r;
-            agx_destroy(held.ctxs[i]);
-            held.ctxs[i] = nullptr;
         }
+        {   // its contexts may now take the scenes of group g + SETS
+            std::lock_guard<std::mutex> lk(mu);
+            finished_upto = int64_t(g);
+        }
+        cv.notify_all();
         return AGX_OK;
     };
+    trace("tuned; rendering starts", 0);
     for (uint32_t g = 0; g < n_groups; ++g) {
         if (int r = wait_group(g); r != AGX_OK) return r;
+        trace("group built", g);
         if (int r = launch(group_items(g), cfg, 0, &slots[g]); r != AGX_OK) return r;
-        if (g >= 1)
+        if (int r = copy_out(g); r != AGX_OK) return r;
+        trace("group launched", g);
+        if (g >= 1) {
             if (int r = finish_group(g - 1); r != AGX_OK) return r;
+            trace("group finished", g - 1);
+        }
     }
-    if (n_groups)
+    if (n_groups) {
         if (int r = finish_group(n_groups - 1); r != AGX_OK) return r;
+        trace("group finished", n_groups - 1);
+    }
     rep.render_ms = ms_since(tr);
     for (auto &t : held.builders) t.join();
     held.builders.clear();

@@ -134,6 +134,40 @@ void build_alias(const std::vector<float> &w, std::vector<float> *prob, std::vec
 
 }  // namespace
 
+namespace {
+// Atrc's Primitive (a, b - a, c - a; the pad carries the original id) and
+// PrimitiveInfo (triangle_bvh.cpp:24-35, 533-558) of triangle t
+void emit_triangle(const agx_vertex *tris, uint32_t t, std::vector<vec4> *prims, std::vector<vec4> *info) {
+    const agx_vertex &va = tris[3 * t], &vb = tris[3 * t + 1], &vc = tris[3 * t + 2];
+    const F3 A = vpos(va), B_A = sub(vpos(vb), A), C_A = sub(vpos(vc), A);
+    prims->push_back(v4(A, bits_f(t)));
+    prims->push_back(v4(B_A, 0.0f));
+    prims->push_back(v4(C_A, 0.0f));
+    const F3 na = unit(vnor(va)), nb = unit(vnor(vb)), nc = unit(vnor(vc));
+    const float tbu = vb.uv[0] - va.uv[0], tbv = vb.uv[1] - va.uv[1];
+    const float tcu = vc.uv[0] - va.uv[0], tcv = vc.uv[1] - va.uv[1];
+    F3 z = unit(cross3(B_A, C_A));
+    if (dot3(add(add(na, nb), nc), z) < 0) z = neg(z);
+    const F3 x = tangent_x(B_A, C_A, tbu, tbv, tcu, tcv, z);
+    info->push_back(v4(na, va.uv[0]));
+    info->push_back(v4(sub(nb, na), va.uv[1]));
+    info->push_back(v4(sub(nc, na), tbu));
+    info->push_back(v4(x, tbv));
+    info->push_back(v4(z, tcu));
+    info->push_back(vec4{tcv, 0.0f, 0.0f, 0.0f});
+}
+
+// surface area of a mesh, summed in triangle order as build_triangle_bvh does
+float mesh_area(const agx_vertex *tris, uint32_t n) {
+    float area = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const F3 p0 = vpos(tris[3 * i]), p1 = vpos(tris[3 * i + 1]), p2 = vpos(tris[3 * i + 2]);
+        area += tri_area(sub(p1, p0), sub(p2, p0));
+    }
+    return area;
+}
+}  // namespace
+
 int build_triangle_bvh(const agx_vertex *tris, uint32_t n, std::vector<vec4> *nodes, std::vector<vec4> *prims,
                        std::vector<vec4> *info, double *surface_area) {
     // triangle references in build order
@@ -219,26 +253,7 @@ int build_triangle_bvh(const agx_vertex *tris, uint32_t n, std::vector<vec4> *no
         const uint32_t first = next_prim, last = next_prim + (b.end - b.begin);
         nodes->push_back(v4(b.box.lo, bits_f(first)));
         nodes->push_back(v4(b.box.hi, bits_f(last)));
-        for (uint32_t j = b.begin; j < b.end; ++j) {
-            const uint32_t t = refs[j].id;
-            const agx_vertex &va = tris[3 * t], &vb = tris[3 * t + 1], &vc = tris[3 * t + 2];
-            const F3 A = vpos(va), B_A = sub(vpos(vb), A), C_A = sub(vpos(vc), A);
-            prims->push_back(v4(A, bits_f(t)));
-            prims->push_back(v4(B_A, 0.0f));
-            prims->push_back(v4(C_A, 0.0f));
-            const F3 na = unit(vnor(va)), nb = unit(vnor(vb)), nc = unit(vnor(vc));
-            const float tbu = vb.uv[0] - va.uv[0], tbv = vb.uv[1] - va.uv[1];
-            const float tcu = vc.uv[0] - va.uv[0], tcv = vc.uv[1] - va.uv[1];
-            F3 z = unit(cross3(B_A, C_A));
-            if (dot3(add(add(na, nb), nc), z) < 0) z = neg(z);
-            const F3 x = tangent_x(B_A, C_A, tbu, tbv, tcu, tcv, z);
-            info->push_back(v4(na, va.uv[0]));
-            info->push_back(v4(sub(nb, na), va.uv[1]));
-            info->push_back(v4(sub(nc, na), tbu));
-            info->push_back(v4(x, tbv));
-            info->push_back(v4(z, tcu));
-            info->push_back(vec4{tcv, 0.0f, 0.0f, 0.0f});
-        }
+        for (uint32_t j = b.begin; j < b.end; ++j) emit_triangle(tris, refs[j].id, prims, info);
         next_prim = last;
     }
     return max_depth;
@@ -667,8 +682,19 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
             if (!e.tris || !e.tri_count) { *err = "entity " + std::to_string(i) + ": empty triangle mesh"; return AGX_E_INVALID_ARG; }
             std::vector<vec4> nodes, prims, info;
             double sa = 0;
-            const int depth = build_triangle_bvh(e.tris, e.tri_count, &nodes, &prims, &info, &sa);
-            S.max_blas_depth = std::max(S.max_blas_depth, depth);
+            const bool emissive = e.emit_radiance[0] != 0 || e.emit_radiance[1] != 0 || e.emit_radiance[2] != 0;
+            if (desc.accel != AGX_ACCEL_ATRC && !emissive) {
+                // the BVH4 modes traverse their own tree: Atrc's triangle BVH is
+                // needed only for the primitive order its light sampling uses, so
+                // a mesh that emits nothing keeps its triangles in input order
+                prims.reserve(size_t(e.tri_count) * 3);
+                info.reserve(size_t(e.tri_count) * 6);
+                for (uint32_t t = 0; t < e.tri_count; ++t) emit_triangle(e.tris, t, &prims, &info);
+                sa = double(mesh_area(e.tris, e.tri_count));
+            } else {
+                const int depth = build_triangle_bvh(e.tris, e.tri_count, &nodes, &prims, &info, &sa);
+                S.max_blas_depth = std::max(S.max_blas_depth, depth);
+            }
             const uint32_t node_base = uint32_t(S.blas_nodes.size() / 2);
             const uint32_t prim_base = uint32_t(S.blas_prims.size() / 3);
             // rebase indices to the global arrays
@@ -696,7 +722,6 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
                 box.add_point(add(a, F3{prims[k + 2].x, prims[k + 2].y, prims[k + 2].z}));
             }
             box.pad_flat();
-            const bool emissive = e.emit_radiance[0] != 0 || e.emit_radiance[1] != 0 || e.emit_radiance[2] != 0;
             if (emissive) {
                 std::vector<float> areas(e.tri_count);
                 for (uint32_t k = 0; k < e.tri_count; ++k) {

@@ -407,15 +407,17 @@ int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t w
 
 /* Batched rendering in hipGraph launches (no counterpart in Atrc).  Items are
  * taken cheapest first (pixels x spp + triangles) in groups of 8.  Host
- * threads (one per core, at most 16) build and upload the scenes, each into
- * a context of its own, in that order; as soon as a group is built, its
+ * threads (one per core, at most 16) build the scenes in that order
+ * and upload each into one of three sets of 8 pooled contexts, once the group
+ * that last used the set has finished; as soon as a group is built, its
  * renders are captured into one hipGraph with `branches` parallel branches
  * (the costliest scene first onto the least loaded branch; a branch renders
  * its scenes one after another) and launched, so later scenes build while
- * earlier groups render.  A finished group's images are copied out and its
- * contexts released while the next group renders.  A captured render
- * launches every bounce up to the depth bound (kernels of an empty queue
- * exit at once) and counts its rays on the device.  branches = 0: an
+ * earlier groups render.  A group's images go to pinned host memory on the
+ * stream right after its graph and to the items' buffers once it is done.
+ * A captured render launches every bounce up to the depth bound (max_depth +
+ * 2, plus specular_depth in scenes with delta surfaces; kernels of an empty
+ * queue exit at once) and counts its rays on the device.  branches = 0: an
  * autotuner times the first group at 2 spp for 2, 4 and 8 branches, passes
  * of 2^21 / 2^22 / 2^23 paths and 128- / 256-thread traversal blocks
  * (coordinate descent; later groups build meanwhile) and renders the batch

@@ -117,17 +117,23 @@ def test_bvh4_structure(which):
     assert len(prims) == n_tris + len(s.entities) - len(mesh_ents)
     analytic = slot == ANALYTIC
     assert sorted(ent[analytic].tolist()) == [i for i in range(len(s.entities)) if i not in mesh_ents]
-    # mesh triangles: the records of Atrc's triangle BVH, slots contiguous per entity in entity order
+    # mesh triangles: Atrc's triangle records (a, b - a, c - a), slots contiguous per
+    # entity in entity order; a mesh that emits nothing keeps its input triangle
+    # order (the BVH4 needs no Atrc tree for it)
     base = 0
     for i in mesh_ents:
         tris = s.entities[i].tris
         n = len(tris) // 3
         ref_nodes, ref_prims, _ = capi.debug_triangle_bvh(tris)
+        ref_prims = ref_prims.reshape(n, 3, 4)
+        ids = ref_prims[:, 0, 3].view(np.uint32)
+        if not any(s.entities[i].emit):
+            ref_prims = ref_prims[np.argsort(ids)]
         sel = np.where(~analytic & (ent == i))[0]
         assert len(sel) == n
         local = slot[sel].astype(np.int64) - base
         assert sorted(local.tolist()) == list(range(n))
-        np.testing.assert_array_equal(prims[sel, :, :3], ref_prims.reshape(n, 3, 4)[local][:, :, :3])
+        np.testing.assert_array_equal(prims[sel, :, :3], ref_prims[local][:, :, :3])
         base += n
     summ = capi.debug_scene_summary(desc)
     assert summ["bvh4_nodes"] == len(nodes) and summ["bvh4_depth"] >= 1
@@ -136,11 +142,16 @@ def test_bvh4_structure(which):
 
 def test_bvh4_is_shallower_than_atrc_bvh():
     s, _, _ = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=3, extent=20.0)
-    s.accel = capi.ACCEL_BVH4
     s.aggregate = capi.AGGREGATE_BVH
+    s.accel = capi.ACCEL_ATRC
+    desc, keep = s.to_desc()
+    atrc = capi.debug_scene_summary(desc)
+    s.accel = capi.ACCEL_BVH4
     desc, keep = s.to_desc()
     summ = capi.debug_scene_summary(desc)
-    assert summ["bvh4_depth"] < summ["max_blas_depth"] + summ["max_tlas_depth"]
+    assert summ["bvh4_depth"] < atrc["max_blas_depth"] + atrc["max_tlas_depth"]
+    # the BVH4 mode builds no Atrc triangle BVH for meshes that emit nothing
+    assert summ["blas_nodes"] == 0 and summ["blas_prims"] == atrc["blas_prims"]
 
 
 def test_unknown_accel_rejected():
@@ -168,11 +179,10 @@ def test_watertight_records_hold_the_mesh_vertices():
     ent = pw[:, 0, 3].view(np.int32)
     for i in mesh_ents:
         tris = s.entities[i].tris
-        ref_nodes, ref_prims, _ = capi.debug_triangle_bvh(tris)
-        orig = ref_prims.reshape(-1, 3, 4)[:, 0, 3].view(np.int32)   # original triangle id per slot
+        assert not any(s.entities[i].emit)   # slots then follow the input triangle order
         sel = np.where((slotw != ANALYTIC) & (ent == i))[0]
         base = int(slotw[sel].min())
-        ids = orig[slotw[sel].astype(np.int64) - base]
+        ids = slotw[sel].astype(np.int64) - base
         verts = np.asarray(tris, np.float32)[:, :3].reshape(-1, 3, 3)
         np.testing.assert_array_equal(pw[sel, :, :3], verts[ids])
 
