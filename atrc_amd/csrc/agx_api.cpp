@@ -97,7 +97,6 @@ struct agx_ctx {
     uint32_t *g_count = nullptr;
     uint32_t *g_gather_blk = nullptr;  // per-block counts / offsets of the sample gather
     uint32_t *g_ring = nullptr;      // sample ring (next slot, valid samples) over g_samples
-    __bf16 *g_acts = nullptr, *g_dy = nullptr;  // matrix-core training: H_1..H_4, dY_1..dY_5 (per chunk, feature-major)
     uint32_t g_ring_cap = 0;         // capacity the ring state refers to (0: empty)
     uint32_t g_batch_cap = 0;
     uint64_t g_steps = 0;
@@ -223,8 +222,6 @@ struct agx_ctx {
     int ensure_guide_batch(uint32_t batch) {
         if (batch > g_batch_cap) {
             HIP_TRY(galloc(guide_allocs, &g_samples, size_t(batch) * (G_IN + 4)));
-            HIP_TRY(galloc(guide_allocs, &g_acts, g_train_h_elems(batch)));
-            HIP_TRY(galloc(guide_allocs, &g_dy, g_train_dy_elems(batch)));
             g_batch_cap = batch;
         }
         if (g_ring_cap != batch) {
@@ -745,13 +742,13 @@ int agx_render(agx_ctx *c, const agx_render_params *p, const volatile int32_t *c
             for (int k = 0; k < guide_steps; ++k) {
                 // matrix-core step: forward with the current BF16 weights, backward, reduce, Adam
                 // (which also repacks the BF16 weights)
-                launch_guide_train_mfma(c->g_params, c->g_packed, c->g_bias, c->g_samples, c->g_ring + 1, guide_batch,
-                                        c->g_wsum, c->g_acts, c->g_dy, c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS,
-                                        c->stream);
+                launch_guide_train_mfma(c->g_packed, c->g_bias, c->g_samples, c->g_ring + 1, guide_batch, c->g_wsum,
+                                        c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->stream);
                 c->g_steps += 1;
                 launch_guide_reduce_adam(c->g_partial, agx_ctx::GUIDE_TRAIN_BLOCKS, c->g_params, c->g_m, c->g_v,
-                                         guide_lr, 0.9f, 0.99f, 1e-8f, int(std::min<uint64_t>(c->g_steps, 1u << 30)),
-                                         c->stream, c->g_packed, c->g_bias);
+                                         guide_lr, 0.9f, 0.99f, 1e-8f,
+                                         int(std::min<uint64_t>(c->g_steps, 1u << 30)), c->stream, c->g_packed,
+                                         c->g_bias);
             }
             HIP_TRY(hipMemcpyAsync(&guide_samples_h, c->g_count, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
             HIP_TRY(hipMemcpyAsync(&guide_valid_h, c->g_ring + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1740,6 +1737,20 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
     if (n) HIP_TRY(hipMemcpyAsync(ds.p, samples, size_t(n) * (G_IN + 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
     launch_guide_train(dp.get<float>(), ds.get<float>(), dscal.get<uint32_t>(), n, dscal.get<float>() + 1,
                        dpart.get<float>(), BLOCKS, c->stream);
+    // the renders' one-launch reduction + Adam runs on a copy of the partials
+    // and of the state, and must give the same parameters bit for bit
+    DevBuf dpart2, dp2, dm2, dv2;
+    if (lr != 0.0f) {
+        HIP_TRY(hipMalloc(&dpart2.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
+        HIP_TRY(hipMalloc(&dp2.p, G_PARAMS * sizeof(float)));
+        HIP_TRY(hipMalloc(&dm2.p, G_PARAMS * sizeof(float)));
+        HIP_TRY(hipMalloc(&dv2.p, G_PARAMS * sizeof(float)));
+        HIP_TRY(hipMemcpyAsync(dpart2.p, dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float), hipMemcpyDeviceToDevice,
+                               c->stream));
+        HIP_TRY(hipMemcpyAsync(dp2.p, dp.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(dm2.p, dm.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(dv2.p, dv.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    }
     launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
     DevBuf dpk, dbi, dpk2, dbi2;
     if (lr != 0.0f) {
@@ -1760,6 +1771,16 @@ int agx_debug_guide_train(agx_ctx *c, float *params, float *m, float *v, const f
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (a != b || std::memcmp(ba.data(), bb.data(), G_BIASES * 4) != 0)
             return fail(AGX_E_HIP, "Adam's repacked weights differ from the packing kernel's");
+        launch_guide_reduce_adam(dpart2.get<float>(), BLOCKS, dp2.get<float>(), dm2.get<float>(), dv2.get<float>(), lr,
+                                 0.9f, 0.99f, 1e-8f, t, c->stream, nullptr, nullptr);
+        std::vector<float> pa(G_PARAMS), pb(G_PARAMS), va(G_PARAMS), vb(G_PARAMS);
+        HIP_TRY(hipMemcpyAsync(pa.data(), dp.p, G_PARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(pb.data(), dp2.p, G_PARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(va.data(), dv.p, G_PARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(vb.data(), dv2.p, G_PARAMS * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (std::memcmp(pa.data(), pb.data(), G_PARAMS * 4) != 0 || std::memcmp(va.data(), vb.data(), G_PARAMS * 4) != 0)
+            return fail(AGX_E_HIP, "the one-launch reduction + Adam differs from the two-pass reduction + Adam");
     }
     HIP_TRY(hipGetLastError());
     if (grad_out) HIP_TRY(hipMemcpyAsync(grad_out, dg.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -1778,13 +1799,11 @@ int agx_debug_guide_grad_mfma(agx_ctx *c, const float *params, const float *samp
         return fail(AGX_E_UNSUPPORTED, std::string("guiding kernels: ") + hipGetErrorString(e));
     constexpr int BLOCKS = 64;
     const uint32_t cap = std::max<uint32_t>(n, 1);
-    DevBuf dp, dpack, dbias, ds, dacts, ddy, dpart, dg, dscal;
+    DevBuf dp, dpack, dbias, ds, dpart, dg, dscal;
     HIP_TRY(hipMalloc(&dp.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dpack.p, G_PACK_ALL * sizeof(__bf16)));
     HIP_TRY(hipMalloc(&dbias.p, G_BIASES * sizeof(float)));
     HIP_TRY(hipMalloc(&ds.p, size_t(cap) * (G_IN + 4) * sizeof(float)));
-    HIP_TRY(hipMalloc(&dacts.p, g_train_h_elems(cap) * sizeof(__bf16)));
-    HIP_TRY(hipMalloc(&ddy.p, g_train_dy_elems(cap) * sizeof(__bf16)));
     HIP_TRY(hipMalloc(&dpart.p, size_t(BLOCKS) * G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dg.p, G_PARAMS * sizeof(float)));
     HIP_TRY(hipMalloc(&dscal.p, 2 * sizeof(float)));
@@ -1796,9 +1815,8 @@ int agx_debug_guide_grad_mfma(agx_ctx *c, const float *params, const float *samp
     if (n) HIP_TRY(hipMemcpyAsync(ds.p, samples, size_t(n) * (G_IN + 4) * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     launch_guide_pack(dp.get<float>(), dpack.get<__bf16>(), dbias.get<float>(), c->stream);
-    launch_guide_train_mfma(dp.get<float>(), dpack.get<__bf16>(), dbias.get<float>(), ds.get<float>(),
-                            dscal.get<uint32_t>(), cap, dscal.get<float>() + 1, dacts.get<__bf16>(), ddy.get<__bf16>(),
-                            dpart.get<float>(), BLOCKS, c->stream);
+    launch_guide_train_mfma(dpack.get<__bf16>(), dbias.get<float>(), ds.get<float>(), dscal.get<uint32_t>(), cap,
+                            dscal.get<float>() + 1, dpart.get<float>(), BLOCKS, c->stream);
     launch_guide_reduce(dpart.get<float>(), BLOCKS, dg.get<float>(), c->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(grad_out, dg.p, G_PARAMS * sizeof(float), hipMemcpyDeviceToHost, c->stream));

@@ -51,13 +51,6 @@ __host__ __device__ constexpr int g_pack_bwd_off(int l) {
 }
 constexpr int G_PACK_BWD = g_pack_bwd_off(G_LAYERS);     // 14336 bf16
 constexpr int G_PACK_ALL = G_PACK + G_PACK_BWD;
-// Training activations and gradients, feature-major per chunk of 32 samples:
-// [layer][chunk][64 features][32 samples] BF16, H_1..H_4 and dY_1..dY_5
-// (dY_5 uses 32 of the 64 feature rows)
-constexpr int G_CHUNK = 32;
-__host__ __device__ constexpr size_t g_chunks(uint32_t cap) { return (size_t(cap) + G_CHUNK - 1) / G_CHUNK; }
-__host__ __device__ constexpr size_t g_train_h_elems(uint32_t cap) { return 4 * g_chunks(cap) * G_HID * G_CHUNK; }
-__host__ __device__ constexpr size_t g_train_dy_elems(uint32_t cap) { return 5 * g_chunks(cap) * G_HID * G_CHUNK; }
 __host__ __device__ constexpr int g_bias_off(int l) {
     int o = 0;
     for (int k = 0; k < l; ++k) o += g_out(k);

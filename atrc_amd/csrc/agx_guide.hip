@@ -462,26 +462,34 @@ __global__ void __launch_bounds__(256) k_guide_train(const float *params, const 
 }
 
 // ---------------------------------------------------------------- matrix-core training
-// One Adam step over the sample ring takes three kernels and no host wait:
-//   k_gtrain_fb: per 16-sample tile, the forward pass through all five layers
-//     and the back-propagation dY_5 -> dY_1 with activations and gradients
-//     kept in registers (the forward's chaining, with the weights packed again
-//     as the A operand of W_l dY^T); it writes H_1..H_4 and dY_1..dY_5 BF16,
-//     feature-major per chunk of 32 samples, through a per-wave LDS transpose;
-//   k_gtrain_dw: dW_l = H_l^T dY_(l+1) and db_l for every layer, split over
-//     the chunks: each block sums its chunks' products (one MFMA per 16 x 16
-//     tile and chunk, 16-byte operand loads; layer 0's one-blob features
-//     rebuilt in LDS per chunk) and writes its partial gradient;
-//   then the fixed-order reduction over blocks and Adam.
+// One Adam step over the sample ring is k_gtrain_step, then the fixed-order
+// reduction over blocks and Adam, with no host wait.  A k_gtrain_step block
+// takes TW * 16 samples per iteration, one 16-sample tile per wave:
+//   - the forward pass through all five layers and the back-propagation
+//     dY_5 -> dY_1, activations and gradients kept in registers (the inference
+//     kernel's chaining; the weights packed again as the A operand of W_l dY^T);
+//   - per layer l, from the top: every wave writes its H_l and dY_(l+1) tiles
+//     to LDS, feature-major over the block's samples, and the block takes
+//     dW_l = H_l^T dY_(l+1), one 16 x 16 tile per wave with K = the block's
+//     samples, and db_l (MFMAs with an all-ones A operand); layer 0's one-blob
+//     features are rebuilt in LDS 64 at a time;
+//   - each tile goes to the block's own partial gradient (stored on the
+//     block's first iteration, added by the same lane on later ones), so no
+//     activation leaves the CU and the sums keep a fixed order.
 // dY_(l+1) = d loss / d (H_l W_l + b_l); dY_l = (dY_(l+1) W_l^T) masked by
 // ReLU'(H_l), H_0 = the one-blob features.
-constexpr int TW = 16;  // waves per k_gtrain_fb block
+constexpr int TW = 16;               // waves per k_gtrain_step block
+constexpr int TB_S = TW * 16;        // samples per block iteration
+constexpr int TS_ROW = TB_S + 8;       // an LDS row: 16 bytes of padding put the 16 rows an operand reads in distinct banks
 
-struct FbSmem {
-    __bf16 w[G_PACK_ALL];      // forward fragments, then the back-propagation fragments
+struct StepSmem {
+    __bf16 w[G_PACK_ALL];  // forward fragments, then the back-propagation fragments
     float b[G_BIASES];
-    __bf16 stage[TW][G_HID][16];  // per-wave transpose: [feature][sample of the tile]
-    float smp[TW][16 * G_SAMPLE];  // each wave's tile of samples, row-major
+    __bf16 a[G_HID][TS_ROW];  // H_l, or 64 one-blob features, of the block's samples
+    union {
+        __bf16 d[G_HID][TS_ROW];       // dY_(l+1) of the block's samples
+        float smp[TW][16 * G_SAMPLE];  // during the forward pass: each wave's tile of samples, row-major
+    } u;
 };
 
 // Back-propagation weight fragment (A operand of W_l dY^T): inputs 16 m .., k-step s over outputs.
@@ -489,22 +497,15 @@ __device__ __forceinline__ bf16x8 wbfrag(const __bf16 *w, int l, int m, int s, i
     return *reinterpret_cast<const bf16x8 *>(&w[G_PACK + g_pack_bwd_off(l) + ((m * (g_out(l) / 32) + s) * 64 + lane) * 8]);
 }
 
-// A 64-feature tile held as two B fragments (sample lane % 16, features in
-// g_kperm order) -> dst[f * 32 + k0 + sample], f = 0..63, through the wave's
-// staging area; rows_valid = 0 skips the global stores.
-__device__ __forceinline__ void store_tile64(__bf16 (*st)[16], const bf16x8 (&h)[2], int lane, __bf16 *dst, int k0) {
+// A wave's tile of 32 * NF features held as B fragments (sample lane % 16,
+// features in g_kperm order) -> dst[feature][16 wave + sample]
+template <int NF>
+__device__ __forceinline__ void stage_tile(__bf16 (*dst)[TS_ROW], const bf16x8 (&h)[2], int wave, int lane) {
     const int r = lane % 16, g = lane / 16;
     #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int s = 0; s < NF; ++s)
         #pragma unroll
-        for (int i = 0; i < 8; ++i) st[32 * s + g_kperm(1, g, i)][r] = h[s][i];
-    wave_sync();
-    #pragma unroll
-    for (int q = lane; q < 128; q += 64) {
-        const int f = q / 2, hh = q % 2;
-        *reinterpret_cast<uint4 *>(dst + f * G_CHUNK + k0 + 8 * hh) = *reinterpret_cast<const uint4 *>(&st[f][8 * hh]);
-    }
-    wave_sync();
+        for (int i = 0; i < 8; ++i) dst[32 * s + g_kperm(1, g, i)][16 * wave + r] = h[s][i];
 }
 
 // d(-w log q(dir)) / d(raw outputs) with the 8 lobes of a row spread over its
@@ -591,45 +592,100 @@ __device__ __forceinline__ void bwd_layer(const __bf16 *W, int lane, const bf16x
     }
 }
 
-__global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, const float *bias, const float *samples,
-                                                       const uint32_t *n_dev, uint32_t cap, const float *wsum_dev,
-                                                       __bf16 *hT, __bf16 *dT) {
+// acc = A[16 mt ..][samples] x B[16 nt ..][samples]^T over the block's TB_S
+// samples; with ONES the A operand is all ones (every row: B's column sums)
+template <bool ONES>
+__device__ __forceinline__ f32x4 block_tile(const __bf16 (*A)[TS_ROW], int mt, const __bf16 (*Bm)[TS_ROW], int nt,
+                                            int lane) {
+    const int r = lane % 16, g = lane / 16;
+    bf16x8 ones;
+    #pragma unroll
+    for (int i = 0; i < 8; ++i) ones[i] = __bf16(1.0f);
+    f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    #pragma unroll 2
+    for (int ks = 0; ks < TB_S / 32; ++ks) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bm[16 * nt + r][32 * ks + 8 * g]);
+        const bf16x8 a = ONES ? ones : *reinterpret_cast<const bf16x8 *>(&A[16 * mt + r][32 * ks + 8 * g]);
+        acc[ks % 2] = mfma(a, b, acc[ks % 2]);
+    }
+    return acc[0] + acc[1];
+}
+
+// D layout (lane: column lane % 16, rows 4 (lane / 16) + j) -> rows 16 mt + ..,
+// columns 16 nt + .. of layer l's weight gradient in the block's partial
+__device__ __forceinline__ void put_w(float *part, int l, int mt, int nt, int lane, f32x4 v, bool first) {
+    const int r = lane % 16, g = lane / 16;
+    #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        float *q = part + g_w_off(l) + (16 * mt + 4 * g + j) * g_out(l) + 16 * nt + r;
+        *q = first ? v[j] : *q + v[j];
+    }
+}
+// row 0 of a column-sum tile -> biases 16 nt .. 16 nt + 15 of layer l
+__device__ __forceinline__ void put_b(float *part, int l, int nt, int lane, f32x4 v, bool first) {
+    if (lane < 16) {
+        float *q = part + g_b_off(l) + 16 * nt + lane;
+        *q = first ? v[0] : *q + v[0];
+    }
+}
+
+// Layer L (1..3) over the block's samples: H_L and dY_(L+1) to LDS, the
+// back-propagation dY_L into out, then dW_L (wave: rows 16 (wave / 4) ..,
+// columns 16 (wave % 4) ..) and db_L (waves 4 L .. 4 L + 3)
+template <int L>
+__device__ __forceinline__ void train_layer64(StepSmem &sm, const __bf16 *W, const bf16x8 (&h)[2],
+                                              const bf16x8 (&dy)[2], bf16x8 (&out)[2], float *part, int wave,
+                                              int lane, bool first) {
+    __syncthreads();  // the layer above has read sm.a and sm.u.d
+    stage_tile<2>(sm.a, h, wave, lane);
+    stage_tile<2>(sm.u.d, dy, wave, lane);
+    bwd_layer<L>(W, lane, dy, h, out);
+    __syncthreads();
+    put_w(part, L, wave / 4, wave % 4, lane, block_tile<false>(sm.a, wave / 4, sm.u.d, wave % 4, lane), first);
+    if (wave / 4 == L) put_b(part, L, wave % 4, lane, block_tile<true>(sm.a, 0, sm.u.d, wave % 4, lane), first);
+}
+
+__global__ void __launch_bounds__(TW * 64) k_gtrain_step(const __bf16 *packed, const float *bias,
+                                                         const float *samples, const uint32_t *n_dev, uint32_t cap,
+                                                         const float *wsum_dev, float *partial) {
+    static_assert(TW == 16, "the dW tiling and the one-blob staging assume 16 waves");
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    FbSmem &sm = *reinterpret_cast<FbSmem *>(smem_raw);
+    StepSmem &sm = *reinterpret_cast<StepSmem *>(smem_raw);
     const uint32_t n = min(*n_dev, cap);
-    const uint32_t tiles = 2u * uint32_t((n + G_CHUNK - 1) / G_CHUNK);  // whole chunks: rows past n get w = 0
+    const uint32_t tiles = (n + 15) / 16;  // rows past n in the last tile get w = 0
     const float wsum = *wsum_dev;
     const float wscale = wsum > 0.0f ? 1.0f / wsum : 0.0f;
-    const size_t lstride = g_chunks(cap) * G_HID * G_CHUNK;  // one layer of hT / dT
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    if (uint32_t(blockIdx.x) * TW >= tiles) return;  // whole block past the tiles (block-uniform)
+    float *part = partial + size_t(blockIdx.x) * G_PARAMS;
+    if (uint32_t(blockIdx.x) * TW >= tiles) {  // no samples for this block (block-uniform): a zero partial
+        for (int i = tid; i < G_PARAMS; i += TW * 64) part[i] = 0.0f;
+        return;
+    }
     {
         const uint4 *src = reinterpret_cast<const uint4 *>(packed);
         uint4 *dst = reinterpret_cast<uint4 *>(sm.w);
         for (int i = tid; i < G_PACK_ALL / 8; i += TW * 64) dst[i] = src[i];
         for (int i = tid; i < G_BIASES; i += TW * 64) sm.b[i] = bias[i];
     }
-    __syncthreads();
     const int r = lane % 16, g = lane / 16;
-    __bf16 (*st)[16] = sm.stage[wave];
-    for (uint32_t tile = blockIdx.x * TW + uint32_t(wave); tile < tiles; tile += gridDim.x * TW) {
+    // block-uniform trip count: every wave reaches every barrier
+    for (uint32_t base = blockIdx.x * TW; base < tiles; base += gridDim.x * TW) {
+        const bool first = base == blockIdx.x * TW;
+        const uint32_t tile = base + uint32_t(wave);
         int wz = 0;
         asm volatile("" : "+s"(wz));
         const __bf16 *W = sm.w + wz;
         const float *B = sm.b + wz;
         const uint32_t row = tile * 16u + uint32_t(r);
         const bool valid = row < n;
-        const size_t cbase = size_t(tile / 2) * G_HID * G_CHUNK;
-        const int k0 = 16 * int(tile % 2);
-        // the tile's 16 samples (12 floats each), staged in LDS with 16-byte loads;
-        // rows past n read as zeros
-        float *xs = sm.smp[wave];
-        wave_sync();  // the previous tile's reads of xs are done
+        __syncthreads();  // the weights are in LDS; the previous iteration has read sm.a and sm.u
+        // the tile's 16 samples (12 floats each), staged in LDS with 16-byte loads; rows past n read as zeros
+        float *xs = sm.u.smp[wave];
         if (lane < 16 * G_SAMPLE / 4) {
             const uint32_t rr = tile * 16u + uint32_t(lane / (G_SAMPLE / 4));
-            const float4 a = rr < n ? *reinterpret_cast<const float4 *>(samples + size_t(tile) * 16 * G_SAMPLE + 4 * lane)
+            const float4 v = rr < n ? *reinterpret_cast<const float4 *>(samples + size_t(tile) * 16 * G_SAMPLE + 4 * lane)
                                     : make_float4(0, 0, 0, 0);
-            *reinterpret_cast<float4 *>(xs + 4 * lane) = a;
+            *reinterpret_cast<float4 *>(xs + 4 * lane) = v;
         }
         wave_sync();
         const float *sx = xs + r * G_SAMPLE;
@@ -646,16 +702,12 @@ __global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, con
         }
         bf16x8 h1[1][2], h2[1][2], h3[1][2], h4[1][2];
         relu_frags<1>(acc, h1);
-        store_tile64(st, h1[0], lane, hT + 0 * lstride + cbase, k0);
         chained_layer<1, 1>(W, B, lane, h1, acc);
         relu_frags<1>(acc, h2);
-        store_tile64(st, h2[0], lane, hT + 1 * lstride + cbase, k0);
         chained_layer<2, 1>(W, B, lane, h2, acc);
         relu_frags<1>(acc, h3);
-        store_tile64(st, h3[0], lane, hT + 2 * lstride + cbase, k0);
         chained_layer<3, 1>(W, B, lane, h3, acc);
         relu_frags<1>(acc, h4);
-        store_tile64(st, h4[0], lane, hT + 3 * lstride + cbase, k0);
         chained_layer<4, 1>(W, B, lane, h4, acc);
         // d loss / d outputs: lobes g and 4 + g of row r
         float dir[3] = {0.0f, 0.0f, 1.0f};
@@ -678,117 +730,52 @@ __global__ void __launch_bounds__(TW * 64) k_gtrain_fb(const __bf16 *packed, con
                              bf16_pair(gr[1][2], gr[1][3])};
             dy5[0] = __builtin_bit_cast(bf16x8, u);
             dy5[1] = dy5[0];  // unused: the output layer has one k-step
-            // dY_5: rows 16 t + 4 g + c of the chunk's 32-feature block
-            #pragma unroll
-            for (int i = 0; i < 8; ++i) st[g_kperm(1, g, i)][r] = dy5[0][i];
-            wave_sync();
-            const int f = lane / 2, hh = lane % 2;
-            *reinterpret_cast<uint4 *>(dT + 4 * lstride + cbase + f * G_CHUNK + k0 + 8 * hh) =
-                *reinterpret_cast<const uint4 *>(&st[f][8 * hh]);
-            wave_sync();
         }
-        // back-propagation
-        bf16x8 d4[2], d3[2], d2[2], d1[2];
+        // layer 4 (64 -> 32): dW_4 on waves 0..7, db_4 on waves 8 and 9
+        __syncthreads();  // every wave has read its samples: sm.u holds dY from here on
+        stage_tile<2>(sm.a, h4[0], wave, lane);
+        stage_tile<1>(sm.u.d, dy5, wave, lane);
+        bf16x8 d4[2];
         bwd_layer<4>(W, lane, dy5, h4[0], d4);
-        store_tile64(st, d4, lane, dT + 3 * lstride + cbase, k0);
-        bwd_layer<3>(W, lane, d4, h3[0], d3);
-        store_tile64(st, d3, lane, dT + 2 * lstride + cbase, k0);
-        bwd_layer<2>(W, lane, d3, h2[0], d2);
-        store_tile64(st, d2, lane, dT + 1 * lstride + cbase, k0);
-        bwd_layer<1>(W, lane, d2, h1[0], d1);
-        store_tile64(st, d1, lane, dT + 0 * lstride + cbase, k0);
-    }
-}
-
-// dW / db of every layer over the block's chunks.  Tiles of 16 inputs x 16
-// outputs, one MFMA per tile and chunk: waves 0..3 take input tiles 4 w ..
-// 4 w + 3 of layer 0 against its four output tiles; wave 4 + l' takes all of
-// layer l' + 1 (4 x 4 tiles, layer 4: 4 x 2), so each operand fragment is
-// loaded once per chunk.  db_l (the column sums of dY_(l+1) over the samples)
-// comes from MFMAs with an all-ones A operand (wave 0 for layer 0).
-constexpr int DW_WAVES = 8;
-constexpr int DW_BATCH = 8;  // chunks whose one-blob features are in LDS at once
-
-__global__ void __launch_bounds__(DW_WAVES * 64) k_gtrain_dw(const float *samples, const uint32_t *n_dev, uint32_t cap,
-                                                            const __bf16 *hT, const __bf16 *dT, float *partial) {
-    // one-blob features of up to DW_BATCH chunks, feature-major (128 KB)
-    __shared__ __align__(16) __bf16 x0[DW_BATCH][G_FEAT][G_CHUNK];
-    const uint32_t n = min(*n_dev, cap);
-    const uint32_t chunks = (n + G_CHUNK - 1) / G_CHUNK;
-    const size_t lstride = g_chunks(cap) * G_HID * G_CHUNK;
-    const uint32_t per = (chunks + gridDim.x - 1) / gridDim.x;
-    const uint32_t c_begin = blockIdx.x * per, c_end = min(chunks, c_begin + per);
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int r = lane % 16, g = lane / 16;
-    // this wave's layer and tile grid
-    const int L = wave < 4 ? 0 : wave - 3;
-    const int NT = g_out(L) / 16;                 // output tiles (4, or 2 for layer 4)
-    bf16x8 ones;
-    #pragma unroll
-    for (int i = 0; i < 8; ++i) ones[i] = __bf16(1.0f);
-    f32x4 acc[16], accb[4];
-    #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = f32x4{0, 0, 0, 0};
-    #pragma unroll
-    for (int j = 0; j < 4; ++j) accb[j] = f32x4{0, 0, 0, 0};
-    for (uint32_t cb = c_begin; cb < c_end; cb += DW_BATCH) {
-        const uint32_t nb = min(uint32_t(DW_BATCH), c_end - cb);
-        __syncthreads();  // the previous batch's features are read
-        // item (sample k, input s, bins 8 part ..): rows past n take input 0 (their dY is 0)
-        for (int it = tid; it < G_CHUNK * G_IN * 4; it += DW_WAVES * 64) {
-            const int k = it % G_CHUNK, s = (it / G_CHUNK) % G_IN, part = it / (G_CHUNK * G_IN);
-            for (uint32_t q = 0; q < nb; ++q) {
-                const uint32_t row = (cb + q) * G_CHUNK + uint32_t(k);
-                const float x = row < n ? fminf(1.0f, fmaxf(0.0f, samples[size_t(row) * G_SAMPLE + s])) : 0.0f;
-                const bf16x8 v = one_blob8(x, part);
-                #pragma unroll
-                for (int i = 0; i < 8; ++i) x0[q][32 * s + 8 * part + i][k] = v[i];
-            }
-        }
         __syncthreads();
-        for (uint32_t q = 0; q < nb; ++q) {
-            const size_t cbase = size_t(cb + q) * G_HID * G_CHUNK;
-            bf16x8 b[4], a[4];
+        if (wave < 8)
+            put_w(part, 4, wave / 2, wave % 2, lane, block_tile<false>(sm.a, wave / 2, sm.u.d, wave % 2, lane), first);
+        else if (wave < 10)
+            put_b(part, 4, wave - 8, lane, block_tile<true>(sm.a, 0, sm.u.d, wave - 8, lane), first);
+        bf16x8 d3[2], d2[2], d1[2];
+        train_layer64<3>(sm, W, h3[0], d4, d3, part, wave, lane, first);
+        train_layer64<2>(sm, W, h2[0], d3, d2, part, wave, lane, first);
+        train_layer64<1>(sm, W, h1[0], d2, d1, part, wave, lane, first);
+        // layer 0 (256 -> 64): dY_1 stays in LDS while the one-blob features
+        // of inputs 2 p and 2 p + 1 (64 features) take sm.a in pass p
+        __syncthreads();  // layer 1 has read sm.a and sm.u.d
+        stage_tile<2>(sm.u.d, d1, wave, lane);
+        // thread: samples 2 sp and 2 sp + 1 of the block, bins 8 part8 .. + 7 of input 2 p + ii
+        const int sp = tid % (TB_S / 2), part8 = (tid / (TB_S / 2)) % 4, ii = tid / (2 * TB_S);
+        for (int p = 0; p < G_IN / 2; ++p) {
+            float x[2];
             #pragma unroll
-            for (int nt = 0; nt < 4; ++nt)
-                if (nt < NT)
-                    b[nt] = *reinterpret_cast<const bf16x8 *>(dT + size_t(L) * lstride + cbase + (16 * nt + r) * G_CHUNK + 8 * g);
-            #pragma unroll
-            for (int m = 0; m < 4; ++m)
-                a[m] = wave < 4 ? *reinterpret_cast<const bf16x8 *>(&x0[q][16 * (4 * wave + m) + r][8 * g])
-                                : *reinterpret_cast<const bf16x8 *>(hT + size_t(L - 1) * lstride + cbase +
-                                                                    (16 * m + r) * G_CHUNK + 8 * g);
-            #pragma unroll
-            for (int m = 0; m < 4; ++m)
-                #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    if (nt < NT) acc[4 * m + nt] = mfma(a[m], b[nt], acc[4 * m + nt]);
-            if (wave >= 3) {  // bias sums: wave 3 for layer 0, waves 4..7 for their layer
-                #pragma unroll
-                for (int nt = 0; nt < 4; ++nt)
-                    if (nt < NT) accb[nt] = mfma(ones, b[nt], accb[nt]);
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t rw = base * 16u + uint32_t(2 * sp + q);
+                x[q] = rw < n ? fminf(1.0f, fmaxf(0.0f, samples[size_t(rw) * G_SAMPLE + 2 * p + ii])) : 0.0f;
             }
-        }
-    }
-    // this block's partial gradient (zeros when it had no chunks): tile rows 4 g + j, column r
-    float *part = partial + size_t(blockIdx.x) * G_PARAMS;
-    const int m0 = wave < 4 ? 4 * wave : 0;
-    #pragma unroll
-    for (int m = 0; m < 4; ++m)
-        #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-            if (nt < NT)
+            if (p) __syncthreads();  // pass p - 1 has read sm.a (the loads above are in flight meanwhile)
+            {
+                const bf16x8 v0 = one_blob8(x[0], part8), v1 = one_blob8(x[1], part8);
                 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    part[g_w_off(L) + (16 * (m0 + m) + 4 * g + j) * g_out(L) + 16 * nt + r] = acc[4 * m + nt][j];
-    if (wave >= 3 && g == 0) {  // every row of accb holds the column sums
-        #pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-            if (nt < NT) part[g_b_off(L) + 16 * nt + r] = accb[nt][0];
+                for (int k = 0; k < 8; ++k) {
+                    const bf16x2 pr = {v0[k], v1[k]};
+                    *reinterpret_cast<bf16x2 *>(&sm.a[32 * ii + 8 * part8 + k][2 * sp]) = pr;
+                }
+            }
+            __syncthreads();
+            put_w(part, 0, 4 * p + wave / 4, wave % 4, lane, block_tile<false>(sm.a, wave / 4, sm.u.d, wave % 4, lane),
+                  first);
+            if (p == 0 && wave < 4) put_b(part, 0, wave, lane, block_tile<true>(sm.a, 0, sm.u.d, wave, lane), first);
+        }
     }
 }
 
-// Sum of the per-block partial gradients in a fixed order.
 // one block: advance the ring, then a fixed-order sum of the valid weights
 __global__ void __launch_bounds__(1024) k_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap,
                                                             const float *samples, float *wsum) {
@@ -848,16 +835,33 @@ __global__ void __launch_bounds__(256) k_guide_reduce(const float *partial, int 
 __device__ __forceinline__ void adam_update(int i, float g, float *params, float *m, float *v, float lr, float b1,
                                             float b2, float eps, int t, __bf16 *packed, float *bias);
 
-// The second pass of the fixed-order reduction fused with Adam: gradient i =
-// the group sums of k_guide_reduce_groups added in group order.
-__global__ void __launch_bounds__(256) k_guide_reduce_adam(const float *partial, int blocks, int per, float *params,
-                                                           float *m, float *v, float lr, float b1, float b2, float eps,
-                                                           int t, __bf16 *packed, float *bias) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= G_PARAMS) return;
-    float g = 0;
-    for (int b = 0; b < blocks; b += per) g += partial[size_t(b) * G_PARAMS + i];
-    adam_update(i, g, params, m, v, lr, b1, b2, eps, t, packed, bias);
+// The fixed-order reduction and Adam in one launch: a block takes RA_COLS
+// parameters; its thread (j, y) adds the partials of group y (blocks y per ..
+// y per + per - 1, in order) of parameter j, and the group sums are then
+// added in group order: the sums and their order are k_guide_reduce_groups'
+// and k_guide_reduce's, with no pass through memory in between.
+constexpr int RA_COLS = 64;
+
+__global__ void __launch_bounds__(RA_COLS * GR_GROUPS) k_guide_reduce_adam(const float *partial, int blocks, int per,
+                                                                           float *params, float *m, float *v, float lr,
+                                                                           float b1, float b2, float eps, int t,
+                                                                           __bf16 *packed, float *bias) {
+    __shared__ float gs[GR_GROUPS][RA_COLS];
+    const int j = threadIdx.x % RA_COLS, y = threadIdx.x / RA_COLS;
+    const int i = blockIdx.x * RA_COLS + j;
+    const int b0 = y * per;
+    float g = 0.0f;
+    if (i < G_PARAMS && b0 < blocks) {
+        const int b1g = min(blocks, b0 + per);
+        g = partial[size_t(b0) * G_PARAMS + i];
+        for (int b = b0 + 1; b < b1g; ++b) g += partial[size_t(b) * G_PARAMS + i];
+    }
+    gs[y][j] = g;
+    __syncthreads();
+    if (y != 0 || i >= G_PARAMS) return;
+    float sum = 0.0f;
+    for (int yy = 0; yy < GR_GROUPS && yy * per < blocks; ++yy) sum += gs[yy][j];
+    adam_update(i, sum, params, m, v, lr, b1, b2, eps, t, packed, bias);
 }
 
 // One Adam step on the FP32 master parameters: m, v moments; lr; bias
@@ -905,7 +909,7 @@ __global__ void __launch_bounds__(256) k_guide_adam(float *params, float *m, flo
 
 }  // namespace
 
-size_t guide_smem_bytes() { return sizeof(InferSmem) > sizeof(FbSmem) ? sizeof(InferSmem) : sizeof(FbSmem); }
+size_t guide_smem_bytes() { return sizeof(InferSmem) > sizeof(StepSmem) ? sizeof(InferSmem) : sizeof(StepSmem); }
 size_t guide_train_smem_bytes() { return sizeof(TrainSmem); }
 
 void launch_guide_train(const float *params, const float *samples, const uint32_t *n_dev, uint32_t cap,
@@ -914,16 +918,10 @@ void launch_guide_train(const float *params, const float *samples, const uint32_
                        wsum_dev, partial);
 }
 
-void launch_guide_train_mfma(const float *params, const __bf16 *packed, const float *bias, const float *samples,
-                             const uint32_t *n_dev, uint32_t cap, const float *wsum_dev, __bf16 *acts, __bf16 *dy,
-                             float *partial, int blocks, hipStream_t st) {
-    (void)params;
-    // forward + back-propagation: one 16-sample tile per wave, blocks of TW waves
-    const uint32_t tiles = uint32_t(2 * g_chunks(cap));
-    const uint32_t fb_blocks = (tiles + TW - 1) / TW;
-    hipLaunchKernelGGL(k_gtrain_fb, dim3(fb_blocks < 256u ? fb_blocks : 256u), dim3(TW * 64), sizeof(FbSmem), st,
-                       packed, bias, samples, n_dev, cap, wsum_dev, acts, dy);
-    hipLaunchKernelGGL(k_gtrain_dw, dim3(blocks), dim3(DW_WAVES * 64), 0, st, samples, n_dev, cap, acts, dy, partial);
+void launch_guide_train_mfma(const __bf16 *packed, const float *bias, const float *samples, const uint32_t *n_dev,
+                             uint32_t cap, const float *wsum_dev, float *partial, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_gtrain_step, dim3(blocks), dim3(TW * 64), sizeof(StepSmem), st, packed, bias, samples, n_dev,
+                       cap, wsum_dev, partial);
 }
 
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
@@ -938,13 +936,11 @@ void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st
     hipLaunchKernelGGL(k_guide_reduce, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, per, grad);
 }
 
-void launch_guide_reduce_adam(float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
+void launch_guide_reduce_adam(const float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
                               float b2, float eps, int t, hipStream_t st, __bf16 *packed, float *bias) {
     const int per = (blocks + GR_GROUPS - 1) / GR_GROUPS;
-    hipLaunchKernelGGL(k_guide_reduce_groups, dim3((G_PARAMS + 255) / 256, GR_GROUPS), dim3(256), 0, st, partial,
-                       blocks, per);
-    hipLaunchKernelGGL(k_guide_reduce_adam, dim3((G_PARAMS + 255) / 256), dim3(256), 0, st, partial, blocks, per,
-                       params, m, v, lr, b1, b2, eps, t, packed, bias);
+    hipLaunchKernelGGL(k_guide_reduce_adam, dim3((G_PARAMS + RA_COLS - 1) / RA_COLS), dim3(RA_COLS * GR_GROUPS), 0, st,
+                       partial, blocks, per, params, m, v, lr, b1, b2, eps, t, packed, bias);
 }
 
 void launch_guide_adam(float *params, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
@@ -961,7 +957,7 @@ hipError_t guide_prepare(int device) {
     int lds = 0;
     hipError_t e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) return e;
-    if (size_t(lds) < sizeof(InferSmem) || size_t(lds) < sizeof(TrainSmem) || size_t(lds) < sizeof(FbSmem))
+    if (size_t(lds) < sizeof(InferSmem) || size_t(lds) < sizeof(TrainSmem) || size_t(lds) < sizeof(StepSmem))
         return hipErrorInvalidConfiguration;
     for (const void *k : {reinterpret_cast<const void *>(k_guide_infer<1>), reinterpret_cast<const void *>(k_guide_infer<4>)}) {
         e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(InferSmem)));
@@ -970,8 +966,8 @@ hipError_t guide_prepare(int device) {
     e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_guide_train),
                             hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(TrainSmem)));
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_fb), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               int(sizeof(FbSmem)));
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(k_gtrain_step),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(sizeof(StepSmem)));
 }
 
 void launch_guide_infer(const __bf16 *packed, const float *bias, const float *in, float *out, const uint32_t *n_dev,

@@ -27,20 +27,20 @@ void launch_guide_train(const float *params, const float *samples, const uint32_
                         const float *wsum_dev, float *partial, int blocks, hipStream_t st);
 // sums the per-block partial gradients in a fixed order (overwrites partial)
 void launch_guide_reduce(float *partial, int blocks, float *grad, hipStream_t st);
-// the same gradient on the matrix cores (BF16 operands, FP32 accumulation): forward
-// of the ring's samples into acts (4 x cap x 64 bf16), backward through dy
-// (2 x cap x 64 bf16); blocks x G_PARAMS partial gradients
-void launch_guide_train_mfma(const float *params, const __bf16 *packed, const float *bias, const float *samples,
-                             const uint32_t *n_dev, uint32_t cap, const float *wsum_dev, __bf16 *acts, __bf16 *dy,
-                             float *partial, int blocks, hipStream_t st);
+// the same gradient on the matrix cores (BF16 operands, FP32 accumulation),
+// one fused kernel over blocks x (16 x 16)-sample iterations: blocks x G_PARAMS
+// partial gradients (every block writes its whole partial)
+void launch_guide_train_mfma(const __bf16 *packed, const float *bias, const float *samples, const uint32_t *n_dev,
+                             uint32_t cap, const float *wsum_dev, float *partial, int blocks, hipStream_t st);
 // sample ring: ring[0] = next write slot, ring[1] = valid samples (<= cap); a
 // gather appended *added samples from slot ring[0] on; advance the ring and
 // sum the weights of the valid samples into *wsum in a fixed order
 void launch_guide_ring_commit(uint32_t *ring, const uint32_t *added, uint32_t cap, const float *samples,
                               float *wsum, hipStream_t st);
 // the fixed-order sum of the per-block partial gradients fused with one Adam
-// step (packed / bias as below)
-void launch_guide_reduce_adam(float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
+// step (packed / bias as below), in one launch; the sums and their order are
+// launch_guide_reduce's (partial is left as it is)
+void launch_guide_reduce_adam(const float *partial, int blocks, float *params, float *m, float *v, float lr, float b1,
                               float b2, float eps, int t, hipStream_t st, __bf16 *packed, float *bias);
 // packed / bias (may be NULL): also refresh the BF16 weights and biases that
 // launch_guide_pack would produce from the updated parameters
