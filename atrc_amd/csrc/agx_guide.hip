@@ -498,14 +498,27 @@ __device__ __forceinline__ bf16x8 wbfrag(const __bf16 *w, int l, int m, int s, i
 }
 
 // A wave's tile of 32 * NF features held as B fragments (sample lane % 16,
-// features in g_kperm order) -> dst[feature][16 wave + sample]
+// features in g_kperm order) -> dst[feature][16 wave + sample].  Lanes of
+// neighbouring samples r and r ^ 1 swap halves (a DPP quad permute), so each
+// lane writes whole dwords: the even lane feature 2 d's pair of samples, the
+// odd lane feature 2 d + 1's, for the d-th dword of each fragment.
 template <int NF>
 __device__ __forceinline__ void stage_tile(__bf16 (*dst)[TS_ROW], const bf16x8 (&h)[2], int wave, int lane) {
-    const int r = lane % 16, g = lane / 16;
+    const int r = lane % 16, g = lane / 16, odd = r & 1;
+    __bf16 *base = &dst[0][16 * wave + (r & ~1)];
     #pragma unroll
-    for (int s = 0; s < NF; ++s)
+    for (int s = 0; s < NF; ++s) {
+        const u32x4 u = __builtin_bit_cast(u32x4, h[s]);
         #pragma unroll
-        for (int i = 0; i < 8; ++i) dst[32 * s + g_kperm(1, g, i)][16 * wave + r] = h[s][i];
+        for (int d = 0; d < 4; ++d) {
+            const uint32_t mine = u[d];
+            // quad_perm [1, 0, 3, 2]: the value of lane r ^ 1
+            const uint32_t other = uint32_t(__builtin_amdgcn_update_dpp(0, int(mine), 0xB1, 0xF, 0xF, false));
+            const uint32_t word = odd ? (mine & 0xFFFF0000u) | (other >> 16) : (other << 16) | (mine & 0xFFFFu);
+            const int i = 2 * d + odd;
+            *reinterpret_cast<uint32_t *>(base + (32 * s + g_kperm(1, g, i)) * TS_ROW) = word;
+        }
+    }
 }
 
 // d(-w log q(dir)) / d(raw outputs) with the 8 lobes of a row spread over its
