@@ -664,19 +664,35 @@ static_assert((BVH4_RING_CLOSEST & (BVH4_RING_CLOSEST - 1)) == 0 && (BVH4_RING_A
 
 // Closest hit over the BVH4: children visited near-first, pending children
 // kept with their entry distance and skipped once a closer hit is known.
-// ring = this lane's column of a [BVH4_RING_CLOSEST][TB] LDS array.
+// ring = this lane's column of a [BVH4_RING_CLOSEST][TB] LDS array.  The state
+// of one lane's traversal: start() takes a ray, step() does one node visit, or
+// one leaf and the pops that follow it, and returns true once the traversal is
+// over (hit() then tells whether it found a hit, rec the hit).  The stack
+// entries below the ring live in the caller's spill array (a separate local,
+// so that the state itself stays in registers).
 template <bool STATS, int CAP, bool WT, int TB>
-__device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c,
-                           uint2 *ring) {
-    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
-    const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
-    const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
-    const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
-    uint2 spill[CAP - BVH4_RING_CLOSEST];  // (node code, entry distance bits)
-    int sp = 0;
-    uint32_t code = 0;
-    int32_t best_rank = -1;  // tie rank of the entity of the current hit, -1 = no hit yet
-    for (;;) {
+struct B4Closest {
+    static constexpr int SPILL = CAP - BVH4_RING_CLOSEST;  // (node code, entry distance bits)
+    V o, d, inv;
+    float tmin, tmax;
+    WtRay wr;
+    uint32_t code;
+    int sp;
+    int32_t best_rank;  // tie rank of the entity of the current hit, -1 = no hit yet
+    HitRec rec;
+
+    __device__ __forceinline__ void start(V o_, V d_, float tmin_, float tmax_) {
+        o = o_; d = d_; tmin = tmin_; tmax = tmax_;
+        inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+        if (WT) wr = wt_ray(o, d);
+        code = 0; sp = 0; best_rank = -1;
+        rec.ent = -1; rec.prim = -1; rec.t = -1; rec.u = 0; rec.v = 0;
+    }
+    __device__ __forceinline__ bool hit() const { return best_rank >= 0; }
+
+    __device__ __forceinline__ bool step(const DevScene &s, Counts &c, uint2 *ring, uint2 (&spill)[SPILL]) {
+        const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
+        const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
         if (!(code & BVH4_LEAF)) {
             if (STATS) c.visits++;
             float t[4];
@@ -691,7 +707,7 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                 if (t[1] != INF_F && sp < CAP)
                     ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[1], __float_as_uint(t[1])));
                 code = k[0];
-                continue;
+                return false;
             }
         } else {
             const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
@@ -709,10 +725,10 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                     // an exact tie with another entity's hit goes to the entity that
                     // Atrc's aggregate tests later (its tie rank)
                     const int32_t rank = __float_as_int(p2.w);
-                    if (hit_tri && (tt < tmax || ent == h->ent || rank > best_rank)) {
+                    if (hit_tri && (tt < tmax || ent == rec.ent || rank > best_rank)) {
                         tmax = tt;
-                        h->t = tt; h->u = uu; h->v = vv; h->ent = ent;
-                        h->prim = (WT && slot >= BVH4_QUAD_ACD) ? (slot == BVH4_QUAD_ABC ? 0 : 1) : int32_t(slot);
+                        rec.t = tt; rec.u = uu; rec.v = vv; rec.ent = ent;
+                        rec.prim = (WT && slot >= BVH4_QUAD_ACD) ? (slot == BVH4_QUAD_ABC ? 0 : 1) : int32_t(slot);
                         best_rank = rank;
                     }
                 } else {
@@ -720,7 +736,7 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
                     // against the next float below the current hit distance
                     const int32_t rank = __float_as_int(P[3 * size_t(i) + 2].w);
                     float tb = rank < best_rank ? __uint_as_float(__float_as_uint(tmax) - 1u) : tmax;
-                    if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tb, h, c)) {
+                    if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tb, &rec, c)) {
                         tmax = tb;
                         best_rank = rank;
                     }
@@ -729,25 +745,50 @@ __device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, 
         }
         // next pending child that may still hold a closer hit
         for (;;) {
-            if (sp == 0) return best_rank >= 0;
+            if (sp == 0) return true;
             const uint2 e = ring_pop<BVH4_RING_CLOSEST, TB>(ring, spill, sp);
-            if (__uint_as_float(e.y) <= tmax) { code = e.x; break; }
+            if (__uint_as_float(e.y) <= tmax) { code = e.x; return false; }
         }
     }
+};
+
+template <bool STATS, int CAP, bool WT, int TB>
+__device__ bool b4_closest(const DevScene &s, V o, V d, float tmin, float tmax, HitRec *h, Counts &c,
+                           uint2 *ring) {
+    B4Closest<STATS, CAP, WT, TB> q;
+    uint2 spill[B4Closest<STATS, CAP, WT, TB>::SPILL];
+    q.start(o, d, tmin, tmax);
+    q.rec = *h;
+    while (!q.step(s, c, ring, spill)) {
+    }
+    *h = q.rec;
+    return q.hit();
 }
 
-// Any hit over the BVH4; ring = this lane's column of a
-// [BVH4_RING_ANY][TB] LDS array.
+// Any hit over the BVH4; ring = this lane's column of a [BVH4_RING_ANY][TB]
+// LDS array.  The same shape as B4Closest: step() does one node visit, or one
+// leaf and the pop after it, and returns true once the traversal is over
+// (occluded then tells whether it found a hit).
 template <bool STATS, int CAP, bool WT, int TB>
-__device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c, uint32_t *ring) {
-    const V inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
-    const WtRay wr = WT ? wt_ray(o, d) : WtRay{};
-    const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
-    const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
-    uint32_t spill[CAP - BVH4_RING_ANY];
-    int sp = 0;
-    uint32_t code = 0;
-    for (;;) {
+struct B4Any {
+    static constexpr int SPILL = CAP - BVH4_RING_ANY;
+    V o, d, inv;
+    float tmin, tmax;
+    WtRay wr;
+    uint32_t code;
+    int sp;
+    bool occluded;
+
+    __device__ __forceinline__ void start(V o_, V d_, float tmin_, float tmax_) {
+        o = o_; d = d_; tmin = tmin_; tmax = tmax_;
+        inv = mk(1 / d.x, 1 / d.y, 1 / d.z);
+        if (WT) wr = wt_ray(o, d);
+        code = 0; sp = 0; occluded = false;
+    }
+
+    __device__ __forceinline__ bool step(const DevScene &s, Counts &c, uint32_t *ring, uint32_t (&spill)[SPILL]) {
+        const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
+        const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
         if (!(code & BVH4_LEAF)) {
             if (STATS) c.visits++;
             float t[4];
@@ -759,7 +800,7 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
                 if (next == BVH4_EMPTY) next = k[j];
                 else if (sp < CAP) ring_push<BVH4_RING_ANY, TB>(ring, spill, sp, k[j]);
             }
-            if (next != BVH4_EMPTY) { code = next; continue; }
+            if (next != BVH4_EMPTY) { code = next; return false; }
         } else {
             const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
             for (uint32_t i = first; i < first + cnt; ++i) {
@@ -770,18 +811,31 @@ __device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Coun
                     const float4 p2 = P[3 * size_t(i) + 2];
                     if (WT) {
                         float tt, uu, vv;
-                        if (wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)) return true;
+                        if (wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)) { occluded = true; return true; }
                     } else if (mt_any(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2))) {
+                        occluded = true;
                         return true;
                     }
                 } else if (analytic_any<STATS>(s.ents[__float_as_int(p0.w)], o, d, tmin, tmax, c)) {
+                    occluded = true;
                     return true;
                 }
             }
         }
-        if (sp == 0) return false;
+        if (sp == 0) return true;
         code = ring_pop<BVH4_RING_ANY, TB>(ring, spill, sp);
+        return false;
     }
+};
+
+template <bool STATS, int CAP, bool WT, int TB>
+__device__ bool b4_any(const DevScene &s, V o, V d, float tmin, float tmax, Counts &c, uint32_t *ring) {
+    B4Any<STATS, CAP, WT, TB> q;
+    uint32_t spill[B4Any<STATS, CAP, WT, TB>::SPILL];
+    q.start(o, d, tmin, tmax);
+    while (!q.step(s, c, ring, spill)) {
+    }
+    return q.occluded;
 }
 
 // Traversal variants: Atrc's structures with the private or the LDS stack, or
