@@ -73,7 +73,7 @@ struct agx_ctx {
     size_t pass_paths = 0;
     int pass_paths_L = -1;
     size_t pass_paths_limit = 0;  // 0: none; batch workers, which render side by side, use 2^22
-    int trace_block = 256;        // threads per block of the BVH4 traversal launches (128 or 256)
+    int trace_block = 128;        // threads per block of the BVH4 traversal launches (128 or 256)
     int64_t cap_wh = 0;
     uint32_t *counters = nullptr;    // device: two sets of 4 queue sizes (ping-pong)
     DevCounters *dcnt = nullptr;
@@ -505,10 +505,12 @@ static int upload_host_scene(agx_ctx *c, const HostScene &hs) {
     ds.grids = gr;
     ds.grids16 = gr16;
     ds.accel = hs.accel;
-    // AGX_TRACE_BLOCK=128 selects 128-thread traversal blocks for contexts left at the default
+    // AGX_TRACE_BLOCK=128 or 256 sets the traversal block size of contexts left at the default (128)
     ds.trace_block = c->trace_block;
-    if (const char *e = std::getenv("AGX_TRACE_BLOCK"))
-        if (std::string(e) == "128" && c->trace_block == 256) ds.trace_block = 128;
+    if (const char *e = std::getenv("AGX_TRACE_BLOCK")) {
+        if (std::string(e) == "256") ds.trace_block = 256;
+        else if (std::string(e) == "128") ds.trace_block = 128;
+    }
     ds.b4_nodes = b4n;
     ds.b4_prims = b4p;
     // a BVH4 traversal keeps at most 3 pending children per level
@@ -1412,7 +1414,7 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     auto trace = [&](const char *what, uint32_t g) {
         if (batch_trace()) std::fprintf(stderr, "[batch graph] %8.1f ms  %s %u\n", ms_since(t0), what, g);
     };
-    Config cfg{branches > 0 ? branches : 4, 22, 256};
+    Config cfg{branches > 0 ? branches : 4, 22, 128};
     if (branches == 0 && n > 0) {
         // autotuner: coordinate descent over branches, pass size and traversal block
         // size, each configuration timed on the first group's scenes at 2 spp
@@ -1455,10 +1457,10 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
         }
         {
             double rate = 0;
-            const Config k{cfg.branches, cfg.pass_log2, 128};
+            const Config k{cfg.branches, cfg.pass_log2, 256};
             if (int r = time_cfg(k, &rate); r != AGX_OK) return r;
             rep.tune_rays_per_s[3] = rate;
-            if (rate > best * 1.02) { best = rate; cfg.trace_block = 128; }
+            if (rate > best * 1.02) { best = rate; cfg.trace_block = 256; }
         }
         rep.tuned = 1;
         HIP_TRY(hipMemset(held.rays, 0, std::max<size_t>(n, 1) * sizeof(unsigned long long)));

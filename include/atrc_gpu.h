@@ -331,7 +331,7 @@ typedef struct agx_batch_report {
     double  wall_ms;           /* whole batch, builds included */
     uint64_t rays;
     double  tune_rays_per_s[4];/* autotuner probe throughput: agx_render_batch with 1, 2, 3, 4 workers;
-                                  agx_render_batch_graph with 2, 4, 8 branches, then 128-thread
+                                  agx_render_batch_graph with 2, 4, 8 branches, then 256-thread
                                   traversal blocks (0 = not tried) */
     /* agx_render_batch_graph only */
     int32_t pass_paths_log2;   /* wavefront pass size of each scene: at most 2^this paths */

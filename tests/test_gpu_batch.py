@@ -126,8 +126,8 @@ def test_batch_graph_autotuned_and_grouped(gpu_ctx):
 
 
 def test_traversal_block_128_matches_256(gpu_ctx, monkeypatch):
-    """128-thread traversal blocks (an autotuner choice; AGX_TRACE_BLOCK=128 at
-    upload selects them) give the image of the default 256-thread blocks bit
+    """256-thread traversal blocks (an autotuner choice; AGX_TRACE_BLOCK=256 at
+    upload selects them) give the image of the default 128-thread blocks bit
     for bit (BVH4 mode), with the same ray counts."""
     s, cam, tr = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=5)
     s.accel = capi.ACCEL_BVH4
@@ -136,7 +136,7 @@ def test_traversal_block_128_matches_256(gpu_ctx, monkeypatch):
     gpu_ctx.upload_scene(d)
     a = gpu_ctx.render(p, aovs=False)["image"]
     sa = gpu_ctx.stats()
-    monkeypatch.setenv("AGX_TRACE_BLOCK", "128")
+    monkeypatch.setenv("AGX_TRACE_BLOCK", "256")
     gpu_ctx.upload_scene(d)
     b = gpu_ctx.render(p, aovs=False)["image"]
     sb = gpu_ctx.stats()
