@@ -16,6 +16,7 @@ constexpr int BINS = 32;
 constexpr float C_TRAV = 1.0f;   // SAH cost of a node visit relative to one primitive test
 constexpr int MEDIAN_DEPTH = 40; // below this binary depth splits are object medians
 constexpr uint32_t PARALLEL_MIN = 1u << 16;
+constexpr uint32_t SPARSE_MAX = 32;  // nodes of at most this many references take the sparse SAH sweep
 
 struct Box {
     float lo[3] = {std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity(),
@@ -48,24 +49,80 @@ struct Subtree {
     std::vector<BNode> nodes;  // nodes[0] = subtree root; child indices are local
 };
 
+// A reference as the builder moves it: bounds, centroid and input index.  The
+// records themselves are partitioned (not an index array into the inputs), so
+// every pass over a node's references reads memory in order.
+struct Item {
+    float lo[3], hi[3], c[3];
+    uint32_t id;
+};
+
 struct Builder {
-    const std::vector<PrimRef> &refs;
-    std::vector<float> cen;     // 3 per reference
-    std::vector<uint32_t> order;
+    std::vector<Item> items;  // the references, permuted in place as nodes split
     int max_leaf;
 
-    Builder(const std::vector<PrimRef> &r, int ml) : refs(r), max_leaf(ml) {
-        cen.resize(3 * r.size());
-        order.resize(r.size());
+    Builder(const std::vector<PrimRef> &r, int ml) : max_leaf(ml) {
+        items.resize(r.size());
         for (size_t i = 0; i < r.size(); ++i) {
-            for (int k = 0; k < 3; ++k) cen[3 * i + k] = 0.5f * (r[i].lo[k] + r[i].hi[k]);
-            order[i] = uint32_t(i);
+            Item &it = items[i];
+            for (int k = 0; k < 3; ++k) {
+                it.lo[k] = r[i].lo[k];
+                it.hi[k] = r[i].hi[k];
+                it.c[k] = 0.5f * (r[i].lo[k] + r[i].hi[k]);
+            }
+            it.id = uint32_t(i);
         }
     }
 
     void make_leaf(Subtree &t, int32_t self, uint32_t b, uint32_t e) {
         t.nodes[size_t(self)].start = b;
         t.nodes[size_t(self)].count = e - b;
+    }
+    // The binned SAH of a small node without the bin arrays: per axis the
+    // references sorted by bin (stably), and only the boundaries between
+    // occupied bins evaluated.  Box unions are exact, so each candidate costs
+    // what the 32-bin sweep computes for the same two sets, and the lowest bin
+    // of equal cost wins as there: the same split, without the fixed cost of
+    // clearing and sweeping 3 x 32 bins at every small node.
+    void sparse_sah(uint32_t b, uint32_t e, const Box &cb, float *best_cost, int *best_axis, int *best_bin) const {
+        const uint32_t n = e - b;
+        int key[SPARSE_MAX];
+        uint32_t id[SPARSE_MAX];  // positions in items
+        Box suf[SPARSE_MAX + 1];  // suf[j] = union of references j .. n - 1 in bin order
+        for (int axis = 0; axis < 3; ++axis) {
+            const float ext = cb.hi[axis] - cb.lo[axis];
+            if (!(ext > 0)) continue;
+            const float scale = float(BINS) / ext;
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t r = b + i;
+                int k = int((items[r].c[axis] - cb.lo[axis]) * scale);
+                k = std::min(BINS - 1, std::max(0, k));
+                uint32_t j = i;  // insertion: ascending bins, equal bins in reference order
+                while (j > 0 && key[j - 1] > k) {
+                    key[j] = key[j - 1];
+                    id[j] = id[j - 1];
+                    --j;
+                }
+                key[j] = k;
+                id[j] = r;
+            }
+            suf[n] = Box();
+            for (uint32_t j = n; j-- > 0;) {
+                suf[j] = suf[j + 1];
+                suf[j].grow(items[id[j]].lo, items[id[j]].hi);
+            }
+            Box acc;
+            for (uint32_t j = 0; j + 1 < n; ++j) {
+                acc.grow(items[id[j]].lo, items[id[j]].hi);
+                if (key[j] == key[j + 1]) continue;
+                const float cost = acc.area() * float(j + 1) + suf[j + 1].area() * float(n - j - 1);
+                if (cost < *best_cost) {
+                    *best_cost = cost;
+                    *best_axis = axis;
+                    *best_bin = key[j];
+                }
+            }
+        }
     }
 
     // Builds [b, e) into t; returns the local index of its root.
@@ -74,9 +131,8 @@ struct Builder {
         t.nodes.emplace_back();
         Box box, cb;
         for (uint32_t i = b; i < e; ++i) {
-            const uint32_t r = order[i];
-            box.grow(refs[r].lo, refs[r].hi);
-            cb.grow(&cen[3 * r], &cen[3 * r]);
+            box.grow(items[i].lo, items[i].hi);
+            cb.grow(items[i].c, items[i].c);
         }
         t.nodes[size_t(self)].box = box;
         const uint32_t n = e - b;
@@ -88,35 +144,46 @@ struct Builder {
         int best_axis = -1, best_bin = -1;
         float best_cost = std::numeric_limits<float>::infinity();
         const float parent_area = box.area();
-        if (depth < MEDIAN_DEPTH) {
+        if (depth < MEDIAN_DEPTH && n <= SPARSE_MAX) {
+            sparse_sah(b, e, cb, &best_cost, &best_axis, &best_bin);
+        } else if (depth < MEDIAN_DEPTH) {
+            // the three axes' bins in one pass over the references
+            Box bins[3][BINS];
+            uint32_t cnt[3][BINS] = {};
+            float scale[3];
+            bool live[3];
             for (int axis = 0; axis < 3; ++axis) {
                 const float ext = cb.hi[axis] - cb.lo[axis];
-                if (!(ext > 0)) continue;
-                const float scale = float(BINS) / ext;
-                Box bins[BINS];
-                uint32_t cnt[BINS] = {};
-                for (uint32_t i = b; i < e; ++i) {
-                    const uint32_t r = order[i];
-                    int k = int((cen[3 * r + axis] - cb.lo[axis]) * scale);
+                live[axis] = ext > 0;
+                scale[axis] = live[axis] ? float(BINS) / ext : 0.0f;
+            }
+            for (uint32_t i = b; i < e; ++i) {
+                const Item &it = items[i];
+                for (int axis = 0; axis < 3; ++axis) {
+                    if (!live[axis]) continue;
+                    int k = int((it.c[axis] - cb.lo[axis]) * scale[axis]);
                     k = std::min(BINS - 1, std::max(0, k));
-                    bins[k].grow(refs[r].lo, refs[r].hi);
-                    ++cnt[k];
+                    bins[axis][k].grow(it.lo, it.hi);
+                    ++cnt[axis][k];
                 }
+            }
+            for (int axis = 0; axis < 3; ++axis) {
+                if (!live[axis]) continue;
                 float right_area[BINS];
                 uint32_t right_cnt[BINS];
                 Box acc;
                 uint32_t c = 0;
                 for (int k = BINS - 1; k > 0; --k) {
-                    acc.grow(bins[k]);
-                    c += cnt[k];
+                    acc.grow(bins[axis][k]);
+                    c += cnt[axis][k];
                     right_area[k] = acc.area();
                     right_cnt[k] = c;
                 }
                 acc = Box();
                 c = 0;
                 for (int k = 0; k < BINS - 1; ++k) {
-                    acc.grow(bins[k]);
-                    c += cnt[k];
+                    acc.grow(bins[axis][k]);
+                    c += cnt[axis][k];
                     if (c == 0 || right_cnt[k + 1] == 0) continue;
                     const float cost = acc.area() * float(c) + right_area[k + 1] * float(right_cnt[k + 1]);
                     if (cost < best_cost) {
@@ -142,20 +209,20 @@ struct Builder {
             const float ext = cb.hi[best_axis] - cb.lo[best_axis];
             const float scale = float(BINS) / ext;
             const float lo = cb.lo[best_axis];
-            auto it = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t r) {
-                int k = int((cen[3 * r + best_axis] - lo) * scale);
+            auto it = std::partition(items.begin() + b, items.begin() + e, [&](const Item &r) {
+                int k = int((r.c[best_axis] - lo) * scale);
                 k = std::min(BINS - 1, std::max(0, k));
                 return k <= best_bin;
             });
-            mid = uint32_t(it - order.begin());
+            mid = uint32_t(it - items.begin());
         } else {
             // object median on the largest centroid extent (deep trees, coincident centroids)
             int axis = 0;
             for (int k = 1; k < 3; ++k)
                 if (cb.hi[k] - cb.lo[k] > cb.hi[axis] - cb.lo[axis]) axis = k;
             mid = b + n / 2;
-            std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
-                             [&](uint32_t x, uint32_t y) { return cen[3 * x + axis] < cen[3 * y + axis]; });
+            std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
+                             [&](const Item &x, const Item &y) { return x.c[axis] < y.c[axis]; });
         }
         if (mid == b || mid == e) mid = b + n / 2;
         if (n >= PARALLEL_MIN) {
@@ -300,33 +367,37 @@ struct Collapser {
     // leaf root); returns its index.
     uint32_t emit(int32_t bi, int depth) {
         max_depth = std::max(max_depth, depth);
-        std::vector<int32_t> kids;
+        int32_t kids[4];
+        int nk = 0;
         const BNode &self = bn[size_t(bi)];
         if (self.child[0] < 0) {
-            kids.push_back(bi);
+            kids[nk++] = bi;
         } else {
-            kids.push_back(self.child[0]);
-            kids.push_back(self.child[1]);
-            while (kids.size() < 4) {
+            kids[nk++] = self.child[0];
+            kids[nk++] = self.child[1];
+            while (nk < 4) {
+                // open the interior child of largest area (the first of equal areas)
                 int pick = -1;
                 float best = -1;
-                for (size_t k = 0; k < kids.size(); ++k) {
+                for (int k = 0; k < nk; ++k) {
                     const BNode &c = bn[size_t(kids[k])];
                     if (c.child[0] < 0) continue;
                     const float a = c.box.area();
-                    if (a > best) { best = a; pick = int(k); }
+                    if (a > best) { best = a; pick = k; }
                 }
                 if (pick < 0) break;
-                const BNode &c = bn[size_t(kids[size_t(pick)])];
-                kids[size_t(pick)] = c.child[0];
-                kids.insert(kids.begin() + pick + 1, c.child[1]);
+                const BNode &c = bn[size_t(kids[pick])];
+                for (int k = nk; k > pick + 1; --k) kids[k] = kids[k - 1];
+                kids[pick] = c.child[0];
+                kids[pick + 1] = c.child[1];
+                ++nk;
             }
         }
         const uint32_t idx = uint32_t(out->nodes.size() / BVH4_NODE_VEC4);
         out->nodes.resize(out->nodes.size() + BVH4_NODE_VEC4, vec4{0, 0, 0, 0});
         uint32_t codes[4] = {BVH4_EMPTY, BVH4_EMPTY, BVH4_EMPTY, BVH4_EMPTY};
         Box boxes[4];
-        for (size_t k = 0; k < kids.size(); ++k) {
+        for (int k = 0; k < nk; ++k) {
             const BNode &c = bn[size_t(kids[k])];
             boxes[k] = c.box;
             pad(&boxes[k]);
@@ -357,7 +428,8 @@ void build_bvh4(const std::vector<PrimRef> &refs, int max_leaf, Bvh4 *out) {
     Collapser c{t.nodes, out};
     c.emit(0, 1);
     out->depth = c.max_depth;
-    out->order = std::move(b.order);
+    out->order.resize(b.items.size());
+    for (size_t i = 0; i < b.items.size(); ++i) out->order[i] = b.items[i].id;
     // SAH cost of the 4-wide tree relative to the root area
     const float root_area = std::max(t.nodes[0].box.area(), 1e-30f);
     double cost = 0;
