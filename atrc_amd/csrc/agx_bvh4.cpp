@@ -43,12 +43,6 @@ struct BNode {
     uint32_t start = 0, count = 0;  // leaf range in `order` when child[0] < 0
 };
 
-// Each subtree is built into its own node vector (so that large subtrees can be
-// built on worker threads) and spliced into the parent's afterwards.
-struct Subtree {
-    std::vector<BNode> nodes;  // nodes[0] = subtree root; child indices are local
-};
-
 // A reference as the builder moves it: bounds, centroid and input index.  The
 // records themselves are partitioned (not an index array into the inputs), so
 // every pass over a node's references reads memory in order.
@@ -57,8 +51,15 @@ struct Item {
     uint32_t id;
 };
 
+// The binary tree lives in one array with a fixed slot range per subtree: the
+// subtree over references [b, e) rooted at slot s uses slots s .. s + 2 (e - b)
+// - 2 (a binary tree over n references has at most 2 n - 1 nodes), its left
+// child (references [b, m)) is slot s + 1 and its right child slot
+// s + 2 (m - b).  Subtrees built on worker threads therefore write disjoint
+// slots of the same array, and nothing is copied afterwards.
 struct Builder {
     std::vector<Item> items;  // the references, permuted in place as nodes split
+    std::vector<BNode> nodes;
     int max_leaf;
 
     Builder(const std::vector<PrimRef> &r, int ml) : max_leaf(ml) {
@@ -72,11 +73,12 @@ struct Builder {
             }
             it.id = uint32_t(i);
         }
+        nodes.resize(2 * r.size() - 1);
     }
 
-    void make_leaf(Subtree &t, int32_t self, uint32_t b, uint32_t e) {
-        t.nodes[size_t(self)].start = b;
-        t.nodes[size_t(self)].count = e - b;
+    void make_leaf(int32_t self, uint32_t b, uint32_t e) {
+        nodes[size_t(self)].start = b;
+        nodes[size_t(self)].count = e - b;
     }
     // The binned SAH of a small node without the bin arrays: per axis the
     // references sorted by bin (stably), and only the boundaries between
@@ -125,20 +127,18 @@ struct Builder {
         }
     }
 
-    // Builds [b, e) into t; returns the local index of its root.
-    int32_t build(Subtree &t, uint32_t b, uint32_t e, int depth) {
-        const int32_t self = int32_t(t.nodes.size());
-        t.nodes.emplace_back();
+    // Builds the subtree over [b, e) rooted at slot self.
+    void build(int32_t self, uint32_t b, uint32_t e, int depth) {
         Box box, cb;
         for (uint32_t i = b; i < e; ++i) {
             box.grow(items[i].lo, items[i].hi);
             cb.grow(items[i].c, items[i].c);
         }
-        t.nodes[size_t(self)].box = box;
+        nodes[size_t(self)].box = box;
         const uint32_t n = e - b;
         if (n <= 1 || (depth >= MEDIAN_DEPTH && n <= uint32_t(max_leaf))) {
-            make_leaf(t, self, b, e);
-            return self;
+            make_leaf(self, b, e);
+            return;
         }
         // best SAH split over the three axes
         int best_axis = -1, best_bin = -1;
@@ -197,12 +197,12 @@ struct Builder {
         if (best_axis >= 0 && parent_area > 0) {
             const float split_cost = C_TRAV + best_cost / parent_area;
             if (n <= uint32_t(max_leaf) && float(n) <= split_cost) {
-                make_leaf(t, self, b, e);
-                return self;
+                make_leaf(self, b, e);
+                return;
             }
         } else if (n <= uint32_t(max_leaf)) {
-            make_leaf(t, self, b, e);  // coincident centroids, few references
-            return self;
+            make_leaf(self, b, e);  // coincident centroids, few references
+            return;
         }
         uint32_t mid;
         if (best_axis >= 0) {
@@ -225,41 +225,17 @@ struct Builder {
                              [&](const Item &x, const Item &y) { return x.c[axis] < y.c[axis]; });
         }
         if (mid == b || mid == e) mid = b + n / 2;
+        const int32_t left = self + 1, right = self + 2 * int32_t(mid - b);
+        nodes[size_t(self)].child[0] = left;
+        nodes[size_t(self)].child[1] = right;
         if (n >= PARALLEL_MIN) {
-            // left half on a worker thread, right half here; splice both
-            auto fut = std::async(std::launch::async, [this, b, mid, depth]() {
-                Subtree l;
-                l.nodes.reserve(2 * size_t(mid - b) / size_t(std::max(1, max_leaf)) + 1);
-                build(l, b, mid, depth + 1);
-                return l;
-            });
-            Subtree r;
-            build(r, mid, e, depth + 1);
-            Subtree l = fut.get();
-            const int32_t lbase = int32_t(t.nodes.size());
-            splice(t, l);
-            const int32_t rbase = int32_t(t.nodes.size());
-            splice(t, r);
-            t.nodes[size_t(self)].child[0] = lbase;
-            t.nodes[size_t(self)].child[1] = rbase;
+            // left half on a worker thread, right half here
+            auto fut = std::async(std::launch::async, [this, left, b, mid, depth]() { build(left, b, mid, depth + 1); });
+            build(right, mid, e, depth + 1);
+            fut.get();
         } else {
-            const int32_t l = build(t, b, mid, depth + 1);
-            const int32_t r = build(t, mid, e, depth + 1);
-            t.nodes[size_t(self)].child[0] = l;
-            t.nodes[size_t(self)].child[1] = r;
-        }
-        return self;
-    }
-
-    static void splice(Subtree &dst, const Subtree &src) {
-        const int32_t base = int32_t(dst.nodes.size());
-        for (const BNode &n : src.nodes) {
-            BNode m = n;
-            if (m.child[0] >= 0) {
-                m.child[0] += base;
-                m.child[1] += base;
-            }
-            dst.nodes.push_back(m);
+            build(left, b, mid, depth + 1);
+            build(right, mid, e, depth + 1);
         }
     }
 };
@@ -420,18 +396,16 @@ void build_bvh4(const std::vector<PrimRef> &refs, int max_leaf, Bvh4 *out) {
     if (refs.size() > size_t(BVH4_FIRST_MASK)) throw std::invalid_argument("build_bvh4: too many primitives");
     max_leaf = std::min(BVH4_MAX_LEAF, std::max(1, max_leaf));
     Builder b(refs, max_leaf);
-    Subtree t;
-    t.nodes.reserve(2 * refs.size() / size_t(max_leaf) + 16);
-    b.build(t, 0, uint32_t(refs.size()), 0);
+    b.build(0, 0, uint32_t(refs.size()), 0);
     *out = Bvh4{};
-    out->nodes.reserve(t.nodes.size() / 2 * BVH4_NODE_VEC4 + BVH4_NODE_VEC4);
-    Collapser c{t.nodes, out};
+    out->nodes.reserve(b.nodes.size() / 4 * BVH4_NODE_VEC4 + BVH4_NODE_VEC4);
+    Collapser c{b.nodes, out};
     c.emit(0, 1);
     out->depth = c.max_depth;
     out->order.resize(b.items.size());
     for (size_t i = 0; i < b.items.size(); ++i) out->order[i] = b.items[i].id;
     // SAH cost of the 4-wide tree relative to the root area
-    const float root_area = std::max(t.nodes[0].box.area(), 1e-30f);
+    const float root_area = std::max(b.nodes[0].box.area(), 1e-30f);
     double cost = 0;
     const size_t nn = out->nodes.size() / BVH4_NODE_VEC4;
     for (size_t i = 0; i < nn; ++i) {
