@@ -1196,11 +1196,19 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
         for (uint32_t e = 0; e < items[i].scene->entity_count; ++e) tris += items[i].scene->entities[e].tri_count;
         return double(p->width) * p->height * p->spp + 4.0 * double(items[i].scene->entity_count) + double(tris);
     };
-    // cheapest first: the first group builds quickly and the GPU starts while the
-    // costlier scenes are still being built
+    // smallest builds first: the first group builds quickly and the GPU starts
+    // while the larger scenes are still being built.  The order ignores the
+    // render cost, so the group left to render after the last build costs
+    // what an average group does, not the most.
+    auto build_cost = [&](uint32_t i) {
+        uint64_t tris = 0;
+        for (uint32_t e = 0; e < items[i].scene->entity_count; ++e) tris += items[i].scene->entities[e].tri_count;
+        return double(tris) + 4.0 * double(items[i].scene->entity_count);
+    };
     std::vector<uint32_t> order(n);
     for (uint32_t i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost(a) < cost(b); });
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return build_cost(a) < build_cost(b); });
     const uint32_t group_size = 8;
     const uint32_t n_groups = (n + group_size - 1) / group_size;
 

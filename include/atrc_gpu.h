@@ -406,7 +406,7 @@ int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t w
                       agx_batch_report *report);
 
 /* Batched rendering in hipGraph launches (no counterpart in Atrc).  Items are
- * taken cheapest first (pixels x spp + triangles) in groups of 8.  Host
+ * taken smallest build first (triangles) in groups of 8.  Host
  * threads (one per core, at most 16) build the scenes in that order
  * and upload each into one of three sets of 8 pooled contexts, once the group
  * that last used the set has finished; as soon as a group is built, its
