@@ -127,13 +127,20 @@ struct Builder {
         }
     }
 
-    // Builds the subtree over [b, e) rooted at slot self.
-    void build(int32_t self, uint32_t b, uint32_t e, int depth) {
-        Box box, cb;
+    // bounds and centroid bounds of references [b, e)
+    void bounds(uint32_t b, uint32_t e, Box *box, Box *cb) const {
+        *box = Box();
+        *cb = Box();
         for (uint32_t i = b; i < e; ++i) {
-            box.grow(items[i].lo, items[i].hi);
-            cb.grow(items[i].c, items[i].c);
+            box->grow(items[i].lo, items[i].hi);
+            cb->grow(items[i].c, items[i].c);
         }
+    }
+
+    // Builds the subtree over [b, e) rooted at slot self; box and cb are the
+    // references' bounds and centroid bounds (the parent's partition pass
+    // gathers them).
+    void build(int32_t self, uint32_t b, uint32_t e, int depth, const Box &box, const Box &cb) {
         nodes[size_t(self)].box = box;
         const uint32_t n = e - b;
         if (n <= 1 || (depth >= MEDIAN_DEPTH && n <= uint32_t(max_leaf))) {
@@ -205,16 +212,37 @@ struct Builder {
             return;
         }
         uint32_t mid;
+        Box lbox, lcb, rbox, rcb;  // the children's bounds and centroid bounds
         if (best_axis >= 0) {
             const float ext = cb.hi[best_axis] - cb.lo[best_axis];
             const float scale = float(BINS) / ext;
             const float lo = cb.lo[best_axis];
-            auto it = std::partition(items.begin() + b, items.begin() + e, [&](const Item &r) {
+            auto left_side = [&](const Item &r) {
                 int k = int((r.c[best_axis] - lo) * scale);
                 k = std::min(BINS - 1, std::max(0, k));
                 return k <= best_bin;
-            });
-            mid = uint32_t(it - items.begin());
+            };
+            auto to_left = [&](const Item &r) { lbox.grow(r.lo, r.hi); lcb.grow(r.c, r.c); };
+            auto to_right = [&](const Item &r) { rbox.grow(r.lo, r.hi); rcb.grow(r.c, r.c); };
+            // two-sided (Hoare) partition: scan from the front past left-side
+            // references and from the back past right-side ones, swap the pair
+            // that stops both scans; each reference's side is settled once, so
+            // the children's bounds are gathered on the way
+            uint32_t f = b, l = e;
+            for (;;) {
+                while (f != l && left_side(items[f])) to_left(items[f++]);
+                if (f == l) break;
+                --l;
+                while (f != l && !left_side(items[l])) to_right(items[l--]);
+                if (f == l) {
+                    to_right(items[f]);  // the right-side reference that stopped the front scan
+                    break;
+                }
+                std::swap(items[f], items[l]);
+                to_left(items[f++]);
+                to_right(items[l]);
+            }
+            mid = f;
         } else {
             // object median on the largest centroid extent (deep trees, coincident centroids)
             int axis = 0;
@@ -223,19 +251,26 @@ struct Builder {
             mid = b + n / 2;
             std::nth_element(items.begin() + b, items.begin() + mid, items.begin() + e,
                              [&](const Item &x, const Item &y) { return x.c[axis] < y.c[axis]; });
+            bounds(b, mid, &lbox, &lcb);
+            bounds(mid, e, &rbox, &rcb);
         }
-        if (mid == b || mid == e) mid = b + n / 2;
+        if (mid == b || mid == e) {
+            mid = b + n / 2;
+            bounds(b, mid, &lbox, &lcb);
+            bounds(mid, e, &rbox, &rcb);
+        }
         const int32_t left = self + 1, right = self + 2 * int32_t(mid - b);
         nodes[size_t(self)].child[0] = left;
         nodes[size_t(self)].child[1] = right;
         if (n >= PARALLEL_MIN) {
             // left half on a worker thread, right half here
-            auto fut = std::async(std::launch::async, [this, left, b, mid, depth]() { build(left, b, mid, depth + 1); });
-            build(right, mid, e, depth + 1);
+            auto fut = std::async(std::launch::async,
+                                  [this, left, b, mid, depth, &lbox, &lcb]() { build(left, b, mid, depth + 1, lbox, lcb); });
+            build(right, mid, e, depth + 1, rbox, rcb);
             fut.get();
         } else {
-            build(left, b, mid, depth + 1);
-            build(right, mid, e, depth + 1);
+            build(left, b, mid, depth + 1, lbox, lcb);
+            build(right, mid, e, depth + 1, rbox, rcb);
         }
     }
 };
@@ -396,7 +431,9 @@ void build_bvh4(const std::vector<PrimRef> &refs, int max_leaf, Bvh4 *out) {
     if (refs.size() > size_t(BVH4_FIRST_MASK)) throw std::invalid_argument("build_bvh4: too many primitives");
     max_leaf = std::min(BVH4_MAX_LEAF, std::max(1, max_leaf));
     Builder b(refs, max_leaf);
-    b.build(0, 0, uint32_t(refs.size()), 0);
+    Box box, cb;
+    b.bounds(0, uint32_t(refs.size()), &box, &cb);
+    b.build(0, 0, uint32_t(refs.size()), 0, box, cb);
     *out = Bvh4{};
     out->nodes.reserve(b.nodes.size() / 4 * BVH4_NODE_VEC4 + BVH4_NODE_VEC4);
     Collapser c{b.nodes, out};
