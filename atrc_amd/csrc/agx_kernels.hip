@@ -712,13 +712,14 @@ struct B4Closest {
         } else {
             const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
             for (uint32_t i = first; i < first + cnt; ++i) {
-                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
+                // the record's three rows in flight together (the third does not
+                // wait for the slot test: every kind of record uses it)
+                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1], p2 = P[3 * size_t(i) + 2];
                 const int32_t ent = __float_as_int(p0.w);
                 const uint32_t slot = __float_as_uint(p1.w);
                 if (slot < BVH4_QUAD_ACD || (WT && slot != BVH4_ANALYTIC)) {
                     // a mesh triangle, or (watertight mode) one triangle of a quad
                     if (STATS) c.tests++;
-                    const float4 p2 = P[3 * size_t(i) + 2];
                     float tt, uu, vv;
                     const bool hit_tri = WT ? wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)
                                             : mt_closest(o, d, tmin, tmax, mk(p0), mk(p1), mk(p2), &tt, &uu, &vv);
@@ -734,7 +735,7 @@ struct B4Closest {
                 } else {
                     // a lower-ranked entity must hit strictly closer: its test runs
                     // against the next float below the current hit distance
-                    const int32_t rank = __float_as_int(P[3 * size_t(i) + 2].w);
+                    const int32_t rank = __float_as_int(p2.w);
                     float tb = rank < best_rank ? __uint_as_float(__float_as_uint(tmax) - 1u) : tmax;
                     if (analytic_closest<STATS>(s.ents[ent], ent, o, d, tmin, &tb, &rec, c)) {
                         tmax = tb;
@@ -804,11 +805,12 @@ struct B4Any {
         } else {
             const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
             for (uint32_t i = first; i < first + cnt; ++i) {
-                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1];
+                // the third row is loaded with the others (analytic records
+                // leave it unused; triangles need it)
+                const float4 p0 = P[3 * size_t(i)], p1 = P[3 * size_t(i) + 1], p2 = P[3 * size_t(i) + 2];
                 const uint32_t slot = __float_as_uint(p1.w);
                 if (slot < BVH4_QUAD_ACD || (WT && slot != BVH4_ANALYTIC)) {
                     if (STATS) c.tests++;
-                    const float4 p2 = P[3 * size_t(i) + 2];
                     if (WT) {
                         float tt, uu, vv;
                         if (wt_test(wr, mk(p0), mk(p1), mk(p2), tmin, tmax, &tt, &uu, &vv)) { occluded = true; return true; }
