@@ -2337,9 +2337,20 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
     // 1. direct illumination of the previous vertex (path_tracing.cpp:85-97, 134-146)
     if (flags & PF_PENDING) {
         V direct = mk(0, 0, 0);
-        for (int32_t l = 0; l < rc.L; ++l) {
-            const float4 c = pb.nee[size_t(p) * rc.L + l];
-            if (c.w != 0.0f && !pb.occluded[size_t(p) * rc.L + l]) direct = direct + mk(c);
+        // two lights' records and occlusion bits in flight at a time (the
+        // light count is a run-time value, so the compiler would issue one
+        // light's loads per iteration); the sum keeps light order
+        const size_t nb = size_t(p) * rc.L;
+        int32_t l = 0;
+        for (; l + 1 < rc.L; l += 2) {
+            const float4 c0 = pb.nee[nb + l], c1 = pb.nee[nb + l + 1];
+            const uint8_t o0 = pb.occluded[nb + l], o1 = pb.occluded[nb + l + 1];
+            if (c0.w != 0.0f && !o0) direct = direct + mk(c0);
+            if (c1.w != 0.0f && !o1) direct = direct + mk(c1);
+        }
+        if (l < rc.L) {
+            const float4 c = pb.nee[nb + l];
+            if (c.w != 0.0f && !pb.occluded[nb + l]) direct = direct + mk(c);
         }
         if (flags & PF_MIS) direct = direct + resolve_mis<F>(s, pb, p, flags, eps, rtr);
         L = L + direct;
@@ -2379,6 +2390,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         dep = pb.depth[p];
     }
     const float4 h4 = pb.hit[2 * p];
+    const int32_t hit_prim = pb.hit_prim[2 * p];  // loaded with the hit (a miss leaves it unused)
     const int32_t ent = __float_as_int(h4.w);
     const V ro = mk(pb.ray_o[2 * p]), rd = mk(pb.ray_d[2 * p]);
     if (ent < 0) {  // path_tracing.cpp:41-49 (trace_nomis, :250-256: at every depth)
@@ -2393,7 +2405,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene s, RenderConst rc, PathB
         return;
     }
     HitRec h;
-    h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = pb.hit_prim[2 * p];
+    h.t = h4.x; h.u = h4.y; h.v = h4.z; h.ent = ent; h.prim = hit_prim;
     const Surf sp = surface_at(s, h, ro, rd, eps);
     const DevEntity &e = s.ents[ent];
     const Bsdf b = make_bsdf(s.mats[e.material], sp, eps);
