@@ -3115,7 +3115,7 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
         if (stats) hipLaunchKernelGGL((K<true, TM, TB>), grid, dim3(TB), 0, st, __VA_ARGS__);     \
         else hipLaunchKernelGGL((K<false, TM, TB>), grid, dim3(TB), 0, st, __VA_ARGS__);          \
     } while (0)
-// tb = threads per block: 128 or 256 in the BVH4 (Moller-Trumbore) mode, 256 otherwise
+// tb = threads per block: 128 or 256 in the BVH4 modes, 256 in the Atrc modes
 #define AGX_LAUNCH_STACKED(K, tb, grid, st, ...)                                                  \
     do {                                                                                          \
         if (s.accel == AGX_ACCEL_BVH4) {                                                          \
@@ -3127,8 +3127,13 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
                 else AGX_LAUNCH_TM(K, TM_B4_SMALL, 256, grid, st, __VA_ARGS__);                   \
             }                                                                                     \
         } else if (s.accel == AGX_ACCEL_BVH4_WATERTIGHT) {                                        \
-            if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, 256, grid, st, __VA_ARGS__);   \
-            else AGX_LAUNCH_TM(K, TM_B4W_SMALL, 256, grid, st, __VA_ARGS__);                      \
+            if ((tb) == 128) {                                                                    \
+                if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, 128, grid, st, __VA_ARGS__);\
+                else AGX_LAUNCH_TM(K, TM_B4W_SMALL, 128, grid, st, __VA_ARGS__);                  \
+            } else {                                                                              \
+                if (s.bvh4_large_stack) AGX_LAUNCH_TM(K, TM_B4W_LARGE, 256, grid, st, __VA_ARGS__);\
+                else AGX_LAUNCH_TM(K, TM_B4W_SMALL, 256, grid, st, __VA_ARGS__);                  \
+            }                                                                                     \
         } else if (s.lds_stack) {                                                                 \
             AGX_LAUNCH_TM(K, TM_ATRC_LDS, 256, grid, st, __VA_ARGS__);                            \
         } else {                                                                                  \
@@ -3138,7 +3143,8 @@ void launch_raygen(const DevScene &s, const RenderConst &rc, const PathBuffers &
 
 // threads per block of the wavefront's traversal launches for this scene
 inline unsigned trace_block_of(const DevScene &s) {
-    return (s.accel == AGX_ACCEL_BVH4 && s.trace_block == 128) ? 128u : TRACE_BLOCK;
+    const bool b4 = s.accel == AGX_ACCEL_BVH4 || s.accel == AGX_ACCEL_BVH4_WATERTIGHT;
+    return (b4 && s.trace_block == 128) ? 128u : TRACE_BLOCK;
 }
 
 void launch_trace_closest(const DevScene &s, const PathBuffers &pb, uint32_t max_n, bool stats, DevCounters *cnt,

@@ -125,12 +125,13 @@ def test_batch_graph_autotuned_and_grouped(gpu_ctx):
     assert rep["rays"] == sum(r["rays"] for r in recs)
 
 
-def test_traversal_block_128_matches_256(gpu_ctx, monkeypatch):
+@pytest.mark.parametrize("accel", [capi.ACCEL_BVH4, capi.ACCEL_BVH4_WATERTIGHT])
+def test_traversal_block_128_matches_256(gpu_ctx, monkeypatch, accel):
     """256-thread traversal blocks (an autotuner choice; AGX_TRACE_BLOCK=256 at
     upload selects them) give the image of the default 128-thread blocks bit
-    for bit (BVH4 mode), with the same ray counts."""
+    for bit (both BVH4 modes), with the same ray counts."""
     s, cam, tr = scenes.procedural_spheres(n_spheres=24, subdiv=3, seed=5)
-    s.accel = capi.ACCEL_BVH4
+    s.accel = accel
     d, keep = s.to_desc()
     p = scenes.render_params(64, 48, 4, cam, seed=9, **tr)
     gpu_ctx.upload_scene(d)
