@@ -608,10 +608,15 @@ __device__ __forceinline__ float b4_q(float org, uint32_t w, int j, float step) 
     return org + m;
 }
 
+struct B4Node {
+    float4 h, cc, qa, qb;  // origin + step exponents, child codes, quantized x / y and z bounds
+};
+__device__ __forceinline__ B4Node b4_load(const float4 *n) { return B4Node{n[0], n[1], n[2], n[3]}; }
+
 // 64-byte node: decode the four children's (conservative) boxes and slab-test them
-__device__ __forceinline__ void b4_children(const float4 *n, V o, V inv, float tmin, float tmax, float t[4],
+__device__ __forceinline__ void b4_children(const B4Node &nd, V o, V inv, float tmin, float tmax, float t[4],
                                             uint32_t k[4]) {
-    const float4 h = n[0], cc = n[1], qa = n[2], qb = n[3];
+    const float4 h = nd.h, cc = nd.cc, qa = nd.qa, qb = nd.qb;
     const uint32_t eb = __float_as_uint(h.w);
     const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
                 sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
@@ -625,6 +630,12 @@ __device__ __forceinline__ void b4_children(const float4 *n, V o, V inv, float t
                    ? b4_slab(b4_q(h.x, wlx, j, sx), b4_q(h.x, whx, j, sx), b4_q(h.y, wly, j, sy),
                              b4_q(h.y, why, j, sy), b4_q(h.z, wlz, j, sz), b4_q(h.z, whz, j, sz), o, inv, tmin, tmax)
                    : INF_F;
+}
+
+// the same for the node at n
+__device__ __forceinline__ void b4_children(const float4 *n, V o, V inv, float tmin, float tmax, float t[4],
+                                            uint32_t k[4]) {
+    b4_children(b4_load(n), o, inv, tmin, tmax, t, k);
 }
 
 __device__ __forceinline__ void cswap(float t[4], uint32_t k[4], int a, int b) {
@@ -787,21 +798,36 @@ struct B4Any {
         code = 0; sp = 0; occluded = false;
     }
 
+    // Visit of an interior node whose rows are loaded: the first child the ray
+    // enters becomes the current node and the others are pushed (true), or
+    // none is entered (false: the caller pops).
+    __device__ __forceinline__ bool visit(const B4Node &nd, Counts &c, uint32_t *ring, uint32_t (&spill)[SPILL]) {
+        if (STATS) c.visits++;
+        float t[4];
+        uint32_t k[4];
+        b4_children(nd, o, inv, tmin, tmax, t, k);
+        uint32_t next = BVH4_EMPTY;
+        for (int j = 0; j < 4; ++j) {
+            if (t[j] == INF_F) continue;
+            if (next == BVH4_EMPTY) next = k[j];
+            else if (sp < CAP) ring_push<BVH4_RING_ANY, TB>(ring, spill, sp, k[j]);
+        }
+        if (next == BVH4_EMPTY) return false;
+        code = next;
+        return true;
+    }
+    // the next pending node; true once none is left
+    __device__ __forceinline__ bool pop_next(uint32_t *ring, const uint32_t (&spill)[SPILL]) {
+        if (sp == 0) return true;
+        code = ring_pop<BVH4_RING_ANY, TB>(ring, spill, sp);
+        return false;
+    }
+
     __device__ __forceinline__ bool step(const DevScene &s, Counts &c, uint32_t *ring, uint32_t (&spill)[SPILL]) {
         const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
         const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
         if (!(code & BVH4_LEAF)) {
-            if (STATS) c.visits++;
-            float t[4];
-            uint32_t k[4];
-            b4_children(N + size_t(code) * BVH4_NODE_VEC4, o, inv, tmin, tmax, t, k);
-            uint32_t next = BVH4_EMPTY;
-            for (int j = 0; j < 4; ++j) {
-                if (t[j] == INF_F) continue;
-                if (next == BVH4_EMPTY) next = k[j];
-                else if (sp < CAP) ring_push<BVH4_RING_ANY, TB>(ring, spill, sp, k[j]);
-            }
-            if (next != BVH4_EMPTY) { code = next; return false; }
+            if (visit(b4_load(N + size_t(code) * BVH4_NODE_VEC4), c, ring, spill)) return false;
         } else {
             const uint32_t first = code & BVH4_FIRST_MASK, cnt = ((code >> 27) & 15u) + 1;
             for (uint32_t i = first; i < first + cnt; ++i) {
@@ -824,9 +850,7 @@ struct B4Any {
                 }
             }
         }
-        if (sp == 0) return true;
-        code = ring_pop<BVH4_RING_ANY, TB>(ring, spill, sp);
-        return false;
+        return pop_next(ring, spill);
     }
 };
 
@@ -1751,7 +1775,26 @@ __global__ void __launch_bounds__(TB) k_trace_shadow(DevScene s, PathBuffers pb,
     const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool occ = false;
-    if (tid < n) {
+    if constexpr (TM >= TM_B4_SMALL) {
+        if (tid < n) {
+            // the root node's rows do not depend on the ray: they are loaded
+            // first, while the queue entry and then the ray arrive, and the
+            // root is visited from registers
+            const B4Node root = b4_load(reinterpret_cast<const float4 *>(s.b4_nodes));
+            const uint32_t ri = pb.q_shadow[tid];
+            const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
+            constexpr int CAP = (TM == TM_B4_LARGE || TM == TM_B4W_LARGE) ? BVH4_STACK_LARGE : BVH4_STACK_SMALL;
+            using Q = B4Any<STATS, CAP, TM == TM_B4W_SMALL || TM == TM_B4W_LARGE, TB>;
+            uint32_t *ring = lds_col + threadIdx.x;
+            Q q;
+            uint32_t spill[Q::SPILL];
+            q.start(mk(o4), mk(d4), d4.w, o4.w);
+            bool done = q.visit(root, c, ring, spill) ? false : q.pop_next(ring, spill);
+            while (!done) done = q.step(s, c, ring, spill);
+            occ = q.occluded;
+            pb.occluded[ri] = occ ? 1 : 0;
+        }
+    } else if (tid < n) {
         const uint32_t ri = pb.q_shadow[tid];
         const float4 o4 = pb.sh_o[ri], d4 = pb.sh_d[ri];
         occ = query_any<STATS, TM, TB>(s, mk(o4), mk(d4), d4.w, o4.w, c, lds_col);
