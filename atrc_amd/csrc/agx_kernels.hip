@@ -701,6 +701,27 @@ struct B4Closest {
     }
     __device__ __forceinline__ bool hit() const { return best_rank >= 0; }
 
+    // The first step, with the root node's rows already loaded (the root is
+    // always an interior node): its entered children are pushed farthest
+    // first and the nearest becomes the current node.  True if the ray enters
+    // none (the traversal is over).
+    __device__ __forceinline__ bool visit_root(const B4Node &root, Counts &c, uint2 *ring, uint2 (&spill)[SPILL]) {
+        if (STATS) c.visits++;
+        float t[4];
+        uint32_t k[4];
+        b4_children(root, o, inv, tmin, tmax, t, k);
+        cswap(t, k, 0, 1); cswap(t, k, 2, 3); cswap(t, k, 0, 2); cswap(t, k, 1, 3); cswap(t, k, 1, 2);
+        if (t[0] == INF_F) return true;
+        if (t[3] != INF_F)
+            ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[3], __float_as_uint(t[3])));
+        if (t[2] != INF_F)
+            ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[2], __float_as_uint(t[2])));
+        if (t[1] != INF_F)
+            ring_push<BVH4_RING_CLOSEST, TB>(ring, spill, sp, make_uint2(k[1], __float_as_uint(t[1])));
+        code = k[0];
+        return false;
+    }
+
     __device__ __forceinline__ bool step(const DevScene &s, Counts &c, uint2 *ring, uint2 (&spill)[SPILL]) {
         const float4 *N = reinterpret_cast<const float4 *>(s.b4_nodes);
         const float4 *P = reinterpret_cast<const float4 *>(s.b4_prims);
@@ -1753,7 +1774,27 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(8, 8)))
     const uint32_t tid = uint32_t(lb) * blockDim.x + threadIdx.x;
     Counts c;
     bool found = false;
-    if (tid < n) {
+    if constexpr (TM >= TM_B4_SMALL) {
+        if (tid < n) {
+            // the root node's rows do not depend on the ray: they are loaded
+            // first, while the queue entry and then the ray arrive
+            const B4Node root = b4_load(reinterpret_cast<const float4 *>(s.b4_nodes));
+            const uint32_t ri = pb.q_closest[tid];
+            const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
+            constexpr int CAP = (TM == TM_B4_LARGE || TM == TM_B4W_LARGE) ? BVH4_STACK_LARGE : BVH4_STACK_SMALL;
+            using Q = B4Closest<STATS, CAP, TM == TM_B4W_SMALL || TM == TM_B4W_LARGE, TB>;
+            uint2 *ring = reinterpret_cast<uint2 *>(lds_col) + threadIdx.x;
+            Q q;
+            uint2 spill[Q::SPILL];
+            q.start(mk(o4), mk(d4), o4.w, d4.w);
+            if (!q.visit_root(root, c, ring, spill))
+                while (!q.step(s, c, ring, spill)) {
+                }
+            found = q.hit();
+            pb.hit[ri] = make_float4(q.rec.t, q.rec.u, q.rec.v, __int_as_float(found ? q.rec.ent : -1));
+            pb.hit_prim[ri] = q.rec.prim;
+        }
+    } else if (tid < n) {
         const uint32_t ri = pb.q_closest[tid];
         const float4 o4 = pb.ray_o[ri], d4 = pb.ray_d[ri];
         HitRec h;
