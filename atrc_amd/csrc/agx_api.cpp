@@ -84,6 +84,9 @@ struct agx_ctx {
     static constexpr int RING = 64;
     // worker contexts of agx_render_batch (created on first use, same device)
     std::vector<agx_ctx *> batch_workers;
+    // pooled contexts of agx_render_batch_graph, kept after a successful call
+    // (with their path buffers and last scenes) for the next one
+    std::vector<agx_ctx *> graph_pool;
     // ray capture of agx_debug_replay_rays (null outside it): [0] closest-hit, [1] any-hit rays
     float4 *capture_o[2] = {nullptr, nullptr}, *capture_d[2] = {nullptr, nullptr};
     uint32_t *capture_count = nullptr;  // device: 2 counters
@@ -434,6 +437,8 @@ void agx_destroy(agx_ctx *c) {
     if (!c) return;
     for (agx_ctx *w : c->batch_workers) agx_destroy(w);
     c->batch_workers.clear();
+    for (agx_ctx *w : c->graph_pool) agx_destroy(w);
+    c->graph_pool.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->free_scene();
@@ -1223,6 +1228,8 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
         std::vector<hipGraphExec_t> execs;
         std::vector<std::thread> builders;
         std::atomic<bool> stop{false};
+        agx_ctx *owner = nullptr;  // the calling context
+        bool keep = false;         // the call succeeded: ctxs go back to owner->graph_pool
         ~Held() {
             stop = true;
             for (auto &t : builders) if (t.joinable()) t.join();
@@ -1233,9 +1240,14 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
                 for (hipEvent_t e : *v) if (e) (void)hipEventDestroy(e);
             for (hipStream_t st : streams) (void)hipStreamDestroy(st);
             if (rays) (void)hipFree(rays);
-            for (agx_ctx *w : ctxs) if (w) agx_destroy(w);
+            for (agx_ctx *w : ctxs) {
+                if (!w) continue;
+                if (keep) owner->graph_pool.push_back(w);
+                else agx_destroy(w);
+            }
         }
     } held;
+    held.owner = c;
     // SETS sets of group_size contexts, used by the groups in turn: a group's
     // contexts take the scenes of group g + SETS once it has finished, so path
     // buffers are allocated once and grow with the scenes instead of being
@@ -1245,8 +1257,16 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     constexpr uint32_t SETS = 3;
     const uint32_t pool_size = std::min(n, SETS * group_size);
     held.ctxs.assign(pool_size, nullptr);
-    for (auto &w : held.ctxs)
-        if (int r = agx_create(c->device, &w); r != AGX_OK) return r;
+    for (auto &w : held.ctxs) {
+        // the contexts the last call left, then new ones (their pass limits,
+        // block sizes and scenes are set per launch, as for reuse within a call)
+        if (!c->graph_pool.empty()) {
+            w = c->graph_pool.back();
+            c->graph_pool.pop_back();
+        } else if (int r = agx_create(c->device, &w); r != AGX_OK) {
+            return r;
+        }
+    }
     auto ctx_of = [&](uint32_t i) {
         return held.ctxs[((pos[i] / group_size) % SETS) * group_size + pos[i] % group_size];
     };
@@ -1556,6 +1576,7 @@ int agx_render_batch_graph(agx_ctx *c, agx_batch_item *items, uint32_t n, int32_
     rep.build_wall_ms = double(build_end_ns.load()) * 1e-6;
     rep.wall_ms = ms_since(t0);
     if (report) *report = rep;
+    held.keep = true;
     return AGX_OK;
 }
 

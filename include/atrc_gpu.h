@@ -422,9 +422,12 @@ int  agx_render_batch(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t w
  * of 2^21 / 2^22 / 2^23 paths and 128- / 256-thread traversal blocks
  * (coordinate descent; later groups build meanwhile) and renders the batch
  * with the fastest; otherwise `branches` branches, passes of at most 2^22
- * paths and 256-thread blocks.  Images are bit-identical to agx_render's for
+ * paths and 128-thread blocks.  Images are bit-identical to agx_render's for
  * each item alone.  item.worker = the item's branch; item.render_ms = its
- * group's graph time. */
+ * group's graph time.  After a successful call the pooled contexts (their
+ * path buffers and last scenes) stay with ctx for its next batch, so a call
+ * neither starts by allocating them again nor ends by freeing them;
+ * agx_destroy(ctx) releases them. */
 int  agx_render_batch_graph(agx_ctx *ctx, agx_batch_item *items, uint32_t n, int32_t branches,
                             agx_batch_report *report);
 
