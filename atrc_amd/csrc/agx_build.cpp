@@ -596,6 +596,14 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
     }
 
     std::vector<Box> bounds(desc.entity_count);
+    {
+        // the triangle records of every mesh, reserved once
+        size_t mesh_tris = 0;
+        for (uint32_t i = 0; i < desc.entity_count; ++i)
+            if (desc.entities[i].geometry == AGX_GEOM_TRIANGLE_BVH) mesh_tris += desc.entities[i].tri_count;
+        S.blas_prims.reserve(3 * mesh_tris);
+        S.blas_info.reserve(6 * mesh_tris);
+    }
     for (uint32_t i = 0; i < desc.entity_count; ++i) {
         const agx_entity_desc &e = desc.entities[i];
         DevEntity d;
@@ -680,42 +688,43 @@ int build_host_scene(const agx_scene_desc &desc, HostScene *out, std::string *er
         }
         case AGX_GEOM_TRIANGLE_BVH: {
             if (!e.tris || !e.tri_count) { *err = "entity " + std::to_string(i) + ": empty triangle mesh"; return AGX_E_INVALID_ARG; }
-            std::vector<vec4> nodes, prims, info;
             double sa = 0;
             const bool emissive = e.emit_radiance[0] != 0 || e.emit_radiance[1] != 0 || e.emit_radiance[2] != 0;
+            const uint32_t node_base = uint32_t(S.blas_nodes.size() / 2);
+            const uint32_t prim_base = uint32_t(S.blas_prims.size() / 3);
             if (desc.accel != AGX_ACCEL_ATRC && !emissive) {
                 // the BVH4 modes traverse their own tree: Atrc's triangle BVH is
                 // needed only for the primitive order its light sampling uses, so
-                // a mesh that emits nothing keeps its triangles in input order
-                prims.reserve(size_t(e.tri_count) * 3);
-                info.reserve(size_t(e.tri_count) * 6);
-                for (uint32_t t = 0; t < e.tri_count; ++t) emit_triangle(e.tris, t, &prims, &info);
+                // a mesh that emits nothing keeps its triangles in input order,
+                // written straight into the scene's arrays
+                for (uint32_t t = 0; t < e.tri_count; ++t) emit_triangle(e.tris, t, &S.blas_prims, &S.blas_info);
                 sa = double(mesh_area(e.tris, e.tri_count));
             } else {
+                std::vector<vec4> nodes, prims, info;
                 const int depth = build_triangle_bvh(e.tris, e.tri_count, &nodes, &prims, &info, &sa);
                 S.max_blas_depth = std::max(S.max_blas_depth, depth);
+                // rebase indices to the global arrays
+                for (size_t k = 0; k < nodes.size(); k += 2) {
+                    uint32_t s, r;
+                    std::memcpy(&s, &nodes[k].w, 4);
+                    std::memcpy(&r, &nodes[k + 1].w, 4);
+                    if (s == INTERIOR) r += node_base;
+                    else { s += prim_base; r += prim_base; }
+                    nodes[k].w = bits_f(s);
+                    nodes[k + 1].w = bits_f(r);
+                }
+                S.blas_nodes.insert(S.blas_nodes.end(), nodes.begin(), nodes.end());
+                S.blas_prims.insert(S.blas_prims.end(), prims.begin(), prims.end());
+                S.blas_info.insert(S.blas_info.end(), info.begin(), info.end());
             }
-            const uint32_t node_base = uint32_t(S.blas_nodes.size() / 2);
-            const uint32_t prim_base = uint32_t(S.blas_prims.size() / 3);
-            // rebase indices to the global arrays
-            for (size_t k = 0; k < nodes.size(); k += 2) {
-                uint32_t s, r;
-                std::memcpy(&s, &nodes[k].w, 4);
-                std::memcpy(&r, &nodes[k + 1].w, 4);
-                if (s == INTERIOR) r += node_base;
-                else { s += prim_base; r += prim_base; }
-                nodes[k].w = bits_f(s);
-                nodes[k + 1].w = bits_f(r);
-            }
-            S.blas_nodes.insert(S.blas_nodes.end(), nodes.begin(), nodes.end());
-            S.blas_prims.insert(S.blas_prims.end(), prims.begin(), prims.end());
-            S.blas_info.insert(S.blas_info.end(), info.begin(), info.end());
             d.node_base = node_base;
             d.prim_base = prim_base;
             d.prim_count = e.tri_count;
             area = float(sa);
+            // the mesh's records (a, b - a, c - a) in the scene's array
+            const vec4 *prims = S.blas_prims.data() + 3 * size_t(prim_base);
             // world bound of TriangleBVH (triangle_bvh.cpp:570-583)
-            for (size_t k = 0; k < prims.size(); k += 3) {
+            for (size_t k = 0; k < 3 * size_t(e.tri_count); k += 3) {
                 const F3 a{prims[k].x, prims[k].y, prims[k].z};
                 box.add_point(a);
                 box.add_point(add(a, F3{prims[k + 1].x, prims[k + 1].y, prims[k + 1].z}));
