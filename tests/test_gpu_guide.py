@@ -137,19 +137,22 @@ def test_guide_training_gradient_and_adam(gpu_ctx):
     np.testing.assert_allclose(p2, p - 1e-2 * g / (np.abs(g) + 1e-8), rtol=1e-5, atol=1e-7)
 
 
-def test_guide_matrix_core_gradient(gpu_ctx):
+@pytest.mark.parametrize("n", [1000, 40000])
+def test_guide_matrix_core_gradient(gpu_ctx, n):
     """The matrix-core training path (BF16 operands, FP32 accumulation) that
     renders use gives the autograd gradient to BF16 accuracy: per layer, the
     relative error of the weight and bias blocks is within 8 % and the
     direction within 4 degrees (BF16 rounding of activations, weights and
     output gradients compounds through the five layers; a wrong index or a
-    dropped term gives errors of order 1); 1000 samples span several
-    32-sample chunks, blocks and a partial last chunk."""
+    dropped term gives errors of order 1).  1000 samples span several
+    16-sample tiles, blocks and a partial last tile; 40,000 samples over the
+    debug entry's 64 blocks of 256 samples make blocks take up to three
+    iterations, whose gradients add to the block's partial."""
     rng = np.random.default_rng(11)
     p = init_params(7)
-    s = make_samples(rng, 1000)
+    s = make_samples(rng, n)
     wscale = float(1 / s[:, 11].sum())
-    log("[guide] matrix-core training gradient vs torch autograd, 1000 samples")
+    log(f"[guide] matrix-core training gradient vs torch autograd, {n} samples")
     g = gpu_ctx.guide_grad_mfma(p, s, wscale)
     ref = torch_loss(p, s, wscale)
     off = 0
